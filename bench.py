@@ -1,0 +1,202 @@
+"""Benchmark: converged SCF solves/s at |dE| <= 1e-6 Ha vs the FP64 oracle, with p50/p99 latency.
+
+Workload (default): batched 1D solves of uneven size -- one step is one solve of a seeded
+64-system batch on the 513-point grid [-25.6, 25.6]: 1-4 nuclei, Z in {1..4}, sum Z <= 8,
+N_e = sum Z, LocalMLP XC on a 17-point window, hidden [64, 64], SiLU, Glorot-normal weights
+from the seed, 20 SCF iterations, linear mixing 0.3.  Inputs are resident in HBM before the
+timed region.  A solve counts as converged when its final total energy is within 1e-6 Ha of
+the FP64 oracle's (which runs before any GPU work, in a process pool on the host cores).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch 64] [--seed 1234]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import multiprocessing as mp
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_GRID = 513
+T_ITERS = 20
+ALPHA = 0.3
+WIDTHS = [17, 64, 64, 1]
+
+
+def workload(batch, seed):
+    from oracle import od as O
+
+    grids = O.h2_config_grid()
+    mols = O.random_molecules(seed, batch, grids=grids)
+    W, b = O.glorot_normal_mlp(seed, WIDTHS)
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in mols])
+    return grids, mols, W, b, n0
+
+
+def _oracle_one(args):
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from threadpoolctl import threadpool_limits
+
+    from oracle import od as O
+
+    grids, (loc, z, ne), W, b, n0 = args
+    with threadpool_limits(1):
+        t = time.perf_counter()
+        st = O.jit_kohn_sham(loc, z, ne, T_ITERS, grids, O.LocalMLP(W, b, "silu", 17, 2.0), O.exponential_coulomb,
+                             n0, alpha=ALPHA, alpha_decay=1.0, num_mixing_iterations=1)
+        dt = time.perf_counter() - t
+    return st.total_energy, st.density[-1], dt
+
+
+def oracle_batch(grids, mols, W, b, n0, procs):
+    args = [(grids, m, W, b, n0[i]) for i, m in enumerate(mols)]
+    ctx = mp.get_context("spawn")
+    t = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        out = pool.map(_oracle_one, args, chunksize=1)
+    wall = time.perf_counter() - t
+    return out, wall
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--procs", type=int, default=0, help="host processes for the oracle (default: min(16, cores))")
+    ap.add_argument("--skip-oracle", action="store_true",
+                    help="no oracle / CPU baseline (for profiler runs, which must not start child processes)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+
+    grids, mols, W, b, n0 = workload(args.batch, args.seed)
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    procs = args.procs or max(1, min(16, ncores))
+    # ---- FP64 oracle on the host cores (before the GPU is touched) ----
+    if args.skip_oracle:
+        oracle, oracle_wall = None, float("nan")
+        ref_E = np.full(args.batch, np.nan)
+        per_solve = np.full(args.batch, np.nan)
+    else:
+        oracle, oracle_wall = oracle_batch(grids, mols, W, b, n0, procs)
+        ref_E = np.array([o[0][-1] for o in oracle])
+        per_solve = np.array([o[2] for o in oracle])
+
+    import torch
+
+    from qex_amd import _lib
+    from qex_amd.od import engine, utils, xc
+
+    lib = _lib.load()
+    f = xc.LocalMLPXC(list(zip(W, b)), window=17, activation="silu", norm=2.0)
+    prob = engine.Problem1D(grids, [m[0] for m in mols], [m[1] for m in mols], [m[2] for m in mols],
+                            utils.exponential_coulomb, f, T_ITERS, alpha=ALPHA, alpha_decay=1.0,
+                            num_mixing_iterations=1, loop="scan")
+    n0_dev = engine.as_f64(n0, prob.dev)
+    out = prob.allocate_outputs()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        prob.run(n0_dev, out)
+    torch.cuda.synchronize()
+    # ---- timed region: K batch solves; per-step events for latency; per-kernel events ----
+    launches_per_step = 5 * T_ITERS + 8
+    _lib.check(lib.qex_profile_enable(1, launches_per_step * args.steps + 16), "qex_profile_enable")
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record()
+        prob.run(n0_dev, out)
+        ev[k][1].record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nk = len(_lib.KERNEL_KINDS)
+    ms = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int32 * nk)()
+    _lib.check(lib.qex_profile_read(ms, cnt, nk), "qex_profile_read")
+    lib.qex_profile_enable(0, 1)
+    step_ms = np.array([a.elapsed_time(b_) for a, b_ in ev])
+
+    E = out.total_energy[:, -1].double().cpu().numpy()
+    dE = np.abs(E - ref_E)
+    conv = dE <= 1e-6 if oracle is not None else np.ones(args.batch, dtype=bool)
+    D = out.density[:, -1].cpu().numpy()
+    dx = float(grids[1] - grids[0])
+    dl2 = (np.array([np.sqrt(dx * np.sum((D[i] - oracle[i][1]) ** 2)) for i in range(args.batch)])
+           if oracle is not None else np.full(args.batch, np.nan))
+    n_conv = int(conv.sum())
+    value = n_conv * args.steps / elapsed
+
+    # ---- roofline of the dominant kernel (largest summed time) ----
+    kern = {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.KERNEL_KINDS)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    avg_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
+    ne = np.array([m[2] for m in mols])
+    keig = ne // 2 + 1
+    # algorithmic FP64 work of one eigen launch (the method as implemented, per eigenpair):
+    # Sturm multisection to 2e-12 from the Gershgorin width (~log_255(range/tol) rounds of 256
+    # shifts x N steps x 9 flops) + pivoted band LU (~40 N) + 2 solves (~14 N each)
+    rounds = 5.0
+    eig_flops = float(np.sum(keig)) * (rounds * 256 * N_GRID * 9 + 40 * N_GRID + 28 * N_GRID)
+    flops_per_launch = {"eigen": eig_flops,
+                        "mlp": args.batch * N_GRID * 3 * 2 * (17 * 64 + 64 * 64 + 64) * 1.0,
+                        "potential": args.batch * 2 * 2 * N_GRID * N_GRID * 1.0}
+    fpl = flops_per_launch.get(dom, 0.0)
+    achieved = fpl / (avg_ms[dom] * 1e-3) / 1e12 if avg_ms[dom] > 0 else 0.0
+    peak = 78.6  # FP64 TFLOP/s (MI355X)
+    result = {
+        "metric": "converged SCF solves/s at |dE|<=1e-6 Ha vs FP64 oracle",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(elapsed / args.steps * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded molecules and Glorot-normal LocalMLP weights)",
+        "config": {"workload": "c2 batched 1D KS-SCF: 64 uneven systems, 513-pt grid, LocalMLP-17 [64,64] SiLU, "
+                               "20 iterations, mixing 0.3, one batch per step",
+                   "batch": args.batch, "num_grids": N_GRID, "iterations": T_ITERS, "seed": args.seed,
+                   "path": "plain FP64 batched"},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "parity": {"converged": n_conv, "of": args.batch, "max_abs_dE": float(dE.max()),
+                   "median_abs_dE": float(np.median(dE)), "max_density_L2": float(dl2.max())},
+        "kernels_ms_avg": {k: round(v, 4) for k, v in avg_ms.items()},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None},
+        "cpu_baseline": {"value": float(len(per_solve) / oracle_wall), "unit": "solves/s", "cores": procs,
+                         "kind": "port",
+                         "sample": f"FP64 NumPy/SciPy oracle, the same {args.batch} solves, {procs} processes x 1 "
+                                   f"thread (median {np.median(per_solve):.2f} s/solve/core); {cpu_model()}"},
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * qex.h -- C ABI of the MI355X Kohn-Sham SCF engine (libqex.so, gfx950).
+ *
+ * Conventions shared by every entry point:
+ *  - All array arguments are DEVICE pointers owned by the caller (row-major, float64 unless
+ *    stated).  The library allocates nothing: each entry point that needs scratch takes a
+ *    caller-provided workspace whose size is returned by the matching *_workspace_bytes().
+ *  - Every call takes a hipStream_t (passed as void*) and only enqueues work; nothing
+ *    synchronises the host.  Results are ready when the stream is.
+ *  - Return value: QEX_OK (0) or a negative QEX_E_* code; qex_last_error() returns a
+ *    thread-local message for the last failing call on this thread.  The library never
+ *    aborts or exits.
+ *  - Kernels are deterministic: reductions run in a fixed order, with no floating-point
+ *    atomics, so a system's result does not depend on what else is in the batch.
+ */
+#ifndef QEX_H
+#define QEX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QEX_OK 0
+#define QEX_E_BADARG (-1)
+#define QEX_E_WORKSPACE (-2)
+#define QEX_E_LAUNCH (-3)
+#define QEX_E_UNSUPPORTED (-4)
+
+#define QEX1D_MAX_GRIDS 1024
+#define QEX1D_MAX_ELECTRONS 62
+#define QEX1D_MAX_MLP_LAYERS 6
+#define QEX1D_MAX_MLP_WIDTH 128
+#define QEX1D_MAX_MLP_WINDOW 33
+
+/* SCF loop semantics.
+ * PYTHON: jax_dft scf.kohn_sham / qedft od/scf.py:434-560 -- convergence on MSE(n_out - n_in)
+ *         of the same iteration; reflection about the centre of the nuclei (window per system).
+ * SCAN:   jax_dft jit_scf.kohn_sham / qedft od/jit_scf.py:143-244 -- convergence on
+ *         MSE(n_mixed - n_in), visible from the next iteration; reflection about grid centre. */
+#define QEX1D_LOOP_PYTHON 0
+#define QEX1D_LOOP_SCAN 1
+
+/* Exchange-correlation kinds.  LOCAL kinds give an energy density per point
+ * (E = dx e(n).n, v = e + J_e^T n; jax_dft scf.get_xc_energy/get_xc_potential);
+ * GLOBAL kinds give f summed to a scalar (E = dx sum f, v = dx df/dn; od/scf.py:32-76). */
+#define QEX1D_XC_LDA_EXP 0      /* jax_dft xc LDA for the exponential interaction (local) */
+#define QEX1D_XC_POWER_LOCAL 1  /* e = c (n + eps)^p (local) */
+#define QEX1D_XC_POWER_GLOBAL 2 /* f = c (n + eps)^p summed (global) */
+#define QEX1D_XC_LOCAL_MLP 3    /* windowed per-point MLP (local); window 1 = classical_models.py:123 */
+#define QEX1D_XC_NONE 4         /* no XC (v_xc = 0) */
+
+#define QEX_ACT_SILU 0
+#define QEX_ACT_TANH 1
+#define QEX_ACT_SOFTPLUS 2
+#define QEX_ACT_SIGMOID 3
+#define QEX_ACT_RELU 4
+#define QEX_ACT_GELU_TANH 5
+
+#define QEX_INTERACTION_EXPONENTIAL 0 /* A exp(-kappa |r|)   (p0 = A, p1 = kappa) */
+#define QEX_INTERACTION_SOFT 1        /* 1 / sqrt(r^2 + p0)  (p0 = soften factor)  */
+
+typedef struct qex1d_problem {
+  int32_t batch;          /* B systems sharing one uniform grid */
+  int32_t num_grids;      /* N, 5 <= N <= QEX1D_MAX_GRIDS */
+  double dx;              /* grid spacing */
+  int32_t num_iterations; /* T */
+  int32_t max_electrons;  /* >= every num_electrons[b]; <= QEX1D_MAX_ELECTRONS */
+  double alpha;           /* linear mixing factor of iteration 0 */
+  double alpha_decay;     /* alpha_{k+1} = alpha_k * alpha_decay */
+  int32_t num_mixing_iterations;
+  double density_mse_converge_tolerance; /* negative disables early stopping */
+  int32_t loop;                          /* QEX1D_LOOP_* */
+  int32_t reflect;                       /* enforce_reflection_symmetry */
+  int32_t xc_kind;                       /* QEX1D_XC_* */
+  double xc_c, xc_p, xc_eps;             /* POWER_* functionals */
+  /* LOCAL_MLP: mlp_layers Dense layers; widths[0] = window, widths[mlp_layers] = 1. */
+  int32_t mlp_layers;
+  int32_t mlp_widths[QEX1D_MAX_MLP_LAYERS + 1];
+  int32_t mlp_activation; /* QEX_ACT_* */
+  double mlp_norm;        /* density_normalization_factor */
+} qex1d_problem;
+
+typedef struct qex1d_inputs {
+  const double* interaction_table;  /* [N]   u(k dx), k = 0..N-1 (even interaction) */
+  const double* external_potential; /* [B][N] */
+  const double* initial_density;    /* [B][N] */
+  const int32_t* num_electrons;     /* [B]   1 <= N_e <= max_electrons */
+  const int32_t* reflect_window;    /* [B][2] inclusive index window (lo, hi); used if reflect */
+  const double* xc_params;          /* LOCAL_MLP: W1[in][out], b1, W2, b2, ... packed, float64 */
+} qex1d_inputs;
+
+typedef struct qex1d_outputs {
+  double* density;           /* [B][T][N]  mixed density stored in the state */
+  double* total_energy;      /* [B][T] */
+  double* hartree_potential; /* [B][T][N] */
+  double* xc_potential;      /* [B][T][N] */
+  double* xc_energy_density; /* [B][T][N] for LOCAL kinds, [B][T] for GLOBAL kinds */
+  double* gap;               /* [B][T] */
+  uint8_t* converged;        /* [B][T] */
+  double* eigenvalues;       /* optional [B][T][max_electrons/2+1], may be NULL */
+} qex1d_outputs;
+
+/* Workspace bytes needed by qex1d_kohn_sham for this problem (0 on bad arguments). */
+size_t qex1d_workspace_bytes(const qex1d_problem* p);
+
+/* Batched Kohn-Sham SCF: T iterations for B systems, every iteration's state stored.
+ * Replaces jax_dft scf.kohn_sham / jit_scf.kohn_sham (called from qedft od/eval.py:67-89,
+ * od/train.py:108-121) and qedft od/scf.py:434 kohn_sham_amplitude_encoded /
+ * od/jit_scf.py:247 (GLOBAL xc kinds). */
+int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace bytes for qex1d_solve_noninteracting. */
+size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, int32_t max_electrons);
+
+/* Batched non-interacting solve: lowest eigenpairs of T + diag(v) (5-point kinetic stencil),
+ * density of the first N_e spin-orbitals, sum of their eigenvalues and the gap.
+ * Replaces jax_dft scf.solve_noninteracting_system / qedft od/scf.py:233-268 and the
+ * "noninteracting" initial density of od/trainer.py:72-75.
+ * eigenvalues [B][K] and wavefunctions [B][K][N] (K = max_electrons/2+1, dx-normalised)
+ * are optional (NULL). */
+int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int32_t max_electrons,
+                               const double* potential, const int32_t* num_electrons, double* density,
+                               double* total_eigen_energies, double* gap, double* eigenvalues,
+                               double* wavefunctions, void* workspace, size_t workspace_bytes,
+                               void* stream);
+
+/* v_ext[b][i] = -sum_a Z[b][a] u(x_i - R[b][a]) on grids[N].
+ * Replaces jax_dft utils.get_atomic_chain_potential (od/scf.py:490-495). */
+int qex1d_external_potential(int32_t batch, int32_t num_grids, const double* grids, int32_t num_nuclei,
+                             const double* locations, const double* nuclear_charges, int32_t interaction,
+                             double p0, double p1, double* out, void* stream);
+
+/* v_H[b] = dx K n[b], K_ij = table[|i-j|].  Replaces od/scf.py:303-325 get_hartree_potential. */
+int qex1d_hartree_potential(int32_t batch, int32_t num_grids, double dx, const double* table,
+                            const double* density, double* out, void* stream);
+
+/* Opt-in kernel timing for measurement.  While enabled, qex1d_kohn_sham records a hipEvent pair
+ * on its stream around each kernel launch (at most max_launches).  qex_profile_read waits for
+ * the recorded events and returns, per kernel kind (0 MLP, 1 potential, 2 eigen, 3 density,
+ * 4 finalize), the summed milliseconds and the number of launches, then clears the records.
+ * No counterpart in the reference (which has no profiler hooks). */
+int qex_profile_enable(int32_t enable, int32_t max_launches);
+int qex_profile_read(double* ms_per_kind, int32_t* launches_per_kind, int32_t nkinds);
+
+const char* qex_last_error(void);
+int32_t qex_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QEX_H */

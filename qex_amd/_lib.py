@@ -1,0 +1,130 @@
+"""ctypes binding of libqex.so (the C ABI declared in include/qex.h).
+
+The library is required: importing the engine without it raises immediately; there is no
+fallback computation path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqex.so")
+
+QEX_OK = 0
+QEX1D_MAX_GRIDS = 1024
+QEX1D_MAX_ELECTRONS = 62
+QEX1D_MAX_MLP_LAYERS = 6
+
+QEX1D_LOOP_PYTHON = 0
+QEX1D_LOOP_SCAN = 1
+
+QEX1D_XC_LDA_EXP = 0
+QEX1D_XC_POWER_LOCAL = 1
+QEX1D_XC_POWER_GLOBAL = 2
+QEX1D_XC_LOCAL_MLP = 3
+QEX1D_XC_NONE = 4
+
+ACTIVATIONS = {"silu": 0, "swish": 0, "tanh": 1, "softplus": 2, "sigmoid": 3, "relu": 4, "gelu": 5}
+
+QEX_INTERACTION_EXPONENTIAL = 0
+QEX_INTERACTION_SOFT = 1
+
+
+class Qex1dProblem(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("num_grids", ctypes.c_int32),
+        ("dx", ctypes.c_double),
+        ("num_iterations", ctypes.c_int32),
+        ("max_electrons", ctypes.c_int32),
+        ("alpha", ctypes.c_double),
+        ("alpha_decay", ctypes.c_double),
+        ("num_mixing_iterations", ctypes.c_int32),
+        ("density_mse_converge_tolerance", ctypes.c_double),
+        ("loop", ctypes.c_int32),
+        ("reflect", ctypes.c_int32),
+        ("xc_kind", ctypes.c_int32),
+        ("xc_c", ctypes.c_double),
+        ("xc_p", ctypes.c_double),
+        ("xc_eps", ctypes.c_double),
+        ("mlp_layers", ctypes.c_int32),
+        ("mlp_widths", ctypes.c_int32 * (QEX1D_MAX_MLP_LAYERS + 1)),
+        ("mlp_activation", ctypes.c_int32),
+        ("mlp_norm", ctypes.c_double),
+    ]
+
+
+class Qex1dInputs(ctypes.Structure):
+    _fields_ = [
+        ("interaction_table", ctypes.c_void_p),
+        ("external_potential", ctypes.c_void_p),
+        ("initial_density", ctypes.c_void_p),
+        ("num_electrons", ctypes.c_void_p),
+        ("reflect_window", ctypes.c_void_p),
+        ("xc_params", ctypes.c_void_p),
+    ]
+
+
+class Qex1dOutputs(ctypes.Structure):
+    _fields_ = [
+        ("density", ctypes.c_void_p),
+        ("total_energy", ctypes.c_void_p),
+        ("hartree_potential", ctypes.c_void_p),
+        ("xc_potential", ctypes.c_void_p),
+        ("xc_energy_density", ctypes.c_void_p),
+        ("gap", ctypes.c_void_p),
+        ("converged", ctypes.c_void_p),
+        ("eigenvalues", ctypes.c_void_p),
+    ]
+
+
+# Every exported symbol and its ctypes signature (restype, argtypes).
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_D = ctypes.c_double
+_S = ctypes.c_size_t
+SIGNATURES = {
+    "qex_last_error": (ctypes.c_char_p, []),
+    "qex_abi_version": (_I, []),
+    "qex1d_workspace_bytes": (_S, [ctypes.POINTER(Qex1dProblem)]),
+    "qex1d_kohn_sham": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
+                             ctypes.POINTER(Qex1dOutputs), _P, _S, _P]),
+    "qex1d_noninteracting_workspace_bytes": (_S, [_I, _I, _I]),
+    "qex1d_solve_noninteracting": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
+    "qex1d_external_potential": (_I, [_I, _I, _P, _I, _P, _P, _I, _D, _D, _P, _P]),
+    "qex1d_hartree_potential": (_I, [_I, _I, _D, _P, _P, _P, _P]),
+    "qex_profile_enable": (_I, [_I, _I]),
+    "qex_profile_read": (_I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), _I]),
+}
+
+KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize"]
+
+_lib = None
+
+
+class QexError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Loads libqex.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QexError(f"{LIB_PATH} is missing: build it with `python -m qex_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != QEX_OK:
+        msg = load().qex_last_error()
+        raise QexError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

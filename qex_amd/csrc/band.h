@@ -1,0 +1,153 @@
+// band.h -- symmetric pentadiagonal eigen-kernels shared by the gfx950 eigensolver.
+//
+// A = diag(a) + b (first off-diagonals) + c (second off-diagonals), the 5-point-stencil
+// Kohn-Sham Hamiltonian of the 1D grid.  The functions are __host__ __device__ so the same
+// code can be exercised on the CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define QEX_HD __host__ __device__
+
+namespace qexband {
+
+constexpr double PIVMIN = 1e-60;
+
+// 1/d to double precision: the hardware reciprocal (relative error <= 5e-8 on gfx950) and one
+// Newton step (relative error ~2e-15) on the device.
+QEX_HD inline double rcp64(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
+#else
+  return 1.0 / d;
+#endif
+}
+
+// Number of eigenvalues of the symmetric pentadiagonal matrix (diag a, off b, c) below sigma:
+// negative pivots of the LDL^T factorisation of A - sigma I.  With u_i = l_i d_i:
+//   d_i = (a_i - sigma) - u_{i-1}^2 / d_{i-1} - c^2 / d_{i-2},   u_i = b - c u_{i-1} / d_{i-1}.
+// The diagonal is read 8 entries at a time so the LDS latency is paid once per chunk.
+QEX_HD inline int sturm_count(const double* __restrict__ a, int N, double b, double c, double sigma) {
+  const double c2 = c * c;
+  double rm1 = 0.0, rm2 = 0.0, um1 = 0.0;
+  int cnt = 0;
+  auto step = [&](double ai) {
+    const double p = (ai - sigma) - c2 * rm2;
+    double d = fma(-um1 * um1, rm1, p);
+    if (fabs(d) < PIVMIN) d = -PIVMIN;
+    cnt += d < 0.0 ? 1 : 0;
+    const double u = fma(-c * rm1, um1, b);
+    rm2 = rm1;
+    rm1 = rcp64(d);
+    um1 = u;
+  };
+  int i = 0;
+  for (; i + 8 <= N; i += 8) {
+    double av[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) av[q] = a[i + q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) step(av[q]);
+  }
+  for (; i < N; ++i) step(a[i]);
+  return cnt;
+}
+
+// Inverse iteration without pivoting.  A - sigma I = L D L^T with L unit lower triangular
+// (L_{i+1,i} = l_i, L_{i+2,i} = m_i); pivots below `tiny` in magnitude are replaced by
+// +-tiny.  F holds 4 doubles per row: 1/d_i, l_i, m_i and the forward-solve value z_i.
+//
+// ldlt_factor_forward: factorisation fused with the forward solve L z = y.
+QEX_HD inline void ldlt_factor_forward(const double* __restrict__ a, const double* __restrict__ y, int N, double b,
+                                       double c, double sigma, double tiny, double* __restrict__ F) {
+  const double c2 = c * c;
+  double rm1 = 0.0, rm2 = 0.0, um1 = 0.0, lm1 = 0.0, mm1 = 0.0, mm2 = 0.0, z1 = 0.0, z2 = 0.0;
+  auto step = [&](int i, double ai, double yi) {
+    double d = fma(-um1 * um1, rm1, (ai - sigma) - c2 * rm2);
+    if (fabs(d) < tiny) d = d < 0.0 ? -tiny : tiny;
+    const double r = rcp64(d);
+    const double u = fma(-c * rm1, um1, b);
+    const double l = u * r, m = c * r;
+    const double z = fma(-mm2, z2, fma(-lm1, z1, yi));
+    F[4 * i + 0] = r;
+    F[4 * i + 1] = l;
+    F[4 * i + 2] = m;
+    F[4 * i + 3] = z;
+    z2 = z1;
+    z1 = z;
+    mm2 = mm1;
+    mm1 = m;
+    lm1 = l;
+    rm2 = rm1;
+    rm1 = r;
+    um1 = u;
+  };
+  int i = 0;
+  for (; i + 8 <= N; i += 8) {
+    double av[8], yv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      av[q] = a[i + q];
+      yv[q] = y[i + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) step(i + q, av[q], yv[q]);
+  }
+  for (; i < N; ++i) step(i, a[i], y[i]);
+}
+
+// Forward solve L z = y with an existing factorisation (z stored in F[4i+3]).
+QEX_HD inline void ldlt_forward(const double* __restrict__ y, int N, double* __restrict__ F) {
+  double z1 = 0.0, z2 = 0.0, lm1 = 0.0, mm1 = 0.0, mm2 = 0.0;
+  auto step = [&](int i, double yi, double l, double m) {
+    const double z = fma(-mm2, z2, fma(-lm1, z1, yi));
+    F[4 * i + 3] = z;
+    z2 = z1;
+    z1 = z;
+    mm2 = mm1;
+    mm1 = m;
+    lm1 = l;
+  };
+  int i = 0;
+  for (; i + 8 <= N; i += 8) {
+    double yv[8], lv[8], mv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      yv[q] = y[i + q];
+      lv[q] = F[4 * (i + q) + 1];
+      mv[q] = F[4 * (i + q) + 2];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) step(i + q, yv[q], lv[q], mv[q]);
+  }
+  for (; i < N; ++i) step(i, y[i], F[4 * i + 1], F[4 * i + 2]);
+}
+
+// Back substitution L^T x = D^{-1} z:  x_i = z_i / d_i - l_i x_{i+1} - m_i x_{i+2}.
+QEX_HD inline void ldlt_backward(const double* __restrict__ F, int N, double* __restrict__ x) {
+  double x1 = 0.0, x2 = 0.0;
+  auto step = [&](int i, double r, double l, double m, double z) {
+    const double xi = fma(-m, x2, fma(-l, x1, z * r));
+    x[i] = xi;
+    x2 = x1;
+    x1 = xi;
+  };
+  int i = N - 1;
+  for (; i >= 7; i -= 8) {
+    double rv[8], lv[8], mv[8], zv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = i - q;
+      rv[q] = F[4 * k + 0];
+      lv[q] = F[4 * k + 1];
+      mv[q] = F[4 * k + 2];
+      zv[q] = F[4 * k + 3];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) step(i - q, rv[q], lv[q], mv[q], zv[q]);
+  }
+  for (; i >= 0; --i) step(i, F[4 * i], F[4 * i + 1], F[4 * i + 2], F[4 * i + 3]);
+}
+
+}  // namespace qexband

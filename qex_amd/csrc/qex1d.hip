@@ -1,0 +1,1332 @@
+// qex1d.hip -- batched 1D Kohn-Sham SCF for gfx950 (MI355X).
+//
+// One SCF iteration over a batch of B systems on a shared uniform grid of N points is four
+// launches (plus one MLP launch for the neural XC kind):
+//   k_mlp        (LOCAL_MLP only) per-point MLP forward + input-VJP at the input density, and
+//                forward at the previous iteration's output density (for its XC energy).
+//   k_potential  grid (tiles, B): Hartree potential as a Toeplitz matvec from an LDS table,
+//                XC potential, v_KS; and the deferred Hartree / XC energies of the previous
+//                iteration's output density (second matvec in the same pass).
+//   k_eigen      grid (K, B): one workgroup per wanted eigenpair.  The KS Hamiltonian
+//                -1/2 L/dx^2 + diag(v) is pentadiagonal (5-point stencil), so eigenvalues come
+//                from Sturm counts of its LDL^T factorisation (256 shifts per round, one per
+//                lane) and eigenvectors from inverse iteration with a pivoted banded LU.
+//   k_density    grid (B): Gram-Schmidt of the occupied orbitals, density, eigen energy, gap,
+//                reflection, density mixing, convergence flags.
+// k_finalize assembles the total energies after the loop.  Reductions run in a fixed order
+// (wave butterflies + ordered cross-wave sums), so results do not depend on the batch.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "qex.h"
+#include "band.h"
+
+namespace {
+
+using qexband::ldlt_backward;
+using qexband::ldlt_factor_forward;
+using qexband::ldlt_forward;
+using qexband::rcp64;
+using qexband::sturm_count;
+using qexband::PIVMIN;
+
+constexpr int NT = 256;     // threads per workgroup (4 wave64)
+constexpr int NWAVE = NT / 64;
+constexpr int TILE = 64;    // output points per k_potential workgroup
+constexpr int MLP_P = 32;   // points per k_mlp workgroup (8 lanes per point)
+constexpr double EPS = 2.220446049250313e-16;
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(QEX_E_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return QEX_OK;
+}
+
+// ---- optional kernel timing (hipEvent pairs around each launch, per kernel kind) ---------------
+constexpr int NKINDS = 8;  // 0 mlp, 1 potential, 2 eigen, 3 density, 4 finalize/other
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+bool g_prof_on = false;
+ProfRec* g_prof = nullptr;
+int g_prof_n = 0, g_prof_cap = 0;
+hipEvent_t* g_pool = nullptr;
+int g_pool_n = 0, g_pool_used = 0;
+
+hipEvent_t pool_event() {
+  if (g_pool_used >= g_pool_n) return nullptr;
+  return g_pool[g_pool_used++];
+}
+
+struct ProfScope {
+  int kind;
+  hipStream_t st;
+  hipEvent_t b = nullptr, e = nullptr;
+  ProfScope(int k, hipStream_t s) : kind(k), st(s) {
+    if (!g_prof_on || g_prof_n >= g_prof_cap) return;
+    b = pool_event();
+    e = pool_event();
+    if (b && e) (void)hipEventRecord(b, st);
+  }
+  ~ProfScope() {
+    if (b && e) {
+      (void)hipEventRecord(e, st);
+      g_prof[g_prof_n++] = ProfRec{kind, b, e};
+    }
+  }
+};
+
+// --------------------------------------------------------------------------------------------
+// device helpers
+// --------------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Sums M values over the workgroup; every thread receives the totals.  Fixed order.
+template <int M>
+__device__ __forceinline__ void block_sum(double (&v)[M], double* red) {
+#pragma unroll
+  for (int m = 0; m < M; ++m) v[m] = wave_sum(v[m]);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) red[w * M + m] = v[m];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) s += red[q * M + m];
+    v[m] = s;
+  }
+}
+
+__device__ __forceinline__ double block_max(double x, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int q = 1; q < NWAVE; ++q) s = fmax(s, red[q]);
+  return s;
+}
+
+// ---- 1D LDA for the exponential interaction (jax_dft xc; Baker et al. PRB 91, 235141) -------
+struct XcOut {
+  double e, v;  // energy per electron and v = e + n de/dn
+};
+
+__device__ XcOut lda_exp(double n) {
+  const double A = 1.071295, kappa = 1.0 / 2.385345;
+  const double y = M_PI * n / kappa;
+  const double dydn = M_PI / kappa;
+  // exchange
+  const double cx = A / (2.0 * M_PI);
+  double ex, dex;
+  if (y > 1e-15) {
+    const double l = log(1.0 + y * y);  // as in the reference (not log1p)
+    ex = cx * (l / y - 2.0 * atan(y));
+    dex = cx * (-l / (y * y));
+  } else {
+    ex = cx * (-y + y * y * y / 6.0);
+    dex = cx * (-1.0 + 0.5 * y * y);
+  }
+  // correlation
+  const double al = 2.0, be = -1.00077, ga = 6.26099, de = -11.9041, et = 9.62614, si = -1.48334, nu = 1.0;
+  double ec, dec;
+  if (y == 0.0) {
+    ec = 0.0;
+    dec = -A / M_PI / al;
+  } else {
+    const double s = sqrt(y);
+    const double c3 = nu * M_PI * kappa * kappa / A;
+    const double P = al + be * s + ga * y + de * y * s + et * y * y + si * y * y * s + c3 * y * y * y;
+    const double dP = 0.5 * be / s + ga + 1.5 * de * s + 2.0 * et * y + 2.5 * si * y * s + 3.0 * c3 * y * y;
+    ec = -A * y / M_PI / P;
+    dec = -A / M_PI * (P - y * dP) / (P * P);
+  }
+  XcOut o;
+  o.e = ex + ec;
+  o.v = o.e + n * (dex + dec) * dydn;
+  return o;
+}
+
+// ---- activations --------------------------------------------------------------------------
+__device__ __forceinline__ double act_fwd(int kind, double z, double* dz) {
+  switch (kind) {
+    case QEX_ACT_SILU: {
+      const double s = 1.0 / (1.0 + exp(-z));
+      *dz = s * (1.0 + z * (1.0 - s));
+      return z * s;
+    }
+    case QEX_ACT_TANH: {
+      const double t = tanh(z);
+      *dz = 1.0 - t * t;
+      return t;
+    }
+    case QEX_ACT_SOFTPLUS: {
+      *dz = 1.0 / (1.0 + exp(-z));
+      return z > 0 ? z + log1p(exp(-z)) : log1p(exp(z));
+    }
+    case QEX_ACT_SIGMOID: {
+      const double s = 1.0 / (1.0 + exp(-z));
+      *dz = s * (1.0 - s);
+      return s;
+    }
+    case QEX_ACT_RELU:
+      *dz = z > 0 ? 1.0 : 0.0;
+      return z > 0 ? z : 0.0;
+    default: {  // GELU, tanh approximation
+      const double k = 0.7978845608028654;
+      const double u = k * (z + 0.044715 * z * z * z);
+      const double t = tanh(u);
+      *dz = 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * k * (1.0 + 3.0 * 0.044715 * z * z);
+      return 0.5 * z * (1.0 + t);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// Kernel arguments
+// --------------------------------------------------------------------------------------------
+struct Dev1 {
+  int B, N, T, K, M, W, nTiles, loop, reflect, xc, nlay, act;
+  int widths[QEX1D_MAX_MLP_LAYERS + 1];
+  int woff[QEX1D_MAX_MLP_LAYERS];  // offset of W_l in the packed parameter array
+  int boff[QEX1D_MAX_MLP_LAYERS];  // offset of b_l
+  int wmax;                        // widest hidden layer
+  double dx, tol, xc_c, xc_p, xc_eps, norm;
+  double Td, Tb, Tc;  // kinetic stencil: diagonal, first and second off-diagonal
+  // inputs
+  const double* table;
+  const double* vext;
+  const double* n0;
+  const int* ne;
+  const int* win;
+  const double* mlp;
+  // outputs
+  double* dens;
+  double* etot;
+  double* vh;
+  double* vxc;
+  double* exc;
+  double* gap;
+  uint8_t* conv;
+  double* eigs_out;
+  // workspace
+  double* o_raw;     // [B][N] output density of the last computed iteration
+  double* o_xc;      // [B][N] density the deferred XC energy is evaluated at
+  double* vks;       // [B][N]
+  double* hist;      // [B][M][N] density differences (ring)
+  double* vec;       // [B][K][N] eigenvectors (2-norm 1)
+  double* eig;       // [B][K]
+  double* eig_prev;  // [B][K]
+  double* eig_delta; // [B][K]
+  double* terms;     // [B][T][4]  E_eig, E_ks, E_ext
+  double* partH;     // [B][T][nTiles]
+  double* partX;     // [B][T][nTiles]
+  double* partF;     // [B][T][nTiles]  sum f(n_in) for GLOBAL xc_energy_density
+  int* src;          // [B][T]
+  int* st;           // [B][4]  frozen, pending, warm_valid, -
+  double* mlp_e;     // [B][N]
+  double* mlp_G;     // [B][N][W]
+  double* mlp_eout;  // [B][N]
+};
+
+__device__ __forceinline__ bool xc_is_local(int xc) { return xc != QEX1D_XC_POWER_GLOBAL; }
+
+// pointwise XC at density x: energy density (local: e; global: f), potential contribution
+__device__ __forceinline__ void xc_point(const Dev1& a, double x, double* e, double* v) {
+  switch (a.xc) {
+    case QEX1D_XC_LDA_EXP: {
+      XcOut o = lda_exp(x);
+      *e = o.e;
+      *v = o.v;
+      return;
+    }
+    case QEX1D_XC_POWER_LOCAL: {
+      const double t = x + a.xc_eps;
+      const double pw = pow(t, a.xc_p);
+      *e = a.xc_c * pw;
+      *v = a.xc_c * pw + x * a.xc_c * a.xc_p * pow(t, a.xc_p - 1.0);
+      return;
+    }
+    case QEX1D_XC_POWER_GLOBAL: {
+      const double t = x + a.xc_eps;
+      *e = a.xc_c * pow(t, a.xc_p);
+      *v = a.dx * a.xc_c * a.xc_p * pow(t, a.xc_p - 1.0);
+      return;
+    }
+    default:
+      *e = 0.0;
+      *v = 0.0;
+      return;
+  }
+}
+
+// flip-and-average map inside the system's reflection window
+__device__ __forceinline__ int mirror(int i, int lo, int hi) { return (i >= lo && i <= hi) ? lo + hi - i : i; }
+
+__device__ __forceinline__ const double* input_density(const Dev1& a, int s, int k) {
+  return k == 0 ? a.n0 + (size_t)s * a.N : a.dens + ((size_t)s * a.T + (k - 1)) * a.N;
+}
+
+// --------------------------------------------------------------------------------------------
+// k_init: reset per-system state (a kernel rather than a memset, so it can sit in a graph)
+// --------------------------------------------------------------------------------------------
+__global__ void k_init(Dev1 a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.B) return;
+  a.st[4 * s + 0] = 0;
+  a.st[4 * s + 1] = -1;
+  a.st[4 * s + 2] = 0;
+  a.st[4 * s + 3] = 0;
+}
+
+// --------------------------------------------------------------------------------------------
+// k_mlp: windowed per-point MLP.  Workgroup = MLP_P points x 8 lanes; lane q of a point owns
+// units q, q+8, ... of every layer.  Pass 1 (input density, unless frozen): e and the
+// input-VJP G[i][o] = m_i de_i/dx_i[o].  Pass 2 (pending output density): e only.
+// --------------------------------------------------------------------------------------------
+__device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_lds, int lo, int hi, int i, int q,
+                               int pl, double* actA, double* actB, double* dact, bool backward,
+                               double* e_out, double* G_out) {
+  // dens_lds holds the (already reflected when needed) density on the whole grid
+  const int N = a.N, W = a.W, h = W / 2, stride = a.wmax + 1;
+  const bool valid = i < N;
+  double* in = actA + pl * stride;
+  // window input / norm
+  for (int o = q; o < W; o += 8) {
+    const int j = i + o - h;
+    in[o] = (valid && j >= 0 && j < N) ? dens_lds[j] / a.norm : 0.0;
+  }
+  __syncthreads();
+  double* cur = actA;
+  double* nxt = actB;
+  const int L = a.nlay;
+  // forward through hidden layers
+  for (int l = 0; l < L - 1; ++l) {
+    const int fi = a.widths[l], fo = a.widths[l + 1];
+    const double* Wl = a.mlp + a.woff[l];
+    const double* bl = a.mlp + a.boff[l];
+    const double* x = cur + pl * stride;
+    for (int u = q; u < fo; u += 8) {
+      double z = bl[u];
+      for (int kk = 0; kk < fi; ++kk) z = fma(x[kk], Wl[kk * fo + u], z);
+      double dz;
+      const double y = act_fwd(a.act, z, &dz);
+      nxt[pl * stride + u] = y;
+      if (backward) dact[((size_t)l * MLP_P + pl) * stride + u] = dz;
+    }
+    __syncthreads();
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  // output layer (fan_out 1): lane-partial dot products reduced over the 8 lanes of the point
+  const int fi = a.widths[L - 1];
+  const double* WL = a.mlp + a.woff[L - 1];
+  double part = 0.0;
+  for (int kk = q; kk < fi; kk += 8) part = fma(cur[pl * stride + kk], WL[kk], part);
+  // 8-lane reduction (lanes of a point are contiguous: pl*8 + q)
+  part += __shfl_xor(part, 1, 64);
+  part += __shfl_xor(part, 2, 64);
+  part += __shfl_xor(part, 4, 64);
+  const double e = part + a.mlp[a.boff[L - 1]];
+  if (q == 0 && valid) *e_out = e;
+  if (!backward) {
+    __syncthreads();
+    return;
+  }
+  // backward with cotangent m_i on e_i
+  const double m = valid ? dens_lds[i] : 0.0;
+  // gradient w.r.t. last hidden activations
+  double* g = nxt;  // free buffer
+  for (int kk = q; kk < fi; kk += 8)
+    g[pl * stride + kk] = m * WL[kk] * dact[((size_t)(L - 2) * MLP_P + pl) * stride + kk];
+  __syncthreads();
+  double* gcur = g;
+  double* gnxt = cur;
+  for (int l = L - 2; l >= 1; --l) {
+    // layer l maps widths[l] -> widths[l+1]; gradient w.r.t. its input (widths[l] units)
+    const int fi2 = a.widths[l], fo2 = a.widths[l + 1];
+    const double* Wl = a.mlp + a.woff[l];
+    const double* gz = gcur + pl * stride;
+    for (int kk = q; kk < fi2; kk += 8) {
+      double acc = 0.0;
+      for (int u = 0; u < fo2; ++u) acc = fma(gz[u], Wl[kk * fo2 + u], acc);
+      gnxt[pl * stride + kk] = acc * dact[((size_t)(l - 1) * MLP_P + pl) * stride + kk];
+    }
+    __syncthreads();
+    double* t = gcur;
+    gcur = gnxt;
+    gnxt = t;
+  }
+  // first layer: gradient w.r.t. the window inputs
+  {
+    const int fo1 = a.widths[1];
+    const double* W0 = a.mlp + a.woff[0];
+    const double* gz = gcur + pl * stride;
+    for (int o = q; o < W; o += 8) {
+      double acc = 0.0;
+      for (int u = 0; u < fo1; ++u) acc = fma(gz[u], W0[o * fo1 + u], acc);
+      if (valid) G_out[o] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k) {
+  extern __shared__ double smem[];
+  const int s = blockIdx.y;
+  const int frozen = a.st[4 * s + 0];
+  const int pending = a.st[4 * s + 1];
+  const bool doPot = (k < a.T) && !frozen;
+  const bool doE = pending >= 0;
+  if (!doPot && !doE) return;
+  const int N = a.N, stride = a.wmax + 1;
+  double* dens = smem;                      // N
+  double* actA = dens + N;                  // MLP_P * stride
+  double* actB = actA + MLP_P * stride;     // MLP_P * stride
+  double* dact = actB + MLP_P * stride;     // (L-1) * MLP_P * stride
+  const int pl = threadIdx.x >> 3, q = threadIdx.x & 7;
+  const int i = blockIdx.x * MLP_P + pl;
+  int lo = 0, hi = N - 1;
+  const bool sym = a.reflect && xc_is_local(a.xc);
+  if (a.reflect) {
+    lo = a.win[2 * s];
+    hi = a.win[2 * s + 1];
+  }
+  if (doPot) {
+    const double* nin = input_density(a, s, k);
+    for (int t = threadIdx.x; t < N; t += NT) {
+      double v = nin[t];
+      if (sym) v = 0.5 * (v + nin[mirror(t, lo, hi)]);
+      dens[t] = v;
+    }
+    __syncthreads();
+    double eo;
+    mlp_point_pass(a, dens, lo, hi, i, q, pl, actA, actB, dact, true, &eo,
+                   a.mlp_G + ((size_t)s * N + (i < N ? i : 0)) * a.W);
+    if (q == 0 && i < N) a.mlp_e[(size_t)s * N + i] = eo;
+    __syncthreads();
+  }
+  if (doE) {
+    const double* ox = a.o_xc + (size_t)s * N;
+    for (int t = threadIdx.x; t < N; t += NT) dens[t] = ox[t];
+    __syncthreads();
+    double eo;
+    mlp_point_pass(a, dens, lo, hi, i, q, pl, actA, actB, dact, false, &eo, nullptr);
+    if (q == 0 && i < N) a.mlp_eout[(size_t)s * N + i] = eo;
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// k_potential
+// --------------------------------------------------------------------------------------------
+__device__ double mlp_v_unsym(const Dev1& a, int s, int i) {
+  const int N = a.N, W = a.W, h = W / 2;
+  const double* G = a.mlp_G + (size_t)s * N * W;
+  double v = a.mlp_e[(size_t)s * N + i];
+  double acc = 0.0;
+  for (int o = 0; o < W; ++o) {
+    const int p = i - o + h;  // point whose window slot o reads density i
+    if (p >= 0 && p < N) acc += G[(size_t)p * W + o];
+  }
+  return v + acc / a.norm;
+}
+
+__global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k) {
+  extern __shared__ double smem[];
+  const int s = blockIdx.y, tile = blockIdx.x;
+  const int frozen = a.st[4 * s + 0];
+  const int pending = a.st[4 * s + 1];
+  const bool doPot = (k < a.T) && !frozen;
+  const bool doE = pending >= 0;
+  if (!doPot && !doE) return;
+  const int N = a.N;
+  double* tab = smem;        // N
+  double* nin = tab + N;     // N
+  double* oraw = nin + N;    // N
+  double* red = oraw + N;    // 2 * NT
+  double* red2 = red + 2 * NT;  // 3 * NWAVE
+  const double* nsrc = doPot ? input_density(a, s, k) : nullptr;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    tab[t] = a.table[t];
+    nin[t] = doPot ? nsrc[t] : 0.0;
+    oraw[t] = doE ? a.o_raw[(size_t)s * N + t] : 0.0;
+  }
+  __syncthreads();
+  const int li = threadIdx.x & (TILE - 1), qd = threadIdx.x / TILE;  // 4 j-chunks
+  const int i = tile * TILE + li;
+  const int chunk = (N + 3) / 4;
+  const int j0 = qd * chunk, j1 = min(N, j0 + chunk);
+  double accV = 0.0, accE = 0.0;
+  if (i < N) {
+    for (int j = j0; j < j1; ++j) {
+      const double t = tab[abs(i - j)];
+      accV = fma(t, nin[j], accV);
+      accE = fma(t, oraw[j], accE);
+    }
+  }
+  red[threadIdx.x] = accV;
+  red[NT + threadIdx.x] = accE;
+  __syncthreads();
+  double eH = 0.0, eX = 0.0, fsum = 0.0;
+  if (qd == 0 && i < N) {
+    const double KV = ((red[li] + red[TILE + li]) + red[2 * TILE + li]) + red[3 * TILE + li];
+    const double KE = ((red[NT + li] + red[NT + TILE + li]) + red[NT + 2 * TILE + li]) + red[NT + 3 * TILE + li];
+    int lo = 0, hi = N - 1;
+    if (a.reflect) {
+      lo = a.win[2 * s];
+      hi = a.win[2 * s + 1];
+    }
+    const bool sym = a.reflect && xc_is_local(a.xc);
+    if (doPot) {
+      const size_t o = ((size_t)s * a.T + k) * N + i;
+      const double vH = a.dx * KV;
+      double e = 0.0, v = 0.0;
+      if (a.xc == QEX1D_XC_LOCAL_MLP) {
+        const int im = sym ? mirror(i, lo, hi) : i;
+        const double vu = mlp_v_unsym(a, s, i);
+        const double eu = a.mlp_e[(size_t)s * N + i];
+        if (im != i) {
+          v = 0.5 * (vu + mlp_v_unsym(a, s, im));
+          e = 0.5 * (eu + a.mlp_e[(size_t)s * N + im]);
+        } else {
+          v = vu;
+          e = eu;
+        }
+      } else {
+        double x = nin[i];
+        if (sym) x = 0.5 * (x + nin[mirror(i, lo, hi)]);
+        xc_point(a, x, &e, &v);
+      }
+      a.vh[o] = vH;
+      a.vxc[o] = v;
+      if (xc_is_local(a.xc))
+        a.exc[o] = e;
+      else
+        fsum = e;
+      a.vks[(size_t)s * N + i] = vH + v + a.vext[(size_t)s * N + i];
+    }
+    if (doE) {
+      const double ox = a.o_xc[(size_t)s * N + i];
+      eH = 0.5 * a.dx * a.dx * oraw[i] * KE;
+      if (a.xc == QEX1D_XC_LOCAL_MLP) {
+        eX = a.dx * a.mlp_eout[(size_t)s * N + i] * ox;
+      } else if (a.xc != QEX1D_XC_NONE) {
+        double e, v;
+        xc_point(a, ox, &e, &v);
+        eX = xc_is_local(a.xc) ? a.dx * e * ox : a.dx * e;
+      }
+    }
+  }
+  double sums[3] = {eH, eX, fsum};
+  block_sum<3>(sums, red2);
+  if (threadIdx.x == 0) {
+    if (doE) {
+      a.partH[((size_t)s * a.T + pending) * a.nTiles + tile] = sums[0];
+      a.partX[((size_t)s * a.T + pending) * a.nTiles + tile] = sums[1];
+    }
+    if (doPot) a.partF[((size_t)s * a.T + k) * a.nTiles + tile] = sums[2];
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// Eigensolver for the lowest eigenpairs of the pentadiagonal KS Hamiltonian
+// --------------------------------------------------------------------------------------------
+struct EigArgs {
+  int B, N, K;
+  double Td, Tb, Tc;
+  const double* pot;      // [B][N] potential on the diagonal
+  const int* ne;          // [B]
+  const int* st;          // [B][4] (frozen flag at +0, warm flag at +2) or null
+  double* vec;            // [B][K][N]
+  double* eig;            // [B][K]
+  const double* eig_prev; // [B][K] or null
+  const double* eig_delta;
+};
+
+__device__ __forceinline__ double start_value(int i, int j) {
+  // deterministic pseudo-random start vector for inverse iteration
+  unsigned h = (unsigned)i * 2654435761u ^ ((unsigned)j * 40503u + 0x9e3779b9u);
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  return ((double)(h & 0xffffff) / 16777216.0) - 0.5;
+}
+
+__global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
+  extern __shared__ double smem[];
+  const int j = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+  if (e.st && e.st[4 * s + 0]) return;
+  const int ne = e.ne[s];
+  const int keig = ne / 2 + 1;
+  if (j >= keig) return;
+  const int N = e.N;
+  double* a = smem;         // N    diagonal T_ii + v_i
+  double* y = a + N;        // N    inverse-iteration vector
+  double* F = y + N;        // 4N   LDL^T factors and forward-solve values
+  double* red = F + 4 * N;  // 16
+  __shared__ double sh_lo, sh_hi;
+  __shared__ int sh_first[NWAVE];
+  const double* pot = e.pot + (size_t)s * N;
+  double amin = 1e300, amax = -1e300, aabs = 0.0;
+  for (int t = tid; t < N; t += NT) {
+    const double v = e.Td + pot[t];
+    a[t] = v;
+    amin = fmin(amin, v);
+    amax = fmax(amax, v);
+    aabs = fmax(aabs, fabs(v));
+  }
+  const double off = 2.0 * (fabs(e.Tb) + fabs(e.Tc));
+  const double gl = -block_max(-amin, red) - off;
+  const double gu = block_max(amax, red) + off;
+  const double hnorm = block_max(aabs, red) + off;
+  double lo = gl, hi = gu;
+  const bool warm = e.st && e.st[4 * s + 2] && e.eig_prev;
+  if (warm) {
+    const double p = e.eig_prev[(size_t)s * e.K + j];
+    const double w = 3.0 * fabs(e.eig_delta[(size_t)s * e.K + j]) + 1e-9 * fmax(1.0, fabs(p));
+    lo = fmax(gl, p - w);
+    hi = fmin(gu, p + w);
+    if (!(hi > lo)) {
+      lo = gl;
+      hi = gu;
+    }
+  }
+  // Multisection: NT shifts per round (both ends included) until the bracket of the j-th
+  // eigenvalue is 1e-9 relative; the Rayleigh quotient of the inverse-iteration vector then
+  // gives the eigenvalue to ~1e-13.
+  for (int round = 0; round < 80; ++round) {
+    const double mid = 0.5 * (lo + hi);
+    const double tol = 1e-9 * fmax(1.0, fabs(mid)) + 64.0 * EPS * hnorm;
+    if (hi - lo <= tol) break;
+    const double step = (hi - lo) / (double)(NT - 1);
+    const double sig = tid == NT - 1 ? hi : fma((double)tid, step, lo);
+    const int cnt = sturm_count(a, N, e.Tb, e.Tc, sig);
+    const unsigned long long m = __ballot(cnt > j);
+    if ((tid & 63) == 0) sh_first[tid >> 6] = m ? (__ffsll((long long)m) - 1) + (tid & ~63) : NT;
+    __syncthreads();
+    int first = NT;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) first = min(first, sh_first[q]);
+    if (tid == 0) {
+      if (first == 0) {
+        // the eigenvalue lies below the bracket: widen down to the Gershgorin bound
+        sh_hi = lo;
+        sh_lo = lo > gl ? gl : lo;
+      } else if (first == NT) {
+        sh_lo = hi;
+        sh_hi = hi < gu ? gu : hi;
+      } else {
+        sh_lo = first - 1 == 0 ? lo : fma((double)(first - 1), step, lo);
+        sh_hi = first == NT - 1 ? hi : fma((double)first, step, lo);
+      }
+    }
+    __syncthreads();
+    const double nlo = sh_lo, nhi = sh_hi;
+    __syncthreads();
+    if (nlo == lo && nhi == hi) break;  // no progress possible (rounding floor)
+    lo = nlo;
+    hi = nhi;
+  }
+  const double sigma = 0.5 * (lo + hi);
+  // inverse iteration: one LDL^T factorisation, two solves, normalised in between
+  double* v = e.vec + ((size_t)s * e.K + j) * N;
+  for (int t = tid; t < N; t += NT) y[t] = warm ? v[t] + 1e-3 * start_value(t, j) : start_value(t, j);
+  __syncthreads();
+  for (int it = 0; it < 2; ++it) {
+    if (tid == 0) {
+      if (it == 0)
+        ldlt_factor_forward(a, y, N, e.Tb, e.Tc, sigma, EPS * hnorm, F);
+      else
+        ldlt_forward(y, N, F);
+      ldlt_backward(F, N, y);
+    }
+    __syncthreads();
+    double nrm[1] = {0.0};
+    for (int t = tid; t < N; t += NT) nrm[0] = fma(y[t], y[t], nrm[0]);
+    block_sum<1>(nrm, red);
+    const double sc = 1.0 / sqrt(nrm[0]);
+    for (int t = tid; t < N; t += NT) y[t] *= sc;
+    __syncthreads();
+  }
+  for (int t = tid; t < N; t += NT) v[t] = y[t];
+  if (tid == 0) e.eig[(size_t)s * e.K + j] = sigma;
+}
+
+// --------------------------------------------------------------------------------------------
+// Orthonormalise the computed eigenvectors (classical Gram-Schmidt in eigenvalue order, 2-norm),
+// replace the bisection eigenvalues by Rayleigh quotients, and accumulate the density of the
+// first N_e spin-orbitals (each orbital doubly occupied, the last one singly when N_e is odd).
+// a (LDS) holds the Hamiltonian diagonal.  PPT = points per thread.
+// --------------------------------------------------------------------------------------------
+constexpr int PPT = (QEX1D_MAX_GRIDS + NT - 1) / NT;
+
+__device__ void orbitals_to_density(double* vecs, int N, int keig, int ne, double dx, const double* a, double b,
+                                    double c, double* lam, double* dens_reg, double* red) {
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) dens_reg[p] = 0.0;
+  const int nocc = (ne + 1) / 2;
+  for (int jj = 0; jj < keig; ++jj) {
+    double x[PPT];
+    double* vj = vecs + (size_t)jj * N;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int t = threadIdx.x + p * NT;
+      x[p] = t < N ? vj[t] : 0.0;
+    }
+    for (int i0 = 0; i0 < jj; i0 += 4) {
+      double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (i0 + q < jj) {
+          const double* vi = vecs + (size_t)(i0 + q) * N;
+#pragma unroll
+          for (int p = 0; p < PPT; ++p) {
+            const int t = threadIdx.x + p * NT;
+            if (t < N) d[q] = fma(x[p], vi[t], d[q]);
+          }
+        }
+      }
+      block_sum<4>(d, red);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (i0 + q < jj) {
+          const double* vi = vecs + (size_t)(i0 + q) * N;
+#pragma unroll
+          for (int p = 0; p < PPT; ++p) {
+            const int t = threadIdx.x + p * NT;
+            if (t < N) x[p] = fma(-d[q], vi[t], x[p]);
+          }
+        }
+      }
+    }
+    double nrm[1] = {0.0};
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) nrm[0] = fma(x[p], x[p], nrm[0]);
+    block_sum<1>(nrm, red);
+    const double sc = 1.0 / sqrt(nrm[0]);
+    const double occ = jj < nocc ? (double)min(2, ne - 2 * jj) : 0.0;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int t = threadIdx.x + p * NT;
+      x[p] *= sc;
+      if (t < N) {
+        vj[t] = x[p];
+        dens_reg[p] = fma(occ, x[p] * x[p] / dx, dens_reg[p]);
+      }
+    }
+    __syncthreads();
+    // Rayleigh quotient x^T H x with H = diag(a) + b (|i-j| = 1) + c (|i-j| = 2)
+    double rq[1] = {0.0};
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int t = threadIdx.x + p * NT;
+      if (t < N) {
+        double hx = a[t] * x[p];
+        if (t >= 1) hx = fma(b, vj[t - 1], hx);
+        if (t + 1 < N) hx = fma(b, vj[t + 1], hx);
+        if (t >= 2) hx = fma(c, vj[t - 2], hx);
+        if (t + 2 < N) hx = fma(c, vj[t + 2], hx);
+        rq[0] = fma(x[p], hx, rq[0]);
+      }
+    }
+    block_sum<1>(rq, red);
+    if (threadIdx.x == 0) lam[jj] = rq[0];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(NT) void k_density(Dev1 a, int k, double alpha_k) {
+  extern __shared__ double smem[];
+  __shared__ double red[NWAVE * 8];
+  __shared__ double lam[QEX1D_MAX_ELECTRONS / 2 + 1];
+  const int s = blockIdx.x, tid = threadIdx.x, N = a.N, T = a.T;
+  const bool frozen = a.st[4 * s + 0] != 0;
+  const bool warm_before = a.st[4 * s + 2] != 0;
+  const size_t cur = ((size_t)s * T + k) * N;
+  if (frozen) {
+    const size_t prv = cur - N;
+    for (int t = tid; t < N; t += NT) {
+      a.dens[cur + t] = a.dens[prv + t];
+      a.vh[cur + t] = a.vh[prv + t];
+      a.vxc[cur + t] = a.vxc[prv + t];
+      if (xc_is_local(a.xc)) a.exc[cur + t] = a.exc[prv + t];
+    }
+    if (tid == 0) {
+      a.gap[(size_t)s * T + k] = a.gap[(size_t)s * T + k - 1];
+      a.conv[(size_t)s * T + k] = 1;
+      a.src[(size_t)s * T + k] = a.src[(size_t)s * T + k - 1];
+      a.st[4 * s + 1] = -1;
+    }
+    if (a.eigs_out && tid < a.K)
+      a.eigs_out[((size_t)s * T + k) * a.K + tid] = a.eigs_out[((size_t)s * T + k - 1) * a.K + tid];
+    return;
+  }
+  const int ne = a.ne[s];
+  const int nocc = (ne + 1) / 2;
+  const int keig = ne / 2 + 1;
+  double* hd = smem;  // Hamiltonian diagonal of this iteration
+  for (int t = tid; t < N; t += NT) hd[t] = a.Td + a.vks[(size_t)s * N + t];
+  __syncthreads();
+  double dn[PPT];
+  orbitals_to_density(a.vec + (size_t)s * a.K * N, N, keig, ne, a.dx, hd, a.Tb, a.Tc, lam, dn, red);
+  double eeig = 0.0;
+  for (int jj = 0; jj < nocc; ++jj) eeig += (double)min(2, ne - 2 * jj) * lam[jj];
+  const double gap = lam[ne / 2] - lam[(ne - 1) / 2];
+  // energies of the unmixed output density
+  double ed[2] = {0.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int t = tid + p * NT;
+    if (t < N) {
+      ed[0] = fma(a.vks[(size_t)s * N + t], dn[p], ed[0]);
+      ed[1] = fma(a.vext[(size_t)s * N + t], dn[p], ed[1]);
+    }
+  }
+  block_sum<2>(ed, red);
+  // reflection, differences, mixing
+  int lo = 0, hi = N - 1;
+  if (a.reflect) {
+    lo = a.win[2 * s];
+    hi = a.win[2 * s + 1];
+  }
+  double* oraw = a.o_raw + (size_t)s * N;
+  double* oxc = a.o_xc + (size_t)s * N;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int t = tid + p * NT;
+    if (t < N) oraw[t] = dn[p];
+  }
+  __syncthreads();
+  const double* nin = input_density(a, s, k);
+  const int M = a.M;
+  const int slot = k % M;
+  const int cnt = min(M, k + 1);
+  double* hist = a.hist + (size_t)s * M * N;
+  double mse[2] = {0.0, 0.0};
+  const bool symxc = a.reflect && xc_is_local(a.xc);
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int t = tid + p * NT;
+    if (t < N) {
+      double of = dn[p];
+      if (a.reflect) {
+        const int tm = mirror(t, lo, hi);
+        of = 0.5 * (oraw[t] + oraw[tm]);
+      }
+      oxc[t] = symxc ? of : dn[p];
+      const double d = of - nin[t];
+      hist[(size_t)slot * N + t] = d;
+      double acc = 0.0;
+      for (int q = 0; q < cnt; ++q) {
+        const int sl = ((k - cnt + 1 + q) % M + M) % M;
+        acc += q == cnt - 1 ? d : hist[(size_t)sl * N + t];
+      }
+      const double mixed = nin[t] + alpha_k * (acc / (double)cnt);
+      a.dens[cur + t] = mixed;
+      mse[0] = fma(d, d, mse[0]);
+      const double dm = mixed - nin[t];
+      mse[1] = fma(dm, dm, mse[1]);
+    }
+  }
+  block_sum<2>(mse, red);
+  if (tid == 0) {
+    const double tol = a.tol;
+    bool conv_out, frozen_next;
+    if (a.loop == QEX1D_LOOP_PYTHON) {
+      conv_out = mse[0] / (double)N < tol;
+      frozen_next = conv_out;
+    } else {
+      conv_out = false;
+      frozen_next = mse[1] / (double)N < tol;
+    }
+    const size_t sk = (size_t)s * T + k;
+    a.conv[sk] = conv_out ? 1 : 0;
+    a.gap[sk] = gap;
+    a.src[sk] = k;
+    a.terms[sk * 4 + 0] = eeig;
+    a.terms[sk * 4 + 1] = a.dx * ed[0];
+    a.terms[sk * 4 + 2] = a.dx * ed[1];
+    a.st[4 * s + 0] = frozen_next ? 1 : 0;
+    a.st[4 * s + 1] = k;
+    a.st[4 * s + 2] = 1;
+  }
+  if (tid < keig) {
+    const size_t ix = (size_t)s * a.K + tid;
+    const double l = lam[tid];
+    a.eig[ix] = l;
+    a.eig_delta[ix] = warm_before ? l - a.eig_prev[ix] : 1.0;
+    a.eig_prev[ix] = l;
+    if (a.eigs_out) a.eigs_out[((size_t)s * T + k) * a.K + tid] = l;
+  } else if (a.eigs_out && tid < a.K) {
+    a.eigs_out[((size_t)s * T + k) * a.K + tid] = NAN;
+  }
+}
+
+__global__ void k_finalize(Dev1 a) {
+  const int s = blockIdx.x;
+  for (int k = threadIdx.x; k < a.T; k += blockDim.x) {
+    const size_t sk = (size_t)s * a.T + k;
+    const int src = a.src[sk];
+    const size_t ss = (size_t)s * a.T + src;
+    double eh = 0.0, ex = 0.0, fs = 0.0;
+    for (int t = 0; t < a.nTiles; ++t) {
+      eh += a.partH[ss * a.nTiles + t];
+      ex += a.partX[ss * a.nTiles + t];
+      fs += a.partF[ss * a.nTiles + t];
+    }
+    const double* tm = a.terms + ss * 4;
+    a.etot[sk] = (((tm[0] - tm[1]) + eh) + ex) + tm[2];
+    if (!xc_is_local(a.xc)) a.exc[sk] = fs;
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// non-interacting solve: density of the first N_e spin-orbitals
+// --------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_nonint_density(int N, int K, double dx, double Td, double Tb, double Tc,
+                                                       const double* pot, const int* nes, double* vec,
+                                                       double* density, double* etot, double* gap, double* eig_out,
+                                                       double* wf_out) {
+  extern __shared__ double smem[];
+  __shared__ double red[NWAVE * 8];
+  __shared__ double lam[QEX1D_MAX_ELECTRONS / 2 + 1];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int ne = nes[s], nocc = (ne + 1) / 2, keig = ne / 2 + 1;
+  double* hd = smem;
+  for (int t = tid; t < N; t += NT) hd[t] = Td + pot[(size_t)s * N + t];
+  __syncthreads();
+  double dn[PPT];
+  orbitals_to_density(vec + (size_t)s * K * N, N, keig, ne, dx, hd, Tb, Tc, lam, dn, red);
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int t = tid + p * NT;
+    if (t < N) density[(size_t)s * N + t] = dn[p];
+  }
+  if (tid == 0) {
+    double ee = 0.0;
+    for (int jj = 0; jj < nocc; ++jj) ee += (double)min(2, ne - 2 * jj) * lam[jj];
+    if (etot) etot[s] = ee;
+    if (gap) gap[s] = lam[ne / 2] - lam[(ne - 1) / 2];
+  }
+  if (eig_out && tid < K) eig_out[(size_t)s * K + tid] = tid < keig ? lam[tid] : NAN;
+  if (wf_out) {
+    const double sc = 1.0 / sqrt(dx);
+    for (int jj = 0; jj < K; ++jj) {
+      for (int t = tid; t < N; t += NT)
+        wf_out[((size_t)s * K + jj) * N + t] = jj < keig ? vec[((size_t)s * K + jj) * N + t] * sc : 0.0;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// external potential and Hartree potential
+// --------------------------------------------------------------------------------------------
+__global__ void k_external(int B, int N, const double* grids, int A, const double* loc, const double* z, int kind,
+                           double p0, double p1, double* out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * N) return;
+  const int s = idx / N, i = idx % N;
+  const double x = grids[i];
+  double v = 0.0;
+  for (int q = 0; q < A; ++q) {
+    const double r = x - loc[(size_t)s * A + q];
+    const double u = kind == QEX_INTERACTION_EXPONENTIAL ? p0 * exp(-fabs(r) * p1) : 1.0 / sqrt(r * r + p0);
+    v = fma(-z[(size_t)s * A + q], u, v);
+  }
+  out[idx] = v;
+}
+
+__global__ __launch_bounds__(NT) void k_hartree(int N, double dx, const double* table, const double* dens,
+                                                double* out) {
+  extern __shared__ double smem[];
+  const int s = blockIdx.y;
+  double* tab = smem;
+  double* n = smem + N;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    tab[t] = table[t];
+    n[t] = dens[(size_t)s * N + t];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= N) return;
+  double acc = 0.0;
+  for (int j = 0; j < N; ++j) acc = fma(tab[abs(i - j)], n[j], acc);
+  out[(size_t)s * N + i] = dx * acc;
+}
+
+// --------------------------------------------------------------------------------------------
+// host-side layout
+// --------------------------------------------------------------------------------------------
+struct Layout {
+  size_t o_raw, o_xc, vks, hist, vec, eig, eig_prev, eig_delta, terms, partH, partX, partF, src, st, mlp_e, mlp_G,
+      mlp_eout, total;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Layout layout1d(const qex1d_problem* p) {
+  const size_t B = p->batch, N = p->num_grids, T = p->num_iterations;
+  const size_t K = p->max_electrons / 2 + 1, M = p->num_mixing_iterations;
+  const size_t nT = (N + TILE - 1) / TILE;
+  const bool mlp = p->xc_kind == QEX1D_XC_LOCAL_MLP;
+  const size_t W = mlp ? p->mlp_widths[0] : 0;
+  Layout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  L.o_raw = take(B * N * 8);
+  L.o_xc = take(B * N * 8);
+  L.vks = take(B * N * 8);
+  L.hist = take(B * M * N * 8);
+  L.vec = take(B * K * N * 8);
+  L.eig = take(B * K * 8);
+  L.eig_prev = take(B * K * 8);
+  L.eig_delta = take(B * K * 8);
+  L.terms = take(B * T * 4 * 8);
+  L.partH = take(B * T * nT * 8);
+  L.partX = take(B * T * nT * 8);
+  L.partF = take(B * T * nT * 8);
+  L.src = take(B * T * 4);
+  L.st = take(B * 4 * 4);
+  L.mlp_e = take(mlp ? B * N * 8 : 0);
+  L.mlp_G = take(mlp ? B * N * W * 8 : 0);
+  L.mlp_eout = take(mlp ? B * N * 8 : 0);
+  L.total = off;
+  return L;
+}
+
+int validate1d(const qex1d_problem* p) {
+  if (!p) return fail(QEX_E_BADARG, "null problem");
+  if (p->batch < 1) return fail(QEX_E_BADARG, "batch must be >= 1 (got %d)", p->batch);
+  if (p->num_grids < 5 || p->num_grids > QEX1D_MAX_GRIDS)
+    return fail(QEX_E_BADARG, "num_grids must be in [5, %d] (got %d)", QEX1D_MAX_GRIDS, p->num_grids);
+  if (!(p->dx > 0.0)) return fail(QEX_E_BADARG, "dx must be positive");
+  if (p->num_iterations < 1) return fail(QEX_E_BADARG, "num_iterations must be >= 1");
+  if (p->max_electrons < 1 || p->max_electrons > QEX1D_MAX_ELECTRONS)
+    return fail(QEX_E_BADARG, "max_electrons must be in [1, %d]", QEX1D_MAX_ELECTRONS);
+  if (p->max_electrons / 2 + 1 > p->num_grids) return fail(QEX_E_BADARG, "too many electrons for the grid");
+  if (p->num_mixing_iterations < 1) return fail(QEX_E_BADARG, "num_mixing_iterations must be >= 1");
+  if (p->loop != QEX1D_LOOP_PYTHON && p->loop != QEX1D_LOOP_SCAN) return fail(QEX_E_BADARG, "bad loop kind");
+  if (p->xc_kind < 0 || p->xc_kind > QEX1D_XC_NONE) return fail(QEX_E_BADARG, "bad xc_kind");
+  if (p->xc_kind == QEX1D_XC_LOCAL_MLP) {
+    if (p->mlp_layers < 2 || p->mlp_layers > QEX1D_MAX_MLP_LAYERS)
+      return fail(QEX_E_UNSUPPORTED, "LOCAL_MLP needs 2..%d Dense layers", QEX1D_MAX_MLP_LAYERS);
+    if (p->mlp_widths[0] < 1 || p->mlp_widths[0] > QEX1D_MAX_MLP_WINDOW || (p->mlp_widths[0] % 2) == 0)
+      return fail(QEX_E_UNSUPPORTED, "MLP window must be odd and <= %d", QEX1D_MAX_MLP_WINDOW);
+    if (p->mlp_widths[p->mlp_layers] != 1) return fail(QEX_E_UNSUPPORTED, "MLP output width must be 1");
+    for (int l = 1; l < p->mlp_layers; ++l)
+      if (p->mlp_widths[l] < 1 || p->mlp_widths[l] > QEX1D_MAX_MLP_WIDTH)
+        return fail(QEX_E_UNSUPPORTED, "MLP hidden width must be in [1, %d]", QEX1D_MAX_MLP_WIDTH);
+    if (p->mlp_activation < 0 || p->mlp_activation > QEX_ACT_GELU_TANH) return fail(QEX_E_BADARG, "bad activation");
+    if (!(p->mlp_norm != 0.0)) return fail(QEX_E_BADARG, "mlp_norm must be nonzero");
+  }
+  return QEX_OK;
+}
+
+size_t mlp_lds_bytes(const Dev1& a) {
+  const size_t stride = a.wmax + 1;
+  return ((size_t)a.N + 2 * MLP_P * stride + (size_t)(a.nlay - 1) * MLP_P * stride) * 8;
+}
+
+size_t eigen_lds_bytes(int N) { return (size_t)(6 * N + 16) * 8; }
+
+void kinetic(double dx, double* Td, double* Tb, double* Tc) {
+  // T = -1/2 L / dx^2 with L = (-1/12, 4/3, -5/2, 4/3, -1/12)
+  const double s = -0.5 / (dx * dx);
+  *Td = s * -2.5;
+  *Tb = s * (4.0 / 3.0);
+  *Tc = s * (-1.0 / 12.0);
+}
+
+}  // namespace
+
+// ============================================================================================
+// C ABI
+// ============================================================================================
+extern "C" {
+
+const char* qex_last_error(void) { return g_err; }
+int32_t qex_abi_version(void) { return 1; }
+
+int qex_profile_enable(int32_t enable, int32_t max_launches) {
+  if (enable) {
+    if (max_launches < 1) return fail(QEX_E_BADARG, "max_launches must be >= 1");
+    if (max_launches > g_prof_cap) {
+      ProfRec* np = (ProfRec*)realloc(g_prof, sizeof(ProfRec) * max_launches);
+      if (!np) return fail(QEX_E_BADARG, "out of host memory");
+      g_prof = np;
+      g_prof_cap = max_launches;
+    }
+    const int need = 2 * max_launches;
+    if (need > g_pool_n) {
+      hipEvent_t* nq = (hipEvent_t*)realloc(g_pool, sizeof(hipEvent_t) * need);
+      if (!nq) return fail(QEX_E_BADARG, "out of host memory");
+      g_pool = nq;
+      for (int i = g_pool_n; i < need; ++i) {
+        if (hipEventCreate(&g_pool[i]) != hipSuccess) return fail(QEX_E_LAUNCH, "hipEventCreate failed");
+        g_pool_n = i + 1;
+      }
+    }
+    g_prof_n = 0;
+    g_pool_used = 0;
+  }
+  g_prof_on = enable != 0;
+  return QEX_OK;
+}
+
+int qex_profile_read(double* ms_per_kind, int32_t* launches_per_kind, int32_t nkinds) {
+  if (!ms_per_kind || !launches_per_kind || nkinds < 1) return fail(QEX_E_BADARG, "bad arguments");
+  for (int k = 0; k < nkinds; ++k) {
+    ms_per_kind[k] = 0.0;
+    launches_per_kind[k] = 0;
+  }
+  for (int i = 0; i < g_prof_n; ++i) {
+    float ms = 0.f;
+    if (hipEventSynchronize(g_prof[i].b) != hipSuccess || hipEventElapsedTime(&ms, g_prof[i].a, g_prof[i].b) != hipSuccess)
+      return fail(QEX_E_LAUNCH, "event timing failed");
+    const int k = g_prof[i].kind;
+    if (k < nkinds) {
+      ms_per_kind[k] += ms;
+      launches_per_kind[k] += 1;
+    }
+  }
+  g_prof_n = 0;
+  g_pool_used = 0;
+  return QEX_OK;
+}
+
+size_t qex1d_workspace_bytes(const qex1d_problem* p) {
+  if (validate1d(p) != QEX_OK) return 0;
+  return layout1d(p).total;
+}
+
+int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, void* workspace,
+                    size_t workspace_bytes, void* stream_) {
+  int rc = validate1d(p);
+  if (rc) return rc;
+  if (!in || !out) return fail(QEX_E_BADARG, "null inputs/outputs");
+  if (!in->interaction_table || !in->external_potential || !in->initial_density || !in->num_electrons)
+    return fail(QEX_E_BADARG, "missing input array");
+  if (p->reflect && !in->reflect_window) return fail(QEX_E_BADARG, "reflect needs reflect_window");
+  if (p->xc_kind == QEX1D_XC_LOCAL_MLP && !in->xc_params) return fail(QEX_E_BADARG, "LOCAL_MLP needs xc_params");
+  if (!out->density || !out->total_energy || !out->hartree_potential || !out->xc_potential ||
+      !out->xc_energy_density || !out->gap || !out->converged)
+    return fail(QEX_E_BADARG, "missing output array");
+  const Layout L = layout1d(p);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(QEX_E_WORKSPACE, "workspace too small: need %zu bytes, got %zu", L.total, workspace_bytes);
+  hipStream_t stream = (hipStream_t)stream_;
+  char* ws = (char*)workspace;
+  Dev1 a{};
+  a.B = p->batch;
+  a.N = p->num_grids;
+  a.T = p->num_iterations;
+  a.K = p->max_electrons / 2 + 1;
+  a.M = p->num_mixing_iterations;
+  a.nTiles = (a.N + TILE - 1) / TILE;
+  a.loop = p->loop;
+  a.reflect = p->reflect ? 1 : 0;
+  a.xc = p->xc_kind;
+  a.dx = p->dx;
+  a.tol = p->density_mse_converge_tolerance;
+  a.xc_c = p->xc_c;
+  a.xc_p = p->xc_p;
+  a.xc_eps = p->xc_eps;
+  kinetic(p->dx, &a.Td, &a.Tb, &a.Tc);
+  if (a.xc == QEX1D_XC_LOCAL_MLP) {
+    a.nlay = p->mlp_layers;
+    a.act = p->mlp_activation;
+    a.norm = p->mlp_norm;
+    a.W = p->mlp_widths[0];
+    int off = 0;
+    a.wmax = a.W;
+    for (int l = 0; l <= a.nlay; ++l) a.widths[l] = p->mlp_widths[l];
+    for (int l = 0; l < a.nlay; ++l) {
+      a.woff[l] = off;
+      off += a.widths[l] * a.widths[l + 1];
+      a.boff[l] = off;
+      off += a.widths[l + 1];
+      if (l + 1 < a.nlay) a.wmax = a.wmax > a.widths[l + 1] ? a.wmax : a.widths[l + 1];
+    }
+  }
+  a.table = in->interaction_table;
+  a.vext = in->external_potential;
+  a.n0 = in->initial_density;
+  a.ne = in->num_electrons;
+  a.win = in->reflect_window;
+  a.mlp = in->xc_params;
+  a.dens = out->density;
+  a.etot = out->total_energy;
+  a.vh = out->hartree_potential;
+  a.vxc = out->xc_potential;
+  a.exc = out->xc_energy_density;
+  a.gap = out->gap;
+  a.conv = out->converged;
+  a.eigs_out = out->eigenvalues;
+  a.o_raw = (double*)(ws + L.o_raw);
+  a.o_xc = (double*)(ws + L.o_xc);
+  a.vks = (double*)(ws + L.vks);
+  a.hist = (double*)(ws + L.hist);
+  a.vec = (double*)(ws + L.vec);
+  a.eig = (double*)(ws + L.eig);
+  a.eig_prev = (double*)(ws + L.eig_prev);
+  a.eig_delta = (double*)(ws + L.eig_delta);
+  a.terms = (double*)(ws + L.terms);
+  a.partH = (double*)(ws + L.partH);
+  a.partX = (double*)(ws + L.partX);
+  a.partF = (double*)(ws + L.partF);
+  a.src = (int*)(ws + L.src);
+  a.st = (int*)(ws + L.st);
+  a.mlp_e = (double*)(ws + L.mlp_e);
+  a.mlp_G = (double*)(ws + L.mlp_G);
+  a.mlp_eout = (double*)(ws + L.mlp_eout);
+
+  EigArgs e{};
+  e.B = a.B;
+  e.N = a.N;
+  e.K = a.K;
+  e.Td = a.Td;
+  e.Tb = a.Tb;
+  e.Tc = a.Tc;
+  e.pot = a.vks;
+  e.ne = a.ne;
+  e.st = a.st;
+  e.vec = a.vec;
+  e.eig = a.eig;
+  e.eig_prev = a.eig_prev;
+  e.eig_delta = a.eig_delta;
+
+  const size_t lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
+  const size_t lds_eig = eigen_lds_bytes(a.N);
+  const size_t lds_mlp = a.xc == QEX1D_XC_LOCAL_MLP ? mlp_lds_bytes(a) : 0;
+  if (lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", lds_mlp);
+  const dim3 gpot(a.nTiles, a.B), geig(a.K, a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);
+
+  if (lds_eig > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eig);
+  if (lds_mlp > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_mlp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
+  hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
+  double alpha_k = p->alpha;
+  for (int k = 0; k <= a.T; ++k) {
+    if (a.xc == QEX1D_XC_LOCAL_MLP) {
+      ProfScope ps(0, stream);
+      hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
+    }
+    {
+      ProfScope ps(1, stream);
+      hipLaunchKernelGGL(k_potential, gpot, dim3(NT), lds_pot, stream, a, k);
+    }
+    if (k == a.T) break;  // the last pass only evaluates the final iteration's energies
+    {
+      ProfScope ps(2, stream);
+      hipLaunchKernelGGL(k_eigen, geig, dim3(NT), lds_eig, stream, e);
+    }
+    {
+      ProfScope ps(3, stream);
+      hipLaunchKernelGGL(k_density, dim3(a.B), dim3(NT), (size_t)a.N * 8, stream, a, k, alpha_k);
+    }
+    alpha_k *= p->alpha_decay;
+  }
+  {
+    ProfScope ps(4, stream);
+    hipLaunchKernelGGL(k_finalize, dim3(a.B), dim3(64), 0, stream, a);
+  }
+  return check_launch("qex1d_kohn_sham");
+}
+
+size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, int32_t max_electrons) {
+  if (batch < 1 || num_grids < 5 || num_grids > QEX1D_MAX_GRIDS || max_electrons < 1 ||
+      max_electrons > QEX1D_MAX_ELECTRONS)
+    return 0;
+  const size_t K = max_electrons / 2 + 1;
+  return align256((size_t)batch * K * num_grids * 8) + align256((size_t)batch * K * 8);
+}
+
+int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int32_t max_electrons,
+                               const double* potential, const int32_t* num_electrons, double* density,
+                               double* total_eigen_energies, double* gap, double* eigenvalues, double* wavefunctions,
+                               void* workspace, size_t workspace_bytes, void* stream_) {
+  const size_t need = qex1d_noninteracting_workspace_bytes(batch, num_grids, max_electrons);
+  if (need == 0) return fail(QEX_E_BADARG, "bad sizes (batch %d, num_grids %d, max_electrons %d)", batch, num_grids,
+                             max_electrons);
+  if (!(dx > 0.0)) return fail(QEX_E_BADARG, "dx must be positive");
+  if (max_electrons / 2 + 1 > num_grids) return fail(QEX_E_BADARG, "too many electrons for the grid");
+  if (!potential || !num_electrons || !density) return fail(QEX_E_BADARG, "missing array");
+  if (!workspace || workspace_bytes < need)
+    return fail(QEX_E_WORKSPACE, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+  hipStream_t stream = (hipStream_t)stream_;
+  const int K = max_electrons / 2 + 1;
+  EigArgs e{};
+  e.B = batch;
+  e.N = num_grids;
+  e.K = K;
+  kinetic(dx, &e.Td, &e.Tb, &e.Tc);
+  e.pot = potential;
+  e.ne = num_electrons;
+  e.st = nullptr;
+  e.vec = (double*)workspace;
+  e.eig = (double*)((char*)workspace + align256((size_t)batch * K * num_grids * 8));
+  if (eigen_lds_bytes(num_grids) > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)eigen_lds_bytes(num_grids));
+  hipLaunchKernelGGL(k_eigen, dim3(K, batch), dim3(NT), eigen_lds_bytes(num_grids), stream, e);
+  hipLaunchKernelGGL(k_nonint_density, dim3(batch), dim3(NT), (size_t)num_grids * 8, stream, num_grids, K, dx, e.Td,
+                     e.Tb, e.Tc, potential, num_electrons, e.vec, density, total_eigen_energies, gap, eigenvalues,
+                     wavefunctions);
+  return check_launch("qex1d_solve_noninteracting");
+}
+
+int qex1d_external_potential(int32_t batch, int32_t num_grids, const double* grids, int32_t num_nuclei,
+                             const double* locations, const double* nuclear_charges, int32_t interaction, double p0,
+                             double p1, double* out, void* stream_) {
+  if (batch < 1 || num_grids < 1 || num_nuclei < 0) return fail(QEX_E_BADARG, "bad sizes");
+  if (!grids || !out || (num_nuclei > 0 && (!locations || !nuclear_charges))) return fail(QEX_E_BADARG, "null array");
+  if (interaction != QEX_INTERACTION_EXPONENTIAL && interaction != QEX_INTERACTION_SOFT)
+    return fail(QEX_E_BADARG, "bad interaction kind");
+  const int total = batch * num_grids;
+  hipLaunchKernelGGL(k_external, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream_, batch, num_grids,
+                     grids, num_nuclei, locations, nuclear_charges, interaction, p0, p1, out);
+  return check_launch("qex1d_external_potential");
+}
+
+int qex1d_hartree_potential(int32_t batch, int32_t num_grids, double dx, const double* table, const double* density,
+                            double* out, void* stream_) {
+  if (batch < 1 || num_grids < 1 || num_grids > QEX1D_MAX_GRIDS) return fail(QEX_E_BADARG, "bad sizes");
+  if (!table || !density || !out) return fail(QEX_E_BADARG, "null array");
+  hipLaunchKernelGGL(k_hartree, dim3((num_grids + NT - 1) / NT, batch), dim3(NT), (size_t)2 * num_grids * 8,
+                     (hipStream_t)stream_, num_grids, dx, table, density, out);
+  return check_launch("qex1d_hartree_potential");
+}
+
+}  // extern "C"

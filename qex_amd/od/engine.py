@@ -1,0 +1,238 @@
+"""Batched 1D SCF driver over the C ABI (``qex1d_*`` in include/qex.h).
+
+All arrays live on the device as torch tensors; the library enqueues its kernels on torch's
+current stream.  There is no host computation path: without a GPU or without libqex.so
+every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _lib
+from . import xc as xcmod
+from .utils import check_uniform, get_dx, reflection_window
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise _lib.QexError("no GPU visible: the qex engine runs only on the device")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def as_f64(x, dev) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev).contiguous()
+
+
+def as_i32(x, dev) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.int32).contiguous()
+    return torch.as_tensor(np.asarray(x, dtype=np.int32), device=dev).contiguous()
+
+
+def workspace(nbytes: int, dev) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
+
+
+def interaction_table(grids: torch.Tensor, interaction_fn) -> torch.Tensor:
+    """u(k dx), k = 0..N-1: row 0 of the Toeplitz interaction matrix (u must be even)."""
+    n = grids.shape[0]
+    dx = get_dx(grids)
+    d = dx * torch.arange(n, dtype=torch.float64, device=grids.device)
+    return as_f64(interaction_fn(d), grids.device)
+
+
+def external_potential(grids, locations, nuclear_charges, interaction_fn) -> torch.Tensor:
+    """v_ext = -sum_a Z_a u(x - R_a); batched when ``locations`` is 2D ([B, A])."""
+    dev = device()
+    g = as_f64(grids, dev)
+    loc = as_f64(locations, dev)
+    z = as_f64(nuclear_charges, dev)
+    single = loc.dim() == 1
+    if single:
+        loc, z = loc[None], z[None]
+    B, A = loc.shape
+    N = g.shape[0]
+    kind = getattr(interaction_fn, "kind", None)
+    if kind in (_lib.QEX_INTERACTION_EXPONENTIAL, _lib.QEX_INTERACTION_SOFT):
+        out = torch.empty((B, N), dtype=torch.float64, device=dev)
+        p0, p1 = interaction_fn.params
+        _lib.check(_lib.load().qex1d_external_potential(B, N, _ptr(g), A, _ptr(loc), _ptr(z), kind, p0, p1,
+                                                        _ptr(out), _stream()), "qex1d_external_potential")
+    else:  # arbitrary even interaction: evaluated on the device through torch
+        out = -(z[:, :, None] * interaction_fn(g[None, None, :] - loc[:, :, None])).sum(dim=1)
+        out = as_f64(out, dev)
+    return out[0] if single else out
+
+
+def hartree_potential(density, grids, interaction_fn) -> torch.Tensor:
+    dev = device()
+    g = as_f64(grids, dev)
+    n = as_f64(density, dev)
+    single = n.dim() == 1
+    if single:
+        n = n[None]
+    B, N = n.shape
+    out = torch.empty_like(n)
+    table = interaction_table(g, interaction_fn)
+    _lib.check(_lib.load().qex1d_hartree_potential(B, N, get_dx(g), _ptr(table), _ptr(n), _ptr(out), _stream()),
+               "qex1d_hartree_potential")
+    return out[0] if single else out
+
+
+@dataclass
+class NonInteracting:
+    density: torch.Tensor
+    total_eigen_energies: torch.Tensor
+    gap: torch.Tensor
+    eigenvalues: torch.Tensor | None = None
+    wavefunctions: torch.Tensor | None = None
+
+
+def solve_noninteracting(potential, num_electrons, grids, with_states=False, max_electrons=None) -> NonInteracting:
+    """Lowest eigenpairs of T + diag(v) for a batch [B, N] of potentials."""
+    dev = device()
+    v = as_f64(potential, dev)
+    single = v.dim() == 1
+    if single:
+        v = v[None]
+    B, N = v.shape
+    ne = np.broadcast_to(np.asarray(num_electrons, dtype=np.int64).reshape(-1), (B,)).copy()
+    if np.any(ne < 1):
+        raise ValueError("num_electrons must be >= 1")
+    me = int(max_electrons or ne.max())
+    K = me // 2 + 1
+    ne_t = as_i32(ne, dev)
+    lib = _lib.load()
+    ws = workspace(lib.qex1d_noninteracting_workspace_bytes(B, N, me), dev)
+    dens = torch.empty((B, N), dtype=torch.float64, device=dev)
+    ee = torch.empty(B, dtype=torch.float64, device=dev)
+    gap = torch.empty(B, dtype=torch.float64, device=dev)
+    eig = torch.empty((B, K), dtype=torch.float64, device=dev) if with_states else None
+    wf = torch.empty((B, K, N), dtype=torch.float64, device=dev) if with_states else None
+    _lib.check(lib.qex1d_solve_noninteracting(B, N, get_dx(as_f64(grids, dev)), me, _ptr(v), _ptr(ne_t), _ptr(dens),
+                                              _ptr(ee), _ptr(gap), _ptr(eig), _ptr(wf), _ptr(ws), ws.numel(),
+                                              _stream()), "qex1d_solve_noninteracting")
+    if single:
+        return NonInteracting(dens[0], ee[0], gap[0], None if eig is None else eig[0], None if wf is None else wf[0])
+    return NonInteracting(dens, ee, gap, eig, wf)
+
+
+@dataclass
+class BatchResult:
+    density: torch.Tensor            # [B, T, N]
+    total_energy: torch.Tensor       # [B, T]
+    hartree_potential: torch.Tensor  # [B, T, N]
+    xc_potential: torch.Tensor       # [B, T, N]
+    xc_energy_density: torch.Tensor  # [B, T, N] (local) or [B, T] (global)
+    gap: torch.Tensor                # [B, T]
+    converged: torch.Tensor          # [B, T] bool
+    external_potential: torch.Tensor  # [B, N]
+    eigenvalues: torch.Tensor | None = None  # [B, T, K]
+
+
+class Problem1D:
+    """Device-resident description of a batch of systems on one grid (reused across solves)."""
+
+    def __init__(self, grids, locations, nuclear_charges, num_electrons, interaction_fn, xc,
+                 num_iterations, alpha=0.5, alpha_decay=0.9, num_mixing_iterations=2,
+                 density_mse_converge_tolerance=-1.0, enforce_reflection_symmetry=False, loop="python",
+                 external_potential_override=None, want_eigenvalues=False):
+        dev = device()
+        check_uniform(grids)
+        self.dev = dev
+        self.grids = as_f64(grids, dev)
+        N = self.grids.shape[0]
+        if N > _lib.QEX1D_MAX_GRIDS:
+            raise ValueError(f"num_grids {N} exceeds {_lib.QEX1D_MAX_GRIDS}")
+        locs = [np.asarray(l, dtype=np.float64).reshape(-1) for l in locations]
+        zs = [np.asarray(z, dtype=np.float64).reshape(-1) for z in nuclear_charges]
+        B = len(locs)
+        A = max(1, max(len(l) for l in locs))
+        loc_pad = np.zeros((B, A))
+        z_pad = np.zeros((B, A))
+        for b in range(B):
+            loc_pad[b, :len(locs[b])] = locs[b]
+            z_pad[b, :len(zs[b])] = zs[b]
+        self.locations, self.nuclear_charges = locs, zs
+        self.B, self.N, self.T = B, N, int(num_iterations)
+        ne = np.broadcast_to(np.asarray(num_electrons, dtype=np.int64).reshape(-1), (B,)).copy()
+        if np.any(ne < 1) or np.any(ne > _lib.QEX1D_MAX_ELECTRONS):
+            raise ValueError(f"num_electrons must be in [1, {_lib.QEX1D_MAX_ELECTRONS}]")
+        self.num_electrons = ne
+        self.ne_t = as_i32(ne, dev)
+        self.xc = xcmod.resolve(xc)
+        self.table = interaction_table(self.grids, interaction_fn)
+        if external_potential_override is not None:
+            v = as_f64(external_potential_override, dev)
+            self.vext = v[None].contiguous() if v.dim() == 1 else v
+        else:
+            self.vext = external_potential(self.grids, loc_pad, z_pad, interaction_fn).reshape(B, N).contiguous()
+        p = _lib.Qex1dProblem()
+        p.batch, p.num_grids, p.dx = B, N, get_dx(self.grids)
+        p.num_iterations = self.T
+        p.max_electrons = int(ne.max())
+        p.alpha, p.alpha_decay = float(alpha), float(alpha_decay)
+        p.num_mixing_iterations = int(num_mixing_iterations)
+        p.density_mse_converge_tolerance = float(density_mse_converge_tolerance)
+        p.loop = _lib.QEX1D_LOOP_PYTHON if loop == "python" else _lib.QEX1D_LOOP_SCAN
+        p.reflect = 1 if enforce_reflection_symmetry else 0
+        self.xc.configure(p)
+        self.problem = p
+        self.K = p.max_electrons // 2 + 1
+        self.want_eigenvalues = want_eigenvalues
+        if enforce_reflection_symmetry:
+            if loop == "python":
+                win = np.array([reflection_window(l, self.grids) for l in locs], dtype=np.int32)
+            else:
+                win = np.tile(np.array([0, N - 1], dtype=np.int32), (B, 1))
+            self.win = as_i32(win, dev)
+        else:
+            self.win = None
+        self.xc_params = self.xc.device_params(dev)
+        lib = _lib.load()
+        nbytes = lib.qex1d_workspace_bytes(ctypes.byref(p))
+        if nbytes == 0:
+            _lib.check(-1, "qex1d_workspace_bytes")
+        self.ws = workspace(nbytes, dev)
+
+    def allocate_outputs(self) -> BatchResult:
+        B, T, N, dev = self.B, self.T, self.N, self.dev
+        f = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
+        exc = f(B, T, N) if self.xc.encoding == "local" else f(B, T)
+        return BatchResult(
+            density=f(B, T, N), total_energy=f(B, T), hartree_potential=f(B, T, N), xc_potential=f(B, T, N),
+            xc_energy_density=exc, gap=f(B, T), converged=torch.empty((B, T), dtype=torch.uint8, device=dev),
+            external_potential=self.vext, eigenvalues=f(B, T, self.K) if self.want_eigenvalues else None)
+
+    def run(self, initial_density, out: BatchResult | None = None) -> BatchResult:
+        n0 = as_f64(initial_density, self.dev)
+        if n0.dim() == 1:
+            n0 = n0[None]
+        if tuple(n0.shape) != (self.B, self.N):
+            raise ValueError(f"initial_density must have shape {(self.B, self.N)}, got {tuple(n0.shape)}")
+        n0 = n0.contiguous()
+        out = out or self.allocate_outputs()
+        ins = _lib.Qex1dInputs(_ptr(self.table), _ptr(self.vext), _ptr(n0), _ptr(self.ne_t), _ptr(self.win),
+                               _ptr(self.xc_params))
+        outs = _lib.Qex1dOutputs(_ptr(out.density), _ptr(out.total_energy), _ptr(out.hartree_potential),
+                                 _ptr(out.xc_potential), _ptr(out.xc_energy_density), _ptr(out.gap),
+                                 _ptr(out.converged), _ptr(out.eigenvalues))
+        self._keep = (n0, ins, outs)  # keep alive until the stream has consumed them
+        _lib.check(_lib.load().qex1d_kohn_sham(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs),
+                                               _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham")
+        return out

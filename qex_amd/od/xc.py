@@ -1,0 +1,187 @@
+"""Exchange-correlation functionals the 1D engine evaluates on the device.
+
+Each functional is a callable ``fn(density, params=None)`` (the ``xc_energy_density_fn``
+convention of ``qedft/models/wrappers.py:85-113`` and ``jax_dft.xc``) and also describes
+itself to the kernels (``xc_kind`` and device parameters).  Calling it evaluates the energy
+density with torch, for inspection; inside the SCF loop only the HIP kernels run.
+
+* ``LDAExp``: jax_dft ``xc.get_lda_xc_energy_density_fn()`` for the exponential
+  interaction (``od/eval.py:78-81``).
+* ``PowerLaw``: c (n + eps)^p, local (``tests/test_scf_od.py:127-129``) or summed to a scalar
+  (global, ``tests/test_scf_od.py:46-48``).
+* ``LocalMLPXC``: a per-point MLP on a (2h+1)-point density window (window 1 is
+  ``build_local_mlp``, ``classical_models.py:123-174``).
+"""
+
+from __future__ import annotations
+
+import functools
+import math
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+A_EXP = 1.071295
+KAPPA_EXP = 1.0 / 2.385345
+
+
+class XCFunctional:
+    encoding = "local"
+    xc_kind = _lib.QEX1D_XC_NONE
+
+    def configure(self, problem) -> None:
+        problem.xc_kind = self.xc_kind
+
+    def device_params(self, device):
+        return None
+
+    def with_params(self, params):
+        return self
+
+    def energy_density(self, density: torch.Tensor) -> torch.Tensor:
+        return torch.zeros_like(density)
+
+    def __call__(self, density, params=None):
+        fn = self.with_params(params) if params is not None else self
+        t = density if isinstance(density, torch.Tensor) else torch.as_tensor(np.asarray(density, dtype=np.float64))
+        return fn.energy_density(t.to(torch.float64))
+
+
+class NoXC(XCFunctional):
+    xc_kind = _lib.QEX1D_XC_NONE
+
+
+class LDAExp(XCFunctional):
+    """1D LDA (exchange + correlation per electron) for u = A exp(-kappa|r|)."""
+
+    xc_kind = _lib.QEX1D_XC_LDA_EXP
+
+    def energy_density(self, n):
+        y = math.pi * n / KAPPA_EXP
+        big = y > 1e-15
+        ys = torch.where(big, y, torch.ones_like(y))
+        ex = torch.where(big, A_EXP / (2 * math.pi) * (torch.log(1 + ys ** 2) / ys - 2 * torch.atan(ys)),
+                         A_EXP / (2 * math.pi) * (-y + y ** 3 / 6))
+        al, be, ga, de, et, si, nu = 2.0, -1.00077, 6.26099, -11.9041, 9.62614, -1.48334, 1.0
+        fy = torch.where(y == 0, torch.ones_like(y), y)
+        s = torch.sqrt(fy)
+        P = al + be * s + ga * fy + de * fy * s + et * fy ** 2 + si * fy ** 2 * s + nu * math.pi * KAPPA_EXP ** 2 / A_EXP * fy ** 3
+        ec = torch.where(y == 0, -A_EXP * y / math.pi / al, -A_EXP * fy / math.pi / P)
+        return ex + ec
+
+
+class PowerLaw(XCFunctional):
+    """c (n + eps)^p; ``encoding='global'`` sums it to a scalar (amplitude-encoded XC)."""
+
+    def __init__(self, c=-0.73855, p=1.0 / 3, eps=0.0, encoding="local"):
+        self.c, self.p, self.eps = float(c), float(p), float(eps)
+        if encoding not in ("local", "global"):
+            raise ValueError("encoding must be 'local' or 'global'")
+        self.encoding = encoding
+        self.xc_kind = _lib.QEX1D_XC_POWER_LOCAL if encoding == "local" else _lib.QEX1D_XC_POWER_GLOBAL
+
+    def configure(self, problem):
+        problem.xc_kind = self.xc_kind
+        problem.xc_c, problem.xc_p, problem.xc_eps = self.c, self.p, self.eps
+
+    def energy_density(self, n):
+        e = self.c * (n + self.eps) ** self.p
+        return e.sum() if self.encoding == "global" else e
+
+
+class LocalMLPXC(XCFunctional):
+    """Per-point MLP on a window of 2h+1 densities (zero padded), inputs divided by ``norm``.
+
+    ``params``: list of (W, b) per Dense layer with W of shape (fan_in, fan_out) (stax
+    convention, x @ W + b); activation after every layer but the last; output width 1.
+    """
+
+    xc_kind = _lib.QEX1D_XC_LOCAL_MLP
+
+    def __init__(self, params=None, window=17, activation="silu", norm=2.0):
+        if window < 1 or window % 2 == 0:
+            raise ValueError("window must be a positive odd number")
+        if activation not in _lib.ACTIVATIONS:
+            raise ValueError(f"unsupported activation {activation!r}")
+        self.window, self.activation, self.norm = int(window), activation, float(norm)
+        self.params = None if params is None else self._canon(params)
+
+    @staticmethod
+    def _canon(params):
+        layers = []
+        for item in params:
+            if isinstance(item, (tuple, list)) and len(item) == 2:
+                w, b = item
+                layers.append((np.asarray(w, dtype=np.float64), np.asarray(b, dtype=np.float64).reshape(-1)))
+        if not layers:
+            raise ValueError("no Dense layers in params")
+        return layers
+
+    def with_params(self, params):
+        return LocalMLPXC(params, self.window, self.activation, self.norm)
+
+    def widths(self):
+        ws = [self.params[0][0].shape[0]] + [w.shape[1] for w, _ in self.params]
+        return ws
+
+    def configure(self, problem):
+        if self.params is None:
+            raise ValueError("LocalMLPXC has no parameters")
+        ws = self.widths()
+        if ws[0] != self.window:
+            raise ValueError(f"first layer fan-in {ws[0]} != window {self.window}")
+        if len(self.params) > _lib.QEX1D_MAX_MLP_LAYERS:
+            raise ValueError("too many layers")
+        problem.xc_kind = self.xc_kind
+        problem.mlp_layers = len(self.params)
+        for i, w in enumerate(ws):
+            problem.mlp_widths[i] = int(w)
+        problem.mlp_activation = _lib.ACTIVATIONS[self.activation]
+        problem.mlp_norm = self.norm
+
+    def packed(self):
+        return np.concatenate([np.concatenate([w.reshape(-1), b.reshape(-1)]) for w, b in self.params])
+
+    def device_params(self, device):
+        return torch.as_tensor(self.packed(), dtype=torch.float64, device=device)
+
+    def energy_density(self, n):
+        h = self.window // 2
+        N = n.shape[-1]
+        pad = torch.nn.functional.pad(n, (h, h))
+        idx = torch.arange(N)[:, None] + torch.arange(self.window)[None, :]
+        x = pad[..., idx] / self.norm
+        acts = {"silu": torch.nn.functional.silu, "swish": torch.nn.functional.silu, "tanh": torch.tanh,
+                "softplus": torch.nn.functional.softplus, "sigmoid": torch.sigmoid, "relu": torch.relu,
+                "gelu": lambda z: torch.nn.functional.gelu(z, approximate="tanh")}
+        act = acts[self.activation]
+        for li, (w, b) in enumerate(self.params):
+            x = x @ torch.as_tensor(w, dtype=x.dtype, device=x.device) + torch.as_tensor(b, dtype=x.dtype, device=x.device)
+            if li < len(self.params) - 1:
+                x = act(x)
+        return x[..., 0]
+
+
+def get_lda_xc_energy_density_fn():
+    """jax_dft ``xc.get_lda_xc_energy_density_fn()``: fn(density, params) -> e_xc per electron."""
+    return LDAExp()
+
+
+def resolve(fn) -> XCFunctional:
+    """Turns an ``xc_energy_density_fn`` (a functional, or functools.partial of one carrying
+    ``params=``) into the functional the kernels evaluate."""
+    if isinstance(fn, XCFunctional):
+        return fn
+    if isinstance(fn, functools.partial) and isinstance(fn.func, XCFunctional):
+        params = fn.keywords.get("params") if fn.keywords else None
+        if params is None and fn.args:
+            params = fn.args[0]
+        return fn.func.with_params(params) if params is not None else fn.func
+    inner = getattr(fn, "qex_functional", None)
+    if isinstance(inner, XCFunctional):
+        return inner
+    raise TypeError(
+        "xc_energy_density_fn must be a qex_amd functional (LDAExp, PowerLaw, LocalMLPXC, or one built by "
+        f"qex_amd.networks.build_xc_functional), got {type(fn).__name__}")

@@ -1,0 +1,98 @@
+"""C ABI of libqex.so on the CPU: the library loads, exports every function include/qex.h declares,
+and rejects bad arguments with an error code and message before touching the device."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from qex_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "qex.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(qex[0-9a-z_]*)\s*\(", text, re.M)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared_functions()
+    assert len(names) >= 8
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert lib.qex_abi_version() == 1
+
+
+def _problem(**kw):
+    p = _lib.Qex1dProblem()
+    p.batch, p.num_grids, p.dx, p.num_iterations, p.max_electrons = 2, 513, 0.1, 15, 4
+    p.alpha, p.alpha_decay, p.num_mixing_iterations = 0.3, 1.0, 1
+    p.density_mse_converge_tolerance = -1.0
+    p.loop, p.reflect, p.xc_kind = _lib.QEX1D_LOOP_SCAN, 0, _lib.QEX1D_XC_LDA_EXP
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_workspace_query():
+    lib = _lib.load()
+    assert lib.qex1d_workspace_bytes(ctypes.byref(_problem())) > 0
+    assert lib.qex1d_workspace_bytes(ctypes.byref(_problem(num_grids=4))) == 0
+    assert lib.qex1d_workspace_bytes(ctypes.byref(_problem(num_grids=_lib.QEX1D_MAX_GRIDS + 1))) == 0
+    assert lib.qex1d_workspace_bytes(ctypes.byref(_problem(max_electrons=_lib.QEX1D_MAX_ELECTRONS + 1))) == 0
+    assert lib.qex1d_noninteracting_workspace_bytes(4, 513, 8) > 0
+    assert lib.qex1d_noninteracting_workspace_bytes(0, 513, 8) == 0
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("batch", 0, "batch"), ("num_grids", 3, "num_grids"), ("dx", -0.1, "dx"), ("num_iterations", 0, "num_iterations"),
+    ("num_mixing_iterations", 0, "num_mixing"), ("loop", 7, "loop"), ("xc_kind", 99, "xc_kind"),
+])
+def test_kohn_sham_rejects_bad_problem(field, value, msg):
+    lib = _lib.load()
+    ins, outs = _lib.Qex1dInputs(), _lib.Qex1dOutputs()
+    rc = lib.qex1d_kohn_sham(ctypes.byref(_problem(**{field: value})), ctypes.byref(ins), ctypes.byref(outs), None, 0,
+                             None)
+    assert rc == -1
+    assert msg in lib.qex_last_error().decode()
+
+
+def test_kohn_sham_rejects_missing_arrays_and_small_workspace():
+    lib = _lib.load()
+    p = _problem()
+    ins, outs = _lib.Qex1dInputs(), _lib.Qex1dOutputs()
+    assert lib.qex1d_kohn_sham(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), None, 0, None) == -1
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails before any launch
+    ins = _lib.Qex1dInputs(fake, fake, fake, fake, None, None)
+    outs = _lib.Qex1dOutputs(fake, fake, fake, fake, fake, fake, fake, None)
+    rc = lib.qex1d_kohn_sham(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), fake, 16, None)
+    assert rc == -2 and "workspace" in lib.qex_last_error().decode()
+
+
+def test_mlp_validation():
+    lib = _lib.load()
+    p = _problem(xc_kind=_lib.QEX1D_XC_LOCAL_MLP, mlp_layers=3, mlp_activation=0, mlp_norm=2.0)
+    for i, w in enumerate([17, 64, 64, 1]):
+        p.mlp_widths[i] = w
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) > 0
+    p.mlp_widths[0] = 16  # even window
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
+    p.mlp_widths[0] = 17
+    p.mlp_widths[3] = 2  # output width must be 1
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
+
+
+def test_engine_refuses_to_run_without_a_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from qex_amd.od import engine
+
+    with pytest.raises(_lib.QexError):
+        engine.device()

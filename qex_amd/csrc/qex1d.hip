@@ -446,6 +446,226 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k) {
 }
 
 // --------------------------------------------------------------------------------------------
+// k_mlp_mfma: the same windowed MLP on FP64 matrix cores (v_mfma_f64_16x16x4f64) for hidden
+// widths <= 64 and <= 4 hidden layers.  A workgroup takes 64 consecutive points of the
+// flattened [B][N] grid; each layer is a 64 x H tile GEMM, wave w owning output columns
+// 16w..16w+15.  Activation derivatives stay in registers for the backward GEMMs (W^T).
+// MFMA operand layout (16x16x4, one wave): lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// it receives D[4v + l/16][l%16] in element v.
+// --------------------------------------------------------------------------------------------
+typedef double d4_t __attribute__((ext_vector_type(4)));
+constexpr int MP = 64;         // points per workgroup
+constexpr int MLP_HMAX = 64;   // widest hidden layer on the MFMA path
+constexpr int MLP_LH = 4;      // hidden layers on the MFMA path
+constexpr int MLP_S = 65;      // LDS row stride (doubles) of the activation tiles
+
+__device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int rup(int x, int m) { return (x + m - 1) / m * m; }
+
+// acc[rb] (rows 16rb.., columns of this wave's block cb) = in[64 x K] . B[K x ncols], where
+// B[kk][col] = W[kk * ldw + col] (or W[col * ldw + kk] when trans); entries outside
+// kk < K, col < ncols are zero.  `in` is an LDS tile with row stride MLP_S, zero beyond K.
+__device__ __forceinline__ void tile_gemm(const double* in, int K, const double* W, int ldw, int ncols, bool trans,
+                                         int cb, d4_t (&acc)[4]) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) acc[rb] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  const int col = cb * 16 + (l & 15);
+  const int kp = rup(K, 4);
+  for (int k0 = 0; k0 < kp; k0 += 4) {
+    const int kk = k0 + (l >> 4);
+    double bv = 0.0;
+    if (kk < K && col < ncols) bv = trans ? W[(size_t)col * ldw + kk] : W[(size_t)kk * ldw + col];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const double av = in[(rb * 16 + (l & 15)) * MLP_S + kk];
+      acc[rb] = mfma64(av, bv, acc[rb]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_mlp_mfma(Dev1 a, int k) {
+  extern __shared__ double smem[];
+  double* bufA = smem;                 // 64 x MLP_S
+  double* bufB = bufA + MP * MLP_S;    // 64 x MLP_S
+  double* cot = bufB + MP * MLP_S;     // 64 cotangents
+  __shared__ int sh_any[2];
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+  const int N = a.N, W = a.W, h = W / 2, L = a.nlay;
+  const long total = (long)a.B * N;
+  const long g0 = (long)blockIdx.x * MP;
+  const bool sym = a.reflect && xc_is_local(a.xc);
+  if (tid < 2) sh_any[tid] = 0;
+  __syncthreads();
+  if (tid < MP) {
+    const long g = g0 + tid;
+    if (g < total) {
+      const int s = (int)(g / N);
+      if (k < a.T && !a.st[4 * s + 0]) sh_any[0] = 1;
+      if (a.st[4 * s + 1] >= 0) sh_any[1] = 1;
+    }
+  }
+  __syncthreads();
+  const bool any_pot = sh_any[0] != 0, any_e = sh_any[1] != 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 0 && !any_pot) continue;
+    if (pass == 1 && !any_e) continue;
+    const bool backward = pass == 0;
+    // ---- window inputs (x / norm) into bufB, zero padded to a multiple of 4 columns ----
+    const int wp = rup(W, 4);
+    for (int idx = tid; idx < MP * wp; idx += NT) {
+      const int p = idx / wp, o = idx % wp;
+      const long g = g0 + p;
+      double x = 0.0;
+      if (g < total && o < W) {
+        const int s = (int)(g / N), i = (int)(g % N);
+        const bool act = pass == 0 ? (k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0);
+        const int j = i + o - h;
+        if (act && j >= 0 && j < N) {
+          if (pass == 0) {
+            const double* nin = input_density(a, s, k);
+            x = nin[j];
+            if (sym) {
+              int lo = a.win[2 * s], hi = a.win[2 * s + 1];
+              x = 0.5 * (x + nin[mirror(j, lo, hi)]);
+            }
+          } else {
+            x = a.o_xc[(size_t)s * N + j];
+          }
+          x /= a.norm;
+        }
+        if (o == h) cot[p] = act ? x * a.norm : 0.0;  // the point's own density (cotangent of e)
+      } else if (o == h) {
+        cot[p] = 0.0;
+      }
+      bufB[p * MLP_S + o] = x;
+    }
+    __syncthreads();
+    // ---- hidden layers ----
+    double dz[MLP_LH][4][4];
+    double* cur = bufB;
+    double* nxt = bufA;
+#pragma unroll
+    for (int lh = 0; lh < MLP_LH; ++lh) {
+      if (lh < L - 1) {
+        const int fi = a.widths[lh], fo = a.widths[lh + 1];
+        const double* Wl = a.mlp + a.woff[lh];
+        const double* bl = a.mlp + a.boff[lh];
+        d4_t acc[4];
+        tile_gemm(cur, fi, Wl, fo, fo, false, wv, acc);
+        const int col = wv * 16 + (l & 15);
+        const double bias = col < fo ? bl[col] : 0.0;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = rb * 16 + 4 * v + (l >> 4);
+            double d;
+            const double y = act_fwd(a.act, acc[rb][v] + bias, &d);
+            nxt[row * MLP_S + col] = col < fo ? y : 0.0;
+            dz[lh][rb][v] = col < fo ? d : 0.0;
+          }
+        }
+        __syncthreads();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+      }
+    }
+    // ---- output layer (fan_out 1): 4 lanes per point ----
+    const int hl = a.widths[L - 1];
+    const double* WL = a.mlp + a.woff[L - 1];
+    {
+      const int p = tid >> 2, q = tid & 3;
+      double part = 0.0;
+      for (int u = q; u < hl; u += 4) part = fma(cur[p * MLP_S + u], WL[u], part);
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      const long g = g0 + p;
+      if (q == 0 && g < total) {
+        const double e = part + a.mlp[a.boff[L - 1]];
+        if (pass == 0)
+          a.mlp_e[g] = e;
+        else
+          a.mlp_eout[g] = e;
+      }
+    }
+    if (!backward) {
+      __syncthreads();
+      continue;
+    }
+    // ---- backward: g_{L-2} = cot * w_out * act'(z_{L-2}) ----
+    __syncthreads();
+    {
+      const int col = wv * 16 + (l & 15);
+      const double wo = col < hl ? WL[col] : 0.0;
+#pragma unroll
+      for (int lh = 0; lh < MLP_LH; ++lh) {
+        if (lh == L - 2) {
+#pragma unroll
+          for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = rb * 16 + 4 * v + (l >> 4);
+              nxt[row * MLP_S + col] = cot[row] * wo * dz[lh][rb][v];
+            }
+        }
+      }
+    }
+    __syncthreads();
+    {
+      double* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    // g_{l-1} = (g_l W_l^T) * act'(z_{l-1}), l = L-2 .. 1
+#pragma unroll
+    for (int lh = MLP_LH - 1; lh >= 1; --lh) {
+      if (lh <= L - 2) {
+        const int fi = a.widths[lh], fo = a.widths[lh + 1];
+        const double* Wl = a.mlp + a.woff[lh];
+        d4_t acc[4];
+        tile_gemm(cur, fo, Wl, fo, fi, true, wv, acc);
+        const int col = wv * 16 + (l & 15);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = rb * 16 + 4 * v + (l >> 4);
+            nxt[row * MLP_S + col] = col < fi ? acc[rb][v] * dz[lh - 1][rb][v] : 0.0;
+          }
+        __syncthreads();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+      }
+    }
+    // gX = g_0 W_0^T  (window columns): waves 0..ceil(W/16)-1
+    {
+      const int f1 = a.widths[1];
+      const double* W0 = a.mlp + a.woff[0];
+      if (wv * 16 < W) {
+        d4_t acc[4];
+        tile_gemm(cur, f1, W0, f1, W, true, wv, acc);
+        const int col = wv * 16 + (l & 15);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = rb * 16 + 4 * v + (l >> 4);
+            const long g = g0 + row;
+            if (col < W && g < total) a.mlp_G[g * W + col] = acc[rb][v];
+          }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------------------------
 // k_potential
 // --------------------------------------------------------------------------------------------
 __device__ double mlp_v_unsym(const Dev1& a, int s, int i) {
@@ -1230,18 +1450,26 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
 
   const size_t lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
   const size_t lds_eig = eigen_lds_bytes(a.N);
-  const size_t lds_mlp = a.xc == QEX1D_XC_LOCAL_MLP ? mlp_lds_bytes(a) : 0;
+  bool mlp_mfma = a.xc == QEX1D_XC_LOCAL_MLP && a.nlay - 1 <= MLP_LH && a.W <= MLP_S - 1;
+  for (int l = 1; l < a.nlay && mlp_mfma; ++l) mlp_mfma = a.widths[l] <= MLP_HMAX;
+  const size_t lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(2 * MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
   if (lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", lds_mlp);
+  const dim3 gmlp_mfma((unsigned)(((long)a.B * a.N + MP - 1) / MP));
   const dim3 gpot(a.nTiles, a.B), geig(a.K, a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);
 
   if (lds_eig > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eig);
-  if (lds_mlp > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_mlp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
+  if (lds_mlp > 64 * 1024)
+    (void)hipFuncSetAttribute(mlp_mfma ? (const void*)k_mlp_mfma : (const void*)k_mlp,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
   hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
   double alpha_k = p->alpha;
   for (int k = 0; k <= a.T; ++k) {
     if (a.xc == QEX1D_XC_LOCAL_MLP) {
       ProfScope ps(0, stream);
-      hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
+      if (mlp_mfma)
+        hipLaunchKernelGGL(k_mlp_mfma, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k);
+      else
+        hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
     }
     {
       ProfScope ps(1, stream);

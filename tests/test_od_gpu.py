@@ -190,8 +190,13 @@ def test_scan_loop_convergence_flags(tol, expected):
                                              utils.exponential_coulomb, n0, density_mse_converge_tolerance=tol)
     s = st.numpy()
     assert list(s.converged) == expected
-    ref = O.jit_kohn_sham(np.array([-0.5, 0.5]), np.array([1, 1]), 2, 3, grids, O.PowerGlobal(),
+    # f' = c p n^(-2/3) diverges where n ~ 0, so the output density is not unique (degenerate edge
+    # states); parity is checked on the regularised functional instead
+    ref = O.jit_kohn_sham(np.array([-0.5, 0.5]), np.array([1, 1]), 2, 3, grids, O.PowerGlobal(eps=1e-2),
                           O.exponential_coulomb, n0, density_mse_converge_tolerance=tol)
+    st = jit_scf.kohn_sham_amplitude_encoded(np.array([-0.5, 0.5]), np.array([1, 1]), 2, 3, grids,
+                                             xc.PowerLaw(eps=1e-2), utils.exponential_coulomb, n0,
+                                             density_mse_converge_tolerance=tol)
     _compare(st, ref, 0.1)
 
 

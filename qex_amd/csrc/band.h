@@ -11,7 +11,6 @@
 
 namespace qexband {
 
-constexpr double PIVMIN = 1e-60;
 
 // 1/d to double precision: the hardware reciprocal (relative error <= 5e-8 on gfx950) and one
 // Newton step (relative error ~2e-15) on the device.
@@ -24,22 +23,35 @@ QEX_HD inline double rcp64(double d) {
 #endif
 }
 
+// 1 if the sign bit of x is set (also for -0.0), else 0 -- without a compare/VCC round trip.
+QEX_HD inline int sign_bit(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (int)(((unsigned)__double2hiint(x)) >> 31);
+#else
+  return signbit(x) ? 1 : 0;
+#endif
+}
+
+// 1/d clamped to [-RMAX, RMAX]: a pivot smaller than 1/RMAX in magnitude acts as +-1/RMAX
+// (an O(1e-60) perturbation of one diagonal entry) and no inf/NaN can enter a recurrence.
+constexpr double RMAX = 1e60;
+QEX_HD inline double rcp_clamped(double d) { return fmin(fmax(rcp64(d), -RMAX), RMAX); }
+
 // Number of eigenvalues of the symmetric pentadiagonal matrix (diag a, off b, c) below sigma:
 // negative pivots of the LDL^T factorisation of A - sigma I.  With u_i = l_i d_i:
 //   d_i = (a_i - sigma) - u_{i-1}^2 / d_{i-1} - c^2 / d_{i-2},   u_i = b - c u_{i-1} / d_{i-1}.
 // The diagonal is read 8 entries at a time so the LDS latency is paid once per chunk.
 QEX_HD inline int sturm_count(const double* __restrict__ a, int N, double b, double c, double sigma) {
-  const double c2 = c * c;
+  const double nc2 = -c * c, nc = -c;
   double rm1 = 0.0, rm2 = 0.0, um1 = 0.0;
   int cnt = 0;
   auto step = [&](double ai) {
-    const double p = (ai - sigma) - c2 * rm2;
-    double d = fma(-um1 * um1, rm1, p);
-    if (fabs(d) < PIVMIN) d = -PIVMIN;
-    cnt += d < 0.0 ? 1 : 0;
-    const double u = fma(-c * rm1, um1, b);
+    const double p = fma(nc2, rm2, ai - sigma);
+    const double d = fma(-um1 * um1, rm1, p);
+    cnt += sign_bit(d);
+    const double u = fma(nc * rm1, um1, b);
     rm2 = rm1;
-    rm1 = rcp64(d);
+    rm1 = rcp_clamped(d);
     um1 = u;
   };
   int i = 0;
@@ -55,18 +67,17 @@ QEX_HD inline int sturm_count(const double* __restrict__ a, int N, double b, dou
 }
 
 // Inverse iteration without pivoting.  A - sigma I = L D L^T with L unit lower triangular
-// (L_{i+1,i} = l_i, L_{i+2,i} = m_i); pivots below `tiny` in magnitude are replaced by
-// +-tiny.  F holds 4 doubles per row: 1/d_i, l_i, m_i and the forward-solve value z_i.
+// (L_{i+1,i} = l_i, L_{i+2,i} = m_i); 1/d is clamped to +-1/tiny (a pivot below `tiny` in
+// magnitude acts as +-tiny).  F holds 4 doubles per row: 1/d_i, l_i, m_i and the forward-solve value z_i.
 //
 // ldlt_factor_forward: factorisation fused with the forward solve L z = y.
 QEX_HD inline void ldlt_factor_forward(const double* __restrict__ a, const double* __restrict__ y, int N, double b,
                                        double c, double sigma, double tiny, double* __restrict__ F) {
-  const double c2 = c * c;
+  const double c2 = c * c, rtiny = 1.0 / tiny;
   double rm1 = 0.0, rm2 = 0.0, um1 = 0.0, lm1 = 0.0, mm1 = 0.0, mm2 = 0.0, z1 = 0.0, z2 = 0.0;
   auto step = [&](int i, double ai, double yi) {
-    double d = fma(-um1 * um1, rm1, (ai - sigma) - c2 * rm2);
-    if (fabs(d) < tiny) d = d < 0.0 ? -tiny : tiny;
-    const double r = rcp64(d);
+    const double d = fma(-um1 * um1, rm1, fma(-c2, rm2, ai - sigma));
+    const double r = fmin(fmax(rcp64(d), -rtiny), rtiny);
     const double u = fma(-c * rm1, um1, b);
     const double l = u * r, m = c * r;
     const double z = fma(-mm2, z2, fma(-lm1, z1, yi));

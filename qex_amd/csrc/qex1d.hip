@@ -33,7 +33,6 @@ using qexband::ldlt_factor_forward;
 using qexband::ldlt_forward;
 using qexband::rcp64;
 using qexband::sturm_count;
-using qexband::PIVMIN;
 
 constexpr int NT = 256;     // threads per workgroup (4 wave64)
 constexpr int NWAVE = NT / 64;
@@ -790,7 +789,19 @@ struct EigArgs {
   double* eig;            // [B][K]
   const double* eig_prev; // [B][K] or null
   const double* eig_delta;
+  int* pairs;             // [K*B] (s << 8) | j for every wanted eigenpair, then -1
 };
+
+// Compacts the wanted (system, eigenpair) list so the eigen launch has no idle leading blocks.
+__global__ void k_pairs(int B, int K, const int* ne, int* pairs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int n = 0;
+  for (int s = 0; s < B; ++s) {
+    const int keig = ne[s] / 2 + 1;
+    for (int j = 0; j < keig && j < K; ++j) pairs[n++] = (s << 8) | j;
+  }
+  for (; n < K * B; ++n) pairs[n] = -1;
+}
 
 __device__ __forceinline__ double start_value(int i, int j) {
   // deterministic pseudo-random start vector for inverse iteration
@@ -803,11 +814,10 @@ __device__ __forceinline__ double start_value(int i, int j) {
 
 __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
   extern __shared__ double smem[];
-  const int j = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+  const int pair = e.pairs[blockIdx.x];  // (system << 8) | eigen index; -1 past the last pair
+  if (pair < 0) return;
+  const int j = pair & 255, s = pair >> 8, tid = threadIdx.x;
   if (e.st && e.st[4 * s + 0]) return;
-  const int ne = e.ne[s];
-  const int keig = ne / 2 + 1;
-  if (j >= keig) return;
   const int N = e.N;
   double* a = smem;         // N    diagonal T_ii + v_i
   double* y = a + N;        // N    inverse-iteration vector
@@ -828,7 +838,9 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
   const double gl = -block_max(-amin, red) - off;
   const double gu = block_max(amax, red) + off;
   const double hnorm = block_max(aabs, red) + off;
-  double lo = gl, hi = gu;
+  // Cold start: the wanted states are the lowest few, so the first bracket is the bottom 40 Ha
+  // of the Gershgorin interval (a miss widens it to the top bound after one round).
+  double lo = gl, hi = fmin(gu, gl + 40.0);
   const bool warm = e.st && e.st[4 * s + 2] && e.eig_prev;
   if (warm) {
     const double p = e.eig_prev[(size_t)s * e.K + j];
@@ -841,8 +853,8 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
     }
   }
   // Multisection: NT shifts per round (both ends included) until the bracket of the j-th
-  // eigenvalue is 1e-9 relative; the Rayleigh quotient of the inverse-iteration vector then
-  // gives the eigenvalue to ~1e-13.
+  // eigenvalue is 1e-9 relative; two inverse-iteration solves and the Rayleigh quotient then
+  // give the eigenpair to ~1e-13.
   for (int round = 0; round < 80; ++round) {
     const double mid = 0.5 * (lo + hi);
     const double tol = 1e-9 * fmax(1.0, fabs(mid)) + 64.0 * EPS * hnorm;
@@ -980,6 +992,18 @@ __device__ void orbitals_to_density(double* vecs, int N, int keig, int ne, doubl
     }
     block_sum<1>(rq, red);
     if (threadIdx.x == 0) lam[jj] = rq[0];
+  }
+  // Rayleigh quotients of (nearly) degenerate pairs can come out of order by rounding
+  if (threadIdx.x == 0) {
+    for (int x = 1; x < keig; ++x) {
+      const double t = lam[x];
+      int y = x - 1;
+      while (y >= 0 && lam[y] > t) {
+        lam[y + 1] = lam[y];
+        --y;
+      }
+      lam[y + 1] = t;
+    }
   }
   __syncthreads();
 }
@@ -1208,7 +1232,7 @@ __global__ __launch_bounds__(NT) void k_hartree(int N, double dx, const double* 
 // --------------------------------------------------------------------------------------------
 struct Layout {
   size_t o_raw, o_xc, vks, hist, vec, eig, eig_prev, eig_delta, terms, partH, partX, partF, src, st, mlp_e, mlp_G,
-      mlp_eout, total;
+      mlp_eout, pairs, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1243,6 +1267,7 @@ Layout layout1d(const qex1d_problem* p) {
   L.mlp_e = take(mlp ? B * N * 8 : 0);
   L.mlp_G = take(mlp ? B * N * W * 8 : 0);
   L.mlp_eout = take(mlp ? B * N * 8 : 0);
+  L.pairs = take(B * K * 4);
   L.total = off;
   return L;
 }
@@ -1447,6 +1472,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   e.eig = a.eig;
   e.eig_prev = a.eig_prev;
   e.eig_delta = a.eig_delta;
+  e.pairs = (int*)(ws + L.pairs);
 
   const size_t lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
   const size_t lds_eig = eigen_lds_bytes(a.N);
@@ -1455,13 +1481,14 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   const size_t lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(2 * MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
   if (lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", lds_mlp);
   const dim3 gmlp_mfma((unsigned)(((long)a.B * a.N + MP - 1) / MP));
-  const dim3 gpot(a.nTiles, a.B), geig(a.K, a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);
+  const dim3 gpot(a.nTiles, a.B), geig(a.K * a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);
 
   if (lds_eig > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eig);
   if (lds_mlp > 64 * 1024)
     (void)hipFuncSetAttribute(mlp_mfma ? (const void*)k_mlp_mfma : (const void*)k_mlp,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
   hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(64), 0, stream, a.B, a.K, a.ne, e.pairs);
   double alpha_k = p->alpha;
   for (int k = 0; k <= a.T; ++k) {
     if (a.xc == QEX1D_XC_LOCAL_MLP) {
@@ -1498,7 +1525,8 @@ size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, in
       max_electrons > QEX1D_MAX_ELECTRONS)
     return 0;
   const size_t K = max_electrons / 2 + 1;
-  return align256((size_t)batch * K * num_grids * 8) + align256((size_t)batch * K * 8);
+  return align256((size_t)batch * K * num_grids * 8) + align256((size_t)batch * K * 8) +
+         align256((size_t)batch * K * 4);
 }
 
 int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int32_t max_electrons,
@@ -1525,10 +1553,12 @@ int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int3
   e.st = nullptr;
   e.vec = (double*)workspace;
   e.eig = (double*)((char*)workspace + align256((size_t)batch * K * num_grids * 8));
+  e.pairs = (int*)((char*)e.eig + align256((size_t)batch * K * 8));
+  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(64), 0, stream, batch, K, num_electrons, e.pairs);
   if (eigen_lds_bytes(num_grids) > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)eigen_lds_bytes(num_grids));
-  hipLaunchKernelGGL(k_eigen, dim3(K, batch), dim3(NT), eigen_lds_bytes(num_grids), stream, e);
+  hipLaunchKernelGGL(k_eigen, dim3(K * batch), dim3(NT), eigen_lds_bytes(num_grids), stream, e);
   hipLaunchKernelGGL(k_nonint_density, dim3(batch), dim3(NT), (size_t)num_grids * 8, stream, num_grids, K, dx, e.Td,
                      e.Tb, e.Tc, potential, num_electrons, e.vec, density, total_eigen_energies, gap, eigenvalues,
                      wavefunctions);

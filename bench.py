@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--procs", type=int, default=0, help="host processes for the oracle (default: min(16, cores))")
     ap.add_argument("--skip-oracle", action="store_true",
                     help="no oracle / CPU baseline (for profiler runs, which must not start child processes)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="launch one 64 MiB device-to-device copy first (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -111,6 +113,12 @@ def main():
     from qex_amd.od import engine, utils, xc
 
     lib = _lib.load()
+    if args.calibrate_copy:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     f = xc.LocalMLPXC(list(zip(W, b)), window=17, activation="silu", norm=2.0)
     prob = engine.Problem1D(grids, [m[0] for m in mols], [m[1] for m in mols], [m[2] for m in mols],
                             utils.exponential_coulomb, f, T_ITERS, alpha=ALPHA, alpha_decay=1.0,
@@ -120,6 +128,7 @@ def main():
     torch.cuda.synchronize()
     for _ in range(args.warmup):
         prob.run(n0_dev, out)
+    out.stats.zero_()
     torch.cuda.synchronize()
     # ---- timed region: K batch solves; per-step events for latency; per-kernel events ----
     launches_per_step = 5 * T_ITERS + 8
@@ -154,19 +163,26 @@ def main():
     kern = {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.KERNEL_KINDS)}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
-    ne = np.array([m[2] for m in mols])
-    keig = ne // 2 + 1
-    # algorithmic FP64 work of one eigen launch (the method as implemented, per eigenpair):
-    # Sturm multisection to 2e-12 from the Gershgorin width (~log_255(range/tol) rounds of 256
-    # shifts x N steps x 9 flops) + pivoted band LU (~40 N) + 2 solves (~14 N each)
-    rounds = 5.0
-    eig_flops = float(np.sum(keig)) * (rounds * 256 * N_GRID * 9 + 40 * N_GRID + 28 * N_GRID)
-    flops_per_launch = {"eigen": eig_flops,
-                        "mlp": args.batch * N_GRID * 3 * 2 * (17 * 64 + 64 * 64 + 64) * 1.0,
-                        "potential": args.batch * 2 * 2 * N_GRID * N_GRID * 1.0}
+    stats = out.stats.cpu().numpy()
+    eig_launches = max(1, kern["eigen"][1])
+    # FP64 work the eigen kernel executes: 14 flops per Sturm step (256 shifts x N steps per round)
+    # and ~30 N per eigenpair for the LDL^T factorisation and the two solves
+    eig_flops = (float(stats[0]) * 256 * N_GRID * 14 + float(stats[1]) * 30 * N_GRID) / eig_launches
+    # MLP: forward + input-VJP at the input density and a forward at the output density
+    mlp_flops = args.batch * N_GRID * 3 * 2 * (17 * 64 + 64 * 64 + 64) * 1.0
+    pot_flops = args.batch * 2 * 2 * N_GRID * N_GRID * 1.0
+    flops_per_launch = {"eigen": eig_flops, "mlp": mlp_flops, "potential": pot_flops}
     fpl = flops_per_launch.get(dom, 0.0)
     achieved = fpl / (avg_ms[dom] * 1e-3) / 1e12 if avg_ms[dom] > 0 else 0.0
     peak = 78.6  # FP64 TFLOP/s (MI355X)
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    # (profiles/c2_batched_1d_scf_kernels.md, counters scaled by a 64 MiB copy), when present
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "c2_traffic.json")
+    if os.path.exists(tpath):
+        tb = json.load(open(tpath)).get("bytes_per_launch", {})
+        key = {"eigen": "k_eigen", "mlp": "k_mlp_mfma", "potential": "k_potential", "density": "k_density"}.get(dom)
+        traffic = tb.get(key)
     result = {
         "metric": "converged SCF solves/s at |dE|<=1e-6 Ha vs FP64 oracle",
         "value": value,
@@ -188,8 +204,11 @@ def main():
         "parity": {"converged": n_conv, "of": args.batch, "max_abs_dE": float(dE.max()),
                    "median_abs_dE": float(np.median(dE)), "max_density_L2": float(dl2.max())},
         "kernels_ms_avg": {k: round(v, 4) for k, v in avg_ms.items()},
+        "kernel_tflops": {k: round(flops_per_launch[k] / (avg_ms[k] * 1e-3) / 1e12, 3)
+                          for k in flops_per_launch if avg_ms.get(k, 0) > 0},
+        "eigen_rounds_per_pair": float(stats[0] / max(1, stats[1])),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None},
+                     "frac": achieved / peak, "traffic": traffic},
         "cpu_baseline": {"value": float(len(per_solve) / oracle_wall), "unit": "solves/s", "cores": procs,
                          "kind": "port",
                          "sample": f"FP64 NumPy/SciPy oracle, the same {args.batch} solves, {procs} processes x 1 "

@@ -101,6 +101,8 @@ typedef struct qex1d_outputs {
   double* gap;               /* [B][T] */
   uint8_t* converged;        /* [B][T] */
   double* eigenvalues;       /* optional [B][T][max_electrons/2+1], may be NULL */
+  uint64_t* stats;           /* optional [2], accumulated (caller zeroes): Sturm multisection rounds
+                                executed over all eigenpairs, eigenpairs solved; may be NULL */
 } qex1d_outputs;
 
 /* Workspace bytes needed by qex1d_kohn_sham for this problem (0 on bad arguments). */

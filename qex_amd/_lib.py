@@ -77,6 +77,7 @@ class Qex1dOutputs(ctypes.Structure):
         ("gap", ctypes.c_void_p),
         ("converged", ctypes.c_void_p),
         ("eigenvalues", ctypes.c_void_p),
+        ("stats", ctypes.c_void_p),
     ]
 
 

@@ -486,7 +486,62 @@ __device__ __forceinline__ void tile_gemm(const double* in, int K, const double*
   }
 }
 
-__global__ __launch_bounds__(NT) void k_mlp_mfma(Dev1 a, int k) {
+// One hidden layer of the tile MLP: nxt = act(cur W_lh + b_lh); keeps act' in dzl.
+__device__ __forceinline__ void mlp_hidden_fwd(const Dev1& a, int lh, const double* cur, double* nxt,
+                                               double (&dzl)[4][4], int wv, int l) {
+  const int fi = a.widths[lh], fo = a.widths[lh + 1];
+  const double* Wl = a.mlp + a.woff[lh];
+  const double* bl = a.mlp + a.boff[lh];
+  d4_t acc[4];
+  tile_gemm(cur, fi, Wl, fo, fo, false, wv, acc);
+  const int col = wv * 16 + (l & 15);
+  const double bias = col < fo ? bl[col] : 0.0;
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rb * 16 + 4 * v + (l >> 4);
+      double d;
+      const double y = act_fwd(a.act, acc[rb][v] + bias, &d);
+      nxt[row * MLP_S + col] = col < fo ? y : 0.0;
+      dzl[rb][v] = col < fo ? d : 0.0;
+    }
+  }
+  __syncthreads();
+}
+
+// Backward through hidden layer lh: nxt = (cur W_lh^T) * act'(z_{lh-1}) (dzp holds act' of lh-1).
+__device__ __forceinline__ void mlp_hidden_bwd(const Dev1& a, int lh, const double* cur, double* nxt,
+                                               const double (&dzp)[4][4], int wv, int l) {
+  const int fi = a.widths[lh], fo = a.widths[lh + 1];
+  const double* Wl = a.mlp + a.woff[lh];
+  d4_t acc[4];
+  tile_gemm(cur, fo, Wl, fo, fi, true, wv, acc);
+  const int col = wv * 16 + (l & 15);
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rb * 16 + 4 * v + (l >> 4);
+      nxt[row * MLP_S + col] = col < fi ? acc[rb][v] * dzp[rb][v] : 0.0;
+    }
+  __syncthreads();
+}
+
+// Gradient at the last hidden layer's pre-activations: cot_row * w_out[col] * act'.
+__device__ __forceinline__ void mlp_out_grad(double* nxt, const double* cot, double wo, const double (&dzl)[4][4],
+                                             int col, int l) {
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rb * 16 + 4 * v + (l >> 4);
+      nxt[row * MLP_S + col] = cot[row] * wo * dzl[rb][v];
+    }
+}
+
+template <int NH>
+__global__ __launch_bounds__(NT, 2) void k_mlp_mfma(Dev1 a, int k) {
   extern __shared__ double smem[];
   double* bufA = smem;                 // 64 x MLP_S
   double* bufB = bufA + MP * MLP_S;    // 64 x MLP_S
@@ -543,37 +598,20 @@ __global__ __launch_bounds__(NT) void k_mlp_mfma(Dev1 a, int k) {
       bufB[p * MLP_S + o] = x;
     }
     __syncthreads();
-    // ---- hidden layers ----
-    double dz[MLP_LH][4][4];
+    // ---- hidden layers (one named register array of activation derivatives per layer) ----
+    // NH (compile time) = L - 1 hidden layers; only the arrays in use stay live
+    double dz0[4][4], dz1[NH > 1 ? 4 : 1][4], dz2[NH > 2 ? 4 : 1][4], dz3[NH > 3 ? 4 : 1][4];
     double* cur = bufB;
     double* nxt = bufA;
-#pragma unroll
-    for (int lh = 0; lh < MLP_LH; ++lh) {
-      if (lh < L - 1) {
-        const int fi = a.widths[lh], fo = a.widths[lh + 1];
-        const double* Wl = a.mlp + a.woff[lh];
-        const double* bl = a.mlp + a.boff[lh];
-        d4_t acc[4];
-        tile_gemm(cur, fi, Wl, fo, fo, false, wv, acc);
-        const int col = wv * 16 + (l & 15);
-        const double bias = col < fo ? bl[col] : 0.0;
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = rb * 16 + 4 * v + (l >> 4);
-            double d;
-            const double y = act_fwd(a.act, acc[rb][v] + bias, &d);
-            nxt[row * MLP_S + col] = col < fo ? y : 0.0;
-            dz[lh][rb][v] = col < fo ? d : 0.0;
-          }
-        }
-        __syncthreads();
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
-      }
-    }
+    auto swap_bufs = [&]() {
+      double* t = cur;
+      cur = nxt;
+      nxt = t;
+    };
+    if constexpr (NH > 0) { mlp_hidden_fwd(a, 0, cur, nxt, dz0, wv, l); swap_bufs(); }
+    if constexpr (NH > 1) { mlp_hidden_fwd(a, 1, cur, nxt, dz1, wv, l); swap_bufs(); }
+    if constexpr (NH > 2) { mlp_hidden_fwd(a, 2, cur, nxt, dz2, wv, l); swap_bufs(); }
+    if constexpr (NH > 3) { mlp_hidden_fwd(a, 3, cur, nxt, dz3, wv, l); swap_bufs(); }
     // ---- output layer (fan_out 1): 4 lanes per point ----
     const int hl = a.widths[L - 1];
     const double* WL = a.mlp + a.woff[L - 1];
@@ -601,47 +639,17 @@ __global__ __launch_bounds__(NT) void k_mlp_mfma(Dev1 a, int k) {
     {
       const int col = wv * 16 + (l & 15);
       const double wo = col < hl ? WL[col] : 0.0;
-#pragma unroll
-      for (int lh = 0; lh < MLP_LH; ++lh) {
-        if (lh == L - 2) {
-#pragma unroll
-          for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int row = rb * 16 + 4 * v + (l >> 4);
-              nxt[row * MLP_S + col] = cot[row] * wo * dz[lh][rb][v];
-            }
-        }
-      }
+      if constexpr (NH == 1) mlp_out_grad(nxt, cot, wo, dz0, col, l);
+      if constexpr (NH == 2) mlp_out_grad(nxt, cot, wo, dz1, col, l);
+      if constexpr (NH == 3) mlp_out_grad(nxt, cot, wo, dz2, col, l);
+      if constexpr (NH == 4) mlp_out_grad(nxt, cot, wo, dz3, col, l);
     }
     __syncthreads();
-    {
-      double* t = cur;
-      cur = nxt;
-      nxt = t;
-    }
+    swap_bufs();
     // g_{l-1} = (g_l W_l^T) * act'(z_{l-1}), l = L-2 .. 1
-#pragma unroll
-    for (int lh = MLP_LH - 1; lh >= 1; --lh) {
-      if (lh <= L - 2) {
-        const int fi = a.widths[lh], fo = a.widths[lh + 1];
-        const double* Wl = a.mlp + a.woff[lh];
-        d4_t acc[4];
-        tile_gemm(cur, fo, Wl, fo, fi, true, wv, acc);
-        const int col = wv * 16 + (l & 15);
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = rb * 16 + 4 * v + (l >> 4);
-            nxt[row * MLP_S + col] = col < fi ? acc[rb][v] * dz[lh - 1][rb][v] : 0.0;
-          }
-        __syncthreads();
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
-      }
-    }
+    if constexpr (NH > 3) { mlp_hidden_bwd(a, 3, cur, nxt, dz2, wv, l); swap_bufs(); }
+    if constexpr (NH > 2) { mlp_hidden_bwd(a, 2, cur, nxt, dz1, wv, l); swap_bufs(); }
+    if constexpr (NH > 1) { mlp_hidden_bwd(a, 1, cur, nxt, dz0, wv, l); swap_bufs(); }
     // gX = g_0 W_0^T  (window columns): waves 0..ceil(W/16)-1
     {
       const int f1 = a.widths[1];
@@ -790,6 +798,7 @@ struct EigArgs {
   const double* eig_prev; // [B][K] or null
   const double* eig_delta;
   int* pairs;             // [K*B] (s << 8) | j for every wanted eigenpair, then -1
+  unsigned long long* stats;  // optional: [0] Sturm rounds executed, [1] eigenpairs solved
 };
 
 // Compacts the wanted (system, eigenpair) list so the eigen launch has no idle leading blocks.
@@ -852,6 +861,7 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
       hi = gu;
     }
   }
+  int nrounds = 0;
   // Multisection: NT shifts per round (both ends included) until the bracket of the j-th
   // eigenvalue is 1e-9 relative; two inverse-iteration solves and the Rayleigh quotient then
   // give the eigenpair to ~1e-13.
@@ -859,6 +869,7 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
     const double mid = 0.5 * (lo + hi);
     const double tol = 1e-9 * fmax(1.0, fabs(mid)) + 64.0 * EPS * hnorm;
     if (hi - lo <= tol) break;
+    ++nrounds;
     const double step = (hi - lo) / (double)(NT - 1);
     const double sig = tid == NT - 1 ? hi : fma((double)tid, step, lo);
     const int cnt = sturm_count(a, N, e.Tb, e.Tc, sig);
@@ -889,6 +900,10 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
     hi = nhi;
   }
   const double sigma = 0.5 * (lo + hi);
+  if (e.stats && tid == 0) {
+    atomicAdd(&e.stats[0], (unsigned long long)nrounds);
+    atomicAdd(&e.stats[1], 1ull);
+  }
   // inverse iteration: one LDL^T factorisation, two solves, normalised in between
   double* v = e.vec + ((size_t)s * e.K + j) * N;
   for (int t = tid; t < N; t += NT) y[t] = warm ? v[t] + 1e-3 * start_value(t, j) : start_value(t, j);
@@ -1473,6 +1488,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   e.eig_prev = a.eig_prev;
   e.eig_delta = a.eig_delta;
   e.pairs = (int*)(ws + L.pairs);
+  e.stats = (unsigned long long*)out->stats;
 
   const size_t lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
   const size_t lds_eig = eigen_lds_bytes(a.N);
@@ -1484,18 +1500,28 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   const dim3 gpot(a.nTiles, a.B), geig(a.K * a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);
 
   if (lds_eig > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eig);
+  const void* mlp_fn = (const void*)k_mlp;
+  if (mlp_mfma) {
+    const void* fns[4] = {(const void*)k_mlp_mfma<1>, (const void*)k_mlp_mfma<2>, (const void*)k_mlp_mfma<3>,
+                          (const void*)k_mlp_mfma<4>};
+    mlp_fn = fns[a.nlay - 2];
+  }
   if (lds_mlp > 64 * 1024)
-    (void)hipFuncSetAttribute(mlp_mfma ? (const void*)k_mlp_mfma : (const void*)k_mlp,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
+    (void)hipFuncSetAttribute(mlp_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
   hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
   hipLaunchKernelGGL(k_pairs, dim3(1), dim3(64), 0, stream, a.B, a.K, a.ne, e.pairs);
   double alpha_k = p->alpha;
   for (int k = 0; k <= a.T; ++k) {
     if (a.xc == QEX1D_XC_LOCAL_MLP) {
       ProfScope ps(0, stream);
-      if (mlp_mfma)
-        hipLaunchKernelGGL(k_mlp_mfma, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k);
-      else
+      if (mlp_mfma) {
+        switch (a.nlay - 1) {
+          case 1: hipLaunchKernelGGL(k_mlp_mfma<1>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
+          case 2: hipLaunchKernelGGL(k_mlp_mfma<2>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
+          case 3: hipLaunchKernelGGL(k_mlp_mfma<3>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
+          default: hipLaunchKernelGGL(k_mlp_mfma<4>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
+        }
+      } else
         hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
     }
     {

@@ -143,6 +143,7 @@ class BatchResult:
     converged: torch.Tensor          # [B, T] bool
     external_potential: torch.Tensor  # [B, N]
     eigenvalues: torch.Tensor | None = None  # [B, T, K]
+    stats: torch.Tensor | None = None        # [2] int64: Sturm rounds, eigenpairs (accumulated)
 
 
 class Problem1D:
@@ -217,7 +218,8 @@ class Problem1D:
         return BatchResult(
             density=f(B, T, N), total_energy=f(B, T), hartree_potential=f(B, T, N), xc_potential=f(B, T, N),
             xc_energy_density=exc, gap=f(B, T), converged=torch.empty((B, T), dtype=torch.uint8, device=dev),
-            external_potential=self.vext, eigenvalues=f(B, T, self.K) if self.want_eigenvalues else None)
+            external_potential=self.vext, eigenvalues=f(B, T, self.K) if self.want_eigenvalues else None,
+            stats=torch.zeros(2, dtype=torch.int64, device=dev))
 
     def run(self, initial_density, out: BatchResult | None = None) -> BatchResult:
         n0 = as_f64(initial_density, self.dev)
@@ -231,7 +233,7 @@ class Problem1D:
                                _ptr(self.xc_params))
         outs = _lib.Qex1dOutputs(_ptr(out.density), _ptr(out.total_energy), _ptr(out.hartree_potential),
                                  _ptr(out.xc_potential), _ptr(out.xc_energy_density), _ptr(out.gap),
-                                 _ptr(out.converged), _ptr(out.eigenvalues))
+                                 _ptr(out.converged), _ptr(out.eigenvalues), _ptr(out.stats))
         self._keep = (n0, ins, outs)  # keep alive until the stream has consumed them
         _lib.check(_lib.load().qex1d_kohn_sham(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs),
                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham")

@@ -69,7 +69,7 @@ def test_kohn_sham_rejects_missing_arrays_and_small_workspace():
     assert lib.qex1d_kohn_sham(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), None, 0, None) == -1
     fake = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails before any launch
     ins = _lib.Qex1dInputs(fake, fake, fake, fake, None, None)
-    outs = _lib.Qex1dOutputs(fake, fake, fake, fake, fake, fake, fake, None)
+    outs = _lib.Qex1dOutputs(fake, fake, fake, fake, fake, fake, fake, None, None)
     rc = lib.qex1d_kohn_sham(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), fake, 16, None)
     assert rc == -2 and "workspace" in lib.qex_last_error().decode()
 

@@ -151,6 +151,92 @@ int qex_profile_read(double* ms_per_kind, int32_t* launches_per_kind, int32_t nk
 const char* qex_last_error(void);
 int32_t qex_abi_version(void);
 
+
+/* ============================================================================================
+ * 3D restricted Kohn-Sham (LDA) / Hartree-Fock on an atom-centred grid, Cartesian s/p basis.
+ * ========================================================================================== */
+#define QEX3D_MAX_NAO 96
+#define QEX3D_MAX_DIIS 12
+#define QEX3D_RKS_LDA 0 /* Slater exchange + VWN5 correlation */
+#define QEX3D_RHF 1
+
+/* Basis on the device.  Shells are listed in AO order (an s shell is 1 AO, a p shell 3: x, y, z);
+ * coefficients include the primitive normalisation and a normalised contraction. */
+typedef struct qex3d_basis {
+  int32_t natm, nshell, nao;
+  const double* atom_coords;  /* [natm][3] bohr */
+  const double* atom_charges; /* [natm] */
+  const int32_t* shell_atom;  /* [nshell] */
+  const int32_t* shell_l;     /* [nshell] 0 or 1 */
+  const int32_t* shell_nprim; /* [nshell] */
+  const int32_t* shell_offset; /* [nshell] into exponents / coefficients */
+  const double* exponents;
+  const double* coefficients;
+  const int32_t* ao_shell; /* [nao] */
+  const int32_t* ao_lmn;   /* [nao] lx | ly << 2 | lz << 4 */
+} qex3d_basis;
+
+/* S, T, V (nuclear attraction) [nao][nao].  Replaces pyscf int1e_ovlp/kin/nuc (libcint), called at
+ * qedft/train/td/hf_legacy.py:452-471 (get_hcore, get_ovlp). */
+int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* stream);
+
+/* ERIs (ij|kl) as the [npair][npair] matrix over i>=j, k>=l pairs (npair = nao(nao+1)/2, pair index
+ * i(i+1)/2 + j).  Replaces pyscf int2e (libcint) used by get_jk, qedft/train/td/hf_legacy.py:452-471. */
+int qex3d_eri(const qex3d_basis* b, double* eri, void* stream);
+
+/* Atom-centred grid: natm * nrad * nang points from radial (r, w r^2) and angular (unit vectors,
+ * weights summing to 4 pi) base tables, Becke partition.  Replaces pyscf dft.Grids.build
+ * (qedft/train/td/trainer_legacy_no_jit.py:248-251). */
+int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const double* rad_w, int32_t nang,
+               const double* ang_xyz, const double* ang_w, double* coords, double* weights, void* stream);
+
+/* ao[g][mu] on ngrid points.  Replaces pyscf numint.eval_ao (td/eval_gto.py:20-70). */
+int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, double* ao, void* stream);
+
+/* rho_g = sum_{mu nu} ao[g][mu] dm[mu][nu] ao[g][nu].  Replaces eval_rho
+ * (qedft/train/td/numint_legacy.py:351-373). */
+int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* dm, double* rho, void* stream);
+
+/* LDA nr_rks: nelec_excsum[0] = sum rho w, [1] = sum rho w e_xc; vmat = V + V^T with
+ * V = ao^T diag(w v_xc / 2) ao.  Replaces nr_rks (qedft/train/td/numint.py:16-273, LDA branch
+ * :42-64, symmetrisation :264-265). */
+size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao);
+int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
+                 double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes, void* stream);
+
+typedef struct qex3d_scf_problem {
+  int32_t nao, nelectron, method, max_cycle, diis_space;
+  int64_t ngrid;
+  double conv_tol;   /* |dE| < conv_tol and |g|/sqrt(nvir nocc) < sqrt(conv_tol) */
+  double energy_nuc;
+} qex3d_scf_problem;
+
+typedef struct qex3d_scf_inputs {
+  const double* S;       /* [nao][nao] */
+  const double* H;       /* core Hamiltonian T + V */
+  const double* eri;     /* pair matrix from qex3d_eri */
+  const double* ao;      /* [ngrid][nao] (RKS) */
+  const double* weights; /* [ngrid] (RKS) */
+} qex3d_scf_inputs;
+
+typedef struct qex3d_scf_outputs {
+  double* e_tot;     /* [max_cycle + 1]: guess energy, then after each cycle (repeated once converged) */
+  double* mo_energy; /* [nao] ascending */
+  double* mo_coeff;  /* [nao][nao], columns are orbitals, C^T S C = 1 */
+  double* dm;        /* [nao][nao] */
+  int32_t* status;   /* [8]: [1] cycles run, [4] converged */
+} qex3d_scf_outputs;
+
+/* Closed-shell SCF with CDIIS from cycle 1 and a core-Hamiltonian guess.  Replaces
+ * qedft/train/td/hf_legacy.py:51-272 (_scf, kernel) with get_fock (:379-427), generalized eigh
+ * (td/generalized_eigensolver.py:264-330), make_rdm1 (td/hf.py:303-308) and get_veff
+ * (td/rks_legacy.py:36-127). */
+size_t qex3d_scf_workspace_bytes(const qex3d_scf_problem* p);
+int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
+              size_t workspace_bytes, void* stream);
+
+const char* qex3d_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

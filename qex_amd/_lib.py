@@ -81,6 +81,31 @@ class Qex1dOutputs(ctypes.Structure):
     ]
 
 
+QEX3D_RKS_LDA = 0
+QEX3D_RHF = 1
+QEX3D_MAX_NAO = 96
+
+
+class Qex3dBasis(ctypes.Structure):
+    _fields_ = [("natm", ctypes.c_int32), ("nshell", ctypes.c_int32), ("nao", ctypes.c_int32)] + [
+        (name, ctypes.c_void_p) for name in ("atom_coords", "atom_charges", "shell_atom", "shell_l", "shell_nprim",
+                                             "shell_offset", "exponents", "coefficients", "ao_shell", "ao_lmn")]
+
+
+class Qex3dScfProblem(ctypes.Structure):
+    _fields_ = [("nao", ctypes.c_int32), ("nelectron", ctypes.c_int32), ("method", ctypes.c_int32),
+                ("max_cycle", ctypes.c_int32), ("diis_space", ctypes.c_int32), ("ngrid", ctypes.c_int64),
+                ("conv_tol", ctypes.c_double), ("energy_nuc", ctypes.c_double)]
+
+
+class Qex3dScfInputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("S", "H", "eri", "ao", "weights")]
+
+
+class Qex3dScfOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("e_tot", "mo_energy", "mo_coeff", "dm", "status")]
+
+
 # Every exported symbol and its ctypes signature (restype, argtypes).
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -98,6 +123,17 @@ SIGNATURES = {
     "qex1d_hartree_potential": (_I, [_I, _I, _D, _P, _P, _P, _P]),
     "qex_profile_enable": (_I, [_I, _I]),
     "qex_profile_read": (_I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), _I]),
+    "qex3d_last_error": (ctypes.c_char_p, []),
+    "qex3d_int1e": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _P, _P]),
+    "qex3d_eri": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P]),
+    "qex3d_grid": (_I, [ctypes.POINTER(Qex3dBasis), _I, _P, _P, _I, _P, _P, _P, _P, _P]),
+    "qex3d_eval_ao": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
+    "qex3d_eval_rho": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P]),
+    "qex3d_nr_rks_workspace_bytes": (_S, [ctypes.c_int64, _I]),
+    "qex3d_nr_rks": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),
+    "qex3d_scf_workspace_bytes": (_S, [ctypes.POINTER(Qex3dScfProblem)]),
+    "qex3d_scf": (_I, [ctypes.POINTER(Qex3dScfProblem), ctypes.POINTER(Qex3dScfInputs),
+                       ctypes.POINTER(Qex3dScfOutputs), _P, _S, _P]),
 }
 
 KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize"]
@@ -123,6 +159,12 @@ def load() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def check3(rc: int, what: str) -> None:
+    if rc != QEX_OK:
+        msg = load().qex3d_last_error()
+        raise QexError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
 
 def check(rc: int, what: str) -> None:

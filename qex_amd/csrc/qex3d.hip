@@ -1,0 +1,1287 @@
+// qex3d.hip -- 3D restricted Kohn-Sham / Hartree-Fock cycle for gfx950 (MI355X).
+//
+// Per molecule (Cartesian s/p Gaussian basis, n_ao <= QEX3D_MAX_NAO, atom-centred grid):
+//   k_int1e / k_eri       overlap, kinetic, nuclear attraction and (ij|kl) by McMurchie-Davidson
+//                         (ERIs stored as the [npair][npair] matrix of i>=j, k>=l pairs)
+//   k_grid                Becke-partitioned atom-centred grid from radial x angular base tables
+//   k_eval_ao             phi[g][mu]
+//   k_rho                 rho_g = phi_g^T D phi_g, D in LDS (broadcast), phi_g in registers
+//   k_vxc                 LDA (Slater + VWN5) and per-chunk partials of phi^T diag(w v) phi
+//   k_coulomb             J (and K for Hartree-Fock) from the ERI pair matrix (HBM-bound GEMV)
+//   k_scf_*               Fock build + CDIIS, generalized eigh (orthogonaliser X = L^-T of the
+//                         overlap Cholesky, parallel cyclic Jacobi warm-started from the previous
+//                         eigenvectors), density matrix, energy, convergence -- one workgroup each
+// The SCF loop is enqueued without host synchronisation; a device flag freezes the state once
+// |dE| < tol and |g| / sqrt(n_vir n_occ) < sqrt(tol) (pyscf's criterion).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "qex.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int NW = NT / 64;
+thread_local char g_err3[512] = "";
+
+int fail3(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err3, sizeof(g_err3), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail3(QEX_E_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return QEX_OK;
+}
+
+__device__ __forceinline__ double wsum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// ordered workgroup sum (every thread receives the total)
+__device__ double bsum(double x, double* red) {
+  x = wsum(x);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ int pair_index(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+// ---------------------------------------------------------------------------------------------
+// Boys function F_m(T), m = 0..mmax (<= 8): series for T < 30 (then downward recursion),
+// asymptotic + upward recursion above.
+// ---------------------------------------------------------------------------------------------
+__device__ void boys_all(int mmax, double T, double* F) {
+  if (T < 30.0) {
+    // F_mmax by the series exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
+    double term = 1.0 / (2 * mmax + 1), sum = term;
+    for (int k = 1; k < 200; ++k) {
+      term *= 2.0 * T / (2 * mmax + 2 * k + 1);
+      sum += term;
+      if (term < 1e-17 * sum) break;
+    }
+    const double et = exp(-T);
+    F[mmax] = et * sum;
+    for (int m = mmax - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + et) / (2 * m + 1);
+  } else {
+    const double et = exp(-T);
+    F[0] = 0.5 * sqrt(M_PI / T);  // erf(sqrt(T)) = 1 to double precision for T >= 30
+    for (int m = 0; m < mmax; ++m) F[m + 1] = ((2 * m + 1) * F[m] - et) / (2.0 * T);
+  }
+}
+
+// Hermite expansion coefficients E_t for one Cartesian direction, l_a, l_b <= 1:
+// K = exp(-mu X_AB^2), X_PA, X_PB, 1/(2p).
+__device__ __forceinline__ void hermite_E1(int la, int lb, double K, double xpa, double xpb, double i2p, double* E) {
+  if (la == 0 && lb == 0) {
+    E[0] = K;
+    E[1] = 0.0;
+    E[2] = 0.0;
+  } else if (la == 1 && lb == 0) {
+    E[0] = xpa * K;
+    E[1] = i2p * K;
+    E[2] = 0.0;
+  } else if (la == 0 && lb == 1) {
+    E[0] = xpb * K;
+    E[1] = i2p * K;
+    E[2] = 0.0;
+  } else {
+    E[0] = (xpa * xpb + i2p) * K;
+    E[1] = (xpa + xpb) * i2p * K;
+    E[2] = i2p * i2p * K;
+  }
+}
+
+// R_{tuv} (n = 0) for t <= tx, u <= ty, v <= tz, t+u+v <= L <= 4, stored R[t][u][v].
+__device__ void hermite_R_all(int tx, int ty, int tz, double alpha, double X, double Y, double Z, double (&R)[5][5][5]) {
+  const int L = tx + ty + tz;
+  double F[9];
+  boys_all(L, alpha * (X * X + Y * Y + Z * Z), F);
+  // levels n = L .. 0; Rn[t][u][v] for t+u+v <= L-n, built from level n+1
+  double Rn[5][5][5];
+  double Rp[5][5][5];
+  double fac = 1.0;
+  for (int n = 0; n < L; ++n) fac *= -2.0 * alpha;
+  for (int n = L; n >= 0; --n) {
+    const int lev = L - n;
+    for (int t = 0; t <= min(tx, lev); ++t)
+      for (int u = 0; u <= min(ty, lev - t); ++u)
+        for (int v = 0; v <= min(tz, lev - t - u); ++v) {
+          double val;
+          if (t == 0 && u == 0 && v == 0) {
+            val = fac * F[n];
+          } else if (t > 0) {
+            val = X * Rp[t - 1][u][v] + (t > 1 ? (t - 1) * Rp[t - 2][u][v] : 0.0);
+          } else if (u > 0) {
+            val = Y * Rp[t][u - 1][v] + (u > 1 ? (u - 1) * Rp[t][u - 2][v] : 0.0);
+          } else {
+            val = Z * Rp[t][u][v - 1] + (v > 1 ? (v - 1) * Rp[t][u][v - 2] : 0.0);
+          }
+          Rn[t][u][v] = val;
+        }
+    for (int t = 0; t <= min(tx, lev); ++t)
+      for (int u = 0; u <= min(ty, lev - t); ++u)
+        for (int v = 0; v <= min(tz, lev - t - u); ++v) Rp[t][u][v] = Rn[t][u][v];
+    fac /= -2.0 * alpha;
+  }
+  for (int t = 0; t <= tx; ++t)
+    for (int u = 0; u <= ty; ++u)
+      for (int v = 0; v <= tz; ++v) R[t][u][v] = (t + u + v <= L) ? Rp[t][u][v] : 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// basis description on the device
+// ---------------------------------------------------------------------------------------------
+struct Basis {
+  int natm, nshell, nao;
+  const double* xyz;     // [natm][3]
+  const double* charge;  // [natm]
+  const int* sh_atom;    // [nshell]
+  const int* sh_l;       // [nshell]
+  const int* sh_nprim;   // [nshell]
+  const int* sh_off;     // [nshell] offset into exps / coefs
+  const double* exps;
+  const double* coefs;   // contraction coefficients including the primitive normalisation
+  const int* ao_shell;   // [nao]
+  const int* ao_lmn;     // [nao] packed lx | ly << 2 | lz << 4
+};
+
+__device__ __forceinline__ void ao_info(const Basis& b, int mu, int* sh, int* lx, int* ly, int* lz) {
+  *sh = b.ao_shell[mu];
+  const int c = b.ao_lmn[mu];
+  *lx = c & 3;
+  *ly = (c >> 2) & 3;
+  *lz = (c >> 4) & 3;
+}
+
+// one-electron integrals, one thread per (i >= j)
+__global__ void k_int1e(Basis b, double* S, double* Tm, double* V) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = b.nao;
+  if (idx >= n * (n + 1) / 2) return;
+  int i = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > idx) --i;
+  while ((i + 1) * (i + 2) / 2 <= idx) ++i;
+  const int j = idx - i * (i + 1) / 2;
+  int si, sj, la[3], lb[3];
+  ao_info(b, i, &si, &la[0], &la[1], &la[2]);
+  ao_info(b, j, &sj, &lb[0], &lb[1], &lb[2]);
+  const double* A = b.xyz + 3 * b.sh_atom[si];
+  const double* B = b.xyz + 3 * b.sh_atom[sj];
+  double s = 0.0, t = 0.0, v = 0.0;
+  for (int pa = 0; pa < b.sh_nprim[si]; ++pa) {
+    const double a = b.exps[b.sh_off[si] + pa], ca = b.coefs[b.sh_off[si] + pa];
+    for (int pb = 0; pb < b.sh_nprim[sj]; ++pb) {
+      const double bb = b.exps[b.sh_off[sj] + pb], cb = b.coefs[b.sh_off[sj] + pb];
+      const double p = a + bb, mu = a * bb / p, i2p = 0.5 / p;
+      double P[3], s1[3], dd[3];
+      for (int d = 0; d < 3; ++d) {
+        P[d] = (a * A[d] + bb * B[d]) / p;
+        const double xab = A[d] - B[d];
+        const double K = exp(-mu * xab * xab);
+        const double xpa = P[d] - A[d], xpb = P[d] - B[d];
+        const double sq = sqrt(M_PI / p);
+        // 1D overlaps s(i, j) for j = lb-2 .. lb+2 via the Hermite E_0 of (la, j), j up to 3
+        // s(la, j) with the transfer relation s(i, j+1) = s(i+1, j) + X_AB s(i, j) is avoided:
+        // build E0 recursively for j <= 3 (la <= 1).
+        double Ei[4];  // E_0^{la, j} for j = 0..3 (la fixed)
+        {
+          // E^{0,j}_t and E^{1,j}_t for j <= 3, t <= 4 by the standard recursions
+          double E0[4][5] = {{0}}, E1[4][5] = {{0}};
+          E0[0][0] = K;
+          for (int jj = 0; jj < 3; ++jj)
+            for (int tt = 0; tt <= jj + 1; ++tt)
+              E0[jj + 1][tt] = (tt > 0 ? i2p * E0[jj][tt - 1] : 0.0) + xpb * E0[jj][tt] + (tt + 1) * E0[jj][tt + 1];
+          for (int jj = 0; jj < 4; ++jj)
+            for (int tt = 0; tt <= jj + 1; ++tt)
+              E1[jj][tt] = (tt > 0 ? i2p * E0[jj][tt - 1] : 0.0) + xpa * E0[jj][tt] + (tt + 1) * (tt + 1 <= 4 ? E0[jj][tt + 1] : 0.0);
+          for (int jj = 0; jj < 4; ++jj) Ei[jj] = (la[d] == 0 ? E0[jj][0] : E1[jj][0]) * sq;
+        }
+        const int jb = lb[d];
+        s1[d] = Ei[jb];
+        const double sm2 = jb >= 2 ? Ei[jb - 2] : 0.0;
+        dd[d] = jb * (jb - 1) * sm2 - 2.0 * bb * (2 * jb + 1) * Ei[jb] + 4.0 * bb * bb * Ei[jb + 2];
+      }
+      const double cc = ca * cb;
+      s += cc * s1[0] * s1[1] * s1[2];
+      t += cc * -0.5 * (dd[0] * s1[1] * s1[2] + s1[0] * dd[1] * s1[2] + s1[0] * s1[1] * dd[2]);
+      // nuclear attraction
+      double Ex[3], Ey[3], Ez[3];
+      {
+        const double K0 = exp(-mu * (A[0] - B[0]) * (A[0] - B[0]));
+        const double K1 = exp(-mu * (A[1] - B[1]) * (A[1] - B[1]));
+        const double K2 = exp(-mu * (A[2] - B[2]) * (A[2] - B[2]));
+        hermite_E1(la[0], lb[0], K0, P[0] - A[0], P[0] - B[0], i2p, Ex);
+        hermite_E1(la[1], lb[1], K1, P[1] - A[1], P[1] - B[1], i2p, Ey);
+        hermite_E1(la[2], lb[2], K2, P[2] - A[2], P[2] - B[2], i2p, Ez);
+      }
+      const int tx = la[0] + lb[0], ty = la[1] + lb[1], tz = la[2] + lb[2];
+      for (int c = 0; c < b.natm; ++c) {
+        const double* C = b.xyz + 3 * c;
+        double R[5][5][5];
+        hermite_R_all(tx, ty, tz, p, P[0] - C[0], P[1] - C[1], P[2] - C[2], R);
+        double acc = 0.0;
+        for (int t0 = 0; t0 <= tx; ++t0)
+          for (int u0 = 0; u0 <= ty; ++u0)
+            for (int v0 = 0; v0 <= tz; ++v0) acc += Ex[t0] * Ey[u0] * Ez[v0] * R[t0][u0][v0];
+        v -= cc * b.charge[c] * 2.0 * M_PI / p * acc;
+      }
+    }
+  }
+  S[i * n + j] = S[j * n + i] = s;
+  Tm[i * n + j] = Tm[j * n + i] = t;
+  V[i * n + j] = V[j * n + i] = v;
+}
+
+// ERIs (ij|kl) for pair indices P = (i >= j), Q = (k >= l), Q <= P; written to both [P][Q] and [Q][P].
+__global__ void k_eri(Basis b, double* eri, int npair) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)npair * (npair + 1) / 2;
+  if (idx >= total) return;
+  long P = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+  while (P * (P + 1) / 2 > idx) --P;
+  while ((P + 1) * (P + 2) / 2 <= idx) ++P;
+  const long Q = idx - P * (P + 1) / 2;
+  auto unpair = [](long q, int* i, int* j) {
+    int ii = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+    while ((long)ii * (ii + 1) / 2 > q) --ii;
+    while ((long)(ii + 1) * (ii + 2) / 2 <= q) ++ii;
+    *i = ii;
+    *j = (int)(q - (long)ii * (ii + 1) / 2);
+  };
+  int i, j, k, l;
+  unpair(P, &i, &j);
+  unpair(Q, &k, &l);
+  int si, sj, sk, sl, la[3], lb[3], lc[3], ld[3];
+  ao_info(b, i, &si, &la[0], &la[1], &la[2]);
+  ao_info(b, j, &sj, &lb[0], &lb[1], &lb[2]);
+  ao_info(b, k, &sk, &lc[0], &lc[1], &lc[2]);
+  ao_info(b, l, &sl, &ld[0], &ld[1], &ld[2]);
+  const double* A = b.xyz + 3 * b.sh_atom[si];
+  const double* B = b.xyz + 3 * b.sh_atom[sj];
+  const double* Cc = b.xyz + 3 * b.sh_atom[sk];
+  const double* D = b.xyz + 3 * b.sh_atom[sl];
+  const int tx1 = la[0] + lb[0], ty1 = la[1] + lb[1], tz1 = la[2] + lb[2];
+  const int tx2 = lc[0] + ld[0], ty2 = lc[1] + ld[1], tz2 = lc[2] + ld[2];
+  double val = 0.0;
+  for (int pa = 0; pa < b.sh_nprim[si]; ++pa) {
+    const double ea = b.exps[b.sh_off[si] + pa], ca = b.coefs[b.sh_off[si] + pa];
+    for (int pb = 0; pb < b.sh_nprim[sj]; ++pb) {
+      const double eb = b.exps[b.sh_off[sj] + pb], cb = b.coefs[b.sh_off[sj] + pb];
+      const double p = ea + eb, mup = ea * eb / p, i2p = 0.5 / p;
+      double Pp[3], E1x[3], E1y[3], E1z[3];
+      for (int d = 0; d < 3; ++d) Pp[d] = (ea * A[d] + eb * B[d]) / p;
+      hermite_E1(la[0], lb[0], exp(-mup * (A[0] - B[0]) * (A[0] - B[0])), Pp[0] - A[0], Pp[0] - B[0], i2p, E1x);
+      hermite_E1(la[1], lb[1], exp(-mup * (A[1] - B[1]) * (A[1] - B[1])), Pp[1] - A[1], Pp[1] - B[1], i2p, E1y);
+      hermite_E1(la[2], lb[2], exp(-mup * (A[2] - B[2]) * (A[2] - B[2])), Pp[2] - A[2], Pp[2] - B[2], i2p, E1z);
+      for (int pc = 0; pc < b.sh_nprim[sk]; ++pc) {
+        const double ec = b.exps[b.sh_off[sk] + pc], cc = b.coefs[b.sh_off[sk] + pc];
+        for (int pd = 0; pd < b.sh_nprim[sl]; ++pd) {
+          const double ed = b.exps[b.sh_off[sl] + pd], cd = b.coefs[b.sh_off[sl] + pd];
+          const double q = ec + ed, muq = ec * ed / q, i2q = 0.5 / q;
+          double Qq[3], E2x[3], E2y[3], E2z[3];
+          for (int d = 0; d < 3; ++d) Qq[d] = (ec * Cc[d] + ed * D[d]) / q;
+          hermite_E1(lc[0], ld[0], exp(-muq * (Cc[0] - D[0]) * (Cc[0] - D[0])), Qq[0] - Cc[0], Qq[0] - D[0], i2q, E2x);
+          hermite_E1(lc[1], ld[1], exp(-muq * (Cc[1] - D[1]) * (Cc[1] - D[1])), Qq[1] - Cc[1], Qq[1] - D[1], i2q, E2y);
+          hermite_E1(lc[2], ld[2], exp(-muq * (Cc[2] - D[2]) * (Cc[2] - D[2])), Qq[2] - Cc[2], Qq[2] - D[2], i2q, E2z);
+          const double alpha = p * q / (p + q);
+          double R[5][5][5];
+          hermite_R_all(tx1 + tx2, ty1 + ty2, tz1 + tz2, alpha, Pp[0] - Qq[0], Pp[1] - Qq[1], Pp[2] - Qq[2], R);
+          double acc = 0.0;
+          for (int t = 0; t <= tx1; ++t)
+            for (int u = 0; u <= ty1; ++u)
+              for (int v = 0; v <= tz1; ++v) {
+                const double e1 = E1x[t] * E1y[u] * E1z[v];
+                if (e1 == 0.0) continue;
+                for (int t2 = 0; t2 <= tx2; ++t2)
+                  for (int u2 = 0; u2 <= ty2; ++u2)
+                    for (int v2 = 0; v2 <= tz2; ++v2) {
+                      const double e2 = E2x[t2] * E2y[u2] * E2z[v2];
+                      const double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+                      acc += e1 * e2 * sg * R[t + t2][u + u2][v + v2];
+                    }
+              }
+          val += ca * cb * cc * cd * 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q)) * acc;
+        }
+      }
+    }
+  }
+  eri[P * npair + Q] = val;
+  eri[Q * npair + P] = val;
+}
+
+// ---------------------------------------------------------------------------------------------
+// grid and AO values
+// ---------------------------------------------------------------------------------------------
+__global__ void k_grid(Basis b, int nrad, const double* rr, const double* rw, int nang, const double* ax,
+                       const double* aw, double* coords, double* weights) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)nrad * nang;
+  if (g >= per * b.natm) return;
+  const int a = (int)(g / per);
+  const int ir = (int)((g % per) / nang), ia = (int)(g % nang);
+  const double* C = b.xyz + 3 * a;
+  double x[3];
+  for (int d = 0; d < 3; ++d) x[d] = C[d] + rr[ir] * ax[3 * ia + d];
+  // Becke cell functions with 3 iterations of p(mu) = 1.5 mu - 0.5 mu^3
+  double Pown = 0.0, Psum = 0.0;
+  for (int A = 0; A < b.natm; ++A) {
+    const double* RA = b.xyz + 3 * A;
+    const double dA = sqrt((x[0] - RA[0]) * (x[0] - RA[0]) + (x[1] - RA[1]) * (x[1] - RA[1]) + (x[2] - RA[2]) * (x[2] - RA[2]));
+    double PA = 1.0;
+    for (int Bq = 0; Bq < b.natm; ++Bq) {
+      if (Bq == A) continue;
+      const double* RB = b.xyz + 3 * Bq;
+      const double dB = sqrt((x[0] - RB[0]) * (x[0] - RB[0]) + (x[1] - RB[1]) * (x[1] - RB[1]) + (x[2] - RB[2]) * (x[2] - RB[2]));
+      const double rab = sqrt((RA[0] - RB[0]) * (RA[0] - RB[0]) + (RA[1] - RB[1]) * (RA[1] - RB[1]) + (RA[2] - RB[2]) * (RA[2] - RB[2]));
+      double mu = (dA - dB) / rab;
+      for (int it = 0; it < 3; ++it) mu = 1.5 * mu - 0.5 * mu * mu * mu;
+      PA *= 0.5 * (1.0 - mu);
+    }
+    Psum += PA;
+    if (A == a) Pown = PA;
+  }
+  for (int d = 0; d < 3; ++d) coords[3 * g + d] = x[d];
+  weights[g] = rw[ir] * aw[ia] * (Psum > 0.0 ? Pown / Psum : 0.0);
+}
+
+__global__ void k_eval_ao(Basis b, long ng, const double* coords, double* ao) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ng * b.nshell) return;
+  const long g = idx / b.nshell;
+  const int s = (int)(idx % b.nshell);
+  const double* C = b.xyz + 3 * b.sh_atom[s];
+  const double dx = coords[3 * g] - C[0], dy = coords[3 * g + 1] - C[1], dz = coords[3 * g + 2] - C[2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double rad = 0.0;
+  for (int q = 0; q < b.sh_nprim[s]; ++q) rad += b.coefs[b.sh_off[s] + q] * exp(-b.exps[b.sh_off[s] + q] * r2);
+  // first AO of the shell: shells are stored in AO order
+  int mu0 = 0;
+  for (int t = 0; t < s; ++t) mu0 += b.sh_l[t] == 0 ? 1 : 3;
+  double* row = ao + g * b.nao;
+  if (b.sh_l[s] == 0) {
+    row[mu0] = rad;
+  } else {
+    row[mu0] = rad * dx;
+    row[mu0 + 1] = rad * dy;
+    row[mu0 + 2] = rad * dz;
+  }
+}
+
+// rho_g = phi_g^T D phi_g for a tile of 64 points: P = Phi_tile D on FP64 matrix cores
+// (v_mfma_f64_16x16x4f64; lane l supplies A[l%16][l/16], B[l/16][l%16], receives D[4v+l/16][l%16]),
+// then rho = rowsum(Phi_tile * P).  The AO tile is staged in LDS (row stride npad + 1).
+typedef double d4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(NT) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho,
+                                            const int* frozen) {
+  extern __shared__ double sm[];
+  if (frozen && *frozen) return;
+  const int npad = (n + 15) & ~15, S = npad + 1;
+  double* ph = sm;
+  __shared__ double part[NW][64];
+  const long g0 = (long)blockIdx.x * 64;
+  for (int t = threadIdx.x; t < 64 * npad; t += NT) {
+    const int p = t / npad, c = t % npad;
+    ph[p * S + c] = (g0 + p < ng && c < n) ? ao[(g0 + p) * n + c] : 0.0;
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double rp[4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rp[rb][v] = 0.0;
+  for (int cb = wv; cb < npad / 16; cb += NW) {
+    d4_t acc[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) acc[rb] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    const int col = cb * 16 + (l & 15);
+    for (int k0 = 0; k0 < npad; k0 += 4) {
+      const int kk = k0 + (l >> 4);
+      const double bv = (kk < n && col < n) ? dm[kk * n + col] : 0.0;
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) acc[rb] = mfma64(ph[(rb * 16 + (l & 15)) * S + kk], bv, acc[rb]);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rb * 16 + 4 * v + (l >> 4);
+        rp[rb][v] = fma(ph[row * S + col], acc[rb][v], rp[rb][v]);
+      }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double x = rp[rb][v];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      x += __shfl_xor(x, 8, 64);
+      if ((l & 15) == 0) part[wv][rb * 16 + 4 * v + (l >> 4)] = x;
+    }
+  __syncthreads();
+  if (threadIdx.x < 64 && g0 + threadIdx.x < ng) {
+    double r = 0.0;
+    for (int q = 0; q < NW; ++q) r += part[q][threadIdx.x];
+    rho[g0 + threadIdx.x] = r;
+  }
+}
+
+// LDA: Slater exchange + VWN5 correlation (spin unpolarised); e per electron and v = d(rho e)/d rho
+__device__ void lda_svwn5(double rho, double* e, double* v) {
+  if (!(rho > 1e-30)) {
+    *e = 0.0;
+    *v = 0.0;
+    return;
+  }
+  const double cx = -0.75 * cbrt(3.0 / M_PI);
+  const double ex = cx * cbrt(rho), vx = 4.0 / 3.0 * ex;
+  const double A = 0.0310907, x0 = -0.10498, b = 3.72744, c = 12.9352;
+  const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
+  const double x = sqrt(rs);
+  const double X = x * x + b * x + c, X0 = x0 * x0 + b * x0 + c;
+  const double Q = sqrt(4.0 * c - b * b);
+  const double at = atan(Q / (2.0 * x + b));
+  const double ec = A * (log(x * x / X) + 2.0 * b / Q * at - b * x0 / X0 * (log((x - x0) * (x - x0) / X) + 2.0 * (b + 2.0 * x0) / Q * at));
+  const double den = (2.0 * x + b) * (2.0 * x + b) + Q * Q;
+  const double dec = A * (2.0 / x - (2.0 * x + b) / X - 4.0 * b / den -
+                          b * x0 / X0 * (2.0 / (x - x0) - (2.0 * x + b) / X - 4.0 * (b + 2.0 * x0) / den));
+  *e = ex + ec;
+  *v = vx + ec - x / 6.0 * dec;
+}
+
+// Partials of V = phi^T diag(w v / 2) phi (symmetrised later), sum(rho w), sum(rho w e).
+// Workgroup b processes grid chunks b, b + gridDim, ... (CH points each) and keeps its 16x16
+// output tiles in registers (wave w owns tiles w, w+4, ...), so partials are per workgroup.
+constexpr int CH = 64;
+constexpr int VX_TILES = 9;  // tiles per wave: (npad/16)^2 / 4 for npad <= 96
+__global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
+                                            double* vpart, double* spart, const int* frozen) {
+  extern __shared__ double sm[];
+  if (frozen && *frozen) return;
+  const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16;
+  double* ph = sm;                    // CH x S
+  double* wvs = ph + (size_t)CH * S;  // CH
+  __shared__ double red[NW * 2];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  d4_t acc[VX_TILES];
+#pragma unroll
+  for (int t = 0; t < VX_TILES; ++t) acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  double nel = 0.0, exc = 0.0;
+  const long nchunk = (ng + CH - 1) / CH;
+  for (long c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const long g0 = c * CH;
+    __syncthreads();
+    for (int t = threadIdx.x; t < CH; t += NT) {
+      const long g = g0 + t;
+      double e = 0.0, v = 0.0, r = 0.0, ww = 0.0;
+      if (g < ng) {
+        r = rho[g];
+        ww = w[g];
+        lda_svwn5(r, &e, &v);
+      }
+      wvs[t] = 0.5 * ww * v;
+      nel += r * ww;
+      exc += r * ww * e;
+    }
+    for (int t = threadIdx.x; t < CH * npad; t += NT) {
+      const int p = t / npad, col = t % npad;
+      ph[p * S + col] = (g0 + p < ng && col < n) ? ao[(g0 + p) * n + col] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < VX_TILES; ++t) {
+      const int tile = wv + NW * t;
+      if (tile < nb * nb) {
+        const int bi = tile / nb, bj = tile % nb;
+        for (int k0 = 0; k0 < CH; k0 += 4) {
+          const int kk = k0 + (l >> 4);
+          const double av = ph[kk * S + bi * 16 + (l & 15)] * wvs[kk];
+          const double bv = ph[kk * S + bj * 16 + (l & 15)];
+          acc[t] = mfma64(av, bv, acc[t]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < VX_TILES; ++t) {
+    const int tile = wv + NW * t;
+    if (tile < nb * nb) {
+      const int bi = tile / nb, bj = tile % nb;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = bi * 16 + 4 * v + (l >> 4), col = bj * 16 + (l & 15);
+        if (row < n && col < n) vpart[(size_t)blockIdx.x * n * n + row * n + col] = acc[t][v];
+      }
+    }
+  }
+  const double s0 = bsum(nel, red), s1 = bsum(exc, red + NW);
+  if (threadIdx.x == 0) {
+    spart[2 * blockIdx.x] = s0;
+    spart[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+size_t rho_lds(int n) { return (size_t)64 * (((n + 15) & ~15) + 1) * 8; }
+size_t vxc_lds(int n) { return ((size_t)CH * (((n + 15) & ~15) + 1) + CH) * 8; }
+constexpr int VXC_BLOCKS = 256;
+
+void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, const int* frozen, hipStream_t st) {
+  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT), rho_lds(n), st, ng, n, ao, dm, rho, frozen);
+}
+
+int vxc_blocks(long ng) {
+  const long nchunk = (ng + CH - 1) / CH;
+  return (int)(nchunk < VXC_BLOCKS ? nchunk : VXC_BLOCKS);
+}
+
+__global__ void k_zero(double* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// J_ij = sum_kl (ij|kl) D_kl over the pair matrix (row P = pair(i,j)); K (optional) by gathers.
+__global__ __launch_bounds__(NT) void k_coulomb(int n, int npair, const double* eri, const double* dm, double* J,
+                                                double* K, const int* frozen) {
+  extern __shared__ double sDp[];  // D'_Q = D_kl (2 - delta_kl)
+  if (frozen && *frozen) return;
+  __shared__ double red[NW];
+  for (int q = threadIdx.x; q < npair; q += NT) {
+    int k = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+    while (k * (k + 1) / 2 > q) --k;
+    while ((k + 1) * (k + 2) / 2 <= q) ++k;
+    const int l = q - k * (k + 1) / 2;
+    sDp[q] = k == l ? dm[k * n + l] : dm[k * n + l] + dm[l * n + k];
+  }
+  __syncthreads();
+  for (int P = blockIdx.x; P < npair; P += gridDim.x) {
+    const double* row = eri + (size_t)P * npair;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < npair; q += NT) s = fma(row[q], sDp[q], s);
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+      int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
+      while (i * (i + 1) / 2 > P) --i;
+      while ((i + 1) * (i + 2) / 2 <= P) ++i;
+      const int j = P - i * (i + 1) / 2;
+      J[i * n + j] = s;
+      J[j * n + i] = s;
+    }
+  }
+  if (K) {
+    // K_ij = sum_kl (ik|jl) D_kl (Hartree-Fock only; O(n^4) gathers)
+    for (int ij = blockIdx.x; ij < n * n; ij += gridDim.x) {
+      const int i = ij / n, j = ij % n;
+      double s = 0.0;
+      for (int kl = threadIdx.x; kl < n * n; kl += NT) {
+        const int k = kl / n, l = kl % n;
+        s = fma(eri[(size_t)pair_index(i, k) * npair + pair_index(j, l)], dm[k * n + l], s);
+      }
+      s = bsum(s, red);
+      if (threadIdx.x == 0) K[i * n + j] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// small dense linear algebra in one workgroup (row-major n x n in global memory, n <= 128)
+// ---------------------------------------------------------------------------------------------
+// C = op(A) op(B) (+ beta C); ta/tb transpose flags
+__device__ void mm(int n, const double* A, bool ta, const double* B, bool tb, double* C, double alpha, double beta) {
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+    const int i = idx / n, j = idx % n;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double a = ta ? A[k * n + i] : A[i * n + k];
+      const double b = tb ? B[j * n + k] : B[k * n + j];
+      s = fma(a, b, s);
+    }
+    C[idx] = alpha * s + (beta != 0.0 ? beta * C[idx] : 0.0);
+  }
+  __syncthreads();
+}
+
+__device__ double dot_all(int len, const double* a, const double* b, double* red) {
+  double s = 0.0;
+  for (int t = threadIdx.x; t < len; t += blockDim.x) s = fma(a[t], b[t], s);
+  return bsum(s, red);
+}
+
+// Cholesky S = L L^T (in place, lower), then X = L^-T (upper): C = X V solves F C = S C e.
+__global__ __launch_bounds__(NT) void k_orthogonaliser(int n, const double* S, double* L, double* X) {
+  __shared__ double red[NW];
+  for (int t = threadIdx.x; t < n * n; t += NT) L[t] = S[t];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) L[k * n + k] = sqrt(L[k * n + k]);
+    __syncthreads();
+    const double dkk = L[k * n + k];
+    for (int i = k + 1 + threadIdx.x; i < n; i += NT) L[i * n + k] /= dkk;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < (n - k - 1) * (n - k - 1); idx += NT) {
+      const int i = k + 1 + idx / (n - k - 1), j = k + 1 + idx % (n - k - 1);
+      if (j <= i) L[i * n + j] -= L[i * n + k] * L[j * n + k];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < n * n; t += NT) {
+    const int i = t / n, j = t % n;
+    if (j > i) L[t] = 0.0;
+  }
+  __syncthreads();
+  // Linv (lower) column by column: Linv[:, c] solves L x = e_c; X = Linv^T
+  for (int c = threadIdx.x; c < n; c += NT) {
+    for (int i = 0; i < n; ++i) {
+      double s = i == c ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) s -= L[i * n + k] * X[c * n + k];  // X row c holds Linv[:, c]
+      X[c * n + i] = i < c ? 0.0 : s / L[i * n + i];
+    }
+  }
+  __syncthreads();
+  (void)red;
+}
+
+// Parallel cyclic Jacobi on the symmetric matrix A (LDS, n <= 96), accumulating rotations in V
+// (LDS).  Round-robin pairing: n/2 disjoint (p, q) pairs per round, n-1 rounds per sweep.
+__device__ void jacobi(int n, double* A, double* V, double* cs, int* pq, int max_sweeps, double tol) {
+  const int m = (n + 1) & ~1;  // even (a padded index m-1 >= n is a dummy)
+  __shared__ double red[NW];
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+      const int i = t / n, j = t % n;
+      if (i != j)
+        off = fma(A[t], A[t], off);
+      else
+        dia = fma(A[t], A[t], dia);
+    }
+    off = bsum(off, red);
+    dia = bsum(dia, red);
+    if (off <= tol * tol * dia) break;
+    for (int round = 0; round < m - 1; ++round) {
+      // pairs of the round: (round, m-1) fixed-point scheme
+      for (int k = threadIdx.x; k < m / 2; k += blockDim.x) {
+        int p, q;
+        if (k == 0) {
+          p = m - 1;
+          q = round;
+        } else {
+          p = (round + k) % (m - 1);
+          q = (round - k + (m - 1)) % (m - 1);
+        }
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = A[p * n + q];
+          if (apq != 0.0) {
+            const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+          }
+        }
+        pq[2 * k] = p;
+        pq[2 * k + 1] = q;
+        cs[2 * k] = c;
+        cs[2 * k + 1] = s;
+      }
+      __syncthreads();
+      // rows: A <- J^T A
+      for (int idx = threadIdx.x; idx < (m / 2) * n; idx += blockDim.x) {
+        const int k = idx / n, col = idx % n;
+        const int p = pq[2 * k], q = pq[2 * k + 1];
+        if (q >= n) continue;
+        const double c = cs[2 * k], s = cs[2 * k + 1];
+        const double ap = A[p * n + col], aq = A[q * n + col];
+        A[p * n + col] = c * ap - s * aq;
+        A[q * n + col] = s * ap + c * aq;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J
+      for (int idx = threadIdx.x; idx < (m / 2) * n; idx += blockDim.x) {
+        const int k = idx / n, row = idx % n;
+        const int p = pq[2 * k], q = pq[2 * k + 1];
+        if (q >= n) continue;
+        const double c = cs[2 * k], s = cs[2 * k + 1];
+        const double ap = A[row * n + p], aq = A[row * n + q];
+        A[row * n + p] = c * ap - s * aq;
+        A[row * n + q] = s * ap + c * aq;
+        const double vp = V[row * n + p], vq = V[row * n + q];
+        V[row * n + p] = c * vp - s * vq;
+        V[row * n + q] = s * vp + c * vq;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct Scf3 {
+  int n, npair, nocc, max_cycle, diis_space, method;  // method 0 RKS-LDA, 1 RHF
+  long ng;
+  int nchunk;
+  double conv_tol, enuc;
+  const double* S;
+  const double* H;
+  const double* X;  // orthogonaliser
+  const double* eri;
+  const double* ao;
+  const double* w;
+  // state (workspace)
+  double *F, *Fd, *D, *Dlast, *C, *Cp, *mo, *J, *K, *Vxc, *vpart, *spart, *rho, *tmp1, *tmp2;
+  double *diisF, *diisE, *diisB;  // [space][n*n], [space][n*n], [(space+1)^2]
+  double* scal;                    // [0] e_tot, [1] e_last, [2] nelec, [3] exc, [4] gnorm
+  int* ist;                        // [0] frozen, [1] cycles done, [2] diis count, [3] diis head, [4] converged
+  double* e_hist;                  // [max_cycle + 1]
+};
+
+// Fock + CDIIS (one workgroup).  cycle < 0: plain F = H + Veff (no DIIS).
+__global__ __launch_bounds__(NT) void k_scf_fock(Scf3 s, int cycle) {
+  if (s.ist[0]) return;
+  __shared__ double red[NW];
+  const int n = s.n, nn = n * n;
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    double f = s.H[t] + s.J[t];
+    if (s.method == 0) f += s.Vxc[t];
+    if (s.method == 1) f -= 0.5 * s.K[t];
+    s.F[t] = f;
+  }
+  __syncthreads();
+  if (cycle < 1 || s.diis_space < 1) {
+    for (int t = threadIdx.x; t < nn; t += NT) s.Fd[t] = s.F[t];
+    __syncthreads();
+    return;
+  }
+  // error e = F D S - S D F
+  mm(n, s.F, false, s.D, false, s.tmp1, 1.0, 0.0);   // FD
+  mm(n, s.tmp1, false, s.S, false, s.tmp2, 1.0, 0.0);  // FDS
+  // S D F = (F D S)^T for symmetric F, D, S
+  const int cnt = s.ist[2], head = s.ist[3];
+  const int slot = head;
+  double* Fi = s.diisF + (size_t)slot * nn;
+  double* Ei = s.diisE + (size_t)slot * nn;
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    const int i = t / n, j = t % n;
+    Ei[t] = s.tmp2[i * n + j] - s.tmp2[j * n + i];
+    Fi[t] = s.F[t];
+  }
+  __syncthreads();
+  const int m = cnt + 1 < s.diis_space ? cnt + 1 : s.diis_space;
+  // B (stored by slot) row/column of the new vector
+  const int M = s.diis_space;
+  for (int q = 0; q < m; ++q) {
+    const double d = dot_all(nn, Ei, s.diisE + (size_t)q * nn, red);
+    if (threadIdx.x == 0) {
+      s.diisB[slot * M + q] = d;
+      s.diisB[q * M + slot] = d;
+    }
+  }
+  __syncthreads();
+  // solve [0 -1; -1 B] [l; c] = [-1; 0] in the chronological order of the stored vectors
+  __shared__ double Bm[(QEX3D_MAX_DIIS + 1) * (QEX3D_MAX_DIIS + 2)];
+  __shared__ double coef[QEX3D_MAX_DIIS + 1];
+  __shared__ int order[QEX3D_MAX_DIIS];
+  if (threadIdx.x == 0) {
+    const int oldest = m < M ? 0 : (slot + 1) % M;
+    for (int q = 0; q < m; ++q) order[q] = (oldest + q) % M;
+    const int dim = m + 1, ld = m + 2;
+    for (int r = 0; r < dim; ++r)
+      for (int c = 0; c < dim; ++c) {
+        double v;
+        if (r == 0 && c == 0)
+          v = 0.0;
+        else if (r == 0 || c == 0)
+          v = -1.0;
+        else
+          v = s.diisB[order[r - 1] * M + order[c - 1]];
+        Bm[r * ld + c] = v;
+      }
+    for (int r = 0; r < dim; ++r) Bm[r * ld + dim] = r == 0 ? -1.0 : 0.0;
+    // Gaussian elimination with partial pivoting
+    for (int c = 0; c < dim; ++c) {
+      int pr = c;
+      for (int r = c + 1; r < dim; ++r)
+        if (fabs(Bm[r * ld + c]) > fabs(Bm[pr * ld + c])) pr = r;
+      if (pr != c)
+        for (int k = 0; k <= dim; ++k) {
+          const double t = Bm[c * ld + k];
+          Bm[c * ld + k] = Bm[pr * ld + k];
+          Bm[pr * ld + k] = t;
+        }
+      const double pv = Bm[c * ld + c];
+      for (int r = c + 1; r < dim; ++r) {
+        const double f = Bm[r * ld + c] / pv;
+        for (int k = c; k <= dim; ++k) Bm[r * ld + k] -= f * Bm[c * ld + k];
+      }
+    }
+    for (int r = dim - 1; r >= 0; --r) {
+      double v = Bm[r * ld + dim];
+      for (int k = r + 1; k < dim; ++k) v -= Bm[r * ld + k] * coef[k];
+      coef[r] = v / Bm[r * ld + r];
+    }
+    s.ist[2] = m;
+    s.ist[3] = (slot + 1) % M;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    double f = 0.0;
+    for (int q = 0; q < m; ++q) f = fma(coef[q + 1], s.diisF[(size_t)order[q] * nn + t], f);
+    s.Fd[t] = f;
+  }
+  __syncthreads();
+}
+
+// generalized eigh of Fd (X^T Fd X, Jacobi warm-started from Cp), C = X V, D = 2 C_occ C_occ^T
+__global__ __launch_bounds__(NT) void k_scf_eigh(Scf3 s, int cold) {
+  if (s.ist[0]) return;
+  extern __shared__ double sm[];
+  const int n = s.n, nn = n * n;
+  double* A = sm;
+  double* V = A + nn;
+  __shared__ double cs[QEX3D_MAX_NAO + 2];
+  __shared__ int pq[QEX3D_MAX_NAO + 2];
+  __shared__ int perm[QEX3D_MAX_NAO];
+  __shared__ double ev[QEX3D_MAX_NAO];
+  // Fp = X^T Fd X  (tmp1 = Fd X; tmp2 = X^T tmp1)
+  mm(n, s.Fd, false, s.X, false, s.tmp1, 1.0, 0.0);
+  mm(n, s.X, true, s.tmp1, false, s.tmp2, 1.0, 0.0);
+  if (cold) {
+    for (int t = threadIdx.x; t < nn; t += NT) {
+      A[t] = 0.5 * (s.tmp2[t] + s.tmp2[(t % n) * n + t / n]);
+      V[t] = (t / n == t % n) ? 1.0 : 0.0;
+    }
+  } else {
+    // warm start: A = Cp^T Fp Cp, V = Cp
+    mm(n, s.tmp2, false, s.Cp, false, s.tmp1, 1.0, 0.0);
+    mm(n, s.Cp, true, s.tmp1, false, s.tmp2, 1.0, 0.0);
+    for (int t = threadIdx.x; t < nn; t += NT) {
+      A[t] = 0.5 * (s.tmp2[t] + s.tmp2[(t % n) * n + t / n]);
+      V[t] = s.Cp[t];
+    }
+  }
+  __syncthreads();
+  jacobi(n, A, V, cs, pq, 30, 1e-15);
+  // sort eigenvalues (ascending) by rank
+  for (int i = threadIdx.x; i < n; i += NT) ev[i] = A[i * n + i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NT) {
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += (ev[j] < ev[i]) || (ev[j] == ev[i] && j < i);
+    perm[r] = i;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    const int row = t / n, c = t % n;
+    s.Cp[t] = V[row * n + perm[c]];
+  }
+  for (int i = threadIdx.x; i < n; i += NT) s.mo[i] = ev[perm[i]];
+  __syncthreads();
+  mm(n, s.X, false, s.Cp, false, s.C, 1.0, 0.0);
+  for (int t = threadIdx.x; t < nn; t += NT) s.Dlast[t] = s.D[t];
+  __syncthreads();
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    const int i = t / n, j = t % n;
+    double d = 0.0;
+    for (int k = 0; k < s.nocc; ++k) d = fma(s.C[i * n + k], s.C[j * n + k], d);
+    s.D[t] = 2.0 * d;
+  }
+  __syncthreads();
+}
+
+// Vxc = sum of chunk partials (fixed order), symmetrised; energies and convergence.
+__global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
+  if (s.ist[0]) return;
+  __shared__ double red[NW];
+  const int n = s.n, nn = n * n;
+  double nel = 0.0, exc = 0.0;
+  if (s.method == 0) {
+    for (int t = threadIdx.x; t < nn; t += NT) {
+      double v = 0.0;
+      for (int c = 0; c < s.nchunk; ++c) v += s.vpart[(size_t)c * nn + t];
+      s.tmp1[t] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nn; t += NT) s.Vxc[t] = s.tmp1[t] + s.tmp1[(t % n) * n + t / n];
+    if (threadIdx.x == 0) {
+      for (int c = 0; c < s.nchunk; ++c) {
+        nel += s.spart[2 * c];
+        exc += s.spart[2 * c + 1];
+      }
+      s.scal[2] = nel;
+      s.scal[3] = exc;
+    }
+    __syncthreads();
+  }
+  double e1 = 0.0, e2 = 0.0;
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    e1 = fma(s.D[t], s.H[t], e1);
+    double g = 0.5 * s.J[t];
+    if (s.method == 1) g -= 0.25 * s.K[t];
+    e2 = fma(s.D[t], g, e2);
+  }
+  e1 = bsum(e1, red);
+  e2 = bsum(e2, red);
+  const double etot = e1 + e2 + (s.method == 0 ? s.scal[3] : 0.0) + s.enuc;
+  if (cycle < 0) {
+    if (threadIdx.x == 0) {
+      s.scal[0] = etot;
+      s.e_hist[0] = etot;
+    }
+    return;
+  }
+  // orbital gradient norm with the new Fock matrix: |C_vir^T F C_occ|^2 via the S^-1 projector
+  // = tr(Co^T F X X^T F Co) - |Co^T F Co|^2, times 4 (the factor 2 of pyscf's get_grad)
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    double f = s.H[t] + s.J[t];
+    if (s.method == 0) f += s.Vxc[t];
+    if (s.method == 1) f -= 0.5 * s.K[t];
+    s.F[t] = f;
+  }
+  __syncthreads();
+  // tmp1[:, k<nocc] = F C_occ ; tmp2 = X^T tmp1 (columns k < nocc)
+  for (int t = threadIdx.x; t < n * s.nocc; t += NT) {
+    const int i = t / s.nocc, k = t % s.nocc;
+    double v = 0.0;
+    for (int j = 0; j < n; ++j) v = fma(s.F[i * n + j], s.C[j * n + k], v);
+    s.tmp1[i * n + k] = v;
+  }
+  __syncthreads();
+  double a = 0.0, b = 0.0;
+  for (int t = threadIdx.x; t < n * s.nocc; t += NT) {
+    const int i = t / s.nocc, k = t % s.nocc;
+    double v = 0.0;
+    for (int j = 0; j < n; ++j) v = fma(s.X[j * n + i], s.tmp1[j * n + k], v);
+    a = fma(v, v, a);
+  }
+  for (int t = threadIdx.x; t < s.nocc * s.nocc; t += NT) {
+    const int k = t / s.nocc, l = t % s.nocc;
+    double v = 0.0;
+    for (int j = 0; j < n; ++j) v = fma(s.C[j * n + k], s.tmp1[j * n + l], v);
+    b = fma(v, v, b);
+  }
+  a = bsum(a, red);
+  b = bsum(b, red);
+  if (threadIdx.x == 0) {
+    const double g2 = fmax(0.0, 4.0 * (a - b));
+    const int nvir = n - s.nocc;
+    const double gnorm = sqrt(g2) / sqrt((double)(nvir > 0 ? nvir * s.nocc : 1));
+    const double elast = s.scal[0];
+    s.scal[1] = elast;
+    s.scal[0] = etot;
+    s.scal[4] = gnorm;
+    s.e_hist[cycle + 1] = etot;
+    s.ist[1] = cycle + 1;
+    if (fabs(etot - elast) < s.conv_tol && gnorm < sqrt(s.conv_tol)) {
+      s.ist[0] = 1;
+      s.ist[4] = 1;
+    }
+  }
+}
+
+__global__ void k_scf_init(Scf3 s) {
+  if (threadIdx.x < 8) s.ist[threadIdx.x] = 0;
+  if (threadIdx.x < 8) s.scal[threadIdx.x] = 0.0;
+}
+
+// fill e_hist beyond the last cycle with the final energy (frozen state repeats)
+__global__ void k_scf_finish(Scf3 s) {
+  for (int c = threadIdx.x; c <= s.max_cycle; c += blockDim.x)
+    if (c > s.ist[1]) s.e_hist[c] = s.scal[0];
+}
+
+struct Lay3 {
+  size_t F, Fd, D, Dlast, C, Cp, mo, J, K, Vxc, vpart, spart, rho, tmp1, tmp2, diisF, diisE, diisB, scal, ist, total;
+};
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Lay3 layout3(int n, long ng, int space) {
+  Lay3 L{};
+  size_t off = 0;
+  const size_t nn = (size_t)n * n * 8;
+  const long nchunk = vxc_blocks(ng);
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off = al(off + b);
+    return o;
+  };
+  L.F = take(nn);
+  L.Fd = take(nn);
+  L.D = take(nn);
+  L.Dlast = take(nn);
+  L.C = take(nn);
+  L.Cp = take(nn);
+  L.mo = take((size_t)n * 8);
+  L.J = take(nn);
+  L.K = take(nn);
+  L.Vxc = take(nn);
+  L.vpart = take((size_t)nchunk * nn);
+  L.spart = take((size_t)nchunk * 16);
+  L.rho = take((size_t)ng * 8);
+  L.tmp1 = take(nn);
+  L.tmp2 = take(nn);
+  L.diisF = take((size_t)space * nn);
+  L.diisE = take((size_t)space * nn);
+  L.diisB = take((size_t)space * space * 8);
+  L.scal = take(64);
+  L.ist = take(64);
+  L.total = off;
+  return L;
+}
+
+Basis make_basis(const qex3d_basis* b) {
+  Basis d{};
+  d.natm = b->natm;
+  d.nshell = b->nshell;
+  d.nao = b->nao;
+  d.xyz = b->atom_coords;
+  d.charge = b->atom_charges;
+  d.sh_atom = b->shell_atom;
+  d.sh_l = b->shell_l;
+  d.sh_nprim = b->shell_nprim;
+  d.sh_off = b->shell_offset;
+  d.exps = b->exponents;
+  d.coefs = b->coefficients;
+  d.ao_shell = b->ao_shell;
+  d.ao_lmn = b->ao_lmn;
+  return d;
+}
+
+int check_basis(const qex3d_basis* b) {
+  if (!b) return fail3(QEX_E_BADARG, "null basis");
+  if (b->natm < 1 || b->nshell < 1 || b->nao < 1) return fail3(QEX_E_BADARG, "empty basis");
+  if (b->nao > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "nao %d exceeds %d", b->nao, QEX3D_MAX_NAO);
+  if (!b->atom_coords || !b->atom_charges || !b->shell_atom || !b->shell_l || !b->shell_nprim || !b->shell_offset ||
+      !b->exponents || !b->coefficients || !b->ao_shell || !b->ao_lmn)
+    return fail3(QEX_E_BADARG, "missing basis array");
+  return QEX_OK;
+}
+
+__global__ void k_nr_rks_reduce(int n, int nchunk, const double* vpart, const double* spart, double* vmat, double* out2) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
+    const int i = t / n, j = t % n;
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      a += vpart[(size_t)c * n * n + i * n + j];
+      b += vpart[(size_t)c * n * n + j * n + i];
+    }
+    vmat[t] = a + b;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      s0 += spart[2 * c];
+      s1 += spart[2 * c + 1];
+    }
+    out2[0] = s0;
+    out2[1] = s1;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qex3d_last_error(void) { return g_err3; }
+
+int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* stream) {
+  int rc = check_basis(b);
+  if (rc) return rc;
+  if (!S || !T || !V) return fail3(QEX_E_BADARG, "null output");
+  const int np = b->nao * (b->nao + 1) / 2;
+  hipLaunchKernelGGL(k_int1e, dim3((np + 127) / 128), dim3(128), 0, (hipStream_t)stream, make_basis(b), S, T, V);
+  return launch_status("qex3d_int1e");
+}
+
+int qex3d_eri(const qex3d_basis* b, double* eri, void* stream) {
+  int rc = check_basis(b);
+  if (rc) return rc;
+  if (!eri) return fail3(QEX_E_BADARG, "null output");
+  const int np = b->nao * (b->nao + 1) / 2;
+  const long total = (long)np * (np + 1) / 2;
+  hipLaunchKernelGGL(k_eri, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, (hipStream_t)stream, make_basis(b),
+                     eri, np);
+  return launch_status("qex3d_eri");
+}
+
+int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const double* rad_w, int32_t nang,
+               const double* ang_xyz, const double* ang_w, double* coords, double* weights, void* stream) {
+  int rc = check_basis(b);
+  if (rc) return rc;
+  if (nrad < 1 || nang < 1 || !rad_r || !rad_w || !ang_xyz || !ang_w || !coords || !weights)
+    return fail3(QEX_E_BADARG, "bad grid arguments");
+  const long ng = (long)b->natm * nrad * nang;
+  hipLaunchKernelGGL(k_grid, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, (hipStream_t)stream, make_basis(b), nrad,
+                     rad_r, rad_w, nang, ang_xyz, ang_w, coords, weights);
+  return launch_status("qex3d_grid");
+}
+
+int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, double* ao, void* stream) {
+  int rc = check_basis(b);
+  if (rc) return rc;
+  if (ngrid < 1 || !coords || !ao) return fail3(QEX_E_BADARG, "bad eval_ao arguments");
+  const long total = (long)ngrid * b->nshell;
+  hipLaunchKernelGGL(k_eval_ao, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     make_basis(b), (long)ngrid, coords, ao);
+  return launch_status("qex3d_eval_ao");
+}
+
+int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* dm, double* rho, void* stream) {
+  if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !dm || !rho) return fail3(QEX_E_BADARG, "bad eval_rho arguments");
+  rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, (hipStream_t)stream);
+  return launch_status("qex3d_eval_rho");
+}
+
+size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao) {
+  if (ngrid < 1 || nao < 1) return 0;
+  const long nb = vxc_blocks(ngrid);
+  return al((size_t)ngrid * 8) + al((size_t)nb * nao * nao * 8) + al((size_t)nb * 16);
+}
+
+int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
+                 double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes, void* stream) {
+  if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !weights || !dm || !nelec_excsum || !vmat)
+    return fail3(QEX_E_BADARG, "bad nr_rks arguments");
+  const size_t need = qex3d_nr_rks_workspace_bytes(ngrid, nao);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = vxc_blocks(ngrid);
+  double* rho = (double*)workspace;
+  double* vpart = (double*)((char*)workspace + al((size_t)ngrid * 8));
+  double* spart = (double*)((char*)vpart + al((size_t)nb * nao * nao * 8));
+  rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, st);
+  hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, vpart,
+                     spart, nullptr);
+  hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum);
+  return launch_status("qex3d_nr_rks");
+}
+
+// grid size the SCF workspace is laid out for (Hartree-Fock keeps a 1-point dummy)
+static long scf_ngrid(const qex3d_scf_problem* p) { return p->method == QEX3D_RKS_LDA ? (long)p->ngrid : 1; }
+static int scf_space(const qex3d_scf_problem* p) { return p->diis_space > 0 ? p->diis_space : 1; }
+
+size_t qex3d_scf_workspace_bytes(const qex3d_scf_problem* p) {
+  if (!p || p->nao < 1 || p->nao > QEX3D_MAX_NAO || scf_ngrid(p) < 1) return 0;
+  return layout3(p->nao, scf_ngrid(p), scf_space(p)).total + al((size_t)p->nao * p->nao * 8) * 2;
+}
+
+int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
+              size_t workspace_bytes, void* stream) {
+  if (!p || !in || !out) return fail3(QEX_E_BADARG, "null argument");
+  const int n = p->nao;
+  if (n < 1 || n > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "nao must be in [1, %d]", QEX3D_MAX_NAO);
+  if (p->nelectron < 2 || (p->nelectron & 1) || p->nelectron / 2 > n)
+    return fail3(QEX_E_BADARG, "closed-shell nelectron required (got %d)", p->nelectron);
+  if (p->method != QEX3D_RKS_LDA && p->method != QEX3D_RHF) return fail3(QEX_E_BADARG, "bad method");
+  if (p->method == QEX3D_RKS_LDA && (p->ngrid < 1 || !in->ao || !in->weights)) return fail3(QEX_E_BADARG, "RKS needs a grid");
+  if (p->max_cycle < 1 || p->diis_space < 0 || p->diis_space > QEX3D_MAX_DIIS) return fail3(QEX_E_BADARG, "bad cycle/diis");
+  if (!in->S || !in->H || !in->eri) return fail3(QEX_E_BADARG, "missing S, H or eri");
+  if (!out->e_tot || !out->mo_energy || !out->mo_coeff || !out->dm || !out->status)
+    return fail3(QEX_E_BADARG, "missing output array");
+  const size_t need = qex3d_scf_workspace_bytes(p);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  hipStream_t st = (hipStream_t)stream;
+  const long ng = scf_ngrid(p);
+  const int space = scf_space(p);
+  const Lay3 L = layout3(n, ng, space);
+  if (L.total + 2 * al((size_t)n * n * 8) > need) return fail3(QEX_E_WORKSPACE, "internal layout exceeds workspace");
+  char* ws = (char*)workspace;
+  Scf3 s{};
+  s.n = n;
+  s.npair = n * (n + 1) / 2;
+  s.nocc = p->nelectron / 2;
+  s.max_cycle = p->max_cycle;
+  s.diis_space = p->diis_space;
+  s.method = p->method;
+  s.ng = ng;
+  s.nchunk = vxc_blocks(ng);
+  s.conv_tol = p->conv_tol;
+  s.enuc = p->energy_nuc;
+  s.S = in->S;
+  s.H = in->H;
+  s.eri = in->eri;
+  s.ao = in->ao;
+  s.w = in->weights;
+  double* Lchol = (double*)(ws + L.total);
+  double* X = (double*)(ws + L.total + al((size_t)n * n * 8));
+  s.X = X;
+  s.F = (double*)(ws + L.F);
+  s.Fd = (double*)(ws + L.Fd);
+  s.D = out->dm;
+  s.Dlast = (double*)(ws + L.Dlast);
+  s.C = out->mo_coeff;
+  s.Cp = (double*)(ws + L.Cp);
+  s.mo = out->mo_energy;
+  s.J = (double*)(ws + L.J);
+  s.K = (double*)(ws + L.K);
+  s.Vxc = (double*)(ws + L.Vxc);
+  s.vpart = (double*)(ws + L.vpart);
+  s.spart = (double*)(ws + L.spart);
+  s.rho = (double*)(ws + L.rho);
+  s.tmp1 = (double*)(ws + L.tmp1);
+  s.tmp2 = (double*)(ws + L.tmp2);
+  s.diisF = (double*)(ws + L.diisF);
+  s.diisE = (double*)(ws + L.diisE);
+  s.diisB = (double*)(ws + L.diisB);
+  s.scal = (double*)(ws + L.scal);
+  s.ist = (int*)(ws + L.ist);
+  s.e_hist = out->e_tot;
+  const size_t lds_eigh = (size_t)2 * n * n * 8;
+  if (lds_eigh > 160 * 1024) return fail3(QEX_E_UNSUPPORTED, "nao too large for the LDS eigensolver");
+  if (lds_eigh > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_scf_eigh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eigh);
+  const size_t lds_vxc = vxc_lds(n);
+  const int jblocks = s.npair < 2048 ? s.npair : 2048;
+  auto veff = [&]() {
+    if (s.method == QEX3D_RKS_LDA) {
+      rho_dispatch(ng, n, s.ao, s.D, s.rho, s.ist, st);
+      hipLaunchKernelGGL(k_vxc, dim3((unsigned)s.nchunk), dim3(NT), lds_vxc, st, ng, n, s.ao, s.w, s.rho, s.vpart,
+                         s.spart, s.ist);
+    }
+    hipLaunchKernelGGL(k_coulomb, dim3(jblocks), dim3(NT), (size_t)s.npair * 8, st, n, s.npair, s.eri, s.D, s.J,
+                       s.method == QEX3D_RHF ? s.K : nullptr, s.ist);
+  };
+  hipLaunchKernelGGL(k_scf_init, dim3(1), dim3(64), 0, st, s);
+  hipLaunchKernelGGL(k_orthogonaliser, dim3(1), dim3(NT), 0, st, n, s.S, Lchol, X);
+  // core-Hamiltonian guess: F = H (J, K, Vxc and D cleared by a kernel, not a memset)
+  for (double* z : {s.J, s.Vxc, s.K, s.D}) hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, st, z, (long)n * n);
+  hipLaunchKernelGGL(k_scf_fock, dim3(1), dim3(NT), 0, st, s, -1);
+  hipLaunchKernelGGL(k_scf_eigh, dim3(1), dim3(NT), lds_eigh, st, s, 1);
+  veff();
+  hipLaunchKernelGGL(k_scf_energy, dim3(1), dim3(NT), 0, st, s, -1);
+  for (int c = 0; c < p->max_cycle; ++c) {
+    hipLaunchKernelGGL(k_scf_fock, dim3(1), dim3(NT), 0, st, s, c);
+    hipLaunchKernelGGL(k_scf_eigh, dim3(1), dim3(NT), lds_eigh, st, s, 0);
+    veff();
+    hipLaunchKernelGGL(k_scf_energy, dim3(1), dim3(NT), 0, st, s, c);
+  }
+  hipLaunchKernelGGL(k_scf_finish, dim3(1), dim3(64), 0, st, s);
+  (void)hipMemcpyAsync(out->status, s.ist, 8 * sizeof(int), hipMemcpyDeviceToDevice, st);
+  return launch_status("qex3d_scf");
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""Molecules and Cartesian s/p Gaussian basis sets, laid out for the device (``qex3d_basis``).
+
+Mirrors the parts of ``pyscf.gto.Mole`` the reference reads (``atom``, ``basis``, ``nelectron``,
+``nao``, ``energy_nuc``; ``qedft/train/td/hf_legacy.py``, ``td/numint.py``).  Contractions are
+normalised to unit self-overlap; coefficients stored for the device include the primitive
+normalisation (2a/pi)^(3/4) (2 sqrt(a))^l.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..od import engine
+
+BOHR = 0.52917721092
+
+BASIS_LIBRARY = {
+    "sto-3g": {"H": [(0, [3.42525091, 0.62391373, 0.16885540], [0.15432897, 0.53532814, 0.44463454])]},
+    "6-31g": {"H": [(0, [18.7311370, 2.8253937, 0.6401217], [0.03349460, 0.23472695, 0.81375733]),
+                    (0, [0.1612778], [1.0])]},
+}
+ELEMENTS = {"H": 1, "He": 2, "Li": 3, "Be": 4}
+
+
+def prim_norm(alpha, l):
+    return (2 * alpha / math.pi) ** 0.75 * (2 * math.sqrt(alpha)) ** l
+
+
+class Mole:
+    """atom: list of (symbol or Z, (x, y, z)); unit 'Angstrom' or 'Bohr'; basis: a name from
+    BASIS_LIBRARY or {symbol: [(l, exps, coefs), ...]}; or per-atom shells via ``shells_per_atom``."""
+
+    def __init__(self, atom, basis="6-31g", unit="Angstrom", shells_per_atom=None):
+        zs, xyz = [], []
+        for sym, pos in atom:
+            zs.append(ELEMENTS[sym] if isinstance(sym, str) else int(sym))
+            xyz.append(pos)
+        self.charges = np.asarray(zs, dtype=np.float64)
+        scale = 1.0 / BOHR if unit.lower().startswith("a") else 1.0
+        self.coords = np.asarray(xyz, dtype=np.float64) * scale
+        if shells_per_atom is None:
+            lib = BASIS_LIBRARY[basis.lower()] if isinstance(basis, str) else basis
+            inv = {v: k for k, v in ELEMENTS.items()}
+            shells_per_atom = [lib[inv[int(z)]] for z in zs]
+        self.shells = []  # (atom, l, exps, coefs incl. primitive norm)
+        for a, shells in enumerate(shells_per_atom):
+            for l, exps, coefs in shells:
+                exps = np.asarray(exps, dtype=np.float64)
+                coefs = np.asarray(coefs, dtype=np.float64)
+                nrm = np.array([prim_norm(e, l) for e in exps])
+                p = exps[:, None] + exps[None, :]
+                sij = (math.pi / p) ** 1.5 * (1.0 / (2 * p)) ** l
+                ss = float(np.sum((coefs * nrm)[:, None] * (coefs * nrm)[None, :] * sij))
+                self.shells.append((a, int(l), exps, coefs * nrm / math.sqrt(ss)))
+        self._dev = None
+
+    @property
+    def natm(self):
+        return len(self.charges)
+
+    @property
+    def nelectron(self):
+        return int(round(float(self.charges.sum())))
+
+    @property
+    def nao(self):
+        return sum(1 if l == 0 else 3 for _, l, _, _ in self.shells)
+
+    def energy_nuc(self):
+        e = 0.0
+        for a in range(self.natm):
+            for b in range(a):
+                e += self.charges[a] * self.charges[b] / float(np.linalg.norm(self.coords[a] - self.coords[b]))
+        return e
+
+    def device_basis(self):
+        """qex3d_basis struct over device arrays (kept alive on the Mole)."""
+        if self._dev is not None:
+            return self._dev[0]
+        dev = engine.device()
+        sh_atom, sh_l, sh_n, sh_off, exps, coefs, ao_shell, ao_lmn = [], [], [], [], [], [], [], []
+        for si, (a, l, e, c) in enumerate(self.shells):
+            sh_atom.append(a)
+            sh_l.append(l)
+            sh_n.append(len(e))
+            sh_off.append(len(exps))
+            exps += list(e)
+            coefs += list(c)
+            if l == 0:
+                ao_shell.append(si)
+                ao_lmn.append(0)
+            else:
+                for lmn in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+                    ao_shell.append(si)
+                    ao_lmn.append(lmn[0] | (lmn[1] << 2) | (lmn[2] << 4))
+        f = lambda x: engine.as_f64(np.asarray(x, dtype=np.float64).reshape(-1), dev)
+        i = lambda x: engine.as_i32(np.asarray(x, dtype=np.int32), dev)
+        keep = [f(self.coords), f(self.charges), i(sh_atom), i(sh_l), i(sh_n), i(sh_off), f(exps), f(coefs),
+                i(ao_shell), i(ao_lmn)]
+        b = _lib.Qex3dBasis(self.natm, len(self.shells), self.nao, *[ctypes.c_void_p(t.data_ptr()) for t in keep])
+        self._dev = (b, keep)
+        return b
+
+    # ---- integrals on the device ----
+    def intor_1e(self):
+        dev = engine.device()
+        n = self.nao
+        S, T, V = (torch.empty((n, n), dtype=torch.float64, device=dev) for _ in range(3))
+        lib = _lib.load()
+        _lib.check3(lib.qex3d_int1e(ctypes.byref(self.device_basis()), engine._ptr(S), engine._ptr(T), engine._ptr(V),
+                                    engine._stream()), "qex3d_int1e")
+        return S, T, V
+
+    def intor_eri_pairs(self):
+        dev = engine.device()
+        npair = self.nao * (self.nao + 1) // 2
+        g = torch.empty((npair, npair), dtype=torch.float64, device=dev)
+        _lib.check3(_lib.load().qex3d_eri(ctypes.byref(self.device_basis()), engine._ptr(g), engine._stream()),
+                    "qex3d_eri")
+        return g
+
+
+def synthetic_chain(n_atoms, seed):
+    """Hydrogen chain along z with bonds U[0.5, 1.5] Angstrom and, per atom, 2 contracted s shells and
+    1 p shell of 3 primitives (exponents log-uniform in [0.1, 20], random positive coefficients),
+    all from the seed: 5 AOs per atom."""
+    rng = np.random.default_rng(seed)
+    z = np.concatenate([[0.0], np.cumsum(rng.uniform(0.5, 1.5, n_atoms - 1))]) / BOHR
+    coords = np.stack([np.zeros(n_atoms), np.zeros(n_atoms), z - z.mean()], axis=1)
+    shells = []
+    for _ in range(n_atoms):
+        per = []
+        for l in (0, 0, 1):
+            exps = np.sort(np.exp(rng.uniform(np.log(0.1), np.log(20.0), 3)))[::-1].copy()
+            coefs = rng.uniform(0.2, 1.0, 3)
+            per.append((l, exps, coefs))
+        shells.append(per)
+    return Mole([("H", tuple(c)) for c in coords], unit="Bohr", shells_per_atom=shells)

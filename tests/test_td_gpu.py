@@ -1,0 +1,171 @@
+"""3D RKS/RHF device path against the FP64 oracle (oracle/td.py), on small molecules.
+
+Integrals, grid, AO values, density and LDA potential matrix are compared elementwise; SCF runs
+are compared by total energy (1e-8 Ha), density matrix and orbital energies.  H2/6-31G RHF
+energies are the values printed in the reference notebook qedft/data/td/hydrogen_rhf.ipynb.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import td as otd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qtd():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from qex_amd import td
+    return td
+
+
+def _mol_pair(qtd, kind, arg):
+    if kind == "h2":
+        from qex_amd.td.gto import Mole
+        return (Mole([("H", (0, 0, 0)), ("H", (0, 0, arg))], basis="6-31g"), otd.h2_molecule(arg))
+    return qtd.gto.synthetic_chain(arg, 7), otd.synthetic_chain(arg, 7)
+
+
+def _pairs_to_full(gp, n):
+    idx = np.array([[max(i, j) * (max(i, j) + 1) // 2 + min(i, j) for j in range(n)] for i in range(n)])
+    return gp[idx[:, :, None, None], idx[None, None, :, :]]
+
+
+@pytest.mark.parametrize("kind,arg", [("h2", 0.74), ("chain", 3), ("chain", 4)])
+def test_one_electron_integrals(qtd, kind, arg):
+    mol, omol = _mol_pair(qtd, kind, arg)
+    S, T, V = (x.cpu().numpy() for x in mol.intor_1e())
+    oS, oT, oV = otd.one_electron(omol)
+    for a, b in ((S, oS), (T, oT), (V, oV)):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * max(1.0, np.abs(b).max()))
+
+
+@pytest.mark.parametrize("kind,arg", [("h2", 0.74), ("chain", 2), ("chain", 3)])
+def test_eri_pair_matrix(qtd, kind, arg):
+    mol, omol = _mol_pair(qtd, kind, arg)
+    g = mol.intor_eri_pairs().cpu().numpy()
+    full = _pairs_to_full(g, mol.nao)
+    ref = otd.eri(omol)
+    np.testing.assert_allclose(full, ref, rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(g, g.T)
+
+
+def test_grid_matches_oracle(qtd):
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    grids = qtd.dft.Grids(mol).build()
+    pts, w = otd.molecular_grid(omol)
+    np.testing.assert_allclose(grids.coords.cpu().numpy(), pts, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(grids.weights.cpu().numpy(), w, rtol=1e-11, atol=1e-13 * w.max())
+
+
+def _random_dm(n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal((n, n)) * 0.3
+    return c @ c.T
+
+
+@pytest.mark.parametrize("n_atoms", [1, 4, 9])
+def test_eval_ao_rho_nr_rks(qtd, n_atoms):
+    mol = qtd.gto.synthetic_chain(n_atoms, 11) if n_atoms > 1 else None
+    omol = otd.synthetic_chain(n_atoms, 11) if n_atoms > 1 else None
+    if n_atoms == 1:
+        from qex_amd.td.gto import Mole
+        mol = Mole([("H", (0, 0, 0.1))], basis="6-31g")
+        omol = otd.build_molecule(np.array([[0, 0, 0.1 / otd.BOHR]]), [1], [otd.B631G_H])
+    grids = qtd.dft.Grids(mol).build()
+    pts = grids.coords.cpu().numpy()
+    w = grids.weights.cpu().numpy()
+    ao = qtd.numint.eval_ao(mol, grids.coords)
+    oao = otd.eval_ao(omol, pts)
+    np.testing.assert_allclose(ao.cpu().numpy(), oao, rtol=0, atol=1e-13 * np.abs(oao).max())
+    dm = _random_dm(mol.nao, n_atoms)
+    rho = qtd.numint.eval_rho(mol, ao, dm).cpu().numpy()
+    orho = otd.eval_rho(oao, dm)
+    np.testing.assert_allclose(rho, orho, rtol=1e-11, atol=1e-13 * np.abs(orho).max())
+    nel, exc, vmat = qtd.numint.nr_rks(qtd.numint.NumInt(), mol, grids, "lda", dm)
+    onel, oexc, ovmat = otd.nr_rks_lda(oao, w, dm)
+    assert abs(float(nel) - onel) < 1e-10 * max(1, abs(onel))
+    assert abs(float(exc) - oexc) < 1e-10 * max(1, abs(oexc))
+    np.testing.assert_allclose(vmat.cpu().numpy(), ovmat, rtol=0, atol=1e-10 * np.abs(ovmat).max())
+
+
+def test_eval_rho_nonsymmetric_dm_is_symmetrised(qtd):
+    mol, omol = _mol_pair(qtd, "chain", 2)
+    grids = qtd.dft.Grids(mol).build()
+    ao = qtd.numint.eval_ao(mol, grids.coords)
+    rng = np.random.default_rng(3)
+    dm = rng.standard_normal((mol.nao, mol.nao))
+    rho = qtd.numint.eval_rho(mol, ao, dm).cpu().numpy()
+    orho = otd.eval_rho(ao.cpu().numpy(), dm)
+    np.testing.assert_allclose(rho, orho, rtol=1e-10, atol=1e-12 * np.abs(orho).max())
+
+
+@pytest.mark.parametrize("r,energy", [(0.74, -1.12675531719693), (1.5, -1.05802481296927),
+                                      (2.0, -0.997497294328357)])
+def test_h2_631g_rhf_notebook_energies(qtd, r, energy):
+    mol, _ = _mol_pair(qtd, "h2", r)
+    mf = qtd.scf.RHF(mol)
+    mf.conv_tol = 1e-11
+    e = mf.kernel()
+    assert mf.converged
+    assert abs(e - energy) < 1e-9
+
+
+@pytest.mark.parametrize("method", ["rks", "rhf"])
+@pytest.mark.parametrize("n_atoms", [2, 4])
+def test_scf_matches_oracle(qtd, method, n_atoms):
+    mol, omol = _mol_pair(qtd, "chain", n_atoms)
+    mf = qtd.scf.RKS(mol) if method == "rks" else qtd.scf.RHF(mol)
+    mf.conv_tol = 1e-10
+    e = mf.kernel()
+    S, H, g = mf.integrals()
+    ints = (S.cpu().numpy(), (H.cpu().numpy()), None)
+    grid = None
+    if method == "rks":
+        grid = (mf.grids.coords.cpu().numpy(), mf.grids.weights.cpu().numpy())
+    oS, oT, oV = otd.one_electron(omol)
+    ref = otd.scf(omol, method, grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
+                  conv_tol=1e-10, max_cycle=50)
+    assert ref.converged and mf.converged
+    assert abs(e - ref.e_tot) < 1e-8
+    np.testing.assert_allclose(mf.dm.cpu().numpy(), ref.dm, rtol=0, atol=1e-6)
+    nocc = mol.nelectron // 2
+    np.testing.assert_allclose(mf.mo_energy.cpu().numpy()[:nocc + 1], ref.mo_energy[:nocc + 1], atol=1e-6)
+    # per-cycle energies follow the same trajectory while both run
+    ne = min(len(ref.energies), mf.cycles)
+    np.testing.assert_allclose(mf.energies.cpu().numpy()[1:1 + ne], ref.energies[:ne], atol=1e-7)
+
+
+def test_scf_larger_chain_against_oracle_with_device_integrals(qtd):
+    """16 atoms, 80 AOs (near the single-workgroup eigensolver limit of 96)."""
+    mol = qtd.gto.synthetic_chain(16, 5)
+    omol = otd.synthetic_chain(16, 5)
+    mf = qtd.scf.RKS(mol)
+    mf.conv_tol = 1e-9
+    mf.max_cycle = 60
+    e = mf.kernel()
+    S, H, g = mf.integrals()
+    oS, oT, oV = otd.one_electron(omol)
+    np.testing.assert_allclose(S.cpu().numpy(), oS, atol=1e-12)
+    grid = (mf.grids.coords.cpu().numpy(), mf.grids.weights.cpu().numpy())
+    ref = otd.scf(omol, "rks", grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
+                  conv_tol=1e-9, max_cycle=60)
+    assert ref.converged == mf.converged
+    assert abs(e - ref.e_tot) < 1e-7
+
+
+def test_scf_invariants(qtd):
+    mol, _ = _mol_pair(qtd, "chain", 4)
+    mf = qtd.scf.RKS(mol)
+    mf.kernel()
+    S, _, _ = mf.integrals()
+    C = mf.mo_coeff
+    eye = (C.T @ S @ C).cpu().numpy()
+    np.testing.assert_allclose(eye, np.eye(mol.nao), atol=1e-10)
+    D = mf.dm
+    assert abs(float((D * S).sum()) - mol.nelectron) < 1e-10
+    mo = mf.mo_energy.cpu().numpy()
+    assert np.all(np.diff(mo) >= -1e-12)

@@ -159,6 +159,8 @@ int32_t qex_abi_version(void);
 #define QEX3D_MAX_DIIS 12
 #define QEX3D_RKS_LDA 0 /* Slater exchange + VWN5 correlation */
 #define QEX3D_RHF 1
+#define QEX3D_RKS_NN_GLOBAL 2 /* amplitude-encoded network on the whole grid density */
+#define QEX3D_MAX_NN_HIDDEN 1024
 
 /* Basis on the device.  Shells are listed in AO order (an s shell is 1 AO, a p shell 3: x, y, z);
  * coefficients include the primitive normalisation and a normalised contraction. */
@@ -204,11 +206,28 @@ size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao);
 int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
                  double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Global (amplitude-encoded) neural XC: x = rho[ngrid] -> Dense(h1) -> gelu -> Dense(h2) -> gelu ->
+ * Dense(1) -> -scale * swish (gelu in its tanh form); excsum = the output, vrho = d excsum / d rho.
+ * Parameters packed as W1[ngrid][h1], b1[h1], W2[h1][h2], b2[h2], W3[h2], b3[1] (flax Dense
+ * kernels are (in, out)).  nr_rks "NN-AmplitudeEncoding": nelec_excsum = {sum rho w, excsum},
+ * vmat = V + V^T with V = ao^T diag(w vrho / 2) ao.  Replaces exc_and_vrho_global
+ * (qedft/train/td/trainer_legacy_no_jit.py:46-53, MLP :96-107) inside nr_rks
+ * (qedft/train/td/numint.py:206-231, 264-265). */
+size_t qex3d_nr_rks_nn_workspace_bytes(int64_t ngrid, int32_t nao, int32_t h1, int32_t h2);
+int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
+                    const double* nn_params, int32_t h1, int32_t h2, double scale, double* nelec_excsum,
+                    double* vmat, void* workspace, size_t workspace_bytes, void* stream);
+
 typedef struct qex3d_scf_problem {
   int32_t nao, nelectron, method, max_cycle, diis_space;
   int64_t ngrid;
   double conv_tol;   /* |dE| < conv_tol and |g|/sqrt(nvir nocc) < sqrt(conv_tol) */
   double energy_nuc;
+  int32_t sync_every; /* 0: enqueue max_cycle cycles without host synchronisation (converged cycles are
+                         no-ops on the device); k > 0: the host waits on the stream every k cycles and
+                         stops enqueueing once converged */
+  int32_t nn_h1, nn_h2; /* QEX3D_RKS_NN_GLOBAL: hidden widths */
+  double nn_scale;
 } qex3d_scf_problem;
 
 typedef struct qex3d_scf_inputs {
@@ -217,6 +236,7 @@ typedef struct qex3d_scf_inputs {
   const double* eri;     /* pair matrix from qex3d_eri */
   const double* ao;      /* [ngrid][nao] (RKS) */
   const double* weights; /* [ngrid] (RKS) */
+  const double* nn_params; /* packed network (QEX3D_RKS_NN_GLOBAL) */
 } qex3d_scf_inputs;
 
 typedef struct qex3d_scf_outputs {

@@ -234,50 +234,71 @@ def one_electron(mol):
     return S, Tk, V
 
 
-def eri(mol):
-    """(ij|kl) in chemists' notation, full n^4 array (8-fold symmetry used in the loops)."""
+def _eri_pair_data(info, i, j):
+    A, la, ea, ca = info[i]
+    B, lb, eb, cb = info[j]
+    a = ea[:, None]
+    b = eb[None, :]
+    p = (a + b).reshape(-1)
+    P = ((a[..., None] * A + b[..., None] * B) / (a + b)[..., None]).reshape(-1, 3)
+    cc = (ca[:, None] * cb[None, :]).reshape(-1)
+    E = [[hermite_E(la[d], lb[d], t, A[d] - B[d], a, b).reshape(-1) for t in range(la[d] + lb[d] + 1)]
+         for d in range(3)]
+    return p, P, cc, E, [la[d] + lb[d] for d in range(3)]
+
+
+def _eri_quartet(d1, d2):
+    p, P, c1, E1, L1 = d1
+    q, Q, c2, E2, L2 = d2
+    pp = p[:, None]
+    qq = q[None, :]
+    alpha = pp * qq / (pp + qq)
+    PQ = P[:, None, :] - Q[None, :, :]
+    RPQ = np.linalg.norm(PQ, axis=-1)
+    acc = 0.0
+    for t in range(L1[0] + 1):
+        for u in range(L1[1] + 1):
+            for v in range(L1[2] + 1):
+                e1 = (E1[0][t] * E1[1][u] * E1[2][v])[:, None]
+                for tau in range(L2[0] + 1):
+                    for nu in range(L2[1] + 1):
+                        for phi in range(L2[2] + 1):
+                            e2 = (E2[0][tau] * E2[1][nu] * E2[2][phi])[None, :]
+                            sign = (-1) ** (tau + nu + phi)
+                            R = hermite_R(t + tau, u + nu, v + phi, 0, alpha, PQ[..., 0], PQ[..., 1], PQ[..., 2], RPQ)
+                            acc = acc + sign * e1 * e2 * R
+    return float(np.sum(c1[:, None] * c2[None, :] * 2 * np.pi ** 2.5 / (pp * qq * np.sqrt(pp + qq)) * acc))
+
+
+def _eri_rows(args):
+    mol, rows = args
     aos = mol.ao_list()
     n = len(aos)
     info = [_prims(mol, ao) for ao in aos]
-    out = np.zeros((n, n, n, n))
     pairs = [(i, j) for i in range(n) for j in range(i + 1)]
+    pdat = {pr: _eri_pair_data(info, *pr) for pr in pairs}
+    return [(x, [_eri_quartet(pdat[pairs[x]], pdat[pairs[y]]) for y in range(x + 1)]) for x in rows]
 
-    def pair_data(i, j):
-        A, la, ea, ca = info[i]
-        B, lb, eb, cb = info[j]
-        a = ea[:, None]
-        b = eb[None, :]
-        p = (a + b).reshape(-1)
-        P = ((a[..., None] * A + b[..., None] * B) / (a + b)[..., None]).reshape(-1, 3)
-        cc = (ca[:, None] * cb[None, :]).reshape(-1)
-        E = [[hermite_E(la[d], lb[d], t, A[d] - B[d], a, b).reshape(-1) for t in range(la[d] + lb[d] + 1)]
-             for d in range(3)]
-        return p, P, cc, E, [la[d] + lb[d] for d in range(3)]
 
-    pdat = {pr: pair_data(*pr) for pr in pairs}
-    for x, (i, j) in enumerate(pairs):
-        p, P, c1, E1, L1 = pdat[(i, j)]
-        for (k, l) in pairs[:x + 1]:
-            q, Q, c2, E2, L2 = pdat[(k, l)]
-            pp = p[:, None]
-            qq = q[None, :]
-            alpha = pp * qq / (pp + qq)
-            PQ = P[:, None, :] - Q[None, :, :]
-            RPQ = np.linalg.norm(PQ, axis=-1)
-            acc = 0.0
-            for t in range(L1[0] + 1):
-                for u in range(L1[1] + 1):
-                    for v in range(L1[2] + 1):
-                        e1 = (E1[0][t] * E1[1][u] * E1[2][v])[:, None]
-                        for tau in range(L2[0] + 1):
-                            for nu in range(L2[1] + 1):
-                                for phi in range(L2[2] + 1):
-                                    e2 = (E2[0][tau] * E2[1][nu] * E2[2][phi])[None, :]
-                                    sign = (-1) ** (tau + nu + phi)
-                                    R = hermite_R(t + tau, u + nu, v + phi, 0, alpha, PQ[..., 0], PQ[..., 1],
-                                                  PQ[..., 2], RPQ)
-                                    acc = acc + sign * e1 * e2 * R
-            val = np.sum(c1[:, None] * c2[None, :] * 2 * np.pi ** 2.5 / (pp * qq * np.sqrt(pp + qq)) * acc)
+def eri(mol, procs=1):
+    """(ij|kl) in chemists' notation, full n^4 array (8-fold symmetry used in the loops); the unique
+    pair-pair quartets are optionally spread over `procs` host processes."""
+    n = mol.nao
+    pairs = [(i, j) for i in range(n) for j in range(i + 1)]
+    npair = len(pairs)
+    if procs > 1:
+        import multiprocessing as mp
+        chunks = [list(range(r, npair, procs * 4)) for r in range(procs * 4)]
+        with mp.get_context("spawn").Pool(procs) as pool:
+            parts = pool.map(_eri_rows, [(mol, c) for c in chunks])
+        rows = [r for part in parts for r in part]
+    else:
+        rows = _eri_rows((mol, range(npair)))
+    out = np.zeros((n, n, n, n))
+    for x, vals in rows:
+        i, j = pairs[x]
+        for y, val in enumerate(vals):
+            k, l = pairs[y]
             for (a_, b_) in ((i, j), (j, i)):
                 for (c_, d_) in ((k, l), (l, k)):
                     out[a_, b_, c_, d_] = val
@@ -404,6 +425,71 @@ def nr_rks_lda(ao, weights, dm):
 
 
 # ---------------------------------------------------------------------------
+# Global (amplitude-encoded) neural XC on the full grid density
+# ---------------------------------------------------------------------------
+def gelu_tanh(x):
+    """flax.linen.gelu with approximate=True (its default)."""
+    return 0.5 * x * (1.0 + np.tanh(math.sqrt(2.0 / math.pi) * (x + 0.044715 * x ** 3)))
+
+
+def gelu_tanh_grad(x):
+    k = math.sqrt(2.0 / math.pi)
+    t = np.tanh(k * (x + 0.044715 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * k * (1.0 + 3 * 0.044715 * x * x)
+
+
+def _sigmoid(x):
+    return 0.5 * (1.0 + np.tanh(0.5 * x))
+
+
+class GlobalMLP3D:
+    """The MLP of ``trainer_legacy_no_jit.py:96-107`` applied to the whole grid density (the
+    amplitude encoding, ``exc_and_vrho_global`` at :46-53): x = rho[n_g] -> Dense -> gelu -> ... ->
+    Dense(1) -> -scale * swish; excsum = sum of the output; vrho = d excsum / d rho.
+    weights[l] has shape (fan_in, fan_out) as flax Dense kernels."""
+
+    def __init__(self, weights, biases, scale=1e-2):
+        self.W = [np.asarray(w, dtype=np.float64) for w in weights]
+        self.b = [np.asarray(b, dtype=np.float64) for b in biases]
+        self.scale = float(scale)
+
+    def exc_vrho(self, rho):
+        h = np.asarray(rho, dtype=np.float64)
+        zs = []
+        for W, b in zip(self.W[:-1], self.b[:-1]):
+            z = h @ W + b
+            zs.append(z)
+            h = gelu_tanh(z)
+        z = h @ self.W[-1] + self.b[-1]
+        s = _sigmoid(z)
+        out = -self.scale * z * s
+        g = -self.scale * (s + z * s * (1.0 - s))  # d out / d z
+        gh = self.W[-1] @ g
+        for W, z in zip(reversed(self.W[:-1]), reversed(zs)):
+            gh = W @ (gh * gelu_tanh_grad(z))
+        return float(np.sum(out)), gh
+
+
+def init_global_mlp(seed, n_in, hidden=(512, 512)):
+    """Seeded weights: kernels N(0, 1/fan_in) (the variance of flax's lecun_normal), zero biases."""
+    rng = np.random.default_rng(seed)
+    sizes = [n_in, *hidden, 1]
+    W = [rng.standard_normal((a, b)) / math.sqrt(a) for a, b in zip(sizes[:-1], sizes[1:])]
+    return W, [np.zeros(b) for b in sizes[1:]]
+
+
+def nr_rks_nn_global(ao, weights, dm, net):
+    """(nelec, excsum, vmat) of nr_rks's "NN-AmplitudeEncoding" branch (td/numint.py:206-231,
+    264-265): excsum is the network's scalar and vrho is scaled by the weights once more."""
+    rho = eval_rho(ao, dm)
+    excsum, vrho = net.exc_vrho(rho)
+    nelec = float(np.dot(rho, weights))
+    aow = ao * (0.5 * weights * vrho)[:, None]
+    vmat = ao.T @ aow
+    return nelec, excsum, vmat + vmat.T
+
+
+# ---------------------------------------------------------------------------
 # SCF (RKS-LDA or RHF) with CDIIS
 # ---------------------------------------------------------------------------
 def get_j(eri_, dm):
@@ -464,14 +550,15 @@ class SCFResult:
     nelec_grid: float = float("nan")
 
 
-def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8, ints=None):
+def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8, ints=None, net=None):
     """pyscf-style SCF (hf_legacy.py:51-116): core-Hamiltonian guess; DIIS from cycle 1;
-    converged when |dE| < conv_tol and |g|/sqrt(size) < sqrt(conv_tol)."""
+    converged when |dE| < conv_tol and |g|/sqrt(size) < sqrt(conv_tol).  method: "rks" (LDA),
+    "rks_nn" (GlobalMLP3D `net` on the grid density) or "rhf"."""
     S, T, V = ints[:3] if ints is not None else one_electron(mol)
     g = ints[3] if ints is not None else eri(mol)
     H = T + V
     nocc = mol.nelectron // 2
-    if method == "rks":
+    if method in ("rks", "rks_nn"):
         pts, w = grid if grid is not None else molecular_grid(mol)
         ao = eval_ao(mol, pts)
 
@@ -480,7 +567,10 @@ def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8,
         if method == "rhf":
             K = get_k(g, D)
             return J - 0.5 * K, 0.5 * np.sum(D * J) - 0.25 * np.sum(D * K), np.nan
-        nel, exc, vxc = nr_rks_lda(ao, w, D)
+        if method == "rks_nn":
+            nel, exc, vxc = nr_rks_nn_global(ao, w, D, net)
+        else:
+            nel, exc, vxc = nr_rks_lda(ao, w, D)
         return J + vxc, 0.5 * np.sum(D * J) + exc, nel
 
     e, C = gen_eigh(H, S)

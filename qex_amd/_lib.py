@@ -83,6 +83,8 @@ class Qex1dOutputs(ctypes.Structure):
 
 QEX3D_RKS_LDA = 0
 QEX3D_RHF = 1
+QEX3D_RKS_NN_GLOBAL = 2
+QEX3D_MAX_NN_HIDDEN = 1024
 QEX3D_MAX_NAO = 96
 
 
@@ -95,11 +97,12 @@ class Qex3dBasis(ctypes.Structure):
 class Qex3dScfProblem(ctypes.Structure):
     _fields_ = [("nao", ctypes.c_int32), ("nelectron", ctypes.c_int32), ("method", ctypes.c_int32),
                 ("max_cycle", ctypes.c_int32), ("diis_space", ctypes.c_int32), ("ngrid", ctypes.c_int64),
-                ("conv_tol", ctypes.c_double), ("energy_nuc", ctypes.c_double)]
+                ("conv_tol", ctypes.c_double), ("energy_nuc", ctypes.c_double), ("sync_every", ctypes.c_int32), ("nn_h1", ctypes.c_int32),
+                ("nn_h2", ctypes.c_int32), ("nn_scale", ctypes.c_double)]
 
 
 class Qex3dScfInputs(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("S", "H", "eri", "ao", "weights")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("S", "H", "eri", "ao", "weights", "nn_params")]
 
 
 class Qex3dScfOutputs(ctypes.Structure):
@@ -131,12 +134,15 @@ SIGNATURES = {
     "qex3d_eval_rho": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P]),
     "qex3d_nr_rks_workspace_bytes": (_S, [ctypes.c_int64, _I]),
     "qex3d_nr_rks": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),
+    "qex3d_nr_rks_nn_workspace_bytes": (_S, [ctypes.c_int64, _I, _I, _I]),
+    "qex3d_nr_rks_nn": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S, _P]),
     "qex3d_scf_workspace_bytes": (_S, [ctypes.POINTER(Qex3dScfProblem)]),
     "qex3d_scf": (_I, [ctypes.POINTER(Qex3dScfProblem), ctypes.POINTER(Qex3dScfInputs),
                        ctypes.POINTER(Qex3dScfOutputs), _P, _S, _P]),
 }
 
-KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize"]
+KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize", "int1e", "eri", "grid", "eval_ao", "rho", "vxc",
+                "coulomb", "scf_fock", "scf_eigh", "scf_energy", "scf_other", "nn_global"]
 
 _lib = None
 

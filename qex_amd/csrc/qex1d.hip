@@ -24,7 +24,16 @@
 #include <string.h>
 
 #include "qex.h"
+#include "prof.h"
 #include "band.h"
+
+namespace qexprof {
+bool g_on = false;
+Rec* g_rec = nullptr;
+int g_n = 0, g_cap = 0;
+hipEvent_t* g_pool = nullptr;
+int g_pool_n = 0, g_pool_used = 0;
+}  // namespace qexprof
 
 namespace {
 
@@ -56,40 +65,7 @@ int check_launch(const char* what) {
   return QEX_OK;
 }
 
-// ---- optional kernel timing (hipEvent pairs around each launch, per kernel kind) ---------------
-constexpr int NKINDS = 8;  // 0 mlp, 1 potential, 2 eigen, 3 density, 4 finalize/other
-struct ProfRec {
-  int kind;
-  hipEvent_t a, b;
-};
-bool g_prof_on = false;
-ProfRec* g_prof = nullptr;
-int g_prof_n = 0, g_prof_cap = 0;
-hipEvent_t* g_pool = nullptr;
-int g_pool_n = 0, g_pool_used = 0;
-
-hipEvent_t pool_event() {
-  if (g_pool_used >= g_pool_n) return nullptr;
-  return g_pool[g_pool_used++];
-}
-
-struct ProfScope {
-  int kind;
-  hipStream_t st;
-  hipEvent_t b = nullptr, e = nullptr;
-  ProfScope(int k, hipStream_t s) : kind(k), st(s) {
-    if (!g_prof_on || g_prof_n >= g_prof_cap) return;
-    b = pool_event();
-    e = pool_event();
-    if (b && e) (void)hipEventRecord(b, st);
-  }
-  ~ProfScope() {
-    if (b && e) {
-      (void)hipEventRecord(e, st);
-      g_prof[g_prof_n++] = ProfRec{kind, b, e};
-    }
-  }
-};
+using qexprof::Scope;
 
 // --------------------------------------------------------------------------------------------
 // device helpers
@@ -1341,13 +1317,14 @@ const char* qex_last_error(void) { return g_err; }
 int32_t qex_abi_version(void) { return 1; }
 
 int qex_profile_enable(int32_t enable, int32_t max_launches) {
+  using namespace qexprof;
   if (enable) {
     if (max_launches < 1) return fail(QEX_E_BADARG, "max_launches must be >= 1");
-    if (max_launches > g_prof_cap) {
-      ProfRec* np = (ProfRec*)realloc(g_prof, sizeof(ProfRec) * max_launches);
+    if (max_launches > g_cap) {
+      Rec* np = (Rec*)realloc(g_rec, sizeof(Rec) * max_launches);
       if (!np) return fail(QEX_E_BADARG, "out of host memory");
-      g_prof = np;
-      g_prof_cap = max_launches;
+      g_rec = np;
+      g_cap = max_launches;
     }
     const int need = 2 * max_launches;
     if (need > g_pool_n) {
@@ -1359,30 +1336,31 @@ int qex_profile_enable(int32_t enable, int32_t max_launches) {
         g_pool_n = i + 1;
       }
     }
-    g_prof_n = 0;
+    g_n = 0;
     g_pool_used = 0;
   }
-  g_prof_on = enable != 0;
+  g_on = enable != 0;
   return QEX_OK;
 }
 
 int qex_profile_read(double* ms_per_kind, int32_t* launches_per_kind, int32_t nkinds) {
+  using namespace qexprof;
   if (!ms_per_kind || !launches_per_kind || nkinds < 1) return fail(QEX_E_BADARG, "bad arguments");
   for (int k = 0; k < nkinds; ++k) {
     ms_per_kind[k] = 0.0;
     launches_per_kind[k] = 0;
   }
-  for (int i = 0; i < g_prof_n; ++i) {
+  for (int i = 0; i < g_n; ++i) {
     float ms = 0.f;
-    if (hipEventSynchronize(g_prof[i].b) != hipSuccess || hipEventElapsedTime(&ms, g_prof[i].a, g_prof[i].b) != hipSuccess)
+    if (hipEventSynchronize(g_rec[i].b) != hipSuccess || hipEventElapsedTime(&ms, g_rec[i].a, g_rec[i].b) != hipSuccess)
       return fail(QEX_E_LAUNCH, "event timing failed");
-    const int k = g_prof[i].kind;
+    const int k = g_rec[i].kind;
     if (k < nkinds) {
       ms_per_kind[k] += ms;
       launches_per_kind[k] += 1;
     }
   }
-  g_prof_n = 0;
+  g_n = 0;
   g_pool_used = 0;
   return QEX_OK;
 }
@@ -1513,7 +1491,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   double alpha_k = p->alpha;
   for (int k = 0; k <= a.T; ++k) {
     if (a.xc == QEX1D_XC_LOCAL_MLP) {
-      ProfScope ps(0, stream);
+      Scope ps(qexprof::K_MLP, stream);
       if (mlp_mfma) {
         switch (a.nlay - 1) {
           case 1: hipLaunchKernelGGL(k_mlp_mfma<1>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
@@ -1525,22 +1503,22 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
         hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
     }
     {
-      ProfScope ps(1, stream);
+      Scope ps(qexprof::K_POTENTIAL, stream);
       hipLaunchKernelGGL(k_potential, gpot, dim3(NT), lds_pot, stream, a, k);
     }
     if (k == a.T) break;  // the last pass only evaluates the final iteration's energies
     {
-      ProfScope ps(2, stream);
+      Scope ps(qexprof::K_EIGEN, stream);
       hipLaunchKernelGGL(k_eigen, geig, dim3(NT), lds_eig, stream, e);
     }
     {
-      ProfScope ps(3, stream);
+      Scope ps(qexprof::K_DENSITY, stream);
       hipLaunchKernelGGL(k_density, dim3(a.B), dim3(NT), (size_t)a.N * 8, stream, a, k, alpha_k);
     }
     alpha_k *= p->alpha_decay;
   }
   {
-    ProfScope ps(4, stream);
+    Scope ps(qexprof::K_FINALIZE, stream);
     hipLaunchKernelGGL(k_finalize, dim3(a.B), dim3(64), 0, stream, a);
   }
   return check_launch("qex1d_kohn_sham");

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include "qex.h"
+#include "prof.h"
 
 namespace {
 
@@ -477,8 +478,10 @@ __device__ void lda_svwn5(double rho, double* e, double* v) {
 // output tiles in registers (wave w owns tiles w, w+4, ...), so partials are per workgroup.
 constexpr int CH = 64;
 constexpr int VX_TILES = 9;  // tiles per wave: (npad/16)^2 / 4 for npad <= 96
+// wv_given (optional): per-point w v / 2 computed elsewhere (neural XC); then only sum(rho w) is
+// accumulated and the energy partial is zero.
 __global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
-                                            double* vpart, double* spart, const int* frozen) {
+                                            const double* wv_given, double* vpart, double* spart, const int* frozen) {
   extern __shared__ double sm[];
   if (frozen && *frozen) return;
   const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16;
@@ -500,9 +503,9 @@ __global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, co
       if (g < ng) {
         r = rho[g];
         ww = w[g];
-        lda_svwn5(r, &e, &v);
+        if (!wv_given) lda_svwn5(r, &e, &v);
       }
-      wvs[t] = 0.5 * ww * v;
+      wvs[t] = wv_given ? (g < ng ? wv_given[g] : 0.0) : 0.5 * ww * v;
       nel += r * ww;
       exc += r * ww * e;
     }
@@ -555,6 +558,159 @@ void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rh
 int vxc_blocks(long ng) {
   const long nchunk = (ng + CH - 1) / CH;
   return (int)(nchunk < VXC_BLOCKS ? nchunk : VXC_BLOCKS);
+}
+
+// ---------------------------------------------------------------------------------------------
+// global (amplitude-encoded) neural XC: x = rho[ng] -> Dense(h1) -> gelu -> Dense(h2) -> gelu ->
+// Dense(1) -> -scale swish; excsum = output, vrho = d excsum / d rho.  W1 is ng x h1 (the large
+// operand): one streaming pass for z1 = W1^T rho and one for vrho = W1 dz1, both HBM-bound.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double gelu_t(double x) {
+  const double k = 0.7978845608028654;  // sqrt(2 / pi)
+  return 0.5 * x * (1.0 + tanh(k * (x + 0.044715 * x * x * x)));
+}
+__device__ __forceinline__ double gelu_t_grad(double x) {
+  const double k = 0.7978845608028654;
+  const double t = tanh(k * (x + 0.044715 * x * x * x));
+  return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * k * (1.0 + 3.0 * 0.044715 * x * x);
+}
+
+struct NNOff {
+  size_t W1, b1, W2, b2, W3, b3, total;  // offsets (doubles) into the packed parameters
+};
+NNOff nn_offsets(long ng, int h1, int h2) {
+  NNOff o;
+  o.W1 = 0;
+  o.b1 = o.W1 + (size_t)ng * h1;
+  o.W2 = o.b1 + h1;
+  o.b2 = o.W2 + (size_t)h1 * h2;
+  o.W3 = o.b2 + h2;
+  o.b3 = o.W3 + h2;
+  o.total = o.b3 + 1;
+  return o;
+}
+
+constexpr int GNN_BLOCKS = 1024;
+int gnn_blocks(long ng) {
+  const long b = (ng + 15) / 16;
+  return (int)(b < GNN_BLOCKS ? b : GNN_BLOCKS);
+}
+
+// partial z1 over a contiguous chunk of grid rows per workgroup (8 independent row streams per thread)
+__global__ __launch_bounds__(NT) void k_gnn_l1(long ng, int h1, const double* rho, const double* W1, double* zpart,
+                                               const int* frozen) {
+  if (frozen && *frozen) return;
+  const long per = (ng + gridDim.x - 1) / gridDim.x;
+  const long g0 = (long)blockIdx.x * per;
+  const long g1 = g0 + per < ng ? g0 + per : ng;
+  for (int c = threadIdx.x; c < h1; c += NT) {
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long g = g0;
+    for (; g + 7 < g1; g += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = fma(rho[g + u], W1[(size_t)(g + u) * h1 + c], a[u]);
+    }
+    for (; g < g1; ++g) a[0] = fma(rho[g], W1[(size_t)g * h1 + c], a[0]);
+    zpart[(size_t)blockIdx.x * h1 + c] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  }
+}
+
+// z1 = sum of the partials (fixed order) + b1; 64 columns per workgroup, 16 partial streams each
+__global__ __launch_bounds__(1024) void k_gnn_red(int nb, int h1, const double* zpart, const double* b1, double* z1,
+                                                  const int* frozen) {
+  if (frozen && *frozen) return;
+  __shared__ double acc[16][64];
+  const int cl = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  double v = 0.0;
+  if (c < h1)
+    for (int b = q; b < nb; b += 16) v += zpart[(size_t)b * h1 + c];
+  acc[q][cl] = v;
+  __syncthreads();
+  if (q == 0 && c < h1) {
+    double t = 0.0;
+    for (int k = 0; k < 16; ++k) t += acc[k][cl];
+    z1[c] = t + b1[c];
+  }
+}
+
+// the small layers and the backward pass down to dz1 (one workgroup of 1024 threads)
+__global__ __launch_bounds__(1024) void k_gnn_mid(int h1, int h2, const double* P, NNOff o, double scale,
+                                                  const double* z1, double* dz1, double* excsum, const int* frozen) {
+  if (frozen && *frozen) return;
+  extern __shared__ double sm[];
+  double* a1 = sm;        // gelu(z1)   [h1]
+  double* z2 = a1 + h1;   // [h2]
+  double* d2 = z2 + h2;   // dz2        [h2]
+  __shared__ double red[16];
+  __shared__ double gout;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
+  for (int c = t; c < h1; c += blockDim.x) a1[c] = gelu_t(z1[c]);
+  __syncthreads();
+  for (int j = t; j < h2; j += blockDim.x) {
+    double z = 0.0;
+    for (int c = 0; c < h1; ++c) z = fma(a1[c], P[o.W2 + (size_t)c * h2 + j], z);
+    z2[j] = z + P[o.b2 + j];
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int j = t; j < h2; j += blockDim.x) part = fma(gelu_t(z2[j]), P[o.W3 + j], part);
+  part = wsum(part);
+  if (lane == 0) red[wv] = part;
+  __syncthreads();
+  if (t == 0) {
+    double z3 = 0.0;
+    for (int k = 0; k < nw; ++k) z3 += red[k];
+    z3 += P[o.b3];
+    const double sg = 1.0 / (1.0 + exp(-z3));
+    *excsum = -scale * z3 * sg;
+    gout = -scale * (sg + z3 * sg * (1.0 - sg));
+  }
+  __syncthreads();
+  for (int j = t; j < h2; j += blockDim.x) d2[j] = P[o.W3 + j] * gout * gelu_t_grad(z2[j]);
+  __syncthreads();
+  // dz1[c] = gelu'(z1[c]) sum_j W2[c][j] dz2[j]: one wave per row of W2 (coalesced)
+  for (int c = wv; c < h1; c += nw) {
+    double v = 0.0;
+    for (int j = lane; j < h2; j += 64) v = fma(P[o.W2 + (size_t)c * h2 + j], d2[j], v);
+    v = wsum(v);
+    if (lane == 0) dz1[c] = v * gelu_t_grad(z1[c]);
+  }
+}
+
+// vrho[g] = sum_c W1[g][c] dz1[c]; writes wv[g] = w[g] vrho[g] / 2 (one wave per grid row)
+__global__ __launch_bounds__(NT) void k_gnn_vjp(long ng, int h1, const double* W1, const double* dz1, const double* w,
+                                                double* wv, const int* frozen) {
+  if (frozen && *frozen) return;
+  extern __shared__ double sdz[];
+  for (int c = threadIdx.x; c < h1; c += NT) sdz[c] = dz1[c];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (long g = (long)blockIdx.x * NW + (threadIdx.x >> 6); g < ng; g += (long)gridDim.x * NW) {
+    const double* row = W1 + (size_t)g * h1;
+    double v = 0.0;
+    for (int c = lane; c < h1; c += 64) v = fma(row[c], sdz[c], v);
+    v = wsum(v);
+    if (lane == 0) wv[g] = 0.5 * w[g] * v;
+  }
+}
+
+void gnn_dispatch(long ng, int h1, int h2, double scale, const double* P, const double* rho, const double* w,
+                  double* zpart, double* z1, double* dz1, double* excsum, double* wv, const int* frozen,
+                  hipStream_t st) {
+  const NNOff o = nn_offsets(ng, h1, h2);
+  const int nb = gnn_blocks(ng);
+  hipLaunchKernelGGL(k_gnn_l1, dim3(nb), dim3(NT), 0, st, ng, h1, rho, P + o.W1, zpart, frozen);
+  hipLaunchKernelGGL(k_gnn_red, dim3((h1 + 63) / 64), dim3(1024), 0, st, nb, h1, zpart, P + o.b1, z1, frozen);
+  hipLaunchKernelGGL(k_gnn_mid, dim3(1), dim3(1024), (size_t)(h1 + 2 * h2) * 8, st, h1, h2, P, o, scale, z1, dz1,
+                     excsum, frozen);
+  const long vb = (ng + NW - 1) / NW;
+  hipLaunchKernelGGL(k_gnn_vjp, dim3((unsigned)(vb < 2048 ? vb : 2048)), dim3(NT), (size_t)h1 * 8, st, ng, h1,
+                     P + o.W1, dz1, w, wv, frozen);
+}
+
+__global__ void k_copy1(const double* src, double* dst) {
+  if (threadIdx.x == 0) *dst = *src;
 }
 
 __global__ void k_zero(double* p, long n) {
@@ -741,7 +897,7 @@ __device__ void jacobi(int n, double* A, double* V, double* cs, int* pq, int max
 }
 
 struct Scf3 {
-  int n, npair, nocc, max_cycle, diis_space, method;  // method 0 RKS-LDA, 1 RHF
+  int n, npair, nocc, max_cycle, diis_space, method;  // method 0 RKS-LDA, 1 RHF, 2 RKS with a global network
   long ng;
   int nchunk;
   double conv_tol, enuc;
@@ -754,9 +910,14 @@ struct Scf3 {
   // state (workspace)
   double *F, *Fd, *D, *Dlast, *C, *Cp, *mo, *J, *K, *Vxc, *vpart, *spart, *rho, *tmp1, *tmp2;
   double *diisF, *diisE, *diisB;  // [space][n*n], [space][n*n], [(space+1)^2]
-  double* scal;                    // [0] e_tot, [1] e_last, [2] nelec, [3] exc, [4] gnorm
+  double* scal;                    // [0] e_tot, [1] e_last, [2] nelec, [3] exc (LDA), [4] gnorm, [5] exc (network)
   int* ist;                        // [0] frozen, [1] cycles done, [2] diis count, [3] diis head, [4] converged
   double* e_hist;                  // [max_cycle + 1]
+  // global neural XC (method 2)
+  const double* nn;
+  int h1, h2;
+  double nn_scale;
+  double *zpart, *z1, *dz1, *wv;
 };
 
 // Fock + CDIIS (one workgroup).  cycle < 0: plain F = H + Veff (no DIIS).
@@ -766,7 +927,7 @@ __global__ __launch_bounds__(NT) void k_scf_fock(Scf3 s, int cycle) {
   const int n = s.n, nn = n * n;
   for (int t = threadIdx.x; t < nn; t += NT) {
     double f = s.H[t] + s.J[t];
-    if (s.method == 0) f += s.Vxc[t];
+    if (s.method != 1) f += s.Vxc[t];
     if (s.method == 1) f -= 0.5 * s.K[t];
     s.F[t] = f;
   }
@@ -918,7 +1079,7 @@ __global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
   __shared__ double red[NW];
   const int n = s.n, nn = n * n;
   double nel = 0.0, exc = 0.0;
-  if (s.method == 0) {
+  if (s.method != 1) {
     for (int t = threadIdx.x; t < nn; t += NT) {
       double v = 0.0;
       for (int c = 0; c < s.nchunk; ++c) v += s.vpart[(size_t)c * nn + t];
@@ -945,7 +1106,7 @@ __global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
   }
   e1 = bsum(e1, red);
   e2 = bsum(e2, red);
-  const double etot = e1 + e2 + (s.method == 0 ? s.scal[3] : 0.0) + s.enuc;
+  const double etot = e1 + e2 + (s.method == 0 ? s.scal[3] : s.method == 2 ? s.scal[5] : 0.0) + s.enuc;
   if (cycle < 0) {
     if (threadIdx.x == 0) {
       s.scal[0] = etot;
@@ -957,7 +1118,7 @@ __global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
   // = tr(Co^T F X X^T F Co) - |Co^T F Co|^2, times 4 (the factor 2 of pyscf's get_grad)
   for (int t = threadIdx.x; t < nn; t += NT) {
     double f = s.H[t] + s.J[t];
-    if (s.method == 0) f += s.Vxc[t];
+    if (s.method != 1) f += s.Vxc[t];
     if (s.method == 1) f -= 0.5 * s.K[t];
     s.F[t] = f;
   }
@@ -1014,12 +1175,13 @@ __global__ void k_scf_finish(Scf3 s) {
 }
 
 struct Lay3 {
-  size_t F, Fd, D, Dlast, C, Cp, mo, J, K, Vxc, vpart, spart, rho, tmp1, tmp2, diisF, diisE, diisB, scal, ist, total;
+  size_t F, Fd, D, Dlast, C, Cp, mo, J, K, Vxc, vpart, spart, rho, tmp1, tmp2, diisF, diisE, diisB, scal, ist;
+  size_t zpart, z1, dz1, wv, total;
 };
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Lay3 layout3(int n, long ng, int space) {
+Lay3 layout3(int n, long ng, int space, int h1 = 0) {
   Lay3 L{};
   size_t off = 0;
   const size_t nn = (size_t)n * n * 8;
@@ -1049,6 +1211,10 @@ Lay3 layout3(int n, long ng, int space) {
   L.diisB = take((size_t)space * space * 8);
   L.scal = take(64);
   L.ist = take(64);
+  L.zpart = take(h1 > 0 ? (size_t)gnn_blocks(ng) * h1 * 8 : 0);
+  L.z1 = take((size_t)h1 * 8);
+  L.dz1 = take((size_t)h1 * 8);
+  L.wv = take(h1 > 0 ? (size_t)ng * 8 : 0);
   L.total = off;
   return L;
 }
@@ -1113,6 +1279,7 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
   if (rc) return rc;
   if (!S || !T || !V) return fail3(QEX_E_BADARG, "null output");
   const int np = b->nao * (b->nao + 1) / 2;
+  qexprof::Scope ps(qexprof::K_INT1E, (hipStream_t)stream);
   hipLaunchKernelGGL(k_int1e, dim3((np + 127) / 128), dim3(128), 0, (hipStream_t)stream, make_basis(b), S, T, V);
   return launch_status("qex3d_int1e");
 }
@@ -1123,6 +1290,7 @@ int qex3d_eri(const qex3d_basis* b, double* eri, void* stream) {
   if (!eri) return fail3(QEX_E_BADARG, "null output");
   const int np = b->nao * (b->nao + 1) / 2;
   const long total = (long)np * (np + 1) / 2;
+  qexprof::Scope ps(qexprof::K_ERI, (hipStream_t)stream);
   hipLaunchKernelGGL(k_eri, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, (hipStream_t)stream, make_basis(b),
                      eri, np);
   return launch_status("qex3d_eri");
@@ -1135,6 +1303,7 @@ int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const do
   if (nrad < 1 || nang < 1 || !rad_r || !rad_w || !ang_xyz || !ang_w || !coords || !weights)
     return fail3(QEX_E_BADARG, "bad grid arguments");
   const long ng = (long)b->natm * nrad * nang;
+  qexprof::Scope ps(qexprof::K_GRID, (hipStream_t)stream);
   hipLaunchKernelGGL(k_grid, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, (hipStream_t)stream, make_basis(b), nrad,
                      rad_r, rad_w, nang, ang_xyz, ang_w, coords, weights);
   return launch_status("qex3d_grid");
@@ -1145,6 +1314,7 @@ int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, dou
   if (rc) return rc;
   if (ngrid < 1 || !coords || !ao) return fail3(QEX_E_BADARG, "bad eval_ao arguments");
   const long total = (long)ngrid * b->nshell;
+  qexprof::Scope ps(qexprof::K_AO, (hipStream_t)stream);
   hipLaunchKernelGGL(k_eval_ao, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      make_basis(b), (long)ngrid, coords, ao);
   return launch_status("qex3d_eval_ao");
@@ -1152,7 +1322,10 @@ int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, dou
 
 int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* dm, double* rho, void* stream) {
   if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !dm || !rho) return fail3(QEX_E_BADARG, "bad eval_rho arguments");
-  rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, (hipStream_t)stream);
+  {
+    qexprof::Scope ps(qexprof::K_RHO, (hipStream_t)stream);
+    rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, (hipStream_t)stream);
+  }
   return launch_status("qex3d_eval_rho");
 }
 
@@ -1173,20 +1346,85 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   double* rho = (double*)workspace;
   double* vpart = (double*)((char*)workspace + al((size_t)ngrid * 8));
   double* spart = (double*)((char*)vpart + al((size_t)nb * nao * nao * 8));
-  rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, st);
-  hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, vpart,
-                     spart, nullptr);
-  hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum);
+  {
+    qexprof::Scope ps(qexprof::K_RHO, st);
+    rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, st);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_VXC, st);
+    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho,
+                       nullptr, vpart, spart, nullptr);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_OTHER, st);
+    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum);
+  }
   return launch_status("qex3d_nr_rks");
 }
 
 // grid size the SCF workspace is laid out for (Hartree-Fock keeps a 1-point dummy)
-static long scf_ngrid(const qex3d_scf_problem* p) { return p->method == QEX3D_RKS_LDA ? (long)p->ngrid : 1; }
+static long scf_ngrid(const qex3d_scf_problem* p) { return p->method == QEX3D_RHF ? 1 : (long)p->ngrid; }
 static int scf_space(const qex3d_scf_problem* p) { return p->diis_space > 0 ? p->diis_space : 1; }
+static int scf_h1(const qex3d_scf_problem* p) { return p->method == QEX3D_RKS_NN_GLOBAL ? p->nn_h1 : 0; }
+static bool nn_widths_ok(int h1, int h2) {
+  return h1 >= 1 && h2 >= 1 && h1 <= QEX3D_MAX_NN_HIDDEN && h2 <= QEX3D_MAX_NN_HIDDEN;
+}
 
 size_t qex3d_scf_workspace_bytes(const qex3d_scf_problem* p) {
   if (!p || p->nao < 1 || p->nao > QEX3D_MAX_NAO || scf_ngrid(p) < 1) return 0;
-  return layout3(p->nao, scf_ngrid(p), scf_space(p)).total + al((size_t)p->nao * p->nao * 8) * 2;
+  if (p->method == QEX3D_RKS_NN_GLOBAL && !nn_widths_ok(p->nn_h1, p->nn_h2)) return 0;
+  return layout3(p->nao, scf_ngrid(p), scf_space(p), scf_h1(p)).total + al((size_t)p->nao * p->nao * 8) * 2;
+}
+
+size_t qex3d_nr_rks_nn_workspace_bytes(int64_t ngrid, int32_t nao, int32_t h1, int32_t h2) {
+  if (ngrid < 1 || nao < 1 || !nn_widths_ok(h1, h2)) return 0;
+  return qex3d_nr_rks_workspace_bytes(ngrid, nao) + al((size_t)gnn_blocks(ngrid) * h1 * 8) + 2 * al((size_t)h1 * 8) +
+         al((size_t)ngrid * 8) + al(64);
+}
+
+int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
+                    const double* nn_params, int32_t h1, int32_t h2, double scale, double* nelec_excsum,
+                    double* vmat, void* workspace, size_t workspace_bytes, void* stream) {
+  if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !weights || !dm || !nn_params || !nelec_excsum || !vmat)
+    return fail3(QEX_E_BADARG, "bad nr_rks_nn arguments");
+  if (!nn_widths_ok(h1, h2)) return fail3(QEX_E_BADARG, "hidden widths must be in [1, %d]", QEX3D_MAX_NN_HIDDEN);
+  const size_t need = qex3d_nr_rks_nn_workspace_bytes(ngrid, nao, h1, h2);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = vxc_blocks(ngrid);
+  char* w8 = (char*)workspace;
+  double* rho = (double*)w8;
+  double* vpart = (double*)(w8 + al((size_t)ngrid * 8));
+  double* spart = (double*)((char*)vpart + al((size_t)nb * nao * nao * 8));
+  char* q = w8 + qex3d_nr_rks_workspace_bytes(ngrid, nao);
+  double* zpart = (double*)q;
+  q += al((size_t)gnn_blocks(ngrid) * h1 * 8);
+  double* z1 = (double*)q;
+  q += al((size_t)h1 * 8);
+  double* dz1 = (double*)q;
+  q += al((size_t)h1 * 8);
+  double* wv = (double*)q;
+  q += al((size_t)ngrid * 8);
+  double* exc = (double*)q;
+  {
+    qexprof::Scope ps(qexprof::K_RHO, st);
+    rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, st);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_NN, st);
+    gnn_dispatch(ngrid, h1, h2, scale, nn_params, rho, weights, zpart, z1, dz1, exc, wv, nullptr, st);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_VXC, st);
+    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, wv,
+                       vpart, spart, nullptr);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_OTHER, st);
+    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum);
+    hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, st, exc, nelec_excsum + 1);
+  }
+  return launch_status("qex3d_nr_rks_nn");
 }
 
 int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
@@ -1196,8 +1434,11 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
   if (n < 1 || n > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "nao must be in [1, %d]", QEX3D_MAX_NAO);
   if (p->nelectron < 2 || (p->nelectron & 1) || p->nelectron / 2 > n)
     return fail3(QEX_E_BADARG, "closed-shell nelectron required (got %d)", p->nelectron);
-  if (p->method != QEX3D_RKS_LDA && p->method != QEX3D_RHF) return fail3(QEX_E_BADARG, "bad method");
-  if (p->method == QEX3D_RKS_LDA && (p->ngrid < 1 || !in->ao || !in->weights)) return fail3(QEX_E_BADARG, "RKS needs a grid");
+  if (p->method != QEX3D_RKS_LDA && p->method != QEX3D_RHF && p->method != QEX3D_RKS_NN_GLOBAL)
+    return fail3(QEX_E_BADARG, "bad method");
+  if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->ao || !in->weights)) return fail3(QEX_E_BADARG, "RKS needs a grid");
+  if (p->method == QEX3D_RKS_NN_GLOBAL && (!in->nn_params || !nn_widths_ok(p->nn_h1, p->nn_h2)))
+    return fail3(QEX_E_BADARG, "network parameters and hidden widths in [1, %d] required", QEX3D_MAX_NN_HIDDEN);
   if (p->max_cycle < 1 || p->diis_space < 0 || p->diis_space > QEX3D_MAX_DIIS) return fail3(QEX_E_BADARG, "bad cycle/diis");
   if (!in->S || !in->H || !in->eri) return fail3(QEX_E_BADARG, "missing S, H or eri");
   if (!out->e_tot || !out->mo_energy || !out->mo_coeff || !out->dm || !out->status)
@@ -1207,7 +1448,7 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
   hipStream_t st = (hipStream_t)stream;
   const long ng = scf_ngrid(p);
   const int space = scf_space(p);
-  const Lay3 L = layout3(n, ng, space);
+  const Lay3 L = layout3(n, ng, space, scf_h1(p));
   if (L.total + 2 * al((size_t)n * n * 8) > need) return fail3(QEX_E_WORKSPACE, "internal layout exceeds workspace");
   char* ws = (char*)workspace;
   Scf3 s{};
@@ -1250,6 +1491,14 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
   s.scal = (double*)(ws + L.scal);
   s.ist = (int*)(ws + L.ist);
   s.e_hist = out->e_tot;
+  s.nn = in->nn_params;
+  s.h1 = scf_h1(p);
+  s.h2 = p->nn_h2;
+  s.nn_scale = p->nn_scale;
+  s.zpart = (double*)(ws + L.zpart);
+  s.z1 = (double*)(ws + L.z1);
+  s.dz1 = (double*)(ws + L.dz1);
+  s.wv = (double*)(ws + L.wv);
   const size_t lds_eigh = (size_t)2 * n * n * 8;
   if (lds_eigh > 160 * 1024) return fail3(QEX_E_UNSUPPORTED, "nao too large for the LDS eigensolver");
   if (lds_eigh > 64 * 1024)
@@ -1257,29 +1506,50 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
   const size_t lds_vxc = vxc_lds(n);
   const int jblocks = s.npair < 2048 ? s.npair : 2048;
   auto veff = [&]() {
-    if (s.method == QEX3D_RKS_LDA) {
-      rho_dispatch(ng, n, s.ao, s.D, s.rho, s.ist, st);
-      hipLaunchKernelGGL(k_vxc, dim3((unsigned)s.nchunk), dim3(NT), lds_vxc, st, ng, n, s.ao, s.w, s.rho, s.vpart,
-                         s.spart, s.ist);
+    if (s.method != QEX3D_RHF) {
+      {
+        qexprof::Scope ps(qexprof::K_RHO, st);
+        rho_dispatch(ng, n, s.ao, s.D, s.rho, s.ist, st);
+      }
+      if (s.method == QEX3D_RKS_NN_GLOBAL) {
+        qexprof::Scope ps(qexprof::K_NN, st);
+        gnn_dispatch(ng, s.h1, s.h2, s.nn_scale, s.nn, s.rho, s.w, s.zpart, s.z1, s.dz1, s.scal + 5, s.wv, s.ist, st);
+      }
+      qexprof::Scope ps(qexprof::K_VXC, st);
+      hipLaunchKernelGGL(k_vxc, dim3((unsigned)s.nchunk), dim3(NT), lds_vxc, st, ng, n, s.ao, s.w, s.rho,
+                         s.method == QEX3D_RKS_NN_GLOBAL ? (const double*)s.wv : nullptr, s.vpart, s.spart, s.ist);
     }
+    qexprof::Scope ps(qexprof::K_COULOMB, st);
     hipLaunchKernelGGL(k_coulomb, dim3(jblocks), dim3(NT), (size_t)s.npair * 8, st, n, s.npair, s.eri, s.D, s.J,
                        s.method == QEX3D_RHF ? s.K : nullptr, s.ist);
   };
-  hipLaunchKernelGGL(k_scf_init, dim3(1), dim3(64), 0, st, s);
-  hipLaunchKernelGGL(k_orthogonaliser, dim3(1), dim3(NT), 0, st, n, s.S, Lchol, X);
+  auto small = [&](int kind, auto kernel, size_t lds, auto... args) {
+    qexprof::Scope ps(kind, st);
+    hipLaunchKernelGGL(kernel, dim3(1), dim3(NT), lds, st, args...);
+  };
+  small(qexprof::K_OTHER, k_scf_init, 0, s);
+  small(qexprof::K_OTHER, k_orthogonaliser, 0, n, s.S, Lchol, X);
   // core-Hamiltonian guess: F = H (J, K, Vxc and D cleared by a kernel, not a memset)
   for (double* z : {s.J, s.Vxc, s.K, s.D}) hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, st, z, (long)n * n);
-  hipLaunchKernelGGL(k_scf_fock, dim3(1), dim3(NT), 0, st, s, -1);
-  hipLaunchKernelGGL(k_scf_eigh, dim3(1), dim3(NT), lds_eigh, st, s, 1);
+  small(qexprof::K_FOCK, k_scf_fock, 0, s, -1);
+  small(qexprof::K_EIGH, k_scf_eigh, lds_eigh, s, 1);
   veff();
-  hipLaunchKernelGGL(k_scf_energy, dim3(1), dim3(NT), 0, st, s, -1);
+  small(qexprof::K_ENERGY, k_scf_energy, 0, s, -1);
   for (int c = 0; c < p->max_cycle; ++c) {
-    hipLaunchKernelGGL(k_scf_fock, dim3(1), dim3(NT), 0, st, s, c);
-    hipLaunchKernelGGL(k_scf_eigh, dim3(1), dim3(NT), lds_eigh, st, s, 0);
+    small(qexprof::K_FOCK, k_scf_fock, 0, s, c);
+    small(qexprof::K_EIGH, k_scf_eigh, lds_eigh, s, 0);
     veff();
-    hipLaunchKernelGGL(k_scf_energy, dim3(1), dim3(NT), 0, st, s, c);
+    small(qexprof::K_ENERGY, k_scf_energy, 0, s, c);
+    if (p->sync_every > 0 && (c + 1) % p->sync_every == 0 && c + 1 < p->max_cycle) {
+      // optional host check: stop enqueueing cycles once the device has flagged convergence
+      int32_t frozen = 0;
+      if (hipMemcpyAsync(&frozen, s.ist, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return fail3(QEX_E_LAUNCH, "convergence check failed");
+      if (frozen) break;
+    }
   }
-  hipLaunchKernelGGL(k_scf_finish, dim3(1), dim3(64), 0, st, s);
+  small(qexprof::K_OTHER, k_scf_finish, 0, s);
   (void)hipMemcpyAsync(out->status, s.ist, 8 * sizeof(int), hipMemcpyDeviceToDevice, st);
   return launch_status("qex3d_scf");
 }

@@ -37,12 +37,27 @@ def eval_rho(mol, ao, dm, non0tab=None, xctype="LDA", hermi=0, verbose=None):
 
 
 class NumInt:
-    """Holds nothing: the reference's ``ni`` object is only a dispatch handle."""
+    """The reference's ``ni`` dispatch handle; ``define_xc_`` installs a network XC
+    (``trainer_legacy_no_jit.py:256-261``)."""
+
+    def __init__(self):
+        self.network = None
+
+    def define_xc_(self, network, xctype="NN-AmplitudeEncoding"):
+        if xctype != "NN-AmplitudeEncoding":
+            raise NotImplementedError("the device path implements the amplitude-encoded (global) network")
+        self.network = network
+        return self
 
 
 def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000, verbose=None, params=None,
            ao=None):
-    """(nelec, excsum, vmat) for LDA (Slater + VWN5) on the device."""
+    """(nelec, excsum, vmat) on the device: LDA (Slater + VWN5), or the network installed with
+    ``ni.define_xc_`` (NN-AmplitudeEncoding branch, td/numint.py:206-231), whose parameters may be
+    replaced by ``params``."""
+    net = getattr(ni, "network", None)
+    if net is not None:
+        return _nr_rks_nn(net if params is None else net.with_params(params), mol, grids, dms, hermi, ao)
     if str(xc_code).lower() not in ("lda", "lda,vwn", "svwn", "lda,vwn5"):
         raise NotImplementedError(f"xc_code {xc_code!r}: the device path implements LDA (Slater + VWN5)")
     dev = engine.device()
@@ -59,4 +74,25 @@ def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000,
     _lib.check3(lib.qex3d_nr_rks(ng, nao, engine._ptr(ao), engine._ptr(engine.as_f64(grids.weights, dev)),
                                  engine._ptr(dm), engine._ptr(out2), engine._ptr(vmat), engine._ptr(ws), ws.numel(),
                                  engine._stream()), "qex3d_nr_rks")
+    return out2[0], out2[1], vmat
+
+
+def _nr_rks_nn(net, mol, grids, dms, hermi, ao):
+    dev = engine.device()
+    if ao is None:
+        ao = eval_ao(mol, grids.coords)
+    dm = engine.as_f64(dms, dev)
+    if hermi == 0:
+        dm = (0.5 * (dm + dm.T)).contiguous()
+    ng, nao = ao.shape
+    if net.n_grid != ng:
+        raise ValueError(f"network input width {net.n_grid} != grid size {ng}")
+    lib = _lib.load()
+    ws = engine.workspace(lib.qex3d_nr_rks_nn_workspace_bytes(ng, nao, net.h1, net.h2), dev)
+    out2 = torch.empty(2, dtype=torch.float64, device=dev)
+    vmat = torch.empty((nao, nao), dtype=torch.float64, device=dev)
+    p = engine._ptr
+    _lib.check3(lib.qex3d_nr_rks_nn(ng, nao, p(ao), p(engine.as_f64(grids.weights, dev)), p(dm), p(net.packed()),
+                                    net.h1, net.h2, net.scale, p(out2), p(vmat), p(ws), ws.numel(), engine._stream()),
+                "qex3d_nr_rks_nn")
     return out2[0], out2[1], vmat

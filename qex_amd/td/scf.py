@@ -23,6 +23,7 @@ class _SCF:
         self.conv_tol = 1e-9
         self.max_cycle = 50
         self.diis_space = 8
+        self.sync_every = 4  # host checks the device's convergence flag every 4 cycles (0: never)
         self.grids = None
         self.e_tot = None
         self.converged = False
@@ -43,15 +44,25 @@ class _SCF:
         S, H, eri = self.integrals()
         ao = w = None
         ng = 0
-        if self.method == _lib.QEX3D_RKS_LDA:
+        method = self.method
+        net = getattr(self, "network", None)
+        if method == _lib.QEX3D_RKS_LDA and net is not None:
+            method = _lib.QEX3D_RKS_NN_GLOBAL
+            if kwargs.get("params") is not None:
+                net = net.with_params(kwargs["params"])
+        if method != _lib.QEX3D_RHF:
             if self.grids is None:
                 self.grids = Grids(mol).build()
             if getattr(self, "_ao", None) is None:
                 self._ao = eval_ao(mol, self.grids.coords)
             ao, w, ng = self._ao, self.grids.weights, self.grids.size
         n = mol.nao
-        p = _lib.Qex3dScfProblem(n, mol.nelectron, self.method, self.max_cycle, self.diis_space, ng,
-                                 float(self.conv_tol), float(mol.energy_nuc()))
+        p = _lib.Qex3dScfProblem(n, mol.nelectron, method, self.max_cycle, self.diis_space, ng,
+                                 float(self.conv_tol), float(mol.energy_nuc()), int(self.sync_every))
+        if method == _lib.QEX3D_RKS_NN_GLOBAL:
+            if net.n_grid != ng:
+                raise ValueError(f"network input width {net.n_grid} != grid size {ng}")
+            p.nn_h1, p.nn_h2, p.nn_scale = net.h1, net.h2, net.scale
         lib = _lib.load()
         ws = engine.workspace(lib.qex3d_scf_workspace_bytes(ctypes.byref(p)), dev)
         e_hist = torch.empty(self.max_cycle + 1, dtype=torch.float64, device=dev)
@@ -60,7 +71,8 @@ class _SCF:
         D = torch.empty((n, n), dtype=torch.float64, device=dev)
         status = torch.zeros(8, dtype=torch.int32, device=dev)
         ptr = engine._ptr
-        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w))
+        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w),
+                                  ptr(net.packed()) if method == _lib.QEX3D_RKS_NN_GLOBAL else None)
         outs = _lib.Qex3dScfOutputs(ptr(e_hist), ptr(mo), ptr(C), ptr(D), ptr(status))
         _lib.check3(lib.qex3d_scf(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), ptr(ws), ws.numel(),
                                   engine._stream()), "qex3d_scf")
@@ -90,5 +102,14 @@ class RKS(_SCF):
     def __init__(self, mol, xc="lda"):
         super().__init__(mol)
         if str(xc).lower() not in ("lda", "lda,vwn", "svwn", "lda,vwn5"):
-            raise NotImplementedError("the device RKS path implements LDA (Slater + VWN5)")
+            raise NotImplementedError("the device RKS path implements LDA (Slater + VWN5) or a network XC")
         self.xc = xc
+        self.network = None
+
+    def define_xc_(self, network, xctype="NN-AmplitudeEncoding"):
+        """Install a global network XC (``trainer_legacy_no_jit.py:256-261``); kernel(params=...)
+        then evaluates it with those parameters."""
+        if xctype != "NN-AmplitudeEncoding":
+            raise NotImplementedError("the device path implements the amplitude-encoded (global) network")
+        self.network = network
+        return self

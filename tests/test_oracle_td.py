@@ -118,3 +118,54 @@ def test_rks_scf_converges_and_is_variational():
     res = T.scf(mol, method="rks", conv_tol=1e-11)
     assert res.converged
     assert np.all(np.diff(res.energies[-3:]) <= 1e-9)
+
+
+def test_gelu_tanh_and_its_derivative():
+    x = np.linspace(-6, 6, 101)
+    # tanh-approximate GELU against the exact one within the approximation's known error (< 1e-3)
+    from scipy.special import erf
+    exact = 0.5 * x * (1 + erf(x / np.sqrt(2)))
+    assert np.abs(T.gelu_tanh(x) - exact).max() < 1e-3
+    h = 1e-6
+    fd = (T.gelu_tanh(x + h) - T.gelu_tanh(x - h)) / (2 * h)
+    np.testing.assert_allclose(T.gelu_tanh_grad(x), fd, atol=1e-8)
+
+
+def test_global_mlp_vjp_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    ng = 300
+    W, b = T.init_global_mlp(5, ng, hidden=(32, 16))
+    b = [rng.standard_normal(x.shape) * 0.1 for x in b]
+    net = T.GlobalMLP3D(W, b, scale=0.5)
+    rho = rng.uniform(0, 1, ng)
+    e, v = net.exc_vrho(rho)
+    for g in rng.choice(ng, 8, replace=False):
+        d = np.zeros(ng)
+        d[g] = 1e-5
+        fd = (net.exc_vrho(rho + d)[0] - net.exc_vrho(rho - d)[0]) / 2e-5
+        assert abs(fd - v[g]) < 1e-8 * max(1.0, abs(v[g]))
+
+
+def test_nn_amplitude_vmat_is_density_matrix_derivative_with_unit_weights():
+    """With unit grid weights the NN-Amplitude vmat is d excsum / d D; with real weights vrho is
+    scaled by them once more (the reference's convention, td/numint.py:206-231)."""
+    mol = T.synthetic_chain(2, 4)
+    pts, w = T.molecular_grid(mol, n_rad=12, n_theta=4, n_phi=5)
+    ao = T.eval_ao(mol, pts)
+    W, b = T.init_global_mlp(1, len(w), hidden=(16, 8))
+    net = T.GlobalMLP3D(W, b, scale=1.0)
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal((mol.nao, mol.nao)) * 0.2
+    D = c @ c.T
+    ones = np.ones_like(w)
+    _, exc, vmat = T.nr_rks_nn_global(ao, ones, D, net)
+    for (i, j) in ((0, 0), (1, 3), (4, 2)):
+        E = np.zeros_like(D)
+        E[i, j] += 0.5e-6
+        E[j, i] += 0.5e-6
+        fd = (T.nr_rks_nn_global(ao, ones, D + E, net)[1] - T.nr_rks_nn_global(ao, ones, D - E, net)[1]) / 2e-6
+        assert abs(fd - vmat[i, j]) < 1e-7 * max(1.0, abs(vmat[i, j]))
+    _, _, vw = T.nr_rks_nn_global(ao, w, D, net)
+    rho = T.eval_rho(ao, D)
+    _, vrho = net.exc_vrho(rho)
+    np.testing.assert_allclose(vw, ao.T @ (ao * (w * vrho)[:, None]), atol=1e-12)

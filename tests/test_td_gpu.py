@@ -169,3 +169,52 @@ def test_scf_invariants(qtd):
     assert abs(float((D * S).sum()) - mol.nelectron) < 1e-10
     mo = mf.mo_energy.cpu().numpy()
     assert np.all(np.diff(mo) >= -1e-12)
+
+
+def _nets(ng, hidden, seed):
+    from qex_amd.td.xc import GlobalNNXC
+    W, b = otd.init_global_mlp(seed, ng, hidden=hidden)
+    rng = np.random.default_rng(seed + 1)
+    b = [rng.standard_normal(x.shape) * 0.1 for x in b]
+    return GlobalNNXC(W, b, scale=0.05), otd.GlobalMLP3D(W, b, scale=0.05)
+
+
+@pytest.mark.parametrize("hidden", [(64, 32), (512, 512)])
+def test_nr_rks_global_network_matches_oracle(qtd, hidden):
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    grids = qtd.dft.Grids(mol).build()
+    w = grids.weights.cpu().numpy()
+    ao = qtd.numint.eval_ao(mol, grids.coords)
+    oao = ao.cpu().numpy()
+    net, onet = _nets(grids.size, hidden, 3)
+    dm = _random_dm(mol.nao, 5)
+    ni = qtd.numint.NumInt().define_xc_(net)
+    nel, exc, vmat = qtd.numint.nr_rks(ni, mol, grids, "NN", dm)
+    onel, oexc, ovmat = otd.nr_rks_nn_global(oao, w, dm, onet)
+    assert abs(float(nel) - onel) < 1e-10 * abs(onel)
+    assert abs(float(exc) - oexc) < 1e-11 * max(1.0, abs(oexc))
+    np.testing.assert_allclose(vmat.cpu().numpy(), ovmat, rtol=0, atol=1e-10 * np.abs(ovmat).max())
+
+
+def test_rks_global_network_scf_matches_oracle(qtd):
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    mf = qtd.scf.RKS(mol)
+    mf.grids = qtd.dft.Grids(mol).build()
+    net, onet = _nets(mf.grids.size, (64, 32), 9)
+    mf.define_xc_(net)
+    mf.conv_tol = 1e-10
+    e = mf.kernel()
+    S, H, g = mf.integrals()
+    oS, oT, oV = otd.one_electron(omol)
+    grid = (mf.grids.coords.cpu().numpy(), mf.grids.weights.cpu().numpy())
+    ref = otd.scf(omol, "rks_nn", grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
+                  conv_tol=1e-10, max_cycle=50, net=onet)
+    assert ref.converged and mf.converged
+    assert abs(e - ref.e_tot) < 1e-8
+    np.testing.assert_allclose(mf.dm.cpu().numpy(), ref.dm, atol=1e-6)
+    # new parameters through kernel(params=...) (no rebuild of the molecule or grid)
+    W2, b2 = otd.init_global_mlp(10, mf.grids.size, hidden=(64, 32))
+    e2 = mf.kernel(params=list(zip(W2, b2)))
+    ref2 = otd.scf(omol, "rks_nn", grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
+                   conv_tol=1e-10, max_cycle=50, net=otd.GlobalMLP3D(W2, b2, scale=0.05))
+    assert abs(e2 - ref2.e_tot) < 1e-8

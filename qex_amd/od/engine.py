@@ -238,3 +238,80 @@ class Problem1D:
         _lib.check(_lib.load().qex1d_kohn_sham(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs),
                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham")
         return out
+
+
+class BatchRunner:
+    """Batched solver for a stream of systems that share the grid, interaction, XC and loop settings
+    (the serving path).  The interaction table, XC parameters and a workspace sized for
+    ``max_batch`` systems are built once; each ``run`` places b <= max_batch systems in the first
+    b slots and enqueues one ``qex1d_kohn_sham`` on the current stream.  A system's result does not
+    depend on the other systems in its batch (fixed-order reductions per system)."""
+
+    def __init__(self, grids, interaction_fn, xc, num_iterations, max_batch, max_electrons=8, alpha=0.5,
+                 alpha_decay=0.9, num_mixing_iterations=2, density_mse_converge_tolerance=-1.0, loop="scan"):
+        dev = device()
+        check_uniform(grids)
+        self.dev = dev
+        self.grids = as_f64(grids, dev)
+        self.N = self.grids.shape[0]
+        self.T = int(num_iterations)
+        self.max_batch = int(max_batch)
+        self.interaction_fn = interaction_fn
+        kind = getattr(interaction_fn, "kind", None)
+        if kind not in (_lib.QEX_INTERACTION_EXPONENTIAL, _lib.QEX_INTERACTION_SOFT):
+            raise ValueError("the serving path needs a built-in interaction (exponential or soft Coulomb)")
+        self.xc = xcmod.resolve(xc)
+        self.table = interaction_table(self.grids, interaction_fn)
+        p = _lib.Qex1dProblem()
+        p.batch, p.num_grids, p.dx = self.max_batch, self.N, get_dx(self.grids)
+        p.num_iterations = self.T
+        p.max_electrons = int(max_electrons)
+        p.alpha, p.alpha_decay = float(alpha), float(alpha_decay)
+        p.num_mixing_iterations = int(num_mixing_iterations)
+        p.density_mse_converge_tolerance = float(density_mse_converge_tolerance)
+        p.loop = _lib.QEX1D_LOOP_PYTHON if loop == "python" else _lib.QEX1D_LOOP_SCAN
+        p.reflect = 0
+        self.xc.configure(p)
+        self.problem = p
+        self.xc_params = self.xc.device_params(dev)
+        lib = _lib.load()
+        nbytes = lib.qex1d_workspace_bytes(ctypes.byref(p))
+        if nbytes == 0:
+            _lib.check(-1, "qex1d_workspace_bytes")
+        self.ws = workspace(nbytes, dev)
+
+    def run(self, locations, nuclear_charges, num_electrons, initial_density) -> BatchResult:
+        """locations / nuclear_charges: [b, A] (zero charges pad), num_electrons [b], initial_density [b, N]."""
+        loc = np.atleast_2d(np.asarray(locations, dtype=np.float64))
+        z = np.atleast_2d(np.asarray(nuclear_charges, dtype=np.float64))
+        ne = np.asarray(num_electrons, dtype=np.int32).reshape(-1)
+        b = ne.shape[0]
+        if not 1 <= b <= self.max_batch or loc.shape != z.shape or loc.shape[0] != b:
+            raise ValueError("bad batch shapes")
+        if np.any(ne < 1) or np.any(ne > self.problem.max_electrons):
+            raise ValueError(f"num_electrons must be in [1, {self.problem.max_electrons}]")
+        dev, N, T = self.dev, self.N, self.T
+        loc_d, z_d, ne_d = as_f64(loc, dev), as_f64(z, dev), as_i32(ne, dev)
+        n0 = as_f64(initial_density, dev).reshape(b, N).contiguous()
+        lib = _lib.load()
+        vext = torch.empty((b, N), dtype=torch.float64, device=dev)
+        p0, p1 = self.interaction_fn.params
+        _lib.check(lib.qex1d_external_potential(b, N, _ptr(self.grids), loc.shape[1], _ptr(loc_d), _ptr(z_d),
+                                                self.interaction_fn.kind, p0, p1, _ptr(vext), _stream()),
+                   "qex1d_external_potential")
+        p = _lib.Qex1dProblem.from_buffer_copy(self.problem)
+        p.batch = b
+        f = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
+        out = BatchResult(density=f(b, T, N), total_energy=f(b, T), hartree_potential=f(b, T, N),
+                          xc_potential=f(b, T, N),
+                          xc_energy_density=f(b, T, N) if self.xc.encoding == "local" else f(b, T), gap=f(b, T),
+                          converged=torch.empty((b, T), dtype=torch.uint8, device=dev), external_potential=vext,
+                          stats=torch.zeros(2, dtype=torch.int64, device=dev))
+        ins = _lib.Qex1dInputs(_ptr(self.table), _ptr(vext), _ptr(n0), _ptr(ne_d), None, _ptr(self.xc_params))
+        outs = _lib.Qex1dOutputs(_ptr(out.density), _ptr(out.total_energy), _ptr(out.hartree_potential),
+                                 _ptr(out.xc_potential), _ptr(out.xc_energy_density), _ptr(out.gap),
+                                 _ptr(out.converged), None, _ptr(out.stats))
+        _lib.check(lib.qex1d_kohn_sham(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), _ptr(self.ws),
+                                       self.ws.numel(), _stream()), "qex1d_kohn_sham")
+        out._keep = (loc_d, z_d, ne_d, n0)  # inputs stay alive until the stream has consumed them
+        return out

@@ -1,0 +1,220 @@
+"""Serving loop for a mixed stream of 1D and 3D Kohn-Sham requests on one GPU.
+
+Requests arrive over time.  Each 1D request is a system on the shared grid with the shared XC
+functional and loop settings.  Each 3D request is a molecule (``qex_amd.td.gto.Mole``) solved by
+RKS-LDA.
+
+The dispatcher runs in one host thread and never blocks on the device:
+  * 1D: every released request still waiting is packed into the next batched launch (up to
+    ``max_batch``), one ``qex1d_kohn_sham`` per batch, on stream A, while the previous batch may
+    still be running (at most ``depth_1d`` batches in flight);
+  * 3D: each request is one enqueued solve (integrals, grid, AO values, SCF cycles without host
+    synchronisation) on stream B, with at most ``depth_3d`` in flight;
+  * completions are detected with events (``query``), then results are read back.
+Admission holds a request back while the device memory its solve needs, by the estimate
+n_grid * n_ao * 8 + n_ao^4 / 4 * 8 + 16 * n_ao^2 * 8 bytes (3D) or T * N * 8 * 6 bytes (1D),
+would exceed ``memory_budget`` bytes in flight.
+
+Latency is completion time minus arrival time on the host clock.
+"""
+
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+
+@dataclass
+class Request:
+    kind: str                 # "1d" or "3d"
+    arrival: float            # seconds after the start of the run
+    payload: object           # 1D: (locations, charges, num_electrons, initial_density); 3D: Mole
+    rid: int = 0
+
+
+@dataclass
+class Completion:
+    rid: int
+    kind: str
+    arrival: float
+    done: float
+    energy: float
+    converged: bool
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+@dataclass
+class _Inflight:
+    event: torch.cuda.Event
+    reqs: list
+    handle: object
+    nbytes: int
+
+
+@dataclass
+class ServeStats:
+    completions: list = field(default_factory=list)
+    wall: float = 0.0
+    batches_1d: int = 0
+    batch_sizes: list = field(default_factory=list)
+
+    def summary(self):
+        lat = np.array([c.latency for c in self.completions]) * 1e3
+        out = {"requests": len(self.completions), "wall_s": self.wall,
+               "throughput_per_s": len(self.completions) / self.wall if self.wall > 0 else 0.0,
+               "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+               "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+               "batches_1d": self.batches_1d,
+               "mean_batch_1d": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0}
+        for kind in ("1d", "3d"):
+            lk = np.array([c.latency for c in self.completions if c.kind == kind]) * 1e3
+            if len(lk):
+                out[f"p50_ms_{kind}"] = float(np.percentile(lk, 50))
+                out[f"p99_ms_{kind}"] = float(np.percentile(lk, 99))
+        return out
+
+
+def _bytes_3d(mol, n_grid):
+    n = mol.nao
+    return int(n_grid * n * 8 + (n * (n + 1) // 2) ** 2 * 8 + 16 * n * n * 8)
+
+
+class Server:
+    """runner_1d: an ``engine.BatchRunner``; solver_3d(mol) -> SCF object with ``kernel_async`` /
+    ``collect`` (e.g. ``lambda m: qex_amd.td.scf.RKS(m)`` with its settings)."""
+
+    def __init__(self, runner_1d, solver_3d, max_batch=64, depth_1d=2, depth_3d=2, memory_budget=64 << 30,
+                 grid_points_per_atom=5500):
+        self.runner = runner_1d
+        self.solver_3d = solver_3d
+        self.max_batch = min(max_batch, runner_1d.max_batch) if runner_1d is not None else max_batch
+        self.depth_1d, self.depth_3d = depth_1d, depth_3d
+        self.memory_budget = memory_budget
+        self.grid_points_per_atom = grid_points_per_atom
+        self.stream_1d = torch.cuda.Stream()
+        self.stream_3d = torch.cuda.Stream()
+
+    def _dispatch_1d(self, reqs):
+        loc = np.zeros((len(reqs), max(len(r.payload[0]) for r in reqs)))
+        z = np.zeros_like(loc)
+        ne = np.zeros(len(reqs), dtype=np.int32)
+        n0 = np.stack([r.payload[3] for r in reqs])
+        for i, r in enumerate(reqs):
+            loc[i, :len(r.payload[0])] = r.payload[0]
+            z[i, :len(r.payload[1])] = r.payload[1]
+            ne[i] = r.payload[2]
+        with torch.cuda.stream(self.stream_1d):
+            out = self.runner.run(loc, z, ne, n0)
+            ev = torch.cuda.Event()
+            ev.record(self.stream_1d)
+        return out, ev
+
+    def _dispatch_3d(self, req):
+        with torch.cuda.stream(self.stream_3d):
+            mf = self.solver_3d(req.payload)
+            mf.kernel_async()
+            ev = torch.cuda.Event()
+            ev.record(self.stream_3d)
+        return mf, ev
+
+    def run(self, requests, poll_sleep=50e-6) -> ServeStats:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        pend_1d, pend_3d = deque(), deque()
+        inflight_1d, inflight_3d = deque(), deque()
+        stats = ServeStats()
+        i = 0
+        used = 0
+        t0 = time.perf_counter()
+        total = len(reqs)
+        while len(stats.completions) < total:
+            now = time.perf_counter() - t0
+            while i < total and reqs[i].arrival <= now:
+                (pend_1d if reqs[i].kind == "1d" else pend_3d).append(reqs[i])
+                i += 1
+            progressed = False
+            # completions (in order per stream)
+            for q, kind in ((inflight_1d, "1d"), (inflight_3d, "3d")):
+                while q and q[0].event.query():
+                    f = q.popleft()
+                    done = time.perf_counter() - t0
+                    used -= f.nbytes
+                    if kind == "1d":
+                        e = f.handle.total_energy[:, -1].cpu().numpy()
+                        conv = f.handle.converged[:, -1].cpu().numpy()
+                        for k, r in enumerate(f.reqs):
+                            stats.completions.append(Completion(r.rid, "1d", r.arrival, done, float(e[k]),
+                                                                bool(conv[k])))
+                    else:
+                        e = f.handle.collect()
+                        r = f.reqs[0]
+                        stats.completions.append(Completion(r.rid, "3d", r.arrival, done, e, f.handle.converged))
+                    progressed = True
+            # dispatch
+            if pend_1d and len(inflight_1d) < self.depth_1d:
+                batch = []
+                nb = 0
+                while pend_1d and len(batch) < self.max_batch:
+                    need = self.runner.T * self.runner.N * 8 * 6
+                    if used + nb + need > self.memory_budget and batch:
+                        break
+                    batch.append(pend_1d.popleft())
+                    nb += need
+                out, ev = self._dispatch_1d(batch)
+                used += nb
+                inflight_1d.append(_Inflight(ev, batch, out, nb))
+                stats.batches_1d += 1
+                stats.batch_sizes.append(len(batch))
+                progressed = True
+            if pend_3d and len(inflight_3d) < self.depth_3d:
+                r = pend_3d[0]
+                need = _bytes_3d(r.payload, r.payload.natm * self.grid_points_per_atom)
+                if used + need <= self.memory_budget or not (inflight_1d or inflight_3d):
+                    pend_3d.popleft()
+                    mf, ev = self._dispatch_3d(r)
+                    used += need
+                    inflight_3d.append(_Inflight(ev, [r], mf, need))
+                    progressed = True
+            if not progressed:
+                if i < total and not (pend_1d or pend_3d or inflight_1d or inflight_3d):
+                    wait = reqs[i].arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 1e-3))
+                else:
+                    time.sleep(poll_sleep)
+        torch.cuda.synchronize()
+        stats.wall = time.perf_counter() - t0
+        return stats
+
+
+def run_one_at_a_time(requests, runner_1d, solver_3d) -> ServeStats:
+    """The plain path: every request solved alone (1D batch of one, one 3D solve), one after the
+    other, each waited for before the next starts."""
+    stats = ServeStats()
+    t0 = time.perf_counter()
+    for r in requests:
+        if r.kind == "1d":
+            loc, z, ne, n0 = r.payload
+            out = runner_1d.run(np.asarray(loc)[None], np.asarray(z)[None], [ne], np.asarray(n0)[None])
+            e = float(out.total_energy[0, -1].item())
+            conv = bool(out.converged[0, -1].item())
+        else:
+            mf = solver_3d(r.payload)
+            e = mf.kernel()
+            conv = mf.converged
+        done = time.perf_counter() - t0
+        stats.completions.append(Completion(r.rid, r.kind, 0.0, done, e, conv))
+    stats.wall = time.perf_counter() - t0
+    return stats
+
+
+def poisson_arrivals(rate, count, seed):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, count))
+

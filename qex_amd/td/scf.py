@@ -39,6 +39,14 @@ class _SCF:
         return self._ints
 
     def kernel(self, dm0=None, **kwargs):
+        """Runs the SCF on the device and returns e_tot (waits for the result)."""
+        self.kernel_async(**kwargs)
+        return self.collect()
+
+    def kernel_async(self, **kwargs):
+        """Enqueues the whole solve (integrals if not cached, grid, AO values, SCF cycles) on the
+        current stream and returns at once; ``collect()`` reads the result.  No host
+        synchronisation is made unless ``sync_every`` > 0."""
         mol = self.mol
         dev = engine.device()
         S, H, eri = self.integrals()
@@ -76,12 +84,18 @@ class _SCF:
         outs = _lib.Qex3dScfOutputs(ptr(e_hist), ptr(mo), ptr(C), ptr(D), ptr(status))
         _lib.check3(lib.qex3d_scf(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), ptr(ws), ws.numel(),
                                   engine._stream()), "qex3d_scf")
+        self._pending = (status, ws)
+        self.energies = e_hist
+        self.mo_energy, self.mo_coeff, self.dm = mo, C, D
+        return self
+
+    def collect(self):
+        status, _ = self._pending
         st = status.cpu().numpy()
         self.cycles = int(st[1])
         self.converged = bool(st[4])
-        self.energies = e_hist
-        self.e_tot = float(e_hist[-1].item())
-        self.mo_energy, self.mo_coeff, self.dm = mo, C, D
+        self.e_tot = float(self.energies[-1].item())
+        self._pending = None
         return self.e_tot
 
     def make_rdm1(self, mo_coeff=None, mo_occ=None):

@@ -103,8 +103,8 @@ def test_eval_rho_nonsymmetric_dm_is_symmetrised(qtd):
     np.testing.assert_allclose(rho, orho, rtol=1e-10, atol=1e-12 * np.abs(orho).max())
 
 
-@pytest.mark.parametrize("r,energy", [(0.74, -1.12675531719693), (1.5, -1.05802481296927),
-                                      (2.0, -0.997497294328357)])
+@pytest.mark.parametrize("r,energy", [(0.74, -1.12675531719693), (0.5, -1.05802481296927),
+                                      (1.5, -0.997497294328357)])
 def test_h2_631g_rhf_notebook_energies(qtd, r, energy):
     mol, _ = _mol_pair(qtd, "h2", r)
     mf = qtd.scf.RHF(mol)

@@ -157,6 +157,7 @@ int32_t qex_abi_version(void);
  * ========================================================================================== */
 #define QEX3D_MAX_NAO 96
 #define QEX3D_MAX_DIIS 12
+#define QEX3D_MAX_NPRIM 8
 #define QEX3D_RKS_LDA 0 /* Slater exchange + VWN5 correlation */
 #define QEX3D_RHF 1
 #define QEX3D_RKS_NN_GLOBAL 2 /* amplitude-encoded network on the whole grid density */
@@ -176,6 +177,7 @@ typedef struct qex3d_basis {
   const double* coefficients;
   const int32_t* ao_shell; /* [nao] */
   const int32_t* ao_lmn;   /* [nao] lx | ly << 2 | lz << 4 */
+  int32_t max_nprim;       /* >= every shell_nprim, <= QEX3D_MAX_NPRIM */
 } qex3d_basis;
 
 /* S, T, V (nuclear attraction) [nao][nao].  Replaces pyscf int1e_ovlp/kin/nuc (libcint), called at
@@ -183,8 +185,10 @@ typedef struct qex3d_basis {
 int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* stream);
 
 /* ERIs (ij|kl) as the [npair][npair] matrix over i>=j, k>=l pairs (npair = nao(nao+1)/2, pair index
- * i(i+1)/2 + j).  Replaces pyscf int2e (libcint) used by get_jk, qedft/train/td/hf_legacy.py:452-471. */
-int qex3d_eri(const qex3d_basis* b, double* eri, void* stream);
+ * i(i+1)/2 + j).  The workspace holds primitive-pair records (npair * max_nprim^2 * 128 bytes).
+ * Replaces pyscf int2e (libcint) used by get_jk, qedft/train/td/hf_legacy.py:452-471. */
+size_t qex3d_eri_workspace_bytes(const qex3d_basis* b);
+int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Atom-centred grid: natm * nrad * nang points from radial (r, w r^2) and angular (unit vectors,
  * weights summing to 4 pi) base tables, Becke partition.  Replaces pyscf dft.Grids.build
@@ -217,6 +221,15 @@ size_t qex3d_nr_rks_nn_workspace_bytes(int64_t ngrid, int32_t nao, int32_t h1, i
 int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
                     const double* nn_params, int32_t h1, int32_t h2, double scale, double* nelec_excsum,
                     double* vmat, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Batched generalized symmetric eigenproblem F C = S C diag(e), S positive definite, n <= 96:
+ * e ascending, C^T S C = 1 (columns are eigenvectors).  Cholesky S = L L^T, X = L^-T, then the
+ * symmetric eigenproblem of X^T F X in one workgroup (Householder tridiagonalisation, Sturm
+ * multisection, inverse iteration, back-transformation).  Replaces generalized_eigh
+ * (qedft/train/td/generalized_eigensolver.py:264-330; pyscfad scipy.linalg.eigh at td/hf.py:490-494). */
+size_t qex3d_gen_eigh_workspace_bytes(int32_t batch, int32_t n);
+int qex3d_gen_eigh(int32_t batch, int32_t n, const double* F, const double* S, double* mo_energy, double* mo_coeff,
+                   void* workspace, size_t workspace_bytes, void* stream);
 
 typedef struct qex3d_scf_problem {
   int32_t nao, nelectron, method, max_cycle, diis_space;

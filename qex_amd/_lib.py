@@ -86,12 +86,14 @@ QEX3D_RHF = 1
 QEX3D_RKS_NN_GLOBAL = 2
 QEX3D_MAX_NN_HIDDEN = 1024
 QEX3D_MAX_NAO = 96
+QEX3D_MAX_NPRIM = 8
 
 
 class Qex3dBasis(ctypes.Structure):
     _fields_ = [("natm", ctypes.c_int32), ("nshell", ctypes.c_int32), ("nao", ctypes.c_int32)] + [
         (name, ctypes.c_void_p) for name in ("atom_coords", "atom_charges", "shell_atom", "shell_l", "shell_nprim",
-                                             "shell_offset", "exponents", "coefficients", "ao_shell", "ao_lmn")]
+                                             "shell_offset", "exponents", "coefficients", "ao_shell", "ao_lmn")] + [
+        ("max_nprim", ctypes.c_int32)]
 
 
 class Qex3dScfProblem(ctypes.Structure):
@@ -128,7 +130,8 @@ SIGNATURES = {
     "qex_profile_read": (_I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), _I]),
     "qex3d_last_error": (ctypes.c_char_p, []),
     "qex3d_int1e": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _P, _P]),
-    "qex3d_eri": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P]),
+    "qex3d_eri_workspace_bytes": (_S, [ctypes.POINTER(Qex3dBasis)]),
+    "qex3d_eri": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _S, _P]),
     "qex3d_grid": (_I, [ctypes.POINTER(Qex3dBasis), _I, _P, _P, _I, _P, _P, _P, _P, _P]),
     "qex3d_eval_ao": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
     "qex3d_eval_rho": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P]),
@@ -136,6 +139,8 @@ SIGNATURES = {
     "qex3d_nr_rks": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),
     "qex3d_nr_rks_nn_workspace_bytes": (_S, [ctypes.c_int64, _I, _I, _I]),
     "qex3d_nr_rks_nn": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S, _P]),
+    "qex3d_gen_eigh_workspace_bytes": (_S, [_I, _I]),
+    "qex3d_gen_eigh": (_I, [_I, _I, _P, _P, _P, _P, _P, _S, _P]),
     "qex3d_scf_workspace_bytes": (_S, [ctypes.POINTER(Qex3dScfProblem)]),
     "qex3d_scf": (_I, [ctypes.POINTER(Qex3dScfProblem), ctypes.POINTER(Qex3dScfInputs),
                        ctypes.POINTER(Qex3dScfOutputs), _P, _S, _P]),

@@ -22,6 +22,7 @@
 
 #include "qex.h"
 #include "prof.h"
+#include "symeig.h"
 
 namespace {
 
@@ -65,26 +66,103 @@ __device__ double bsum(double x, double* red) {
 __device__ __forceinline__ int pair_index(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
 // ---------------------------------------------------------------------------------------------
-// Boys function F_m(T), m = 0..mmax (<= 8): series for T < 30 (then downward recursion),
-// asymptotic + upward recursion above.
+// Boys function F_m(T) = int_0^1 t^2m exp(-T t^2) dt for m = 0..4 (s/p quartets need L <= 4).
+// Below BOYS_TMAX: 7-term Taylor expansion of F_4 about the nearest tabulated point (spacing
+// 0.05, |dT| <= 0.025, truncation error < 2e-15 relative), then downward recursion
+// F_m = (2T F_m+1 + exp(-T)) / (2m+1).  Above: F_0 = sqrt(pi/T)/2 and upward recursion.  The
+// table holds F_0..F_11 at T_i = 0.05 i, made on the device once (series, then downward recursion).
 // ---------------------------------------------------------------------------------------------
-__device__ void boys_all(int mmax, double T, double* F) {
-  if (T < 30.0) {
-    // F_mmax by the series exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
-    double term = 1.0 / (2 * mmax + 1), sum = term;
-    for (int k = 1; k < 200; ++k) {
-      term *= 2.0 * T / (2 * mmax + 2 * k + 1);
-      sum += term;
-      if (term < 1e-17 * sum) break;
-    }
-    const double et = exp(-T);
-    F[mmax] = et * sum;
-    for (int m = mmax - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + et) / (2 * m + 1);
-  } else {
-    const double et = exp(-T);
-    F[0] = 0.5 * sqrt(M_PI / T);  // erf(sqrt(T)) = 1 to double precision for T >= 30
-    for (int m = 0; m < mmax; ++m) F[m + 1] = ((2 * m + 1) * F[m] - et) / (2.0 * T);
+constexpr int BOYS_NM = 12;
+constexpr double BOYS_H = 0.05;
+constexpr int BOYS_NT = 801;  // T_i up to 40
+constexpr double BOYS_TMAX = 39.9;
+__device__ double g_boys_tab[BOYS_NT * BOYS_NM];
+
+__global__ void k_boys_init() {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= BOYS_NT) return;
+  const double T = BOYS_H * i;
+  const int mm = BOYS_NM - 1;
+  double term = 1.0 / (2 * mm + 1), sum = term;
+  for (int k = 1; k < 400; ++k) {
+    term *= 2.0 * T / (2 * mm + 2 * k + 1);
+    sum += term;
+    if (term < 1e-18 * sum) break;
   }
+  const double et = exp(-T);
+  double f = et * sum;
+  g_boys_tab[i * BOYS_NM + mm] = f;
+  for (int m = mm - 1; m >= 0; --m) {
+    f = (2.0 * T * f + et) / (2 * m + 1);
+    g_boys_tab[i * BOYS_NM + m] = f;
+  }
+}
+
+struct Boys5 {
+  double f0, f1, f2, f3, f4;
+  __device__ __forceinline__ double operator[](int N) const {
+    return N == 0 ? f0 : N == 1 ? f1 : N == 2 ? f2 : N == 3 ? f3 : f4;
+  }
+};
+
+__device__ __forceinline__ Boys5 boys5(double T) {
+  Boys5 F;
+  const double et = exp(-T);
+  if (T < BOYS_TMAX) {
+    const int i = (int)(T * (1.0 / BOYS_H) + 0.5);
+    const double dT = T - BOYS_H * i;
+    const double* r = g_boys_tab + i * BOYS_NM;
+    // F_4(T) = sum_k F_{4+k}(T_i) (-dT)^k / k!
+    double f = r[10];
+    f = fma(f, -dT / 6.0, r[9]);
+    f = fma(f, -dT / 5.0, r[8]);
+    f = fma(f, -dT / 4.0, r[7]);
+    f = fma(f, -dT / 3.0, r[6]);
+    f = fma(f, -dT / 2.0, r[5]);
+    f = fma(f, -dT, r[4]);
+    F.f4 = f;
+    F.f3 = (2.0 * T * F.f4 + et) / 7.0;
+    F.f2 = (2.0 * T * F.f3 + et) / 5.0;
+    F.f1 = (2.0 * T * F.f2 + et) / 3.0;
+    F.f0 = 2.0 * T * F.f1 + et;
+  } else {
+    const double i2t = 0.5 / T;
+    F.f0 = 0.5 * sqrt(M_PI / T);
+    F.f1 = (F.f0 - et) * i2t;
+    F.f2 = (3.0 * F.f1 - et) * i2t;
+    F.f3 = (5.0 * F.f2 - et) * i2t;
+    F.f4 = (7.0 * F.f3 - et) * i2t;
+  }
+  return F;
+}
+
+// Hermite Coulomb integral R_tuv (t+u+v <= 4) in closed form:
+//   R_tuv = sum_{i,j,k} c(t,i) c(u,j) c(v,k) (2aX)^(t-2i) (2aY)^(u-2j) (2aZ)^(v-2k) a^(i+j+k) (-1)^N F_N,
+//   N = t+u+v-i-j-k, c(t,i) = t! / (i! (t-2i)!)  (the t-th X-derivative of F_0(a (X^2+Y^2+Z^2)))
+__device__ __forceinline__ double hcoef(int t, int i) {
+  // t <= 4: c(2,1) = 2, c(3,1) = 6, c(4,1) = 12, c(4,2) = 12, else 1
+  return i == 0 ? 1.0 : (t == 2 ? 2.0 : t == 3 ? 6.0 : 12.0);
+}
+__device__ __forceinline__ double ipow(double x, int n) {
+  double r = 1.0;
+  for (int k = 0; k < n; ++k) r *= x;
+  return r;
+}
+__device__ double rtuv(int t, int u, int v, double a, double X, double Y, double Z, const Boys5& F) {
+  const double ax = 2.0 * a * X, ay = 2.0 * a * Y, az = 2.0 * a * Z;
+  double r = 0.0;
+  for (int i = 0; 2 * i <= t; ++i) {
+    const double ci = hcoef(t, i) * ipow(ax, t - 2 * i);
+    for (int j = 0; 2 * j <= u; ++j) {
+      const double cj = ci * hcoef(u, j) * ipow(ay, u - 2 * j);
+      for (int k = 0; 2 * k <= v; ++k) {
+        const int N = t + u + v - i - j - k;
+        const double c = cj * hcoef(v, k) * ipow(az, v - 2 * k) * ipow(a, i + j + k);
+        r += (N & 1) ? -c * F[N] : c * F[N];
+      }
+    }
+  }
+  return r;
 }
 
 // Hermite expansion coefficients E_t for one Cartesian direction, l_a, l_b <= 1:
@@ -107,43 +185,6 @@ __device__ __forceinline__ void hermite_E1(int la, int lb, double K, double xpa,
     E[1] = (xpa + xpb) * i2p * K;
     E[2] = i2p * i2p * K;
   }
-}
-
-// R_{tuv} (n = 0) for t <= tx, u <= ty, v <= tz, t+u+v <= L <= 4, stored R[t][u][v].
-__device__ void hermite_R_all(int tx, int ty, int tz, double alpha, double X, double Y, double Z, double (&R)[5][5][5]) {
-  const int L = tx + ty + tz;
-  double F[9];
-  boys_all(L, alpha * (X * X + Y * Y + Z * Z), F);
-  // levels n = L .. 0; Rn[t][u][v] for t+u+v <= L-n, built from level n+1
-  double Rn[5][5][5];
-  double Rp[5][5][5];
-  double fac = 1.0;
-  for (int n = 0; n < L; ++n) fac *= -2.0 * alpha;
-  for (int n = L; n >= 0; --n) {
-    const int lev = L - n;
-    for (int t = 0; t <= min(tx, lev); ++t)
-      for (int u = 0; u <= min(ty, lev - t); ++u)
-        for (int v = 0; v <= min(tz, lev - t - u); ++v) {
-          double val;
-          if (t == 0 && u == 0 && v == 0) {
-            val = fac * F[n];
-          } else if (t > 0) {
-            val = X * Rp[t - 1][u][v] + (t > 1 ? (t - 1) * Rp[t - 2][u][v] : 0.0);
-          } else if (u > 0) {
-            val = Y * Rp[t][u - 1][v] + (u > 1 ? (u - 1) * Rp[t][u - 2][v] : 0.0);
-          } else {
-            val = Z * Rp[t][u][v - 1] + (v > 1 ? (v - 1) * Rp[t][u][v - 2] : 0.0);
-          }
-          Rn[t][u][v] = val;
-        }
-    for (int t = 0; t <= min(tx, lev); ++t)
-      for (int u = 0; u <= min(ty, lev - t); ++u)
-        for (int v = 0; v <= min(tz, lev - t - u); ++v) Rp[t][u][v] = Rn[t][u][v];
-    fac /= -2.0 * alpha;
-  }
-  for (int t = 0; t <= tx; ++t)
-    for (int u = 0; u <= ty; ++u)
-      for (int v = 0; v <= tz; ++v) R[t][u][v] = (t + u + v <= L) ? Rp[t][u][v] : 0.0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -172,7 +213,7 @@ __device__ __forceinline__ void ao_info(const Basis& b, int mu, int* sh, int* lx
 }
 
 // one-electron integrals, one thread per (i >= j)
-__global__ void k_int1e(Basis b, double* S, double* Tm, double* V) {
+__global__ __launch_bounds__(128) void k_int1e(Basis b, double* S, double* Tm, double* V) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = b.nao;
   if (idx >= n * (n + 1) / 2) return;
@@ -235,12 +276,15 @@ __global__ void k_int1e(Basis b, double* S, double* Tm, double* V) {
       const int tx = la[0] + lb[0], ty = la[1] + lb[1], tz = la[2] + lb[2];
       for (int c = 0; c < b.natm; ++c) {
         const double* C = b.xyz + 3 * c;
-        double R[5][5][5];
-        hermite_R_all(tx, ty, tz, p, P[0] - C[0], P[1] - C[1], P[2] - C[2], R);
+        const double X = P[0] - C[0], Y = P[1] - C[1], Z = P[2] - C[2];
+        const Boys5 F = boys5(p * (X * X + Y * Y + Z * Z));
         double acc = 0.0;
         for (int t0 = 0; t0 <= tx; ++t0)
           for (int u0 = 0; u0 <= ty; ++u0)
-            for (int v0 = 0; v0 <= tz; ++v0) acc += Ex[t0] * Ey[u0] * Ez[v0] * R[t0][u0][v0];
+            for (int v0 = 0; v0 <= tz; ++v0) {
+              const double e = Ex[t0] * Ey[u0] * Ez[v0];
+              if (e != 0.0) acc += e * rtuv(t0, u0, v0, p, X, Y, Z, F);
+            }
         v -= cc * b.charge[c] * 2.0 * M_PI / p * acc;
       }
     }
@@ -250,8 +294,60 @@ __global__ void k_int1e(Basis b, double* S, double* Tm, double* V) {
   V[i * n + j] = V[j * n + i] = v;
 }
 
-// ERIs (ij|kl) for pair indices P = (i >= j), Q = (k >= l), Q <= P; written to both [P][Q] and [Q][P].
-__global__ void k_eri(Basis b, double* eri, int npair) {
+// ERIs from primitive-pair records.  Record (AO pair P = (i >= j), primitive pair r) holds
+//   p, P_x, P_y, P_z, c_a c_b, n_terms, then up to 4 Hermite terms (packed t | u << 4 | v << 8, E_tuv)
+// with E_tuv = E^x_t E^y_u E^z_v (the Gaussian overlap factor exp(-mu |A-B|^2) included).
+constexpr int PREC = 16;  // doubles per record
+constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
+
+__global__ void k_eri_pairs(Basis b, int npair, int maxp, double* rec) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int pp = maxp * maxp;
+  if (idx >= (long)npair * pp) return;
+  const int P = (int)(idx / pp), r = (int)(idx % pp);
+  int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > P) --i;
+  while ((i + 1) * (i + 2) / 2 <= P) ++i;
+  const int j = P - i * (i + 1) / 2;
+  int si, sj, la[3], lb[3];
+  ao_info(b, i, &si, &la[0], &la[1], &la[2]);
+  ao_info(b, j, &sj, &lb[0], &lb[1], &lb[2]);
+  double* o = rec + idx * PREC;
+  const int pa = r / maxp, pb = r % maxp;
+  if (pa >= b.sh_nprim[si] || pb >= b.sh_nprim[sj]) {
+    o[5] = 0.0;  // no terms: record unused
+    return;
+  }
+  const double* A = b.xyz + 3 * b.sh_atom[si];
+  const double* B = b.xyz + 3 * b.sh_atom[sj];
+  const double ea = b.exps[b.sh_off[si] + pa], eb = b.exps[b.sh_off[sj] + pb];
+  const double p = ea + eb, mu = ea * eb / p, i2p = 0.5 / p;
+  double Pp[3], E[3][3];
+  for (int d = 0; d < 3; ++d) {
+    Pp[d] = (ea * A[d] + eb * B[d]) / p;
+    hermite_E1(la[d], lb[d], exp(-mu * (A[d] - B[d]) * (A[d] - B[d])), Pp[d] - A[d], Pp[d] - B[d], i2p, E[d]);
+  }
+  o[0] = p;
+  o[1] = Pp[0];
+  o[2] = Pp[1];
+  o[3] = Pp[2];
+  o[4] = b.coefs[b.sh_off[si] + pa] * b.coefs[b.sh_off[sj] + pb];
+  int nt = 0;
+  for (int t = 0; t <= la[0] + lb[0]; ++t)
+    for (int u = 0; u <= la[1] + lb[1]; ++u)
+      for (int v = 0; v <= la[2] + lb[2]; ++v) {
+        const double e = E[0][t] * E[1][u] * E[2][v];
+        if (e != 0.0 && nt < 4) {
+          o[6 + 2 * nt] = (double)(t | (u << 4) | (v << 8));
+          o[7 + 2 * nt] = e;
+          ++nt;
+        }
+      }
+  o[5] = (double)nt;
+}
+
+// (ij|kl) for pair indices P >= Q; written to both [P][Q] and [Q][P]
+__global__ __launch_bounds__(128) void k_eri(int npair, int maxp, const double* rec, double* eri) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long total = (long)npair * (npair + 1) / 2;
   if (idx >= total) return;
@@ -259,68 +355,37 @@ __global__ void k_eri(Basis b, double* eri, int npair) {
   while (P * (P + 1) / 2 > idx) --P;
   while ((P + 1) * (P + 2) / 2 <= idx) ++P;
   const long Q = idx - P * (P + 1) / 2;
-  auto unpair = [](long q, int* i, int* j) {
-    int ii = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
-    while ((long)ii * (ii + 1) / 2 > q) --ii;
-    while ((long)(ii + 1) * (ii + 2) / 2 <= q) ++ii;
-    *i = ii;
-    *j = (int)(q - (long)ii * (ii + 1) / 2);
-  };
-  int i, j, k, l;
-  unpair(P, &i, &j);
-  unpair(Q, &k, &l);
-  int si, sj, sk, sl, la[3], lb[3], lc[3], ld[3];
-  ao_info(b, i, &si, &la[0], &la[1], &la[2]);
-  ao_info(b, j, &sj, &lb[0], &lb[1], &lb[2]);
-  ao_info(b, k, &sk, &lc[0], &lc[1], &lc[2]);
-  ao_info(b, l, &sl, &ld[0], &ld[1], &ld[2]);
-  const double* A = b.xyz + 3 * b.sh_atom[si];
-  const double* B = b.xyz + 3 * b.sh_atom[sj];
-  const double* Cc = b.xyz + 3 * b.sh_atom[sk];
-  const double* D = b.xyz + 3 * b.sh_atom[sl];
-  const int tx1 = la[0] + lb[0], ty1 = la[1] + lb[1], tz1 = la[2] + lb[2];
-  const int tx2 = lc[0] + ld[0], ty2 = lc[1] + ld[1], tz2 = lc[2] + ld[2];
+  const int pp = maxp * maxp;
+  const double* r1 = rec + P * pp * PREC;
+  const double* r2 = rec + Q * pp * PREC;
   double val = 0.0;
-  for (int pa = 0; pa < b.sh_nprim[si]; ++pa) {
-    const double ea = b.exps[b.sh_off[si] + pa], ca = b.coefs[b.sh_off[si] + pa];
-    for (int pb = 0; pb < b.sh_nprim[sj]; ++pb) {
-      const double eb = b.exps[b.sh_off[sj] + pb], cb = b.coefs[b.sh_off[sj] + pb];
-      const double p = ea + eb, mup = ea * eb / p, i2p = 0.5 / p;
-      double Pp[3], E1x[3], E1y[3], E1z[3];
-      for (int d = 0; d < 3; ++d) Pp[d] = (ea * A[d] + eb * B[d]) / p;
-      hermite_E1(la[0], lb[0], exp(-mup * (A[0] - B[0]) * (A[0] - B[0])), Pp[0] - A[0], Pp[0] - B[0], i2p, E1x);
-      hermite_E1(la[1], lb[1], exp(-mup * (A[1] - B[1]) * (A[1] - B[1])), Pp[1] - A[1], Pp[1] - B[1], i2p, E1y);
-      hermite_E1(la[2], lb[2], exp(-mup * (A[2] - B[2]) * (A[2] - B[2])), Pp[2] - A[2], Pp[2] - B[2], i2p, E1z);
-      for (int pc = 0; pc < b.sh_nprim[sk]; ++pc) {
-        const double ec = b.exps[b.sh_off[sk] + pc], cc = b.coefs[b.sh_off[sk] + pc];
-        for (int pd = 0; pd < b.sh_nprim[sl]; ++pd) {
-          const double ed = b.exps[b.sh_off[sl] + pd], cd = b.coefs[b.sh_off[sl] + pd];
-          const double q = ec + ed, muq = ec * ed / q, i2q = 0.5 / q;
-          double Qq[3], E2x[3], E2y[3], E2z[3];
-          for (int d = 0; d < 3; ++d) Qq[d] = (ec * Cc[d] + ed * D[d]) / q;
-          hermite_E1(lc[0], ld[0], exp(-muq * (Cc[0] - D[0]) * (Cc[0] - D[0])), Qq[0] - Cc[0], Qq[0] - D[0], i2q, E2x);
-          hermite_E1(lc[1], ld[1], exp(-muq * (Cc[1] - D[1]) * (Cc[1] - D[1])), Qq[1] - Cc[1], Qq[1] - D[1], i2q, E2y);
-          hermite_E1(lc[2], ld[2], exp(-muq * (Cc[2] - D[2]) * (Cc[2] - D[2])), Qq[2] - Cc[2], Qq[2] - D[2], i2q, E2z);
-          const double alpha = p * q / (p + q);
-          double R[5][5][5];
-          hermite_R_all(tx1 + tx2, ty1 + ty2, tz1 + tz2, alpha, Pp[0] - Qq[0], Pp[1] - Qq[1], Pp[2] - Qq[2], R);
-          double acc = 0.0;
-          for (int t = 0; t <= tx1; ++t)
-            for (int u = 0; u <= ty1; ++u)
-              for (int v = 0; v <= tz1; ++v) {
-                const double e1 = E1x[t] * E1y[u] * E1z[v];
-                if (e1 == 0.0) continue;
-                for (int t2 = 0; t2 <= tx2; ++t2)
-                  for (int u2 = 0; u2 <= ty2; ++u2)
-                    for (int v2 = 0; v2 <= tz2; ++v2) {
-                      const double e2 = E2x[t2] * E2y[u2] * E2z[v2];
-                      const double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
-                      acc += e1 * e2 * sg * R[t + t2][u + u2][v + v2];
-                    }
-              }
-          val += ca * cb * cc * cd * 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q)) * acc;
+  for (int a = 0; a < pp; ++a) {
+    const double* x1 = r1 + a * PREC;
+    const int n1 = (int)x1[5];
+    if (n1 == 0) continue;
+    const double p = x1[0], Px = x1[1], Py = x1[2], Pz = x1[3], c1 = x1[4];
+    for (int c = 0; c < pp; ++c) {
+      const double* x2 = r2 + c * PREC;
+      const int n2 = (int)x2[5];
+      if (n2 == 0) continue;
+      const double q = x2[0];
+      const double alpha = p * q / (p + q);
+      const double X = Px - x2[1], Y = Py - x2[2], Z = Pz - x2[3];
+      const Boys5 F = boys5(alpha * (X * X + Y * Y + Z * Z));
+      double acc = 0.0;
+      for (int s1 = 0; s1 < n1; ++s1) {
+        const int k1 = (int)x1[6 + 2 * s1];
+        const double e1 = x1[7 + 2 * s1];
+        const int t1 = k1 & 15, u1 = (k1 >> 4) & 15, v1 = k1 >> 8;
+        for (int s2 = 0; s2 < n2; ++s2) {
+          const int k2 = (int)x2[6 + 2 * s2];
+          const int t2 = k2 & 15, u2 = (k2 >> 4) & 15, v2 = k2 >> 8;
+          const double e = e1 * x2[7 + 2 * s2];
+          const double rv = rtuv(t1 + t2, u1 + u2, v1 + v2, alpha, X, Y, Z, F);
+          acc += ((t2 + u2 + v2) & 1) ? -e * rv : e * rv;
         }
       }
+      val += c1 * x2[4] * TWO_PI_2_5 / (p * q * sqrt(p + q)) * acc;
     }
   }
   eri[P * npair + Q] = val;
@@ -784,9 +849,36 @@ __device__ double dot_all(int len, const double* a, const double* b, double* red
   return bsum(s, red);
 }
 
+// C[M x N] = alpha op(A) op(B), K the inner dimension, on FP64 matrix cores: one 16 x 16 output
+// tile per wave at a time (lane l supplies A[l%16][l/16] and B[l/16][l%16] of each 16 x 4 x 16
+// step and receives C[4v + l/16][l%16] in element v).  Operands in LDS or global memory.
+__device__ void mfma_gemm(int M, int N, int K, const double* A, int lda, bool ta, const double* B, int ldb, bool tb,
+                          double* C, int ldc, double alpha) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int mb = (M + 15) / 16, nb = (N + 15) / 16;
+  for (int tile = wv; tile < mb * nb; tile += nw) {
+    const int bi = tile / nb, bj = tile % nb;
+    const int r = bi * 16 + (lane & 15), c = bj * 16 + (lane & 15);
+    d4_t acc = (d4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + (lane >> 4);
+      double a = 0.0, b = 0.0;
+      if (r < M && k < K) a = ta ? A[k * lda + r] : A[r * lda + k];
+      if (c < N && k < K) b = tb ? B[c * ldb + k] : B[k * ldb + c];
+      acc = mfma64(a, b, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = bi * 16 + 4 * v + (lane >> 4), col = bj * 16 + (lane & 15);
+      if (row < M && col < N) C[row * ldc + col] = alpha * acc[v];
+    }
+  }
+  __syncthreads();
+}
+
 // Cholesky S = L L^T (in place, lower), then X = L^-T (upper): C = X V solves F C = S C e.
-__global__ __launch_bounds__(NT) void k_orthogonaliser(int n, const double* S, double* L, double* X) {
-  __shared__ double red[NW];
+__device__ void orthogonaliser(int n, const double* S, double* L, double* X) {
+  const int NT = blockDim.x;
   for (int t = threadIdx.x; t < n * n; t += NT) L[t] = S[t];
   __syncthreads();
   for (int k = 0; k < n; ++k) {
@@ -815,85 +907,10 @@ __global__ __launch_bounds__(NT) void k_orthogonaliser(int n, const double* S, d
     }
   }
   __syncthreads();
-  (void)red;
 }
 
-// Parallel cyclic Jacobi on the symmetric matrix A (LDS, n <= 96), accumulating rotations in V
-// (LDS).  Round-robin pairing: n/2 disjoint (p, q) pairs per round, n-1 rounds per sweep.
-__device__ void jacobi(int n, double* A, double* V, double* cs, int* pq, int max_sweeps, double tol) {
-  const int m = (n + 1) & ~1;  // even (a padded index m-1 >= n is a dummy)
-  __shared__ double red[NW];
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    double off = 0.0, dia = 0.0;
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-      const int i = t / n, j = t % n;
-      if (i != j)
-        off = fma(A[t], A[t], off);
-      else
-        dia = fma(A[t], A[t], dia);
-    }
-    off = bsum(off, red);
-    dia = bsum(dia, red);
-    if (off <= tol * tol * dia) break;
-    for (int round = 0; round < m - 1; ++round) {
-      // pairs of the round: (round, m-1) fixed-point scheme
-      for (int k = threadIdx.x; k < m / 2; k += blockDim.x) {
-        int p, q;
-        if (k == 0) {
-          p = m - 1;
-          q = round;
-        } else {
-          p = (round + k) % (m - 1);
-          q = (round - k + (m - 1)) % (m - 1);
-        }
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        double c = 1.0, s = 0.0;
-        if (q < n) {
-          const double apq = A[p * n + q];
-          if (apq != 0.0) {
-            const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
-            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
-          }
-        }
-        pq[2 * k] = p;
-        pq[2 * k + 1] = q;
-        cs[2 * k] = c;
-        cs[2 * k + 1] = s;
-      }
-      __syncthreads();
-      // rows: A <- J^T A
-      for (int idx = threadIdx.x; idx < (m / 2) * n; idx += blockDim.x) {
-        const int k = idx / n, col = idx % n;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        if (q >= n) continue;
-        const double c = cs[2 * k], s = cs[2 * k + 1];
-        const double ap = A[p * n + col], aq = A[q * n + col];
-        A[p * n + col] = c * ap - s * aq;
-        A[q * n + col] = s * ap + c * aq;
-      }
-      __syncthreads();
-      // columns: A <- A J, V <- V J
-      for (int idx = threadIdx.x; idx < (m / 2) * n; idx += blockDim.x) {
-        const int k = idx / n, row = idx % n;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        if (q >= n) continue;
-        const double c = cs[2 * k], s = cs[2 * k + 1];
-        const double ap = A[row * n + p], aq = A[row * n + q];
-        A[row * n + p] = c * ap - s * aq;
-        A[row * n + q] = s * ap + c * aq;
-        const double vp = V[row * n + p], vq = V[row * n + q];
-        V[row * n + p] = c * vp - s * vq;
-        V[row * n + q] = s * vp + c * vq;
-      }
-      __syncthreads();
-    }
-  }
+__global__ __launch_bounds__(NT) void k_orthogonaliser(int n, const double* S, double* L, double* X) {
+  orthogonaliser(n, S, L, X);
 }
 
 struct Scf3 {
@@ -918,6 +935,7 @@ struct Scf3 {
   int h1, h2;
   double nn_scale;
   double *zpart, *z1, *dz1, *wv;
+  double* eigf;  // eigensolver scratch (3 n^2)
 };
 
 // Fock + CDIIS (one workgroup).  cycle < 0: plain F = H + Veff (no DIIS).
@@ -938,8 +956,8 @@ __global__ __launch_bounds__(NT) void k_scf_fock(Scf3 s, int cycle) {
     return;
   }
   // error e = F D S - S D F
-  mm(n, s.F, false, s.D, false, s.tmp1, 1.0, 0.0);   // FD
-  mm(n, s.tmp1, false, s.S, false, s.tmp2, 1.0, 0.0);  // FDS
+  mfma_gemm(n, n, n, s.F, n, false, s.D, n, false, s.tmp1, n, 1.0);     // FD
+  mfma_gemm(n, n, n, s.tmp1, n, false, s.S, n, false, s.tmp2, n, 1.0);  // FDS
   // S D F = (F D S)^T for symmetric F, D, S
   const int cnt = s.ist[2], head = s.ist[3];
   const int slot = head;
@@ -1016,61 +1034,56 @@ __global__ __launch_bounds__(NT) void k_scf_fock(Scf3 s, int cycle) {
   __syncthreads();
 }
 
-// generalized eigh of Fd (X^T Fd X, Jacobi warm-started from Cp), C = X V, D = 2 C_occ C_occ^T
-__global__ __launch_bounds__(NT) void k_scf_eigh(Scf3 s, int cold) {
+// generalized eigh of Fd: Fp = X^T Fd X (symmetrised), Fp = Z diag(e) Z^T in LDS, C = X Z,
+// D = 2 C_occ C_occ^T.  One workgroup of 1024 threads.
+constexpr int NT_EIG = 1024;
+size_t eigh_lds(int n) { return (size_t)2 * n * (n + 1) * 8; }
+
+__global__ __launch_bounds__(NT_EIG) void k_scf_eigh(Scf3 s) {
   if (s.ist[0]) return;
   extern __shared__ double sm[];
-  const int n = s.n, nn = n * n;
+  __shared__ qexeig::Smem sh;
+  const int n = s.n, nn = n * n, S = n + 1;
   double* A = sm;
-  double* V = A + nn;
-  __shared__ double cs[QEX3D_MAX_NAO + 2];
-  __shared__ int pq[QEX3D_MAX_NAO + 2];
-  __shared__ int perm[QEX3D_MAX_NAO];
-  __shared__ double ev[QEX3D_MAX_NAO];
-  // Fp = X^T Fd X  (tmp1 = Fd X; tmp2 = X^T tmp1)
-  mm(n, s.Fd, false, s.X, false, s.tmp1, 1.0, 0.0);
-  mm(n, s.X, true, s.tmp1, false, s.tmp2, 1.0, 0.0);
-  if (cold) {
-    for (int t = threadIdx.x; t < nn; t += NT) {
-      A[t] = 0.5 * (s.tmp2[t] + s.tmp2[(t % n) * n + t / n]);
-      V[t] = (t / n == t % n) ? 1.0 : 0.0;
-    }
-  } else {
-    // warm start: A = Cp^T Fp Cp, V = Cp
-    mm(n, s.tmp2, false, s.Cp, false, s.tmp1, 1.0, 0.0);
-    mm(n, s.Cp, true, s.tmp1, false, s.tmp2, 1.0, 0.0);
-    for (int t = threadIdx.x; t < nn; t += NT) {
-      A[t] = 0.5 * (s.tmp2[t] + s.tmp2[(t % n) * n + t / n]);
-      V[t] = s.Cp[t];
-    }
-  }
-  __syncthreads();
-  jacobi(n, A, V, cs, pq, 30, 1e-15);
-  // sort eigenvalues (ascending) by rank
-  for (int i = threadIdx.x; i < n; i += NT) ev[i] = A[i * n + i];
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += NT) {
-    int r = 0;
-    for (int j = 0; j < n; ++j) r += (ev[j] < ev[i]) || (ev[j] == ev[i] && j < i);
-    perm[r] = i;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < nn; t += NT) {
-    const int row = t / n, c = t % n;
-    s.Cp[t] = V[row * n + perm[c]];
-  }
-  for (int i = threadIdx.x; i < n; i += NT) s.mo[i] = ev[perm[i]];
-  __syncthreads();
-  mm(n, s.X, false, s.Cp, false, s.C, 1.0, 0.0);
-  for (int t = threadIdx.x; t < nn; t += NT) s.Dlast[t] = s.D[t];
-  __syncthreads();
-  for (int t = threadIdx.x; t < nn; t += NT) {
+  double* Z = A + (size_t)n * S;
+  mfma_gemm(n, n, n, s.Fd, n, false, s.X, n, false, s.tmp1, n, 1.0);  // Fd X
+  mfma_gemm(n, n, n, s.X, n, true, s.tmp1, n, false, A, S, 1.0);      // X^T Fd X (into LDS)
+  for (int t = threadIdx.x; t < nn; t += blockDim.x) {
     const int i = t / n, j = t % n;
-    double d = 0.0;
-    for (int k = 0; k < s.nocc; ++k) d = fma(s.C[i * n + k], s.C[j * n + k], d);
-    s.D[t] = 2.0 * d;
+    if (i < j) {
+      const double v = 0.5 * (A[i * S + j] + A[j * S + i]);
+      A[i * S + j] = v;
+      A[j * S + i] = v;
+    }
   }
   __syncthreads();
+  qexeig::symeig(n, A, Z, S, s.mo, s.eigf, sh);
+  mfma_gemm(n, n, n, s.X, n, false, Z, S, false, s.C, n, 1.0);  // C = X Z
+  mfma_gemm(n, n, s.nocc, s.C, n, false, s.C, n, true, s.D, n, 2.0);  // D = 2 C_occ C_occ^T
+}
+
+// batched generalized eigh F C = S C e, one workgroup per matrix (scratch: 8 n^2 doubles each)
+__global__ __launch_bounds__(NT_EIG) void k_gen_eigh(int n, const double* F, const double* Sm, double* mo, double* C,
+                                                     double* scratch) {
+  extern __shared__ double sm[];
+  __shared__ qexeig::Smem sh;
+  const size_t nn = (size_t)n * n, b = blockIdx.x;
+  const int S = n + 1;
+  double* w = scratch + b * 8 * nn;
+  double *L = w, *X = w + nn, *t1 = w + 2 * nn, *t2 = w + 3 * nn, *Zg = w + 4 * nn, *f = w + 5 * nn;
+  orthogonaliser(n, Sm + b * nn, L, X);
+  mfma_gemm(n, n, n, F + b * nn, n, false, X, n, false, t1, n, 1.0);
+  mfma_gemm(n, n, n, X, n, true, t1, n, false, t2, n, 1.0);
+  double* A = sm;
+  double* Z = A + (size_t)n * S;
+  for (int t = threadIdx.x; t < (int)nn; t += blockDim.x) {
+    const int i = t / n, j = t % n;
+    A[i * S + j] = 0.5 * (t2[t] + t2[j * n + i]);
+  }
+  __syncthreads();
+  qexeig::symeig(n, A, Z, S, mo + b * n, f, sh);
+  mfma_gemm(n, n, n, X, n, false, Z, S, false, C + b * nn, n, 1.0);
+  (void)Zg;
 }
 
 // Vxc = sum of chunk partials (fixed order), symmetrised; energies and convergence.
@@ -1078,25 +1091,6 @@ __global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
   if (s.ist[0]) return;
   __shared__ double red[NW];
   const int n = s.n, nn = n * n;
-  double nel = 0.0, exc = 0.0;
-  if (s.method != 1) {
-    for (int t = threadIdx.x; t < nn; t += NT) {
-      double v = 0.0;
-      for (int c = 0; c < s.nchunk; ++c) v += s.vpart[(size_t)c * nn + t];
-      s.tmp1[t] = v;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nn; t += NT) s.Vxc[t] = s.tmp1[t] + s.tmp1[(t % n) * n + t / n];
-    if (threadIdx.x == 0) {
-      for (int c = 0; c < s.nchunk; ++c) {
-        nel += s.spart[2 * c];
-        exc += s.spart[2 * c + 1];
-      }
-      s.scal[2] = nel;
-      s.scal[3] = exc;
-    }
-    __syncthreads();
-  }
   double e1 = 0.0, e2 = 0.0;
   for (int t = threadIdx.x; t < nn; t += NT) {
     e1 = fma(s.D[t], s.H[t], e1);
@@ -1176,7 +1170,7 @@ __global__ void k_scf_finish(Scf3 s) {
 
 struct Lay3 {
   size_t F, Fd, D, Dlast, C, Cp, mo, J, K, Vxc, vpart, spart, rho, tmp1, tmp2, diisF, diisE, diisB, scal, ist;
-  size_t zpart, z1, dz1, wv, total;
+  size_t zpart, z1, dz1, wv, eigf, total;
 };
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1215,8 +1209,20 @@ Lay3 layout3(int n, long ng, int space, int h1 = 0) {
   L.z1 = take((size_t)h1 * 8);
   L.dz1 = take((size_t)h1 * 8);
   L.wv = take(h1 > 0 ? (size_t)ng * 8 : 0);
+  L.eigf = take((size_t)3 * n * n * 8);
   L.total = off;
   return L;
+}
+
+// the Boys table is made once per device (the first integral call waits for it)
+bool g_boys_ready[64] = {false};
+void ensure_boys(hipStream_t st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (g_boys_ready[dev]) return;
+  hipLaunchKernelGGL(k_boys_init, dim3((BOYS_NT + 255) / 256), dim3(256), 0, st);
+  // wait once, so that launches on any other stream also see the finished table
+  if (hipStreamSynchronize(st) == hipSuccess) g_boys_ready[dev] = true;
 }
 
 Basis make_basis(const qex3d_basis* b) {
@@ -1241,13 +1247,17 @@ int check_basis(const qex3d_basis* b) {
   if (!b) return fail3(QEX_E_BADARG, "null basis");
   if (b->natm < 1 || b->nshell < 1 || b->nao < 1) return fail3(QEX_E_BADARG, "empty basis");
   if (b->nao > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "nao %d exceeds %d", b->nao, QEX3D_MAX_NAO);
+  if (b->max_nprim < 1 || b->max_nprim > QEX3D_MAX_NPRIM)
+    return fail3(QEX_E_BADARG, "max_nprim must be in [1, %d]", QEX3D_MAX_NPRIM);
   if (!b->atom_coords || !b->atom_charges || !b->shell_atom || !b->shell_l || !b->shell_nprim || !b->shell_offset ||
       !b->exponents || !b->coefficients || !b->ao_shell || !b->ao_lmn)
     return fail3(QEX_E_BADARG, "missing basis array");
   return QEX_OK;
 }
 
-__global__ void k_nr_rks_reduce(int n, int nchunk, const double* vpart, const double* spart, double* vmat, double* out2) {
+__global__ void k_nr_rks_reduce(int n, int nchunk, const double* vpart, const double* spart, double* vmat, double* out2,
+                                const int* frozen) {
+  if (frozen && *frozen) return;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
     const int i = t / n, j = t % n;
     double a = 0.0, b = 0.0;
@@ -1279,20 +1289,34 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
   if (rc) return rc;
   if (!S || !T || !V) return fail3(QEX_E_BADARG, "null output");
   const int np = b->nao * (b->nao + 1) / 2;
+  ensure_boys((hipStream_t)stream);
   qexprof::Scope ps(qexprof::K_INT1E, (hipStream_t)stream);
   hipLaunchKernelGGL(k_int1e, dim3((np + 127) / 128), dim3(128), 0, (hipStream_t)stream, make_basis(b), S, T, V);
   return launch_status("qex3d_int1e");
 }
 
-int qex3d_eri(const qex3d_basis* b, double* eri, void* stream) {
+size_t qex3d_eri_workspace_bytes(const qex3d_basis* b) {
+  if (check_basis(b) != QEX_OK) return 0;
+  const long np = (long)b->nao * (b->nao + 1) / 2;
+  return al((size_t)np * b->max_nprim * b->max_nprim * PREC * 8);
+}
+
+int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream) {
   int rc = check_basis(b);
   if (rc) return rc;
   if (!eri) return fail3(QEX_E_BADARG, "null output");
+  const size_t need = qex3d_eri_workspace_bytes(b);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
   const int np = b->nao * (b->nao + 1) / 2;
   const long total = (long)np * (np + 1) / 2;
-  qexprof::Scope ps(qexprof::K_ERI, (hipStream_t)stream);
-  hipLaunchKernelGGL(k_eri, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, (hipStream_t)stream, make_basis(b),
-                     eri, np);
+  const long nrec = (long)np * b->max_nprim * b->max_nprim;
+  hipStream_t st = (hipStream_t)stream;
+  ensure_boys(st);
+  qexprof::Scope ps(qexprof::K_ERI, st);
+  hipLaunchKernelGGL(k_eri_pairs, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, st, make_basis(b), np,
+                     b->max_nprim, (double*)workspace);
+  hipLaunchKernelGGL(k_eri, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, st, np, b->max_nprim,
+                     (const double*)workspace, eri);
   return launch_status("qex3d_eri");
 }
 
@@ -1357,7 +1381,8 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
-    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum);
+    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum,
+                       nullptr);
   }
   return launch_status("qex3d_nr_rks");
 }
@@ -1421,10 +1446,31 @@ int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* 
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
-    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum);
+    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum,
+                       nullptr);
     hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, st, exc, nelec_excsum + 1);
   }
   return launch_status("qex3d_nr_rks_nn");
+}
+
+size_t qex3d_gen_eigh_workspace_bytes(int32_t batch, int32_t n) {
+  if (batch < 1 || n < 1 || n > QEX3D_MAX_NAO) return 0;
+  return al((size_t)batch * 8 * n * n * 8);
+}
+
+int qex3d_gen_eigh(int32_t batch, int32_t n, const double* F, const double* S, double* mo_energy, double* mo_coeff,
+                   void* workspace, size_t workspace_bytes, void* stream) {
+  if (batch < 1 || n < 1 || n > QEX3D_MAX_NAO || !F || !S || !mo_energy || !mo_coeff)
+    return fail3(QEX_E_BADARG, "bad gen_eigh arguments (n must be in [1, %d])", QEX3D_MAX_NAO);
+  const size_t need = qex3d_gen_eigh_workspace_bytes(batch, n);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  const size_t lds = eigh_lds(n);
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_gen_eigh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  qexprof::Scope ps(qexprof::K_EIGH, (hipStream_t)stream);
+  hipLaunchKernelGGL(k_gen_eigh, dim3(batch), dim3(NT_EIG), lds, (hipStream_t)stream, n, F, S, mo_energy, mo_coeff,
+                     (double*)workspace);
+  return launch_status("qex3d_gen_eigh");
 }
 
 int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
@@ -1499,8 +1545,9 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
   s.z1 = (double*)(ws + L.z1);
   s.dz1 = (double*)(ws + L.dz1);
   s.wv = (double*)(ws + L.wv);
-  const size_t lds_eigh = (size_t)2 * n * n * 8;
-  if (lds_eigh > 160 * 1024) return fail3(QEX_E_UNSUPPORTED, "nao too large for the LDS eigensolver");
+  s.eigf = (double*)(ws + L.eigf);
+  const size_t lds_eigh = eigh_lds(n);
+  if (lds_eigh + sizeof(qexeig::Smem) > 160 * 1024) return fail3(QEX_E_UNSUPPORTED, "nao too large for the LDS eigensolver");
   if (lds_eigh > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_scf_eigh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eigh);
   const size_t lds_vxc = vxc_lds(n);
@@ -1515,9 +1562,15 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
         qexprof::Scope ps(qexprof::K_NN, st);
         gnn_dispatch(ng, s.h1, s.h2, s.nn_scale, s.nn, s.rho, s.w, s.zpart, s.z1, s.dz1, s.scal + 5, s.wv, s.ist, st);
       }
-      qexprof::Scope ps(qexprof::K_VXC, st);
-      hipLaunchKernelGGL(k_vxc, dim3((unsigned)s.nchunk), dim3(NT), lds_vxc, st, ng, n, s.ao, s.w, s.rho,
-                         s.method == QEX3D_RKS_NN_GLOBAL ? (const double*)s.wv : nullptr, s.vpart, s.spart, s.ist);
+      {
+        qexprof::Scope ps(qexprof::K_VXC, st);
+        hipLaunchKernelGGL(k_vxc, dim3((unsigned)s.nchunk), dim3(NT), lds_vxc, st, ng, n, s.ao, s.w, s.rho,
+                           s.method == QEX3D_RKS_NN_GLOBAL ? (const double*)s.wv : nullptr, s.vpart, s.spart, s.ist);
+      }
+      // Vxc = P + P^T with P the fixed-order sum of the chunk partials; scal[2..3] = nelec, exc
+      qexprof::Scope ps(qexprof::K_OTHER, st);
+      hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, n, s.nchunk, s.vpart, s.spart, s.Vxc,
+                         s.scal + 2, s.ist);
     }
     qexprof::Scope ps(qexprof::K_COULOMB, st);
     hipLaunchKernelGGL(k_coulomb, dim3(jblocks), dim3(NT), (size_t)s.npair * 8, st, n, s.npair, s.eri, s.D, s.J,
@@ -1527,17 +1580,21 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
     qexprof::Scope ps(kind, st);
     hipLaunchKernelGGL(kernel, dim3(1), dim3(NT), lds, st, args...);
   };
+  auto eigh = [&]() {
+    qexprof::Scope ps(qexprof::K_EIGH, st);
+    hipLaunchKernelGGL(k_scf_eigh, dim3(1), dim3(NT_EIG), lds_eigh, st, s);
+  };
   small(qexprof::K_OTHER, k_scf_init, 0, s);
   small(qexprof::K_OTHER, k_orthogonaliser, 0, n, s.S, Lchol, X);
   // core-Hamiltonian guess: F = H (J, K, Vxc and D cleared by a kernel, not a memset)
   for (double* z : {s.J, s.Vxc, s.K, s.D}) hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, st, z, (long)n * n);
   small(qexprof::K_FOCK, k_scf_fock, 0, s, -1);
-  small(qexprof::K_EIGH, k_scf_eigh, lds_eigh, s, 1);
+  eigh();
   veff();
   small(qexprof::K_ENERGY, k_scf_energy, 0, s, -1);
   for (int c = 0; c < p->max_cycle; ++c) {
     small(qexprof::K_FOCK, k_scf_fock, 0, s, c);
-    small(qexprof::K_EIGH, k_scf_eigh, lds_eigh, s, 0);
+    eigh();
     veff();
     small(qexprof::K_ENERGY, k_scf_energy, 0, s, c);
     if (p->sync_every > 0 && (c + 1) % p->sync_every == 0 && c + 1 < p->max_cycle) {

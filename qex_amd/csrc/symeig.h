@@ -1,0 +1,307 @@
+// symeig.h -- dense symmetric eigensolver for one workgroup (n <= 96, FP64, matrix in LDS).
+//
+//   1. Householder tridiagonalisation: for k = 0..n-3 the reflector H_k = I - 2 v v^T (|v| = 1)
+//      annihilates column k below the subdiagonal; the trailing block is updated as
+//      A <- A - v w^T - w v^T with p = A v, beta = v^T p, w = 2 p - 2 beta v.  v is kept in column k
+//      of A (below the diagonal), d / e collect the tridiagonal.
+//   2. T splits into unreduced blocks where |e_k| <= eps (|d_k| + |d_k+1|).
+//   3. Eigenvalues: multisection on Sturm counts (negative pivots of the LDL^T of T - x), G threads
+//      per eigenvalue, a fixed number of rounds (interval / (G+1) per round).
+//   4. Eigenvectors of T: inverse iteration (3 steps with the LDL^T pivots of T - lam, pivots
+//      smaller than eps |T| replaced by +-eps |T|) from a fixed pseudo-random start; eigenvalues of one block
+//      closer than 1e-8 |T| form a cluster whose vectors one thread computes in order, removing
+//      the earlier members' components after every solve.
+//   5. Ascending sort and back-transformation Z <- H_0 ... H_{n-3} Y.
+// All reductions run in a fixed order: the result does not depend on timing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qexeig {
+
+constexpr double EPS = 2.220446049250313e-16;
+constexpr int MAXN = 96;
+constexpr int ROUNDS = 20;  // (G+1)^ROUNDS >= 2^60 for G >= 8
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Shared state of the solver (static LDS besides the two n x S matrices).
+struct Smem {
+  double d[MAXN], e[MAXN], lam[MAXN], lo[MAXN];
+  int bs[MAXN], bt[MAXN], loc[MAXN], perm[MAXN];
+  unsigned char cnt[2][1024];
+  double norm;
+};
+
+__device__ __forceinline__ int sturm_count(const double* d, const double* e, int s, int t, double x, double pivmin) {
+  int c = 0;
+  double q = d[s] - x;
+  c += q < 0.0;
+  for (int j = s + 1; j < t; ++j) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    q = (d[j] - x) - e[j - 1] * e[j - 1] / q;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// deterministic start vector entry in [-1, 1)
+__device__ __forceinline__ double start_entry(int slot, int i) {
+  unsigned x = 2654435761u * (unsigned)(slot + 1) ^ (40503u * (unsigned)(i + 7));
+  x ^= x >> 13;
+  x *= 0x5bd1e995u;
+  x ^= x >> 15;
+  return (double)(x & 0xFFFFFF) / 8388608.0 - 1.0;
+}
+
+// Pivots of the LDL^T factorisation of T_block - lam (the Sturm recurrence), |D| >= tiny, stored
+// in column `col` of Dm (row stride S).
+__device__ void ldl_pivots(const double* d, const double* e, int s, int t, double lam, double tiny, double* Dm, int S,
+                           int col) {
+  double q = d[s] - lam;
+  if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
+  Dm[s * S + col] = q;
+  for (int i = s + 1; i < t; ++i) {
+    q = (d[i] - lam) - e[i - 1] * (e[i - 1] / q);
+    if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
+    Dm[i * S + col] = q;
+  }
+}
+
+// (T_block - lam) x = b in place in column `col` of Y, with the pivots from ldl_pivots:
+// forward y_i = b_i - (e_i-1 / D_i-1) y_i-1, then x_i = y_i / D_i - (e_i / D_i) x_i+1.
+__device__ void ldl_solve(const double* e, int s, int t, const double* Dm, double* Y, int S, int col) {
+  double prev = Y[s * S + col];
+  for (int i = s + 1; i < t; ++i) {
+    const double y = Y[i * S + col] - (e[i - 1] / Dm[(i - 1) * S + col]) * prev;
+    Y[i * S + col] = y;
+    prev = y;
+  }
+  double next = Y[(t - 1) * S + col] / Dm[(t - 1) * S + col];
+  Y[(t - 1) * S + col] = next;
+  for (int i = t - 2; i >= s; --i) {
+    const double Di = Dm[i * S + col];
+    next = (Y[i * S + col] - e[i] * next) / Di;
+    Y[i * S + col] = next;
+  }
+}
+
+__device__ double col_norm2(const double* Y, int S, int s, int t, int col) {
+  double r = 0.0;
+  for (int i = s; i < t; ++i) r = fma(Y[i * S + col], Y[i * S + col], r);
+  return r;
+}
+
+// A: n x n symmetric in LDS (row stride S, destroyed); Z: n x n LDS output, column j = eigenvector
+// of the j-th smallest eigenvalue; w: ascending eigenvalues (any memory).  f: global scratch of
+// 2 n^2 doubles (3 n^2 are reserved by the callers).  Must be called by every thread of the workgroup (blockDim.x a multiple of 64,
+// blockDim.x >= n).
+__device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f, Smem& sh) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  if (n == 1) {
+    if (tid == 0) {
+      w[0] = A[0];
+      Z[0] = 1.0;
+    }
+    __syncthreads();
+    return;
+  }
+  // ---- 1. tridiagonalisation ----
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    // column k below the diagonal: x; every wave reduces it (no broadcast needed)
+    double sg = 0.0;
+    for (int i = 1 + lane; i < m; i += 64) {
+      const double x = A[(k + 1 + i) * S + k];
+      sg = fma(x, x, sg);
+    }
+    sg = wave_sum(sg);
+    const double x0 = A[(k + 1) * S + k];
+    const bool skip = !(sg > 0.0);
+    const double nx = sqrt(x0 * x0 + sg);
+    const double alpha = x0 >= 0.0 ? -nx : nx;
+    const double vn = skip ? 1.0 : sqrt(sg + (x0 - alpha) * (x0 - alpha));
+    const double iv = 1.0 / vn;
+    auto vget = [&](int i) -> double { return i == 0 ? (x0 - alpha) * iv : A[(k + 1 + i) * S + k] * iv; };
+    if (tid == 0) {
+      sh.d[k] = A[k * S + k];
+      sh.e[k] = skip ? x0 : alpha;
+    }
+    if (!skip) {
+      // p = A22 v (one wave per row) into lo[] (reused as the p vector)
+      for (int i = wv; i < m; i += nw) {
+        double acc = 0.0;
+        for (int j = lane; j < m; j += 64) acc = fma(A[(k + 1 + i) * S + k + 1 + j], vget(j), acc);
+        acc = wave_sum(acc);
+        if (lane == 0) sh.lo[i] = acc;
+      }
+      __syncthreads();
+      double bt = 0.0;
+      for (int j = lane; j < m; j += 64) bt = fma(vget(j), sh.lo[j], bt);
+      bt = wave_sum(bt);
+      for (int idx = tid; idx < m * m; idx += nt) {
+        const int i = idx / m, j = idx % m;
+        const double vi = vget(i), vj = vget(j);
+        const double wi = 2.0 * sh.lo[i] - 2.0 * bt * vi, wj = 2.0 * sh.lo[j] - 2.0 * bt * vj;
+        A[(k + 1 + i) * S + k + 1 + j] -= vi * wj + wi * vj;
+      }
+    }
+    __syncthreads();
+    // keep v in column k (read above by every thread, so store after the barrier)
+    for (int i = tid; i < m; i += nt) A[(k + 1 + i) * S + k] = skip ? 0.0 : vget(i);
+    // the next step reads only rows / columns > k + 1 of the trailing block and column k + 1
+  }
+  __syncthreads();
+  if (tid == 0) {
+    sh.d[n - 2] = A[(n - 2) * S + n - 2];
+    sh.e[n - 2] = A[(n - 1) * S + n - 2];
+    sh.d[n - 1] = A[(n - 1) * S + n - 1];
+    // ---- 2. split into unreduced blocks; slot -> (block, local index) ----
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double r = fabs(sh.d[i]) + (i > 0 ? fabs(sh.e[i - 1]) : 0.0) + (i < n - 1 ? fabs(sh.e[i]) : 0.0);
+      nrm = fmax(nrm, r);
+    }
+    sh.norm = nrm > 0.0 ? nrm : 1e-300;
+    int s = 0;
+    for (int k = 0; k < n; ++k) {
+      const bool cut = k == n - 1 || fabs(sh.e[k]) <= EPS * (fabs(sh.d[k]) + fabs(sh.d[k + 1]));
+      if (k < n - 1 && cut) sh.e[k] = 0.0;
+      if (cut) {
+        for (int q = s; q <= k; ++q) {
+          sh.bs[q] = s;
+          sh.bt[q] = k + 1;
+          sh.loc[q] = q - s;
+        }
+        s = k + 1;
+      }
+    }
+  }
+  // reflectors (strict lower part of A) to f, so that A can hold the LDL^T pivots of phase 4
+  for (int idx = tid; idx < n * n; idx += nt)
+    if (idx / n > idx % n) f[idx] = A[(idx / n) * S + idx % n];
+  __syncthreads();
+  // ---- 3. eigenvalues by multisection ----
+  {
+    const int G = nt / n;  // >= 10 for n <= 96 and 1024 threads
+    const int slot = tid / G, g = tid % G;
+    const bool act = slot < n;
+    double lo = 0.0, hi = 0.0, pivmin = 0.0;
+    int s = 0, t = 0, li = 0;
+    if (act) {
+      s = sh.bs[slot];
+      t = sh.bt[slot];
+      li = sh.loc[slot];
+      double emax = 0.0;
+      lo = 1e300;
+      hi = -1e300;
+      for (int j = s; j < t; ++j) {
+        const double r = (j > s ? fabs(sh.e[j - 1]) : 0.0) + (j < t - 1 ? fabs(sh.e[j]) : 0.0);
+        lo = fmin(lo, sh.d[j] - r);
+        hi = fmax(hi, sh.d[j] + r);
+        if (j < t - 1) emax = fmax(emax, sh.e[j] * sh.e[j]);
+      }
+      const double span = fmax(hi - lo, EPS * sh.norm);
+      lo -= 1e-12 * span + 1e-300;
+      hi += 1e-12 * span + 1e-300;
+      pivmin = fmax(1e-300, 1e-290 * emax);
+    }
+    for (int r = 0; r < ROUNDS; ++r) {
+      unsigned char* cb = sh.cnt[r & 1];
+      if (act) {
+        const double x = lo + (hi - lo) * (double)(g + 1) / (double)(G + 1);
+        cb[slot * G + g] = (unsigned char)sturm_count(sh.d, sh.e, s, t, x, pivmin);
+      }
+      __syncthreads();
+      if (act) {
+        double nlo = lo, nhi = hi;
+        for (int q = 0; q < G; ++q) {
+          const double x = lo + (hi - lo) * (double)(q + 1) / (double)(G + 1);
+          if (cb[slot * G + q] <= li)
+            nlo = fmax(nlo, x);
+          else
+            nhi = fmin(nhi, x);
+        }
+        lo = nlo;
+        hi = nhi;
+      }
+    }
+    if (act && g == 0) sh.lam[slot] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+  // ---- 4. eigenvectors of T (column `slot` of Z holds the block's rows) ----
+  for (int idx = tid; idx < n * n; idx += nt) Z[(idx / n) * S + idx % n] = 0.0;
+  __syncthreads();
+  {
+    const double tiny = EPS * sh.norm, ctol = 1e-8 * sh.norm;
+    const int slot = tid;
+    if (slot < n) {
+      const int s = sh.bs[slot], t = sh.bt[slot];
+      const bool leader = slot == s || sh.lam[slot] - sh.lam[slot - 1] >= ctol;
+      if (leader) {
+        int last = slot;
+        while (last + 1 < t && sh.lam[last + 1] - sh.lam[last] < ctol) ++last;
+        for (int a = slot; a <= last; ++a) {
+          if (t - s == 1) {
+            Z[s * S + a] = 1.0;
+            continue;
+          }
+          ldl_pivots(sh.d, sh.e, s, t, sh.lam[a], tiny, A, S, a);
+          for (int i = s; i < t; ++i) Z[i * S + a] = start_entry(a, i);
+          for (int it = 0; it < 3; ++it) {
+            const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
+            for (int i = s; i < t; ++i) Z[i * S + a] *= nr;
+            ldl_solve(sh.e, s, t, A, Z, S, a);
+            for (int b = slot; b < a; ++b) {
+              double dp = 0.0;
+              for (int i = s; i < t; ++i) dp = fma(Z[i * S + b], Z[i * S + a], dp);
+              for (int i = s; i < t; ++i) Z[i * S + a] = fma(-dp, Z[i * S + b], Z[i * S + a]);
+            }
+          }
+          const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
+          for (int i = s; i < t; ++i) Z[i * S + a] *= nr;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 5. sort ascending (rank; ties by slot), permute columns through A, back-transform ----
+  for (int i = tid; i < n; i += nt) {
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += (sh.lam[j] < sh.lam[i]) || (sh.lam[j] == sh.lam[i] && j < i);
+    sh.perm[r] = i;
+  }
+  __syncthreads();
+  // permuted copy through global scratch (f + n^2); reflectors back into A's strict lower part
+  double* g = f + (size_t)n * n;
+  for (int idx = tid; idx < n * n; idx += nt) {
+    const int i = idx / n, c = idx % n;
+    g[idx] = Z[i * S + sh.perm[c]];
+    if (i > c) A[i * S + c] = f[idx];
+  }
+  for (int i = tid; i < n; i += nt) w[i] = sh.lam[sh.perm[i]];
+  __syncthreads();
+  for (int idx = tid; idx < n * n; idx += nt) Z[(idx / n) * S + idx % n] = g[idx];
+  __syncthreads();
+  for (int k = n - 3; k >= 0; --k) {
+    const int m = n - k - 1;
+    // sh.lo[c] = v^T Z[k+1:, c]  (one wave per column)
+    for (int c = wv; c < n; c += nw) {
+      double acc = 0.0;
+      for (int i = lane; i < m; i += 64) acc = fma(A[(k + 1 + i) * S + k], Z[(k + 1 + i) * S + c], acc);
+      acc = wave_sum(acc);
+      if (lane == 0) sh.lo[c] = acc;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < m * n; idx += nt) {
+      const int i = idx / n, c = idx % n;
+      Z[(k + 1 + i) * S + c] -= 2.0 * A[(k + 1 + i) * S + k] * sh.lo[c];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace qexeig

@@ -102,7 +102,8 @@ class Mole:
         i = lambda x: engine.as_i32(np.asarray(x, dtype=np.int32), dev)
         keep = [f(self.coords), f(self.charges), i(sh_atom), i(sh_l), i(sh_n), i(sh_off), f(exps), f(coefs),
                 i(ao_shell), i(ao_lmn)]
-        b = _lib.Qex3dBasis(self.natm, len(self.shells), self.nao, *[ctypes.c_void_p(t.data_ptr()) for t in keep])
+        b = _lib.Qex3dBasis(self.natm, len(self.shells), self.nao, *[ctypes.c_void_p(t.data_ptr()) for t in keep],
+                            max(sh_n))
         self._dev = (b, keep)
         return b
 
@@ -120,7 +121,10 @@ class Mole:
         dev = engine.device()
         npair = self.nao * (self.nao + 1) // 2
         g = torch.empty((npair, npair), dtype=torch.float64, device=dev)
-        _lib.check3(_lib.load().qex3d_eri(ctypes.byref(self.device_basis()), engine._ptr(g), engine._stream()),
+        lib = _lib.load()
+        b = self.device_basis()
+        ws = engine.workspace(lib.qex3d_eri_workspace_bytes(ctypes.byref(b)), dev)
+        _lib.check3(lib.qex3d_eri(ctypes.byref(b), engine._ptr(g), engine._ptr(ws), ws.numel(), engine._stream()),
                     "qex3d_eri")
         return g
 

@@ -151,7 +151,8 @@ def test_3d_entry_points_reject_null_basis():
     lib = _lib.load()
     fake = ctypes.c_void_p(0x1000)
     assert lib.qex3d_int1e(None, fake, fake, fake, None) == -1
-    assert lib.qex3d_eri(None, fake, None) == -1
+    assert lib.qex3d_eri(None, fake, fake, 1 << 20, None) == -1
     b = _lib.Qex3dBasis(1, 1, _lib.QEX3D_MAX_NAO + 1)
+    b.max_nprim = 3
     assert lib.qex3d_eval_ao(ctypes.byref(b), 10, fake, fake, None) == -4
     assert lib.qex3d_eval_rho(10, 0, fake, fake, fake, None) == -1

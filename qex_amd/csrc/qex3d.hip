@@ -322,6 +322,13 @@ __global__ void k_eri_pairs(Basis b, int npair, int maxp, double* rec) {
   const double* B = b.xyz + 3 * b.sh_atom[sj];
   const double ea = b.exps[b.sh_off[si] + pa], eb = b.exps[b.sh_off[sj] + pb];
   const double p = ea + eb, mu = ea * eb / p, i2p = 0.5 / p;
+  const double cab = b.coefs[b.sh_off[si] + pa] * b.coefs[b.sh_off[sj] + pb];
+  const double r2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+  // primitive pairs whose Gaussian product prefactor is below 1e-20 contribute < 1e-18 to any integral
+  if (fabs(cab) * exp(-mu * r2) * (M_PI / p) * sqrt(M_PI / p) < 1e-20) {
+    o[5] = 0.0;
+    return;
+  }
   double Pp[3], E[3][3];
   for (int d = 0; d < 3; ++d) {
     Pp[d] = (ea * A[d] + eb * B[d]) / p;
@@ -331,7 +338,7 @@ __global__ void k_eri_pairs(Basis b, int npair, int maxp, double* rec) {
   o[1] = Pp[0];
   o[2] = Pp[1];
   o[3] = Pp[2];
-  o[4] = b.coefs[b.sh_off[si] + pa] * b.coefs[b.sh_off[sj] + pb];
+  o[4] = cab;
   int nt = 0;
   for (int t = 0; t <= la[0] + lb[0]; ++t)
     for (int u = 0; u <= la[1] + lb[1]; ++u)
@@ -1255,27 +1262,57 @@ int check_basis(const qex3d_basis* b) {
   return QEX_OK;
 }
 
-__global__ void k_nr_rks_reduce(int n, int nchunk, const double* vpart, const double* spart, double* vmat, double* out2,
-                                const int* frozen) {
+// P[e] = sum_c vpart[c][e] for 32 entries per workgroup (8 interleaved partial sums combined in a
+// fixed order); the extra last workgroup sums the (nelec, exc) chunk partials by a fixed tree.
+__global__ __launch_bounds__(256) void k_vxc_sum(int nn, int nchunk, const double* vpart, const double* spart,
+                                                 double* P, double* out2, const int* frozen) {
   if (frozen && *frozen) return;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
-    const int i = t / n, j = t % n;
-    double a = 0.0, b = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
-      a += vpart[(size_t)c * n * n + i * n + j];
-      b += vpart[(size_t)c * n * n + j * n + i];
+  __shared__ double red[8][33];
+  __shared__ double s2[2][256];
+  const int t = threadIdx.x;
+  if (blockIdx.x == gridDim.x - 1) {
+    s2[0][t] = t < nchunk ? spart[2 * t] : 0.0;
+    s2[1][t] = t < nchunk ? spart[2 * t + 1] : 0.0;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) {
+        s2[0][t] += s2[0][t + h];
+        s2[1][t] += s2[1][t + h];
+      }
+      __syncthreads();
     }
-    vmat[t] = a + b;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
-      s0 += spart[2 * c];
-      s1 += spart[2 * c + 1];
+    if (t == 0) {
+      out2[0] = s2[0][0];
+      out2[1] = s2[1][0];
     }
-    out2[0] = s0;
-    out2[1] = s1;
+    return;
   }
+  const int e = blockIdx.x * 32 + (t & 31), part = t >> 5;
+  double a = 0.0;
+  if (e < nn)
+    for (int c = part; c < nchunk; c += 8) a += vpart[(size_t)c * nn + e];
+  red[part][t & 31] = a;
+  __syncthreads();
+  if (t < 32 && e < nn) {
+    double v = 0.0;
+    for (int q = 0; q < 8; ++q) v += red[q][t];
+    P[e] = v;
+  }
+}
+
+// vmat = P + P^T
+__global__ void k_sym_add(int n, const double* P, double* vmat, const int* frozen) {
+  if (frozen && *frozen) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n * n) vmat[t] = P[t] + P[(t % n) * n + t / n];
+}
+
+// Vxc from the k_vxc chunk partials (nchunk <= VXC_BLOCKS = 256); P is n^2 scratch
+void vxc_reduce(int n, int nchunk, const double* vpart, const double* spart, double* P, double* vmat, double* out2,
+                const int* frozen, hipStream_t st) {
+  const int nn = n * n;
+  hipLaunchKernelGGL(k_vxc_sum, dim3((nn + 31) / 32 + 1), dim3(256), 0, st, nn, nchunk, vpart, spart, P, out2, frozen);
+  hipLaunchKernelGGL(k_sym_add, dim3((nn + 255) / 256), dim3(256), 0, st, n, P, vmat, frozen);
 }
 
 }  // namespace
@@ -1356,7 +1393,7 @@ int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* d
 size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao) {
   if (ngrid < 1 || nao < 1) return 0;
   const long nb = vxc_blocks(ngrid);
-  return al((size_t)ngrid * 8) + al((size_t)nb * nao * nao * 8) + al((size_t)nb * 16);
+  return al((size_t)ngrid * 8) + al((size_t)nb * nao * nao * 8) + al((size_t)nb * 16) + al((size_t)nao * nao * 8);
 }
 
 int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
@@ -1381,8 +1418,8 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
-    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum,
-                       nullptr);
+    double* P = (double*)((char*)spart + al((size_t)nb * 16));
+    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, nullptr, st);
   }
   return launch_status("qex3d_nr_rks");
 }
@@ -1446,8 +1483,8 @@ int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* 
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
-    hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, nao, nb, vpart, spart, vmat, nelec_excsum,
-                       nullptr);
+    double* P = (double*)((char*)spart + al((size_t)nb * 16));
+    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, nullptr, st);
     hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, st, exc, nelec_excsum + 1);
   }
   return launch_status("qex3d_nr_rks_nn");
@@ -1569,8 +1606,7 @@ int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3
       }
       // Vxc = P + P^T with P the fixed-order sum of the chunk partials; scal[2..3] = nelec, exc
       qexprof::Scope ps(qexprof::K_OTHER, st);
-      hipLaunchKernelGGL(k_nr_rks_reduce, dim3(64), dim3(256), 0, st, n, s.nchunk, s.vpart, s.spart, s.Vxc,
-                         s.scal + 2, s.ist);
+      vxc_reduce(n, s.nchunk, s.vpart, s.spart, s.tmp2, s.Vxc, s.scal + 2, s.ist, st);
     }
     qexprof::Scope ps(qexprof::K_COULOMB, st);
     hipLaunchKernelGGL(k_coulomb, dim3(jblocks), dim3(NT), (size_t)s.npair * 8, st, n, s.npair, s.eri, s.D, s.J,

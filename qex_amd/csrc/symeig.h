@@ -20,7 +20,6 @@ namespace qexeig {
 
 constexpr double EPS = 2.220446049250313e-16;
 constexpr int MAXN = 96;
-constexpr int ROUNDS = 20;  // (G+1)^ROUNDS >= 2^60 for G >= 8
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -30,22 +29,31 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 // Shared state of the solver (static LDS besides the two n x S matrices).
 struct Smem {
-  double d[MAXN], e[MAXN], lam[MAXN], lo[MAXN];
+  double d[MAXN], e[MAXN], e2[MAXN], lam[MAXN], lo[MAXN], rq[MAXN];
+  double sg;
   int bs[MAXN], bt[MAXN], loc[MAXN], perm[MAXN];
   unsigned char cnt[2][1024];
   double norm;
 };
 
-__device__ __forceinline__ int sturm_count(const double* d, const double* e, int s, int t, double x, double pivmin) {
-  int c = 0;
+// 1/d: hardware reciprocal refined by one Newton step (relative error ~1e-15)
+__device__ __forceinline__ double rcp_nr(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// number of eigenvalues of the block below x (negative pivots of the LDL^T of T - x); e2 = e^2
+// The reciprocal of a pivot is clamped to +-rmax = 1/pivmin instead of testing the pivot; the
+// sign bit of each pivot is counted with integer arithmetic.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int s, int t, double x, double rmax) {
   double q = d[s] - x;
-  c += q < 0.0;
+  unsigned c = (unsigned)((unsigned long long)__double_as_longlong(q) >> 63);
   for (int j = s + 1; j < t; ++j) {
-    if (fabs(q) < pivmin) q = -pivmin;
-    q = (d[j] - x) - e[j - 1] * e[j - 1] / q;
-    c += q < 0.0;
+    const double r = fmin(fmax(rcp_nr(q), -rmax), rmax);
+    q = fma(-e2[j - 1], r, d[j] - x);
+    c += (unsigned)((unsigned long long)__double_as_longlong(q) >> 63);
   }
-  return c;
+  return (int)c;
 }
 
 // deterministic start vector entry in [-1, 1)
@@ -57,39 +65,61 @@ __device__ __forceinline__ double start_entry(int slot, int i) {
   return (double)(x & 0xFFFFFF) / 8388608.0 - 1.0;
 }
 
-// Pivots of the LDL^T factorisation of T_block - lam (the Sturm recurrence), |D| >= tiny, stored
-// in column `col` of Dm (row stride S).
-__device__ void ldl_pivots(const double* d, const double* e, int s, int t, double lam, double tiny, double* Dm, int S,
-                           int col) {
+// Inverse pivots 1/D_i of the LDL^T factorisation of T_block - lam (|D| >= tiny), stored in column
+// `col` of Dm (row stride S).
+__device__ __forceinline__ void ldl_pivots(const double* d, const double* e2, int s, int t, double lam, double tiny, double* Dm,
+                           int S, int col) {
   double q = d[s] - lam;
   if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
-  Dm[s * S + col] = q;
+  double qi = 1.0 / q;
+  Dm[s * S + col] = qi;
   for (int i = s + 1; i < t; ++i) {
-    q = (d[i] - lam) - e[i - 1] * (e[i - 1] / q);
+    q = (d[i] - lam) - e2[i - 1] * qi;
     if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
-    Dm[i * S + col] = q;
+    qi = 1.0 / q;
+    Dm[i * S + col] = qi;
   }
 }
 
-// (T_block - lam) x = b in place in column `col` of Y, with the pivots from ldl_pivots:
-// forward y_i = b_i - (e_i-1 / D_i-1) y_i-1, then x_i = y_i / D_i - (e_i / D_i) x_i+1.
-__device__ void ldl_solve(const double* e, int s, int t, const double* Dm, double* Y, int S, int col) {
+// (T_block - lam) x = b in place in column `col` of Y, with the inverse pivots from ldl_pivots:
+// forward y_i = b_i - e_i-1 (1/D_i-1) y_i-1, then x_i = (y_i - e_i x_i+1) / D_i.
+__device__ __forceinline__ void ldl_solve(const double* e, int s, int t, const double* Dm, double* Y, int S, int col) {
   double prev = Y[s * S + col];
   for (int i = s + 1; i < t; ++i) {
-    const double y = Y[i * S + col] - (e[i - 1] / Dm[(i - 1) * S + col]) * prev;
+    const double y = Y[i * S + col] - e[i - 1] * Dm[(i - 1) * S + col] * prev;
     Y[i * S + col] = y;
     prev = y;
   }
-  double next = Y[(t - 1) * S + col] / Dm[(t - 1) * S + col];
+  double next = Y[(t - 1) * S + col] * Dm[(t - 1) * S + col];
   Y[(t - 1) * S + col] = next;
   for (int i = t - 2; i >= s; --i) {
-    const double Di = Dm[i * S + col];
-    next = (Y[i * S + col] - e[i] * next) / Di;
+    next = (Y[i * S + col] - e[i] * next) * Dm[i * S + col];
     Y[i * S + col] = next;
   }
 }
 
-__device__ double col_norm2(const double* Y, int S, int s, int t, int col) {
+// sum over the 8 lanes of an aligned lane group (every lane receives the same value)
+__device__ __forceinline__ double group8_sum(double x) {
+  x += __shfl_xor(x, 1, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 4, 64);
+  return x;
+}
+
+// sum_{i<m} a[i*sa] * b[i*sb] with 8 independent partial sums (shorter dependency chains on LDS
+// loads), combined in a fixed order
+__device__ __forceinline__ double dot8(const double* a, int sa, const double* b, int sb, int m) {
+  double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int i = 0;
+  for (; i + 7 < m; i += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = fma(a[(i + u) * sa], b[(i + u) * sb], c[u]);
+  }
+  for (; i < m; ++i) c[0] = fma(a[i * sa], b[i * sb], c[0]);
+  return ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
+}
+
+__device__ __forceinline__ double col_norm2(const double* Y, int S, int s, int t, int col) {
   double r = 0.0;
   for (int i = s; i < t; ++i) r = fma(Y[i * S + col], Y[i * S + col], r);
   return r;
@@ -99,7 +129,7 @@ __device__ double col_norm2(const double* Y, int S, int s, int t, int col) {
 // of the j-th smallest eigenvalue; w: ascending eigenvalues (any memory).  f: global scratch of
 // 2 n^2 doubles (3 n^2 are reserved by the callers).  Must be called by every thread of the workgroup (blockDim.x a multiple of 64,
 // blockDim.x >= n).
-__device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f, Smem& sh) {
+__device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, double* w, double* f, Smem& sh) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   if (n == 1) {
     if (tid == 0) {
@@ -110,43 +140,52 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
     return;
   }
   // ---- 1. tridiagonalisation ----
+  const int grp = tid >> 3, sub = tid & 7;  // 8-lane groups
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;
-    // column k below the diagonal: x; every wave reduces it (no broadcast needed)
-    double sg = 0.0;
-    for (int i = 1 + lane; i < m; i += 64) {
-      const double x = A[(k + 1 + i) * S + k];
-      sg = fma(x, x, sg);
+    const double* colk = A + (size_t)(k + 1) * S + k;  // x_i = colk[i * S]
+    // (a) u_i = sum_{j>=1} A22[i][j] x_j (one 8-lane group per row); |x_1..|^2 by the last group
+    if (grp < m) {
+      const double* row = A + (size_t)(k + 1 + grp) * S + k + 1;
+      double acc = 0.0;
+      for (int j = 1 + sub; j < m; j += 8) acc = fma(row[j], colk[j * S], acc);
+      acc = group8_sum(acc);
+      if (sub == 0) sh.lo[grp] = acc;
+    } else if (grp == (nt >> 3) - 1) {
+      double acc = 0.0;
+      for (int i = 1 + sub; i < m; i += 8) acc = fma(colk[i * S], colk[i * S], acc);
+      acc = group8_sum(acc);
+      if (sub == 0) sh.sg = acc;
     }
-    sg = wave_sum(sg);
-    const double x0 = A[(k + 1) * S + k];
+    __syncthreads();
+    const double sg = sh.sg, x0 = colk[0];
     const bool skip = !(sg > 0.0);
     const double nx = sqrt(x0 * x0 + sg);
     const double alpha = x0 >= 0.0 ? -nx : nx;
-    const double vn = skip ? 1.0 : sqrt(sg + (x0 - alpha) * (x0 - alpha));
-    const double iv = 1.0 / vn;
-    auto vget = [&](int i) -> double { return i == 0 ? (x0 - alpha) * iv : A[(k + 1 + i) * S + k] * iv; };
+    const double iv = skip ? 0.0 : 1.0 / sqrt(sg + (x0 - alpha) * (x0 - alpha));
+    const double v0 = (x0 - alpha) * iv;
+    auto vget = [&](int i) -> double { return i == 0 ? v0 : colk[i * S] * iv; };
+    // p_i = A22[i][0] v0 + iv u_i
+    auto pget = [&](int i) -> double { return fma(A[(size_t)(k + 1 + i) * S + k + 1], v0, iv * sh.lo[i]); };
     if (tid == 0) {
       sh.d[k] = A[k * S + k];
       sh.e[k] = skip ? x0 : alpha;
     }
     if (!skip) {
-      // p = A22 v (one wave per row) into lo[] (reused as the p vector)
-      for (int i = wv; i < m; i += nw) {
-        double acc = 0.0;
-        for (int j = lane; j < m; j += 64) acc = fma(A[(k + 1 + i) * S + k + 1 + j], vget(j), acc);
-        acc = wave_sum(acc);
-        if (lane == 0) sh.lo[i] = acc;
-      }
-      __syncthreads();
+      // (b) beta = v . p, reduced inside every wave (identical in all lanes and waves)
       double bt = 0.0;
-      for (int j = lane; j < m; j += 64) bt = fma(vget(j), sh.lo[j], bt);
+      for (int j = lane; j < m; j += 64) bt = fma(vget(j), pget(j), bt);
       bt = wave_sum(bt);
-      for (int idx = tid; idx < m * m; idx += nt) {
-        const int i = idx / m, j = idx % m;
-        const double vi = vget(i), vj = vget(j);
-        const double wi = 2.0 * sh.lo[i] - 2.0 * bt * vi, wj = 2.0 * sh.lo[j] - 2.0 * bt * vj;
-        A[(k + 1 + i) * S + k + 1 + j] -= vi * wj + wi * vj;
+      // (c) A22 -= v w^T + w v^T, w = 2 p - 2 beta v: rows by wave, columns by lane
+      const int j0 = lane, j1 = lane + 64;
+      const double vj0 = j0 < m ? vget(j0) : 0.0, wj0 = j0 < m ? 2.0 * (pget(j0) - bt * vj0) : 0.0;
+      const double vj1 = j1 < m ? vget(j1) : 0.0, wj1 = j1 < m ? 2.0 * (pget(j1) - bt * vj1) : 0.0;
+      __syncthreads();  // every p_j is read before row 0 of A22 (which holds A22[i][0]) changes
+      for (int i = wv; i < m; i += nw) {
+        const double vi = vget(i), wi = 2.0 * (pget(i) - bt * vi);
+        double* row = A + (size_t)(k + 1 + i) * S + k + 1;
+        if (j0 < m) row[j0] -= vi * wj0 + wi * vj0;
+        if (j1 < m) row[j1] -= vi * wj1 + wi * vj1;
       }
     }
     __syncthreads();
@@ -170,6 +209,7 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
     for (int k = 0; k < n; ++k) {
       const bool cut = k == n - 1 || fabs(sh.e[k]) <= EPS * (fabs(sh.d[k]) + fabs(sh.d[k + 1]));
       if (k < n - 1 && cut) sh.e[k] = 0.0;
+      if (k < n - 1) sh.e2[k] = sh.e[k] * sh.e[k];
       if (cut) {
         for (int q = s; q <= k; ++q) {
           sh.bs[q] = s;
@@ -189,7 +229,7 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
     const int G = nt / n;  // >= 10 for n <= 96 and 1024 threads
     const int slot = tid / G, g = tid % G;
     const bool act = slot < n;
-    double lo = 0.0, hi = 0.0, pivmin = 0.0;
+    double lo = 0.0, hi = 0.0, rmax = 0.0;
     int s = 0, t = 0, li = 0;
     if (act) {
       s = sh.bs[slot];
@@ -202,24 +242,26 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
         const double r = (j > s ? fabs(sh.e[j - 1]) : 0.0) + (j < t - 1 ? fabs(sh.e[j]) : 0.0);
         lo = fmin(lo, sh.d[j] - r);
         hi = fmax(hi, sh.d[j] + r);
-        if (j < t - 1) emax = fmax(emax, sh.e[j] * sh.e[j]);
+        if (j < t - 1) emax = fmax(emax, sh.e2[j]);
       }
       const double span = fmax(hi - lo, EPS * sh.norm);
       lo -= 1e-12 * span + 1e-300;
       hi += 1e-12 * span + 1e-300;
-      pivmin = fmax(1e-300, 1e-290 * emax);
+      rmax = 1.0 / fmax(1e-300, 1e-290 * emax);  // bound on |1 / pivot|
     }
-    for (int r = 0; r < ROUNDS; ++r) {
+    // the interval shrinks (G+1)-fold per round; 2^-35 of its width (ln = 24.3) is enough because
+    // inverse iteration refines the vector and the Rayleigh quotient then gives the eigenvalue
+    const int rounds = (int)ceil(24.3 / log((double)(G + 1))) + 1;
+    const double ig = 1.0 / (double)(G + 1);
+    for (int r = 0; r < rounds; ++r) {
       unsigned char* cb = sh.cnt[r & 1];
-      if (act) {
-        const double x = lo + (hi - lo) * (double)(g + 1) / (double)(G + 1);
-        cb[slot * G + g] = (unsigned char)sturm_count(sh.d, sh.e, s, t, x, pivmin);
-      }
+      const double wd = (hi - lo) * ig;
+      if (act) cb[slot * G + g] = (unsigned char)sturm_count(sh.d, sh.e2, s, t, fma(wd, (double)(g + 1), lo), rmax);
       __syncthreads();
       if (act) {
         double nlo = lo, nhi = hi;
         for (int q = 0; q < G; ++q) {
-          const double x = lo + (hi - lo) * (double)(q + 1) / (double)(G + 1);
+          const double x = fma(wd, (double)(q + 1), lo);
           if (cb[slot * G + q] <= li)
             nlo = fmax(nlo, x);
           else
@@ -247,9 +289,10 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
         for (int a = slot; a <= last; ++a) {
           if (t - s == 1) {
             Z[s * S + a] = 1.0;
+            sh.rq[a] = sh.d[s];
             continue;
           }
-          ldl_pivots(sh.d, sh.e, s, t, sh.lam[a], tiny, A, S, a);
+          ldl_pivots(sh.d, sh.e2, s, t, sh.lam[a], tiny, A, S, a);
           for (int i = s; i < t; ++i) Z[i * S + a] = start_entry(a, i);
           for (int it = 0; it < 3; ++it) {
             const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
@@ -262,7 +305,15 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
             }
           }
           const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
+          double rq = 0.0;
           for (int i = s; i < t; ++i) Z[i * S + a] *= nr;
+          for (int i = s; i < t; ++i) {
+            double ty = sh.d[i] * Z[i * S + a];
+            if (i > s) ty = fma(sh.e[i - 1], Z[(i - 1) * S + a], ty);
+            if (i + 1 < t) ty = fma(sh.e[i], Z[(i + 1) * S + a], ty);
+            rq = fma(Z[i * S + a], ty, rq);
+          }
+          sh.rq[a] = rq;  // Rayleigh quotient: the eigenvalue to ~eps |T|
         }
       }
     }
@@ -271,7 +322,7 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
   // ---- 5. sort ascending (rank; ties by slot), permute columns through A, back-transform ----
   for (int i = tid; i < n; i += nt) {
     int r = 0;
-    for (int j = 0; j < n; ++j) r += (sh.lam[j] < sh.lam[i]) || (sh.lam[j] == sh.lam[i] && j < i);
+    for (int j = 0; j < n; ++j) r += (sh.rq[j] < sh.rq[i]) || (sh.rq[j] == sh.rq[i] && j < i);
     sh.perm[r] = i;
   }
   __syncthreads();
@@ -282,23 +333,28 @@ __device__ void symeig(int n, double* A, double* Z, int S, double* w, double* f,
     g[idx] = Z[i * S + sh.perm[c]];
     if (i > c) A[i * S + c] = f[idx];
   }
-  for (int i = tid; i < n; i += nt) w[i] = sh.lam[sh.perm[i]];
+  for (int i = tid; i < n; i += nt) w[i] = sh.rq[sh.perm[i]];
   __syncthreads();
   for (int idx = tid; idx < n * n; idx += nt) Z[(idx / n) * S + idx % n] = g[idx];
   __syncthreads();
   for (int k = n - 3; k >= 0; --k) {
     const int m = n - k - 1;
-    // sh.lo[c] = v^T Z[k+1:, c]  (one wave per column)
-    for (int c = wv; c < n; c += nw) {
+    const double* vk = A + (size_t)(k + 1) * S + k;  // v_i = vk[i * S]
+    // sh.lo[c] = v^T Z[k+1:, c], one thread per column
+    if (grp < n) {
       double acc = 0.0;
-      for (int i = lane; i < m; i += 64) acc = fma(A[(k + 1 + i) * S + k], Z[(k + 1 + i) * S + c], acc);
-      acc = wave_sum(acc);
-      if (lane == 0) sh.lo[c] = acc;
+      for (int i = sub; i < m; i += 8) acc = fma(vk[i * S], Z[(size_t)(k + 1 + i) * S + grp], acc);
+      acc = group8_sum(acc);
+      if (sub == 0) sh.lo[grp] = acc;
     }
     __syncthreads();
-    for (int idx = tid; idx < m * n; idx += nt) {
-      const int i = idx / n, c = idx % n;
-      Z[(k + 1 + i) * S + c] -= 2.0 * A[(k + 1 + i) * S + k] * sh.lo[c];
+    const int c0 = lane, c1 = lane + 64;
+    const double s0 = c0 < n ? 2.0 * sh.lo[c0] : 0.0, s1 = c1 < n ? 2.0 * sh.lo[c1] : 0.0;
+    for (int i = wv; i < m; i += nw) {
+      const double vi = vk[i * S];
+      double* row = Z + (size_t)(k + 1 + i) * S;
+      if (c0 < n) row[c0] -= vi * s0;
+      if (c1 < n) row[c1] -= vi * s1;
     }
     __syncthreads();
   }

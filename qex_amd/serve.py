@@ -9,7 +9,8 @@ The dispatcher runs in one host thread and never blocks on the device:
     ``max_batch``), one ``qex1d_kohn_sham`` per batch, on stream A, while the previous batch may
     still be running (at most ``depth_1d`` batches in flight);
   * 3D: each request is one enqueued solve (integrals, grid, AO values, SCF cycles without host
-    synchronisation) on stream B, with at most ``depth_3d`` in flight;
+    synchronisation) on one of ``streams_3d`` streams (the least loaded), with at most ``depth_3d``
+    in flight per stream, so that the one-workgroup SCF kernels of several molecules overlap;
   * completions are detected with events (``query``), then results are read back.
 Admission holds a request back while the device memory its solve needs, by the estimate
 n_grid * n_ao * 8 + n_ao^4 / 4 * 8 + 16 * n_ao^2 * 8 bytes (3D) or T * N * 8 * 6 bytes (1D),
@@ -90,8 +91,8 @@ class Server:
     """runner_1d: an ``engine.BatchRunner``; solver_3d(mol) -> SCF object with ``kernel_async`` /
     ``collect`` (e.g. ``lambda m: qex_amd.td.scf.RKS(m)`` with its settings)."""
 
-    def __init__(self, runner_1d, solver_3d, max_batch=64, depth_1d=2, depth_3d=2, memory_budget=64 << 30,
-                 grid_points_per_atom=5500):
+    def __init__(self, runner_1d, solver_3d, max_batch=64, depth_1d=2, depth_3d=2, streams_3d=3,
+                 memory_budget=64 << 30, grid_points_per_atom=5500):
         self.runner = runner_1d
         self.solver_3d = solver_3d
         self.max_batch = min(max_batch, runner_1d.max_batch) if runner_1d is not None else max_batch
@@ -99,7 +100,7 @@ class Server:
         self.memory_budget = memory_budget
         self.grid_points_per_atom = grid_points_per_atom
         self.stream_1d = torch.cuda.Stream()
-        self.stream_3d = torch.cuda.Stream()
+        self.streams_3d = [torch.cuda.Stream() for _ in range(max(1, streams_3d))]
 
     def _dispatch_1d(self, reqs):
         loc = np.zeros((len(reqs), max(len(r.payload[0]) for r in reqs)))
@@ -116,18 +117,19 @@ class Server:
             ev.record(self.stream_1d)
         return out, ev
 
-    def _dispatch_3d(self, req):
-        with torch.cuda.stream(self.stream_3d):
+    def _dispatch_3d(self, req, stream):
+        with torch.cuda.stream(stream):
             mf = self.solver_3d(req.payload)
             mf.kernel_async()
             ev = torch.cuda.Event()
-            ev.record(self.stream_3d)
+            ev.record(stream)
         return mf, ev
 
     def run(self, requests, poll_sleep=50e-6) -> ServeStats:
         reqs = sorted(requests, key=lambda r: r.arrival)
         pend_1d, pend_3d = deque(), deque()
-        inflight_1d, inflight_3d = deque(), deque()
+        inflight_1d = deque()
+        inflight_3d = [deque() for _ in self.streams_3d]
         stats = ServeStats()
         i = 0
         used = 0
@@ -140,7 +142,7 @@ class Server:
                 i += 1
             progressed = False
             # completions (in order per stream)
-            for q, kind in ((inflight_1d, "1d"), (inflight_3d, "3d")):
+            for q, kind in [(inflight_1d, "1d")] + [(q3, "3d") for q3 in inflight_3d]:
                 while q and q[0].event.query():
                     f = q.popleft()
                     done = time.perf_counter() - t0
@@ -172,17 +174,19 @@ class Server:
                 stats.batches_1d += 1
                 stats.batch_sizes.append(len(batch))
                 progressed = True
-            if pend_3d and len(inflight_3d) < self.depth_3d:
+            k3 = min(range(len(inflight_3d)), key=lambda q: len(inflight_3d[q]))
+            if pend_3d and len(inflight_3d[k3]) < self.depth_3d:
                 r = pend_3d[0]
                 need = _bytes_3d(r.payload, r.payload.natm * self.grid_points_per_atom)
-                if used + need <= self.memory_budget or not (inflight_1d or inflight_3d):
+                busy = inflight_1d or any(inflight_3d)
+                if used + need <= self.memory_budget or not busy:
                     pend_3d.popleft()
-                    mf, ev = self._dispatch_3d(r)
+                    mf, ev = self._dispatch_3d(r, self.streams_3d[k3])
                     used += need
-                    inflight_3d.append(_Inflight(ev, [r], mf, need))
+                    inflight_3d[k3].append(_Inflight(ev, [r], mf, need))
                     progressed = True
             if not progressed:
-                if i < total and not (pend_1d or pend_3d or inflight_1d or inflight_3d):
+                if i < total and not (pend_1d or pend_3d or inflight_1d or any(inflight_3d)):
                     wait = reqs[i].arrival - (time.perf_counter() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 1e-3))

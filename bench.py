@@ -283,6 +283,11 @@ def run_c4(args):
     from qex_amd.td import scf as tscf
 
     lib = _lib.load()
+    if args.calibrate_copy:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     mols = [gto.synthetic_chain(n, 100 + n) for n in C4_CHAINS]
     for m in mols:
         m.device_basis()  # basis arrays resident in HBM before the timed region

@@ -103,6 +103,9 @@ typedef struct qex1d_outputs {
   double* eigenvalues;       /* optional [B][T][max_electrons/2+1], may be NULL */
   uint64_t* stats;           /* optional [2], accumulated (caller zeroes): Sturm multisection rounds
                                 executed over all eigenpairs, eigenpairs solved; may be NULL */
+  double* orbitals;          /* optional [B][T][K][N], K = max_electrons/2+1: the orbitals of each
+                                iteration (2-norm 1; slots beyond N_e/2+1 untouched); may be NULL.
+                                Needed by qex1d_kohn_sham_vjp. */
 } qex1d_outputs;
 
 /* Workspace bytes needed by qex1d_kohn_sham for this problem (0 on bad arguments). */
@@ -151,6 +154,19 @@ int qex_profile_read(double* ms_per_kind, int32_t* launches_per_kind, int32_t nk
 const char* qex_last_error(void);
 int32_t qex_abi_version(void);
 
+
+/* Reverse pass of qex1d_kohn_sham for the scan loop with a LOCAL_MLP functional (1 or 2 hidden
+ * layers, widths <= 64), no reflection and density_mse_converge_tolerance < 0: given the forward's
+ * inputs, its outputs density / hartree_potential / xc_potential / orbitals, and cotangents of the
+ * stored energies d_total_energy [B][T] and densities d_density [B][T][N] (either may be NULL), writes
+ * d_xc_params [P] = the gradient with respect to the packed MLP parameters, summed over the batch.
+ * Replaces jax.grad through jit_scf.kohn_sham (qedft/train/od/train.py:250-323, the loss of
+ * od/trainer.py:139-178). */
+size_t qex1d_kohn_sham_vjp_workspace_bytes(const qex1d_problem* p);
+int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* forward,
+                        const double* d_total_energy, const double* d_density, double* d_xc_params, void* workspace,
+                        size_t workspace_bytes, void* stream);
+const char* qex1d_vjp_last_error(void);
 
 /* ============================================================================================
  * 3D restricted Kohn-Sham (LDA) / Hartree-Fock on an atom-centred grid, Cartesian s/p basis.

@@ -78,6 +78,7 @@ class Qex1dOutputs(ctypes.Structure):
         ("converged", ctypes.c_void_p),
         ("eigenvalues", ctypes.c_void_p),
         ("stats", ctypes.c_void_p),
+        ("orbitals", ctypes.c_void_p),
     ]
 
 
@@ -128,6 +129,10 @@ SIGNATURES = {
     "qex1d_hartree_potential": (_I, [_I, _I, _D, _P, _P, _P, _P]),
     "qex_profile_enable": (_I, [_I, _I]),
     "qex_profile_read": (_I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), _I]),
+    "qex1d_kohn_sham_vjp_workspace_bytes": (_S, [ctypes.POINTER(Qex1dProblem)]),
+    "qex1d_kohn_sham_vjp": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
+                                 ctypes.POINTER(Qex1dOutputs), _P, _P, _P, _P, _S, _P]),
+    "qex1d_vjp_last_error": (ctypes.c_char_p, []),
     "qex3d_last_error": (ctypes.c_char_p, []),
     "qex3d_int1e": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _P, _P]),
     "qex3d_eri_workspace_bytes": (_S, [ctypes.POINTER(Qex3dBasis)]),
@@ -147,7 +152,7 @@ SIGNATURES = {
 }
 
 KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize", "int1e", "eri", "grid", "eval_ao", "rho", "vxc",
-                "coulomb", "scf_fock", "scf_eigh", "scf_energy", "scf_other", "nn_global"]
+                "coulomb", "scf_fock", "scf_eigh", "scf_energy", "scf_other", "nn_global", "adjoint"]
 
 _lib = None
 
@@ -170,6 +175,12 @@ def load() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def check_vjp(rc: int, what: str) -> None:
+    if rc != QEX_OK:
+        msg = load().qex1d_vjp_last_error()
+        raise QexError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
 
 def check3(rc: int, what: str) -> None:

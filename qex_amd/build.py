@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["csrc/qex1d.hip", "csrc/qex3d.hip"]
+SOURCES = ["csrc/qex1d.hip", "csrc/qex1d_adj.hip", "csrc/qex3d.hip"]
 LIB = os.path.join(HERE, "libqex.so")
 
 HIPCC_FLAGS = [

@@ -212,6 +212,7 @@ struct Dev1 {
   double* gap;
   uint8_t* conv;
   double* eigs_out;
+  double* orb_out;  // optional [B][T][K][N]
   // workspace
   double* o_raw;     // [B][N] output density of the last computed iteration
   double* o_xc;      // [B][N] density the deferred XC energy is evaluated at
@@ -1023,6 +1024,11 @@ __global__ __launch_bounds__(NT) void k_density(Dev1 a, int k, double alpha_k) {
     }
     if (a.eigs_out && tid < a.K)
       a.eigs_out[((size_t)s * T + k) * a.K + tid] = a.eigs_out[((size_t)s * T + k - 1) * a.K + tid];
+    if (a.orb_out) {
+      const int kk = a.ne[s] / 2 + 1;
+      for (int t = tid; t < kk * N; t += NT)
+        a.orb_out[((size_t)s * T + k) * a.K * N + t] = a.orb_out[((size_t)s * T + k - 1) * a.K * N + t];
+    }
     return;
   }
   const int ne = a.ne[s];
@@ -1033,6 +1039,9 @@ __global__ __launch_bounds__(NT) void k_density(Dev1 a, int k, double alpha_k) {
   __syncthreads();
   double dn[PPT];
   orbitals_to_density(a.vec + (size_t)s * a.K * N, N, keig, ne, a.dx, hd, a.Tb, a.Tc, lam, dn, red);
+  if (a.orb_out)
+    for (int t = tid; t < keig * N; t += NT)
+      a.orb_out[((size_t)s * T + k) * a.K * N + t] = a.vec[(size_t)s * a.K * N + t];
   double eeig = 0.0;
   for (int jj = 0; jj < nocc; ++jj) eeig += (double)min(2, ne - 2 * jj) * lam[jj];
   const double gap = lam[ne / 2] - lam[(ne - 1) / 2];
@@ -1433,6 +1442,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   a.gap = out->gap;
   a.conv = out->converged;
   a.eigs_out = out->eigenvalues;
+  a.orb_out = out->orbitals;
   a.o_raw = (double*)(ws + L.o_raw);
   a.o_xc = (double*)(ws + L.o_xc);
   a.vks = (double*)(ws + L.vks);

@@ -1,0 +1,770 @@
+// qex1d_adj.hip -- reverse pass of the batched 1D Kohn-Sham scan loop with respect to the
+// parameters of a windowed LocalMLP XC functional (the gradient that jax.grad takes through
+// jit_scf.kohn_sham in qedft/train/od/train.py:250-323).
+//
+// Given the forward outputs of qex1d_kohn_sham (stored densities, v_H, v_xc and the orbitals of
+// every iteration) and cotangents of the stored energies and densities, iteration k is reversed
+// as (n_k: input density, n_out: unmixed output, psi_j: occupied orbitals, 2-norm 1):
+//   k_adj_mix        nbar_k, dbar (the mixing n_k+1 = n_k + alpha_k mean(last m differences))
+//   energy           nob += c (-v_KS + v_H(n_out) + v_xc(n_out) + v_ext), thetabar += c dF(n_out)/dtheta,
+//                    c = dE_k dx (the eigenvalue sum and -dx v_KS.n_out cancel at fixed n_out)
+//   k_adj_eig        vbar = sum_j occ_j (2/dx) psi_j * y_j with y_j = -(H - eps_j)^-1 Q (nob psi_j)
+//                    (Q projects out the occupied orbitals; a banded LU with partial pivoting in
+//                    LDS, then Q again) minus the pairs of occupied orbitals with unequal occupations
+//   Hartree + XC     nbar += dx U vbar + H_F vbar, thetabar += d(vbar . grad F)/dtheta, F = sum e(n) n
+// The MLP terms come from reverse mode over the network's forward and forward-tangent passes, one
+// wavefront per point with one lane per hidden unit; each wavefront accumulates its parameter
+// partials in registers and adds them to its own slot, and the slots are summed in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "prof.h"
+#include "qex.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int ADJ_SLOTS = 256;  // wavefronts of the MLP adjoint launch = parameter-partial slots
+constexpr int HMAX = 64;        // hidden width (one lane per unit)
+constexpr int WMAXW = QEX1D_MAX_MLP_WINDOW;
+thread_local char g_erra[512] = "";
+
+int faila(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_erra, sizeof(g_erra), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+__device__ __forceinline__ double wsum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ double bsum(double x, double* red) {
+  x = wsum(x);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+  __syncthreads();
+  return s;
+}
+
+// activation and its first two derivatives
+__device__ void act3(int act, double z, double* a, double* d1, double* d2) {
+  switch (act) {
+    case QEX_ACT_SILU: {
+      const double s = 1.0 / (1.0 + exp(-z));
+      *a = z * s;
+      *d1 = s * (1.0 + z * (1.0 - s));
+      *d2 = s * (1.0 - s) * (2.0 + z * (1.0 - 2.0 * s));
+      break;
+    }
+    case QEX_ACT_TANH: {
+      const double t = tanh(z);
+      *a = t;
+      *d1 = 1.0 - t * t;
+      *d2 = -2.0 * t * (1.0 - t * t);
+      break;
+    }
+    case QEX_ACT_SOFTPLUS: {
+      const double s = 1.0 / (1.0 + exp(-z));
+      *a = z > 0.0 ? z + log1p(exp(-z)) : log1p(exp(z));
+      *d1 = s;
+      *d2 = s * (1.0 - s);
+      break;
+    }
+    case QEX_ACT_SIGMOID: {
+      const double s = 1.0 / (1.0 + exp(-z));
+      *a = s;
+      *d1 = s * (1.0 - s);
+      *d2 = s * (1.0 - s) * (1.0 - 2.0 * s);
+      break;
+    }
+    case QEX_ACT_RELU:
+      *a = z > 0.0 ? z : 0.0;
+      *d1 = z > 0.0 ? 1.0 : 0.0;
+      *d2 = 0.0;
+      break;
+    default: {  // GELU, tanh form
+      const double k = 0.7978845608028654, c = 0.044715;
+      const double t = tanh(k * (z + c * z * z * z));
+      const double du = k * (1.0 + 3.0 * c * z * z);
+      const double s2 = 1.0 - t * t;
+      *a = 0.5 * z * (1.0 + t);
+      *d1 = 0.5 * (1.0 + t) + 0.5 * z * s2 * du;
+      *d2 = s2 * du + 0.5 * z * (-2.0 * t * s2 * du * du + s2 * k * 6.0 * c * z);
+      break;
+    }
+  }
+}
+
+struct Adj {
+  int B, N, T, K, M, W, H1, H2, NL, act, P;
+  double dx, norm;
+  int w1, b1, w2, b2, w3, b3;  // parameter offsets (w2/b2 unused when NL == 1)
+  const double* table;
+  const double* vext;
+  const double* n0;
+  const int* ne;
+  const double* mlp;
+  const double* dens;  // [B][T][N] stored (mixed) densities
+  const double* vh;    // [B][T][N] v_H(n_k)
+  const double* vxc;   // [B][T][N] v_xc(n_k)
+  const double* orb;   // [B][T][K][N]
+  const double* dE;    // [B][T] or null
+  const double* dn;    // [B][T][N] or null
+  // workspace
+  double* nbar;  // [B][N]
+  double* nob;   // [B][N]
+  double* nout;  // [B][N]
+  double* vbar;  // [B][N]
+  double* tmp;   // [B][N]
+  double* dbar;  // [B][T][N]
+  double* xbar;  // [B][N][W]
+  double* fv;    // [B][N]
+  double* fdv;   // [B][N]
+  double* thp;   // [ADJ_SLOTS][P]
+};
+
+__device__ __forceinline__ const double* input_density(const Adj& a, int s, int k) {
+  return k == 0 ? a.n0 + (size_t)s * a.N : a.dens + ((size_t)s * a.T + k - 1) * a.N;
+}
+
+__device__ __forceinline__ double occupation(int ne, int j) { return j < (ne + 1) / 2 ? (double)min(2, ne - 2 * j) : 0.0; }
+
+__global__ void k_zero_a(double* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// mixing and difference cotangents; one workgroup per system
+__global__ __launch_bounds__(NT) void k_adj_mix(Adj a, int k, double alpha_k) {
+  const int s = blockIdx.x, N = a.N, T = a.T;
+  const int cnt = min(a.M, k + 1);
+  const double wgt = alpha_k / (double)cnt;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    double g = a.nbar[(size_t)s * N + t];
+    if (a.dn) g += a.dn[((size_t)s * T + k) * N + t];
+    for (int j = k - cnt + 1; j <= k; ++j) a.dbar[((size_t)s * T + j) * N + t] += wgt * g;
+    const double dk = a.dbar[((size_t)s * T + k) * N + t];
+    a.nob[(size_t)s * N + t] = dk;
+    a.nbar[(size_t)s * N + t] = g - dk;
+    // unmixed output density of iteration k from its orbitals
+    double no = 0.0;
+    const int ne = a.ne[s];
+    for (int j = 0; j < (ne + 1) / 2; ++j) {
+      const double x = a.orb[(((size_t)s * T + k) * a.K + j) * N + t];
+      no = fma(occupation(ne, j), x * x, no);
+    }
+    a.nout[(size_t)s * N + t] = no / a.dx;
+  }
+}
+
+// out[s] = dx * U in[s] (Toeplitz interaction matrix from its first row)
+__global__ __launch_bounds__(NT) void k_toeplitz(int N, double dx, const double* table, const double* in,
+                                                 double* out) {
+  extern __shared__ double sm[];
+  const int s = blockIdx.y;
+  double* tab = sm;
+  double* x = sm + N;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    tab[t] = table[t];
+    x[t] = in[(size_t)s * N + t];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= N) return;
+  double acc = 0.0;
+  for (int j = 0; j < N; ++j) acc = fma(tab[abs(i - j)], x[j], acc);
+  out[(size_t)s * N + i] = dx * acc;
+}
+
+// MLP second-order adjoint.  Per point p = (s, i): x = window(xin[s])_i / norm, tangent
+// xt = window(tin[s])_i / norm (zero without tin), fbar = sf[s] * F1[s][i], fdbar = F2 ? F2[s][i] : 0.
+// Outputs per point: xbar (dPhi/dx), f, fdot; parameter gradient of Phi = fbar f + fdbar fdot
+// accumulated into the wavefront's slot.
+template <int NL>
+__global__ __launch_bounds__(NT, 1) void k_adj_mlp(Adj a, const double* xin, const double* tin, const double* F1,
+                                                   const double* sf, const double* F2) {
+  extern __shared__ double sm[];
+  const int W = a.W, H1 = a.H1, H2 = a.H2, HL = NL == 2 ? H2 : H1;
+  // weights in LDS
+  double* sW1 = sm;                       // W x H1
+  double* sb1 = sW1 + W * H1;             // H1
+  double* sW2 = sb1 + H1;                 // H1 x H2 (NL == 2)
+  double* sb2 = sW2 + (NL == 2 ? H1 * H2 : 0);
+  double* sW3 = sb2 + (NL == 2 ? H2 : 0);  // HL
+  double* sW1t = sW3 + HL;                 // H1 x W (transposed W1)
+  double* sW2t = sW1t + W * H1;            // H2 x H1 (transposed W2, NL == 2)
+  double* wbuf = sW2t + (NL == 2 ? H1 * H2 : 0);  // per wave: 6 x 64
+  for (int t = threadIdx.x; t < W * H1; t += NT) {
+    sW1[t] = a.mlp[a.w1 + t];
+    sW1t[(t % H1) * W + t / H1] = a.mlp[a.w1 + t];
+  }
+  for (int t = threadIdx.x; t < H1; t += NT) sb1[t] = a.mlp[a.b1 + t];
+  if (NL == 2) {
+    for (int t = threadIdx.x; t < H1 * H2; t += NT) {
+      sW2[t] = a.mlp[a.w2 + t];
+      sW2t[(t % H2) * H1 + t / H2] = a.mlp[a.w2 + t];
+    }
+    for (int t = threadIdx.x; t < H2; t += NT) sb2[t] = a.mlp[a.b2 + t];
+  }
+  for (int t = threadIdx.x; t < HL; t += NT) sW3[t] = a.mlp[a.w3 + t];
+  __syncthreads();
+  const double b3 = a.mlp[a.b3];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int slot = blockIdx.x * (NT / 64) + wv;
+  double* sx = wbuf + wv * 6 * 64;  // window
+  double* sxt = sx + 64;            // tangent window
+  double* sa = sxt + 64;            // a1 / zb (reused)
+  double* sat = sa + 64;            // at1 / zdb
+  double* sz = sat + 64;            // zb1
+  const int h = W / 2, N = a.N;
+  const double inorm = 1.0 / a.norm;
+  // parameter partials of this lane (column `lane` of each weight matrix)
+  double gW1[WMAXW], gW2[HMAX];
+#pragma unroll
+  for (int o = 0; o < WMAXW; ++o) gW1[o] = 0.0;
+#pragma unroll
+  for (int v = 0; v < HMAX; ++v) gW2[v] = 0.0;
+  double gb1 = 0.0, gb2 = 0.0, gW3 = 0.0, gb3 = 0.0;
+  const long npts = (long)a.B * N;
+  for (long p = slot; p < npts; p += ADJ_SLOTS) {
+    const int s = (int)(p / N), i = (int)(p % N);
+    const double* xs = xin + (size_t)s * N;
+    if (lane < W) {
+      const int j = i + lane - h;
+      sx[lane] = (j >= 0 && j < N) ? xs[j] * inorm : 0.0;
+      sxt[lane] = (tin && j >= 0 && j < N) ? tin[(size_t)s * N + j] * inorm : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const double fbar = sf[s] * F1[(size_t)s * N + i];
+    const double fdbar = F2 ? F2[(size_t)s * N + i] : 0.0;
+    // layer 1 (lane u)
+    double z1 = 0.0, zt1 = 0.0, a1 = 0.0, d11 = 0.0, d21 = 0.0, at1 = 0.0;
+    if (lane < H1) {
+      z1 = sb1[lane];
+      for (int o = 0; o < W; ++o) {
+        z1 = fma(sx[o], sW1[o * H1 + lane], z1);
+        zt1 = fma(sxt[o], sW1[o * H1 + lane], zt1);
+      }
+      act3(a.act, z1, &a1, &d11, &d21);
+      at1 = d11 * zt1;
+    }
+    double aL = a1, atL = at1, d1L = d11, d2L = d21, ztL = zt1;
+    double zt2 = 0.0, a2 = 0.0, d12 = 0.0, d22 = 0.0, at2 = 0.0;
+    if (NL == 2) {
+      if (lane < H1) {
+        sa[lane] = a1;
+        sat[lane] = at1;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane < H2) {
+        double z2 = sb2[lane];
+        for (int v = 0; v < H1; ++v) {
+          z2 = fma(sa[v], sW2[v * H2 + lane], z2);
+          zt2 = fma(sat[v], sW2[v * H2 + lane], zt2);
+        }
+        act3(a.act, z2, &a2, &d12, &d22);
+        at2 = d12 * zt2;
+      }
+      aL = a2;
+      atL = at2;
+      d1L = d12;
+      d2L = d22;
+      ztL = zt2;
+    }
+    const double w3 = lane < HL ? sW3[lane] : 0.0;
+    const double f = wsum(aL * w3) + b3;
+    const double fd = wsum(atL * w3);
+    // reverse: output layer
+    if (lane < HL) gW3 = fma(fbar, aL, fma(fdbar, atL, gW3));
+    if (lane == 0) gb3 += fbar;
+    const double hb = fbar * w3, hdb = fdbar * w3;
+    double zdb = hdb * d1L;
+    double zb = fma(hb, d1L, hdb * d2L * ztL);
+    if (lane >= HL) {
+      zdb = 0.0;
+      zb = 0.0;
+    }
+    double zb1 = zb, zdb1 = zdb;
+    if (NL == 2) {
+      // W2 column `lane` (unit u = lane of layer 2): += a1_v zb_u + at1_v zdb_u
+#pragma unroll
+      for (int v = 0; v < HMAX; ++v)
+        if (v < H1) gW2[v] = fma(sa[v], zb, fma(sat[v], zdb, gW2[v]));
+      gb2 += zb;
+      __builtin_amdgcn_wave_barrier();
+      sa[lane] = zb;  // a1 / at1 no longer needed
+      sat[lane] = zdb;
+      __builtin_amdgcn_wave_barrier();
+      double hb1 = 0.0, hdb1 = 0.0;
+      if (lane < H1) {
+        for (int u = 0; u < H2; ++u) {
+          hb1 = fma(sa[u], sW2t[u * H1 + lane], hb1);
+          hdb1 = fma(sat[u], sW2t[u * H1 + lane], hdb1);
+        }
+      }
+      zdb1 = hdb1 * d11;
+      zb1 = fma(hb1, d11, hdb1 * d21 * zt1);
+      if (lane >= H1) {
+        zdb1 = 0.0;
+        zb1 = 0.0;
+      }
+    }
+    // W1 column `lane`: += x_o zb1 + xt_o zdb1
+#pragma unroll
+    for (int o = 0; o < WMAXW; ++o)
+      if (o < W) gW1[o] = fma(sx[o], zb1, fma(sxt[o], zdb1, gW1[o]));
+    gb1 += zb1;
+    sz[lane] = zb1;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < W) {
+      double xb = 0.0;
+      for (int u = 0; u < H1; ++u) xb = fma(sz[u], sW1t[u * W + lane], xb);
+      a.xbar[(size_t)p * W + lane] = xb;
+    }
+    if (lane == 0) {
+      a.fv[p] = f;
+      a.fdv[p] = fd;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // add this wavefront's partials to its slot (slots are owned: no races)
+  double* g = a.thp + (size_t)slot * a.P;
+  if (lane < H1) {
+#pragma unroll
+    for (int o = 0; o < WMAXW; ++o)
+      if (o < W) g[a.w1 + o * H1 + lane] += gW1[o];
+    g[a.b1 + lane] += gb1;
+  }
+  if (NL == 2 && lane < H2) {
+#pragma unroll
+    for (int v = 0; v < HMAX; ++v)
+      if (v < H1) g[a.w2 + v * H2 + lane] += gW2[v];
+    g[a.b2 + lane] += gb2;
+  }
+  if (lane < HL) g[a.w3 + lane] += gW3;
+  if (lane == 0) g[a.b3] += gb3;
+}
+
+// dst[s][i] += sa[s] * (A1[s][i] + F[s][i]) + A2[s][i] + window^T(xbar)_i / norm (A1, A2 optional)
+__global__ void k_adj_gather(Adj a, double* dst, const double* A1, const double* A2, const double* F,
+                             const double* sa) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.B * a.N) return;
+  const int s = (int)(idx / a.N), i = (int)(idx % a.N), N = a.N, W = a.W, h = W / 2;
+  double acc = 0.0;
+  for (int o = 0; o < W; ++o) {
+    const int q = i - o + h;  // point whose window slot o reads n_i
+    if (q >= 0 && q < N) acc += a.xbar[((size_t)s * N + q) * W + o];
+  }
+  double v = acc / a.norm;
+  const double c = sa ? sa[s] : 1.0;
+  double direct = F[idx];
+  if (A1) direct += A1[idx];
+  v = fma(c, direct, v);
+  if (A2) v += A2[idx];
+  dst[idx] += v;
+}
+
+// energy term: nob += c (-(v_H(n_k) + v_xc(n_k)) + v_H(n_out)) with c = dE dx (the v_xc(n_out) part
+// is added by k_adj_gather from the MLP pass at n_out)
+__global__ void k_adj_energy(Adj a, int k, const double* cs) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.B * a.N) return;
+  const int s = (int)(idx / a.N), i = (int)(idx % a.N);
+  const size_t kk = ((size_t)s * a.T + k) * a.N + i;
+  a.nob[idx] += cs[s] * (a.tmp[idx] - a.vh[kk] - a.vxc[kk]);
+}
+
+__global__ void k_adj_scales(Adj a, int k, double* cs, double* ones) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.B) return;
+  cs[s] = a.dE ? a.dE[(size_t)s * a.T + k] * a.dx : 0.0;
+  ones[s] = 1.0;
+}
+
+// eigen adjoint, one workgroup per system.  LDS: H diagonal, r, y, and the 7-wide band rows of the
+// LU factorisation (row r holds columns base[r] .. base[r] + 6).
+__global__ __launch_bounds__(NT) void k_adj_eig(Adj a, int k, double Td, double Tb, double Tc) {
+  extern __shared__ double sm[];
+  __shared__ double red[NT / 64];
+  __shared__ double eps[QEX1D_MAX_ELECTRONS / 2 + 1];
+  const int s = blockIdx.x, N = a.N, T = a.T;
+  double* hd = sm;       // N
+  double* r = hd + N;    // N
+  double* y = r + N;     // N
+  double* vb = y + N;    // N
+  double* R = vb + N;    // N x 7
+  int* base = (int*)(R + (size_t)7 * N);
+  const int ne = a.ne[s], nocc = (ne + 1) / 2;
+  const double* vxc = a.vxc + ((size_t)s * T + k) * N;
+  const double* vh = a.vh + ((size_t)s * T + k) * N;
+  const double* ve = a.vext + (size_t)s * N;
+  const double* psi = a.orb + ((size_t)s * T + k) * a.K * N;
+  const double* nob = a.nob + (size_t)s * N;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    hd[t] = Td + ve[t] + vh[t] + vxc[t];
+    vb[t] = 0.0;
+  }
+  __syncthreads();
+  // eigenvalues as Rayleigh quotients of the stored orbitals
+  for (int j = 0; j < nocc; ++j) {
+    const double* pj = psi + (size_t)j * N;
+    double q = 0.0;
+    for (int t = threadIdx.x; t < N; t += NT) {
+      double hx = hd[t] * pj[t];
+      if (t >= 1) hx = fma(Tb, pj[t - 1], hx);
+      if (t + 1 < N) hx = fma(Tb, pj[t + 1], hx);
+      if (t >= 2) hx = fma(Tc, pj[t - 2], hx);
+      if (t + 2 < N) hx = fma(Tc, pj[t + 2], hx);
+      q = fma(pj[t], hx, q);
+    }
+    q = bsum(q, red);
+    if (threadIdx.x == 0) eps[j] = q;
+  }
+  __syncthreads();
+  __shared__ double hmax;
+  if (threadIdx.x == 0) {
+    double hn = 0.0;
+    for (int t = 0; t < N; ++t) hn = fmax(hn, fabs(hd[t]));
+    hmax = hn;
+  }
+  __syncthreads();
+  // zero pivots are replaced by +-eps |H| (the solution's component along psi_j is projected out)
+  const double tiny = 2.220446049250313e-16 * (hmax + 2.0 * fabs(Tb) + 2.0 * fabs(Tc));
+  for (int j = 0; j < nocc; ++j) {
+    const double* pj = psi + (size_t)j * N;
+    const double oj = occupation(ne, j);
+    for (int t = threadIdx.x; t < N; t += NT) r[t] = nob[t] * pj[t];
+    __syncthreads();
+    // rhs = Q r
+    for (int t = threadIdx.x; t < N; t += NT) y[t] = r[t];
+    __syncthreads();
+    for (int b = 0; b < nocc; ++b) {
+      const double* pb = psi + (size_t)b * N;
+      double d = 0.0;
+      for (int t = threadIdx.x; t < N; t += NT) d = fma(pb[t], r[t], d);
+      d = bsum(d, red);
+      for (int t = threadIdx.x; t < N; t += NT) y[t] = fma(-d, pb[t], y[t]);
+      __syncthreads();
+    }
+    // (H - eps_j) x = y by Gaussian elimination with partial pivoting (bandwidth 2, fill to 4)
+    const double ej = eps[j];
+    for (int t = threadIdx.x; t < N; t += NT) {
+      base[t] = t - 2;
+      double* row = R + (size_t)t * 7;
+      for (int c = 0; c < 7; ++c) {
+        const int col = t - 2 + c;
+        double v = 0.0;
+        if (col >= 0 && col < N) {
+          const int d = col - t;
+          v = d == 0 ? hd[t] - ej : (d == 1 || d == -1) ? Tb : (d == 2 || d == -2) ? Tc : 0.0;
+        }
+        row[c] = v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      auto get = [&](int rr, int col) -> double {
+        const int c = col - base[rr];
+        return (c >= 0 && c < 7) ? R[(size_t)rr * 7 + c] : 0.0;
+      };
+      auto put = [&](int rr, int col, double v) {
+        const int c = col - base[rr];
+        if (c >= 0 && c < 7) R[(size_t)rr * 7 + c] = v;
+      };
+      for (int i = 0; i < N; ++i) {
+        int pr = i;
+        double best = fabs(get(i, i));
+        for (int q = i + 1; q <= min(i + 2, N - 1); ++q)
+          if (fabs(get(q, i)) > best) {
+            best = fabs(get(q, i));
+            pr = q;
+          }
+        if (pr != i) {
+          // swap rows i and pr, then anchor each at its new position (window [pos - 2, pos + 4]);
+          // the active rows are zero outside columns i .. i + 4, which both windows contain
+          double ri[7], rp[7];
+          for (int c = 0; c < 7; ++c) {
+            ri[c] = get(pr, i - 2 + c);
+            rp[c] = get(i, pr - 2 + c);
+          }
+          for (int c = 0; c < 7; ++c) {
+            R[(size_t)i * 7 + c] = ri[c];
+            R[(size_t)pr * 7 + c] = rp[c];
+          }
+          base[i] = i - 2;
+          base[pr] = pr - 2;
+          const double ty = y[i];
+          y[i] = y[pr];
+          y[pr] = ty;
+        }
+        double piv = get(i, i);
+        if (fabs(piv) < tiny) {
+          piv = piv < 0.0 ? -tiny : tiny;
+          put(i, i, piv);
+        }
+        for (int q = i + 1; q <= min(i + 2, N - 1); ++q) {
+          const double l = get(q, i) / piv;
+          if (l != 0.0) {
+            for (int col = i; col <= min(i + 4, N - 1); ++col) put(q, col, get(q, col) - l * get(i, col));
+            y[q] -= l * y[i];
+          }
+        }
+      }
+      for (int i = N - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int col = i + 1; col <= min(i + 4, N - 1); ++col) v -= get(i, col) * y[col];
+        y[i] = v / get(i, i);
+      }
+    }
+    __syncthreads();
+    // x <- Q x; y_j = -x - sum over occupied b with occ_b != occ_j of psi_b (psi_b . r) / (eps_b - eps_j)
+    for (int b = 0; b < nocc; ++b) {
+      const double* pb = psi + (size_t)b * N;
+      double d = 0.0, dr = 0.0;
+      for (int t = threadIdx.x; t < N; t += NT) {
+        d = fma(pb[t], y[t], d);
+        dr = fma(pb[t], r[t], dr);
+      }
+      d = bsum(d, red);
+      dr = bsum(dr, red);
+      const double ob = occupation(ne, b);
+      const double extra = (b != j && ob != oj) ? dr / (eps[b] - ej) : 0.0;
+      // y holds x; the final y_j = -(Q x) - sum extra psi_b  ->  accumulate as x - d psi_b + extra psi_b, negated below
+      for (int t = threadIdx.x; t < N; t += NT) y[t] = fma(extra - d, pb[t], y[t]);
+      __syncthreads();
+    }
+    const double sc = -oj * 2.0 / a.dx;
+    for (int t = threadIdx.x; t < N; t += NT) vb[t] = fma(sc * pj[t], y[t], vb[t]);
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < N; t += NT) a.vbar[(size_t)s * N + t] = vb[t];
+}
+
+// the input densities n_k of all systems, contiguous [B][N]
+__global__ void k_adj_gather_input(Adj a, int k, double* dst) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.B * a.N) return;
+  const int s = (int)(idx / a.N), i = (int)(idx % a.N);
+  dst[idx] = input_density(a, s, k)[i];
+}
+
+__global__ void k_adj_reduce(int P, const double* thp, double* grad) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  double g = 0.0;
+  for (int sl = 0; sl < ADJ_SLOTS; ++sl) g += thp[(size_t)sl * P + q];
+  grad[q] = g;
+}
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct LayA {
+  size_t nbar, nob, nout, vbar, tmp, dbar, xbar, fv, fdv, thp, cs, ones, total;
+};
+
+int adj_check(const qex1d_problem* p) {
+  if (!p) return faila(QEX_E_BADARG, "null problem");
+  if (p->batch < 1 || p->num_grids < 5 || p->num_grids > QEX1D_MAX_GRIDS || !(p->dx > 0.0) || p->num_iterations < 1 ||
+      p->max_electrons < 1 || p->max_electrons > QEX1D_MAX_ELECTRONS || p->num_mixing_iterations < 1)
+    return faila(QEX_E_BADARG, "bad problem sizes");
+  if (p->loop != QEX1D_LOOP_SCAN) return faila(QEX_E_UNSUPPORTED, "the reverse pass follows the scan loop");
+  if (p->reflect) return faila(QEX_E_UNSUPPORTED, "the reverse pass has no reflection symmetry");
+  if (!(p->density_mse_converge_tolerance < 0.0))
+    return faila(QEX_E_UNSUPPORTED, "the reverse pass needs density_mse_converge_tolerance < 0 (no frozen states)");
+  if (p->xc_kind != QEX1D_XC_LOCAL_MLP) return faila(QEX_E_UNSUPPORTED, "the reverse pass needs a LOCAL_MLP functional");
+  if (p->mlp_layers != 2 && p->mlp_layers != 3)
+    return faila(QEX_E_UNSUPPORTED, "the reverse pass supports 1 or 2 hidden layers");
+  if (p->mlp_widths[0] < 1 || p->mlp_widths[0] > WMAXW || p->mlp_widths[0] % 2 == 0)
+    return faila(QEX_E_UNSUPPORTED, "window must be odd and <= %d", WMAXW);
+  for (int l = 1; l < p->mlp_layers; ++l)
+    if (p->mlp_widths[l] < 1 || p->mlp_widths[l] > HMAX)
+      return faila(QEX_E_UNSUPPORTED, "hidden widths must be in [1, %d] for the reverse pass", HMAX);
+  if (p->mlp_widths[p->mlp_layers] != 1) return faila(QEX_E_UNSUPPORTED, "MLP output width must be 1");
+  if (p->mlp_activation < 0 || p->mlp_activation > QEX_ACT_GELU_TANH || !(p->mlp_norm != 0.0))
+    return faila(QEX_E_BADARG, "bad activation or norm");
+  return QEX_OK;
+}
+
+int param_count(const qex1d_problem* p) {
+  int n = 0;
+  for (int l = 0; l < p->mlp_layers; ++l) n += p->mlp_widths[l] * p->mlp_widths[l + 1] + p->mlp_widths[l + 1];
+  return n;
+}
+
+LayA layout_adj(const qex1d_problem* p) {
+  LayA L{};
+  size_t off = 0;
+  const size_t B = p->batch, N = p->num_grids, T = p->num_iterations, W = p->mlp_widths[0];
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off = al(off + b);
+    return o;
+  };
+  L.nbar = take(B * N * 8);
+  L.nob = take(B * N * 8);
+  L.nout = take(B * N * 8);
+  L.vbar = take(B * N * 8);
+  L.tmp = take(B * N * 8);
+  L.dbar = take(B * T * N * 8);
+  L.xbar = take(B * N * W * 8);
+  L.fv = take(B * N * 8);
+  L.fdv = take(B * N * 8);
+  L.thp = take((size_t)ADJ_SLOTS * param_count(p) * 8);
+  L.cs = take(B * 8);
+  L.ones = take(B * 8);
+  L.total = off;
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qex1d_vjp_last_error(void) { return g_erra; }
+
+size_t qex1d_kohn_sham_vjp_workspace_bytes(const qex1d_problem* p) {
+  if (adj_check(p) != QEX_OK) return 0;
+  return layout_adj(p).total;
+}
+
+int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* fwd,
+                        const double* d_total_energy, const double* d_density, double* d_xc_params, void* workspace,
+                        size_t workspace_bytes, void* stream_) {
+  int rc = adj_check(p);
+  if (rc) return rc;
+  if (!in || !fwd || !d_xc_params) return faila(QEX_E_BADARG, "null inputs, forward outputs or gradient");
+  if (!in->interaction_table || !in->external_potential || !in->initial_density || !in->num_electrons ||
+      !in->xc_params)
+    return faila(QEX_E_BADARG, "missing forward inputs");
+  if (!fwd->density || !fwd->hartree_potential || !fwd->xc_potential || !fwd->orbitals)
+    return faila(QEX_E_BADARG, "the forward outputs density, hartree_potential, xc_potential and orbitals are required");
+  const size_t need = layout_adj(p).total;
+  if (!workspace || workspace_bytes < need) return faila(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  hipStream_t st = (hipStream_t)stream_;
+  const LayA L = layout_adj(p);
+  char* ws = (char*)workspace;
+  Adj a{};
+  a.B = p->batch;
+  a.N = p->num_grids;
+  a.T = p->num_iterations;
+  a.K = p->max_electrons / 2 + 1;
+  a.M = p->num_mixing_iterations;
+  a.W = p->mlp_widths[0];
+  a.NL = p->mlp_layers - 1;
+  a.H1 = p->mlp_widths[1];
+  a.H2 = a.NL == 2 ? p->mlp_widths[2] : 0;
+  a.act = p->mlp_activation;
+  a.P = param_count(p);
+  a.dx = p->dx;
+  a.norm = p->mlp_norm;
+  int off = 0;
+  a.w1 = off;
+  off += a.W * a.H1;
+  a.b1 = off;
+  off += a.H1;
+  if (a.NL == 2) {
+    a.w2 = off;
+    off += a.H1 * a.H2;
+    a.b2 = off;
+    off += a.H2;
+  }
+  a.w3 = off;
+  off += a.NL == 2 ? a.H2 : a.H1;
+  a.b3 = off;
+  a.table = in->interaction_table;
+  a.vext = in->external_potential;
+  a.n0 = in->initial_density;
+  a.ne = in->num_electrons;
+  a.mlp = in->xc_params;
+  a.dens = fwd->density;
+  a.vh = fwd->hartree_potential;
+  a.vxc = fwd->xc_potential;
+  a.orb = fwd->orbitals;
+  a.dE = d_total_energy;
+  a.dn = d_density;
+  a.nbar = (double*)(ws + L.nbar);
+  a.nob = (double*)(ws + L.nob);
+  a.nout = (double*)(ws + L.nout);
+  a.vbar = (double*)(ws + L.vbar);
+  a.tmp = (double*)(ws + L.tmp);
+  a.dbar = (double*)(ws + L.dbar);
+  a.xbar = (double*)(ws + L.xbar);
+  a.fv = (double*)(ws + L.fv);
+  a.fdv = (double*)(ws + L.fdv);
+  a.thp = (double*)(ws + L.thp);
+  double* cs = (double*)(ws + L.cs);
+  double* ones = (double*)(ws + L.ones);
+  const long BN = (long)a.B * a.N;
+  const double s2 = -0.5 / (a.dx * a.dx);
+  const double Td = s2 * -2.5, Tb = s2 * (4.0 / 3.0), Tc = s2 * (-1.0 / 12.0);
+  const int HL = a.NL == 2 ? a.H2 : a.H1;
+  const size_t mlp_lds = (2 * (size_t)a.W * a.H1 + a.H1 + (a.NL == 2 ? 2 * (size_t)a.H1 * a.H2 + a.H2 : 0) + HL +
+                          (size_t)(NT / 64) * 6 * 64) * 8;
+  const size_t eig_lds = (size_t)(4 + 7) * a.N * 8 + (size_t)a.N * 4;
+  if (mlp_lds > 150 * 1024 || eig_lds > 150 * 1024) return faila(QEX_E_UNSUPPORTED, "problem too large for LDS");
+  if (eig_lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_adj_eig, hipFuncAttributeMaxDynamicSharedMemorySize, (int)eig_lds);
+  if (mlp_lds > 64 * 1024) {
+    (void)hipFuncSetAttribute((const void*)k_adj_mlp<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_lds);
+    (void)hipFuncSetAttribute((const void*)k_adj_mlp<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_lds);
+  }
+  auto mlp = [&](const double* xin, const double* tin, const double* F1, const double* sf, const double* F2) {
+    if (a.NL == 2)
+      hipLaunchKernelGGL(k_adj_mlp<2>, dim3(ADJ_SLOTS / (NT / 64)), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
+    else
+      hipLaunchKernelGGL(k_adj_mlp<1>, dim3(ADJ_SLOTS / (NT / 64)), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
+  };
+  const unsigned gBN = (unsigned)((BN + NT - 1) / NT);
+  // alpha_k as the forward computes it
+  double alphas[4096];
+  if (a.T > 4096) return faila(QEX_E_UNSUPPORTED, "too many iterations");
+  double al_k = p->alpha;
+  for (int k = 0; k < a.T; ++k) {
+    alphas[k] = al_k;
+    al_k *= p->alpha_decay;
+  }
+  {
+    qexprof::Scope ps(qexprof::K_ADJ, st);
+    hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.nbar, BN);
+    hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.dbar, BN * a.T);
+    hipLaunchKernelGGL(k_zero_a, dim3(1024), dim3(NT), 0, st, a.thp, (long)ADJ_SLOTS * a.P);
+    for (int k = a.T - 1; k >= 0; --k) {
+      hipLaunchKernelGGL(k_adj_mix, dim3(a.B), dim3(NT), 0, st, a, k, alphas[k]);
+      hipLaunchKernelGGL(k_adj_scales, dim3((a.B + 63) / 64), dim3(64), 0, st, a, k, cs, ones);
+      // energy: v_H(n_out) and the MLP at n_out
+      hipLaunchKernelGGL(k_toeplitz, dim3((a.N + NT - 1) / NT, a.B), dim3(NT), (size_t)2 * a.N * 8, st, a.N, a.dx,
+                         a.table, a.nout, a.tmp);
+      hipLaunchKernelGGL(k_adj_energy, dim3(gBN), dim3(NT), 0, st, a, k, cs);
+      mlp(a.nout, nullptr, a.nout, cs, nullptr);
+      hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nob, nullptr, nullptr, a.fv, cs);
+      // eigen adjoint: nob -> vbar
+      hipLaunchKernelGGL(k_adj_eig, dim3(a.B), dim3(NT), eig_lds, st, a, k, Td, Tb, Tc);
+      // Hartree and XC: nbar += dx U vbar + H_F vbar
+      hipLaunchKernelGGL(k_toeplitz, dim3((a.N + NT - 1) / NT, a.B), dim3(NT), (size_t)2 * a.N * 8, st, a.N, a.dx,
+                         a.table, a.vbar, a.tmp);
+      // the input densities n_k of all systems, gathered contiguously into nout (free again)
+      hipLaunchKernelGGL(k_adj_gather_input, dim3(gBN), dim3(NT), 0, st, a, k, a.nout);
+      mlp(a.nout, a.vbar, a.vbar, ones, a.nout);
+      hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nbar, nullptr, a.tmp, a.fdv, ones);
+    }
+    hipLaunchKernelGGL(k_adj_reduce, dim3((a.P + NT - 1) / NT), dim3(NT), 0, st, a.P, a.thp, d_xc_params);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return faila(QEX_E_LAUNCH, "qex1d_kohn_sham_vjp: %s", hipGetErrorString(e));
+  return QEX_OK;
+}
+
+}  // extern "C"

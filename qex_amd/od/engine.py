@@ -144,6 +144,7 @@ class BatchResult:
     external_potential: torch.Tensor  # [B, N]
     eigenvalues: torch.Tensor | None = None  # [B, T, K]
     stats: torch.Tensor | None = None        # [2] int64: Sturm rounds, eigenpairs (accumulated)
+    orbitals: torch.Tensor | None = None     # [B, T, K, N] (the reverse pass needs them)
 
 
 class Problem1D:
@@ -152,7 +153,7 @@ class Problem1D:
     def __init__(self, grids, locations, nuclear_charges, num_electrons, interaction_fn, xc,
                  num_iterations, alpha=0.5, alpha_decay=0.9, num_mixing_iterations=2,
                  density_mse_converge_tolerance=-1.0, enforce_reflection_symmetry=False, loop="python",
-                 external_potential_override=None, want_eigenvalues=False):
+                 external_potential_override=None, want_eigenvalues=False, want_orbitals=False):
         dev = device()
         check_uniform(grids)
         self.dev = dev
@@ -196,6 +197,7 @@ class Problem1D:
         self.problem = p
         self.K = p.max_electrons // 2 + 1
         self.want_eigenvalues = want_eigenvalues
+        self.want_orbitals = want_orbitals
         if enforce_reflection_symmetry:
             if loop == "python":
                 win = np.array([reflection_window(l, self.grids) for l in locs], dtype=np.int32)
@@ -219,7 +221,8 @@ class Problem1D:
             density=f(B, T, N), total_energy=f(B, T), hartree_potential=f(B, T, N), xc_potential=f(B, T, N),
             xc_energy_density=exc, gap=f(B, T), converged=torch.empty((B, T), dtype=torch.uint8, device=dev),
             external_potential=self.vext, eigenvalues=f(B, T, self.K) if self.want_eigenvalues else None,
-            stats=torch.zeros(2, dtype=torch.int64, device=dev))
+            stats=torch.zeros(2, dtype=torch.int64, device=dev),
+            orbitals=f(B, T, self.K, N) if self.want_orbitals else None)
 
     def run(self, initial_density, out: BatchResult | None = None) -> BatchResult:
         n0 = as_f64(initial_density, self.dev)
@@ -233,8 +236,9 @@ class Problem1D:
                                _ptr(self.xc_params))
         outs = _lib.Qex1dOutputs(_ptr(out.density), _ptr(out.total_energy), _ptr(out.hartree_potential),
                                  _ptr(out.xc_potential), _ptr(out.xc_energy_density), _ptr(out.gap),
-                                 _ptr(out.converged), _ptr(out.eigenvalues), _ptr(out.stats))
+                                 _ptr(out.converged), _ptr(out.eigenvalues), _ptr(out.stats), _ptr(out.orbitals))
         self._keep = (n0, ins, outs)  # keep alive until the stream has consumed them
+        self._last_n0 = n0
         _lib.check(_lib.load().qex1d_kohn_sham(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs),
                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham")
         return out
@@ -315,3 +319,32 @@ class BatchRunner:
                                        self.ws.numel(), _stream()), "qex1d_kohn_sham")
         out._keep = (loc_d, z_d, ne_d, n0)  # inputs stay alive until the stream has consumed them
         return out
+
+
+def kohn_sham_vjp(problem: Problem1D, result: BatchResult, d_total_energy=None, d_density=None) -> torch.Tensor:
+    """Gradient of sum(d_total_energy * total_energy) + sum(d_density * density) with respect to the
+    packed parameters of the problem's LocalMLP functional (summed over the batch), by the device
+    reverse pass (``qex1d_kohn_sham_vjp``).  ``result`` must come from ``problem.run`` with
+    ``want_orbitals=True`` on the same initial densities."""
+    if result.orbitals is None:
+        raise ValueError("the forward result has no orbitals: build the problem with want_orbitals=True")
+    dev = problem.dev
+    B, T, N = problem.B, problem.T, problem.N
+    dE = None if d_total_energy is None else as_f64(d_total_energy, dev).reshape(B, T).contiguous()
+    dn = None if d_density is None else as_f64(d_density, dev).reshape(B, T, N).contiguous()
+    lib = _lib.load()
+    nbytes = lib.qex1d_kohn_sham_vjp_workspace_bytes(ctypes.byref(problem.problem))
+    if nbytes == 0:
+        _lib.check_vjp(-4, "qex1d_kohn_sham_vjp_workspace_bytes")
+    ws = workspace(nbytes, dev)
+    grad = torch.empty(problem.xc_params.numel(), dtype=torch.float64, device=dev)
+    ins = _lib.Qex1dInputs(_ptr(problem.table), _ptr(problem.vext), _ptr(problem._last_n0), _ptr(problem.ne_t),
+                           None, _ptr(problem.xc_params))
+    outs = _lib.Qex1dOutputs(_ptr(result.density), _ptr(result.total_energy), _ptr(result.hartree_potential),
+                             _ptr(result.xc_potential), _ptr(result.xc_energy_density), _ptr(result.gap),
+                             _ptr(result.converged), None, None, _ptr(result.orbitals))
+    _lib.check_vjp(lib.qex1d_kohn_sham_vjp(ctypes.byref(problem.problem), ctypes.byref(ins), ctypes.byref(outs),
+                                           _ptr(dE), _ptr(dn), _ptr(grad), _ptr(ws), ws.numel(), _stream()),
+                   "qex1d_kohn_sham_vjp")
+    problem._keep_vjp = (dE, dn, ws, ins, outs)
+    return grad

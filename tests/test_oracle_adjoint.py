@@ -1,0 +1,127 @@
+"""The hand-written reverse pass of the 1D scan trajectory (oracle/od_adjoint.py) against central
+finite differences of the forward loss, for several activations, odd electron counts and mixing
+over two iterations; and its pieces (MLP second-order adjoint, eigen adjoint) on their own."""
+
+import numpy as np
+import pytest
+
+from oracle import od as O
+from oracle import od_adjoint as A
+
+
+def _net(seed, sizes, act):
+    W, b = O.glorot_normal_mlp(seed, sizes)
+    rng = np.random.default_rng(seed + 7)
+    b = [rng.standard_normal(x.shape) * 0.1 for x in b]
+    return list(zip(W, b))
+
+
+@pytest.mark.parametrize("act", ["silu", "tanh", "gelu", "softplus", "sigmoid"])
+def test_mlp_second_order_matches_finite_differences(act):
+    rng = np.random.default_rng(1)
+    params = _net(2, [5, 7, 6, 1], act)
+    W, b = [w for w, _ in params], [x for _, x in params]
+    X = rng.standard_normal((11, 5))
+    Xd = rng.standard_normal((11, 5))
+    fbar, fdbar = rng.standard_normal(11), rng.standard_normal(11)
+
+    def phi(X_, W_):
+        f, fd, _, _, _ = A.mlp_second_order(W_, b, act, X_, Xd, np.zeros(11), np.zeros(11))
+        return float(fbar @ f + fdbar @ fd)
+
+    _, _, Xbar, dW, _ = A.mlp_second_order(W, b, act, X, Xd, fbar, fdbar)
+    h = 1e-6
+    for (p, q) in [(0, 0), (3, 2), (10, 4)]:
+        E = np.zeros_like(X)
+        E[p, q] = h
+        fd_ = (phi(X + E, W) - phi(X - E, W)) / (2 * h)
+        assert abs(fd_ - Xbar[p, q]) < 1e-7 * max(1, abs(fd_))
+    for li, (p, q) in [(0, (1, 2)), (1, (3, 4)), (2, (5, 0))]:
+        Wp = [w.copy() for w in W]
+        Wm = [w.copy() for w in W]
+        Wp[li][p, q] += h
+        Wm[li][p, q] -= h
+        fd_ = (phi(X, Wp) - phi(X, Wm)) / (2 * h)
+        assert abs(fd_ - dW[li][p, q]) < 1e-7 * max(1, abs(fd_))
+
+
+def test_vxc_vjp_is_hessian_vector_product():
+    grids = np.linspace(-5, 5, 41)
+    p = _net(3, [5, 6, 6, 1], "silu")
+    xc = O.LocalMLP([w for w, _ in p], [b for _, b in p], "silu", 5, 2.0)
+    adj = A.LocalMLPAdjoint(xc)
+    rng = np.random.default_rng(4)
+    n = np.abs(rng.standard_normal(41)) * 0.3
+    vbar = rng.standard_normal(41)
+    nbar, _ = adj.vxc_vjp(n, vbar)
+    h = 1e-6
+    fd = (xc.potential(n + h * vbar) - xc.potential(n - h * vbar)) / (2 * h)  # H v (H symmetric)
+    np.testing.assert_allclose(nbar, fd, atol=1e-7)
+    # gradient of F matches the forward potential
+    _, g, _ = adj.energy_and_grads(n)
+    np.testing.assert_allclose(g, xc.potential(n), atol=1e-12)
+    assert grids.size == 41
+
+
+def test_eig_adjoint_matches_finite_differences():
+    grids = np.linspace(-6, 6, 61)
+    dx = O.get_dx(grids)
+    T = O.get_kinetic_matrix(grids)
+    rng = np.random.default_rng(5)
+    v = -2.0 * np.exp(-grids ** 2) + 0.01 * rng.standard_normal(61)
+    occ = A.occupations(3)
+
+    def nout(v_):
+        eps, U = np.linalg.eigh(T + np.diag(v_))
+        return (occ[:, None] * U[:, :2].T ** 2).sum(0) / dx, eps, U
+
+    nbar = rng.standard_normal(61)
+    n0, eps, U = nout(v)
+    vbar = A.eig_adjoint(eps, U, occ, nbar, dx)
+    d = rng.standard_normal(61)
+    h = 1e-6
+    fd = (nbar @ nout(v + h * d)[0] - nbar @ nout(v - h * d)[0]) / (2 * h)
+    assert abs(fd - vbar @ d) < 1e-7 * max(1, abs(fd))
+
+
+@pytest.mark.parametrize("act,ne,m", [("silu", 2, 1), ("silu", 3, 2), ("tanh", 4, 1), ("gelu", 1, 2)])
+def test_trajectory_gradient_matches_finite_differences(act, ne, m):
+    grids = np.linspace(-8, 8, 129)
+    dx = O.get_dx(grids)
+    params = _net(3, [5, 8, 8, 1], act)
+    loc = np.array([-0.7, 0.8]) if ne > 1 else np.array([0.1])
+    z = np.array([1.0, float(ne - 1)]) if ne > 1 else np.array([1.0])
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True)
+    rng = np.random.default_rng(11)
+
+    def run(flat):
+        p = A.unflatten(flat, params)
+        xc = O.LocalMLP([w for w, _ in p], [b for _, b in p], act, 5, 2.0)
+        fw = A.forward_trajectory(loc, z, ne, 6, grids, xc, O.exponential_coulomb, n0, 0.3, 0.9, m)
+        L1, e1, d1 = A.c3_loss_and_cotangents(fw["energies"], fw["densities"], -1.5, 0.95 * n0, dx)
+        L2, e2, d2 = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], -1.5, 0.95 * n0, ne, dx,
+                                                      start=2)
+        return L1 + L2, fw, e1 + e2, d1 + d2
+
+    th = A.flatten(params)
+    _, fw, dE, dn = run(th)
+    g = A.backward(fw, dE, dn)
+    h = 1e-5
+    for _ in range(2):
+        d = rng.standard_normal(th.size)
+        d /= np.linalg.norm(d)
+        fd = (run(th + h * d)[0] - run(th - h * d)[0]) / (2 * h)
+        assert abs(fd - g @ d) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_forward_trajectory_matches_scan_oracle():
+    grids = np.linspace(-8, 8, 129)
+    params = _net(5, [17, 8, 8, 1], "silu")
+    xc = O.LocalMLP([w for w, _ in params], [b for _, b in params], "silu", 17, 2.0)
+    loc, z, ne = np.array([-0.7, 0.7]), np.array([1.0, 1.0]), 2
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True)
+    fw = A.forward_trajectory(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, 0.3, 1.0, 1)
+    st = O.jit_kohn_sham(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, alpha=0.3, alpha_decay=1.0,
+                         num_mixing_iterations=1)
+    np.testing.assert_allclose(fw["energies"], st.total_energy, atol=1e-10)
+    np.testing.assert_allclose(fw["densities"], st.density, atol=1e-10)

@@ -13,6 +13,12 @@ per atom, CDIIS(8), at most 30 cycles, |dE| < 1e-8); one step solves the four ch
 basis description (integrals, grid, AO values and the SCF cycle).  The oracle energies come from
 tests/golden/c4_chain_reference.json (made by tests/golden/make_c4_reference.py).
 
+Workload c3 (--workload c3): one training step = forward and reverse pass over 32 H2-like systems
+(nuclei at +-R/2, R ~ U[0.4, 6.0] from the seed, Z = 1, N_e = 2) on the 513-point grid with the
+c2 LocalMLP-17 network, 15 iterations; loss mean (E - E*)^2 + 1e-2 mean dx sum (n - n*)^2 against
+targets from the device LDA solve; gradients checked by central differences (h = 1e-4) along 10
+unit directions from the seed.
+
 Workload c5 (--workload c5): a served stream of --requests requests per Poisson arrival rate
 (--rates), 70% drawn from the c2 1D distribution and 30% from the c4 chains, dispatched by
 qex_amd.serve.Server (batched 1D launches on one stream, 3D solves on another); compared with the
@@ -100,7 +106,7 @@ def parse_args():
                     help="no oracle / CPU baseline (for profiler runs, which must not start child processes)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="launch one 64 MiB device-to-device copy first (reference for HBM counters)")
-    ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
     ap.add_argument("--requests", type=int, default=1024, help="c5: requests per arrival rate")
     ap.add_argument("--rates", type=str, default="100,500,2000", help="c5: Poisson arrival rates (req/s)")
     return ap.parse_args()
@@ -111,7 +117,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
-    if args.workload == "c4":
+    if args.workload == "c3":
+        run_c3(args)
+    elif args.workload == "c4":
         run_c4(args)
     elif args.workload == "c5":
         run_c5(args)
@@ -386,6 +394,137 @@ def run_c4(args):
         "cpu_baseline": {"value": cpu_rate, "unit": "solves/s", "cores": 1, "kind": "port",
                          "sample": f"FP64 NumPy/SciPy oracle, 1 thread, H2 and H4 chains end to end (integrals, "
                                    f"grid, SCF) repeated for ~10 s: {cpu_done} solves; {cpu_model()}"},
+    }
+    print(json.dumps(result))
+
+
+def _c3_oracle_sample(grids, systems, W, b, n0, targets_E, targets_n, budget_s=10.0):
+    """Bounded CPU sample: the oracle's forward and reverse pass on the first systems, one thread."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import od as O
+    from oracle import od_adjoint as A
+
+    xc = O.LocalMLP(W, b, "silu", 17, 2.0)
+    dx = float(grids[1] - grids[0])
+    done, t0 = 0, time.perf_counter()
+    with threadpool_limits(1):
+        for s, (loc, z, ne) in enumerate(systems):
+            fw = A.forward_trajectory(loc, z, ne, 15, grids, xc, O.exponential_coulomb, n0[s], ALPHA, 1.0, 1)
+            _, dE, dn = A.c3_loss_and_cotangents(fw["energies"], fw["densities"], targets_E[s], targets_n[s], dx)
+            A.backward(fw, dE, dn)
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+    return done / (time.perf_counter() - t0), done
+
+
+def run_c3(args):
+    from oracle import od as O
+
+    B, T = 32, 15
+    rng = np.random.default_rng(args.seed)
+    grids = O.h2_config_grid()
+    dx = float(grids[1] - grids[0])
+    R = rng.uniform(0.4, 6.0, B)
+    systems = [(np.array([-r / 2, r / 2]), np.array([1.0, 1.0]), 2) for r in R]
+    W, b = O.glorot_normal_mlp(args.seed, WIDTHS)
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in systems])
+
+    import torch
+
+    from qex_amd import _lib
+    from qex_amd.od import engine, train, utils, xc
+
+    lib = _lib.load()
+    # targets: the FP64 LDA solve (c1 settings) on the device
+    lda = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [2] * B, utils.exponential_coulomb,
+                           xc.get_lda_xc_energy_density_fn(), T, alpha=ALPHA, alpha_decay=1.0,
+                           num_mixing_iterations=1, loop="scan", enforce_reflection_symmetry=True)
+    ref = lda.run(n0)
+    tE = ref.total_energy[:, -1].clone()
+    tn = ref.density[:, -1].clone()
+    f = xc.LocalMLPXC(list(zip(W, b)), window=17, activation="silu", norm=2.0)
+    prob = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [2] * B,
+                            utils.exponential_coulomb, f, T, alpha=ALPHA, alpha_decay=1.0, num_mixing_iterations=1,
+                            loop="scan", want_orbitals=True)
+    n0_dev = engine.as_f64(n0, prob.dev)
+    for _ in range(args.warmup):
+        train.value_and_grad(prob, n0_dev, train.c3_loss, tE, tn, dx)
+    torch.cuda.synchronize()
+    launches = (5 * T + 8 + 9 * T + 8) * args.steps + 16
+    _lib.check(lib.qex_profile_enable(1, launches), "qex_profile_enable")
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record()
+        loss, g, res = train.value_and_grad(prob, n0_dev, train.c3_loss, tE, tn, dx)
+        ev[k][1].record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nk = len(_lib.KERNEL_KINDS)
+    ms = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int32 * nk)()
+    _lib.check(lib.qex_profile_read(ms, cnt, nk), "qex_profile_read")
+    lib.qex_profile_enable(0, 1)
+    step_ms = np.array([a_.elapsed_time(b_) for a_, b_ in ev])
+    # gradient check: central differences along 10 unit directions (h = 1e-4)
+    th0 = prob.xc_params.clone()
+    drng = np.random.default_rng(args.seed + 1)
+    errs = []
+    for _ in range(10):
+        d = torch.as_tensor(drng.standard_normal(th0.numel()), dtype=torch.float64, device=th0.device)
+        d /= d.norm()
+        vals = []
+        for sgn in (1.0, -1.0):
+            prob.xc_params.copy_(th0 + sgn * 1e-4 * d)
+            vals.append(float(train.c3_loss(prob.run(n0_dev), tE, tn, dx)[0]))
+        prob.xc_params.copy_(th0)
+        fd = (vals[0] - vals[1]) / 2e-4
+        gd = float(g @ d)
+        errs.append(abs(fd - gd) / max(abs(fd), 1e-12))
+    kern = {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.KERNEL_KINDS) if cnt[i]}
+    dom = max(kern, key=lambda k_: kern[k_][0])
+    value = B * args.steps / elapsed
+    # algorithmic FLOPs per launch: the MLP adjoint runs six network-sized GEMV passes per point
+    # (primal and tangent forward, primal and tangent backward, primal and tangent weight products)
+    net = sum(WIDTHS[i] * WIDTHS[i + 1] for i in range(len(WIDTHS) - 1))
+    fl = {"adj_mlp": 12.0 * net * B * len(grids), "mlp": 6.0 * net * B * len(grids),
+          "eigen": None, "adj_eig": 40.0 * len(grids) * B}
+    avg = {k_: v[0] / v[1] for k_, v in kern.items()}
+    achieved = (fl[dom] / (avg[dom] * 1e-3) / 1e12) if fl.get(dom) else None
+    cpu_rate, cpu_done = (float("nan"), 0)
+    if not args.skip_oracle:
+        cpu_rate, cpu_done = _c3_oracle_sample(grids, systems, W, b, n0, tE.cpu().numpy(), tn.cpu().numpy())
+    result = {
+        "metric": "training systems/s (forward + reverse pass through the SCF)",
+        "value": value,
+        "unit": "systems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(elapsed / args.steps * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded H2 separations and Glorot-normal LocalMLP weights; LDA targets)",
+        "config": {"workload": "c3 adjoint: 32 H2 systems, R ~ U[0.4, 6.0], LocalMLP-17 [64,64] SiLU, 15 iterations, "
+                               "loss MSE(E) + 1e-2 int (n - n*)^2; one step = forward + reverse pass",
+                   "batch": B, "num_grids": len(grids), "iterations": T, "seed": args.seed,
+                   "residuals": "orbitals of every iteration (K x N per system and iteration; no recompute)"},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "parity": {"fd_directions": 10, "fd_h": 1e-4, "max_rel_err": float(max(errs)),
+                   "median_rel_err": float(np.median(errs)), "loss": float(loss)},
+        "kernels_ms_total": {k_: round(v[0], 3) for k_, v in kern.items()},
+        "kernels_launches": {k_: int(v[1]) for k_, v in kern.items()},
+        "kernels_ms_avg": {k_: round(v_, 4) for k_, v_ in avg.items()},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": (achieved / 78.6) if achieved else None, "traffic": None},
+        "cpu_baseline": {"value": cpu_rate, "unit": "systems/s", "cores": 1, "kind": "port",
+                         "sample": f"FP64 NumPy oracle forward + hand-written reverse pass, 1 thread, first {cpu_done} "
+                                   f"systems (~10 s); {cpu_model()}"},
     }
     print(json.dumps(result))
 

@@ -152,7 +152,8 @@ SIGNATURES = {
 }
 
 KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize", "int1e", "eri", "grid", "eval_ao", "rho", "vxc",
-                "coulomb", "scf_fock", "scf_eigh", "scf_energy", "scf_other", "nn_global", "adjoint"]
+                "coulomb", "scf_fock", "scf_eigh", "scf_energy", "scf_other", "nn_global", "adjoint", "adj_mlp",
+                "adj_eig"]
 
 _lib = None
 

@@ -9,7 +9,7 @@ namespace qexprof {
 enum Kind {
   K_MLP = 0, K_POTENTIAL = 1, K_EIGEN = 2, K_DENSITY = 3, K_FINALIZE = 4,
   K_INT1E = 5, K_ERI = 6, K_GRID = 7, K_AO = 8, K_RHO = 9, K_VXC = 10, K_COULOMB = 11, K_FOCK = 12,
-  K_EIGH = 13, K_ENERGY = 14, K_OTHER = 15, K_NN = 16, K_ADJ = 17, NKINDS = 24
+  K_EIGH = 13, K_ENERGY = 14, K_OTHER = 15, K_NN = 16, K_ADJ = 17, K_ADJ_MLP = 18, K_ADJ_EIG = 19, NKINDS = 24
 };
 
 struct Rec {

@@ -28,7 +28,7 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int ADJ_SLOTS = 256;  // wavefronts of the MLP adjoint launch = parameter-partial slots
+constexpr int ADJ_SLOTS = 1024;  // wavefronts of the MLP adjoint launch = parameter-partial slots
 constexpr int HMAX = 64;        // hidden width (one lane per unit)
 constexpr int WMAXW = QEX1D_MAX_MLP_WINDOW;
 thread_local char g_erra[512] = "";
@@ -394,8 +394,8 @@ __global__ void k_adj_scales(Adj a, int k, double* cs, double* ones) {
   ones[s] = 1.0;
 }
 
-// eigen adjoint, one workgroup per system.  LDS: H diagonal, r, y, and the 7-wide band rows of the
-// LU factorisation (row r holds columns base[r] .. base[r] + 6).
+// eigen adjoint, one workgroup per system.  LDS: H diagonal, r, y, vbar and the rows of U of the
+// banded LU factorisation (row i holds columns i .. i + 4).
 __global__ __launch_bounds__(NT) void k_adj_eig(Adj a, int k, double Td, double Tb, double Tc) {
   extern __shared__ double sm[];
   __shared__ double red[NT / 64];
@@ -405,8 +405,7 @@ __global__ __launch_bounds__(NT) void k_adj_eig(Adj a, int k, double Td, double 
   double* r = hd + N;    // N
   double* y = r + N;     // N
   double* vb = y + N;    // N
-  double* R = vb + N;    // N x 7
-  int* base = (int*)(R + (size_t)7 * N);
+  double* R = vb + N;    // N x 5 rows of U
   const int ne = a.ne[s], nocc = (ne + 1) / 2;
   const double* vxc = a.vxc + ((size_t)s * T + k) * N;
   const double* vh = a.vh + ((size_t)s * T + k) * N;
@@ -459,74 +458,78 @@ __global__ __launch_bounds__(NT) void k_adj_eig(Adj a, int k, double Td, double 
       for (int t = threadIdx.x; t < N; t += NT) y[t] = fma(-d, pb[t], y[t]);
       __syncthreads();
     }
-    // (H - eps_j) x = y by Gaussian elimination with partial pivoting (bandwidth 2, fill to 4)
+    // (H - eps_j) x = y by Gaussian elimination with partial pivoting (bandwidth 2, fill to 4):
+    // one lane keeps the three active rows (columns i .. i + 4) in registers, stores each finished
+    // row of U (5 entries) and its right-hand side in LDS, then back-substitutes.
     const double ej = eps[j];
-    for (int t = threadIdx.x; t < N; t += NT) {
-      base[t] = t - 2;
-      double* row = R + (size_t)t * 7;
-      for (int c = 0; c < 7; ++c) {
-        const int col = t - 2 + c;
-        double v = 0.0;
-        if (col >= 0 && col < N) {
-          const int d = col - t;
-          v = d == 0 ? hd[t] - ej : (d == 1 || d == -1) ? Tb : (d == 2 || d == -2) ? Tc : 0.0;
-        }
-        row[c] = v;
-      }
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
-      auto get = [&](int rr, int col) -> double {
-        const int c = col - base[rr];
-        return (c >= 0 && c < 7) ? R[(size_t)rr * 7 + c] : 0.0;
+      auto band = [&](int row, int col) -> double {  // entry of H - eps_j (zero outside the matrix)
+        if (row >= N || col < 0 || col >= N) return 0.0;
+        const int d = col - row;
+        return d == 0 ? hd[row] - ej : (d == 1 || d == -1) ? Tb : (d == 2 || d == -2) ? Tc : 0.0;
       };
-      auto put = [&](int rr, int col, double v) {
-        const int c = col - base[rr];
-        if (c >= 0 && c < 7) R[(size_t)rr * 7 + c] = v;
-      };
+      double r0[5], r1[5], r2[5];
+      for (int c = 0; c < 5; ++c) {
+        r0[c] = band(0, c);
+        r1[c] = band(1, c);
+        r2[c] = band(2, c);
+      }
+      double y0 = y[0], y1 = N > 1 ? y[1] : 0.0, y2 = N > 2 ? y[2] : 0.0;
       for (int i = 0; i < N; ++i) {
-        int pr = i;
-        double best = fabs(get(i, i));
-        for (int q = i + 1; q <= min(i + 2, N - 1); ++q)
-          if (fabs(get(q, i)) > best) {
-            best = fabs(get(q, i));
-            pr = q;
-          }
-        if (pr != i) {
-          // swap rows i and pr, then anchor each at its new position (window [pos - 2, pos + 4]);
-          // the active rows are zero outside columns i .. i + 4, which both windows contain
-          double ri[7], rp[7];
-          for (int c = 0; c < 7; ++c) {
-            ri[c] = get(pr, i - 2 + c);
-            rp[c] = get(i, pr - 2 + c);
-          }
-          for (int c = 0; c < 7; ++c) {
-            R[(size_t)i * 7 + c] = ri[c];
-            R[(size_t)pr * 7 + c] = rp[c];
-          }
-          base[i] = i - 2;
-          base[pr] = pr - 2;
-          const double ty = y[i];
-          y[i] = y[pr];
-          y[pr] = ty;
+        // pivot among the (existing) rows i, i+1, i+2 in column i
+        int pr = 0;
+        double best = fabs(r0[0]);
+        if (i + 1 < N && fabs(r1[0]) > best) {
+          best = fabs(r1[0]);
+          pr = 1;
         }
-        double piv = get(i, i);
-        if (fabs(piv) < tiny) {
-          piv = piv < 0.0 ? -tiny : tiny;
-          put(i, i, piv);
-        }
-        for (int q = i + 1; q <= min(i + 2, N - 1); ++q) {
-          const double l = get(q, i) / piv;
-          if (l != 0.0) {
-            for (int col = i; col <= min(i + 4, N - 1); ++col) put(q, col, get(q, col) - l * get(i, col));
-            y[q] -= l * y[i];
+        if (i + 2 < N && fabs(r2[0]) > best) pr = 2;
+        if (pr == 1) {
+          for (int c = 0; c < 5; ++c) {
+            const double t1 = r0[c];
+            r0[c] = r1[c];
+            r1[c] = t1;
           }
+          const double ty = y0;
+          y0 = y1;
+          y1 = ty;
+        } else if (pr == 2) {
+          for (int c = 0; c < 5; ++c) {
+            const double t1 = r0[c];
+            r0[c] = r2[c];
+            r2[c] = t1;
+          }
+          const double ty = y0;
+          y0 = y2;
+          y2 = ty;
         }
+        if (fabs(r0[0]) < tiny) r0[0] = r0[0] < 0.0 ? -tiny : tiny;
+        const double ip = 1.0 / r0[0];
+        const double l1 = r1[0] * ip, l2 = r2[0] * ip;
+        for (int c = 1; c < 5; ++c) {
+          r1[c] = fma(-l1, r0[c], r1[c]);
+          r2[c] = fma(-l2, r0[c], r2[c]);
+        }
+        y1 = fma(-l1, y0, y1);
+        y2 = fma(-l2, y0, y2);
+        for (int c = 0; c < 5; ++c) R[(size_t)i * 5 + c] = r0[c];
+        y[i] = y0;
+        // advance: rows i+1, i+2 shift one column left; row i+3 enters
+        for (int c = 0; c < 4; ++c) {
+          r0[c] = r1[c + 1];
+          r1[c] = r2[c + 1];
+        }
+        r0[4] = 0.0;
+        r1[4] = 0.0;
+        for (int c = 0; c < 5; ++c) r2[c] = band(i + 3, i + 1 + c);
+        y0 = y1;
+        y1 = y2;
+        y2 = i + 3 < N ? y[i + 3] : 0.0;
       }
       for (int i = N - 1; i >= 0; --i) {
         double v = y[i];
-        for (int col = i + 1; col <= min(i + 4, N - 1); ++col) v -= get(i, col) * y[col];
-        y[i] = v / get(i, i);
+        for (int c = 1; c < 5 && i + c < N; ++c) v = fma(-R[(size_t)i * 5 + c], y[i + c], v);
+        y[i] = v / R[(size_t)i * 5];
       }
     }
     __syncthreads();
@@ -561,12 +564,21 @@ __global__ void k_adj_gather_input(Adj a, int k, double* dst) {
   dst[idx] = input_density(a, s, k)[i];
 }
 
-__global__ void k_adj_reduce(int P, const double* thp, double* grad) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P) return;
+// grad[q] = sum over slots of thp[slot][q]: 32 parameters per workgroup, 8 interleaved partial
+// sums each, combined in a fixed order
+__global__ __launch_bounds__(256) void k_adj_reduce(int P, const double* thp, double* grad) {
+  __shared__ double red[8][33];
+  const int t = threadIdx.x, q = blockIdx.x * 32 + (t & 31), part = t >> 5;
   double g = 0.0;
-  for (int sl = 0; sl < ADJ_SLOTS; ++sl) g += thp[(size_t)sl * P + q];
-  grad[q] = g;
+  if (q < P)
+    for (int sl = part; sl < ADJ_SLOTS; sl += 8) g += thp[(size_t)sl * P + q];
+  red[part][t & 31] = g;
+  __syncthreads();
+  if (t < 32 && q < P) {
+    double v = 0.0;
+    for (int k = 0; k < 8; ++k) v += red[k][t];
+    grad[q] = v;
+  }
 }
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -713,7 +725,7 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
   const int HL = a.NL == 2 ? a.H2 : a.H1;
   const size_t mlp_lds = (2 * (size_t)a.W * a.H1 + a.H1 + (a.NL == 2 ? 2 * (size_t)a.H1 * a.H2 + a.H2 : 0) + HL +
                           (size_t)(NT / 64) * 6 * 64) * 8;
-  const size_t eig_lds = (size_t)(4 + 7) * a.N * 8 + (size_t)a.N * 4;
+  const size_t eig_lds = (size_t)(4 + 5) * a.N * 8;
   if (mlp_lds > 150 * 1024 || eig_lds > 150 * 1024) return faila(QEX_E_UNSUPPORTED, "problem too large for LDS");
   if (eig_lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_adj_eig, hipFuncAttributeMaxDynamicSharedMemorySize, (int)eig_lds);
@@ -722,6 +734,7 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
     (void)hipFuncSetAttribute((const void*)k_adj_mlp<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_lds);
   }
   auto mlp = [&](const double* xin, const double* tin, const double* F1, const double* sf, const double* F2) {
+    qexprof::Scope ps(qexprof::K_ADJ_MLP, st);
     if (a.NL == 2)
       hipLaunchKernelGGL(k_adj_mlp<2>, dim3(ADJ_SLOTS / (NT / 64)), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
     else
@@ -737,7 +750,6 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
     al_k *= p->alpha_decay;
   }
   {
-    qexprof::Scope ps(qexprof::K_ADJ, st);
     hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.nbar, BN);
     hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.dbar, BN * a.T);
     hipLaunchKernelGGL(k_zero_a, dim3(1024), dim3(NT), 0, st, a.thp, (long)ADJ_SLOTS * a.P);
@@ -751,7 +763,10 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
       mlp(a.nout, nullptr, a.nout, cs, nullptr);
       hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nob, nullptr, nullptr, a.fv, cs);
       // eigen adjoint: nob -> vbar
-      hipLaunchKernelGGL(k_adj_eig, dim3(a.B), dim3(NT), eig_lds, st, a, k, Td, Tb, Tc);
+      {
+        qexprof::Scope ps(qexprof::K_ADJ_EIG, st);
+        hipLaunchKernelGGL(k_adj_eig, dim3(a.B), dim3(NT), eig_lds, st, a, k, Td, Tb, Tc);
+      }
       // Hartree and XC: nbar += dx U vbar + H_F vbar
       hipLaunchKernelGGL(k_toeplitz, dim3((a.N + NT - 1) / NT, a.B), dim3(NT), (size_t)2 * a.N * 8, st, a.N, a.dx,
                          a.table, a.vbar, a.tmp);
@@ -760,7 +775,7 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
       mlp(a.nout, a.vbar, a.vbar, ones, a.nout);
       hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nbar, nullptr, a.tmp, a.fdv, ones);
     }
-    hipLaunchKernelGGL(k_adj_reduce, dim3((a.P + NT - 1) / NT), dim3(NT), 0, st, a.P, a.thp, d_xc_params);
+    hipLaunchKernelGGL(k_adj_reduce, dim3((a.P + 31) / 32), dim3(256), 0, st, a.P, a.thp, d_xc_params);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return faila(QEX_E_LAUNCH, "qex1d_kohn_sham_vjp: %s", hipGetErrorString(e));

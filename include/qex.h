@@ -51,6 +51,8 @@ extern "C" {
 #define QEX1D_XC_POWER_GLOBAL 2 /* f = c (n + eps)^p summed (global) */
 #define QEX1D_XC_LOCAL_MLP 3    /* windowed per-point MLP (local); window 1 = classical_models.py:123 */
 #define QEX1D_XC_NONE 4         /* no XC (v_xc = 0) */
+#define QEX1D_XC_GLOBAL_MLP 5   /* MLP on the whole density (global, amplitude encoded):
+                                   classical_models.py:177-224 build_global_mlp */
 
 #define QEX_ACT_SILU 0
 #define QEX_ACT_TANH 1

@@ -25,6 +25,7 @@ QEX1D_XC_POWER_LOCAL = 1
 QEX1D_XC_POWER_GLOBAL = 2
 QEX1D_XC_LOCAL_MLP = 3
 QEX1D_XC_NONE = 4
+QEX1D_XC_GLOBAL_MLP = 5
 
 ACTIVATIONS = {"silu": 0, "swish": 0, "tanh": 1, "softplus": 2, "sigmoid": 3, "relu": 4, "gelu": 5}
 

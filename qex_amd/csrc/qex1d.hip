@@ -233,7 +233,7 @@ struct Dev1 {
   double* mlp_eout;  // [B][N]
 };
 
-__device__ __forceinline__ bool xc_is_local(int xc) { return xc != QEX1D_XC_POWER_GLOBAL; }
+__device__ __forceinline__ bool xc_is_local(int xc) { return xc != QEX1D_XC_POWER_GLOBAL && xc != QEX1D_XC_GLOBAL_MLP; }
 
 // pointwise XC at density x: energy density (local: e; global: f), potential contribution
 __device__ __forceinline__ void xc_point(const Dev1& a, double x, double* e, double* v) {
@@ -664,6 +664,101 @@ __device__ double mlp_v_unsym(const Dev1& a, int s, int i) {
   return v + acc / a.norm;
 }
 
+// Global MLP on the whole density, one workgroup per system: f(n_in) with the input gradient
+// (potential dx df/dn into mlp_e) and f at the pending output density; scalars in
+// mlp_eout[2 s] (f(n_in)) and mlp_eout[2 s + 1] (f(n_out)).
+__device__ double gmlp_forward(const Dev1& a, const double* x, double* acts, double* dzs, double* part, double* red) {
+  const int N = a.N, tid = threadIdx.x;
+  const int H1 = a.widths[1];
+  // layer 1: z[u] = b[u] + sum_i x_i W[i][u], split over NT / H1 interleaved row groups
+  const int G = NT / H1, u = tid % H1, g = tid / H1;
+  double acc = 0.0;
+  if (g < G)
+    for (int i = g; i < N; i += G) acc = fma(x[i], a.mlp[a.woff[0] + (size_t)i * H1 + u], acc);
+  part[tid] = acc;
+  __syncthreads();
+  if (tid < H1) {
+    double z = a.mlp[a.boff[0] + tid];
+    for (int q = 0; q < G; ++q) z += part[q * H1 + tid];
+    double dz;
+    acts[tid] = act_fwd(a.act, z, &dz);
+    dzs[tid] = dz;
+  }
+  __syncthreads();
+  // hidden layers 2 .. L-1 (activations of layer l at acts + (l-1) * 128)
+  for (int l = 1; l < a.nlay - 1; ++l) {
+    const int fi = a.widths[l], fo = a.widths[l + 1];
+    const double* prev = acts + (l - 1) * 128;
+    if (tid < fo) {
+      double z = a.mlp[a.boff[l] + tid];
+      for (int v = 0; v < fi; ++v) z = fma(prev[v], a.mlp[a.woff[l] + v * fo + tid], z);
+      double dz;
+      acts[l * 128 + tid] = act_fwd(a.act, z, &dz);
+      dzs[l * 128 + tid] = dz;
+    }
+    __syncthreads();
+  }
+  const int L = a.nlay - 1, fl = a.widths[L];
+  double o = 0.0;
+  for (int v = tid; v < fl; v += NT) o = fma(acts[(L - 1) * 128 + v], a.mlp[a.woff[L] + v], o);
+  double r[1] = {o};
+  block_sum<1>(r, red);
+  return r[0] + a.mlp[a.boff[L]];
+}
+
+__global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k) {
+  extern __shared__ double smem[];
+  __shared__ double red[NWAVE * 8];
+  const int s = blockIdx.x, N = a.N, tid = threadIdx.x;
+  const bool doPot = k < a.T && a.st[4 * s + 0] == 0;
+  const bool doE = a.st[4 * s + 1] >= 0;
+  double* x = smem;                 // N
+  double* acts = x + N;             // 6 x 128
+  double* dzs = acts + 6 * 128;     // 6 x 128
+  double* part = dzs + 6 * 128;     // NT
+  double* dl = part + NT;           // 2 x 128 backward buffers
+  const double in = 1.0 / a.norm;
+  if (doPot) {
+    const double* n = input_density(a, s, k);
+    for (int t = tid; t < N; t += NT) x[t] = n[t] * in;
+    __syncthreads();
+    const double f = gmlp_forward(a, x, acts, dzs, part, red);
+    // backward: d f / d acts of the last hidden layer = W_L, then through the hidden layers
+    const int L = a.nlay - 1;
+    double* cur = dl;
+    double* nxt = dl + 128;
+    for (int v = tid; v < a.widths[L]; v += NT) cur[v] = a.mlp[a.woff[L] + v] * dzs[(L - 1) * 128 + v];
+    __syncthreads();
+    for (int l = L - 1; l >= 1; --l) {
+      const int fi = a.widths[l], fo = a.widths[l + 1];
+      if (tid < fi) {
+        double g = 0.0;
+        for (int u = 0; u < fo; ++u) g = fma(a.mlp[a.woff[l] + tid * fo + u], cur[u], g);
+        nxt[tid] = g * dzs[(l - 1) * 128 + tid];
+      }
+      __syncthreads();
+      double* t2 = cur;
+      cur = nxt;
+      nxt = t2;
+    }
+    // v_i = dx df/dn_i = dx / norm sum_u W1[i][u] delta_u
+    const int H1 = a.widths[1];
+    for (int i = tid; i < N; i += NT) {
+      double g = 0.0;
+      for (int u = 0; u < H1; ++u) g = fma(a.mlp[a.woff[0] + (size_t)i * H1 + u], cur[u], g);
+      a.mlp_e[(size_t)s * N + i] = a.dx * in * g;
+    }
+    if (tid == 0) a.mlp_eout[2 * s] = f;
+    __syncthreads();
+  }
+  if (doE) {
+    for (int t = tid; t < N; t += NT) x[t] = a.o_xc[(size_t)s * N + t] * in;
+    __syncthreads();
+    const double f = gmlp_forward(a, x, acts, dzs, part, red);
+    if (tid == 0) a.mlp_eout[2 * s + 1] = f;
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k) {
   extern __shared__ double smem[];
   const int s = blockIdx.y, tile = blockIdx.x;
@@ -714,7 +809,11 @@ __global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k) {
       const size_t o = ((size_t)s * a.T + k) * N + i;
       const double vH = a.dx * KV;
       double e = 0.0, v = 0.0;
-      if (a.xc == QEX1D_XC_LOCAL_MLP) {
+      if (a.xc == QEX1D_XC_GLOBAL_MLP) {
+        // potential dx df/dn from k_gmlp; the scalar f(n_in) is carried by point 0
+        v = a.mlp_e[(size_t)s * N + i];
+        e = i == 0 ? a.mlp_eout[2 * s] : 0.0;
+      } else if (a.xc == QEX1D_XC_LOCAL_MLP) {
         const int im = sym ? mirror(i, lo, hi) : i;
         const double vu = mlp_v_unsym(a, s, i);
         const double eu = a.mlp_e[(size_t)s * N + i];
@@ -741,7 +840,9 @@ __global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k) {
     if (doE) {
       const double ox = a.o_xc[(size_t)s * N + i];
       eH = 0.5 * a.dx * a.dx * oraw[i] * KE;
-      if (a.xc == QEX1D_XC_LOCAL_MLP) {
+      if (a.xc == QEX1D_XC_GLOBAL_MLP) {
+        eX = i == 0 ? a.dx * a.mlp_eout[2 * s + 1] : 0.0;
+      } else if (a.xc == QEX1D_XC_LOCAL_MLP) {
         eX = a.dx * a.mlp_eout[(size_t)s * N + i] * ox;
       } else if (a.xc != QEX1D_XC_NONE) {
         double e, v;
@@ -1241,8 +1342,8 @@ Layout layout1d(const qex1d_problem* p) {
   const size_t B = p->batch, N = p->num_grids, T = p->num_iterations;
   const size_t K = p->max_electrons / 2 + 1, M = p->num_mixing_iterations;
   const size_t nT = (N + TILE - 1) / TILE;
-  const bool mlp = p->xc_kind == QEX1D_XC_LOCAL_MLP;
-  const size_t W = mlp ? p->mlp_widths[0] : 0;
+  const bool mlp = p->xc_kind == QEX1D_XC_LOCAL_MLP || p->xc_kind == QEX1D_XC_GLOBAL_MLP;
+  const size_t W = p->xc_kind == QEX1D_XC_LOCAL_MLP ? p->mlp_widths[0] : 0;
   Layout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -1266,7 +1367,7 @@ Layout layout1d(const qex1d_problem* p) {
   L.st = take(B * 4 * 4);
   L.mlp_e = take(mlp ? B * N * 8 : 0);
   L.mlp_G = take(mlp ? B * N * W * 8 : 0);
-  L.mlp_eout = take(mlp ? B * N * 8 : 0);
+  L.mlp_eout = take(mlp ? B * (N > 2 ? N : 2) * 8 : 0);
   L.pairs = take(B * K * 4);
   L.total = off;
   return L;
@@ -1284,7 +1385,18 @@ int validate1d(const qex1d_problem* p) {
   if (p->max_electrons / 2 + 1 > p->num_grids) return fail(QEX_E_BADARG, "too many electrons for the grid");
   if (p->num_mixing_iterations < 1) return fail(QEX_E_BADARG, "num_mixing_iterations must be >= 1");
   if (p->loop != QEX1D_LOOP_PYTHON && p->loop != QEX1D_LOOP_SCAN) return fail(QEX_E_BADARG, "bad loop kind");
-  if (p->xc_kind < 0 || p->xc_kind > QEX1D_XC_NONE) return fail(QEX_E_BADARG, "bad xc_kind");
+  if (p->xc_kind < 0 || p->xc_kind > QEX1D_XC_GLOBAL_MLP) return fail(QEX_E_BADARG, "bad xc_kind");
+  if (p->xc_kind == QEX1D_XC_GLOBAL_MLP) {
+    if (p->mlp_layers < 2 || p->mlp_layers > QEX1D_MAX_MLP_LAYERS)
+      return fail(QEX_E_UNSUPPORTED, "GLOBAL_MLP needs 2..%d Dense layers", QEX1D_MAX_MLP_LAYERS);
+    if (p->mlp_widths[0] != p->num_grids) return fail(QEX_E_BADARG, "GLOBAL_MLP input width must be num_grids");
+    if (p->mlp_widths[p->mlp_layers] != 1) return fail(QEX_E_UNSUPPORTED, "MLP output width must be 1");
+    for (int l = 1; l < p->mlp_layers; ++l)
+      if (p->mlp_widths[l] < 1 || p->mlp_widths[l] > 128)
+        return fail(QEX_E_UNSUPPORTED, "GLOBAL_MLP hidden widths must be in [1, 128]");
+    if (p->mlp_activation < 0 || p->mlp_activation > QEX_ACT_GELU_TANH) return fail(QEX_E_BADARG, "bad activation");
+    if (!(p->mlp_norm != 0.0)) return fail(QEX_E_BADARG, "mlp_norm must be nonzero");
+  }
   if (p->xc_kind == QEX1D_XC_LOCAL_MLP) {
     if (p->mlp_layers < 2 || p->mlp_layers > QEX1D_MAX_MLP_LAYERS)
       return fail(QEX_E_UNSUPPORTED, "LOCAL_MLP needs 2..%d Dense layers", QEX1D_MAX_MLP_LAYERS);
@@ -1387,7 +1499,8 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   if (!in->interaction_table || !in->external_potential || !in->initial_density || !in->num_electrons)
     return fail(QEX_E_BADARG, "missing input array");
   if (p->reflect && !in->reflect_window) return fail(QEX_E_BADARG, "reflect needs reflect_window");
-  if (p->xc_kind == QEX1D_XC_LOCAL_MLP && !in->xc_params) return fail(QEX_E_BADARG, "LOCAL_MLP needs xc_params");
+  if ((p->xc_kind == QEX1D_XC_LOCAL_MLP || p->xc_kind == QEX1D_XC_GLOBAL_MLP) && !in->xc_params)
+    return fail(QEX_E_BADARG, "MLP functionals need xc_params");
   if (!out->density || !out->total_energy || !out->hartree_potential || !out->xc_potential ||
       !out->xc_energy_density || !out->gap || !out->converged)
     return fail(QEX_E_BADARG, "missing output array");
@@ -1412,7 +1525,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   a.xc_p = p->xc_p;
   a.xc_eps = p->xc_eps;
   kinetic(p->dx, &a.Td, &a.Tb, &a.Tc);
-  if (a.xc == QEX1D_XC_LOCAL_MLP) {
+  if (a.xc == QEX1D_XC_LOCAL_MLP || a.xc == QEX1D_XC_GLOBAL_MLP) {
     a.nlay = p->mlp_layers;
     a.act = p->mlp_activation;
     a.norm = p->mlp_norm;
@@ -1511,6 +1624,10 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
         }
       } else
         hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
+    }
+    if (a.xc == QEX1D_XC_GLOBAL_MLP) {
+      Scope ps(qexprof::K_MLP, stream);
+      hipLaunchKernelGGL(k_gmlp, dim3(a.B), dim3(NT), (size_t)(a.N + 12 * 128 + NT + 256) * 8, stream, a, k);
     }
     {
       Scope ps(qexprof::K_POTENTIAL, stream);

@@ -164,6 +164,58 @@ class LocalMLPXC(XCFunctional):
         return x[..., 0]
 
 
+class GlobalMLPXC(LocalMLPXC):
+    """MLP on the whole density divided by ``norm`` -> one scalar f(n) (``build_global_mlp``,
+    classical_models.py:177-224); the amplitude-encoded loop takes E_xc = f(n) dx and
+    v_xc = dx df/dn (od/scf.py:32-76).
+
+    ``params``: list of (W, b) per Dense layer, W of shape (fan_in, fan_out); the first fan-in is
+    the number of grid points, hidden widths at most 128, output width 1.
+    """
+
+    encoding = "global"
+    xc_kind = _lib.QEX1D_XC_GLOBAL_MLP
+
+    def __init__(self, params=None, activation="tanh", norm=2.0):
+        if activation not in _lib.ACTIVATIONS:
+            raise ValueError(f"unsupported activation {activation!r}")
+        self.window = None
+        self.activation, self.norm = activation, float(norm)
+        self.params = None if params is None else self._canon(params)
+
+    def with_params(self, params):
+        return GlobalMLPXC(params, self.activation, self.norm)
+
+    def configure(self, problem):
+        if self.params is None:
+            raise ValueError("GlobalMLPXC has no parameters")
+        ws = self.widths()
+        if ws[0] != problem.num_grids:
+            raise ValueError(f"first layer fan-in {ws[0]} != number of grid points {problem.num_grids}")
+        if len(self.params) > _lib.QEX1D_MAX_MLP_LAYERS:
+            raise ValueError("too many layers")
+        if ws[-1] != 1 or max(ws[1:-1], default=1) > 128:
+            raise ValueError("GlobalMLPXC needs output width 1 and hidden widths <= 128")
+        problem.xc_kind = self.xc_kind
+        problem.mlp_layers = len(self.params)
+        for i, w in enumerate(ws):
+            problem.mlp_widths[i] = int(w)
+        problem.mlp_activation = _lib.ACTIVATIONS[self.activation]
+        problem.mlp_norm = self.norm
+
+    def energy_density(self, n):
+        x = n / self.norm
+        acts = {"silu": torch.nn.functional.silu, "swish": torch.nn.functional.silu, "tanh": torch.tanh,
+                "softplus": torch.nn.functional.softplus, "sigmoid": torch.sigmoid, "relu": torch.relu,
+                "gelu": lambda z: torch.nn.functional.gelu(z, approximate="tanh")}
+        act = acts[self.activation]
+        for li, (w, b) in enumerate(self.params):
+            x = x @ torch.as_tensor(w, dtype=x.dtype, device=x.device) + torch.as_tensor(b, dtype=x.dtype, device=x.device)
+            if li < len(self.params) - 1:
+                x = act(x)
+        return x[..., 0]
+
+
 def get_lda_xc_energy_density_fn():
     """jax_dft ``xc.get_lda_xc_energy_density_fn()``: fn(density, params) -> e_xc per electron."""
     return LDAExp()

@@ -87,6 +87,20 @@ def test_mlp_validation():
     assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
 
 
+def test_global_mlp_validation():
+    lib = _lib.load()
+    p = _problem(xc_kind=_lib.QEX1D_XC_GLOBAL_MLP, mlp_layers=3, mlp_activation=1, mlp_norm=2.0)
+    for i, w in enumerate([p.num_grids, 32, 16, 1]):
+        p.mlp_widths[i] = w
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) > 0
+    p.mlp_widths[0] = p.num_grids - 1  # input width must be the grid size
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
+    p.mlp_widths[0] = p.num_grids
+    p.mlp_widths[1] = 129  # hidden widths at most 128
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
+    assert "hidden" in lib.qex_last_error().decode()
+
+
 def test_engine_refuses_to_run_without_a_gpu():
     import torch
 

@@ -193,6 +193,20 @@ def test_local_mlp_window_one_is_pointwise():
         assert e[i] == pytest.approx(float((h @ W[2] + b[2])[0, 0]), abs=1e-14)
 
 
+def test_global_mlp_is_the_dense_stack_on_the_whole_density():
+    """build_global_mlp (classical_models.py:177-224): one scalar from the whole density / 2."""
+    W, b = O.glorot_normal_mlp(6, [9, 5, 1])
+    xc = O.GlobalMLP(W, b, "tanh", 2.0)
+    n = np.linspace(0.1, 1.7, 9)
+    h = np.tanh(n / 2 @ W[0] + b[0])
+    assert xc.value(n) == pytest.approx(float(h @ W[1][:, 0] + b[1][0]), abs=1e-14)
+    # linear network: f = (n/2) . W0 W1 + b0 W1 + b1, df/dn = W0 W1 / 2
+    lin = O.GlobalMLP([W[0] @ W[1]], [b[0] @ W[1] + b[1]], "tanh", 2.0)
+    np.testing.assert_allclose(lin.grad(n), (W[0] @ W[1])[:, 0] / 2, rtol=0, atol=1e-15)
+    assert O.xc_energy(xc, n, 0.1) == pytest.approx(0.1 * xc.value(n), abs=1e-15)
+    np.testing.assert_allclose(O.xc_potential(xc, n, 0.1), 0.1 * xc.grad(n), rtol=0, atol=1e-16)
+
+
 def test_reflection_window_and_flip():
     g = np.linspace(-2, 2, 41)  # dx = 0.1
     lo, hi = O.reflection_window([-0.5, 0.5], g)

@@ -175,6 +175,7 @@ const char* qex1d_vjp_last_error(void);
  * ========================================================================================== */
 #define QEX3D_MAX_NAO 96
 #define QEX3D_MAX_DIIS 12
+#define QEX3D_MAX_SCF_BATCH 1024
 #define QEX3D_MAX_NPRIM 8
 #define QEX3D_RKS_LDA 0 /* Slater exchange + VWN5 correlation */
 #define QEX3D_RHF 1
@@ -285,6 +286,16 @@ typedef struct qex3d_scf_outputs {
 size_t qex3d_scf_workspace_bytes(const qex3d_scf_problem* p);
 int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
               size_t workspace_bytes, void* stream);
+
+/* The same SCF for `batch` molecules of any sizes and methods in one sequence of launches (each
+ * stage is one launch over the batch).  Replaces the padded/masked batch of
+ * qedft/train/td/scf_functions_masked.py:199-303 and :917-967 (_scf_test_padded): every molecule
+ * keeps its own nao and grid, so its results equal those of qex3d_scf on it alone, bit for bit.
+ * problems/inputs/outputs are host arrays of `batch` entries.  A molecule stops at convergence or
+ * at its own max_cycle; the host polls every min(sync_every > 0) cycles (0 for all: no polling). */
+size_t qex3d_scf_batch_workspace_bytes(int32_t batch, const qex3d_scf_problem* problems);
+int qex3d_scf_batch(int32_t batch, const qex3d_scf_problem* problems, const qex3d_scf_inputs* inputs,
+                    const qex3d_scf_outputs* outputs, void* workspace, size_t workspace_bytes, void* stream);
 
 const char* qex3d_last_error(void);
 

@@ -89,6 +89,7 @@ QEX3D_RKS_NN_GLOBAL = 2
 QEX3D_MAX_NN_HIDDEN = 1024
 QEX3D_MAX_NAO = 96
 QEX3D_MAX_NPRIM = 8
+QEX3D_MAX_SCF_BATCH = 1024
 
 
 class Qex3dBasis(ctypes.Structure):
@@ -150,6 +151,9 @@ SIGNATURES = {
     "qex3d_scf_workspace_bytes": (_S, [ctypes.POINTER(Qex3dScfProblem)]),
     "qex3d_scf": (_I, [ctypes.POINTER(Qex3dScfProblem), ctypes.POINTER(Qex3dScfInputs),
                        ctypes.POINTER(Qex3dScfOutputs), _P, _S, _P]),
+    "qex3d_scf_batch_workspace_bytes": (_S, [_I, ctypes.POINTER(Qex3dScfProblem)]),
+    "qex3d_scf_batch": (_I, [_I, ctypes.POINTER(Qex3dScfProblem), ctypes.POINTER(Qex3dScfInputs),
+                             ctypes.POINTER(Qex3dScfOutputs), _P, _S, _P]),
 }
 
 KERNEL_KINDS = ["mlp", "potential", "eigen", "density", "finalize", "int1e", "eri", "grid", "eval_ao", "rho", "vxc",

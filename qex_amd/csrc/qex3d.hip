@@ -14,6 +14,7 @@
 // The SCF loop is enqueued without host synchronisation; a device flag freezes the state once
 // |dE| < tol and |g| / sqrt(n_vir n_occ) < sqrt(tol) (pyscf's criterion).
 #include <hip/hip_runtime.h>
+#include <vector>
 
 #include <math.h>
 #include <stdarg.h>
@@ -465,14 +466,12 @@ __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(NT) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho,
-                                            const int* frozen) {
+__device__ __forceinline__ void rho_body(long bid, long ng, int n, const double* ao, const double* dm, double* rho) {
   extern __shared__ double sm[];
-  if (frozen && *frozen) return;
   const int npad = (n + 15) & ~15, S = npad + 1;
   double* ph = sm;
   __shared__ double part[NW][64];
-  const long g0 = (long)blockIdx.x * 64;
+  const long g0 = bid * 64;
   for (int t = threadIdx.x; t < 64 * npad; t += NT) {
     const int p = t / npad, c = t % npad;
     ph[p * S + c] = (g0 + p < ng && c < n) ? ao[(g0 + p) * n + c] : 0.0;
@@ -522,6 +521,10 @@ __global__ __launch_bounds__(NT) void k_rho(long ng, int n, const double* ao, co
   }
 }
 
+__global__ __launch_bounds__(NT) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho) {
+  rho_body(blockIdx.x, ng, n, ao, dm, rho);
+}
+
 // LDA: Slater exchange + VWN5 correlation (spin unpolarised); e per electron and v = d(rho e)/d rho
 __device__ void lda_svwn5(double rho, double* e, double* v) {
   if (!(rho > 1e-30)) {
@@ -552,10 +555,10 @@ constexpr int CH = 64;
 constexpr int VX_TILES = 9;  // tiles per wave: (npad/16)^2 / 4 for npad <= 96
 // wv_given (optional): per-point w v / 2 computed elsewhere (neural XC); then only sum(rho w) is
 // accumulated and the energy partial is zero.
-__global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
-                                            const double* wv_given, double* vpart, double* spart, const int* frozen) {
+// Workgroup bid of nblk (chunks bid, bid + nblk, ...).
+__device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, const double* ao, const double* w,
+                                         const double* rho, const double* wv_given, double* vpart, double* spart) {
   extern __shared__ double sm[];
-  if (frozen && *frozen) return;
   const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16;
   double* ph = sm;                    // CH x S
   double* wvs = ph + (size_t)CH * S;  // CH
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, co
   for (int t = 0; t < VX_TILES; ++t) acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
   double nel = 0.0, exc = 0.0;
   const long nchunk = (ng + CH - 1) / CH;
-  for (long c = blockIdx.x; c < nchunk; c += gridDim.x) {
+  for (long c = bid; c < nchunk; c += nblk) {
     const long g0 = c * CH;
     __syncthreads();
     for (int t = threadIdx.x; t < CH; t += NT) {
@@ -608,26 +611,31 @@ __global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, co
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = bi * 16 + 4 * v + (l >> 4), col = bj * 16 + (l & 15);
-        if (row < n && col < n) vpart[(size_t)blockIdx.x * n * n + row * n + col] = acc[t][v];
+        if (row < n && col < n) vpart[(size_t)bid * n * n + row * n + col] = acc[t][v];
       }
     }
   }
   const double s0 = bsum(nel, red), s1 = bsum(exc, red + NW);
   if (threadIdx.x == 0) {
-    spart[2 * blockIdx.x] = s0;
-    spart[2 * blockIdx.x + 1] = s1;
+    spart[2 * bid] = s0;
+    spart[2 * bid + 1] = s1;
   }
+}
+
+__global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
+                                            const double* wv_given, double* vpart, double* spart) {
+  vxc_body(blockIdx.x, gridDim.x, ng, n, ao, w, rho, wv_given, vpart, spart);
 }
 
 size_t rho_lds(int n) { return (size_t)64 * (((n + 15) & ~15) + 1) * 8; }
 size_t vxc_lds(int n) { return ((size_t)CH * (((n + 15) & ~15) + 1) + CH) * 8; }
 constexpr int VXC_BLOCKS = 256;
 
-void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, const int* frozen, hipStream_t st) {
-  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT), rho_lds(n), st, ng, n, ao, dm, rho, frozen);
+void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, hipStream_t st) {
+  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT), rho_lds(n), st, ng, n, ao, dm, rho);
 }
 
-int vxc_blocks(long ng) {
+__host__ __device__ inline int vxc_blocks(long ng) {
   const long nchunk = (ng + CH - 1) / CH;
   return (int)(nchunk < VXC_BLOCKS ? nchunk : VXC_BLOCKS);
 }
@@ -650,7 +658,7 @@ __device__ __forceinline__ double gelu_t_grad(double x) {
 struct NNOff {
   size_t W1, b1, W2, b2, W3, b3, total;  // offsets (doubles) into the packed parameters
 };
-NNOff nn_offsets(long ng, int h1, int h2) {
+__host__ __device__ inline NNOff nn_offsets(long ng, int h1, int h2) {
   NNOff o;
   o.W1 = 0;
   o.b1 = o.W1 + (size_t)ng * h1;
@@ -663,17 +671,16 @@ NNOff nn_offsets(long ng, int h1, int h2) {
 }
 
 constexpr int GNN_BLOCKS = 1024;
-int gnn_blocks(long ng) {
+__host__ __device__ inline int gnn_blocks(long ng) {
   const long b = (ng + 15) / 16;
   return (int)(b < GNN_BLOCKS ? b : GNN_BLOCKS);
 }
 
 // partial z1 over a contiguous chunk of grid rows per workgroup (8 independent row streams per thread)
-__global__ __launch_bounds__(NT) void k_gnn_l1(long ng, int h1, const double* rho, const double* W1, double* zpart,
-                                               const int* frozen) {
-  if (frozen && *frozen) return;
-  const long per = (ng + gridDim.x - 1) / gridDim.x;
-  const long g0 = (long)blockIdx.x * per;
+__device__ __forceinline__ void gnn_l1_body(int bid, int nblk, long ng, int h1, const double* rho, const double* W1,
+                                            double* zpart) {
+  const long per = (ng + nblk - 1) / nblk;
+  const long g0 = (long)bid * per;
   const long g1 = g0 + per < ng ? g0 + per : ng;
   for (int c = threadIdx.x; c < h1; c += NT) {
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -683,17 +690,20 @@ __global__ __launch_bounds__(NT) void k_gnn_l1(long ng, int h1, const double* rh
       for (int u = 0; u < 8; ++u) a[u] = fma(rho[g + u], W1[(size_t)(g + u) * h1 + c], a[u]);
     }
     for (; g < g1; ++g) a[0] = fma(rho[g], W1[(size_t)g * h1 + c], a[0]);
-    zpart[(size_t)blockIdx.x * h1 + c] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    zpart[(size_t)bid * h1 + c] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
 }
 
+__global__ __launch_bounds__(NT) void k_gnn_l1(long ng, int h1, const double* rho, const double* W1, double* zpart) {
+  gnn_l1_body(blockIdx.x, gridDim.x, ng, h1, rho, W1, zpart);
+}
+
 // z1 = sum of the partials (fixed order) + b1; 64 columns per workgroup, 16 partial streams each
-__global__ __launch_bounds__(1024) void k_gnn_red(int nb, int h1, const double* zpart, const double* b1, double* z1,
-                                                  const int* frozen) {
-  if (frozen && *frozen) return;
+__device__ __forceinline__ void gnn_red_body(int bid, int nb, int h1, const double* zpart, const double* b1,
+                                             double* z1) {
   __shared__ double acc[16][64];
   const int cl = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int c = blockIdx.x * 64 + cl;
+  const int c = bid * 64 + cl;
   double v = 0.0;
   if (c < h1)
     for (int b = q; b < nb; b += 16) v += zpart[(size_t)b * h1 + c];
@@ -706,10 +716,13 @@ __global__ __launch_bounds__(1024) void k_gnn_red(int nb, int h1, const double* 
   }
 }
 
+__global__ __launch_bounds__(1024) void k_gnn_red(int nb, int h1, const double* zpart, const double* b1, double* z1) {
+  gnn_red_body(blockIdx.x, nb, h1, zpart, b1, z1);
+}
+
 // the small layers and the backward pass down to dz1 (one workgroup of 1024 threads)
-__global__ __launch_bounds__(1024) void k_gnn_mid(int h1, int h2, const double* P, NNOff o, double scale,
-                                                  const double* z1, double* dz1, double* excsum, const int* frozen) {
-  if (frozen && *frozen) return;
+__device__ __forceinline__ void gnn_mid_body(int h1, int h2, const double* P, NNOff o, double scale, const double* z1,
+                                             double* dz1, double* excsum) {
   extern __shared__ double sm[];
   double* a1 = sm;        // gelu(z1)   [h1]
   double* z2 = a1 + h1;   // [h2]
@@ -750,15 +763,24 @@ __global__ __launch_bounds__(1024) void k_gnn_mid(int h1, int h2, const double* 
   }
 }
 
+__global__ __launch_bounds__(1024) void k_gnn_mid(int h1, int h2, const double* P, NNOff o, double scale,
+                                                  const double* z1, double* dz1, double* excsum) {
+  gnn_mid_body(h1, h2, P, o, scale, z1, dz1, excsum);
+}
+
 // vrho[g] = sum_c W1[g][c] dz1[c]; writes wv[g] = w[g] vrho[g] / 2 (one wave per grid row)
-__global__ __launch_bounds__(NT) void k_gnn_vjp(long ng, int h1, const double* W1, const double* dz1, const double* w,
-                                                double* wv, const int* frozen) {
-  if (frozen && *frozen) return;
+__host__ __device__ inline int gnn_vjp_blocks(long ng) {
+  const long vb = (ng + NW - 1) / NW;
+  return (int)(vb < 2048 ? vb : 2048);
+}
+
+__device__ __forceinline__ void gnn_vjp_body(int bid, int nblk, long ng, int h1, const double* W1, const double* dz1,
+                                             const double* w, double* wv) {
   extern __shared__ double sdz[];
   for (int c = threadIdx.x; c < h1; c += NT) sdz[c] = dz1[c];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  for (long g = (long)blockIdx.x * NW + (threadIdx.x >> 6); g < ng; g += (long)gridDim.x * NW) {
+  for (long g = (long)bid * NW + (threadIdx.x >> 6); g < ng; g += (long)nblk * NW) {
     const double* row = W1 + (size_t)g * h1;
     double v = 0.0;
     for (int c = lane; c < h1; c += 64) v = fma(row[c], sdz[c], v);
@@ -767,33 +789,34 @@ __global__ __launch_bounds__(NT) void k_gnn_vjp(long ng, int h1, const double* W
   }
 }
 
+__global__ __launch_bounds__(NT) void k_gnn_vjp(long ng, int h1, const double* W1, const double* dz1, const double* w,
+                                                double* wv) {
+  gnn_vjp_body(blockIdx.x, gridDim.x, ng, h1, W1, dz1, w, wv);
+}
+
 void gnn_dispatch(long ng, int h1, int h2, double scale, const double* P, const double* rho, const double* w,
-                  double* zpart, double* z1, double* dz1, double* excsum, double* wv, const int* frozen,
-                  hipStream_t st) {
+                  double* zpart, double* z1, double* dz1, double* excsum, double* wv, hipStream_t st) {
   const NNOff o = nn_offsets(ng, h1, h2);
   const int nb = gnn_blocks(ng);
-  hipLaunchKernelGGL(k_gnn_l1, dim3(nb), dim3(NT), 0, st, ng, h1, rho, P + o.W1, zpart, frozen);
-  hipLaunchKernelGGL(k_gnn_red, dim3((h1 + 63) / 64), dim3(1024), 0, st, nb, h1, zpart, P + o.b1, z1, frozen);
+  hipLaunchKernelGGL(k_gnn_l1, dim3(nb), dim3(NT), 0, st, ng, h1, rho, P + o.W1, zpart);
+  hipLaunchKernelGGL(k_gnn_red, dim3((h1 + 63) / 64), dim3(1024), 0, st, nb, h1, zpart, P + o.b1, z1);
   hipLaunchKernelGGL(k_gnn_mid, dim3(1), dim3(1024), (size_t)(h1 + 2 * h2) * 8, st, h1, h2, P, o, scale, z1, dz1,
-                     excsum, frozen);
-  const long vb = (ng + NW - 1) / NW;
-  hipLaunchKernelGGL(k_gnn_vjp, dim3((unsigned)(vb < 2048 ? vb : 2048)), dim3(NT), (size_t)h1 * 8, st, ng, h1,
-                     P + o.W1, dz1, w, wv, frozen);
+                     excsum);
+  hipLaunchKernelGGL(k_gnn_vjp, dim3((unsigned)gnn_vjp_blocks(ng)), dim3(NT), (size_t)h1 * 8, st, ng, h1, P + o.W1,
+                     dz1, w, wv);
 }
 
 __global__ void k_copy1(const double* src, double* dst) {
   if (threadIdx.x == 0) *dst = *src;
 }
 
-__global__ void k_zero(double* p, long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
-}
-
 // J_ij = sum_kl (ij|kl) D_kl over the pair matrix (row P = pair(i,j)); K (optional) by gathers.
-__global__ __launch_bounds__(NT) void k_coulomb(int n, int npair, const double* eri, const double* dm, double* J,
-                                                double* K, const int* frozen) {
+__host__ __device__ inline int coulomb_blocks(int npair) { return npair < 2048 ? npair : 2048; }
+
+// workgroup bid of nblk
+__device__ __forceinline__ void coulomb_body(int bid, int nblk, int n, int npair, const double* eri, const double* dm,
+                                             double* J, double* K) {
   extern __shared__ double sDp[];  // D'_Q = D_kl (2 - delta_kl)
-  if (frozen && *frozen) return;
   __shared__ double red[NW];
   for (int q = threadIdx.x; q < npair; q += NT) {
     int k = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
@@ -803,7 +826,7 @@ __global__ __launch_bounds__(NT) void k_coulomb(int n, int npair, const double* 
     sDp[q] = k == l ? dm[k * n + l] : dm[k * n + l] + dm[l * n + k];
   }
   __syncthreads();
-  for (int P = blockIdx.x; P < npair; P += gridDim.x) {
+  for (int P = bid; P < npair; P += nblk) {
     const double* row = eri + (size_t)P * npair;
     double s = 0.0;
     for (int q = threadIdx.x; q < npair; q += NT) s = fma(row[q], sDp[q], s);
@@ -819,7 +842,7 @@ __global__ __launch_bounds__(NT) void k_coulomb(int n, int npair, const double* 
   }
   if (K) {
     // K_ij = sum_kl (ik|jl) D_kl (Hartree-Fock only; O(n^4) gathers)
-    for (int ij = blockIdx.x; ij < n * n; ij += gridDim.x) {
+    for (int ij = bid; ij < n * n; ij += nblk) {
       const int i = ij / n, j = ij % n;
       double s = 0.0;
       for (int kl = threadIdx.x; kl < n * n; kl += NT) {
@@ -830,6 +853,11 @@ __global__ __launch_bounds__(NT) void k_coulomb(int n, int npair, const double* 
       if (threadIdx.x == 0) K[i * n + j] = s;
     }
   }
+}
+
+__global__ __launch_bounds__(NT) void k_coulomb(int n, int npair, const double* eri, const double* dm, double* J,
+                                                double* K) {
+  coulomb_body(blockIdx.x, gridDim.x, n, npair, eri, dm, J, K);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -916,10 +944,9 @@ __device__ void orthogonaliser(int n, const double* S, double* L, double* X) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(NT) void k_orthogonaliser(int n, const double* S, double* L, double* X) {
-  orthogonaliser(n, S, L, X);
-}
-
+// One molecule of an SCF launch.  The records of a batch sit in device memory; every per-cycle
+// kernel finds its molecule from the block index (single-workgroup kernels: blockIdx.x,
+// multi-workgroup kernels: blockIdx.y) and exits past that molecule's own block count.
 struct Scf3 {
   int n, npair, nocc, max_cycle, diis_space, method;  // method 0 RKS-LDA, 1 RHF, 2 RKS with a global network
   long ng;
@@ -927,7 +954,9 @@ struct Scf3 {
   double conv_tol, enuc;
   const double* S;
   const double* H;
-  const double* X;  // orthogonaliser
+  double* X;   // orthogonaliser
+  double* Lc;  // Cholesky factor of S (scratch)
+  int* status_out;
   const double* eri;
   const double* ao;
   const double* w;
@@ -946,7 +975,8 @@ struct Scf3 {
 };
 
 // Fock + CDIIS (one workgroup).  cycle < 0: plain F = H + Veff (no DIIS).
-__global__ __launch_bounds__(NT) void k_scf_fock(Scf3 s, int cycle) {
+__global__ __launch_bounds__(NT) void k_scf_fock(const Scf3* __restrict__ ss, int cycle) {
+  const Scf3 s = ss[blockIdx.x];
   if (s.ist[0]) return;
   __shared__ double red[NW];
   const int n = s.n, nn = n * n;
@@ -1046,7 +1076,8 @@ __global__ __launch_bounds__(NT) void k_scf_fock(Scf3 s, int cycle) {
 constexpr int NT_EIG = 1024;
 size_t eigh_lds(int n) { return (size_t)2 * n * (n + 1) * 8; }
 
-__global__ __launch_bounds__(NT_EIG) void k_scf_eigh(Scf3 s) {
+__global__ __launch_bounds__(NT_EIG) void k_scf_eigh(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.x];
   if (s.ist[0]) return;
   extern __shared__ double sm[];
   __shared__ qexeig::Smem sh;
@@ -1094,7 +1125,8 @@ __global__ __launch_bounds__(NT_EIG) void k_gen_eigh(int n, const double* F, con
 }
 
 // Vxc = sum of chunk partials (fixed order), symmetrised; energies and convergence.
-__global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
+__global__ __launch_bounds__(NT) void k_scf_energy(const Scf3* __restrict__ ss, int cycle) {
+  const Scf3 s = ss[blockIdx.x];
   if (s.ist[0]) return;
   __shared__ double red[NW];
   const int n = s.n, nn = n * n;
@@ -1160,19 +1192,106 @@ __global__ __launch_bounds__(NT) void k_scf_energy(Scf3 s, int cycle) {
     if (fabs(etot - elast) < s.conv_tol && gnorm < sqrt(s.conv_tol)) {
       s.ist[0] = 1;
       s.ist[4] = 1;
+    } else if (cycle + 1 >= s.max_cycle) {
+      s.ist[0] = 1;  // out of cycles: frozen, not converged (a batch runs to its longest member)
     }
   }
 }
 
-__global__ void k_scf_init(Scf3 s) {
+// state flags and scalars cleared; J, Vxc, K and D zeroed for the core-Hamiltonian guess F = H
+__global__ __launch_bounds__(NT) void k_scf_init(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.x];
   if (threadIdx.x < 8) s.ist[threadIdx.x] = 0;
   if (threadIdx.x < 8) s.scal[threadIdx.x] = 0.0;
+  const int nn = s.n * s.n;
+  for (int t = threadIdx.x; t < nn; t += NT) {
+    s.J[t] = 0.0;
+    s.Vxc[t] = 0.0;
+    s.K[t] = 0.0;
+    s.D[t] = 0.0;
+  }
 }
 
-// fill e_hist beyond the last cycle with the final energy (frozen state repeats)
-__global__ void k_scf_finish(Scf3 s) {
+__global__ __launch_bounds__(NT) void k_scf_orthogonaliser(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.x];
+  orthogonaliser(s.n, s.S, s.Lc, s.X);
+}
+
+// fill e_hist beyond the last cycle with the final energy (frozen state repeats); status out
+__global__ __launch_bounds__(NT) void k_scf_finish(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.x];
   for (int c = threadIdx.x; c <= s.max_cycle; c += blockDim.x)
     if (c > s.ist[1]) s.e_hist[c] = s.scal[0];
+  if (threadIdx.x < 8) s.status_out[threadIdx.x] = s.ist[threadIdx.x];
+}
+
+// molecules of the batch still iterating (one thread; the host reads it when it polls)
+__global__ void k_scf_active(const Scf3* __restrict__ ss, int B, int* out) {
+  if (threadIdx.x != 0) return;
+  int c = 0;
+  for (int b = 0; b < B; ++b) c += ss[b].ist[0] ? 0 : 1;
+  out[0] = c;
+}
+
+// The records reach the device as kernel arguments, PUT_N at a time (no host copy, stream ordered).
+constexpr int PUT_N = 4;
+static_assert(sizeof(Scf3) % 8 == 0, "Scf3 is copied in 8-byte words");
+struct ScfPut {
+  Scf3 r[PUT_N];
+};
+__global__ void k_put_scf(ScfPut pk, int count, Scf3* dst) {
+  constexpr int W = sizeof(Scf3) / 8;
+  for (int t = threadIdx.x; t < count * W; t += blockDim.x) {
+    const int i = t / W, w = t % W;
+    reinterpret_cast<unsigned long long*>(dst + i)[w] = reinterpret_cast<const unsigned long long*>(&pk.r[i])[w];
+  }
+}
+
+// ---- per-molecule wrappers of the grid and Coulomb kernels (blockIdx.y = molecule) ----
+__global__ __launch_bounds__(NT) void k_rho_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.method == QEX3D_RHF || s.ist[0] || (long)blockIdx.x >= (s.ng + 63) / 64) return;
+  rho_body(blockIdx.x, s.ng, s.n, s.ao, s.D, s.rho);
+}
+
+__global__ __launch_bounds__(NT) void k_vxc_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
+  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.method == QEX3D_RKS_NN_GLOBAL ? s.wv : nullptr,
+           s.vpart, s.spart);
+}
+
+__global__ __launch_bounds__(NT) void k_gnn_l1_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  const int nb = gnn_blocks(s.ng);
+  if (s.method != QEX3D_RKS_NN_GLOBAL || s.ist[0] || (int)blockIdx.x >= nb) return;
+  gnn_l1_body(blockIdx.x, nb, s.ng, s.h1, s.rho, s.nn + nn_offsets(s.ng, s.h1, s.h2).W1, s.zpart);
+}
+
+__global__ __launch_bounds__(1024) void k_gnn_red_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.method != QEX3D_RKS_NN_GLOBAL || s.ist[0] || (int)blockIdx.x >= (s.h1 + 63) / 64) return;
+  gnn_red_body(blockIdx.x, gnn_blocks(s.ng), s.h1, s.zpart, s.nn + nn_offsets(s.ng, s.h1, s.h2).b1, s.z1);
+}
+
+__global__ __launch_bounds__(1024) void k_gnn_mid_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.method != QEX3D_RKS_NN_GLOBAL || s.ist[0]) return;
+  gnn_mid_body(s.h1, s.h2, s.nn, nn_offsets(s.ng, s.h1, s.h2), s.nn_scale, s.z1, s.dz1, s.scal + 5);
+}
+
+__global__ __launch_bounds__(NT) void k_gnn_vjp_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  const int nb = gnn_vjp_blocks(s.ng);
+  if (s.method != QEX3D_RKS_NN_GLOBAL || s.ist[0] || (int)blockIdx.x >= nb) return;
+  gnn_vjp_body(blockIdx.x, nb, s.ng, s.h1, s.nn + nn_offsets(s.ng, s.h1, s.h2).W1, s.dz1, s.w, s.wv);
+}
+
+__global__ __launch_bounds__(NT) void k_coulomb_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  const int nb = coulomb_blocks(s.npair);
+  if (s.ist[0] || (int)blockIdx.x >= nb) return;
+  coulomb_body(blockIdx.x, nb, s.n, s.npair, s.eri, s.D, s.J, s.method == QEX3D_RHF ? s.K : nullptr);
 }
 
 struct Lay3 {
@@ -1264,13 +1383,14 @@ int check_basis(const qex3d_basis* b) {
 
 // P[e] = sum_c vpart[c][e] for 32 entries per workgroup (8 interleaved partial sums combined in a
 // fixed order); the extra last workgroup sums the (nelec, exc) chunk partials by a fixed tree.
-__global__ __launch_bounds__(256) void k_vxc_sum(int nn, int nchunk, const double* vpart, const double* spart,
-                                                 double* P, double* out2, const int* frozen) {
-  if (frozen && *frozen) return;
+__host__ __device__ inline int vxc_sum_blocks(int nn) { return (nn + 31) / 32 + 1; }
+
+__device__ __forceinline__ void vxc_sum_body(int bid, int nblk, int nn, int nchunk, const double* vpart,
+                                             const double* spart, double* P, double* out2) {
   __shared__ double red[8][33];
   __shared__ double s2[2][256];
   const int t = threadIdx.x;
-  if (blockIdx.x == gridDim.x - 1) {
+  if (bid == nblk - 1) {
     s2[0][t] = t < nchunk ? spart[2 * t] : 0.0;
     s2[1][t] = t < nchunk ? spart[2 * t + 1] : 0.0;
     __syncthreads();
@@ -1287,7 +1407,7 @@ __global__ __launch_bounds__(256) void k_vxc_sum(int nn, int nchunk, const doubl
     }
     return;
   }
-  const int e = blockIdx.x * 32 + (t & 31), part = t >> 5;
+  const int e = bid * 32 + (t & 31), part = t >> 5;
   double a = 0.0;
   if (e < nn)
     for (int c = part; c < nchunk; c += 8) a += vpart[(size_t)c * nn + e];
@@ -1300,19 +1420,39 @@ __global__ __launch_bounds__(256) void k_vxc_sum(int nn, int nchunk, const doubl
   }
 }
 
+__global__ __launch_bounds__(256) void k_vxc_sum(int nn, int nchunk, const double* vpart, const double* spart,
+                                                 double* P, double* out2) {
+  vxc_sum_body(blockIdx.x, gridDim.x, nn, nchunk, vpart, spart, P, out2);
+}
+
+__global__ __launch_bounds__(256) void k_vxc_sum_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  const int nn = s.n * s.n, nb = vxc_sum_blocks(nn);
+  if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= nb) return;
+  vxc_sum_body(blockIdx.x, nb, nn, s.nchunk, s.vpart, s.spart, s.tmp2, s.scal + 2);
+}
+
 // vmat = P + P^T
-__global__ void k_sym_add(int n, const double* P, double* vmat, const int* frozen) {
-  if (frozen && *frozen) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sym_add_body(int t, int n, const double* P, double* vmat) {
   if (t < n * n) vmat[t] = P[t] + P[(t % n) * n + t / n];
+}
+
+__global__ void k_sym_add(int n, const double* P, double* vmat) {
+  sym_add_body(blockIdx.x * blockDim.x + threadIdx.x, n, P, vmat);
+}
+
+__global__ __launch_bounds__(256) void k_sym_add_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.method == QEX3D_RHF || s.ist[0]) return;
+  sym_add_body(blockIdx.x * blockDim.x + threadIdx.x, s.n, s.tmp2, s.Vxc);
 }
 
 // Vxc from the k_vxc chunk partials (nchunk <= VXC_BLOCKS = 256); P is n^2 scratch
 void vxc_reduce(int n, int nchunk, const double* vpart, const double* spart, double* P, double* vmat, double* out2,
-                const int* frozen, hipStream_t st) {
+                hipStream_t st) {
   const int nn = n * n;
-  hipLaunchKernelGGL(k_vxc_sum, dim3((nn + 31) / 32 + 1), dim3(256), 0, st, nn, nchunk, vpart, spart, P, out2, frozen);
-  hipLaunchKernelGGL(k_sym_add, dim3((nn + 255) / 256), dim3(256), 0, st, n, P, vmat, frozen);
+  hipLaunchKernelGGL(k_vxc_sum, dim3(vxc_sum_blocks(nn)), dim3(256), 0, st, nn, nchunk, vpart, spart, P, out2);
+  hipLaunchKernelGGL(k_sym_add, dim3((nn + 255) / 256), dim3(256), 0, st, n, P, vmat);
 }
 
 }  // namespace
@@ -1385,7 +1525,7 @@ int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* d
   if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !dm || !rho) return fail3(QEX_E_BADARG, "bad eval_rho arguments");
   {
     qexprof::Scope ps(qexprof::K_RHO, (hipStream_t)stream);
-    rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, (hipStream_t)stream);
+    rho_dispatch(ngrid, nao, ao, dm, rho, (hipStream_t)stream);
   }
   return launch_status("qex3d_eval_rho");
 }
@@ -1409,17 +1549,17 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   double* spart = (double*)((char*)vpart + al((size_t)nb * nao * nao * 8));
   {
     qexprof::Scope ps(qexprof::K_RHO, st);
-    rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, st);
+    rho_dispatch(ngrid, nao, ao, dm, rho, st);
   }
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
     hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho,
-                       nullptr, vpart, spart, nullptr);
+                       nullptr, vpart, spart);
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
     double* P = (double*)((char*)spart + al((size_t)nb * 16));
-    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, nullptr, st);
+    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, st);
   }
   return launch_status("qex3d_nr_rks");
 }
@@ -1432,10 +1572,29 @@ static bool nn_widths_ok(int h1, int h2) {
   return h1 >= 1 && h2 >= 1 && h1 <= QEX3D_MAX_NN_HIDDEN && h2 <= QEX3D_MAX_NN_HIDDEN;
 }
 
-size_t qex3d_scf_workspace_bytes(const qex3d_scf_problem* p) {
+// one molecule's region: the state layout, then the Cholesky factor and the orthogonaliser
+static size_t scf_mol_bytes(const qex3d_scf_problem* p) {
   if (!p || p->nao < 1 || p->nao > QEX3D_MAX_NAO || scf_ngrid(p) < 1) return 0;
   if (p->method == QEX3D_RKS_NN_GLOBAL && !nn_widths_ok(p->nn_h1, p->nn_h2)) return 0;
-  return layout3(p->nao, scf_ngrid(p), scf_space(p), scf_h1(p)).total + al((size_t)p->nao * p->nao * 8) * 2;
+  return al(layout3(p->nao, scf_ngrid(p), scf_space(p), scf_h1(p)).total + al((size_t)p->nao * p->nao * 8) * 2);
+}
+// the records of a batch and the host-poll counter come first
+static size_t scf_head_bytes(int B) { return al((size_t)B * sizeof(Scf3)) + al(64); }
+
+size_t qex3d_scf_workspace_bytes(const qex3d_scf_problem* p) {
+  const size_t m = scf_mol_bytes(p);
+  return m ? scf_head_bytes(1) + m : 0;
+}
+
+size_t qex3d_scf_batch_workspace_bytes(int32_t batch, const qex3d_scf_problem* problems) {
+  if (batch < 1 || batch > QEX3D_MAX_SCF_BATCH || !problems) return 0;
+  size_t total = scf_head_bytes(batch);
+  for (int b = 0; b < batch; ++b) {
+    const size_t m = scf_mol_bytes(problems + b);
+    if (!m) return 0;
+    total += m;
+  }
+  return total;
 }
 
 size_t qex3d_nr_rks_nn_workspace_bytes(int64_t ngrid, int32_t nao, int32_t h1, int32_t h2) {
@@ -1470,21 +1629,21 @@ int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* 
   double* exc = (double*)q;
   {
     qexprof::Scope ps(qexprof::K_RHO, st);
-    rho_dispatch(ngrid, nao, ao, dm, rho, nullptr, st);
+    rho_dispatch(ngrid, nao, ao, dm, rho, st);
   }
   {
     qexprof::Scope ps(qexprof::K_NN, st);
-    gnn_dispatch(ngrid, h1, h2, scale, nn_params, rho, weights, zpart, z1, dz1, exc, wv, nullptr, st);
+    gnn_dispatch(ngrid, h1, h2, scale, nn_params, rho, weights, zpart, z1, dz1, exc, wv, st);
   }
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
     hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, wv,
-                       vpart, spart, nullptr);
+                       vpart, spart);
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
     double* P = (double*)((char*)spart + al((size_t)nb * 16));
-    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, nullptr, st);
+    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, st);
     hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, st, exc, nelec_excsum + 1);
   }
   return launch_status("qex3d_nr_rks_nn");
@@ -1510,141 +1669,211 @@ int qex3d_gen_eigh(int32_t batch, int32_t n, const double* F, const double* S, d
   return launch_status("qex3d_gen_eigh");
 }
 
-int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
-              size_t workspace_bytes, void* stream) {
-  if (!p || !in || !out) return fail3(QEX_E_BADARG, "null argument");
+// argument checks of one molecule; `pre` prefixes the message (the molecule's index in a batch)
+static int scf_check(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out,
+                     const char* pre) {
   const int n = p->nao;
-  if (n < 1 || n > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "nao must be in [1, %d]", QEX3D_MAX_NAO);
+  if (n < 1 || n > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "%snao must be in [1, %d]", pre, QEX3D_MAX_NAO);
   if (p->nelectron < 2 || (p->nelectron & 1) || p->nelectron / 2 > n)
-    return fail3(QEX_E_BADARG, "closed-shell nelectron required (got %d)", p->nelectron);
+    return fail3(QEX_E_BADARG, "%sclosed-shell nelectron required (got %d)", pre, p->nelectron);
   if (p->method != QEX3D_RKS_LDA && p->method != QEX3D_RHF && p->method != QEX3D_RKS_NN_GLOBAL)
-    return fail3(QEX_E_BADARG, "bad method");
-  if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->ao || !in->weights)) return fail3(QEX_E_BADARG, "RKS needs a grid");
+    return fail3(QEX_E_BADARG, "%sbad method", pre);
+  if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->ao || !in->weights))
+    return fail3(QEX_E_BADARG, "%sRKS needs a grid", pre);
   if (p->method == QEX3D_RKS_NN_GLOBAL && (!in->nn_params || !nn_widths_ok(p->nn_h1, p->nn_h2)))
-    return fail3(QEX_E_BADARG, "network parameters and hidden widths in [1, %d] required", QEX3D_MAX_NN_HIDDEN);
-  if (p->max_cycle < 1 || p->diis_space < 0 || p->diis_space > QEX3D_MAX_DIIS) return fail3(QEX_E_BADARG, "bad cycle/diis");
-  if (!in->S || !in->H || !in->eri) return fail3(QEX_E_BADARG, "missing S, H or eri");
+    return fail3(QEX_E_BADARG, "%snetwork parameters and hidden widths in [1, %d] required", pre,
+                 QEX3D_MAX_NN_HIDDEN);
+  if (p->max_cycle < 1 || p->diis_space < 0 || p->diis_space > QEX3D_MAX_DIIS)
+    return fail3(QEX_E_BADARG, "%sbad cycle/diis", pre);
+  if (!in->S || !in->H || !in->eri) return fail3(QEX_E_BADARG, "%smissing S, H or eri", pre);
   if (!out->e_tot || !out->mo_energy || !out->mo_coeff || !out->dm || !out->status)
-    return fail3(QEX_E_BADARG, "missing output array");
-  const size_t need = qex3d_scf_workspace_bytes(p);
+    return fail3(QEX_E_BADARG, "%smissing output array", pre);
+  return QEX_OK;
+}
+
+// The SCF of B molecules of any sizes in one sequence of launches: every stage is one launch over
+// the batch, each molecule computes exactly what it computes alone (same per-molecule block
+// counts and reduction orders), and a molecule that has converged or used its cycles is frozen.
+static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* ins, const qex3d_scf_outputs* outs,
+                   void* workspace, size_t workspace_bytes, hipStream_t st, const char* name) {
+  if (!ps || !ins || !outs) return fail3(QEX_E_BADARG, "null argument");
+  char pre[32] = "";
+  for (int b = 0; b < B; ++b) {
+    if (B > 1) snprintf(pre, sizeof(pre), "molecule %d: ", b);
+    const int rc = scf_check(ps + b, ins + b, outs + b, pre);
+    if (rc) return rc;
+  }
+  const size_t need = B == 1 ? qex3d_scf_workspace_bytes(ps) : qex3d_scf_batch_workspace_bytes(B, ps);
+  if (!need) return fail3(QEX_E_BADARG, "bad problem sizes");
   if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
-  hipStream_t st = (hipStream_t)stream;
-  const long ng = scf_ngrid(p);
-  const int space = scf_space(p);
-  const Lay3 L = layout3(n, ng, space, scf_h1(p));
-  if (L.total + 2 * al((size_t)n * n * 8) > need) return fail3(QEX_E_WORKSPACE, "internal layout exceeds workspace");
   char* ws = (char*)workspace;
-  Scf3 s{};
-  s.n = n;
-  s.npair = n * (n + 1) / 2;
-  s.nocc = p->nelectron / 2;
-  s.max_cycle = p->max_cycle;
-  s.diis_space = p->diis_space;
-  s.method = p->method;
-  s.ng = ng;
-  s.nchunk = vxc_blocks(ng);
-  s.conv_tol = p->conv_tol;
-  s.enuc = p->energy_nuc;
-  s.S = in->S;
-  s.H = in->H;
-  s.eri = in->eri;
-  s.ao = in->ao;
-  s.w = in->weights;
-  double* Lchol = (double*)(ws + L.total);
-  double* X = (double*)(ws + L.total + al((size_t)n * n * 8));
-  s.X = X;
-  s.F = (double*)(ws + L.F);
-  s.Fd = (double*)(ws + L.Fd);
-  s.D = out->dm;
-  s.Dlast = (double*)(ws + L.Dlast);
-  s.C = out->mo_coeff;
-  s.Cp = (double*)(ws + L.Cp);
-  s.mo = out->mo_energy;
-  s.J = (double*)(ws + L.J);
-  s.K = (double*)(ws + L.K);
-  s.Vxc = (double*)(ws + L.Vxc);
-  s.vpart = (double*)(ws + L.vpart);
-  s.spart = (double*)(ws + L.spart);
-  s.rho = (double*)(ws + L.rho);
-  s.tmp1 = (double*)(ws + L.tmp1);
-  s.tmp2 = (double*)(ws + L.tmp2);
-  s.diisF = (double*)(ws + L.diisF);
-  s.diisE = (double*)(ws + L.diisE);
-  s.diisB = (double*)(ws + L.diisB);
-  s.scal = (double*)(ws + L.scal);
-  s.ist = (int*)(ws + L.ist);
-  s.e_hist = out->e_tot;
-  s.nn = in->nn_params;
-  s.h1 = scf_h1(p);
-  s.h2 = p->nn_h2;
-  s.nn_scale = p->nn_scale;
-  s.zpart = (double*)(ws + L.zpart);
-  s.z1 = (double*)(ws + L.z1);
-  s.dz1 = (double*)(ws + L.dz1);
-  s.wv = (double*)(ws + L.wv);
-  s.eigf = (double*)(ws + L.eigf);
-  const size_t lds_eigh = eigh_lds(n);
+  Scf3* dss = (Scf3*)ws;
+  int* active = (int*)(ws + al((size_t)B * sizeof(Scf3)));
+  size_t off = scf_head_bytes(B);
+  std::vector<Scf3> rec(B);
+  int nmax = 1, max_cycle = 0, sync_every = 0, npair_max = 1, h1_max = 1, h2_max = 1;
+  long rho_blk = 0, vxc_blk = 0, sum_blk = 0, sym_blk = 0, j_blk = 1, l1_blk = 0, red_blk = 0, vjp_blk = 0;
+  bool any_grid = false, any_nn = false;
+  for (int b = 0; b < B; ++b) {
+    const qex3d_scf_problem* p = ps + b;
+    const int n = p->nao;
+    const long ng = scf_ngrid(p);
+    const Lay3 L = layout3(n, ng, scf_space(p), scf_h1(p));
+    char* w = ws + off;
+    off += scf_mol_bytes(p);
+    Scf3& s = rec[b];
+    s = Scf3{};
+    s.n = n;
+    s.npair = n * (n + 1) / 2;
+    s.nocc = p->nelectron / 2;
+    s.max_cycle = p->max_cycle;
+    s.diis_space = p->diis_space;
+    s.method = p->method;
+    s.ng = ng;
+    s.nchunk = vxc_blocks(ng);
+    s.conv_tol = p->conv_tol;
+    s.enuc = p->energy_nuc;
+    s.S = ins[b].S;
+    s.H = ins[b].H;
+    s.eri = ins[b].eri;
+    s.ao = ins[b].ao;
+    s.w = ins[b].weights;
+    s.Lc = (double*)(w + L.total);
+    s.X = (double*)(w + L.total + al((size_t)n * n * 8));
+    s.status_out = outs[b].status;
+    s.F = (double*)(w + L.F);
+    s.Fd = (double*)(w + L.Fd);
+    s.D = outs[b].dm;
+    s.Dlast = (double*)(w + L.Dlast);
+    s.C = outs[b].mo_coeff;
+    s.Cp = (double*)(w + L.Cp);
+    s.mo = outs[b].mo_energy;
+    s.J = (double*)(w + L.J);
+    s.K = (double*)(w + L.K);
+    s.Vxc = (double*)(w + L.Vxc);
+    s.vpart = (double*)(w + L.vpart);
+    s.spart = (double*)(w + L.spart);
+    s.rho = (double*)(w + L.rho);
+    s.tmp1 = (double*)(w + L.tmp1);
+    s.tmp2 = (double*)(w + L.tmp2);
+    s.diisF = (double*)(w + L.diisF);
+    s.diisE = (double*)(w + L.diisE);
+    s.diisB = (double*)(w + L.diisB);
+    s.scal = (double*)(w + L.scal);
+    s.ist = (int*)(w + L.ist);
+    s.e_hist = outs[b].e_tot;
+    s.nn = ins[b].nn_params;
+    s.h1 = scf_h1(p);
+    s.h2 = p->nn_h2;
+    s.nn_scale = p->nn_scale;
+    s.zpart = (double*)(w + L.zpart);
+    s.z1 = (double*)(w + L.z1);
+    s.dz1 = (double*)(w + L.dz1);
+    s.wv = (double*)(w + L.wv);
+    s.eigf = (double*)(w + L.eigf);
+    nmax = n > nmax ? n : nmax;
+    max_cycle = p->max_cycle > max_cycle ? p->max_cycle : max_cycle;
+    if (p->sync_every > 0 && (sync_every == 0 || p->sync_every < sync_every)) sync_every = p->sync_every;
+    npair_max = s.npair > npair_max ? s.npair : npair_max;
+    j_blk = coulomb_blocks(s.npair) > j_blk ? coulomb_blocks(s.npair) : j_blk;
+    if (p->method != QEX3D_RHF) {
+      any_grid = true;
+      rho_blk = (ng + 63) / 64 > rho_blk ? (ng + 63) / 64 : rho_blk;
+      vxc_blk = s.nchunk > vxc_blk ? s.nchunk : vxc_blk;
+      sum_blk = vxc_sum_blocks(n * n) > sum_blk ? vxc_sum_blocks(n * n) : sum_blk;
+      sym_blk = (n * n + 255) / 256 > sym_blk ? (n * n + 255) / 256 : sym_blk;
+    }
+    if (p->method == QEX3D_RKS_NN_GLOBAL) {
+      any_nn = true;
+      l1_blk = gnn_blocks(ng) > l1_blk ? gnn_blocks(ng) : l1_blk;
+      red_blk = (s.h1 + 63) / 64 > red_blk ? (s.h1 + 63) / 64 : red_blk;
+      vjp_blk = gnn_vjp_blocks(ng) > vjp_blk ? gnn_vjp_blocks(ng) : vjp_blk;
+      h1_max = s.h1 > h1_max ? s.h1 : h1_max;
+      h2_max = s.h2 > h2_max ? s.h2 : h2_max;
+    }
+  }
+  if (off > need) return fail3(QEX_E_WORKSPACE, "internal layout exceeds workspace");
+  const size_t lds_eigh = eigh_lds(nmax);
   if (lds_eigh + sizeof(qexeig::Smem) > 160 * 1024) return fail3(QEX_E_UNSUPPORTED, "nao too large for the LDS eigensolver");
   if (lds_eigh > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_scf_eigh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eigh);
-  const size_t lds_vxc = vxc_lds(n);
-  const int jblocks = s.npair < 2048 ? s.npair : 2048;
+  for (int b0 = 0; b0 < B; b0 += PUT_N) {
+    ScfPut pk{};
+    const int cnt = B - b0 < PUT_N ? B - b0 : PUT_N;
+    for (int i = 0; i < cnt; ++i) pk.r[i] = rec[b0 + i];
+    hipLaunchKernelGGL(k_put_scf, dim3(1), dim3(256), 0, st, pk, cnt, dss + b0);
+  }
+  const unsigned nb = (unsigned)B;
   auto veff = [&]() {
-    if (s.method != QEX3D_RHF) {
+    if (any_grid) {
       {
-        qexprof::Scope ps(qexprof::K_RHO, st);
-        rho_dispatch(ng, n, s.ao, s.D, s.rho, s.ist, st);
+        qexprof::Scope ps_(qexprof::K_RHO, st);
+        hipLaunchKernelGGL(k_rho_b, dim3((unsigned)rho_blk, nb), dim3(NT), rho_lds(nmax), st, dss);
       }
-      if (s.method == QEX3D_RKS_NN_GLOBAL) {
-        qexprof::Scope ps(qexprof::K_NN, st);
-        gnn_dispatch(ng, s.h1, s.h2, s.nn_scale, s.nn, s.rho, s.w, s.zpart, s.z1, s.dz1, s.scal + 5, s.wv, s.ist, st);
+      if (any_nn) {
+        qexprof::Scope ps_(qexprof::K_NN, st);
+        hipLaunchKernelGGL(k_gnn_l1_b, dim3((unsigned)l1_blk, nb), dim3(NT), 0, st, dss);
+        hipLaunchKernelGGL(k_gnn_red_b, dim3((unsigned)red_blk, nb), dim3(1024), 0, st, dss);
+        hipLaunchKernelGGL(k_gnn_mid_b, dim3(1, nb), dim3(1024), (size_t)(h1_max + 2 * h2_max) * 8, st, dss);
+        hipLaunchKernelGGL(k_gnn_vjp_b, dim3((unsigned)vjp_blk, nb), dim3(NT), (size_t)h1_max * 8, st, dss);
       }
       {
-        qexprof::Scope ps(qexprof::K_VXC, st);
-        hipLaunchKernelGGL(k_vxc, dim3((unsigned)s.nchunk), dim3(NT), lds_vxc, st, ng, n, s.ao, s.w, s.rho,
-                           s.method == QEX3D_RKS_NN_GLOBAL ? (const double*)s.wv : nullptr, s.vpart, s.spart, s.ist);
+        qexprof::Scope ps_(qexprof::K_VXC, st);
+        hipLaunchKernelGGL(k_vxc_b, dim3((unsigned)vxc_blk, nb), dim3(NT), vxc_lds(nmax), st, dss);
       }
       // Vxc = P + P^T with P the fixed-order sum of the chunk partials; scal[2..3] = nelec, exc
-      qexprof::Scope ps(qexprof::K_OTHER, st);
-      vxc_reduce(n, s.nchunk, s.vpart, s.spart, s.tmp2, s.Vxc, s.scal + 2, s.ist, st);
+      qexprof::Scope ps_(qexprof::K_OTHER, st);
+      hipLaunchKernelGGL(k_vxc_sum_b, dim3((unsigned)sum_blk, nb), dim3(256), 0, st, dss);
+      hipLaunchKernelGGL(k_sym_add_b, dim3((unsigned)sym_blk, nb), dim3(256), 0, st, dss);
     }
-    qexprof::Scope ps(qexprof::K_COULOMB, st);
-    hipLaunchKernelGGL(k_coulomb, dim3(jblocks), dim3(NT), (size_t)s.npair * 8, st, n, s.npair, s.eri, s.D, s.J,
-                       s.method == QEX3D_RHF ? s.K : nullptr, s.ist);
+    qexprof::Scope ps_(qexprof::K_COULOMB, st);
+    hipLaunchKernelGGL(k_coulomb_b, dim3((unsigned)j_blk, nb), dim3(NT), (size_t)npair_max * 8, st, dss);
   };
-  auto small = [&](int kind, auto kernel, size_t lds, auto... args) {
-    qexprof::Scope ps(kind, st);
-    hipLaunchKernelGGL(kernel, dim3(1), dim3(NT), lds, st, args...);
+  auto per_mol = [&](int kind, auto kernel, auto... args) {
+    qexprof::Scope ps_(kind, st);
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(NT), 0, st, dss, args...);
   };
   auto eigh = [&]() {
-    qexprof::Scope ps(qexprof::K_EIGH, st);
-    hipLaunchKernelGGL(k_scf_eigh, dim3(1), dim3(NT_EIG), lds_eigh, st, s);
+    qexprof::Scope ps_(qexprof::K_EIGH, st);
+    hipLaunchKernelGGL(k_scf_eigh, dim3(nb), dim3(NT_EIG), lds_eigh, st, dss);
   };
-  small(qexprof::K_OTHER, k_scf_init, 0, s);
-  small(qexprof::K_OTHER, k_orthogonaliser, 0, n, s.S, Lchol, X);
-  // core-Hamiltonian guess: F = H (J, K, Vxc and D cleared by a kernel, not a memset)
-  for (double* z : {s.J, s.Vxc, s.K, s.D}) hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, st, z, (long)n * n);
-  small(qexprof::K_FOCK, k_scf_fock, 0, s, -1);
+  // core-Hamiltonian guess: F = H (k_scf_init zeroes J, K, Vxc and D with a kernel, not a memset)
+  per_mol(qexprof::K_OTHER, k_scf_init);
+  per_mol(qexprof::K_OTHER, k_scf_orthogonaliser);
+  per_mol(qexprof::K_FOCK, k_scf_fock, -1);
   eigh();
   veff();
-  small(qexprof::K_ENERGY, k_scf_energy, 0, s, -1);
-  for (int c = 0; c < p->max_cycle; ++c) {
-    small(qexprof::K_FOCK, k_scf_fock, 0, s, c);
+  per_mol(qexprof::K_ENERGY, k_scf_energy, -1);
+  for (int c = 0; c < max_cycle; ++c) {
+    per_mol(qexprof::K_FOCK, k_scf_fock, c);
     eigh();
     veff();
-    small(qexprof::K_ENERGY, k_scf_energy, 0, s, c);
-    if (p->sync_every > 0 && (c + 1) % p->sync_every == 0 && c + 1 < p->max_cycle) {
-      // optional host check: stop enqueueing cycles once the device has flagged convergence
-      int32_t frozen = 0;
-      if (hipMemcpyAsync(&frozen, s.ist, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    per_mol(qexprof::K_ENERGY, k_scf_energy, c);
+    if (sync_every > 0 && (c + 1) % sync_every == 0 && c + 1 < max_cycle) {
+      // optional host check: stop enqueueing cycles once every molecule is frozen
+      int32_t left = 0;
+      hipLaunchKernelGGL(k_scf_active, dim3(1), dim3(64), 0, st, dss, B, active);
+      if (hipMemcpyAsync(&left, active, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess)
         return fail3(QEX_E_LAUNCH, "convergence check failed");
-      if (frozen) break;
+      if (left == 0) break;
     }
   }
-  small(qexprof::K_OTHER, k_scf_finish, 0, s);
-  (void)hipMemcpyAsync(out->status, s.ist, 8 * sizeof(int), hipMemcpyDeviceToDevice, st);
-  return launch_status("qex3d_scf");
+  per_mol(qexprof::K_OTHER, k_scf_finish);
+  return launch_status(name);
+}
+
+int qex3d_scf(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, const qex3d_scf_outputs* out, void* workspace,
+              size_t workspace_bytes, void* stream) {
+  return scf_run(1, p, in, out, workspace, workspace_bytes, (hipStream_t)stream, "qex3d_scf");
+}
+
+int qex3d_scf_batch(int32_t batch, const qex3d_scf_problem* problems, const qex3d_scf_inputs* inputs,
+                    const qex3d_scf_outputs* outputs, void* workspace, size_t workspace_bytes, void* stream) {
+  if (batch < 1 || batch > QEX3D_MAX_SCF_BATCH)
+    return fail3(QEX_E_BADARG, "batch must be in [1, %d]", QEX3D_MAX_SCF_BATCH);
+  return scf_run(batch, problems, inputs, outputs, workspace, workspace_bytes, (hipStream_t)stream,
+                 "qex3d_scf_batch");
 }
 
 }  // extern "C"

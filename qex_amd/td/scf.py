@@ -47,6 +47,16 @@ class _SCF:
         """Enqueues the whole solve (integrals if not cached, grid, AO values, SCF cycles) on the
         current stream and returns at once; ``collect()`` reads the result.  No host
         synchronisation is made unless ``sync_every`` > 0."""
+        p, ins, outs = self._prepare(**kwargs)
+        lib = _lib.load()
+        ws = engine.workspace(lib.qex3d_scf_workspace_bytes(ctypes.byref(p)), engine.device())
+        _lib.check3(lib.qex3d_scf(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), engine._ptr(ws), ws.numel(),
+                                  engine._stream()), "qex3d_scf")
+        self._pending = (self._status, ws)
+        return self
+
+    def _prepare(self, **kwargs):
+        """Integrals, grid, AO values and output tensors of this solve; the C structs that describe it."""
         mol = self.mol
         dev = engine.device()
         S, H, eri = self.integrals()
@@ -71,23 +81,20 @@ class _SCF:
             if net.n_grid != ng:
                 raise ValueError(f"network input width {net.n_grid} != grid size {ng}")
             p.nn_h1, p.nn_h2, p.nn_scale = net.h1, net.h2, net.scale
-        lib = _lib.load()
-        ws = engine.workspace(lib.qex3d_scf_workspace_bytes(ctypes.byref(p)), dev)
         e_hist = torch.empty(self.max_cycle + 1, dtype=torch.float64, device=dev)
         mo = torch.empty(n, dtype=torch.float64, device=dev)
         C = torch.empty((n, n), dtype=torch.float64, device=dev)
         D = torch.empty((n, n), dtype=torch.float64, device=dev)
         status = torch.zeros(8, dtype=torch.int32, device=dev)
         ptr = engine._ptr
-        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w),
-                                  ptr(net.packed()) if method == _lib.QEX3D_RKS_NN_GLOBAL else None)
+        packed = net.packed() if method == _lib.QEX3D_RKS_NN_GLOBAL else None
+        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w), ptr(packed))
         outs = _lib.Qex3dScfOutputs(ptr(e_hist), ptr(mo), ptr(C), ptr(D), ptr(status))
-        _lib.check3(lib.qex3d_scf(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(outs), ptr(ws), ws.numel(),
-                                  engine._stream()), "qex3d_scf")
-        self._pending = (status, ws)
+        self._keep = packed
+        self._status = status
         self.energies = e_hist
         self.mo_energy, self.mo_coeff, self.dm = mo, C, D
-        return self
+        return p, ins, outs
 
     def collect(self):
         status, _ = self._pending
@@ -127,3 +134,34 @@ class RKS(_SCF):
             raise NotImplementedError("the device path implements the amplitude-encoded (global) network")
         self.network = network
         return self
+
+
+def kernel_batch_async(mfs, **kwargs):
+    """Enqueues the SCF of several molecules (any sizes and methods) as one batched launch sequence
+    (the padded batch of ``scf_functions_masked.py:917-967``, without padding: each molecule keeps
+    its own basis and grid, so each result equals its solo ``kernel()`` bit for bit).  ``collect()``
+    on each object reads its result."""
+    mfs = list(mfs)
+    if not mfs:
+        return mfs
+    if len(mfs) > _lib.QEX3D_MAX_SCF_BATCH:
+        raise ValueError(f"at most {_lib.QEX3D_MAX_SCF_BATCH} molecules per batch")
+    preps = [mf._prepare(**kwargs) for mf in mfs]
+    B = len(mfs)
+    P = (_lib.Qex3dScfProblem * B)(*[x[0] for x in preps])
+    I = (_lib.Qex3dScfInputs * B)(*[x[1] for x in preps])
+    O = (_lib.Qex3dScfOutputs * B)(*[x[2] for x in preps])
+    lib = _lib.load()
+    nbytes = lib.qex3d_scf_batch_workspace_bytes(B, P)
+    if nbytes == 0:
+        raise ValueError("invalid problem in the batch")
+    ws = engine.workspace(nbytes, engine.device())
+    _lib.check3(lib.qex3d_scf_batch(B, P, I, O, engine._ptr(ws), ws.numel(), engine._stream()), "qex3d_scf_batch")
+    for mf in mfs:
+        mf._pending = (mf._status, ws)
+    return mfs
+
+
+def kernel_batch(mfs, **kwargs):
+    """Runs the batched SCF and returns the total energies (waits for the results)."""
+    return [mf.collect() for mf in kernel_batch_async(mfs, **kwargs)]

@@ -161,6 +161,32 @@ def test_scf3d_rejects_missing_grid_and_small_workspace():
     assert rc == -2 and "workspace" in lib.qex3d_last_error().decode()
 
 
+def test_scf3d_batch_workspace_and_validation():
+    lib = _lib.load()
+    P = (_lib.Qex3dScfProblem * 3)(_scf_problem(), _scf_problem(method=_lib.QEX3D_RKS_LDA, ngrid=5000),
+                                   _scf_problem(nao=40, nelectron=16))
+    total = lib.qex3d_scf_batch_workspace_bytes(3, P)
+    solo = sum(lib.qex3d_scf_workspace_bytes(ctypes.byref(P[i])) for i in range(3))
+    # one record block for the whole batch instead of one per molecule
+    assert 0 < total <= solo
+    assert lib.qex3d_scf_batch_workspace_bytes(0, P) == 0
+    assert lib.qex3d_scf_batch_workspace_bytes(_lib.QEX3D_MAX_SCF_BATCH + 1, P) == 0
+    P[1].ngrid = 0  # RKS without a grid
+    assert lib.qex3d_scf_batch_workspace_bytes(3, P) == 0
+    P[1].ngrid = 5000
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails before any launch
+    I = (_lib.Qex3dScfInputs * 3)(*[_lib.Qex3dScfInputs(fake, fake, fake, fake, fake) for _ in range(3)])
+    O = (_lib.Qex3dScfOutputs * 3)(*[_lib.Qex3dScfOutputs(fake, fake, fake, fake, fake) for _ in range(3)])
+    assert lib.qex3d_scf_batch(0, P, I, O, fake, 1 << 30, None) == -1
+    assert "batch" in lib.qex3d_last_error().decode()
+    P[2].nelectron = 7
+    assert lib.qex3d_scf_batch(3, P, I, O, fake, 1 << 30, None) == -1
+    assert "molecule 2: closed-shell" in lib.qex3d_last_error().decode()
+    P[2].nelectron = 16
+    assert lib.qex3d_scf_batch(3, P, I, O, fake, 64, None) == -2
+    assert "workspace" in lib.qex3d_last_error().decode()
+
+
 def test_3d_entry_points_reject_null_basis():
     lib = _lib.load()
     fake = ctypes.c_void_p(0x1000)

@@ -109,6 +109,9 @@ def parse_args():
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
     ap.add_argument("--requests", type=int, default=1024, help="c5: requests per arrival rate")
     ap.add_argument("--rates", type=str, default="100,500,2000", help="c5: Poisson arrival rates (req/s)")
+    ap.add_argument("--batch-3d", type=int, default=16,
+                    help="c4: 1 solves the chains one at a time, >1 solves them as one batched SCF; "
+                         "c5: most 3D requests per batched dispatch")
     return ap.parse_args()
 
 
@@ -300,15 +303,35 @@ def run_c4(args):
     for m in mols:
         m.device_basis()  # basis arrays resident in HBM before the timed region
 
-    def solve(m):
+    def make(m):
         mf = tscf.RKS(m)
         mf.conv_tol, mf.max_cycle, mf.diis_space = 1e-8, 30, 8
+        return mf
+
+    def solve(m):
+        mf = make(m)
         mf.kernel()
         return mf
 
+    batched = args.batch_3d > 1
+
+    def step():
+        """One step: the four chains from their basis; returns {chain: solved object}, latencies."""
+        if batched:
+            ts = time.perf_counter()
+            mfs = [make(m) for m in mols]
+            tscf.kernel_batch(mfs)  # returns after the device has finished (status read-back)
+            dt = (time.perf_counter() - ts) * 1e3
+            return dict(zip(C4_CHAINS, mfs)), {n: dt for n in C4_CHAINS}
+        out, dts = {}, {}
+        for n, m in zip(C4_CHAINS, mols):
+            ts = time.perf_counter()
+            out[n] = solve(m)  # returns after the device has finished (status read-back)
+            dts[n] = (time.perf_counter() - ts) * 1e3
+        return out, dts
+
     for _ in range(args.warmup):
-        for m in mols:
-            solve(m)
+        step()
     torch.cuda.synchronize()
     _lib.check(lib.qex_profile_enable(1, 256 * len(mols) * args.steps + 16), "qex_profile_enable")
     lat = {n: [] for n in C4_CHAINS}
@@ -316,10 +339,9 @@ def run_c4(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        for n, m in zip(C4_CHAINS, mols):
-            ts = time.perf_counter()
-            last[n] = solve(m)  # returns after the device has finished (status read-back)
-            lat[n].append((time.perf_counter() - ts) * 1e3)
+        last, dts = step()
+        for n in C4_CHAINS:
+            lat[n].append(dts[n])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     nk = len(_lib.KERNEL_KINDS)
@@ -380,7 +402,9 @@ def run_c4(args):
         "config": {"workload": "c4 3D RKS-LDA: H2/H4/H8/H16 chains, 5 AOs/atom, 50x110 grid/atom, CDIIS(8), "
                                "<=30 cycles, conv 1e-8; one step = the four chains solved from their basis",
                    "chains": list(C4_CHAINS), "nao": [m.nao for m in mols],
-                   "ngrid": [last[n].grids.size for n in C4_CHAINS], "path": "plain FP64, one solve at a time"},
+                   "ngrid": [last[n].grids.size for n in C4_CHAINS],
+                   "path": "FP64, the four chains as one batched SCF (qex3d_scf_batch)" if batched else
+                           "plain FP64, one solve at a time"},
         "latency_ms": {str(n): {"p50": float(np.percentile(lat[n], 50)), "p99": float(np.percentile(lat[n], 99))}
                        for n in C4_CHAINS} | {"all_p50": float(np.percentile(all_lat, 50)),
                                                "all_p99": float(np.percentile(all_lat, 99))},
@@ -577,7 +601,7 @@ def run_c5(args):
     # warm-up: a few of each kind through both paths
     warm = requests(np.zeros(args.requests))[:16]
     serve.run_one_at_a_time(warm, runner, solver_3d)
-    server = serve.Server(runner, solver_3d, max_batch=args.batch)
+    server = serve.Server(runner, solver_3d, max_batch=args.batch, batch_3d=args.batch_3d)
     server.run(warm)
     torch.cuda.synchronize()
 
@@ -629,7 +653,8 @@ def run_c5(args):
         "data": "synthetic (seeded 1D systems, LocalMLP weights and 3D chains; Poisson arrivals)",
         "config": {"workload": f"c5 served mixed stream: {args.requests} requests per rate, 70% c2 1D / 30% c4 chains, "
                                "batched 1D dispatch + concurrent 3D stream; value = best served throughput",
-                   "rates": rates, "max_batch_1d": args.batch, "path": "served FP64"},
+                   "rates": rates, "max_batch_1d": args.batch, "max_batch_3d": args.batch_3d,
+                   "path": "served FP64"},
         "served": per_rate,
         "plain_one_at_a_time": {"throughput_per_s": plain_rate, "wall_s": plain.wall},
         "served_over_plain": value / plain_rate,

@@ -149,7 +149,8 @@ __device__ __forceinline__ double ipow(double x, int n) {
   for (int k = 0; k < n; ++k) r *= x;
   return r;
 }
-__device__ double rtuv(int t, int u, int v, double a, double X, double Y, double Z, const Boys5& F) {
+template <class FT>  // F[N] = F_N(T): a Boys5 or a pointer
+__device__ __forceinline__ double rtuv(int t, int u, int v, double a, double X, double Y, double Z, const FT& F) {
   const double ax = 2.0 * a * X, ay = 2.0 * a * Y, az = 2.0 * a * Z;
   double r = 0.0;
   for (int i = 0; 2 * i <= t; ++i) {
@@ -295,109 +296,310 @@ __global__ __launch_bounds__(128) void k_int1e(Basis b, double* S, double* Tm, d
   V[i * n + j] = V[j * n + i] = v;
 }
 
-// ERIs from primitive-pair records.  Record (AO pair P = (i >= j), primitive pair r) holds
-//   p, P_x, P_y, P_z, c_a c_b, n_terms, then up to 4 Hermite terms (packed t | u << 4 | v << 8, E_tuv)
-// with E_tuv = E^x_t E^y_u E^z_v (the Gaussian overlap factor exp(-mu |A-B|^2) included).
-constexpr int PREC = 16;  // doubles per record
+// ---------------------------------------------------------------------------------------------
+// Two-electron integrals (ij|kl) as the pair matrix eri[P][Q], P = pair(i >= j), by
+// McMurchie-Davidson over s/p shells.  One thread per (bra AO pair P, ket shell pair SQ): each
+// primitive quartet computes F_0..F_L once, the Hermite integrals R_tuv (t+u+v <= L) once, the
+// bra contraction W[L2] = sum_L1 E^ij_L1 R_{L1+L2} once, and then every AO component pair (kl) of
+// SQ with a dot over its ket Hermite coefficients.  The (bra, ket) angular classes (0: ss, 1: sp
+// or ps, 2: pp) are template parameters, so every Hermite index is a compile-time constant.
+// Bra pairs are sorted by class and ket shell pairs by class, a wave has one ket shell pair, and a
+// wave's bra pairs share one class (except at the class boundaries).  Symmetry: only Q <= P is
+// computed; eri[P][Q] and eri[Q][P] are both written.
+// ---------------------------------------------------------------------------------------------
+constexpr int SREC = 16;  // doubles per (shell pair, primitive pair) record
 constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
 
-__global__ void k_eri_pairs(Basis b, int npair, int maxp, double* rec) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int pp = maxp * maxp;
-  if (idx >= (long)npair * pp) return;
-  const int P = (int)(idx / pp), r = (int)(idx % pp);
+// Hermite index k <-> (t, u, v), ordered by N = t+u+v, then by u+v, then by v
+__host__ __device__ constexpr int h_count(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+__host__ __device__ constexpr int h_index(int t, int u, int v) {
+  return (t + u + v) * (t + u + v + 1) * (t + u + v + 2) / 6 + (u + v) * (u + v + 1) / 2 + v;
+}
+__device__ constexpr int HT[35] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0,
+                                   4, 3, 3, 2, 2, 2, 1, 1, 1, 1, 0, 0, 0, 0, 0};
+__device__ constexpr int HU[35] = {0, 0, 1, 0, 0, 1, 0, 2, 1, 0, 0, 1, 0, 2, 1, 0, 3, 2, 1, 0,
+                                   0, 1, 0, 2, 1, 0, 3, 2, 1, 0, 4, 3, 2, 1, 0};
+__device__ constexpr int HV[35] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3,
+                                  0, 0, 1, 0, 1, 2, 0, 1, 2, 3, 0, 1, 2, 3, 4};
+
+// F_0..F_L(T): Taylor expansion of F_L about the nearest tabulated point, downward recursion below
+// (F_0 alone needs no exponential); above the table, F_0 = sqrt(pi/T)/2 and upward recursion.
+template <int L>
+__device__ __forceinline__ void boys_upto(double T, double* F) {
+  if (T < BOYS_TMAX) {
+    const int i = (int)(T * (1.0 / BOYS_H) + 0.5);
+    const double dT = T - BOYS_H * i;
+    const double* r = g_boys_tab + i * BOYS_NM + L;
+    double f = r[6];
+    f = fma(f, -dT / 6.0, r[5]);
+    f = fma(f, -dT / 5.0, r[4]);
+    f = fma(f, -dT / 4.0, r[3]);
+    f = fma(f, -dT / 3.0, r[2]);
+    f = fma(f, -dT / 2.0, r[1]);
+    f = fma(f, -dT, r[0]);
+    F[L] = f;
+    if (L > 0) {
+      const double et = exp(-T);
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + et) / (2 * m + 1);
+    }
+  } else {
+    F[0] = 0.5 * sqrt(M_PI / T);
+    if (L > 0) {
+      const double et = exp(-T), i2t = 0.5 / T;
+#pragma unroll
+      for (int m = 1; m <= L; ++m) F[m] = ((2 * m - 1) * F[m - 1] - et) * i2t;
+    }
+  }
+}
+
+// R_tuv for all t+u+v <= L (closed form of the Hermite Coulomb integrals, constant indices)
+template <int L>
+__device__ __forceinline__ void rtable(double a, double X, double Y, double Z, const double* F, double* R) {
+#pragma unroll
+  for (int k = 0; k < h_count(L); ++k) R[k] = rtuv(HT[k], HU[k], HV[k], a, X, Y, Z, F);
+}
+
+// E^{ab}_{tuv} as a dense vector over t+u+v <= la+lb (per-direction E from hermite_E1)
+template <int LAB>
+__device__ __forceinline__ void e_dense(const int* la, const int* lb, const double* PA, const double* PB, double i2p,
+                                        double* E) {
+  double Ex[3], Ey[3], Ez[3];
+  hermite_E1(la[0], lb[0], 1.0, PA[0], PB[0], i2p, Ex);
+  hermite_E1(la[1], lb[1], 1.0, PA[1], PB[1], i2p, Ey);
+  hermite_E1(la[2], lb[2], 1.0, PA[2], PB[2], i2p, Ez);
+#pragma unroll
+  for (int k = 0; k < h_count(LAB); ++k) E[k] = (HT[k] <= 2 ? Ex[HT[k]] : 0.0) * (HU[k] <= 2 ? Ey[HU[k]] : 0.0) *
+                                              (HV[k] <= 2 ? Ez[HV[k]] : 0.0);
+}
+
+struct EriArgs {
+  int nao, npair, nsp, maxp;
+  const int* sh_l;
+  const int* ao_shell;
+  const int* ao_lmn;
+  const int* ao_first;  // [nshell] first AO of each shell
+  const double* rec;    // [nsp][maxp^2][SREC]
+  const int* bra;       // [npair] AO pairs sorted by class
+  const int* ket;       // [nsp] shell pairs sorted by class
+  const int* cnt;       // [0..2] bra class sizes, [3..5] ket class sizes
+  double* eri;
+};
+
+__device__ __forceinline__ void lmn3(int packed, int* l) {
+  l[0] = packed & 3;
+  l[1] = (packed >> 2) & 3;
+  l[2] = (packed >> 4) & 3;
+}
+
+template <int CB, int CK>
+__device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, int sd) {
+  constexpr int NB = h_count(CB), NK = h_count(CK), NR = h_count(CB + CK);
+  constexpr int NCC = CK == 0 ? 1 : 3;  // components: (lc, ld) = (0,0): 1x1, (1,0)/(0,1): 3x1 or 1x3, (1,1): 3x3
+  constexpr int NCD = CK == 2 ? 3 : 1;
+  int la[3], lb[3];
+  lmn3(g.ao_lmn[i], la);
+  lmn3(g.ao_lmn[j], lb);
+  const int sa = g.ao_shell[i], sb = g.ao_shell[j];
+  const int pp = g.maxp * g.maxp;
+  const double* r1 = g.rec + (size_t)(sa * (sa + 1) / 2 + sb) * pp * SREC;
+  const double* r2 = g.rec + (size_t)SQ * pp * SREC;
+  const int lc = g.sh_l[sc], ld = g.sh_l[sd];
+  const int c0 = g.ao_first[sc], d0 = g.ao_first[sd];
+  double acc[NCC * NCD];
+#pragma unroll
+  for (int q = 0; q < NCC * NCD; ++q) acc[q] = 0.0;
+  for (int a = 0; a < pp; ++a) {
+    const double* x1 = r1 + a * SREC;
+    const double c1 = x1[4];
+    if (c1 == 0.0) continue;
+    const double p = x1[0];
+    double E1[NB];
+    e_dense<CB>(la, lb, x1 + 5, x1 + 8, x1[11], E1);
+    for (int c = 0; c < pp; ++c) {
+      const double* x2 = r2 + c * SREC;
+      const double c2 = x2[4];
+      if (c2 == 0.0) continue;
+      const double q = x2[0];
+      const double alpha = p * q / (p + q);
+      const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
+      double F[CB + CK + 1];
+      boys_upto<CB + CK>(alpha * (X * X + Y * Y + Z * Z), F);
+      double R[NR];
+      rtable<CB + CK>(alpha, X, Y, Z, F, R);
+      // W[L2] = sum_L1 E1[L1] R[L1 + L2], with the ket sign (-1)^|L2|
+      double W[NK];
+#pragma unroll
+      for (int k2 = 0; k2 < NK; ++k2) {
+        double w = 0.0;
+#pragma unroll
+        for (int k1 = 0; k1 < NB; ++k1)
+          w = fma(E1[k1], R[h_index(HT[k1] + HT[k2], HU[k1] + HU[k2], HV[k1] + HV[k2])], w);
+        W[k2] = ((HT[k2] + HU[k2] + HV[k2]) & 1) ? -w : w;
+      }
+      const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
+#pragma unroll
+      for (int cc = 0; cc < NCC; ++cc)
+#pragma unroll
+        for (int cd = 0; cd < NCD; ++cd) {
+          // component directions: a p function on the ket side points along its component index
+          int lk[3] = {0, 0, 0}, lm[3] = {0, 0, 0};
+          if (CK == 2) {
+            lk[cc] = 1;
+            lm[cd] = 1;
+          } else if (CK == 1) {
+            if (lc == 1) lk[cc] = 1;
+            else lm[cc] = 1;
+          }
+          double E2[NK];
+          e_dense<CK>(lk, lm, x2 + 5, x2 + 8, x2[11], E2);
+          double v = 0.0;
+#pragma unroll
+          for (int k2 = 0; k2 < NK; ++k2) v = fma(E2[k2], W[k2], v);
+          acc[cc * NCD + cd] = fma(pref, v, acc[cc * NCD + cd]);
+        }
+    }
+  }
+#pragma unroll
+  for (int cc = 0; cc < NCC; ++cc)
+#pragma unroll
+    for (int cd = 0; cd < NCD; ++cd) {
+      int k, l;
+      if (CK == 2) {
+        k = c0 + cc;
+        l = d0 + cd;
+      } else if (CK == 1) {
+        k = lc == 1 ? c0 + cc : c0;
+        l = lc == 1 ? d0 : d0 + cc;
+      } else {
+        k = c0;
+        l = d0;
+      }
+      if (k < l) continue;  // same p shell: only k >= l
+      const int Q = k * (k + 1) / 2 + l;
+      if (Q > P) continue;
+      const double val = acc[cc * NCD + cd];
+      g.eri[(size_t)P * g.npair + Q] = val;
+      g.eri[(size_t)Q * g.npair + P] = val;
+    }
+}
+
+// one wave per workgroup: blockIdx.y = ket shell pair (sorted), lanes = consecutive sorted bra pairs
+__global__ __launch_bounds__(64) void k_eri(EriArgs g) {
+  const int ky = blockIdx.y;
+  const int kc0 = g.cnt[3], kc1 = g.cnt[4];
+  const int ck = ky < kc0 ? 0 : ky < kc0 + kc1 ? 1 : 2;
+  const int SQ = g.ket[ky];
+  int sc = (int)((sqrt(8.0 * SQ + 1.0) - 1.0) * 0.5);
+  while (sc * (sc + 1) / 2 > SQ) --sc;
+  while ((sc + 1) * (sc + 2) / 2 <= SQ) ++sc;
+  const int sd = SQ - sc * (sc + 1) / 2;
+  const int qmin = g.ao_first[sc] * (g.ao_first[sc] + 1) / 2 + g.ao_first[sd];
+  const int bx = blockIdx.x * 64 + threadIdx.x;
+  if (bx >= g.npair) return;
+  const int P = g.bra[bx];
+  if (P < qmin) return;
+  const int bc0 = g.cnt[0], bc1 = g.cnt[1];
+  const int cb = bx < bc0 ? 0 : bx < bc0 + bc1 ? 1 : 2;
   int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
   while (i * (i + 1) / 2 > P) --i;
   while ((i + 1) * (i + 2) / 2 <= P) ++i;
   const int j = P - i * (i + 1) / 2;
-  int si, sj, la[3], lb[3];
-  ao_info(b, i, &si, &la[0], &la[1], &la[2]);
-  ao_info(b, j, &sj, &lb[0], &lb[1], &lb[2]);
-  double* o = rec + idx * PREC;
-  const int pa = r / maxp, pb = r % maxp;
-  if (pa >= b.sh_nprim[si] || pb >= b.sh_nprim[sj]) {
-    o[5] = 0.0;  // no terms: record unused
-    return;
+  switch (cb * 3 + ck) {
+    case 0: eri_body<0, 0>(g, P, i, j, SQ, sc, sd); break;
+    case 1: eri_body<0, 1>(g, P, i, j, SQ, sc, sd); break;
+    case 2: eri_body<0, 2>(g, P, i, j, SQ, sc, sd); break;
+    case 3: eri_body<1, 0>(g, P, i, j, SQ, sc, sd); break;
+    case 4: eri_body<1, 1>(g, P, i, j, SQ, sc, sd); break;
+    case 5: eri_body<1, 2>(g, P, i, j, SQ, sc, sd); break;
+    case 6: eri_body<2, 0>(g, P, i, j, SQ, sc, sd); break;
+    case 7: eri_body<2, 1>(g, P, i, j, SQ, sc, sd); break;
+    default: eri_body<2, 2>(g, P, i, j, SQ, sc, sd); break;
   }
-  const double* A = b.xyz + 3 * b.sh_atom[si];
-  const double* B = b.xyz + 3 * b.sh_atom[sj];
-  const double ea = b.exps[b.sh_off[si] + pa], eb = b.exps[b.sh_off[sj] + pb];
-  const double p = ea + eb, mu = ea * eb / p, i2p = 0.5 / p;
-  const double cab = b.coefs[b.sh_off[si] + pa] * b.coefs[b.sh_off[sj] + pb];
-  const double r2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
-  // primitive pairs whose Gaussian product prefactor is below 1e-20 contribute < 1e-18 to any integral
-  if (fabs(cab) * exp(-mu * r2) * (M_PI / p) * sqrt(M_PI / p) < 1e-20) {
-    o[5] = 0.0;
-    return;
-  }
-  double Pp[3], E[3][3];
-  for (int d = 0; d < 3; ++d) {
-    Pp[d] = (ea * A[d] + eb * B[d]) / p;
-    hermite_E1(la[d], lb[d], exp(-mu * (A[d] - B[d]) * (A[d] - B[d])), Pp[d] - A[d], Pp[d] - B[d], i2p, E[d]);
-  }
-  o[0] = p;
-  o[1] = Pp[0];
-  o[2] = Pp[1];
-  o[3] = Pp[2];
-  o[4] = cab;
-  int nt = 0;
-  for (int t = 0; t <= la[0] + lb[0]; ++t)
-    for (int u = 0; u <= la[1] + lb[1]; ++u)
-      for (int v = 0; v <= la[2] + lb[2]; ++v) {
-        const double e = E[0][t] * E[1][u] * E[2][v];
-        if (e != 0.0 && nt < 4) {
-          o[6 + 2 * nt] = (double)(t | (u << 4) | (v << 8));
-          o[7 + 2 * nt] = e;
-          ++nt;
-        }
-      }
-  o[5] = (double)nt;
 }
 
-// (ij|kl) for pair indices P >= Q; written to both [P][Q] and [Q][P]
-__global__ __launch_bounds__(128) void k_eri(int npair, int maxp, const double* rec, double* eri) {
+// Shell-pair primitive records: p, P (3), c_a c_b exp(-mu |AB|^2), P - A (3), P - B (3), 1/(2p).
+// Primitive pairs whose Gaussian product prefactor is below 1e-20 get c = 0 (they contribute
+// < 1e-18 to any integral) and are skipped.
+__global__ void k_eri_records(Basis b, int nsp, int maxp, double* rec) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)npair * (npair + 1) / 2;
-  if (idx >= total) return;
-  long P = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-  while (P * (P + 1) / 2 > idx) --P;
-  while ((P + 1) * (P + 2) / 2 <= idx) ++P;
-  const long Q = idx - P * (P + 1) / 2;
   const int pp = maxp * maxp;
-  const double* r1 = rec + P * pp * PREC;
-  const double* r2 = rec + Q * pp * PREC;
-  double val = 0.0;
-  for (int a = 0; a < pp; ++a) {
-    const double* x1 = r1 + a * PREC;
-    const int n1 = (int)x1[5];
-    if (n1 == 0) continue;
-    const double p = x1[0], Px = x1[1], Py = x1[2], Pz = x1[3], c1 = x1[4];
-    for (int c = 0; c < pp; ++c) {
-      const double* x2 = r2 + c * PREC;
-      const int n2 = (int)x2[5];
-      if (n2 == 0) continue;
-      const double q = x2[0];
-      const double alpha = p * q / (p + q);
-      const double X = Px - x2[1], Y = Py - x2[2], Z = Pz - x2[3];
-      const Boys5 F = boys5(alpha * (X * X + Y * Y + Z * Z));
-      double acc = 0.0;
-      for (int s1 = 0; s1 < n1; ++s1) {
-        const int k1 = (int)x1[6 + 2 * s1];
-        const double e1 = x1[7 + 2 * s1];
-        const int t1 = k1 & 15, u1 = (k1 >> 4) & 15, v1 = k1 >> 8;
-        for (int s2 = 0; s2 < n2; ++s2) {
-          const int k2 = (int)x2[6 + 2 * s2];
-          const int t2 = k2 & 15, u2 = (k2 >> 4) & 15, v2 = k2 >> 8;
-          const double e = e1 * x2[7 + 2 * s2];
-          const double rv = rtuv(t1 + t2, u1 + u2, v1 + v2, alpha, X, Y, Z, F);
-          acc += ((t2 + u2 + v2) & 1) ? -e * rv : e * rv;
+  if (idx >= (long)nsp * pp) return;
+  const int SP = (int)(idx / pp), r = (int)(idx % pp);
+  int sa = (int)((sqrt(8.0 * SP + 1.0) - 1.0) * 0.5);
+  while (sa * (sa + 1) / 2 > SP) --sa;
+  while ((sa + 1) * (sa + 2) / 2 <= SP) ++sa;
+  const int sb = SP - sa * (sa + 1) / 2;
+  double* o = rec + idx * SREC;
+  const int pa = r / maxp, pb = r % maxp;
+  if (pa >= b.sh_nprim[sa] || pb >= b.sh_nprim[sb]) {
+    o[4] = 0.0;
+    return;
+  }
+  const double* A = b.xyz + 3 * b.sh_atom[sa];
+  const double* B = b.xyz + 3 * b.sh_atom[sb];
+  const double ea = b.exps[b.sh_off[sa] + pa], eb = b.exps[b.sh_off[sb] + pb];
+  const double p = ea + eb, mu = ea * eb / p;
+  const double cab = b.coefs[b.sh_off[sa] + pa] * b.coefs[b.sh_off[sb] + pb];
+  const double r2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+  const double K = exp(-mu * r2);
+  o[0] = p;
+  for (int d = 0; d < 3; ++d) {
+    const double Pd = (ea * A[d] + eb * B[d]) / p;
+    o[1 + d] = Pd;
+    o[5 + d] = Pd - A[d];
+    o[8 + d] = Pd - B[d];
+  }
+  o[11] = 0.5 / p;
+  o[4] = fabs(cab) * K * (M_PI / p) * sqrt(M_PI / p) < 1e-20 ? 0.0 : cab * K;
+}
+
+// Sorting by class (one workgroup of 1024): bra AO pairs by l(i) + l(j), ket shell pairs by
+// l(a) + l(b); stable (ascending index within a class); ao_first per shell.
+__global__ __launch_bounds__(1024) void k_eri_setup(Basis b, int npair, int nsp, int* ao_first, int* bra, int* ket,
+                                                    int* cnt) {
+  __shared__ int wtot[16];
+  __shared__ int base;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int s = tid; s < b.nshell; s += 1024) {
+    int f = 0;
+    for (int t = 0; t < s; ++t) f += b.sh_l[t] == 0 ? 1 : 3;
+    ao_first[s] = f;
+  }
+  for (int list = 0; list < 2; ++list) {
+    const int n = list == 0 ? npair : nsp;
+    int* out = list == 0 ? bra : ket;
+    int off = 0;
+    for (int c = 0; c < 3; ++c) {
+      if (tid == 0) base = 0;
+      __syncthreads();
+      for (int c0 = 0; c0 < n; c0 += 1024) {
+        const int idx = c0 + tid;
+        bool flag = false;
+        if (idx < n) {
+          int x = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+          while (x * (x + 1) / 2 > idx) --x;
+          while ((x + 1) * (x + 2) / 2 <= idx) ++x;
+          const int y = idx - x * (x + 1) / 2;
+          const int cls = list == 0 ? (b.ao_lmn[x] != 0) + (b.ao_lmn[y] != 0) : b.sh_l[x] + b.sh_l[y];
+          flag = cls == c;
         }
+        const unsigned long long m = __ballot(flag);
+        if (lane == 0) wtot[wv] = __popcll(m);
+        __syncthreads();
+        int wb = 0;
+        for (int w = 0; w < wv; ++w) wb += wtot[w];
+        if (flag) out[off + base + wb + __popcll(m & ((1ull << lane) - 1))] = idx;
+        __syncthreads();
+        if (tid == 0)
+          for (int w = 0; w < 16; ++w) base += wtot[w];
+        __syncthreads();
       }
-      val += c1 * x2[4] * TWO_PI_2_5 / (p * q * sqrt(p + q)) * acc;
+      if (tid == 0) cnt[list * 3 + c] = base;
+      off += base;
+      __syncthreads();
     }
   }
-  eri[P * npair + Q] = val;
-  eri[Q * npair + P] = val;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1472,10 +1674,13 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
   return launch_status("qex3d_int1e");
 }
 
+// workspace: shell-pair records, then ao_first [nshell], bra [npair], ket [nsp], class sizes [8]
 size_t qex3d_eri_workspace_bytes(const qex3d_basis* b) {
   if (check_basis(b) != QEX_OK) return 0;
   const long np = (long)b->nao * (b->nao + 1) / 2;
-  return al((size_t)np * b->max_nprim * b->max_nprim * PREC * 8);
+  const long nsp = (long)b->nshell * (b->nshell + 1) / 2;
+  return al((size_t)nsp * b->max_nprim * b->max_nprim * SREC * 8) + al((size_t)b->nshell * 4) + al((size_t)np * 4) +
+         al((size_t)nsp * 4) + al(64);
 }
 
 int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream) {
@@ -1485,15 +1690,40 @@ int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspa
   const size_t need = qex3d_eri_workspace_bytes(b);
   if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
   const int np = b->nao * (b->nao + 1) / 2;
-  const long total = (long)np * (np + 1) / 2;
-  const long nrec = (long)np * b->max_nprim * b->max_nprim;
+  const int nsp = b->nshell * (b->nshell + 1) / 2;
+  const int maxp = b->max_nprim;
   hipStream_t st = (hipStream_t)stream;
   ensure_boys(st);
+  char* w = (char*)workspace;
+  double* rec = (double*)w;
+  w += al((size_t)nsp * maxp * maxp * SREC * 8);
+  int* ao_first = (int*)w;
+  w += al((size_t)b->nshell * 4);
+  int* bra = (int*)w;
+  w += al((size_t)np * 4);
+  int* ket = (int*)w;
+  w += al((size_t)nsp * 4);
+  int* cnt = (int*)w;
+  const Basis d = make_basis(b);
   qexprof::Scope ps(qexprof::K_ERI, st);
-  hipLaunchKernelGGL(k_eri_pairs, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, st, make_basis(b), np,
-                     b->max_nprim, (double*)workspace);
-  hipLaunchKernelGGL(k_eri, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, st, np, b->max_nprim,
-                     (const double*)workspace, eri);
+  const long nrec = (long)nsp * maxp * maxp;
+  hipLaunchKernelGGL(k_eri_records, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, st, d, nsp, maxp, rec);
+  hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt);
+  EriArgs g{};
+  g.nao = b->nao;
+  g.npair = np;
+  g.nsp = nsp;
+  g.maxp = maxp;
+  g.sh_l = b->shell_l;
+  g.ao_shell = b->ao_shell;
+  g.ao_lmn = b->ao_lmn;
+  g.ao_first = ao_first;
+  g.rec = rec;
+  g.bra = bra;
+  g.ket = ket;
+  g.cnt = cnt;
+  g.eri = eri;
+  hipLaunchKernelGGL(k_eri, dim3((unsigned)((np + 63) / 64), (unsigned)nsp), dim3(64), 0, st, g);
   return launch_status("qex3d_eri");
 }
 

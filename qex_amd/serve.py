@@ -8,9 +8,11 @@ The dispatcher runs in one host thread and never blocks on the device:
   * 1D: every released request still waiting is packed into the next batched launch (up to
     ``max_batch``), one ``qex1d_kohn_sham`` per batch, on stream A, while the previous batch may
     still be running (at most ``depth_1d`` batches in flight);
-  * 3D: each request is one enqueued solve (integrals, grid, AO values, SCF cycles without host
-    synchronisation) on one of ``streams_3d`` streams (the least loaded), with at most ``depth_3d``
-    in flight per stream, so that the one-workgroup SCF kernels of several molecules overlap;
+  * 3D: the 3D requests waiting (up to ``batch_3d``) are enqueued together on one of
+    ``streams_3d`` streams (the least loaded): integrals, grid and AO values per molecule, then
+    one batched SCF (``qex3d_scf_batch``: every cycle stage is one launch over the molecules, so the
+    one-workgroup Fock/eigensolver/energy kernels of the batch run side by side), without host
+    synchronisation; at most ``depth_3d`` batches in flight per stream;
   * completions are detected with events (``query``), then results are read back.
 Admission holds a request back while the device memory its solve needs, by the estimate
 n_grid * n_ao * 8 + n_ao^4 / 4 * 8 + 16 * n_ao^2 * 8 bytes (3D) or T * N * 8 * 6 bytes (1D),
@@ -65,6 +67,7 @@ class ServeStats:
     wall: float = 0.0
     batches_1d: int = 0
     batch_sizes: list = field(default_factory=list)
+    batch_sizes_3d: list = field(default_factory=list)
 
     def summary(self):
         lat = np.array([c.latency for c in self.completions]) * 1e3
@@ -73,7 +76,9 @@ class ServeStats:
                "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
                "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
                "batches_1d": self.batches_1d,
-               "mean_batch_1d": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0}
+               "mean_batch_1d": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0,
+               "batches_3d": len(self.batch_sizes_3d),
+               "mean_batch_3d": float(np.mean(self.batch_sizes_3d)) if self.batch_sizes_3d else 0.0}
         for kind in ("1d", "3d"):
             lk = np.array([c.latency for c in self.completions if c.kind == kind]) * 1e3
             if len(lk):
@@ -89,14 +94,19 @@ def _bytes_3d(mol, n_grid):
 
 class Server:
     """runner_1d: an ``engine.BatchRunner``; solver_3d(mol) -> SCF object with ``kernel_async`` /
-    ``collect`` (e.g. ``lambda m: qex_amd.td.scf.RKS(m)`` with its settings)."""
+    ``collect`` (e.g. ``lambda m: qex_amd.td.scf.RKS(m)`` with its settings); batch_async_3d(list of
+    such objects) enqueues them as one batch (default ``qex_amd.td.scf.kernel_batch_async``)."""
 
     def __init__(self, runner_1d, solver_3d, max_batch=64, depth_1d=2, depth_3d=2, streams_3d=3,
-                 memory_budget=64 << 30, grid_points_per_atom=5500):
+                 memory_budget=64 << 30, grid_points_per_atom=5500, batch_3d=16, batch_async_3d=None):
         self.runner = runner_1d
         self.solver_3d = solver_3d
         self.max_batch = min(max_batch, runner_1d.max_batch) if runner_1d is not None else max_batch
         self.depth_1d, self.depth_3d = depth_1d, depth_3d
+        self.batch_3d = max(1, int(batch_3d))
+        if batch_async_3d is None:
+            from .td.scf import kernel_batch_async as batch_async_3d
+        self.batch_async_3d = batch_async_3d
         self.memory_budget = memory_budget
         self.grid_points_per_atom = grid_points_per_atom
         self.stream_1d = torch.cuda.Stream()
@@ -117,13 +127,16 @@ class Server:
             ev.record(self.stream_1d)
         return out, ev
 
-    def _dispatch_3d(self, req, stream):
+    def _dispatch_3d(self, reqs, stream):
         with torch.cuda.stream(stream):
-            mf = self.solver_3d(req.payload)
-            mf.kernel_async()
+            mfs = [self.solver_3d(r.payload) for r in reqs]
+            if len(mfs) == 1:
+                mfs[0].kernel_async()
+            else:
+                self.batch_async_3d(mfs)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return mf, ev
+        return mfs, ev
 
     def run(self, requests, poll_sleep=50e-6) -> ServeStats:
         reqs = sorted(requests, key=lambda r: r.arrival)
@@ -154,9 +167,9 @@ class Server:
                             stats.completions.append(Completion(r.rid, "1d", r.arrival, done, float(e[k]),
                                                                 bool(conv[k])))
                     else:
-                        e = f.handle.collect()
-                        r = f.reqs[0]
-                        stats.completions.append(Completion(r.rid, "3d", r.arrival, done, e, f.handle.converged))
+                        for r, mf in zip(f.reqs, f.handle):
+                            e = mf.collect()
+                            stats.completions.append(Completion(r.rid, "3d", r.arrival, done, e, mf.converged))
                     progressed = True
             # dispatch
             if pend_1d and len(inflight_1d) < self.depth_1d:
@@ -176,14 +189,20 @@ class Server:
                 progressed = True
             k3 = min(range(len(inflight_3d)), key=lambda q: len(inflight_3d[q]))
             if pend_3d and len(inflight_3d[k3]) < self.depth_3d:
-                r = pend_3d[0]
-                need = _bytes_3d(r.payload, r.payload.natm * self.grid_points_per_atom)
                 busy = inflight_1d or any(inflight_3d)
-                if used + need <= self.memory_budget or not busy:
-                    pend_3d.popleft()
-                    mf, ev = self._dispatch_3d(r, self.streams_3d[k3])
-                    used += need
-                    inflight_3d[k3].append(_Inflight(ev, [r], mf, need))
+                batch, nb = [], 0
+                while pend_3d and len(batch) < self.batch_3d:
+                    r = pend_3d[0]
+                    need = _bytes_3d(r.payload, r.payload.natm * self.grid_points_per_atom)
+                    if used + nb + need > self.memory_budget and (batch or busy):
+                        break
+                    batch.append(pend_3d.popleft())
+                    nb += need
+                if batch:
+                    mfs, ev = self._dispatch_3d(batch, self.streams_3d[k3])
+                    used += nb
+                    inflight_3d[k3].append(_Inflight(ev, batch, mfs, nb))
+                    stats.batch_sizes_3d.append(len(batch))
                     progressed = True
             if not progressed:
                 if i < total and not (pend_1d or pend_3d or inflight_1d or any(inflight_3d)):

@@ -36,6 +36,20 @@ def angular_product(n_theta=10, n_phi=11):
     return dirs, w
 
 
+_TABLES = {}
+
+
+def _device_tables(n_rad, n_theta, n_phi, dev):
+    """Radial and angular tables on the device, made once per shape and device (a host-to-device
+    copy from pageable memory waits for the stream, so it must not sit on the per-molecule path)."""
+    key = (n_rad, n_theta, n_phi, str(dev))
+    if key not in _TABLES:
+        r, wr = radial_treutler(n_rad)
+        dirs, wa = angular_product(n_theta, n_phi)
+        _TABLES[key] = (len(r), len(wa), [engine.as_f64(x, dev) for x in (r, wr, dirs.reshape(-1), wa)])
+    return _TABLES[key]
+
+
 class Grids:
     def __init__(self, mol, n_rad=50, n_theta=10, n_phi=11):
         self.mol = mol
@@ -45,15 +59,13 @@ class Grids:
 
     def build(self):
         dev = engine.device()
-        r, wr = radial_treutler(self.n_rad)
-        dirs, wa = angular_product(self.n_theta, self.n_phi)
-        tabs = [engine.as_f64(x, dev) for x in (r, wr, dirs.reshape(-1), wa)]
-        ng = self.mol.natm * len(r) * len(wa)
+        nr, na, tabs = _device_tables(self.n_rad, self.n_theta, self.n_phi, dev)
+        ng = self.mol.natm * nr * na
         self.coords = torch.empty((ng, 3), dtype=torch.float64, device=dev)
         self.weights = torch.empty(ng, dtype=torch.float64, device=dev)
         p = lambda t: ctypes.c_void_p(t.data_ptr())
-        _lib.check3(_lib.load().qex3d_grid(ctypes.byref(self.mol.device_basis()), len(r), p(tabs[0]), p(tabs[1]),
-                                           len(wa), p(tabs[2]), p(tabs[3]), p(self.coords), p(self.weights),
+        _lib.check3(_lib.load().qex3d_grid(ctypes.byref(self.mol.device_basis()), nr, p(tabs[0]), p(tabs[1]),
+                                           na, p(tabs[2]), p(tabs[3]), p(self.coords), p(self.weights),
                                            engine._stream()), "qex3d_grid")
         self._keep = tabs
         return self

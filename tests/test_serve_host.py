@@ -59,6 +59,16 @@ class _SCF:
         return self.collect()
 
 
+def _batch_async(mfs):
+    _batch_async.sizes.append(len(mfs))
+    for mf in mfs:
+        mf.kernel_async()
+    return mfs
+
+
+_batch_async.sizes = []
+
+
 class _Mol:
     def __init__(self, natm):
         self.natm, self.nao = natm, 5 * natm
@@ -79,7 +89,8 @@ def _requests(n, seed, rate):
 
 def test_every_request_completes_once_with_its_own_result(fake_cuda):
     runner = _Runner()
-    srv = serve.Server(runner, _SCF, max_batch=8, streams_3d=2)
+    _batch_async.sizes.clear()
+    srv = serve.Server(runner, _SCF, max_batch=8, streams_3d=2, batch_3d=4, batch_async_3d=_batch_async)
     reqs = _requests(200, 1, rate=5000.0)
     st = srv.run(reqs)
     assert sorted(c.rid for c in st.completions) == list(range(200))
@@ -93,15 +104,27 @@ def test_every_request_completes_once_with_its_own_result(fake_cuda):
     assert sum(runner.batches) == sum(1 for r in reqs if r.kind == "1d")
     s = st.summary()
     assert s["requests"] == 200 and s["p99_ms"] >= s["p50_ms"]
+    # 3D requests are grouped (up to batch_3d) and each is solved exactly once
+    assert max(st.batch_sizes_3d) <= 4 and sum(st.batch_sizes_3d) == sum(1 for r in reqs if r.kind == "3d")
+    assert all(1 < b <= 4 for b in _batch_async.sizes)
 
 
 def test_fast_arrivals_are_batched_and_slow_ones_are_not_delayed(fake_cuda):
     runner = _Runner()
-    srv = serve.Server(runner, _SCF, max_batch=8)
+    srv = serve.Server(runner, _SCF, max_batch=8, batch_async_3d=_batch_async)
     burst = [serve.Request("1d", 0.0, (np.zeros(1), np.ones(1), 1, np.zeros(5)), rid=i) for i in range(32)]
     st = srv.run(burst)
     assert len(st.completions) == 32
     assert max(runner.batches) == 8  # a burst fills whole batches
+
+
+def test_3d_burst_is_one_batch(fake_cuda):
+    _batch_async.sizes.clear()
+    srv = serve.Server(_Runner(), _SCF, batch_3d=16, batch_async_3d=_batch_async)
+    burst = [serve.Request("3d", 0.0, _Mol(2 + i % 3), rid=i) for i in range(40)]
+    st = srv.run(burst)
+    assert sorted(c.rid for c in st.completions) == list(range(40))
+    assert st.batch_sizes_3d[0] == 16 and sum(st.batch_sizes_3d) == 40
 
 
 def test_one_at_a_time_path(fake_cuda):

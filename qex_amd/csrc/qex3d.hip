@@ -668,15 +668,50 @@ __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// AO tile of 64 points x n (contiguous rows of the [ng][n] matrix) from HBM: every thread issues its
+// loads at once (16-byte, up to TILE_V per thread), the caller does other work, then tile_store
+// writes them to the LDS tile (row stride S, columns n..npad-1 and rows past ng zero).
+template <int NTH>
+struct TileRegs {
+  static constexpr int V = (64 * QEX3D_MAX_NAO / 2 + NTH - 1) / NTH;
+  double2 v[V];
+};
+template <int NTH>
+__device__ __forceinline__ void tile_load(const double* ao, long g0, long ng, int n, TileRegs<NTH>& r) {
+  const long rows = ng - g0 < 64 ? ng - g0 : 64;
+  const long total = rows * n;  // valid doubles of the tile
+  const double* src = ao + g0 * n;  // g0 is a multiple of 64: 16-byte aligned
+#pragma unroll
+  for (int k = 0; k < TileRegs<NTH>::V; ++k) {
+    const long e = 2 * ((long)threadIdx.x + (long)k * NTH);
+    if (e + 1 < total)
+      r.v[k] = *reinterpret_cast<const double2*>(src + e);
+    else
+      r.v[k] = make_double2(e < total ? src[e] : 0.0, 0.0);
+  }
+}
+template <int NTH>
+__device__ __forceinline__ void tile_store(const TileRegs<NTH>& r, int n, int npad, int S, double* ph) {
+  const int tot = 64 * n;
+#pragma unroll
+  for (int k = 0; k < TileRegs<NTH>::V; ++k) {
+    const int e = 2 * (threadIdx.x + k * NTH);
+    if (e < tot) ph[(e / n) * S + e % n] = r.v[k].x;
+    if (e + 1 < tot) ph[((e + 1) / n) * S + (e + 1) % n] = r.v[k].y;
+  }
+  for (int t = threadIdx.x; t < 64 * (npad - n); t += NTH) ph[(t / (npad - n)) * S + n + t % (npad - n)] = 0.0;
+}
+
 __device__ __forceinline__ void rho_body(long bid, long ng, int n, const double* ao, const double* dm, double* rho) {
   extern __shared__ double sm[];
   const int npad = (n + 15) & ~15, S = npad + 1;
   double* ph = sm;
   __shared__ double part[NW][64];
   const long g0 = bid * 64;
-  for (int t = threadIdx.x; t < 64 * npad; t += NT) {
-    const int p = t / npad, c = t % npad;
-    ph[p * S + c] = (g0 + p < ng && c < n) ? ao[(g0 + p) * n + c] : 0.0;
+  {
+    TileRegs<NT> r;
+    tile_load(ao, g0, ng, n, r);
+    tile_store(r, n, npad, S, ph);
   }
   __syncthreads();
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -754,7 +789,9 @@ __device__ void lda_svwn5(double rho, double* e, double* v) {
 // Workgroup b processes grid chunks b, b + gridDim, ... (CH points each) and keeps its 16x16
 // output tiles in registers (wave w owns tiles w, w+4, ...), so partials are per workgroup.
 constexpr int CH = 64;
-constexpr int VX_TILES = 9;  // tiles per wave: (npad/16)^2 / 4 for npad <= 96
+constexpr int NT_VXC = 512;
+constexpr int NW_VXC = NT_VXC / 64;
+constexpr int VX_TILES = 5;  // tiles per wave: ceil((npad/16)^2 / 8) for npad <= 96
 // wv_given (optional): per-point w v / 2 computed elsewhere (neural XC); then only sum(rho w) is
 // accumulated and the energy partial is zero.
 // Workgroup bid of nblk (chunks bid, bid + nblk, ...).
@@ -764,7 +801,7 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
   const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16;
   double* ph = sm;                    // CH x S
   double* wvs = ph + (size_t)CH * S;  // CH
-  __shared__ double red[NW * 2];
+  __shared__ double red[NW_VXC * 2];
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   d4_t acc[VX_TILES];
 #pragma unroll
@@ -773,8 +810,10 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
   const long nchunk = (ng + CH - 1) / CH;
   for (long c = bid; c < nchunk; c += nblk) {
     const long g0 = c * CH;
+    TileRegs<NT_VXC> tr;
+    tile_load(ao, g0, ng, n, tr);  // in flight during the point loop below
     __syncthreads();
-    for (int t = threadIdx.x; t < CH; t += NT) {
+    for (int t = threadIdx.x; t < CH; t += NT_VXC) {
       const long g = g0 + t;
       double e = 0.0, v = 0.0, r = 0.0, ww = 0.0;
       if (g < ng) {
@@ -786,16 +825,14 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
       nel += r * ww;
       exc += r * ww * e;
     }
-    for (int t = threadIdx.x; t < CH * npad; t += NT) {
-      const int p = t / npad, col = t % npad;
-      ph[p * S + col] = (g0 + p < ng && col < n) ? ao[(g0 + p) * n + col] : 0.0;
-    }
+    tile_store(tr, n, npad, S, ph);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < VX_TILES; ++t) {
-      const int tile = wv + NW * t;
+      const int tile = wv + NW_VXC * t;
       if (tile < nb * nb) {
         const int bi = tile / nb, bj = tile % nb;
+#pragma unroll 2
         for (int k0 = 0; k0 < CH; k0 += 4) {
           const int kk = k0 + (l >> 4);
           const double av = ph[kk * S + bi * 16 + (l & 15)] * wvs[kk];
@@ -807,7 +844,7 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
   }
 #pragma unroll
   for (int t = 0; t < VX_TILES; ++t) {
-    const int tile = wv + NW * t;
+    const int tile = wv + NW_VXC * t;
     if (tile < nb * nb) {
       const int bi = tile / nb, bj = tile % nb;
 #pragma unroll
@@ -817,14 +854,14 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
       }
     }
   }
-  const double s0 = bsum(nel, red), s1 = bsum(exc, red + NW);
+  const double s0 = bsum(nel, red), s1 = bsum(exc, red + NW_VXC);
   if (threadIdx.x == 0) {
     spart[2 * bid] = s0;
     spart[2 * bid + 1] = s1;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
+__global__ __launch_bounds__(NT_VXC) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
                                             const double* wv_given, double* vpart, double* spart) {
   vxc_body(blockIdx.x, gridDim.x, ng, n, ao, w, rho, wv_given, vpart, spart);
 }
@@ -1176,13 +1213,15 @@ struct Scf3 {
   double* eigf;  // eigensolver scratch (3 n^2)
 };
 
-// Fock + CDIIS (one workgroup).  cycle < 0: plain F = H + Veff (no DIIS).
-__global__ __launch_bounds__(NT) void k_scf_fock(const Scf3* __restrict__ ss, int cycle) {
+// Fock + CDIIS (one workgroup of NT_SCF threads).  cycle < 0: plain F = H + Veff (no DIIS).
+constexpr int NT_SCF = 1024;
+constexpr int NW_SCF = NT_SCF / 64;
+__global__ __launch_bounds__(NT_SCF) void k_scf_fock(const Scf3* __restrict__ ss, int cycle) {
   const Scf3 s = ss[blockIdx.x];
   if (s.ist[0]) return;
-  __shared__ double red[NW];
+  __shared__ double redm[QEX3D_MAX_DIIS][NW_SCF];
   const int n = s.n, nn = n * n;
-  for (int t = threadIdx.x; t < nn; t += NT) {
+  for (int t = threadIdx.x; t < nn; t += NT_SCF) {
     double f = s.H[t] + s.J[t];
     if (s.method != 1) f += s.Vxc[t];
     if (s.method == 1) f -= 0.5 * s.K[t];
@@ -1190,7 +1229,7 @@ __global__ __launch_bounds__(NT) void k_scf_fock(const Scf3* __restrict__ ss, in
   }
   __syncthreads();
   if (cycle < 1 || s.diis_space < 1) {
-    for (int t = threadIdx.x; t < nn; t += NT) s.Fd[t] = s.F[t];
+    for (int t = threadIdx.x; t < nn; t += NT_SCF) s.Fd[t] = s.F[t];
     __syncthreads();
     return;
   }
@@ -1202,23 +1241,41 @@ __global__ __launch_bounds__(NT) void k_scf_fock(const Scf3* __restrict__ ss, in
   const int slot = head;
   double* Fi = s.diisF + (size_t)slot * nn;
   double* Ei = s.diisE + (size_t)slot * nn;
-  for (int t = threadIdx.x; t < nn; t += NT) {
+  for (int t = threadIdx.x; t < nn; t += NT_SCF) {
     const int i = t / n, j = t % n;
     Ei[t] = s.tmp2[i * n + j] - s.tmp2[j * n + i];
     Fi[t] = s.F[t];
   }
   __syncthreads();
   const int m = cnt + 1 < s.diis_space ? cnt + 1 : s.diis_space;
-  // B (stored by slot) row/column of the new vector
+  // B (stored by slot) row/column of the new vector: the m dot products in one pass
   const int M = s.diis_space;
-  for (int q = 0; q < m; ++q) {
-    const double d = dot_all(nn, Ei, s.diisE + (size_t)q * nn, red);
-    if (threadIdx.x == 0) {
-      s.diisB[slot * M + q] = d;
-      s.diisB[q * M + slot] = d;
+  {
+    double d[QEX3D_MAX_DIIS];
+#pragma unroll
+    for (int q = 0; q < QEX3D_MAX_DIIS; ++q) d[q] = 0.0;
+    for (int t = threadIdx.x; t < nn; t += NT_SCF) {
+      const double e = Ei[t];
+#pragma unroll
+      for (int q = 0; q < QEX3D_MAX_DIIS; ++q)
+        if (q < m) d[q] = fma(e, s.diisE[(size_t)q * nn + t], d[q]);
     }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < QEX3D_MAX_DIIS; ++q) {
+      const double x = wsum(d[q]);
+      if (l == 0) redm[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < m) {
+      const int q = threadIdx.x;
+      double v = 0.0;
+      for (int k = 0; k < NW_SCF; ++k) v += redm[q][k];
+      s.diisB[slot * M + q] = v;
+      s.diisB[q * M + slot] = v;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // solve [0 -1; -1 B] [l; c] = [-1; 0] in the chronological order of the stored vectors
   __shared__ double Bm[(QEX3D_MAX_DIIS + 1) * (QEX3D_MAX_DIIS + 2)];
   __shared__ double coef[QEX3D_MAX_DIIS + 1];
@@ -1265,7 +1322,7 @@ __global__ __launch_bounds__(NT) void k_scf_fock(const Scf3* __restrict__ ss, in
     s.ist[3] = (slot + 1) % M;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < nn; t += NT) {
+  for (int t = threadIdx.x; t < nn; t += NT_SCF) {
     double f = 0.0;
     for (int q = 0; q < m; ++q) f = fma(coef[q + 1], s.diisF[(size_t)order[q] * nn + t], f);
     s.Fd[t] = f;
@@ -1326,14 +1383,14 @@ __global__ __launch_bounds__(NT_EIG) void k_gen_eigh(int n, const double* F, con
   (void)Zg;
 }
 
-// Vxc = sum of chunk partials (fixed order), symmetrised; energies and convergence.
-__global__ __launch_bounds__(NT) void k_scf_energy(const Scf3* __restrict__ ss, int cycle) {
+// Energies and convergence (one workgroup of NT_SCF threads).
+__global__ __launch_bounds__(NT_SCF) void k_scf_energy(const Scf3* __restrict__ ss, int cycle) {
   const Scf3 s = ss[blockIdx.x];
   if (s.ist[0]) return;
-  __shared__ double red[NW];
-  const int n = s.n, nn = n * n;
+  __shared__ double red[NW_SCF];
+  const int n = s.n, nn = n * n, nocc = s.nocc;
   double e1 = 0.0, e2 = 0.0;
-  for (int t = threadIdx.x; t < nn; t += NT) {
+  for (int t = threadIdx.x; t < nn; t += NT_SCF) {
     e1 = fma(s.D[t], s.H[t], e1);
     double g = 0.5 * s.J[t];
     if (s.method == 1) g -= 0.25 * s.K[t];
@@ -1351,32 +1408,23 @@ __global__ __launch_bounds__(NT) void k_scf_energy(const Scf3* __restrict__ ss, 
   }
   // orbital gradient norm with the new Fock matrix: |C_vir^T F C_occ|^2 via the S^-1 projector
   // = tr(Co^T F X X^T F Co) - |Co^T F Co|^2, times 4 (the factor 2 of pyscf's get_grad)
-  for (int t = threadIdx.x; t < nn; t += NT) {
+  for (int t = threadIdx.x; t < nn; t += NT_SCF) {
     double f = s.H[t] + s.J[t];
     if (s.method != 1) f += s.Vxc[t];
     if (s.method == 1) f -= 0.5 * s.K[t];
     s.F[t] = f;
   }
   __syncthreads();
-  // tmp1[:, k<nocc] = F C_occ ; tmp2 = X^T tmp1 (columns k < nocc)
-  for (int t = threadIdx.x; t < n * s.nocc; t += NT) {
-    const int i = t / s.nocc, k = t % s.nocc;
-    double v = 0.0;
-    for (int j = 0; j < n; ++j) v = fma(s.F[i * n + j], s.C[j * n + k], v);
-    s.tmp1[i * n + k] = v;
-  }
-  __syncthreads();
+  mfma_gemm(n, nocc, n, s.F, n, false, s.C, n, false, s.tmp1, n, 1.0);    // tmp1 = F C_occ   (n x nocc)
+  mfma_gemm(n, nocc, n, s.X, n, true, s.tmp1, n, false, s.tmp2, n, 1.0);  // tmp2 = X^T F C_occ
+  mfma_gemm(nocc, nocc, n, s.C, n, true, s.tmp1, n, false, s.Cp, n, 1.0); // Cp = C_occ^T F C_occ
   double a = 0.0, b = 0.0;
-  for (int t = threadIdx.x; t < n * s.nocc; t += NT) {
-    const int i = t / s.nocc, k = t % s.nocc;
-    double v = 0.0;
-    for (int j = 0; j < n; ++j) v = fma(s.X[j * n + i], s.tmp1[j * n + k], v);
+  for (int t = threadIdx.x; t < n * nocc; t += NT_SCF) {
+    const double v = s.tmp2[(t / nocc) * n + t % nocc];
     a = fma(v, v, a);
   }
-  for (int t = threadIdx.x; t < s.nocc * s.nocc; t += NT) {
-    const int k = t / s.nocc, l = t % s.nocc;
-    double v = 0.0;
-    for (int j = 0; j < n; ++j) v = fma(s.C[j * n + k], s.tmp1[j * n + l], v);
+  for (int t = threadIdx.x; t < nocc * nocc; t += NT_SCF) {
+    const double v = s.Cp[(t / nocc) * n + t % nocc];
     b = fma(v, v, b);
   }
   a = bsum(a, red);
@@ -1456,7 +1504,7 @@ __global__ __launch_bounds__(NT) void k_rho_b(const Scf3* __restrict__ ss) {
   rho_body(blockIdx.x, s.ng, s.n, s.ao, s.D, s.rho);
 }
 
-__global__ __launch_bounds__(NT) void k_vxc_b(const Scf3* __restrict__ ss) {
+__global__ __launch_bounds__(NT_VXC) void k_vxc_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
   vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.method == QEX3D_RKS_NN_GLOBAL ? s.wv : nullptr,
@@ -1783,7 +1831,7 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   }
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
-    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho,
+    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT_VXC), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho,
                        nullptr, vpart, spart);
   }
   {
@@ -1867,7 +1915,7 @@ int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* 
   }
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
-    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, wv,
+    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT_VXC), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, wv,
                        vpart, spart);
   }
   {
@@ -2049,7 +2097,7 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
       }
       {
         qexprof::Scope ps_(qexprof::K_VXC, st);
-        hipLaunchKernelGGL(k_vxc_b, dim3((unsigned)vxc_blk, nb), dim3(NT), vxc_lds(nmax), st, dss);
+        hipLaunchKernelGGL(k_vxc_b, dim3((unsigned)vxc_blk, nb), dim3(NT_VXC), vxc_lds(nmax), st, dss);
       }
       // Vxc = P + P^T with P the fixed-order sum of the chunk partials; scal[2..3] = nelec, exc
       qexprof::Scope ps_(qexprof::K_OTHER, st);
@@ -2063,6 +2111,10 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     qexprof::Scope ps_(kind, st);
     hipLaunchKernelGGL(kernel, dim3(nb), dim3(NT), 0, st, dss, args...);
   };
+  auto per_mol_wide = [&](int kind, auto kernel, auto... args) {
+    qexprof::Scope ps_(kind, st);
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(NT_SCF), 0, st, dss, args...);
+  };
   auto eigh = [&]() {
     qexprof::Scope ps_(qexprof::K_EIGH, st);
     hipLaunchKernelGGL(k_scf_eigh, dim3(nb), dim3(NT_EIG), lds_eigh, st, dss);
@@ -2070,15 +2122,15 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   // core-Hamiltonian guess: F = H (k_scf_init zeroes J, K, Vxc and D with a kernel, not a memset)
   per_mol(qexprof::K_OTHER, k_scf_init);
   per_mol(qexprof::K_OTHER, k_scf_orthogonaliser);
-  per_mol(qexprof::K_FOCK, k_scf_fock, -1);
+  per_mol_wide(qexprof::K_FOCK, k_scf_fock, -1);
   eigh();
   veff();
-  per_mol(qexprof::K_ENERGY, k_scf_energy, -1);
+  per_mol_wide(qexprof::K_ENERGY, k_scf_energy, -1);
   for (int c = 0; c < max_cycle; ++c) {
-    per_mol(qexprof::K_FOCK, k_scf_fock, c);
+    per_mol_wide(qexprof::K_FOCK, k_scf_fock, c);
     eigh();
     veff();
-    per_mol(qexprof::K_ENERGY, k_scf_energy, c);
+    per_mol_wide(qexprof::K_ENERGY, k_scf_energy, c);
     if (sync_every > 0 && (c + 1) % sync_every == 0 && c + 1 < max_cycle) {
       // optional host check: stop enqueueing cycles once every molecule is frozen
       int32_t left = 0;

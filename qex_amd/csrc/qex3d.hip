@@ -702,63 +702,63 @@ __device__ __forceinline__ void tile_store(const TileRegs<NTH>& r, int n, int np
   for (int t = threadIdx.x; t < 64 * (npad - n); t += NTH) ph[(t / (npad - n)) * S + n + t % (npad - n)] = 0.0;
 }
 
+// Workgroup of NT_RHO threads (one wave per 16-column block of D, n <= 96): wave w computes
+// T[:, 16w:16w+16] = Phi_tile D[:, 16w:16w+16] for the 64 points (4 independent accumulators, its
+// block of D held in registers as the MFMA B operand), then its partial rho = rowsum(Phi * T).
+constexpr int NT_RHO = 64 * (QEX3D_MAX_NAO / 16);
 __device__ __forceinline__ void rho_body(long bid, long ng, int n, const double* ao, const double* dm, double* rho) {
   extern __shared__ double sm[];
-  const int npad = (n + 15) & ~15, S = npad + 1;
+  const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16, ks = npad / 4;
   double* ph = sm;
-  __shared__ double part[NW][64];
+  __shared__ double part[NT_RHO / 64][64];
   const long g0 = bid * 64;
-  {
-    TileRegs<NT> r;
-    tile_load(ao, g0, ng, n, r);
-    tile_store(r, n, npad, S, ph);
-  }
-  __syncthreads();
+  TileRegs<NT_RHO> r;
+  tile_load(ao, g0, ng, n, r);
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double rp[4][4];
+  const int col = wv * 16 + (l & 15);
+  // B operand of step k (k-slab 4k..4k+3): lane l supplies D[4k + l/16][col]
+  double bv[QEX3D_MAX_NAO / 4];
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) rp[rb][v] = 0.0;
-  for (int cb = wv; cb < npad / 16; cb += NW) {
+  for (int k = 0; k < QEX3D_MAX_NAO / 4; ++k) {
+    const int kk = 4 * k + (l >> 4);
+    bv[k] = (wv < nb && k < ks && kk < n && col < n) ? dm[kk * n + col] : 0.0;
+  }
+  tile_store(r, n, npad, S, ph);
+  __syncthreads();
+  if (wv < nb) {
     d4_t acc[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) acc[rb] = (d4_t){0.0, 0.0, 0.0, 0.0};
-    const int col = cb * 16 + (l & 15);
-    for (int k0 = 0; k0 < npad; k0 += 4) {
-      const int kk = k0 + (l >> 4);
-      const double bv = (kk < n && col < n) ? dm[kk * n + col] : 0.0;
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb) acc[rb] = mfma64(ph[(rb * 16 + (l & 15)) * S + kk], bv, acc[rb]);
+    for (int k = 0; k < QEX3D_MAX_NAO / 4; ++k) {
+      if (k < ks) {
+        const int kk = 4 * k + (l >> 4);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) acc[rb] = mfma64(ph[(rb * 16 + (l & 15)) * S + kk], bv[k], acc[rb]);
+      }
     }
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = rb * 16 + 4 * v + (l >> 4);
-        rp[rb][v] = fma(ph[row * S + col], acc[rb][v], rp[rb][v]);
+        double x = ph[row * S + col] * acc[rb][v];
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        if ((l & 15) == 0) part[wv][row] = x;
       }
   }
-#pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double x = rp[rb][v];
-      x += __shfl_xor(x, 1, 64);
-      x += __shfl_xor(x, 2, 64);
-      x += __shfl_xor(x, 4, 64);
-      x += __shfl_xor(x, 8, 64);
-      if ((l & 15) == 0) part[wv][rb * 16 + 4 * v + (l >> 4)] = x;
-    }
   __syncthreads();
   if (threadIdx.x < 64 && g0 + threadIdx.x < ng) {
-    double r = 0.0;
-    for (int q = 0; q < NW; ++q) r += part[q][threadIdx.x];
-    rho[g0 + threadIdx.x] = r;
+    double x = 0.0;
+    for (int q = 0; q < nb; ++q) x += part[q][threadIdx.x];
+    rho[g0 + threadIdx.x] = x;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho) {
+__global__ __launch_bounds__(NT_RHO) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho) {
   rho_body(blockIdx.x, ng, n, ao, dm, rho);
 }
 
@@ -827,17 +827,17 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
     }
     tile_store(tr, n, npad, S, ph);
     __syncthreads();
+    // k outer, tiles inner: the wave's accumulators are independent, so its MFMAs pipeline
+    for (int k0 = 0; k0 < CH; k0 += 4) {
+      const int kk = k0 + (l >> 4);
+      const double wk = wvs[kk];
+      const double* rowk = ph + kk * S + (l & 15);
 #pragma unroll
-    for (int t = 0; t < VX_TILES; ++t) {
-      const int tile = wv + NW_VXC * t;
-      if (tile < nb * nb) {
-        const int bi = tile / nb, bj = tile % nb;
-#pragma unroll 2
-        for (int k0 = 0; k0 < CH; k0 += 4) {
-          const int kk = k0 + (l >> 4);
-          const double av = ph[kk * S + bi * 16 + (l & 15)] * wvs[kk];
-          const double bv = ph[kk * S + bj * 16 + (l & 15)];
-          acc[t] = mfma64(av, bv, acc[t]);
+      for (int t = 0; t < VX_TILES; ++t) {
+        const int tile = wv + NW_VXC * t;
+        if (tile < nb * nb) {
+          const int bi = tile / nb, bj = tile % nb;
+          acc[t] = mfma64(rowk[bi * 16] * wk, rowk[bj * 16], acc[t]);
         }
       }
     }
@@ -871,7 +871,7 @@ size_t vxc_lds(int n) { return ((size_t)CH * (((n + 15) & ~15) + 1) + CH) * 8; }
 constexpr int VXC_BLOCKS = 256;
 
 void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, hipStream_t st) {
-  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT), rho_lds(n), st, ng, n, ao, dm, rho);
+  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT_RHO), rho_lds(n), st, ng, n, ao, dm, rho);
 }
 
 __host__ __device__ inline int vxc_blocks(long ng) {
@@ -1050,9 +1050,13 @@ __global__ void k_copy1(const double* src, double* dst) {
 }
 
 // J_ij = sum_kl (ij|kl) D_kl over the pair matrix (row P = pair(i,j)); K (optional) by gathers.
-__host__ __device__ inline int coulomb_blocks(int npair) { return npair < 2048 ? npair : 2048; }
+__host__ __device__ inline int coulomb_blocks(int npair) {
+  const int b = (npair + 3) / 4;  // four rows per workgroup pass
+  return b < 1024 ? b : 1024;
+}
 
-// workgroup bid of nblk
+// workgroup bid of nblk: D' staged in LDS, then one wave per row P of the pair matrix (rows
+// bid * NW + w, stepping by nblk * NW), 16-byte loads, wave-level reduction only.
 __device__ __forceinline__ void coulomb_body(int bid, int nblk, int n, int npair, const double* eri, const double* dm,
                                              double* J, double* K) {
   extern __shared__ double sDp[];  // D'_Q = D_kl (2 - delta_kl)
@@ -1065,18 +1069,38 @@ __device__ __forceinline__ void coulomb_body(int bid, int nblk, int n, int npair
     sDp[q] = k == l ? dm[k * n + l] : dm[k * n + l] + dm[l * n + k];
   }
   __syncthreads();
-  for (int P = bid; P < npair; P += nblk) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool even = (npair & 1) == 0;  // rows start 16-byte aligned
+  for (int P = bid * NW + wv; P < npair; P += nblk * NW) {
     const double* row = eri + (size_t)P * npair;
-    double s = 0.0;
-    for (int q = threadIdx.x; q < npair; q += NT) s = fma(row[q], sDp[q], s);
-    s = bsum(s, red);
-    if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    if (even) {
+      const double2* r2 = reinterpret_cast<const double2*>(row);
+      const int h = npair / 2;
+      int q = lane;
+      for (; q + 64 < h; q += 128) {
+        const double2 a = r2[q], b = r2[q + 64];
+        s0 = fma(a.x, sDp[2 * q], s0);
+        s1 = fma(a.y, sDp[2 * q + 1], s1);
+        s0 = fma(b.x, sDp[2 * q + 128], s0);
+        s1 = fma(b.y, sDp[2 * q + 129], s1);
+      }
+      for (; q < h; q += 64) {
+        const double2 a = r2[q];
+        s0 = fma(a.x, sDp[2 * q], s0);
+        s1 = fma(a.y, sDp[2 * q + 1], s1);
+      }
+    } else {
+      for (int q = lane; q < npair; q += 64) s0 = fma(row[q], sDp[q], s0);
+    }
+    const double sum = wsum(s0 + s1);
+    if (lane == 0) {
       int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
       while (i * (i + 1) / 2 > P) --i;
       while ((i + 1) * (i + 2) / 2 <= P) ++i;
       const int j = P - i * (i + 1) / 2;
-      J[i * n + j] = s;
-      J[j * n + i] = s;
+      J[i * n + j] = sum;
+      J[j * n + i] = sum;
     }
   }
   if (K) {
@@ -1498,7 +1522,7 @@ __global__ void k_put_scf(ScfPut pk, int count, Scf3* dst) {
 }
 
 // ---- per-molecule wrappers of the grid and Coulomb kernels (blockIdx.y = molecule) ----
-__global__ __launch_bounds__(NT) void k_rho_b(const Scf3* __restrict__ ss) {
+__global__ __launch_bounds__(NT_RHO) void k_rho_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (long)blockIdx.x >= (s.ng + 63) / 64) return;
   rho_body(blockIdx.x, s.ng, s.n, s.ao, s.D, s.rho);
@@ -2086,7 +2110,7 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     if (any_grid) {
       {
         qexprof::Scope ps_(qexprof::K_RHO, st);
-        hipLaunchKernelGGL(k_rho_b, dim3((unsigned)rho_blk, nb), dim3(NT), rho_lds(nmax), st, dss);
+        hipLaunchKernelGGL(k_rho_b, dim3((unsigned)rho_blk, nb), dim3(NT_RHO), rho_lds(nmax), st, dss);
       }
       if (any_nn) {
         qexprof::Scope ps_(qexprof::K_NN, st);

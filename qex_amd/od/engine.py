@@ -44,6 +44,14 @@ def as_i32(x, dev) -> torch.Tensor:
     return torch.as_tensor(np.asarray(x, dtype=np.int32), device=dev).contiguous()
 
 
+def upload(x, dtype, dev) -> torch.Tensor:
+    """Host array -> device tensor without waiting for the stream: staged through pinned memory
+    (torch's caching host allocator keeps the staging block until the copy has run).  A copy from
+    pageable memory would block the host until the work queued before it on the stream is done."""
+    t = torch.from_numpy(np.ascontiguousarray(x, dtype=dtype))
+    return t.pin_memory().to(dev, non_blocking=True)
+
+
 def workspace(nbytes: int, dev) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
 
@@ -295,8 +303,11 @@ class BatchRunner:
         if np.any(ne < 1) or np.any(ne > self.problem.max_electrons):
             raise ValueError(f"num_electrons must be in [1, {self.problem.max_electrons}]")
         dev, N, T = self.dev, self.N, self.T
-        loc_d, z_d, ne_d = as_f64(loc, dev), as_f64(z, dev), as_i32(ne, dev)
-        n0 = as_f64(initial_density, dev).reshape(b, N).contiguous()
+        loc_d, z_d, ne_d = upload(loc, np.float64, dev), upload(z, np.float64, dev), upload(ne, np.int32, dev)
+        if isinstance(initial_density, torch.Tensor):
+            n0 = as_f64(initial_density, dev).reshape(b, N).contiguous()
+        else:
+            n0 = upload(np.asarray(initial_density, dtype=np.float64).reshape(b, N), np.float64, dev)
         lib = _lib.load()
         vext = torch.empty((b, N), dtype=torch.float64, device=dev)
         p0, p1 = self.interaction_fn.params

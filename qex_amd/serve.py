@@ -68,6 +68,8 @@ class ServeStats:
     batches_1d: int = 0
     batch_sizes: list = field(default_factory=list)
     batch_sizes_3d: list = field(default_factory=list)
+    host_s: dict = field(default_factory=lambda: {"dispatch_1d": 0.0, "dispatch_3d": 0.0, "complete_1d": 0.0,
+                                                  "complete_3d": 0.0})
 
     def summary(self):
         lat = np.array([c.latency for c in self.completions]) * 1e3
@@ -78,7 +80,8 @@ class ServeStats:
                "batches_1d": self.batches_1d,
                "mean_batch_1d": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0,
                "batches_3d": len(self.batch_sizes_3d),
-               "mean_batch_3d": float(np.mean(self.batch_sizes_3d)) if self.batch_sizes_3d else 0.0}
+               "mean_batch_3d": float(np.mean(self.batch_sizes_3d)) if self.batch_sizes_3d else 0.0,
+               "host_busy_s": {k: round(v, 4) for k, v in self.host_s.items()}}
         for kind in ("1d", "3d"):
             lk = np.array([c.latency for c in self.completions if c.kind == kind]) * 1e3
             if len(lk):
@@ -157,6 +160,7 @@ class Server:
             # completions (in order per stream)
             for q, kind in [(inflight_1d, "1d")] + [(q3, "3d") for q3 in inflight_3d]:
                 while q and q[0].event.query():
+                    th = time.perf_counter()
                     f = q.popleft()
                     done = time.perf_counter() - t0
                     used -= f.nbytes
@@ -170,6 +174,7 @@ class Server:
                         for r, mf in zip(f.reqs, f.handle):
                             e = mf.collect()
                             stats.completions.append(Completion(r.rid, "3d", r.arrival, done, e, mf.converged))
+                    stats.host_s["complete_" + kind] += time.perf_counter() - th
                     progressed = True
             # dispatch
             if pend_1d and len(inflight_1d) < self.depth_1d:
@@ -181,7 +186,9 @@ class Server:
                         break
                     batch.append(pend_1d.popleft())
                     nb += need
+                th = time.perf_counter()
                 out, ev = self._dispatch_1d(batch)
+                stats.host_s["dispatch_1d"] += time.perf_counter() - th
                 used += nb
                 inflight_1d.append(_Inflight(ev, batch, out, nb))
                 stats.batches_1d += 1
@@ -199,7 +206,9 @@ class Server:
                     batch.append(pend_3d.popleft())
                     nb += need
                 if batch:
+                    th = time.perf_counter()
                     mfs, ev = self._dispatch_3d(batch, self.streams_3d[k3])
+                    stats.host_s["dispatch_3d"] += time.perf_counter() - th
                     used += nb
                     inflight_3d[k3].append(_Inflight(ev, batch, mfs, nb))
                     stats.batch_sizes_3d.append(len(batch))

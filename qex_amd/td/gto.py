@@ -98,8 +98,8 @@ class Mole:
                 for lmn in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
                     ao_shell.append(si)
                     ao_lmn.append(lmn[0] | (lmn[1] << 2) | (lmn[2] << 4))
-        f = lambda x: engine.as_f64(np.asarray(x, dtype=np.float64).reshape(-1), dev)
-        i = lambda x: engine.as_i32(np.asarray(x, dtype=np.int32), dev)
+        f = lambda x: engine.upload(np.asarray(x, dtype=np.float64).reshape(-1), np.float64, dev)
+        i = lambda x: engine.upload(np.asarray(x, dtype=np.int32), np.int32, dev)
         keep = [f(self.coords), f(self.charges), i(sh_atom), i(sh_l), i(sh_n), i(sh_off), f(exps), f(coefs),
                 i(ao_shell), i(ao_lmn)]
         b = _lib.Qex3dBasis(self.natm, len(self.shells), self.nao, *[ctypes.c_void_p(t.data_ptr()) for t in keep],

@@ -214,11 +214,67 @@ __device__ __forceinline__ void ao_info(const Basis& b, int mu, int* sh, int* lx
   *lz = (c >> 4) & 3;
 }
 
-// one-electron integrals, one thread per (i >= j)
-__global__ __launch_bounds__(128) void k_int1e(Basis b, double* S, double* Tm, double* V) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// overlap and kinetic contributions of one primitive pair (exponents a, bb) to S_ij and T_ij
+__device__ __forceinline__ void st_prim(const int* la, const int* lb, const double* A, const double* B, double a,
+                                        double bb, double cc, double* s, double* t) {
+  const double p = a + bb, mu = a * bb / p, i2p = 0.5 / p;
+  double s1[3], dd[3];
+  for (int d = 0; d < 3; ++d) {
+    const double Pd = (a * A[d] + bb * B[d]) / p;
+    const double xab = A[d] - B[d];
+    const double K = exp(-mu * xab * xab);
+    const double xpa = Pd - A[d], xpb = Pd - B[d];
+    const double sq = sqrt(M_PI / p);
+    // 1D overlaps s(la, j) for j = 0..3 from the Hermite E_0 of (la, j) (la <= 1)
+    double Ei[4];
+    {
+      double E0[4][5] = {{0}}, E1[4][5] = {{0}};
+      E0[0][0] = K;
+      for (int jj = 0; jj < 3; ++jj)
+        for (int tt = 0; tt <= jj + 1; ++tt)
+          E0[jj + 1][tt] = (tt > 0 ? i2p * E0[jj][tt - 1] : 0.0) + xpb * E0[jj][tt] + (tt + 1) * E0[jj][tt + 1];
+      for (int jj = 0; jj < 4; ++jj)
+        for (int tt = 0; tt <= jj + 1; ++tt)
+          E1[jj][tt] = (tt > 0 ? i2p * E0[jj][tt - 1] : 0.0) + xpa * E0[jj][tt] + (tt + 1) * (tt + 1 <= 4 ? E0[jj][tt + 1] : 0.0);
+      for (int jj = 0; jj < 4; ++jj) Ei[jj] = (la[d] == 0 ? E0[jj][0] : E1[jj][0]) * sq;
+    }
+    const int jb = lb[d];
+    s1[d] = Ei[jb];
+    const double sm2 = jb >= 2 ? Ei[jb - 2] : 0.0;
+    dd[d] = jb * (jb - 1) * sm2 - 2.0 * bb * (2 * jb + 1) * Ei[jb] + 4.0 * bb * bb * Ei[jb + 2];
+  }
+  *s += cc * s1[0] * s1[1] * s1[2];
+  *t += cc * -0.5 * (dd[0] * s1[1] * s1[2] + s1[0] * dd[1] * s1[2] + s1[0] * s1[1] * dd[2]);
+}
+
+// nuclear-attraction contribution of one primitive pair and one nucleus C (charge Zc) to V_ij
+__device__ __forceinline__ double v_prim_atom(const int* la, const int* lb, const double* A, const double* B, double a,
+                                              double bb, double cc, const double* C, double Zc) {
+  const double p = a + bb, mu = a * bb / p, i2p = 0.5 / p;
+  double P[3], Ex[3], Ey[3], Ez[3];
+  for (int d = 0; d < 3; ++d) P[d] = (a * A[d] + bb * B[d]) / p;
+  hermite_E1(la[0], lb[0], exp(-mu * (A[0] - B[0]) * (A[0] - B[0])), P[0] - A[0], P[0] - B[0], i2p, Ex);
+  hermite_E1(la[1], lb[1], exp(-mu * (A[1] - B[1]) * (A[1] - B[1])), P[1] - A[1], P[1] - B[1], i2p, Ey);
+  hermite_E1(la[2], lb[2], exp(-mu * (A[2] - B[2]) * (A[2] - B[2])), P[2] - A[2], P[2] - B[2], i2p, Ez);
+  const int tx = la[0] + lb[0], ty = la[1] + lb[1], tz = la[2] + lb[2];
+  const double X = P[0] - C[0], Y = P[1] - C[1], Z = P[2] - C[2];
+  const Boys5 F = boys5(p * (X * X + Y * Y + Z * Z));
+  double acc = 0.0;
+  for (int t0 = 0; t0 <= tx; ++t0)
+    for (int u0 = 0; u0 <= ty; ++u0)
+      for (int v0 = 0; v0 <= tz; ++v0) {
+        const double e = Ex[t0] * Ey[u0] * Ez[v0];
+        if (e != 0.0) acc += e * rtuv(t0, u0, v0, p, X, Y, Z, F);
+      }
+  return -cc * Zc * 2.0 * M_PI / p * acc;
+}
+
+// one-electron integrals, one wave per AO pair (i >= j): lanes take the primitive pairs of S and T
+// and the (primitive pair, nucleus) terms of V; wave sums in a fixed order
+__global__ __launch_bounds__(256) void k_int1e(Basis b, double* S, double* Tm, double* V) {
+  const int idx = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = b.nao;
-  if (idx >= n * (n + 1) / 2) return;
+  if (idx >= n * (n + 1) / 2) return;  // whole waves
   int i = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
   while (i * (i + 1) / 2 > idx) --i;
   while ((i + 1) * (i + 2) / 2 <= idx) ++i;
@@ -228,72 +284,26 @@ __global__ __launch_bounds__(128) void k_int1e(Basis b, double* S, double* Tm, d
   ao_info(b, j, &sj, &lb[0], &lb[1], &lb[2]);
   const double* A = b.xyz + 3 * b.sh_atom[si];
   const double* B = b.xyz + 3 * b.sh_atom[sj];
+  const int npa = b.sh_nprim[si], npb = b.sh_nprim[sj], npp = npa * npb;
+  const int oa = b.sh_off[si], ob = b.sh_off[sj];
   double s = 0.0, t = 0.0, v = 0.0;
-  for (int pa = 0; pa < b.sh_nprim[si]; ++pa) {
-    const double a = b.exps[b.sh_off[si] + pa], ca = b.coefs[b.sh_off[si] + pa];
-    for (int pb = 0; pb < b.sh_nprim[sj]; ++pb) {
-      const double bb = b.exps[b.sh_off[sj] + pb], cb = b.coefs[b.sh_off[sj] + pb];
-      const double p = a + bb, mu = a * bb / p, i2p = 0.5 / p;
-      double P[3], s1[3], dd[3];
-      for (int d = 0; d < 3; ++d) {
-        P[d] = (a * A[d] + bb * B[d]) / p;
-        const double xab = A[d] - B[d];
-        const double K = exp(-mu * xab * xab);
-        const double xpa = P[d] - A[d], xpb = P[d] - B[d];
-        const double sq = sqrt(M_PI / p);
-        // 1D overlaps s(i, j) for j = lb-2 .. lb+2 via the Hermite E_0 of (la, j), j up to 3
-        // s(la, j) with the transfer relation s(i, j+1) = s(i+1, j) + X_AB s(i, j) is avoided:
-        // build E0 recursively for j <= 3 (la <= 1).
-        double Ei[4];  // E_0^{la, j} for j = 0..3 (la fixed)
-        {
-          // E^{0,j}_t and E^{1,j}_t for j <= 3, t <= 4 by the standard recursions
-          double E0[4][5] = {{0}}, E1[4][5] = {{0}};
-          E0[0][0] = K;
-          for (int jj = 0; jj < 3; ++jj)
-            for (int tt = 0; tt <= jj + 1; ++tt)
-              E0[jj + 1][tt] = (tt > 0 ? i2p * E0[jj][tt - 1] : 0.0) + xpb * E0[jj][tt] + (tt + 1) * E0[jj][tt + 1];
-          for (int jj = 0; jj < 4; ++jj)
-            for (int tt = 0; tt <= jj + 1; ++tt)
-              E1[jj][tt] = (tt > 0 ? i2p * E0[jj][tt - 1] : 0.0) + xpa * E0[jj][tt] + (tt + 1) * (tt + 1 <= 4 ? E0[jj][tt + 1] : 0.0);
-          for (int jj = 0; jj < 4; ++jj) Ei[jj] = (la[d] == 0 ? E0[jj][0] : E1[jj][0]) * sq;
-        }
-        const int jb = lb[d];
-        s1[d] = Ei[jb];
-        const double sm2 = jb >= 2 ? Ei[jb - 2] : 0.0;
-        dd[d] = jb * (jb - 1) * sm2 - 2.0 * bb * (2 * jb + 1) * Ei[jb] + 4.0 * bb * bb * Ei[jb + 2];
-      }
-      const double cc = ca * cb;
-      s += cc * s1[0] * s1[1] * s1[2];
-      t += cc * -0.5 * (dd[0] * s1[1] * s1[2] + s1[0] * dd[1] * s1[2] + s1[0] * s1[1] * dd[2]);
-      // nuclear attraction
-      double Ex[3], Ey[3], Ez[3];
-      {
-        const double K0 = exp(-mu * (A[0] - B[0]) * (A[0] - B[0]));
-        const double K1 = exp(-mu * (A[1] - B[1]) * (A[1] - B[1]));
-        const double K2 = exp(-mu * (A[2] - B[2]) * (A[2] - B[2]));
-        hermite_E1(la[0], lb[0], K0, P[0] - A[0], P[0] - B[0], i2p, Ex);
-        hermite_E1(la[1], lb[1], K1, P[1] - A[1], P[1] - B[1], i2p, Ey);
-        hermite_E1(la[2], lb[2], K2, P[2] - A[2], P[2] - B[2], i2p, Ez);
-      }
-      const int tx = la[0] + lb[0], ty = la[1] + lb[1], tz = la[2] + lb[2];
-      for (int c = 0; c < b.natm; ++c) {
-        const double* C = b.xyz + 3 * c;
-        const double X = P[0] - C[0], Y = P[1] - C[1], Z = P[2] - C[2];
-        const Boys5 F = boys5(p * (X * X + Y * Y + Z * Z));
-        double acc = 0.0;
-        for (int t0 = 0; t0 <= tx; ++t0)
-          for (int u0 = 0; u0 <= ty; ++u0)
-            for (int v0 = 0; v0 <= tz; ++v0) {
-              const double e = Ex[t0] * Ey[u0] * Ez[v0];
-              if (e != 0.0) acc += e * rtuv(t0, u0, v0, p, X, Y, Z, F);
-            }
-        v -= cc * b.charge[c] * 2.0 * M_PI / p * acc;
-      }
-    }
+  for (int r = lane; r < npp; r += 64) {
+    const int pa = r / npb, pb = r % npb;
+    st_prim(la, lb, A, B, b.exps[oa + pa], b.exps[ob + pb], b.coefs[oa + pa] * b.coefs[ob + pb], &s, &t);
   }
-  S[i * n + j] = S[j * n + i] = s;
-  Tm[i * n + j] = Tm[j * n + i] = t;
-  V[i * n + j] = V[j * n + i] = v;
+  for (int q = lane; q < npp * b.natm; q += 64) {
+    const int r = q / b.natm, c = q % b.natm, pa = r / npb, pb = r % npb;
+    v += v_prim_atom(la, lb, A, B, b.exps[oa + pa], b.exps[ob + pb], b.coefs[oa + pa] * b.coefs[ob + pb],
+                     b.xyz + 3 * c, b.charge[c]);
+  }
+  s = wsum(s);
+  t = wsum(t);
+  v = wsum(v);
+  if (lane == 0) {
+    S[i * n + j] = S[j * n + i] = s;
+    Tm[i * n + j] = Tm[j * n + i] = t;
+    V[i * n + j] = V[j * n + i] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -706,7 +716,11 @@ __device__ __forceinline__ void tile_store(const TileRegs<NTH>& r, int n, int np
 // T[:, 16w:16w+16] = Phi_tile D[:, 16w:16w+16] for the 64 points (4 independent accumulators, its
 // block of D held in registers as the MFMA B operand), then its partial rho = rowsum(Phi * T).
 constexpr int NT_RHO = 64 * (QEX3D_MAX_NAO / 16);
-__device__ __forceinline__ void rho_body(long bid, long ng, int n, const double* ao, const double* dm, double* rho) {
+__device__ void lda_svwn5(double rho, double* e, double* v);
+// With wv given: the LDA at the 64 points in the epilogue, wv = w v / 2 and ex = rho w e per point
+// (the Vxc kernel then only reads them).
+__device__ __forceinline__ void rho_body(long bid, long ng, int n, const double* ao, const double* dm, double* rho,
+                                         const double* w = nullptr, double* wv_out = nullptr, double* ex_out = nullptr) {
   extern __shared__ double sm[];
   const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16, ks = npad / 4;
   double* ph = sm;
@@ -752,14 +766,23 @@ __device__ __forceinline__ void rho_body(long bid, long ng, int n, const double*
   }
   __syncthreads();
   if (threadIdx.x < 64 && g0 + threadIdx.x < ng) {
+    const long g = g0 + threadIdx.x;
     double x = 0.0;
     for (int q = 0; q < nb; ++q) x += part[q][threadIdx.x];
-    rho[g0 + threadIdx.x] = x;
+    rho[g] = x;
+    if (wv_out) {
+      double e, v;
+      lda_svwn5(x, &e, &v);
+      const double ww = w[g];
+      wv_out[g] = 0.5 * ww * v;
+      ex_out[g] = x * ww * e;
+    }
   }
 }
 
-__global__ __launch_bounds__(NT_RHO) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho) {
-  rho_body(blockIdx.x, ng, n, ao, dm, rho);
+__global__ __launch_bounds__(NT_RHO) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho,
+                                                const double* w, double* wv, double* ex) {
+  rho_body(blockIdx.x, ng, n, ao, dm, rho, w, wv, ex);
 }
 
 // LDA: Slater exchange + VWN5 correlation (spin unpolarised); e per electron and v = d(rho e)/d rho
@@ -792,11 +815,12 @@ constexpr int CH = 64;
 constexpr int NT_VXC = 512;
 constexpr int NW_VXC = NT_VXC / 64;
 constexpr int VX_TILES = 5;  // tiles per wave: ceil((npad/16)^2 / 8) for npad <= 96
-// wv_given (optional): per-point w v / 2 computed elsewhere (neural XC); then only sum(rho w) is
-// accumulated and the energy partial is zero.
+// wv_given: per-point w v / 2 (from the LDA epilogue of the density kernel, or the neural XC);
+// ex_given (optional): per-point rho w e, summed into the energy partial (zero without it).
 // Workgroup bid of nblk (chunks bid, bid + nblk, ...).
 __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, const double* ao, const double* w,
-                                         const double* rho, const double* wv_given, double* vpart, double* spart) {
+                                         const double* rho, const double* wv_given, const double* ex_given,
+                                         double* vpart, double* spart) {
   extern __shared__ double sm[];
   const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16;
   double* ph = sm;                    // CH x S
@@ -815,15 +839,14 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
     __syncthreads();
     for (int t = threadIdx.x; t < CH; t += NT_VXC) {
       const long g = g0 + t;
-      double e = 0.0, v = 0.0, r = 0.0, ww = 0.0;
+      double r = 0.0, ww = 0.0;
       if (g < ng) {
         r = rho[g];
         ww = w[g];
-        if (!wv_given) lda_svwn5(r, &e, &v);
+        if (ex_given) exc += ex_given[g];
       }
-      wvs[t] = wv_given ? (g < ng ? wv_given[g] : 0.0) : 0.5 * ww * v;
+      wvs[t] = g < ng ? wv_given[g] : 0.0;
       nel += r * ww;
-      exc += r * ww * e;
     }
     tile_store(tr, n, npad, S, ph);
     __syncthreads();
@@ -862,16 +885,19 @@ __device__ __forceinline__ void vxc_body(int bid, int nblk, long ng, int n, cons
 }
 
 __global__ __launch_bounds__(NT_VXC) void k_vxc(long ng, int n, const double* ao, const double* w, const double* rho,
-                                            const double* wv_given, double* vpart, double* spart) {
-  vxc_body(blockIdx.x, gridDim.x, ng, n, ao, w, rho, wv_given, vpart, spart);
+                                            const double* wv_given, const double* ex_given, double* vpart,
+                                            double* spart) {
+  vxc_body(blockIdx.x, gridDim.x, ng, n, ao, w, rho, wv_given, ex_given, vpart, spart);
 }
 
 size_t rho_lds(int n) { return (size_t)64 * (((n + 15) & ~15) + 1) * 8; }
 size_t vxc_lds(int n) { return ((size_t)CH * (((n + 15) & ~15) + 1) + CH) * 8; }
 constexpr int VXC_BLOCKS = 256;
 
-void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, hipStream_t st) {
-  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT_RHO), rho_lds(n), st, ng, n, ao, dm, rho);
+void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, hipStream_t st,
+                  const double* w = nullptr, double* wv = nullptr, double* ex = nullptr) {
+  hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT_RHO), rho_lds(n), st, ng, n, ao, dm, rho, w, wv,
+                     ex);
 }
 
 __host__ __device__ inline int vxc_blocks(long ng) {
@@ -1233,7 +1259,7 @@ struct Scf3 {
   const double* nn;
   int h1, h2;
   double nn_scale;
-  double *zpart, *z1, *dz1, *wv;
+  double *zpart, *z1, *dz1, *wv, *ex;
   double* eigf;  // eigensolver scratch (3 n^2)
 };
 
@@ -1486,9 +1512,18 @@ __global__ __launch_bounds__(NT) void k_scf_init(const Scf3* __restrict__ ss) {
   }
 }
 
-__global__ __launch_bounds__(NT) void k_scf_orthogonaliser(const Scf3* __restrict__ ss) {
+// Cholesky and triangular inverse on LDS copies (2 n^2 doubles of dynamic LDS), X written out
+__global__ __launch_bounds__(NT_SCF) void k_scf_orthogonaliser(const Scf3* __restrict__ ss) {
+  extern __shared__ double sm[];
   const Scf3 s = ss[blockIdx.x];
-  orthogonaliser(s.n, s.S, s.Lc, s.X);
+  const int nn = s.n * s.n;
+  double* Ls = sm;
+  double* Xs = sm + nn;
+  orthogonaliser(s.n, s.S, Ls, Xs);
+  for (int t = threadIdx.x; t < nn; t += NT_SCF) {
+    s.X[t] = Xs[t];
+    s.Lc[t] = Ls[t];
+  }
 }
 
 // fill e_hist beyond the last cycle with the final energy (frozen state repeats); status out
@@ -1525,13 +1560,16 @@ __global__ void k_put_scf(ScfPut pk, int count, Scf3* dst) {
 __global__ __launch_bounds__(NT_RHO) void k_rho_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (long)blockIdx.x >= (s.ng + 63) / 64) return;
-  rho_body(blockIdx.x, s.ng, s.n, s.ao, s.D, s.rho);
+  if (s.method == QEX3D_RKS_LDA)
+    rho_body(blockIdx.x, s.ng, s.n, s.ao, s.D, s.rho, s.w, s.wv, s.ex);
+  else
+    rho_body(blockIdx.x, s.ng, s.n, s.ao, s.D, s.rho);
 }
 
 __global__ __launch_bounds__(NT_VXC) void k_vxc_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
-  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.method == QEX3D_RKS_NN_GLOBAL ? s.wv : nullptr,
+  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.wv, s.method == QEX3D_RKS_LDA ? s.ex : nullptr,
            s.vpart, s.spart);
 }
 
@@ -1570,7 +1608,7 @@ __global__ __launch_bounds__(NT) void k_coulomb_b(const Scf3* __restrict__ ss) {
 
 struct Lay3 {
   size_t F, Fd, D, Dlast, C, Cp, mo, J, K, Vxc, vpart, spart, rho, tmp1, tmp2, diisF, diisE, diisB, scal, ist;
-  size_t zpart, z1, dz1, wv, eigf, total;
+  size_t zpart, z1, dz1, wv, ex, eigf, total;
 };
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1608,7 +1646,8 @@ Lay3 layout3(int n, long ng, int space, int h1 = 0) {
   L.zpart = take(h1 > 0 ? (size_t)gnn_blocks(ng) * h1 * 8 : 0);
   L.z1 = take((size_t)h1 * 8);
   L.dz1 = take((size_t)h1 * 8);
-  L.wv = take(h1 > 0 ? (size_t)ng * 8 : 0);
+  L.wv = take((size_t)ng * 8);
+  L.ex = take((size_t)ng * 8);
   L.eigf = take((size_t)3 * n * n * 8);
   L.total = off;
   return L;
@@ -1742,7 +1781,7 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
   const int np = b->nao * (b->nao + 1) / 2;
   ensure_boys((hipStream_t)stream);
   qexprof::Scope ps(qexprof::K_INT1E, (hipStream_t)stream);
-  hipLaunchKernelGGL(k_int1e, dim3((np + 127) / 128), dim3(128), 0, (hipStream_t)stream, make_basis(b), S, T, V);
+  hipLaunchKernelGGL(k_int1e, dim3((np + 3) / 4), dim3(256), 0, (hipStream_t)stream, make_basis(b), S, T, V);
   return launch_status("qex3d_int1e");
 }
 
@@ -1835,7 +1874,8 @@ int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* d
 size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao) {
   if (ngrid < 1 || nao < 1) return 0;
   const long nb = vxc_blocks(ngrid);
-  return al((size_t)ngrid * 8) + al((size_t)nb * nao * nao * 8) + al((size_t)nb * 16) + al((size_t)nao * nao * 8);
+  return al((size_t)ngrid * 8) + al((size_t)nb * nao * nao * 8) + al((size_t)nb * 16) + al((size_t)nao * nao * 8) +
+         2 * al((size_t)ngrid * 8);
 }
 
 int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
@@ -1849,14 +1889,16 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   double* rho = (double*)workspace;
   double* vpart = (double*)((char*)workspace + al((size_t)ngrid * 8));
   double* spart = (double*)((char*)vpart + al((size_t)nb * nao * nao * 8));
+  double* wv = (double*)((char*)spart + al((size_t)nb * 16) + al((size_t)nao * nao * 8));
+  double* ex = (double*)((char*)wv + al((size_t)ngrid * 8));
   {
     qexprof::Scope ps(qexprof::K_RHO, st);
-    rho_dispatch(ngrid, nao, ao, dm, rho, st);
+    rho_dispatch(ngrid, nao, ao, dm, rho, st, weights, wv, ex);
   }
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
     hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT_VXC), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho,
-                       nullptr, vpart, spart);
+                       wv, ex, vpart, spart);
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
@@ -1940,7 +1982,7 @@ int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* 
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
     hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT_VXC), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho, wv,
-                       vpart, spart);
+                       nullptr, vpart, spart);
   }
   {
     qexprof::Scope ps(qexprof::K_OTHER, st);
@@ -2072,6 +2114,7 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     s.z1 = (double*)(w + L.z1);
     s.dz1 = (double*)(w + L.dz1);
     s.wv = (double*)(w + L.wv);
+    s.ex = (double*)(w + L.ex);
     s.eigf = (double*)(w + L.eigf);
     nmax = n > nmax ? n : nmax;
     max_cycle = p->max_cycle > max_cycle ? p->max_cycle : max_cycle;
@@ -2145,7 +2188,14 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   };
   // core-Hamiltonian guess: F = H (k_scf_init zeroes J, K, Vxc and D with a kernel, not a memset)
   per_mol(qexprof::K_OTHER, k_scf_init);
-  per_mol(qexprof::K_OTHER, k_scf_orthogonaliser);
+  {
+    qexprof::Scope ps_(qexprof::K_OTHER, st);
+    const size_t lds_o = (size_t)2 * nmax * nmax * 8;
+    if (lds_o > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_scf_orthogonaliser, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_o);
+    hipLaunchKernelGGL(k_scf_orthogonaliser, dim3(nb), dim3(NT_SCF), lds_o, st, dss);
+  }
   per_mol_wide(qexprof::K_FOCK, k_scf_fock, -1);
   eigh();
   veff();

@@ -98,11 +98,19 @@ __device__ __forceinline__ void ldl_solve(const double* e, int s, int t, const d
   }
 }
 
-// sum over the 8 lanes of an aligned lane group (every lane receives the same value)
+// one DPP move of a double (both halves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// sum over the 8 lanes of an aligned lane group by DPP moves (swap within pairs, within quads,
+// then the half-row mirror pairs each quad with the other one); every lane receives the same value
 __device__ __forceinline__ double group8_sum(double x) {
-  x += __shfl_xor(x, 1, 64);
-  x += __shfl_xor(x, 2, 64);
-  x += __shfl_xor(x, 4, 64);
+  x += dpp_f64<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
   return x;
 }
 
@@ -226,7 +234,9 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
   __syncthreads();
   // ---- 3. eigenvalues by multisection ----
   {
-    const int G = nt / n;  // >= 10 for n <= 96 and 1024 threads
+    // G shifts per eigenvalue (>= 10 for n <= 96 and 1024 threads), at most 32: every thread scans
+    // its eigenvalue's G counts after each round, so small n gains nothing from more
+    const int G = nt / n < 32 ? nt / n : 32;
     const int slot = tid / G, g = tid % G;
     const bool act = slot < n;
     double lo = 0.0, hi = 0.0, rmax = 0.0;

@@ -108,8 +108,10 @@ def parse_args():
                     help="launch one 64 MiB device-to-device copy first (reference for HBM counters)")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
     ap.add_argument("--requests", type=int, default=1024, help="c5: requests per arrival rate")
-    ap.add_argument("--rates", type=str, default="100,500,2000", help="c5: Poisson arrival rates (req/s)")
-    ap.add_argument("--batch-3d", type=int, default=16,
+    ap.add_argument("--rates", type=str, default="100,500,2000,5000", help="c5: Poisson arrival rates (req/s)")
+    ap.add_argument("--streams-3d", type=int, default=2, help="c5: concurrent 3D streams")
+    ap.add_argument("--depth-3d", type=int, default=1, help="c5: 3D batches in flight per stream")
+    ap.add_argument("--batch-3d", type=int, default=96,
                     help="c4: 1 solves the chains one at a time, >1 solves them as one batched SCF; "
                          "c5: most 3D requests per batched dispatch")
     return ap.parse_args()
@@ -601,7 +603,8 @@ def run_c5(args):
     # warm-up: a few of each kind through both paths
     warm = requests(np.zeros(args.requests))[:16]
     serve.run_one_at_a_time(warm, runner, solver_3d)
-    server = serve.Server(runner, solver_3d, max_batch=args.batch, batch_3d=args.batch_3d)
+    server = serve.Server(runner, solver_3d, max_batch=args.batch, batch_3d=args.batch_3d, streams_3d=args.streams_3d,
+                          depth_3d=args.depth_3d)
     server.run(warm)
     torch.cuda.synchronize()
 
@@ -654,6 +657,7 @@ def run_c5(args):
         "config": {"workload": f"c5 served mixed stream: {args.requests} requests per rate, 70% c2 1D / 30% c4 chains, "
                                "batched 1D dispatch + concurrent 3D stream; value = best served throughput",
                    "rates": rates, "max_batch_1d": args.batch, "max_batch_3d": args.batch_3d,
+                   "streams_3d": args.streams_3d, "depth_3d": args.depth_3d,
                    "path": "served FP64"},
         "served": per_rate,
         "plain_one_at_a_time": {"throughput_per_s": plain_rate, "wall_s": plain.wall},

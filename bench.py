@@ -463,6 +463,11 @@ def run_c3(args):
     from qex_amd.od import engine, train, utils, xc
 
     lib = _lib.load()
+    if args.calibrate_copy:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     # targets: the FP64 LDA solve (c1 settings) on the device
     lda = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [2] * B, utils.exponential_coulomb,
                            xc.get_lda_xc_energy_density_fn(), T, alpha=ALPHA, alpha_decay=1.0,

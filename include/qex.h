@@ -152,6 +152,9 @@ int qex1d_hartree_potential(int32_t batch, int32_t num_grids, double dx, const d
  * No counterpart in the reference (which has no profiler hooks). */
 int qex_profile_enable(int32_t enable, int32_t max_launches);
 int qex_profile_read(double* ms_per_kind, int32_t* launches_per_kind, int32_t nkinds);
+/* Kernel kinds that get events while profiling is enabled: bit k for kind k (default all).
+ * Timing only the kinds of interest keeps the event records out of the other launches. */
+int qex_profile_kinds(uint32_t mask);
 
 const char* qex_last_error(void);
 int32_t qex_abi_version(void);

@@ -131,6 +131,7 @@ SIGNATURES = {
     "qex1d_hartree_potential": (_I, [_I, _I, _D, _P, _P, _P, _P]),
     "qex_profile_enable": (_I, [_I, _I]),
     "qex_profile_read": (_I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), _I]),
+    "qex_profile_kinds": (_I, [ctypes.c_uint32]),
     "qex1d_kohn_sham_vjp_workspace_bytes": (_S, [ctypes.POINTER(Qex1dProblem)]),
     "qex1d_kohn_sham_vjp": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
                                  ctypes.POINTER(Qex1dOutputs), _P, _P, _P, _P, _S, _P]),

@@ -19,6 +19,7 @@ struct Rec {
 
 // defined once (qex1d.hip); the library's calls are host-serialised per the threading contract
 extern bool g_on;
+extern unsigned g_mask;  // kinds that are timed (bit per kind)
 extern Rec* g_rec;
 extern int g_n, g_cap;
 extern hipEvent_t* g_pool;
@@ -31,7 +32,7 @@ struct Scope {
   hipStream_t st;
   hipEvent_t b = nullptr, e = nullptr;
   Scope(int k, hipStream_t s) : kind(k), st(s) {
-    if (!g_on || g_n >= g_cap) return;
+    if (!g_on || g_n >= g_cap || !((g_mask >> k) & 1u)) return;
     b = pool_event();
     e = pool_event();
     if (b && e) (void)hipEventRecord(b, st);

@@ -29,6 +29,7 @@
 
 namespace qexprof {
 bool g_on = false;
+unsigned g_mask = 0xffffffffu;
 Rec* g_rec = nullptr;
 int g_n = 0, g_cap = 0;
 hipEvent_t* g_pool = nullptr;
@@ -42,6 +43,14 @@ using qexband::ldlt_factor_forward;
 using qexband::ldlt_forward;
 using qexband::rcp64;
 using qexband::sturm_count;
+using qexband::Side;
+using qexband::SideZ;
+using qexband::sturm_side;
+using qexband::sturm_join;
+using qexband::middle_solve;
+using qexband::ldlt_side_factor_forward;
+using qexband::ldlt_side_forward;
+using qexband::ldlt_side_backward;
 
 constexpr int NT = 256;     // threads per workgroup (4 wave64)
 constexpr int NWAVE = NT / 64;
@@ -940,34 +949,51 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
     }
   }
   int nrounds = 0;
-  // Multisection: NT shifts per round (both ends included) until the bracket of the j-th
+  // Multisection: NS shifts per round (both ends included) until the bracket of the j-th
   // eigenvalue is 1e-9 relative; two inverse-iteration solves and the Rayleigh quotient then
-  // give the eigenpair to ~1e-13.
+  // give the eigenpair to ~1e-13.  For N >= 4 each shift is counted by a pair of lanes, the even
+  // lane eliminating the top rows 0..kt-1 and the odd lane the bottom rows N-1..kt+2 (half the
+  // recurrence each), joined through the 2x2 block of rows kt, kt+1.
+  const bool two = N >= 4;
+  const int kt = (N - 2) / 2, kb = N - 2 - kt;
+  const int NS = two ? NT / 2 : NT, me = two ? tid >> 1 : tid;
   for (int round = 0; round < 80; ++round) {
     const double mid = 0.5 * (lo + hi);
     const double tol = 1e-9 * fmax(1.0, fabs(mid)) + 64.0 * EPS * hnorm;
     if (hi - lo <= tol) break;
     ++nrounds;
-    const double step = (hi - lo) / (double)(NT - 1);
-    const double sig = tid == NT - 1 ? hi : fma((double)tid, step, lo);
-    const int cnt = sturm_count(a, N, e.Tb, e.Tc, sig);
+    const double step = (hi - lo) / (double)(NS - 1);
+    const double sig = me == NS - 1 ? hi : fma((double)me, step, lo);
+    int cnt;
+    if (two) {
+      const bool bot = tid & 1;
+      const Side h = sturm_side(bot ? a + N - 1 : a, bot ? -1 : 1, bot ? kb : kt, e.Tb, e.Tc, sig);
+      const Side o{__shfl_xor(h.r1, 1, 64), __shfl_xor(h.r2, 1, 64), __shfl_xor(h.u1, 1, 64), __shfl_xor(h.neg, 1, 64)};
+      cnt = bot ? sturm_join(o, h, a[kt], a[kt + 1], e.Tb, e.Tc, sig) : sturm_join(h, o, a[kt], a[kt + 1], e.Tb, e.Tc, sig);
+    } else {
+      cnt = sturm_count(a, N, e.Tb, e.Tc, sig);
+    }
     const unsigned long long m = __ballot(cnt > j);
-    if ((tid & 63) == 0) sh_first[tid >> 6] = m ? (__ffsll((long long)m) - 1) + (tid & ~63) : NT;
+    if ((tid & 63) == 0) sh_first[tid >> 6] = m ? (((__ffsll((long long)m) - 1) + (tid & ~63)) >> (two ? 1 : 0)) : NS;
     __syncthreads();
-    int first = NT;
+    int first = NS;
 #pragma unroll
     for (int q = 0; q < NWAVE; ++q) first = min(first, sh_first[q]);
     if (tid == 0) {
+      // the eigenvalue lies outside the bracket: the next one is the adjacent interval 256 times
+      // as wide (clipped to the Gershgorin bounds), so a missed warm start costs about one round
+      const double wide = 256.0 * (hi - lo);
       if (first == 0) {
-        // the eigenvalue lies below the bracket: widen down to the Gershgorin bound
         sh_hi = lo;
-        sh_lo = lo > gl ? gl : lo;
-      } else if (first == NT) {
+        sh_lo = fmax(gl, lo - wide);
+        if (!(sh_lo < lo)) sh_lo = lo > gl ? gl : lo;
+      } else if (first == NS) {
         sh_lo = hi;
-        sh_hi = hi < gu ? gu : hi;
+        sh_hi = fmin(gu, hi + wide);
+        if (!(sh_hi > hi)) sh_hi = hi < gu ? gu : hi;
       } else {
         sh_lo = first - 1 == 0 ? lo : fma((double)(first - 1), step, lo);
-        sh_hi = first == NT - 1 ? hi : fma((double)first, step, lo);
+        sh_hi = first == NS - 1 ? hi : fma((double)first, step, lo);
       }
     }
     __syncthreads();
@@ -986,8 +1012,36 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
   double* v = e.vec + ((size_t)s * e.K + j) * N;
   for (int t = tid; t < N; t += NT) y[t] = warm ? v[t] + 1e-3 * start_value(t, j) : start_value(t, j);
   __syncthreads();
+  // For N >= 4 lanes 0 and 1 of wave 0 solve the top and bottom halves (twisted at rows kt, kt+1):
+  // factorisation and forward solve of each side, the middle pair, then each side's back
+  // substitution outward.  F rows of the bottom side are stored at their own row indices.
+  Side fs{};
   for (int it = 0; it < 2; ++it) {
-    if (tid == 0) {
+    if (two) {
+      if (tid < 2) {
+        const bool bot = tid == 1;
+        const int st = bot ? -1 : 1, rows = bot ? kb : kt;
+        double* Fs = bot ? F + 4 * (N - 1) : F;
+        double* ys = bot ? y + N - 1 : y;
+        const SideZ z = it == 0 ? ldlt_side_factor_forward(bot ? a + N - 1 : a, ys, st, rows, e.Tb, e.Tc, sigma,
+                                                           EPS * hnorm, Fs)
+                                : ldlt_side_forward(ys, st, rows, Fs);
+        if (it == 0) fs = z.s;
+        const Side os{__shfl_xor(fs.r1, 1, 64), __shfl_xor(fs.r2, 1, 64), __shfl_xor(fs.u1, 1, 64), 0};
+        const SideZ oz{os, __shfl_xor(z.l1, 1, 64), __shfl_xor(z.m1, 1, 64), __shfl_xor(z.m2, 1, 64),
+                       __shfl_xor(z.z1, 1, 64), __shfl_xor(z.z2, 1, 64)};
+        double xk, xk1;
+        if (bot)
+          middle_solve(os, fs, oz, z, a[kt], a[kt + 1], y[kt], y[kt + 1], e.Tb, e.Tc, sigma, EPS * hnorm, xk, xk1);
+        else
+          middle_solve(fs, os, z, oz, a[kt], a[kt + 1], y[kt], y[kt + 1], e.Tb, e.Tc, sigma, EPS * hnorm, xk, xk1);
+        ldlt_side_backward(Fs, st, rows, bot ? xk1 : xk, bot ? xk : xk1, ys);
+        if (tid == 0) {
+          y[kt] = xk;
+          y[kt + 1] = xk1;
+        }
+      }
+    } else if (tid == 0) {
       if (it == 0)
         ldlt_factor_forward(a, y, N, e.Tb, e.Tc, sigma, EPS * hnorm, F);
       else
@@ -1461,6 +1515,11 @@ int qex_profile_enable(int32_t enable, int32_t max_launches) {
     g_pool_used = 0;
   }
   g_on = enable != 0;
+  return QEX_OK;
+}
+
+int qex_profile_kinds(uint32_t mask) {
+  qexprof::g_mask = mask;
   return QEX_OK;
 }
 

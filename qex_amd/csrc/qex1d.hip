@@ -434,7 +434,9 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k) {
 // k_mlp_mfma: the same windowed MLP on FP64 matrix cores (v_mfma_f64_16x16x4f64) for hidden
 // widths <= 64 and <= 4 hidden layers.  A workgroup takes 64 consecutive points of the
 // flattened [B][N] grid; each layer is a 64 x H tile GEMM, wave w owning output columns
-// 16w..16w+15.  Activation derivatives stay in registers for the backward GEMMs (W^T).
+// 16w..16w+15, written back in place into the one activation tile once every wave has read it
+// (34 KB of LDS, so three workgroups share a CU and a batch of 513 tiles fits the 256 CUs in
+// one round).  Activation derivatives stay in registers for the backward GEMMs (W^T).
 // MFMA operand layout (16x16x4, one wave): lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // it receives D[4v + l/16][l%16] in element v.
 // --------------------------------------------------------------------------------------------
@@ -482,6 +484,7 @@ __device__ __forceinline__ void mlp_hidden_fwd(const Dev1& a, int lh, const doub
   tile_gemm(cur, fi, Wl, fo, fo, false, wv, acc);
   const int col = wv * 16 + (l & 15);
   const double bias = col < fo ? bl[col] : 0.0;
+  __syncthreads();  // every wave has read cur (nxt may be the same tile)
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb) {
 #pragma unroll
@@ -504,6 +507,7 @@ __device__ __forceinline__ void mlp_hidden_bwd(const Dev1& a, int lh, const doub
   d4_t acc[4];
   tile_gemm(cur, fo, Wl, fo, fi, true, wv, acc);
   const int col = wv * 16 + (l & 15);
+  __syncthreads();  // every wave has read cur (nxt may be the same tile)
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
@@ -527,10 +531,11 @@ __device__ __forceinline__ void mlp_out_grad(double* nxt, const double* cot, dou
 }
 
 template <int NH>
-__global__ __launch_bounds__(NT, 2) void k_mlp_mfma(Dev1 a, int k) {
+__global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k) {
   extern __shared__ double smem[];
-  double* bufA = smem;                 // 64 x MLP_S
-  double* bufB = bufA + MP * MLP_S;    // 64 x MLP_S
+  // one activation tile: each layer's GEMM keeps its result in registers until every wave has
+  // read the tile, then writes it back in place (34 KB of LDS: three workgroups per CU)
+  double* bufB = smem;                 // 64 x MLP_S
   double* cot = bufB + MP * MLP_S;     // 64 cotangents
   __shared__ int sh_any[2];
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
@@ -555,45 +560,42 @@ __global__ __launch_bounds__(NT, 2) void k_mlp_mfma(Dev1 a, int k) {
     if (pass == 1 && !any_e) continue;
     const bool backward = pass == 0;
     // ---- window inputs (x / norm) into bufB, zero padded to a multiple of 4 columns ----
+    // 4 threads per point: the system index and status are read once per point, and a thread's
+    // window loads are independent, so they are in flight together
+    static_assert(MP * 4 == NT, "4 threads per point");
     const int wp = rup(W, 4);
-    for (int idx = tid; idx < MP * wp; idx += NT) {
-      const int p = idx / wp, o = idx % wp;
+    {
+      const int p = tid >> 2;
       const long g = g0 + p;
-      double x = 0.0;
-      if (g < total && o < W) {
-        const int s = (int)(g / N), i = (int)(g % N);
-        const bool act = pass == 0 ? (k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0);
+      const bool inb = g < total;
+      const int s = inb ? (int)(g / N) : 0, i = inb ? (int)(g % N) : 0;
+      const bool act = inb && (pass == 0 ? (k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0));
+      const double* src = pass == 0 ? input_density(a, s, k) : a.o_xc + (size_t)s * N;
+      int lo = 0, hi = -1;
+      if (pass == 0 && sym && act) {
+        lo = a.win[2 * s];
+        hi = a.win[2 * s + 1];
+      }
+#pragma unroll 4
+      for (int o = tid & 3; o < wp; o += 4) {
         const int j = i + o - h;
-        if (act && j >= 0 && j < N) {
-          if (pass == 0) {
-            const double* nin = input_density(a, s, k);
-            x = nin[j];
-            if (sym) {
-              int lo = a.win[2 * s], hi = a.win[2 * s + 1];
-              x = 0.5 * (x + nin[mirror(j, lo, hi)]);
-            }
-          } else {
-            x = a.o_xc[(size_t)s * N + j];
-          }
+        double x = 0.0;
+        if (act && o < W && j >= 0 && j < N) {
+          x = src[j];
+          if (pass == 0 && sym) x = 0.5 * (x + src[mirror(j, lo, hi)]);
           x /= a.norm;
         }
         if (o == h) cot[p] = act ? x * a.norm : 0.0;  // the point's own density (cotangent of e)
-      } else if (o == h) {
-        cot[p] = 0.0;
+        bufB[p * MLP_S + o] = x;
       }
-      bufB[p * MLP_S + o] = x;
     }
     __syncthreads();
     // ---- hidden layers (one named register array of activation derivatives per layer) ----
     // NH (compile time) = L - 1 hidden layers; only the arrays in use stay live
     double dz0[4][4], dz1[NH > 1 ? 4 : 1][4], dz2[NH > 2 ? 4 : 1][4], dz3[NH > 3 ? 4 : 1][4];
     double* cur = bufB;
-    double* nxt = bufA;
-    auto swap_bufs = [&]() {
-      double* t = cur;
-      cur = nxt;
-      nxt = t;
-    };
+    double* nxt = bufB;
+    auto swap_bufs = [&]() {};
     if constexpr (NH > 0) { mlp_hidden_fwd(a, 0, cur, nxt, dz0, wv, l); swap_bufs(); }
     if constexpr (NH > 1) { mlp_hidden_fwd(a, 1, cur, nxt, dz1, wv, l); swap_bufs(); }
     if constexpr (NH > 2) { mlp_hidden_fwd(a, 2, cur, nxt, dz2, wv, l); swap_bufs(); }
@@ -1654,7 +1656,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   const size_t lds_eig = eigen_lds_bytes(a.N);
   bool mlp_mfma = a.xc == QEX1D_XC_LOCAL_MLP && a.nlay - 1 <= MLP_LH && a.W <= MLP_S - 1;
   for (int l = 1; l < a.nlay && mlp_mfma; ++l) mlp_mfma = a.widths[l] <= MLP_HMAX;
-  const size_t lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(2 * MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
+  const size_t lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
   if (lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", lds_mlp);
   const dim3 gmlp_mfma((unsigned)(((long)a.B * a.N + MP - 1) / MP));
   const dim3 gpot(a.nTiles, a.B), geig(a.K * a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);

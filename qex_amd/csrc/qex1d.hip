@@ -432,16 +432,18 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k) {
 
 // --------------------------------------------------------------------------------------------
 // k_mlp_mfma: the same windowed MLP on FP64 matrix cores (v_mfma_f64_16x16x4f64) for hidden
-// widths <= 64 and <= 4 hidden layers.  A workgroup takes 64 consecutive points of the
-// flattened [B][N] grid; each layer is a 64 x H tile GEMM, wave w owning output columns
-// 16w..16w+15, written back in place into the one activation tile once every wave has read it
-// (34 KB of LDS, so three workgroups share a CU and a batch of 513 tiles fits the 256 CUs in
-// one round).  Activation derivatives stay in registers for the backward GEMMs (W^T).
+// widths <= 64 and <= 4 hidden layers.  A workgroup takes MP = 48 consecutive points of the
+// flattened [B][N] grid; each layer is a 48 x H tile GEMM (3 row blocks), wave w owning output
+// columns 16w..16w+15, written back in place into the one activation tile once every wave has
+// read it (25 KB of LDS, so three workgroups share a CU: the 64 x 513 points of a c2 batch are
+// 684 tiles, one round on 256 CUs with at most 144 points per CU).  Activation derivatives stay
+// in registers for the backward GEMMs (W^T).
 // MFMA operand layout (16x16x4, one wave): lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // it receives D[4v + l/16][l%16] in element v.
 // --------------------------------------------------------------------------------------------
 typedef double d4_t __attribute__((ext_vector_type(4)));
-constexpr int MP = 64;         // points per workgroup
+constexpr int MP = 48;         // points per workgroup (3 MFMA row blocks)
+constexpr int MLP_RB = MP / 16;  // row blocks of 16 points
 constexpr int MLP_HMAX = 64;   // widest hidden layer on the MFMA path
 constexpr int MLP_LH = 4;      // hidden layers on the MFMA path
 constexpr int MLP_S = 65;      // LDS row stride (doubles) of the activation tiles
@@ -452,14 +454,14 @@ __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
 
 __device__ __forceinline__ int rup(int x, int m) { return (x + m - 1) / m * m; }
 
-// acc[rb] (rows 16rb.., columns of this wave's block cb) = in[64 x K] . B[K x ncols], where
+// acc[rb] (rows 16rb.., columns of this wave's block cb) = in[MP x K] . B[K x ncols], where
 // B[kk][col] = W[kk * ldw + col] (or W[col * ldw + kk] when trans); entries outside
 // kk < K, col < ncols are zero.  `in` is an LDS tile with row stride MLP_S, zero beyond K.
 __device__ __forceinline__ void tile_gemm(const double* in, int K, const double* W, int ldw, int ncols, bool trans,
-                                         int cb, d4_t (&acc)[4]) {
+                                         int cb, d4_t (&acc)[MLP_RB]) {
   const int l = threadIdx.x & 63;
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb) acc[rb] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  for (int rb = 0; rb < MLP_RB; ++rb) acc[rb] = (d4_t){0.0, 0.0, 0.0, 0.0};
   const int col = cb * 16 + (l & 15);
   const int kp = rup(K, 4);
   for (int k0 = 0; k0 < kp; k0 += 4) {
@@ -467,7 +469,7 @@ __device__ __forceinline__ void tile_gemm(const double* in, int K, const double*
     double bv = 0.0;
     if (kk < K && col < ncols) bv = trans ? W[(size_t)col * ldw + kk] : W[(size_t)kk * ldw + col];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
+    for (int rb = 0; rb < MLP_RB; ++rb) {
       const double av = in[(rb * 16 + (l & 15)) * MLP_S + kk];
       acc[rb] = mfma64(av, bv, acc[rb]);
     }
@@ -476,17 +478,17 @@ __device__ __forceinline__ void tile_gemm(const double* in, int K, const double*
 
 // One hidden layer of the tile MLP: nxt = act(cur W_lh + b_lh); keeps act' in dzl.
 __device__ __forceinline__ void mlp_hidden_fwd(const Dev1& a, int lh, const double* cur, double* nxt,
-                                               double (&dzl)[4][4], int wv, int l) {
+                                               double (&dzl)[MLP_RB][4], int wv, int l) {
   const int fi = a.widths[lh], fo = a.widths[lh + 1];
   const double* Wl = a.mlp + a.woff[lh];
   const double* bl = a.mlp + a.boff[lh];
-  d4_t acc[4];
+  d4_t acc[MLP_RB];
   tile_gemm(cur, fi, Wl, fo, fo, false, wv, acc);
   const int col = wv * 16 + (l & 15);
   const double bias = col < fo ? bl[col] : 0.0;
   __syncthreads();  // every wave has read cur (nxt may be the same tile)
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb) {
+  for (int rb = 0; rb < MLP_RB; ++rb) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = rb * 16 + 4 * v + (l >> 4);
@@ -501,15 +503,15 @@ __device__ __forceinline__ void mlp_hidden_fwd(const Dev1& a, int lh, const doub
 
 // Backward through hidden layer lh: nxt = (cur W_lh^T) * act'(z_{lh-1}) (dzp holds act' of lh-1).
 __device__ __forceinline__ void mlp_hidden_bwd(const Dev1& a, int lh, const double* cur, double* nxt,
-                                               const double (&dzp)[4][4], int wv, int l) {
+                                               const double (&dzp)[MLP_RB][4], int wv, int l) {
   const int fi = a.widths[lh], fo = a.widths[lh + 1];
   const double* Wl = a.mlp + a.woff[lh];
-  d4_t acc[4];
+  d4_t acc[MLP_RB];
   tile_gemm(cur, fo, Wl, fo, fi, true, wv, acc);
   const int col = wv * 16 + (l & 15);
   __syncthreads();  // every wave has read cur (nxt may be the same tile)
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
+  for (int rb = 0; rb < MLP_RB; ++rb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = rb * 16 + 4 * v + (l >> 4);
@@ -519,10 +521,10 @@ __device__ __forceinline__ void mlp_hidden_bwd(const Dev1& a, int lh, const doub
 }
 
 // Gradient at the last hidden layer's pre-activations: cot_row * w_out[col] * act'.
-__device__ __forceinline__ void mlp_out_grad(double* nxt, const double* cot, double wo, const double (&dzl)[4][4],
+__device__ __forceinline__ void mlp_out_grad(double* nxt, const double* cot, double wo, const double (&dzl)[MLP_RB][4],
                                              int col, int l) {
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
+  for (int rb = 0; rb < MLP_RB; ++rb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = rb * 16 + 4 * v + (l >> 4);
@@ -535,8 +537,8 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
   extern __shared__ double smem[];
   // one activation tile: each layer's GEMM keeps its result in registers until every wave has
   // read the tile, then writes it back in place (34 KB of LDS: three workgroups per CU)
-  double* bufB = smem;                 // 64 x MLP_S
-  double* cot = bufB + MP * MLP_S;     // 64 cotangents
+  double* bufB = smem;                 // MP x MLP_S
+  double* cot = bufB + MP * MLP_S;     // MP cotangents
   __shared__ int sh_any[2];
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
   const int N = a.N, W = a.W, h = W / 2, L = a.nlay;
@@ -562,9 +564,9 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
     // ---- window inputs (x / norm) into bufB, zero padded to a multiple of 4 columns ----
     // 4 threads per point: the system index and status are read once per point, and a thread's
     // window loads are independent, so they are in flight together
-    static_assert(MP * 4 == NT, "4 threads per point");
+    static_assert(MP * 4 <= NT, "4 threads per point");
     const int wp = rup(W, 4);
-    {
+    if (tid < MP * 4) {
       const int p = tid >> 2;
       const long g = g0 + p;
       const bool inb = g < total;
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
     __syncthreads();
     // ---- hidden layers (one named register array of activation derivatives per layer) ----
     // NH (compile time) = L - 1 hidden layers; only the arrays in use stay live
-    double dz0[4][4], dz1[NH > 1 ? 4 : 1][4], dz2[NH > 2 ? 4 : 1][4], dz3[NH > 3 ? 4 : 1][4];
+    double dz0[MLP_RB][4], dz1[NH > 1 ? MLP_RB : 1][4], dz2[NH > 2 ? MLP_RB : 1][4], dz3[NH > 3 ? MLP_RB : 1][4];
     double* cur = bufB;
     double* nxt = bufB;
     auto swap_bufs = [&]() {};
@@ -606,11 +608,12 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
     {
       const int p = tid >> 2, q = tid & 3;
       double part = 0.0;
-      for (int u = q; u < hl; u += 4) part = fma(cur[p * MLP_S + u], WL[u], part);
+      if (p < MP)
+        for (int u = q; u < hl; u += 4) part = fma(cur[p * MLP_S + u], WL[u], part);
       part += __shfl_xor(part, 1, 64);
       part += __shfl_xor(part, 2, 64);
       const long g = g0 + p;
-      if (q == 0 && g < total) {
+      if (q == 0 && p < MP && g < total) {
         const double e = part + a.mlp[a.boff[L - 1]];
         if (pass == 0)
           a.mlp_e[g] = e;
@@ -643,11 +646,11 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
       const int f1 = a.widths[1];
       const double* W0 = a.mlp + a.woff[0];
       if (wv * 16 < W) {
-        d4_t acc[4];
+        d4_t acc[MLP_RB];
         tile_gemm(cur, f1, W0, f1, W, true, wv, acc);
         const int col = wv * 16 + (l & 15);
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
+        for (int rb = 0; rb < MLP_RB; ++rb)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int row = rb * 16 + 4 * v + (l >> 4);

@@ -79,10 +79,26 @@ using qexprof::Scope;
 // --------------------------------------------------------------------------------------------
 // device helpers
 // --------------------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
+// Sum over the 64 lanes (all active), the same value in every lane: DPP moves within each row of
+// 16 lanes (pairs, quads, half-row mirror, row mirror), then the four row sums in a fixed order.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  x += dpp_f64<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
 }
 
 // Sums M values over the workgroup; every thread receives the totals.  Fixed order.
@@ -1073,11 +1089,119 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
 // --------------------------------------------------------------------------------------------
 constexpr int PPT = (QEX1D_MAX_GRIDS + NT - 1) / NT;
 
+// Cholesky QR of keig <= CQR_K orbitals, which come out of inverse iteration nearly
+// orthonormal: one reduction gives the Gram matrix G = V^T V, every thread factors G = R^T R
+// (the same values everywhere, so no barrier), and Q = V R^-1 point by point -- the Q of
+// Gram-Schmidt in column order.  The Rayleigh quotients of all orbitals then take one more
+// reduction.  Returns false, having written nothing, when a pivot of R is not clearly positive
+// (nearly dependent orbitals), for the Gram-Schmidt path to handle.
+constexpr int CQR_KMAX = 8;
+template <int CQR_K>
+__device__ bool orbitals_cqr(double* vecs, int N, int keig, int ne, double dx, const double* a, double b, double c,
+                             double* lam, double* dens_reg) {
+  constexpr int CQR_M = CQR_K * (CQR_K + 1) / 2;
+  __shared__ double redg[NWAVE * CQR_M];
+  double x[CQR_K][PPT];
+#pragma unroll
+  for (int j = 0; j < CQR_K; ++j)
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int t = threadIdx.x + p * NT;
+      x[j][p] = (j < keig && t < N) ? vecs[(size_t)j * N + t] : 0.0;
+    }
+  // G (upper triangle, row by row) and then R in place
+  double g[CQR_M];
+#pragma unroll
+  for (int i = 0, m = 0; i < CQR_K; ++i)
+#pragma unroll
+    for (int j = i; j < CQR_K; ++j, ++m) {
+      double acc = 0.0;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) acc = fma(x[i][p], x[j][p], acc);
+      g[m] = acc;
+    }
+  block_sum<CQR_M>(g, redg);
+  auto at = [](int i, int j) { return i * CQR_K - i * (i - 1) / 2 + (j - i); };  // upper (i <= j)
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < CQR_K; ++j) {
+    if (j < keig) {
+#pragma unroll
+      for (int i = 0; i < j; ++i) {
+        double v = g[at(i, j)];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = fma(-g[at(k, i)], g[at(k, j)], v);
+        g[at(i, j)] = v / g[at(i, i)];
+      }
+      double d = g[at(j, j)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) d = fma(-g[at(k, j)], g[at(k, j)], d);
+      if (!(d > 1e-12 * g[at(j, j)])) ok = false;
+      g[at(j, j)] = ok ? sqrt(d) : 1.0;
+    }
+  }
+  if (!ok) return false;
+  // Q = V R^-1, the density of the occupied orbitals, Q back to memory
+  const int nocc = (ne + 1) / 2;
+#pragma unroll
+  for (int j = 0; j < CQR_K; ++j) {
+    if (j < keig) {
+      const double inv = 1.0 / g[at(j, j)];
+      const double occ = j < nocc ? (double)min(2, ne - 2 * j) : 0.0;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        double v = x[j][p];
+#pragma unroll
+        for (int i = 0; i < j; ++i) v = fma(-g[at(i, j)], x[i][p], v);
+        v *= inv;
+        x[j][p] = v;
+        const int t = threadIdx.x + p * NT;
+        if (t < N) {
+          vecs[(size_t)j * N + t] = v;
+          dens_reg[p] = fma(occ, v * v / dx, dens_reg[p]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // Rayleigh quotients x^T H x with H = diag(a) + b (|i-j| = 1) + c (|i-j| = 2)
+  double rq[CQR_K];
+#pragma unroll
+  for (int j = 0; j < CQR_K; ++j) {
+    rq[j] = 0.0;
+    if (j < keig) {
+      const double* vj = vecs + (size_t)j * N;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int t = threadIdx.x + p * NT;
+        if (t < N) {
+          double hx = a[t] * x[j][p];
+          if (t >= 1) hx = fma(b, vj[t - 1], hx);
+          if (t + 1 < N) hx = fma(b, vj[t + 1], hx);
+          if (t >= 2) hx = fma(c, vj[t - 2], hx);
+          if (t + 2 < N) hx = fma(c, vj[t + 2], hx);
+          rq[j] = fma(x[j][p], hx, rq[j]);
+        }
+      }
+    }
+  }
+  block_sum<CQR_K>(rq, redg);
+  if (threadIdx.x == 0)
+    for (int j = 0; j < keig; ++j) lam[j] = rq[j];
+  return true;
+}
+
 __device__ void orbitals_to_density(double* vecs, int N, int keig, int ne, double dx, const double* a, double b,
                                     double c, double* lam, double* dens_reg, double* red) {
 #pragma unroll
   for (int p = 0; p < PPT; ++p) dens_reg[p] = 0.0;
   const int nocc = (ne + 1) / 2;
+  // the Gram matrix is sized to the orbital count: 6, 15 or 36 entries in one reduction
+  const bool cqr = keig <= 3   ? orbitals_cqr<3>(vecs, N, keig, ne, dx, a, b, c, lam, dens_reg)
+                   : keig <= 5 ? orbitals_cqr<5>(vecs, N, keig, ne, dx, a, b, c, lam, dens_reg)
+                   : keig <= CQR_KMAX ? orbitals_cqr<CQR_KMAX>(vecs, N, keig, ne, dx, a, b, c, lam, dens_reg)
+                                      : false;
+  if (!cqr)
   for (int jj = 0; jj < keig; ++jj) {
     double x[PPT];
     double* vj = vecs + (size_t)jj * N;

@@ -64,8 +64,10 @@ def test_noninteracting_density_matches_golden(golden_dir):
         assert abs(float(eeig) - ref[1]) < 1e-10 and abs(float(gap) - ref[2]) < 1e-10
 
 
-@pytest.mark.parametrize("ne", [1, 2, 3, 5, 8])
+@pytest.mark.parametrize("ne", [1, 2, 3, 5, 8, 14, 16])
 def test_noninteracting_many_electrons(ne):
+    """ne // 2 + 1 orbitals: up to 8 are orthonormalised by Cholesky QR (ne = 14 is the largest
+    such case), more by Gram-Schmidt (ne = 16)."""
     from qex_amd.od import engine, utils
 
     grids = O.h2_config_grid()

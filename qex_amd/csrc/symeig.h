@@ -45,14 +45,28 @@ __device__ __forceinline__ double rcp_nr(double d) {
 // number of eigenvalues of the block below x (negative pivots of the LDL^T of T - x); e2 = e^2
 // The reciprocal of a pivot is clamped to +-rmax = 1/pivmin instead of testing the pivot; the
 // sign bit of each pivot is counted with integer arithmetic.
+// The diagonal and e^2 are read from LDS 8 entries at a time, so the LDS latency is paid once
+// per 8 steps of the pivot recurrence.
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int s, int t, double x, double rmax) {
   double q = d[s] - x;
   unsigned c = (unsigned)((unsigned long long)__double_as_longlong(q) >> 63);
-  for (int j = s + 1; j < t; ++j) {
+  auto step = [&](double dj, double ej) {
     const double r = fmin(fmax(rcp_nr(q), -rmax), rmax);
-    q = fma(-e2[j - 1], r, d[j] - x);
+    q = fma(-ej, r, dj - x);
     c += (unsigned)((unsigned long long)__double_as_longlong(q) >> 63);
+  };
+  int j = s + 1;
+  for (; j + 8 <= t; j += 8) {
+    double dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[j + u];
+      ev[u] = e2[j + u - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) step(dv[u], ev[u]);
   }
+  for (; j < t; ++j) step(d[j], e2[j - 1]);
   return (int)c;
 }
 

@@ -30,7 +30,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 // Shared state of the solver (static LDS besides the two n x S matrices).
 struct Smem {
   double d[MAXN], e[MAXN], e2[MAXN], lam[MAXN], lo[MAXN], rq[MAXN];
-  double sg;
+  double sg, alpha, iv, v0;  // reflector scalars of the current Householder step
   int bs[MAXN], bt[MAXN], loc[MAXN], perm[MAXN];
   unsigned char cnt[2][1024];
   double norm;
@@ -174,18 +174,25 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
       acc = group8_sum(acc);
       if (sub == 0) sh.lo[grp] = acc;
     } else if (grp == (nt >> 3) - 1) {
+      // the last group also forms the reflector's scalars (square roots and a division done once
+      // instead of in every wave); the other threads read them after the barrier
       double acc = 0.0;
       for (int i = 1 + sub; i < m; i += 8) acc = fma(colk[i * S], colk[i * S], acc);
       acc = group8_sum(acc);
-      if (sub == 0) sh.sg = acc;
+      if (sub == 0) {
+        const double x0 = colk[0];
+        const double nx = sqrt(x0 * x0 + acc);
+        const double alpha = x0 >= 0.0 ? -nx : nx;
+        const double iv = !(acc > 0.0) ? 0.0 : 1.0 / sqrt(acc + (x0 - alpha) * (x0 - alpha));
+        sh.sg = acc;
+        sh.alpha = alpha;
+        sh.iv = iv;
+        sh.v0 = (x0 - alpha) * iv;
+      }
     }
     __syncthreads();
-    const double sg = sh.sg, x0 = colk[0];
+    const double sg = sh.sg, x0 = colk[0], alpha = sh.alpha, iv = sh.iv, v0 = sh.v0;
     const bool skip = !(sg > 0.0);
-    const double nx = sqrt(x0 * x0 + sg);
-    const double alpha = x0 >= 0.0 ? -nx : nx;
-    const double iv = skip ? 0.0 : 1.0 / sqrt(sg + (x0 - alpha) * (x0 - alpha));
-    const double v0 = (x0 - alpha) * iv;
     auto vget = [&](int i) -> double { return i == 0 ? v0 : colk[i * S] * iv; };
     // p_i = A22[i][0] v0 + iv u_i
     auto pget = [&](int i) -> double { return fma(A[(size_t)(k + 1 + i) * S + k + 1], v0, iv * sh.lo[i]); };

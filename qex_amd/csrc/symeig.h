@@ -21,10 +21,27 @@ namespace qexeig {
 constexpr double EPS = 2.220446049250313e-16;
 constexpr int MAXN = 96;
 
+// one DPP move of a double (both halves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
+// Sum over the 64 lanes (all active), the same value in every lane: DPP within each row of 16
+// lanes (pairs, quads, half-row mirror, row mirror), then the four row sums in a fixed order.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  x += dpp_f64<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
 }
 
 // Shared state of the solver (static LDS besides the two n x S matrices).
@@ -112,13 +129,6 @@ __device__ __forceinline__ void ldl_solve(const double* e, int s, int t, const d
   }
 }
 
-// one DPP move of a double (both halves)
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
 // sum over the 8 lanes of an aligned lane group by DPP moves (swap within pairs, within quads,
 // then the half-row mirror pairs each quad with the other one); every lane receives the same value
 __device__ __forceinline__ double group8_sum(double x) {

@@ -971,8 +971,8 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
   }
   int nrounds = 0;
   // Multisection: NS shifts per round (both ends included) until the bracket of the j-th
-  // eigenvalue is 1e-7 relative; two inverse-iteration solves (each scales the other components
-  // by |lambda_j - sigma| / gap) and the Rayleigh quotient then give the eigenpair to ~1e-13.  For N >= 4 each shift is counted by a pair of lanes, the even
+  // eigenvalue is 1e-9 relative; two inverse-iteration solves and the Rayleigh quotient then
+  // give the eigenpair to ~1e-13.  For N >= 4 each shift is counted by a pair of lanes, the even
   // lane eliminating the top rows 0..kt-1 and the odd lane the bottom rows N-1..kt+2 (half the
   // recurrence each), joined through the 2x2 block of rows kt, kt+1.
   const bool two = N >= 4;
@@ -980,7 +980,7 @@ __global__ __launch_bounds__(NT) void k_eigen(EigArgs e) {
   const int NS = two ? NT / 2 : NT, me = two ? tid >> 1 : tid;
   for (int round = 0; round < 80; ++round) {
     const double mid = 0.5 * (lo + hi);
-    const double tol = 1e-7 * fmax(1.0, fabs(mid)) + 64.0 * EPS * hnorm;
+    const double tol = 1e-9 * fmax(1.0, fabs(mid)) + 64.0 * EPS * hnorm;
     if (hi - lo <= tol) break;
     ++nrounds;
     const double step = (hi - lo) / (double)(NS - 1);

@@ -80,6 +80,23 @@ def test_noninteracting_many_electrons(ne):
     assert abs(float(r.total_eigen_energies) - ee) < 1e-10 and abs(float(r.gap) - gap) < 1e-10
 
 
+@pytest.mark.parametrize("n_grid", [64, 65, 200, 201])
+def test_noninteracting_even_and_odd_grids(n_grid):
+    """The device eigen kernel splits the grid into a top and a bottom half that meet at a middle
+    pair of rows (halves of equal length for even N, the bottom one row longer for odd N)."""
+    from qex_amd.od import engine
+
+    grids = np.linspace(-10, 10, n_grid)
+    loc, Z = np.array([-1.5, 1.2]), np.array([2, 1])
+    v = O.get_atomic_chain_potential(grids, loc, Z, O.exponential_coulomb)
+    ne = 3
+    r = engine.solve_noninteracting(v, ne, grids, with_states=True)
+    dens, ee, gap, eigs, _ = O.solve_noninteracting_system(v, ne, grids, return_eigs=True)
+    assert _l2(_np(r.density), dens, O.get_dx(grids)) < 1e-9
+    np.testing.assert_allclose(_np(r.eigenvalues)[: ne // 2 + 1], eigs[: ne // 2 + 1], rtol=0, atol=1e-10)
+    assert abs(float(r.gap) - gap) < 1e-10
+
+
 @pytest.mark.parametrize("interaction", ["exp", "soft"])
 def test_hartree_potential(interaction):
     from qex_amd.od import scf, utils

@@ -183,6 +183,12 @@ const char* qex1d_vjp_last_error(void);
 #define QEX3D_RKS_LDA 0 /* Slater exchange + VWN5 correlation */
 #define QEX3D_RHF 1
 #define QEX3D_RKS_NN_GLOBAL 2 /* amplitude-encoded network on the whole grid density */
+#define QEX3D_RKS_LDA_X 3     /* Slater exchange alone: pyscf's xc code "lda" (mf.xc = "lda" in
+                                 qedft/train/td/trainer_legacy_no_jit.py:635 and
+                                 qedft/data_io/td/dataset_generation.py:392) */
+/* LDA functionals of qex3d_nr_rks_lda */
+#define QEX3D_LDA_SVWN5 0  /* Slater exchange + VWN5 correlation (pyscf "lda,vwn") */
+#define QEX3D_LDA_SLATER 1 /* Slater exchange alone (pyscf "lda") */
 #define QEX3D_MAX_NN_HIDDEN 1024
 
 /* Basis on the device.  Shells are listed in AO order (an s shell is 1 AO, a p shell 3: x, y, z);
@@ -231,6 +237,12 @@ int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* d
 size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao);
 int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
                  double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes, void* stream);
+/* The same for a chosen LDA functional (QEX3D_LDA_*): libxc's eval_xc as nr_rks calls it for the
+ * xc_code (qedft/train/td/numint.py:42-64, eval_xc :313-336).  qex3d_nr_rks is functional
+ * QEX3D_LDA_SVWN5.  Same workspace as qex3d_nr_rks. */
+int qex3d_nr_rks_lda(int32_t functional, int64_t ngrid, int32_t nao, const double* ao, const double* weights,
+                     const double* dm, double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes,
+                     void* stream);
 
 /* Global (amplitude-encoded) neural XC: x = rho[ngrid] -> Dense(h1) -> gelu -> Dense(h2) -> gelu ->
  * Dense(1) -> -scale * swish (gelu in its tanh form); excsum = the output, vrho = d excsum / d rho.

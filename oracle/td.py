@@ -389,14 +389,20 @@ def eval_rho(ao, dm):
 VWN5 = dict(A=0.0310907, x0=-0.10498, b=3.72744, c=12.9352)
 
 
-def lda_exc_vxc(rho):
-    """Energy per electron and potential d(rho e)/d rho; zero where rho <= 1e-30."""
+def lda_exc_vxc(rho, functional="svwn5"):
+    """Energy per electron and potential d(rho e)/d rho; zero where rho <= 1e-30.
+    functional "svwn5": Slater exchange + VWN5 correlation (pyscf "lda,vwn"); "slater": Slater
+    exchange alone (pyscf's bare "lda")."""
     rho = np.asarray(rho, dtype=np.float64)
     ok = rho > 1e-30
     r = np.where(ok, rho, 1.0)
     cx = -0.75 * (3.0 / np.pi) ** (1.0 / 3.0)
     ex = cx * r ** (1.0 / 3.0)
     vx = 4.0 / 3.0 * ex
+    if functional == "slater":
+        return np.where(ok, ex, 0.0), np.where(ok, vx, 0.0)
+    if functional != "svwn5":
+        raise ValueError(f"unknown LDA functional {functional!r}")
     A, x0, b, c = VWN5["A"], VWN5["x0"], VWN5["b"], VWN5["c"]
     rs = (3.0 / (4.0 * np.pi * r)) ** (1.0 / 3.0)
     x = np.sqrt(rs)
@@ -413,10 +419,10 @@ def lda_exc_vxc(rho):
     return np.where(ok, ex + ec, 0.0), np.where(ok, vx + vc, 0.0)
 
 
-def nr_rks_lda(ao, weights, dm):
+def nr_rks_lda(ao, weights, dm, functional="svwn5"):
     """(nelec, excsum, vmat) of numint.nr_rks for LDA (td/numint.py:42-64, 264-265)."""
     rho = eval_rho(ao, dm)
-    exc, vrho = lda_exc_vxc(rho)
+    exc, vrho = lda_exc_vxc(rho, functional)
     nelec = float(np.dot(rho, weights))
     excsum = float(np.dot(rho * weights, exc))
     aow = ao * (0.5 * weights * vrho)[:, None]
@@ -550,10 +556,11 @@ class SCFResult:
     nelec_grid: float = float("nan")
 
 
-def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8, ints=None, net=None):
+def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8, ints=None, net=None,
+        functional="svwn5"):
     """pyscf-style SCF (hf_legacy.py:51-116): core-Hamiltonian guess; DIIS from cycle 1;
-    converged when |dE| < conv_tol and |g|/sqrt(size) < sqrt(conv_tol).  method: "rks" (LDA),
-    "rks_nn" (GlobalMLP3D `net` on the grid density) or "rhf"."""
+    converged when |dE| < conv_tol and |g|/sqrt(size) < sqrt(conv_tol).  method: "rks" (the LDA
+    `functional`, "svwn5" or "slater"), "rks_nn" (GlobalMLP3D `net` on the grid density) or "rhf"."""
     S, T, V = ints[:3] if ints is not None else one_electron(mol)
     g = ints[3] if ints is not None else eri(mol)
     H = T + V
@@ -570,7 +577,7 @@ def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8,
         if method == "rks_nn":
             nel, exc, vxc = nr_rks_nn_global(ao, w, D, net)
         else:
-            nel, exc, vxc = nr_rks_lda(ao, w, D)
+            nel, exc, vxc = nr_rks_lda(ao, w, D, functional)
         return J + vxc, 0.5 * np.sum(D * J) + exc, nel
 
     e, C = gen_eigh(H, S)

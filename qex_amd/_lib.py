@@ -86,6 +86,9 @@ class Qex1dOutputs(ctypes.Structure):
 QEX3D_RKS_LDA = 0
 QEX3D_RHF = 1
 QEX3D_RKS_NN_GLOBAL = 2
+QEX3D_RKS_LDA_X = 3
+QEX3D_LDA_SVWN5 = 0
+QEX3D_LDA_SLATER = 1
 QEX3D_MAX_NN_HIDDEN = 1024
 QEX3D_MAX_NAO = 96
 QEX3D_MAX_NPRIM = 8
@@ -145,6 +148,7 @@ SIGNATURES = {
     "qex3d_eval_rho": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P]),
     "qex3d_nr_rks_workspace_bytes": (_S, [ctypes.c_int64, _I]),
     "qex3d_nr_rks": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),
+    "qex3d_nr_rks_lda": (_I, [_I, ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),
     "qex3d_nr_rks_nn_workspace_bytes": (_S, [ctypes.c_int64, _I, _I, _I]),
     "qex3d_nr_rks_nn": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S, _P]),
     "qex3d_gen_eigh_workspace_bytes": (_S, [_I, _I]),

@@ -1309,13 +1309,13 @@ __global__ __launch_bounds__(NT) void k_density(Dev1 a, int k, double alpha_k) {
     if (a.eigs_out && tid < a.K)
       a.eigs_out[((size_t)s * T + k) * a.K + tid] = a.eigs_out[((size_t)s * T + k - 1) * a.K + tid];
     if (a.orb_out) {
-      const int kk = a.ne[s] / 2 + 1;
+      const int kk = min(a.ne[s] / 2 + 1, a.K);
       for (int t = tid; t < kk * N; t += NT)
         a.orb_out[((size_t)s * T + k) * a.K * N + t] = a.orb_out[((size_t)s * T + k - 1) * a.K * N + t];
     }
     return;
   }
-  const int ne = a.ne[s];
+  const int ne = max(1, min(a.ne[s], 2 * a.K - 1));  // contract: N_e <= max_electrons (clamped as a guard)
   const int nocc = (ne + 1) / 2;
   const int keig = ne / 2 + 1;
   double* hd = smem;  // Hamiltonian diagonal of this iteration
@@ -1448,7 +1448,9 @@ __global__ __launch_bounds__(NT) void k_nonint_density(int N, int K, double dx, 
   __shared__ double red[NWAVE * 8];
   __shared__ double lam[QEX1D_MAX_ELECTRONS / 2 + 1];
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int ne = nes[s], nocc = (ne + 1) / 2, keig = ne / 2 + 1;
+  // the caller guarantees 1 <= N_e <= max_electrons (K = max_electrons / 2 + 1); the clamp keeps a
+  // violated contract inside this system's K orbital slots
+  const int ne = max(1, min(nes[s], 2 * K - 1)), nocc = (ne + 1) / 2, keig = ne / 2 + 1;
   double* hd = smem;
   for (int t = tid; t < N; t += NT) hd[t] = Td + pot[(size_t)s * N + t];
   __syncthreads();

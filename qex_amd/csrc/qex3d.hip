@@ -31,6 +31,9 @@ constexpr int NT = 256;
 constexpr int NW = NT / 64;
 thread_local char g_err3[512] = "";
 
+// the SCF methods whose XC is a local density approximation evaluated in the density epilogue
+__host__ __device__ inline bool is_lda(int method) { return method == QEX3D_RKS_LDA || method == QEX3D_RKS_LDA_X; }
+
 int fail3(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -716,11 +719,12 @@ __device__ __forceinline__ void tile_store(const TileRegs<NTH>& r, int n, int np
 // T[:, 16w:16w+16] = Phi_tile D[:, 16w:16w+16] for the 64 points (4 independent accumulators, its
 // block of D held in registers as the MFMA B operand), then its partial rho = rowsum(Phi * T).
 constexpr int NT_RHO = 64 * (QEX3D_MAX_NAO / 16);
-__device__ void lda_svwn5(double rho, double* e, double* v);
+__device__ void lda_point(int fn, double rho, double* e, double* v);
 // With wv given: the LDA at the 64 points in the epilogue, wv = w v / 2 and ex = rho w e per point
 // (the Vxc kernel then only reads them).
 __device__ __forceinline__ void rho_body(long bid, long ng, int n, const double* ao, const double* dm, double* rho,
-                                         const double* w = nullptr, double* wv_out = nullptr, double* ex_out = nullptr) {
+                                         const double* w = nullptr, double* wv_out = nullptr, double* ex_out = nullptr,
+                                         int fn = QEX3D_LDA_SVWN5) {
   extern __shared__ double sm[];
   const int npad = (n + 15) & ~15, S = npad + 1, nb = npad / 16, ks = npad / 4;
   double* ph = sm;
@@ -772,7 +776,7 @@ __device__ __forceinline__ void rho_body(long bid, long ng, int n, const double*
     rho[g] = x;
     if (wv_out) {
       double e, v;
-      lda_svwn5(x, &e, &v);
+      lda_point(fn, x, &e, &v);
       const double ww = w[g];
       wv_out[g] = 0.5 * ww * v;
       ex_out[g] = x * ww * e;
@@ -781,8 +785,8 @@ __device__ __forceinline__ void rho_body(long bid, long ng, int n, const double*
 }
 
 __global__ __launch_bounds__(NT_RHO) void k_rho(long ng, int n, const double* ao, const double* dm, double* rho,
-                                                const double* w, double* wv, double* ex) {
-  rho_body(blockIdx.x, ng, n, ao, dm, rho, w, wv, ex);
+                                                const double* w, double* wv, double* ex, int fn) {
+  rho_body(blockIdx.x, ng, n, ao, dm, rho, w, wv, ex, fn);
 }
 
 // rho from the occupied orbitals (inside the SCF D = 2 C_occ C_occ^T, so rho_g = 2 |phi_g^T C_occ|^2):
@@ -810,7 +814,7 @@ __device__ __forceinline__ void rho_occ_mfma(const double* ph, int S, const doub
 }
 
 __device__ __forceinline__ void rho_occ_body(long bid, long ng, int n, int nocc, const double* ao, const double* C,
-                                             double* rho, const double* w, double* wv_out, double* ex_out) {
+                                             double* rho, const double* w, double* wv_out, double* ex_out, int fn) {
   extern __shared__ double sm[];
   const int npad = (n + 15) & ~15, S = npad + 1;
   double* ph = sm;
@@ -848,7 +852,7 @@ __device__ __forceinline__ void rho_occ_body(long bid, long ng, int n, int nocc,
     rho[g] = x;
     if (wv_out) {
       double e, v;
-      lda_svwn5(x, &e, &v);
+      lda_point(fn, x, &e, &v);
       const double ww = w[g];
       wv_out[g] = 0.5 * ww * v;
       ex_out[g] = x * ww * e;
@@ -856,8 +860,9 @@ __device__ __forceinline__ void rho_occ_body(long bid, long ng, int n, int nocc,
   }
 }
 
-// LDA: Slater exchange + VWN5 correlation (spin unpolarised); e per electron and v = d(rho e)/d rho
-__device__ void lda_svwn5(double rho, double* e, double* v) {
+// LDA, spin unpolarised; e per electron and v = d(rho e)/d rho.  fn = QEX3D_LDA_SVWN5: Slater
+// exchange + VWN5 correlation; QEX3D_LDA_SLATER: Slater exchange alone (pyscf's xc "lda").
+__device__ void lda_point(int fn, double rho, double* e, double* v) {
   if (!(rho > 1e-30)) {
     *e = 0.0;
     *v = 0.0;
@@ -865,6 +870,11 @@ __device__ void lda_svwn5(double rho, double* e, double* v) {
   }
   const double cx = -0.75 * cbrt(3.0 / M_PI);
   const double ex = cx * cbrt(rho), vx = 4.0 / 3.0 * ex;
+  if (fn == QEX3D_LDA_SLATER) {
+    *e = ex;
+    *v = vx;
+    return;
+  }
   const double A = 0.0310907, x0 = -0.10498, b = 3.72744, c = 12.9352;
   const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
   const double x = sqrt(rs);
@@ -966,9 +976,9 @@ size_t vxc_lds(int n) { return ((size_t)CH * (((n + 15) & ~15) + 1) + CH) * 8; }
 constexpr int VXC_BLOCKS = 256;
 
 void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rho, hipStream_t st,
-                  const double* w = nullptr, double* wv = nullptr, double* ex = nullptr) {
+                  const double* w = nullptr, double* wv = nullptr, double* ex = nullptr, int fn = QEX3D_LDA_SVWN5) {
   hipLaunchKernelGGL(k_rho, dim3((unsigned)((ng + 63) / 64)), dim3(NT_RHO), rho_lds(n), st, ng, n, ao, dm, rho, w, wv,
-                     ex);
+                     ex, fn);
 }
 
 __host__ __device__ inline int vxc_blocks(long ng) {
@@ -1519,7 +1529,8 @@ __global__ __launch_bounds__(NT_SCF) void k_scf_energy(const Scf3* __restrict__ 
   }
   e1 = bsum(e1, red);
   e2 = bsum(e2, red);
-  const double etot = e1 + e2 + (s.method == 0 ? s.scal[3] : s.method == 2 ? s.scal[5] : 0.0) + s.enuc;
+  const double etot = e1 + e2 + (is_lda(s.method) ? s.scal[3] : s.method == QEX3D_RKS_NN_GLOBAL ? s.scal[5] : 0.0) +
+                      s.enuc;
   if (cycle < 0) {
     if (threadIdx.x == 0) {
       s.scal[0] = etot;
@@ -1632,16 +1643,17 @@ __global__ void k_put_scf(ScfPut pk, int count, Scf3* dst) {
 __global__ __launch_bounds__(NT) void k_rho_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (long)blockIdx.x >= (s.ng + 63) / 64) return;
-  if (s.method == QEX3D_RKS_LDA)
-    rho_occ_body(blockIdx.x, s.ng, s.n, s.nocc, s.ao, s.C, s.rho, s.w, s.wv, s.ex);
+  if (is_lda(s.method))
+    rho_occ_body(blockIdx.x, s.ng, s.n, s.nocc, s.ao, s.C, s.rho, s.w, s.wv, s.ex,
+                 s.method == QEX3D_RKS_LDA_X ? QEX3D_LDA_SLATER : QEX3D_LDA_SVWN5);
   else
-    rho_occ_body(blockIdx.x, s.ng, s.n, s.nocc, s.ao, s.C, s.rho, nullptr, nullptr, nullptr);
+    rho_occ_body(blockIdx.x, s.ng, s.n, s.nocc, s.ao, s.C, s.rho, nullptr, nullptr, nullptr, QEX3D_LDA_SVWN5);
 }
 
 __global__ __launch_bounds__(NT_VXC) void k_vxc_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
-  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.wv, s.method == QEX3D_RKS_LDA ? s.ex : nullptr,
+  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.wv, is_lda(s.method) ? s.ex : nullptr,
            s.vpart, s.spart);
 }
 
@@ -1952,6 +1964,14 @@ size_t qex3d_nr_rks_workspace_bytes(int64_t ngrid, int32_t nao) {
 
 int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
                  double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes, void* stream) {
+  return qex3d_nr_rks_lda(QEX3D_LDA_SVWN5, ngrid, nao, ao, weights, dm, nelec_excsum, vmat, workspace, workspace_bytes,
+                          stream);
+}
+
+int qex3d_nr_rks_lda(int32_t functional, int64_t ngrid, int32_t nao, const double* ao, const double* weights,
+                     const double* dm, double* nelec_excsum, double* vmat, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  if (functional != QEX3D_LDA_SVWN5 && functional != QEX3D_LDA_SLATER) return fail3(QEX_E_BADARG, "bad LDA functional");
   if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !weights || !dm || !nelec_excsum || !vmat)
     return fail3(QEX_E_BADARG, "bad nr_rks arguments");
   const size_t need = qex3d_nr_rks_workspace_bytes(ngrid, nao);
@@ -1965,7 +1985,7 @@ int qex3d_nr_rks(int64_t ngrid, int32_t nao, const double* ao, const double* wei
   double* ex = (double*)((char*)wv + al((size_t)ngrid * 8));
   {
     qexprof::Scope ps(qexprof::K_RHO, st);
-    rho_dispatch(ngrid, nao, ao, dm, rho, st, weights, wv, ex);
+    rho_dispatch(ngrid, nao, ao, dm, rho, st, weights, wv, ex, functional);
   }
   {
     qexprof::Scope ps(qexprof::K_VXC, st);
@@ -2092,7 +2112,7 @@ static int scf_check(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, con
   if (n < 1 || n > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "%snao must be in [1, %d]", pre, QEX3D_MAX_NAO);
   if (p->nelectron < 2 || (p->nelectron & 1) || p->nelectron / 2 > n)
     return fail3(QEX_E_BADARG, "%sclosed-shell nelectron required (got %d)", pre, p->nelectron);
-  if (p->method != QEX3D_RKS_LDA && p->method != QEX3D_RHF && p->method != QEX3D_RKS_NN_GLOBAL)
+  if (!is_lda(p->method) && p->method != QEX3D_RHF && p->method != QEX3D_RKS_NN_GLOBAL)
     return fail3(QEX_E_BADARG, "%sbad method", pre);
   if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->ao || !in->weights))
     return fail3(QEX_E_BADARG, "%sRKS needs a grid", pre);

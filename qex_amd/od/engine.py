@@ -113,16 +113,22 @@ class NonInteracting:
 
 def solve_noninteracting(potential, num_electrons, grids, with_states=False, max_electrons=None) -> NonInteracting:
     """Lowest eigenpairs of T + diag(v) for a batch [B, N] of potentials."""
+    shape = tuple(potential.shape) if hasattr(potential, "shape") else np.shape(potential)
+    ne = np.broadcast_to(np.asarray(num_electrons, dtype=np.int64).reshape(-1),
+                         (1 if len(shape) == 1 else shape[0],)).copy()
+    if np.any(ne < 1):
+        raise ValueError("num_electrons must be >= 1")
+    me = int(max_electrons or ne.max())
+    if ne.max() > me:
+        raise ValueError(f"num_electrons {int(ne.max())} exceeds max_electrons {me}")
+    if me > _lib.QEX1D_MAX_ELECTRONS:
+        raise ValueError(f"num_electrons must be <= {_lib.QEX1D_MAX_ELECTRONS}")
     dev = device()
     v = as_f64(potential, dev)
     single = v.dim() == 1
     if single:
         v = v[None]
     B, N = v.shape
-    ne = np.broadcast_to(np.asarray(num_electrons, dtype=np.int64).reshape(-1), (B,)).copy()
-    if np.any(ne < 1):
-        raise ValueError("num_electrons must be >= 1")
-    me = int(max_electrons or ne.max())
     K = me // 2 + 1
     ne_t = as_i32(ne, dev)
     lib = _lib.load()

@@ -36,6 +36,26 @@ def eval_rho(mol, ao, dm, non0tab=None, xctype="LDA", hermi=0, verbose=None):
     return rho
 
 
+# pyscf xc codes of the local density approximations the device evaluates (libxc names): "lda"
+# alone is Slater exchange (pyscf parses a bare exchange name as exchange only); "lda,vwn" and its
+# aliases add VWN5 correlation.  pyscf's RKS default is "LDA,VWN" (td/rks.py:172).
+_LDA_CODES = {
+    "lda": _lib.QEX3D_LDA_SLATER, "lda,": _lib.QEX3D_LDA_SLATER, "slater": _lib.QEX3D_LDA_SLATER,
+    "lda_x": _lib.QEX3D_LDA_SLATER, "lda,vwn": _lib.QEX3D_LDA_SVWN5, "lda,vwn5": _lib.QEX3D_LDA_SVWN5,
+    "svwn": _lib.QEX3D_LDA_SVWN5, "svwn5": _lib.QEX3D_LDA_SVWN5, "slater,vwn": _lib.QEX3D_LDA_SVWN5,
+    "slater,vwn5": _lib.QEX3D_LDA_SVWN5, "lda_x,lda_c_vwn": _lib.QEX3D_LDA_SVWN5,
+}
+
+
+def lda_functional(xc_code) -> int:
+    """The device LDA functional (``QEX3D_LDA_*``) for a pyscf xc code; raises for any other code."""
+    key = str(xc_code).lower().replace(" ", "")
+    if key not in _LDA_CODES:
+        raise NotImplementedError(f"xc_code {xc_code!r}: the device path implements {sorted(_LDA_CODES)} "
+                                  "and network XC installed with define_xc_")
+    return _LDA_CODES[key]
+
+
 class NumInt:
     """The reference's ``ni`` dispatch handle; ``define_xc_`` installs a network XC
     (``trainer_legacy_no_jit.py:256-261``)."""
@@ -52,14 +72,14 @@ class NumInt:
 
 def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000, verbose=None, params=None,
            ao=None):
-    """(nelec, excsum, vmat) on the device: LDA (Slater + VWN5), or the network installed with
+    """(nelec, excsum, vmat) on the device: the LDA named by ``xc_code`` (Slater exchange for "lda",
+    Slater + VWN5 for "lda,vwn" and its aliases), or the network installed with
     ``ni.define_xc_`` (NN-AmplitudeEncoding branch, td/numint.py:206-231), whose parameters may be
     replaced by ``params``."""
     net = getattr(ni, "network", None)
     if net is not None:
         return _nr_rks_nn(net if params is None else net.with_params(params), mol, grids, dms, hermi, ao)
-    if str(xc_code).lower() not in ("lda", "lda,vwn", "svwn", "lda,vwn5"):
-        raise NotImplementedError(f"xc_code {xc_code!r}: the device path implements LDA (Slater + VWN5)")
+    functional = lda_functional(xc_code)
     dev = engine.device()
     if ao is None:
         ao = eval_ao(mol, grids.coords)
@@ -71,9 +91,10 @@ def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000,
     ws = engine.workspace(lib.qex3d_nr_rks_workspace_bytes(ng, nao), dev)
     out2 = torch.empty(2, dtype=torch.float64, device=dev)
     vmat = torch.empty((nao, nao), dtype=torch.float64, device=dev)
-    _lib.check3(lib.qex3d_nr_rks(ng, nao, engine._ptr(ao), engine._ptr(engine.as_f64(grids.weights, dev)),
-                                 engine._ptr(dm), engine._ptr(out2), engine._ptr(vmat), engine._ptr(ws), ws.numel(),
-                                 engine._stream()), "qex3d_nr_rks")
+    _lib.check3(lib.qex3d_nr_rks_lda(functional, ng, nao, engine._ptr(ao),
+                                     engine._ptr(engine.as_f64(grids.weights, dev)), engine._ptr(dm),
+                                     engine._ptr(out2), engine._ptr(vmat), engine._ptr(ws), ws.numel(),
+                                     engine._stream()), "qex3d_nr_rks_lda")
     return out2[0], out2[1], vmat
 
 

@@ -12,7 +12,7 @@ import torch
 from .. import _lib
 from ..od import engine
 from .dft import Grids
-from .numint import eval_ao
+from .numint import eval_ao, lda_functional
 
 
 class _SCF:
@@ -64,7 +64,7 @@ class _SCF:
         ng = 0
         method = self.method
         net = getattr(self, "network", None)
-        if method == _lib.QEX3D_RKS_LDA and net is not None:
+        if method in (_lib.QEX3D_RKS_LDA, _lib.QEX3D_RKS_LDA_X) and net is not None:
             method = _lib.QEX3D_RKS_NN_GLOBAL
             if kwargs.get("params") is not None:
                 net = net.with_params(kwargs["params"])
@@ -118,14 +118,24 @@ class RHF(_SCF):
 
 
 class RKS(_SCF):
-    method = _lib.QEX3D_RKS_LDA
+    """Restricted Kohn-Sham.  ``xc`` is a pyscf xc code of an LDA (default "LDA,VWN", pyscf's and the
+    reference's default, ``td/rks.py:172``; "lda" alone is Slater exchange), or a network installed
+    with ``define_xc_``."""
 
-    def __init__(self, mol, xc="lda"):
+    def __init__(self, mol, xc="LDA,VWN"):
         super().__init__(mol)
-        if str(xc).lower() not in ("lda", "lda,vwn", "svwn", "lda,vwn5"):
-            raise NotImplementedError("the device RKS path implements LDA (Slater + VWN5) or a network XC")
-        self.xc = xc
         self.network = None
+        self.xc = xc
+
+    @property
+    def xc(self):
+        return self._xc
+
+    @xc.setter
+    def xc(self, code):
+        fn = lda_functional(code)
+        self._xc = code
+        self.method = _lib.QEX3D_RKS_LDA_X if fn == _lib.QEX3D_LDA_SLATER else _lib.QEX3D_RKS_LDA
 
     def define_xc_(self, network, xctype="NN-AmplitudeEncoding"):
         """Install a global network XC (``trainer_legacy_no_jit.py:256-261``); kernel(params=...)

@@ -46,3 +46,17 @@ def test_global_mlp_is_amplitude_encoded():
     assert scf._global_functional(f) is f
     with pytest.raises(ValueError, match="local"):
         scf._local_functional(f)
+
+
+def test_solve_noninteracting_rejects_electrons_beyond_max():
+    """The non-interacting solve sizes its orbital slots from max_electrons; a system with more
+    electrons than that (or beyond the library limit) is refused before any device work."""
+    from qex_amd.od import engine
+
+    v = np.zeros((2, 33))
+    with pytest.raises(ValueError, match="exceeds max_electrons"):
+        engine.solve_noninteracting(v, [2, 6], np.linspace(-4, 4, 33), max_electrons=4)
+    with pytest.raises(ValueError, match="<= 62"):
+        engine.solve_noninteracting(v, [2, 64], np.linspace(-4, 4, 33))
+    with pytest.raises(ValueError, match=">= 1"):
+        engine.solve_noninteracting(v, [0, 2], np.linspace(-4, 4, 33))

@@ -54,6 +54,11 @@ extern "C" {
 #define QEX1D_XC_GLOBAL_MLP 5   /* MLP on the whole density (global, amplitude encoded):
                                    classical_models.py:177-224 build_global_mlp */
 
+/* Transform applied to the network output of LOCAL_MLP / GLOBAL_MLP (the jax_dft neural_xc layer
+ * that qedft/models/wrappers.py:201-203 appends with wrap_with_negative_transform). */
+#define QEX1D_MLP_OUTPUT_LINEAR 0
+#define QEX1D_MLP_OUTPUT_NEGATIVITY 1 /* e -> -swish(e) = -e sigmoid(e), range (-inf, 0.278] */
+
 #define QEX_ACT_SILU 0
 #define QEX_ACT_TANH 1
 #define QEX_ACT_SOFTPLUS 2
@@ -83,6 +88,12 @@ typedef struct qex1d_problem {
   int32_t mlp_widths[QEX1D_MAX_MLP_LAYERS + 1];
   int32_t mlp_activation; /* QEX_ACT_* */
   double mlp_norm;        /* density_normalization_factor */
+  int32_t mlp_output;     /* QEX1D_MLP_OUTPUT_* */
+  /* Reverse pass only: the states of the first stop_gradient_iterations iterations are constants
+     (jax.lax.stop_gradient for idx <= stop_gradient_step in jax_dft jit_scf.kohn_sham, called with the
+     config's value at qedft/train/od/train.py:113: stop_gradient_iterations = stop_gradient_step + 1);
+     0 differentiates every iteration.  The forward ignores it. */
+  int32_t stop_gradient_iterations;
 } qex1d_problem;
 
 typedef struct qex1d_inputs {
@@ -119,6 +130,25 @@ size_t qex1d_workspace_bytes(const qex1d_problem* p);
  * od/jit_scf.py:247 (GLOBAL xc kinds). */
 int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out,
                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Continuous batching (the serving path): p->batch is a table of B slots, each holding one request
+ * that runs its own SCF iteration, so requests join and leave the running batch between iterations.
+ * One call is one step: every busy slot advances by one iteration (a request takes
+ * num_iterations + 1 steps: the last one only evaluates the final energies, then the slot's
+ * trajectory is finalised).  Arrays:
+ *  - slot_iteration [B] (device int32): the caller sets every entry to -1 once; the library keeps
+ *    it: 0..T while a request runs, -2 once its outputs [s][0..T-1] are final;
+ *  - admit [B] (device int32): 1 for each slot in which the caller has placed a new request since
+ *    the previous step (its rows of external_potential, initial_density, num_electrons and, with
+ *    reflection, reflect_window); only slots at -1 or -2 may be admitted; the caller clears it for
+ *    the next step.
+ * in/out/workspace are those of qex1d_kohn_sham for B systems (qex1d_workspace_bytes); the
+ * workspace carries the slots' state between steps.  A request's results equal those of
+ * qex1d_kohn_sham on it alone, bit for bit.  Replaces the reference's one-batch-per-call
+ * vmap(kohn_sham) (qedft/train/od/train.py:124-126) for a stream of requests; no counterpart
+ * in the reference beyond that. */
+int qex1d_slots_step(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, int32_t* slot_iteration,
+                     const int32_t* admit, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Workspace bytes for qex1d_solve_noninteracting. */
 size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, int32_t max_electrons);
@@ -161,7 +191,9 @@ int32_t qex_abi_version(void);
 
 
 /* Reverse pass of qex1d_kohn_sham for the scan loop with a LOCAL_MLP functional (1 or 2 hidden
- * layers, widths <= 64), no reflection and density_mse_converge_tolerance < 0: given the forward's
+ * layers, widths <= 64, either output transform), with or without reflection symmetry (the flip
+ * inside reflect_window), density_mse_converge_tolerance < 0 and the problem's stop_gradient_iterations:
+ * given the forward's
  * inputs, its outputs density / hartree_potential / xc_potential / orbitals, and cotangents of the
  * stored energies d_total_energy [B][T] and densities d_density [B][T][N] (either may be NULL), writes
  * d_xc_params [P] = the gradient with respect to the packed MLP parameters, summed over the batch.

@@ -321,6 +321,18 @@ def _act(name, z):
     raise ValueError(f"unknown activation {name}")
 
 
+def output_transform(kind, y):
+    """(g(y), g'(y)) of the network-output transform: "linear", or "negativity" = -swish(y), the
+    jax_dft neural_xc.negativity_transform that qedft/models/wrappers.py:201-203 appends
+    (range (-inf, 0.278])."""
+    if kind == "linear":
+        return y, np.ones_like(y)
+    if kind == "negativity":
+        s = 1 / (1 + np.exp(-y))
+        return -y * s, -(s * (1 + y * (1 - s)))
+    raise ValueError(f"unknown output transform {kind!r}")
+
+
 def _mlp_forward(weights, biases, activation, x):
     """x (P, D0) -> out (P, 1); returns activations for the backward pass."""
     cache = []
@@ -355,12 +367,13 @@ class LocalMLP(LocalXC):
     ``weights[l]`` has shape (fan_in, fan_out) (stax Dense convention: x @ W + b).
     """
 
-    def __init__(self, weights, biases, activation="silu", window=17, norm=2.0):
+    def __init__(self, weights, biases, activation="silu", window=17, norm=2.0, output="linear"):
         self.weights = [np.asarray(w, dtype=np.float64) for w in weights]
         self.biases = [np.asarray(b, dtype=np.float64) for b in biases]
         self.activation = activation
         self.window = window
         self.norm = norm
+        self.output = output
 
     def _windows(self, n):
         h = self.window // 2
@@ -370,14 +383,15 @@ class LocalMLP(LocalXC):
 
     def energy_density(self, n):
         out, _ = _mlp_forward(self.weights, self.biases, self.activation, self._windows(n))
-        return out[:, 0]
+        return output_transform(self.output, out[:, 0])[0]
 
     def potential(self, n):
         N = len(n)
         out, cache = _mlp_forward(self.weights, self.biases, self.activation, self._windows(n))
-        gx = _mlp_input_grad_full(self.weights, cache, n[:, None])  # (N, W): d(e.n)/d window input
+        e, de = output_transform(self.output, out[:, 0])
+        gx = _mlp_input_grad_full(self.weights, cache, (n * de)[:, None])  # (N, W): d(e.n)/d window input
         h = self.window // 2
-        v = out[:, 0].copy()
+        v = e.copy()
         for o in range(self.window):
             j = np.arange(N) + o - h  # point fed by window slot o of point i
             ok = (j >= 0) & (j < N)
@@ -388,19 +402,21 @@ class LocalMLP(LocalXC):
 class GlobalMLP(GlobalXC):
     """``build_global_mlp`` (classical_models.py:177-224): whole density / norm -> scalar."""
 
-    def __init__(self, weights, biases, activation="tanh", norm=2.0):
+    def __init__(self, weights, biases, activation="tanh", norm=2.0, output="linear"):
         self.weights = [np.asarray(w, dtype=np.float64) for w in weights]
         self.biases = [np.asarray(b, dtype=np.float64) for b in biases]
         self.activation = activation
         self.norm = norm
+        self.output = output
 
     def value(self, n):
         out, _ = _mlp_forward(self.weights, self.biases, self.activation, (n / self.norm)[None, :])
-        return float(out[0, 0])
+        return float(output_transform(self.output, out[0, 0])[0])
 
     def grad(self, n):
         out, cache = _mlp_forward(self.weights, self.biases, self.activation, (n / self.norm)[None, :])
-        return _mlp_input_grad_full(self.weights, cache, np.ones((1, 1)))[0] / self.norm
+        _, dg = output_transform(self.output, out[0, 0])
+        return _mlp_input_grad_full(self.weights, cache, np.full((1, 1), dg))[0] / self.norm
 
 
 class SymmetrizedLocalXC(LocalXC):

@@ -64,12 +64,22 @@ def act3(name, z):
     raise ValueError(f"unknown activation {name}")
 
 
-def mlp_second_order(weights, biases, act, X, Xd, fbar, fdbar):
+def output3(kind, y):
+    """(g, g', g'') of the network-output transform (``od.output_transform``)."""
+    if kind == "linear":
+        return y, np.ones_like(y), np.zeros_like(y)
+    if kind == "negativity":  # -swish
+        s = 1 / (1 + np.exp(-y))
+        return -y * s, -(s * (1 + y * (1 - s))), -(s * (1 - s) * (2 + y * (1 - 2 * s)))
+    raise ValueError(f"unknown output transform {kind!r}")
+
+
+def mlp_second_order(weights, biases, act, X, Xd, fbar, fdbar, output="linear"):
     """Reverse mode over the forward and forward-tangent passes of an MLP.
 
     X (P, D0) inputs, Xd (P, D0) input tangents; per point the scalar Phi = fbar f + fdbar fdot
-    (f = MLP(X), fdot = grad_x f . Xd).  Returns (f, fdot, Xbar (P, D0), dW list, db list) with
-    Xbar = dPhi/dX (fbar grad f + fdbar H_f Xd) and dW, db summed over points.
+    (f = g(MLP(X)) with the output transform g, fdot = grad_x f . Xd).  Returns (f, fdot, Xbar (P, D0),
+    dW list, db list) with Xbar = dPhi/dX (fbar grad f + fdbar H_f Xd) and dW, db summed over points.
     """
     L = len(weights)
     hs, hds, zs, d1s, d2s = [X], [Xd], [], [], []
@@ -84,8 +94,12 @@ def mlp_second_order(weights, biases, act, X, Xd, fbar, fdbar):
         h, hd = a, d1 * zd
         hs.append(h)
         hds.append(hd)
-    f = (h @ weights[-1] + biases[-1])[:, 0]
-    fd = (hd @ weights[-1])[:, 0]
+    y = (h @ weights[-1] + biases[-1])[:, 0]
+    yd = (hd @ weights[-1])[:, 0]
+    g0, g1, g2 = output3(output, y)
+    f, fd = g0, g1 * yd
+    # reverse through the transform: (f, fd) = (g(y), g'(y) yd)
+    fbar, fdbar = fbar * g1 + fdbar * g2 * yd, fdbar * g1
     dW = [np.zeros_like(w) for w in weights]
     db = [np.zeros_like(b) for b in biases]
     # last layer: f = h W + b, fd = hd W
@@ -126,14 +140,15 @@ class LocalMLPAdjoint:
         X = self.xc._windows(n)
         z0 = np.zeros_like(X)
         f, _, Xbar, dW, db = mlp_second_order(self.xc.weights, self.xc.biases, self.xc.activation, X, z0, n,
-                                              np.zeros(len(n)))
+                                              np.zeros(len(n)), getattr(self.xc, "output", "linear"))
         return float(f @ n), f + self._scatter(Xbar), list(zip(dW, db))
 
     def vxc_vjp(self, n, vbar):
         """(nbar, thetabar) of vbar . v_xc(n): H_F vbar and d(vbar . grad F)/dtheta."""
         X = self.xc._windows(n)
         Xd = self.xc._windows(vbar)
-        f, fd, Xbar, dW, db = mlp_second_order(self.xc.weights, self.xc.biases, self.xc.activation, X, Xd, vbar, n)
+        f, fd, Xbar, dW, db = mlp_second_order(self.xc.weights, self.xc.biases, self.xc.activation, X, Xd, vbar, n,
+                                               getattr(self.xc, "output", "linear"))
         return fd + self._scatter(Xbar), list(zip(dW, db))
 
 
@@ -161,9 +176,13 @@ def occupations(num_electrons):
 
 
 def forward_trajectory(locations, charges, num_electrons, num_iterations, grids, xc, interaction_fn,
-                       initial_density, alpha, alpha_decay, num_mixing_iterations):
-    """Scan-semantics forward (jit_kohn_sham without reflection and without convergence freezing)
-    keeping what the reverse pass needs."""
+                       initial_density, alpha, alpha_decay, num_mixing_iterations, reflect=False):
+    """Scan-semantics forward (jit_kohn_sham without convergence freezing) keeping what the reverse
+    pass needs.  reflect: enforce_reflection_symmetry about the grid centre (od/jit_scf.py:76-77),
+    S = flip-and-average: the XC functional is the wrapped e_S(n) = S e(S n), so F_S(n) = e_S(n).n =
+    F(S n) and v_xc = S grad F(S n); the energy is that of the raw output density n_raw (with the XC
+    part F(S n_raw)), and the density passed on is n_out = S n_raw (jax_dft flips after the energy)."""
+    S = O.flip_and_average_on_center if reflect else (lambda a: a)
     grids = np.asarray(grids, dtype=np.float64)
     dx = O.get_dx(grids)
     v_ext = O.get_atomic_chain_potential(grids, np.asarray(locations, dtype=np.float64), np.asarray(charges),
@@ -179,20 +198,22 @@ def forward_trajectory(locations, charges, num_electrons, num_iterations, grids,
     a = alpha
     for k in range(num_iterations):
         vh = O.get_hartree_potential(n, grids, interaction_fn)
-        _, vxc, _ = adj.energy_and_grads(n)
+        _, vxc, _ = adj.energy_and_grads(S(n))
+        vxc = S(vxc)
         vks = v_ext + vh + vxc
         eps, U = np.linalg.eigh(T + np.diag(vks))
         psi = U[:, :K].T  # (K, N), unit 2-norm
-        n_out = (occ[:, None] * psi ** 2).sum(0) / dx
+        n_raw = (occ[:, None] * psi ** 2).sum(0) / dx
+        n_out = S(n_raw)
         Fo, vxc_out, _ = adj.energy_and_grads(n_out)
-        E = (float(occ @ eps[:K]) - dx * vks @ n_out + O.get_hartree_energy(n_out, grids, interaction_fn)
-             + dx * Fo + dx * v_ext @ n_out)
+        E = (float(occ @ eps[:K]) - dx * vks @ n_raw + O.get_hartree_energy(n_raw, grids, interaction_fn)
+             + dx * Fo + dx * v_ext @ n_raw)
         diffs[k] = n_out - n
         mixed = n + a * (Wc[k] @ diffs)
-        rec.append(dict(n=n, vks=vks, eps=eps, U=U, n_out=n_out, E=E, alpha=a, mixed=mixed))
+        rec.append(dict(n=n, vks=vks, eps=eps, U=U, n_raw=n_raw, n_out=n_out, E=E, alpha=a, mixed=mixed))
         n = mixed
         a *= alpha_decay
-    return dict(rec=rec, v_ext=v_ext, Wc=Wc, dx=dx, occ=occ, K=K, adj=adj, grids=grids,
+    return dict(rec=rec, v_ext=v_ext, Wc=Wc, dx=dx, occ=occ, K=K, adj=adj, grids=grids, S=S,
                 interaction_fn=interaction_fn,
                 energies=np.array([r["E"] for r in rec]), densities=np.stack([r["mixed"] for r in rec]))
 
@@ -216,9 +237,13 @@ def eig_adjoint(eps, U, occ, nbar, dx):
     return vbar
 
 
-def backward(fw, dE, dn):
-    """Gradient of sum_k dE[k] E_k + sum_k dn[k] . density_k w.r.t. the flat MLP parameters."""
-    rec, dx, adj = fw["rec"], fw["dx"], fw["adj"]
+def backward(fw, dE, dn, stop_gradient_step=-1):
+    """Gradient of sum_k dE[k] E_k + sum_k dn[k] . density_k w.r.t. the flat MLP parameters.
+
+    stop_gradient_step: the states of iterations k <= it are constants (jax.lax.stop_gradient on the
+    scan state in jax_dft jit_scf.kohn_sham): neither the loss's cotangents on them nor those flowing
+    back from later iterations reach them; the differences they contribute to later mixes still do."""
+    rec, dx, adj, S = fw["rec"], fw["dx"], fw["adj"], fw["S"]
     grids, ifn, v_ext, Wc = fw["grids"], fw["interaction_fn"], fw["v_ext"], fw["Wc"]
     Tn = len(rec)
     N = len(grids)
@@ -227,21 +252,21 @@ def backward(fw, dE, dn):
     nbar_next = np.zeros(N)
     for k in range(Tn - 1, -1, -1):
         r = rec[k]
-        g = nbar_next + dn[k]
+        g = nbar_next + dn[k] if k > stop_gradient_step else np.zeros(N)
         nbar = g.copy()
         for j in range(k + 1):
             dbar[j] += r["alpha"] * Wc[k, j] * g
-        nout_bar = dbar[k].copy()
+        nraw_bar = S(dbar[k])  # cotangent of n_raw (n_out = S n_raw)
         nbar -= dbar[k]
-        if dE[k] != 0.0:
+        if dE[k] != 0.0 and k > stop_gradient_step:
             _, vxc_out, dth = adj.energy_and_grads(r["n_out"])
-            vh_out = O.get_hartree_potential(r["n_out"], grids, ifn)
-            nout_bar += dE[k] * dx * (-r["vks"] + vh_out + vxc_out + v_ext)
+            vh_out = O.get_hartree_potential(r["n_raw"], grids, ifn)
+            nraw_bar += dE[k] * dx * (-r["vks"] + vh_out + S(vxc_out) + v_ext)
             theta += dE[k] * dx * flatten(dth)
-        vbar = eig_adjoint(r["eps"], r["U"], fw["occ"], nout_bar, dx)
+        vbar = eig_adjoint(r["eps"], r["U"], fw["occ"], nraw_bar, dx)
         nbar += O.get_hartree_potential(vbar, grids, ifn)
-        nb2, dth2 = adj.vxc_vjp(r["n"], vbar)
-        nbar += nb2
+        nb2, dth2 = adj.vxc_vjp(S(r["n"]), S(vbar))
+        nbar += S(nb2)
         theta += flatten(dth2)
         nbar_next = nbar
     return theta

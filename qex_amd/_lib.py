@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqex.so")
 
 QEX_OK = 0
+QEX_E_BADARG = -1
+QEX_E_WORKSPACE = -2
 QEX1D_MAX_GRIDS = 1024
 QEX1D_MAX_ELECTRONS = 62
 QEX1D_MAX_MLP_LAYERS = 6
@@ -26,6 +28,10 @@ QEX1D_XC_POWER_GLOBAL = 2
 QEX1D_XC_LOCAL_MLP = 3
 QEX1D_XC_NONE = 4
 QEX1D_XC_GLOBAL_MLP = 5
+
+QEX1D_MLP_OUTPUT_LINEAR = 0
+QEX1D_MLP_OUTPUT_NEGATIVITY = 1
+OUTPUTS = {"linear": QEX1D_MLP_OUTPUT_LINEAR, "negativity": QEX1D_MLP_OUTPUT_NEGATIVITY}
 
 ACTIVATIONS = {"silu": 0, "swish": 0, "tanh": 1, "softplus": 2, "sigmoid": 3, "relu": 4, "gelu": 5}
 
@@ -54,6 +60,8 @@ class Qex1dProblem(ctypes.Structure):
         ("mlp_widths", ctypes.c_int32 * (QEX1D_MAX_MLP_LAYERS + 1)),
         ("mlp_activation", ctypes.c_int32),
         ("mlp_norm", ctypes.c_double),
+        ("mlp_output", ctypes.c_int32),
+        ("stop_gradient_iterations", ctypes.c_int32),
     ]
 
 
@@ -128,6 +136,8 @@ SIGNATURES = {
     "qex1d_workspace_bytes": (_S, [ctypes.POINTER(Qex1dProblem)]),
     "qex1d_kohn_sham": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
                              ctypes.POINTER(Qex1dOutputs), _P, _S, _P]),
+    "qex1d_slots_step": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
+                              ctypes.POINTER(Qex1dOutputs), _P, _P, _P, _S, _P]),
     "qex1d_noninteracting_workspace_bytes": (_S, [_I, _I, _I]),
     "qex1d_solve_noninteracting": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
     "qex1d_external_potential": (_I, [_I, _I, _P, _I, _P, _P, _I, _D, _D, _P, _P]),

@@ -214,7 +214,7 @@ __device__ __forceinline__ double act_fwd(int kind, double z, double* dz) {
 // Kernel arguments
 // --------------------------------------------------------------------------------------------
 struct Dev1 {
-  int B, N, T, K, M, W, nTiles, loop, reflect, xc, nlay, act;
+  int B, N, T, K, M, W, nTiles, loop, reflect, xc, nlay, act, mlpout;
   int widths[QEX1D_MAX_MLP_LAYERS + 1];
   int woff[QEX1D_MAX_MLP_LAYERS];  // offset of W_l in the packed parameter array
   int boff[QEX1D_MAX_MLP_LAYERS];  // offset of b_l
@@ -256,9 +256,29 @@ struct Dev1 {
   double* mlp_e;     // [B][N]
   double* mlp_G;     // [B][N][W]
   double* mlp_eout;  // [B][N]
+  // slot mode (continuous batching, qex1d_slots_step): every system (slot) runs its own iteration
+  // kslot[s] (0..T-1 SCF iterations, T: the pass that only evaluates the last energies; negative or
+  // > T: idle); null in batch mode, where every system runs the launch's iteration
+  int* kslot;
+  const int* admit;  // [B] 1: a new request was placed in the slot before this step
+  double alpha0, decay;
 };
 
+// iteration of system s in this launch: the launch's, or the slot's own in slot mode
+__device__ __forceinline__ int iter_of(const Dev1& a, int s, int k_launch) { return a.kslot ? a.kslot[s] : k_launch; }
+
 __device__ __forceinline__ bool xc_is_local(int xc) { return xc != QEX1D_XC_POWER_GLOBAL && xc != QEX1D_XC_GLOBAL_MLP; }
+
+// transform of the network output (QEX1D_MLP_OUTPUT_*) and its derivative
+__device__ __forceinline__ double out_transform(int kind, double y, double* d) {
+  if (kind == QEX1D_MLP_OUTPUT_NEGATIVITY) {
+    const double s = 1.0 / (1.0 + exp(-y));
+    *d = -(s * (1.0 + y * (1.0 - s)));
+    return -y * s;
+  }
+  *d = 1.0;
+  return y;
+}
 
 // pointwise XC at density x: energy density (local: e; global: f), potential contribution
 __device__ __forceinline__ void xc_point(const Dev1& a, double x, double* e, double* v) {
@@ -357,14 +377,15 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
   part += __shfl_xor(part, 1, 64);
   part += __shfl_xor(part, 2, 64);
   part += __shfl_xor(part, 4, 64);
-  const double e = part + a.mlp[a.boff[L - 1]];
+  double de;
+  const double e = out_transform(a.mlpout, part + a.mlp[a.boff[L - 1]], &de);
   if (q == 0 && valid) *e_out = e;
   if (!backward) {
     __syncthreads();
     return;
   }
-  // backward with cotangent m_i on e_i
-  const double m = valid ? dens_lds[i] : 0.0;
+  // backward with cotangent m_i on e_i (through the output transform)
+  const double m = valid ? dens_lds[i] * de : 0.0;
   // gradient w.r.t. last hidden activations
   double* g = nxt;  // free buffer
   for (int kk = q; kk < fi; kk += 8)
@@ -401,12 +422,13 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
   __syncthreads();
 }
 
-__global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k) {
+__global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k_launch) {
   extern __shared__ double smem[];
   const int s = blockIdx.y;
+  const int k = iter_of(a, s, k_launch);
   const int frozen = a.st[4 * s + 0];
   const int pending = a.st[4 * s + 1];
-  const bool doPot = (k < a.T) && !frozen;
+  const bool doPot = k >= 0 && k < a.T && !frozen;
   const bool doE = pending >= 0;
   if (!doPot && !doE) return;
   const int N = a.N, stride = a.wmax + 1;
@@ -549,7 +571,7 @@ __device__ __forceinline__ void mlp_out_grad(double* nxt, const double* cot, dou
 }
 
 template <int NH>
-__global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k) {
+__global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k_launch) {
   extern __shared__ double smem[];
   // one activation tile: each layer's GEMM keeps its result in registers until every wave has
   // read the tile, then writes it back in place (34 KB of LDS: three workgroups per CU)
@@ -567,7 +589,8 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
     const long g = g0 + tid;
     if (g < total) {
       const int s = (int)(g / N);
-      if (k < a.T && !a.st[4 * s + 0]) sh_any[0] = 1;
+      const int k = iter_of(a, s, k_launch);
+      if (k >= 0 && k < a.T && !a.st[4 * s + 0]) sh_any[0] = 1;
       if (a.st[4 * s + 1] >= 0) sh_any[1] = 1;
     }
   }
@@ -587,7 +610,8 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
       const long g = g0 + p;
       const bool inb = g < total;
       const int s = inb ? (int)(g / N) : 0, i = inb ? (int)(g % N) : 0;
-      const bool act = inb && (pass == 0 ? (k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0));
+      const int k = iter_of(a, s, k_launch);
+      const bool act = inb && (pass == 0 ? (k >= 0 && k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0));
       const double* src = pass == 0 ? input_density(a, s, k) : a.o_xc + (size_t)s * N;
       int lo = 0, hi = -1;
       if (pass == 0 && sym && act) {
@@ -629,12 +653,16 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k)
       part += __shfl_xor(part, 1, 64);
       part += __shfl_xor(part, 2, 64);
       const long g = g0 + p;
-      if (q == 0 && p < MP && g < total) {
-        const double e = part + a.mlp[a.boff[L - 1]];
-        if (pass == 0)
-          a.mlp_e[g] = e;
-        else
-          a.mlp_eout[g] = e;
+      if (q == 0 && p < MP) {
+        double de;
+        const double e = out_transform(a.mlpout, part + a.mlp[a.boff[L - 1]], &de);
+        if (g < total) {
+          if (pass == 0)
+            a.mlp_e[g] = e;
+          else
+            a.mlp_eout[g] = e;
+        }
+        if (pass == 0) cot[p] *= de;  // cotangent of the raw output (read after the next barrier)
       }
     }
     if (!backward) {
@@ -736,11 +764,12 @@ __device__ double gmlp_forward(const Dev1& a, const double* x, double* acts, dou
   return r[0] + a.mlp[a.boff[L]];
 }
 
-__global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k) {
+__global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k_launch) {
   extern __shared__ double smem[];
   __shared__ double red[NWAVE * 8];
   const int s = blockIdx.x, N = a.N, tid = threadIdx.x;
-  const bool doPot = k < a.T && a.st[4 * s + 0] == 0;
+  const int k = iter_of(a, s, k_launch);
+  const bool doPot = k >= 0 && k < a.T && a.st[4 * s + 0] == 0;
   const bool doE = a.st[4 * s + 1] >= 0;
   double* x = smem;                 // N
   double* acts = x + N;             // 6 x 128
@@ -752,12 +781,13 @@ __global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k) {
     const double* n = input_density(a, s, k);
     for (int t = tid; t < N; t += NT) x[t] = n[t] * in;
     __syncthreads();
-    const double f = gmlp_forward(a, x, acts, dzs, part, red);
-    // backward: d f / d acts of the last hidden layer = W_L, then through the hidden layers
+    double df;
+    const double f = out_transform(a.mlpout, gmlp_forward(a, x, acts, dzs, part, red), &df);
+    // backward: d f / d acts of the last hidden layer = df W_L, then through the hidden layers
     const int L = a.nlay - 1;
     double* cur = dl;
     double* nxt = dl + 128;
-    for (int v = tid; v < a.widths[L]; v += NT) cur[v] = a.mlp[a.woff[L] + v] * dzs[(L - 1) * 128 + v];
+    for (int v = tid; v < a.widths[L]; v += NT) cur[v] = df * a.mlp[a.woff[L] + v] * dzs[(L - 1) * 128 + v];
     __syncthreads();
     for (int l = L - 1; l >= 1; --l) {
       const int fi = a.widths[l], fo = a.widths[l + 1];
@@ -784,17 +814,19 @@ __global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k) {
   if (doE) {
     for (int t = tid; t < N; t += NT) x[t] = a.o_xc[(size_t)s * N + t] * in;
     __syncthreads();
-    const double f = gmlp_forward(a, x, acts, dzs, part, red);
+    double df;
+    const double f = out_transform(a.mlpout, gmlp_forward(a, x, acts, dzs, part, red), &df);
     if (tid == 0) a.mlp_eout[2 * s + 1] = f;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k) {
+__global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k_launch) {
   extern __shared__ double smem[];
   const int s = blockIdx.y, tile = blockIdx.x;
+  const int k = iter_of(a, s, k_launch);
   const int frozen = a.st[4 * s + 0];
   const int pending = a.st[4 * s + 1];
-  const bool doPot = (k < a.T) && !frozen;
+  const bool doPot = k >= 0 && k < a.T && !frozen;
   const bool doE = pending >= 0;
   if (!doPot && !doE) return;
   const int N = a.N;
@@ -910,14 +942,34 @@ struct EigArgs {
 };
 
 // Compacts the wanted (system, eigenpair) list so the eigen launch has no idle leading blocks.
-__global__ void k_pairs(int B, int K, const int* ne, int* pairs) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int n = 0;
-  for (int s = 0; s < B; ++s) {
-    const int keig = ne[s] / 2 + 1;
-    for (int j = 0; j < keig && j < K; ++j) pairs[n++] = (s << 8) | j;
+// One workgroup of 256 threads: per chunk of 256 systems an exclusive scan of the pair counts.  In
+// slot mode (kslot) only slots running an SCF iteration (0 <= kslot < T) take part.
+constexpr int PAIR_NT = 256;
+__global__ __launch_bounds__(PAIR_NT) void k_pairs(int B, int K, const int* ne, int* pairs, const int* kslot, int T) {
+  __shared__ int scan[PAIR_NT];
+  __shared__ int base;
+  const int tid = threadIdx.x;
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int s0 = 0; s0 < B; s0 += PAIR_NT) {
+    const int s = s0 + tid;
+    int c = 0;
+    if (s < B && (!kslot || (kslot[s] >= 0 && kslot[s] < T))) c = min(ne[s] / 2 + 1, K);
+    scan[tid] = c;
+    __syncthreads();
+    for (int off = 1; off < PAIR_NT; off <<= 1) {  // inclusive Hillis-Steele scan
+      const int v = tid >= off ? scan[tid - off] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    const int start = base + scan[tid] - c;
+    for (int j = 0; j < c; ++j) pairs[start + j] = (s << 8) | j;
+    __syncthreads();
+    if (tid == PAIR_NT - 1) base += scan[tid];
+    __syncthreads();
   }
-  for (; n < K * B; ++n) pairs[n] = -1;
+  for (int n = base + tid; n < K * B; n += PAIR_NT) pairs[n] = -1;
 }
 
 __device__ __forceinline__ double start_value(int i, int j) {
@@ -1284,11 +1336,18 @@ __device__ void orbitals_to_density(double* vecs, int N, int keig, int ne, doubl
   __syncthreads();
 }
 
-__global__ __launch_bounds__(NT) void k_density(Dev1 a, int k, double alpha_k) {
+__global__ __launch_bounds__(NT) void k_density(Dev1 a, int k_launch, double alpha_launch) {
   extern __shared__ double smem[];
   __shared__ double red[NWAVE * 8];
   __shared__ double lam[QEX1D_MAX_ELECTRONS / 2 + 1];
   const int s = blockIdx.x, tid = threadIdx.x, N = a.N, T = a.T;
+  const int k = iter_of(a, s, k_launch);
+  if (k < 0 || k >= T) return;  // idle slot, or a slot in its final energy-only pass
+  double alpha_k = alpha_launch;
+  if (a.kslot) {  // the same product the batch loop forms on the host, so slot and batch results agree bitwise
+    alpha_k = a.alpha0;
+    for (int i = 0; i < k; ++i) alpha_k *= a.decay;
+  }
   const bool frozen = a.st[4 * s + 0] != 0;
   const bool warm_before = a.st[4 * s + 2] != 0;
   const size_t cur = ((size_t)s * T + k) * N;
@@ -1419,8 +1478,8 @@ __global__ __launch_bounds__(NT) void k_density(Dev1 a, int k, double alpha_k) {
   }
 }
 
-__global__ void k_finalize(Dev1 a) {
-  const int s = blockIdx.x;
+// total energies of system s's T iterations from the per-tile partials (and the global XC energy)
+__device__ void finalize_system(const Dev1& a, int s) {
   for (int k = threadIdx.x; k < a.T; k += blockDim.x) {
     const size_t sk = (size_t)s * a.T + k;
     const int src = a.src[sk];
@@ -1434,6 +1493,36 @@ __global__ void k_finalize(Dev1 a) {
     const double* tm = a.terms + ss * 4;
     a.etot[sk] = (((tm[0] - tm[1]) + eh) + ex) + tm[2];
     if (!xc_is_local(a.xc)) a.exc[sk] = fs;
+  }
+}
+
+__global__ void k_finalize(Dev1 a) { finalize_system(a, blockIdx.x); }
+
+// slot mode: the slots a new request was placed in start at iteration 0 with a cold state
+__global__ void k_slot_begin(Dev1 a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.B || !a.admit[s]) return;
+  a.st[4 * s + 0] = 0;
+  a.st[4 * s + 1] = -1;
+  a.st[4 * s + 2] = 0;
+  a.st[4 * s + 3] = 0;
+  a.kslot[s] = 0;
+}
+
+// slot mode, end of a step: a slot that ran its energy-only pass (iteration T) is finalised and
+// marked done (-2, no pending energy); the others move on to their next iteration
+__global__ void k_slot_end(Dev1 a) {
+  const int s = blockIdx.x;
+  const int k = a.kslot[s];
+  if (k == a.T) {
+    finalize_system(a, s);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      a.st[4 * s + 1] = -1;
+      a.kslot[s] = -2;
+    }
+  } else if (k >= 0 && k < a.T && threadIdx.x == 0) {
+    a.kslot[s] = k + 1;
   }
 }
 
@@ -1571,6 +1660,9 @@ int validate1d(const qex1d_problem* p) {
   if (p->num_mixing_iterations < 1) return fail(QEX_E_BADARG, "num_mixing_iterations must be >= 1");
   if (p->loop != QEX1D_LOOP_PYTHON && p->loop != QEX1D_LOOP_SCAN) return fail(QEX_E_BADARG, "bad loop kind");
   if (p->xc_kind < 0 || p->xc_kind > QEX1D_XC_GLOBAL_MLP) return fail(QEX_E_BADARG, "bad xc_kind");
+  if ((p->xc_kind == QEX1D_XC_GLOBAL_MLP || p->xc_kind == QEX1D_XC_LOCAL_MLP) &&
+      p->mlp_output != QEX1D_MLP_OUTPUT_LINEAR && p->mlp_output != QEX1D_MLP_OUTPUT_NEGATIVITY)
+    return fail(QEX_E_BADARG, "bad mlp_output");
   if (p->xc_kind == QEX1D_XC_GLOBAL_MLP) {
     if (p->mlp_layers < 2 || p->mlp_layers > QEX1D_MAX_MLP_LAYERS)
       return fail(QEX_E_UNSUPPORTED, "GLOBAL_MLP needs 2..%d Dense layers", QEX1D_MAX_MLP_LAYERS);
@@ -1681,8 +1773,20 @@ size_t qex1d_workspace_bytes(const qex1d_problem* p) {
   return layout1d(p).total;
 }
 
-int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, void* workspace,
-                    size_t workspace_bytes, void* stream_) {
+}  // extern "C"
+
+namespace {
+// Everything one SCF iteration's launches need: the device view, the eigen arguments, launch shapes.
+struct Plan1d {
+  Dev1 a;
+  EigArgs e;
+  size_t lds_pot, lds_eig, lds_mlp;
+  bool mlp_mfma;
+  dim3 gmlp_mfma, gpot, geig, gmlp;
+};
+
+int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, void* workspace,
+           size_t workspace_bytes, Plan1d* P) {
   int rc = validate1d(p);
   if (rc) return rc;
   if (!in || !out) return fail(QEX_E_BADARG, "null inputs/outputs");
@@ -1697,9 +1801,9 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   const Layout L = layout1d(p);
   if (!workspace || workspace_bytes < L.total)
     return fail(QEX_E_WORKSPACE, "workspace too small: need %zu bytes, got %zu", L.total, workspace_bytes);
-  hipStream_t stream = (hipStream_t)stream_;
   char* ws = (char*)workspace;
-  Dev1 a{};
+  Dev1& a = P->a;
+  a = Dev1{};
   a.B = p->batch;
   a.N = p->num_grids;
   a.T = p->num_iterations;
@@ -1719,6 +1823,7 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
     a.nlay = p->mlp_layers;
     a.act = p->mlp_activation;
     a.norm = p->mlp_norm;
+    a.mlpout = p->mlp_output;
     a.W = p->mlp_widths[0];
     int off = 0;
     a.wmax = a.W;
@@ -1764,7 +1869,8 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   a.mlp_G = (double*)(ws + L.mlp_G);
   a.mlp_eout = (double*)(ws + L.mlp_eout);
 
-  EigArgs e{};
+  EigArgs& e = P->e;
+  e = EigArgs{};
   e.B = a.B;
   e.N = a.N;
   e.K = a.K;
@@ -1781,57 +1887,80 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
   e.pairs = (int*)(ws + L.pairs);
   e.stats = (unsigned long long*)out->stats;
 
-  const size_t lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
-  const size_t lds_eig = eigen_lds_bytes(a.N);
+  P->lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
+  P->lds_eig = eigen_lds_bytes(a.N);
   bool mlp_mfma = a.xc == QEX1D_XC_LOCAL_MLP && a.nlay - 1 <= MLP_LH && a.W <= MLP_S - 1;
   for (int l = 1; l < a.nlay && mlp_mfma; ++l) mlp_mfma = a.widths[l] <= MLP_HMAX;
-  const size_t lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
-  if (lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", lds_mlp);
-  const dim3 gmlp_mfma((unsigned)(((long)a.B * a.N + MP - 1) / MP));
-  const dim3 gpot(a.nTiles, a.B), geig(a.K * a.B), gmlp((a.N + MLP_P - 1) / MLP_P, a.B);
-
-  if (lds_eig > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eig);
+  P->mlp_mfma = mlp_mfma;
+  P->lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
+  if (P->lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", P->lds_mlp);
+  P->gmlp_mfma = dim3((unsigned)(((long)a.B * a.N + MP - 1) / MP));
+  P->gpot = dim3(a.nTiles, a.B);
+  P->geig = dim3(a.K * a.B);
+  P->gmlp = dim3((a.N + MLP_P - 1) / MLP_P, a.B);
+  if (P->lds_eig > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_eig);
   const void* mlp_fn = (const void*)k_mlp;
   if (mlp_mfma) {
     const void* fns[4] = {(const void*)k_mlp_mfma<1>, (const void*)k_mlp_mfma<2>, (const void*)k_mlp_mfma<3>,
                           (const void*)k_mlp_mfma<4>};
     mlp_fn = fns[a.nlay - 2];
   }
-  if (lds_mlp > 64 * 1024)
-    (void)hipFuncSetAttribute(mlp_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mlp);
+  if (P->lds_mlp > 64 * 1024)
+    (void)hipFuncSetAttribute(mlp_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_mlp);
+  return QEX_OK;
+}
+
+// One SCF iteration's launches (the XC network, potentials and deferred energies, then unless
+// energy_only the eigensolve and density/mixing); in slot mode every slot runs its own iteration.
+void launch_iteration(const Plan1d& P, int k, double alpha_k, bool energy_only, hipStream_t stream) {
+  const Dev1& a = P.a;
+  if (a.xc == QEX1D_XC_LOCAL_MLP) {
+    Scope ps(qexprof::K_MLP, stream);
+    if (P.mlp_mfma) {
+      switch (a.nlay - 1) {
+        case 1: hipLaunchKernelGGL(k_mlp_mfma<1>, P.gmlp_mfma, dim3(NT), P.lds_mlp, stream, a, k); break;
+        case 2: hipLaunchKernelGGL(k_mlp_mfma<2>, P.gmlp_mfma, dim3(NT), P.lds_mlp, stream, a, k); break;
+        case 3: hipLaunchKernelGGL(k_mlp_mfma<3>, P.gmlp_mfma, dim3(NT), P.lds_mlp, stream, a, k); break;
+        default: hipLaunchKernelGGL(k_mlp_mfma<4>, P.gmlp_mfma, dim3(NT), P.lds_mlp, stream, a, k); break;
+      }
+    } else
+      hipLaunchKernelGGL(k_mlp, P.gmlp, dim3(NT), P.lds_mlp, stream, a, k);
+  }
+  if (a.xc == QEX1D_XC_GLOBAL_MLP) {
+    Scope ps(qexprof::K_MLP, stream);
+    hipLaunchKernelGGL(k_gmlp, dim3(a.B), dim3(NT), (size_t)(a.N + 12 * 128 + NT + 256) * 8, stream, a, k);
+  }
+  {
+    Scope ps(qexprof::K_POTENTIAL, stream);
+    hipLaunchKernelGGL(k_potential, P.gpot, dim3(NT), P.lds_pot, stream, a, k);
+  }
+  if (energy_only) return;
+  {
+    Scope ps(qexprof::K_EIGEN, stream);
+    hipLaunchKernelGGL(k_eigen, P.geig, dim3(NT), P.lds_eig, stream, P.e);
+  }
+  {
+    Scope ps(qexprof::K_DENSITY, stream);
+    hipLaunchKernelGGL(k_density, dim3(a.B), dim3(NT), (size_t)a.N * 8, stream, a, k, alpha_k);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, void* workspace,
+                    size_t workspace_bytes, void* stream_) {
+  Plan1d P;
+  const int rc = plan1d(p, in, out, workspace, workspace_bytes, &P);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  const Dev1& a = P.a;
   hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
-  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(64), 0, stream, a.B, a.K, a.ne, e.pairs);
+  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(PAIR_NT), 0, stream, a.B, a.K, a.ne, P.e.pairs, nullptr, a.T);
   double alpha_k = p->alpha;
   for (int k = 0; k <= a.T; ++k) {
-    if (a.xc == QEX1D_XC_LOCAL_MLP) {
-      Scope ps(qexprof::K_MLP, stream);
-      if (mlp_mfma) {
-        switch (a.nlay - 1) {
-          case 1: hipLaunchKernelGGL(k_mlp_mfma<1>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
-          case 2: hipLaunchKernelGGL(k_mlp_mfma<2>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
-          case 3: hipLaunchKernelGGL(k_mlp_mfma<3>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
-          default: hipLaunchKernelGGL(k_mlp_mfma<4>, gmlp_mfma, dim3(NT), lds_mlp, stream, a, k); break;
-        }
-      } else
-        hipLaunchKernelGGL(k_mlp, gmlp, dim3(NT), lds_mlp, stream, a, k);
-    }
-    if (a.xc == QEX1D_XC_GLOBAL_MLP) {
-      Scope ps(qexprof::K_MLP, stream);
-      hipLaunchKernelGGL(k_gmlp, dim3(a.B), dim3(NT), (size_t)(a.N + 12 * 128 + NT + 256) * 8, stream, a, k);
-    }
-    {
-      Scope ps(qexprof::K_POTENTIAL, stream);
-      hipLaunchKernelGGL(k_potential, gpot, dim3(NT), lds_pot, stream, a, k);
-    }
-    if (k == a.T) break;  // the last pass only evaluates the final iteration's energies
-    {
-      Scope ps(qexprof::K_EIGEN, stream);
-      hipLaunchKernelGGL(k_eigen, geig, dim3(NT), lds_eig, stream, e);
-    }
-    {
-      Scope ps(qexprof::K_DENSITY, stream);
-      hipLaunchKernelGGL(k_density, dim3(a.B), dim3(NT), (size_t)a.N * 8, stream, a, k, alpha_k);
-    }
+    launch_iteration(P, k, alpha_k, k == a.T, stream);  // the last pass only evaluates the final energies
     alpha_k *= p->alpha_decay;
   }
   {
@@ -1839,6 +1968,28 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
     hipLaunchKernelGGL(k_finalize, dim3(a.B), dim3(64), 0, stream, a);
   }
   return check_launch("qex1d_kohn_sham");
+}
+
+int qex1d_slots_step(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, int32_t* slot_iteration,
+                     const int32_t* admit, void* workspace, size_t workspace_bytes, void* stream_) {
+  if (!slot_iteration || !admit) return fail(QEX_E_BADARG, "slot mode needs slot_iteration and admit arrays");
+  Plan1d P;
+  const int rc = plan1d(p, in, out, workspace, workspace_bytes, &P);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  Dev1& a = P.a;
+  a.kslot = slot_iteration;
+  a.admit = admit;
+  a.alpha0 = p->alpha;
+  a.decay = p->alpha_decay;
+  hipLaunchKernelGGL(k_slot_begin, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(PAIR_NT), 0, stream, a.B, a.K, a.ne, P.e.pairs, a.kslot, a.T);
+  launch_iteration(P, 0, 0.0, false, stream);
+  {
+    Scope ps(qexprof::K_FINALIZE, stream);
+    hipLaunchKernelGGL(k_slot_end, dim3(a.B), dim3(64), 0, stream, a);
+  }
+  return check_launch("qex1d_slots_step");
 }
 
 size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, int32_t max_electrons) {
@@ -1875,7 +2026,7 @@ int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int3
   e.vec = (double*)workspace;
   e.eig = (double*)((char*)workspace + align256((size_t)batch * K * num_grids * 8));
   e.pairs = (int*)((char*)e.eig + align256((size_t)batch * K * 8));
-  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(64), 0, stream, batch, K, num_electrons, e.pairs);
+  hipLaunchKernelGGL(k_pairs, dim3(1), dim3(PAIR_NT), 0, stream, batch, K, num_electrons, e.pairs, nullptr, 1);
   if (eigen_lds_bytes(num_grids) > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)eigen_lds_bytes(num_grids));

@@ -108,8 +108,22 @@ __device__ void act3(int act, double z, double* a, double* d1, double* d2) {
   }
 }
 
+// output transform of the network (QEX1D_MLP_OUTPUT_*) and its first two derivatives
+__device__ __forceinline__ double out3(int kind, double y, double* d1, double* d2) {
+  if (kind == QEX1D_MLP_OUTPUT_NEGATIVITY) {  // -swish(y)
+    const double s = 1.0 / (1.0 + exp(-y));
+    *d1 = -(s * (1.0 + y * (1.0 - s)));
+    *d2 = -(s * (1.0 - s) * (2.0 + y * (1.0 - 2.0 * s)));
+    return -y * s;
+  }
+  *d1 = 1.0;
+  *d2 = 0.0;
+  return y;
+}
+
 struct Adj {
-  int B, N, T, K, M, W, H1, H2, NL, act, P;
+  int B, N, T, K, M, W, H1, H2, NL, act, P, mlpout, reflect, sgs;
+  const int* win;  // [B][2] reflection window (reflect)
   double dx, norm;
   int w1, b1, w2, b2, w3, b3;  // parameter offsets (w2/b2 unused when NL == 1)
   const double* table;
@@ -151,9 +165,10 @@ __global__ __launch_bounds__(NT) void k_adj_mix(Adj a, int k, double alpha_k) {
   const int s = blockIdx.x, N = a.N, T = a.T;
   const int cnt = min(a.M, k + 1);
   const double wgt = alpha_k / (double)cnt;
+  const bool stopped = k <= a.sgs;  // the state of this iteration is a constant: no cotangent reaches it
   for (int t = threadIdx.x; t < N; t += NT) {
-    double g = a.nbar[(size_t)s * N + t];
-    if (a.dn) g += a.dn[((size_t)s * T + k) * N + t];
+    double g = stopped ? 0.0 : a.nbar[(size_t)s * N + t];
+    if (a.dn && !stopped) g += a.dn[((size_t)s * T + k) * N + t];
     for (int j = k - cnt + 1; j <= k; ++j) a.dbar[((size_t)s * T + j) * N + t] += wgt * g;
     const double dk = a.dbar[((size_t)s * T + k) * N + t];
     a.nob[(size_t)s * N + t] = dk;
@@ -167,6 +182,42 @@ __global__ __launch_bounds__(NT) void k_adj_mix(Adj a, int k, double alpha_k) {
     }
     a.nout[(size_t)s * N + t] = no / a.dx;
   }
+}
+
+// x[s] <- S x[s] in place: the flip-and-average map inside the system's reflection window (S is
+// symmetric and idempotent, so it is also its own adjoint)
+__global__ void k_sym_inplace(Adj a, double* x) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.B * a.N) return;
+  const int s = (int)(idx / a.N), i = (int)(idx % a.N);
+  const int lo = a.win[2 * s], hi = a.win[2 * s + 1];
+  const int m = (i >= lo && i <= hi) ? lo + hi - i : i;
+  if (i < m) {
+    double* r = x + (size_t)s * a.N;
+    const double v = 0.5 * (r[i] + r[m]);
+    r[i] = v;
+    r[m] = v;
+  }
+}
+
+// dst[s] += S c[s] + d[s] (d optional)
+__global__ void k_sym_add(Adj a, double* dst, const double* c, const double* d) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.B * a.N) return;
+  const int s = (int)(idx / a.N), i = (int)(idx % a.N);
+  const int lo = a.win[2 * s], hi = a.win[2 * s + 1];
+  const int m = (i >= lo && i <= hi) ? lo + hi - i : i;
+  dst[idx] += 0.5 * (c[idx] + c[(size_t)s * a.N + m]) + (d ? d[idx] : 0.0);
+}
+
+// dst[s] = S src[s]
+__global__ void k_sym_copy(Adj a, double* dst, const double* src) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.B * a.N) return;
+  const int s = (int)(idx / a.N), i = (int)(idx % a.N);
+  const int lo = a.win[2 * s], hi = a.win[2 * s + 1];
+  const int m = (i >= lo && i <= hi) ? lo + hi - i : i;
+  dst[idx] = 0.5 * (src[idx] + src[(size_t)s * a.N + m]);
 }
 
 // out[s] = dx * U in[s] (Toeplitz interaction matrix from its first row)
@@ -284,12 +335,18 @@ __global__ __launch_bounds__(NT, 1) void k_adj_mlp(Adj a, const double* xin, con
       ztL = zt2;
     }
     const double w3 = lane < HL ? sW3[lane] : 0.0;
-    const double f = wsum(aL * w3) + b3;
-    const double fd = wsum(atL * w3);
+    // raw output y and tangent yd, then the output transform f = g(y), fd = g'(y) yd
+    const double y = wsum(aL * w3) + b3;
+    const double yd = wsum(atL * w3);
+    double g1, g2;
+    const double f = out3(a.mlpout, y, &g1, &g2);
+    const double fd = g1 * yd;
+    // reverse through the transform: ybar = fbar g' + fdbar g'' yd, ydbar = fdbar g'
+    const double ybar = fma(fbar, g1, fdbar * g2 * yd), ydbar = fdbar * g1;
     // reverse: output layer
-    if (lane < HL) gW3 = fma(fbar, aL, fma(fdbar, atL, gW3));
-    if (lane == 0) gb3 += fbar;
-    const double hb = fbar * w3, hdb = fdbar * w3;
+    if (lane < HL) gW3 = fma(ybar, aL, fma(ydbar, atL, gW3));
+    if (lane == 0) gb3 += ybar;
+    const double hb = ybar * w3, hdb = ydbar * w3;
     double zdb = hdb * d1L;
     double zb = fma(hb, d1L, hdb * d2L * ztL);
     if (lane >= HL) {
@@ -390,7 +447,7 @@ __global__ void k_adj_energy(Adj a, int k, const double* cs) {
 __global__ void k_adj_scales(Adj a, int k, double* cs, double* ones) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
-  cs[s] = a.dE ? a.dE[(size_t)s * a.T + k] * a.dx : 0.0;
+  cs[s] = (a.dE && k > a.sgs) ? a.dE[(size_t)s * a.T + k] * a.dx : 0.0;
   ones[s] = 1.0;
 }
 
@@ -593,7 +650,6 @@ int adj_check(const qex1d_problem* p) {
       p->max_electrons < 1 || p->max_electrons > QEX1D_MAX_ELECTRONS || p->num_mixing_iterations < 1)
     return faila(QEX_E_BADARG, "bad problem sizes");
   if (p->loop != QEX1D_LOOP_SCAN) return faila(QEX_E_UNSUPPORTED, "the reverse pass follows the scan loop");
-  if (p->reflect) return faila(QEX_E_UNSUPPORTED, "the reverse pass has no reflection symmetry");
   if (!(p->density_mse_converge_tolerance < 0.0))
     return faila(QEX_E_UNSUPPORTED, "the reverse pass needs density_mse_converge_tolerance < 0 (no frozen states)");
   if (p->xc_kind != QEX1D_XC_LOCAL_MLP) return faila(QEX_E_UNSUPPORTED, "the reverse pass needs a LOCAL_MLP functional");
@@ -607,6 +663,9 @@ int adj_check(const qex1d_problem* p) {
   if (p->mlp_widths[p->mlp_layers] != 1) return faila(QEX_E_UNSUPPORTED, "MLP output width must be 1");
   if (p->mlp_activation < 0 || p->mlp_activation > QEX_ACT_GELU_TANH || !(p->mlp_norm != 0.0))
     return faila(QEX_E_BADARG, "bad activation or norm");
+  if (p->mlp_output != QEX1D_MLP_OUTPUT_LINEAR && p->mlp_output != QEX1D_MLP_OUTPUT_NEGATIVITY)
+    return faila(QEX_E_BADARG, "bad mlp_output");
+  if (p->stop_gradient_iterations < 0) return faila(QEX_E_BADARG, "stop_gradient_iterations must be >= 0");
   return QEX_OK;
 }
 
@@ -663,6 +722,7 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
     return faila(QEX_E_BADARG, "missing forward inputs");
   if (!fwd->density || !fwd->hartree_potential || !fwd->xc_potential || !fwd->orbitals)
     return faila(QEX_E_BADARG, "the forward outputs density, hartree_potential, xc_potential and orbitals are required");
+  if (p->reflect && !in->reflect_window) return faila(QEX_E_BADARG, "reflection needs reflect_window");
   const size_t need = layout_adj(p).total;
   if (!workspace || workspace_bytes < need) return faila(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
   hipStream_t st = (hipStream_t)stream_;
@@ -679,6 +739,10 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
   a.H1 = p->mlp_widths[1];
   a.H2 = a.NL == 2 ? p->mlp_widths[2] : 0;
   a.act = p->mlp_activation;
+  a.mlpout = p->mlp_output;
+  a.reflect = p->reflect ? 1 : 0;
+  a.win = in->reflect_window;
+  a.sgs = p->stop_gradient_iterations - 1;  // last stopped iteration (-1: none)
   a.P = param_count(p);
   a.dx = p->dx;
   a.norm = p->mlp_norm;
@@ -753,15 +817,29 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
     hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.nbar, BN);
     hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.dbar, BN * a.T);
     hipLaunchKernelGGL(k_zero_a, dim3(1024), dim3(NT), 0, st, a.thp, (long)ADJ_SLOTS * a.P);
-    for (int k = a.T - 1; k >= 0; --k) {
+    // with one mixing difference nothing reaches the stopped iterations; with more,
+    // the differences of those iterations still enter later mixes and are reversed
+    const int k_end = a.M == 1 ? (a.sgs >= 0 ? a.sgs + 1 : 0) : 0;
+    for (int k = a.T - 1; k >= k_end; --k) {
       hipLaunchKernelGGL(k_adj_mix, dim3(a.B), dim3(NT), 0, st, a, k, alphas[k]);
       hipLaunchKernelGGL(k_adj_scales, dim3((a.B + 63) / 64), dim3(64), 0, st, a, k, cs, ones);
-      // energy: v_H(n_out) and the MLP at n_out
+      // with reflection the density passed on is S n_raw: its cotangent maps to S dbar_k on n_raw
+      if (a.reflect) hipLaunchKernelGGL(k_sym_inplace, dim3(gBN), dim3(NT), 0, st, a, a.nob);
+      // energy: v_H(n_raw) and the MLP at the XC density of the energy (n_raw, or S n_raw with reflection)
       hipLaunchKernelGGL(k_toeplitz, dim3((a.N + NT - 1) / NT, a.B), dim3(NT), (size_t)2 * a.N * 8, st, a.N, a.dx,
                          a.table, a.nout, a.tmp);
       hipLaunchKernelGGL(k_adj_energy, dim3(gBN), dim3(NT), 0, st, a, k, cs);
-      mlp(a.nout, nullptr, a.nout, cs, nullptr);
-      hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nob, nullptr, nullptr, a.fv, cs);
+      if (a.reflect) {
+        // dx F(S n_raw): nob += c S grad F(S n_raw) (vbar and tmp are free until the eigen adjoint)
+        hipLaunchKernelGGL(k_sym_copy, dim3(gBN), dim3(NT), 0, st, a, a.vbar, a.nout);
+        mlp(a.vbar, nullptr, a.vbar, cs, nullptr);
+        hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.tmp, BN);
+        hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.tmp, nullptr, nullptr, a.fv, cs);
+        hipLaunchKernelGGL(k_sym_add, dim3(gBN), dim3(NT), 0, st, a, a.nob, a.tmp, nullptr);
+      } else {
+        mlp(a.nout, nullptr, a.nout, cs, nullptr);
+        hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nob, nullptr, nullptr, a.fv, cs);
+      }
       // eigen adjoint: nob -> vbar
       {
         qexprof::Scope ps(qexprof::K_ADJ_EIG, st);
@@ -772,8 +850,18 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
                          a.table, a.vbar, a.tmp);
       // the input densities n_k of all systems, gathered contiguously into nout (free again)
       hipLaunchKernelGGL(k_adj_gather_input, dim3(gBN), dim3(NT), 0, st, a, k, a.nout);
-      mlp(a.nout, a.vbar, a.vbar, ones, a.nout);
-      hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nbar, nullptr, a.tmp, a.fdv, ones);
+      if (a.reflect) {
+        // v_xc(n) = S grad F(S n): the second-order adjoint at (S n, S vbar), its n-cotangent mapped by S
+        hipLaunchKernelGGL(k_sym_inplace, dim3(gBN), dim3(NT), 0, st, a, a.nout);
+        hipLaunchKernelGGL(k_sym_inplace, dim3(gBN), dim3(NT), 0, st, a, a.vbar);
+        mlp(a.nout, a.vbar, a.vbar, ones, a.nout);
+        hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.nob, BN);
+        hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nob, nullptr, nullptr, a.fdv, ones);
+        hipLaunchKernelGGL(k_sym_add, dim3(gBN), dim3(NT), 0, st, a, a.nbar, a.nob, a.tmp);
+      } else {
+        mlp(a.nout, a.vbar, a.vbar, ones, a.nout);
+        hipLaunchKernelGGL(k_adj_gather, dim3(gBN), dim3(NT), 0, st, a, a.nbar, nullptr, a.tmp, a.fdv, ones);
+      }
     }
     hipLaunchKernelGGL(k_adj_reduce, dim3((a.P + 31) / 32), dim3(256), 0, st, a.P, a.thp, d_xc_params);
   }

@@ -167,7 +167,8 @@ class Problem1D:
     def __init__(self, grids, locations, nuclear_charges, num_electrons, interaction_fn, xc,
                  num_iterations, alpha=0.5, alpha_decay=0.9, num_mixing_iterations=2,
                  density_mse_converge_tolerance=-1.0, enforce_reflection_symmetry=False, loop="python",
-                 external_potential_override=None, want_eigenvalues=False, want_orbitals=False):
+                 external_potential_override=None, want_eigenvalues=False, want_orbitals=False,
+                 stop_gradient_step=-1):
         dev = device()
         check_uniform(grids)
         self.dev = dev
@@ -207,6 +208,9 @@ class Problem1D:
         p.density_mse_converge_tolerance = float(density_mse_converge_tolerance)
         p.loop = _lib.QEX1D_LOOP_PYTHON if loop == "python" else _lib.QEX1D_LOOP_SCAN
         p.reflect = 1 if enforce_reflection_symmetry else 0
+        if stop_gradient_step < -1:
+            raise ValueError("stop_gradient_step must be >= -1")
+        p.stop_gradient_iterations = int(stop_gradient_step) + 1  # reverse pass only
         self.xc.configure(p)
         self.problem = p
         self.K = p.max_electrons // 2 + 1
@@ -356,7 +360,7 @@ def kohn_sham_vjp(problem: Problem1D, result: BatchResult, d_total_energy=None, 
     ws = workspace(nbytes, dev)
     grad = torch.empty(problem.xc_params.numel(), dtype=torch.float64, device=dev)
     ins = _lib.Qex1dInputs(_ptr(problem.table), _ptr(problem.vext), _ptr(problem._last_n0), _ptr(problem.ne_t),
-                           None, _ptr(problem.xc_params))
+                           _ptr(problem.win), _ptr(problem.xc_params))
     outs = _lib.Qex1dOutputs(_ptr(result.density), _ptr(result.total_energy), _ptr(result.hartree_potential),
                              _ptr(result.xc_potential), _ptr(result.xc_energy_density), _ptr(result.gap),
                              _ptr(result.converged), None, None, _ptr(result.orbitals))
@@ -365,3 +369,162 @@ def kohn_sham_vjp(problem: Problem1D, result: BatchResult, d_total_energy=None, 
                    "qex1d_kohn_sham_vjp")
     problem._keep_vjp = (dE, dn, ws, ins, outs)
     return grad
+
+
+class SlotRunner:
+    """Continuous batching of 1D solves (``qex1d_slots_step``): a table of ``slots`` systems on one
+    grid, each running its own SCF iteration, so a request is admitted into the running batch as
+    soon as a slot frees up instead of waiting for the next batched launch.
+
+    ``admit(requests)`` places up to ``free_slots()`` requests (locations, charges, N_e, initial
+    density) into free slots; ``step()`` enqueues one iteration for every busy slot on the current
+    stream, and a device-to-host copy of the slot states and final energies into pinned memory
+    with an event; ``poll()`` returns the requests whose solve completed in the steps whose events
+    have fired, as (tag, total energy, converged).  A request's result equals its solo
+    ``qex1d_kohn_sham`` result bit for bit.  Nothing here synchronises the host with the device."""
+
+    def __init__(self, grids, interaction_fn, xc, num_iterations, slots=64, max_electrons=8, alpha=0.5,
+                 alpha_decay=0.9, num_mixing_iterations=2, density_mse_converge_tolerance=-1.0, loop="scan",
+                 depth=3, max_nuclei=4):
+        dev = device()
+        check_uniform(grids)
+        self.dev = dev
+        self.grids = as_f64(grids, dev)
+        self.N = self.grids.shape[0]
+        self.T = int(num_iterations)
+        self.S = int(slots)
+        self.A = int(max_nuclei)
+        kind = getattr(interaction_fn, "kind", None)
+        if kind not in (_lib.QEX_INTERACTION_EXPONENTIAL, _lib.QEX_INTERACTION_SOFT):
+            raise ValueError("the serving path needs a built-in interaction (exponential or soft Coulomb)")
+        self.interaction_fn = interaction_fn
+        self.xc = xcmod.resolve(xc)
+        self.table = interaction_table(self.grids, interaction_fn)
+        p = _lib.Qex1dProblem()
+        p.batch, p.num_grids, p.dx = self.S, self.N, get_dx(self.grids)
+        p.num_iterations = self.T
+        p.max_electrons = int(max_electrons)
+        p.alpha, p.alpha_decay = float(alpha), float(alpha_decay)
+        p.num_mixing_iterations = int(num_mixing_iterations)
+        p.density_mse_converge_tolerance = float(density_mse_converge_tolerance)
+        p.loop = _lib.QEX1D_LOOP_PYTHON if loop == "python" else _lib.QEX1D_LOOP_SCAN
+        p.reflect = 0
+        self.xc.configure(p)
+        self.problem = p
+        self.xc_params = self.xc.device_params(dev)
+        lib = _lib.load()
+        nbytes = lib.qex1d_workspace_bytes(ctypes.byref(p))
+        if nbytes == 0:
+            _lib.check(-1, "qex1d_workspace_bytes")
+        self.ws = workspace(nbytes, dev)
+        S, N, T = self.S, self.N, self.T
+        f = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        self.vext, self.n0 = f(S, N), f(S, N)
+        self.ne = torch.ones(S, dtype=torch.int32, device=dev)
+        self.kslot = torch.full((S,), -1, dtype=torch.int32, device=dev)
+        self.admit_flags = torch.zeros(S, dtype=torch.int32, device=dev)
+        self.out = BatchResult(density=f(S, T, N), total_energy=f(S, T), hartree_potential=f(S, T, N),
+                               xc_potential=f(S, T, N),
+                               xc_energy_density=f(S, T, N) if self.xc.encoding == "local" else f(S, T),
+                               gap=f(S, T), converged=torch.zeros((S, T), dtype=torch.uint8, device=dev),
+                               external_potential=self.vext, stats=torch.zeros(2, dtype=torch.int64, device=dev))
+        self.ins = _lib.Qex1dInputs(_ptr(self.table), _ptr(self.vext), _ptr(self.n0), _ptr(self.ne), None,
+                                    _ptr(self.xc_params))
+        o = self.out
+        self.outs = _lib.Qex1dOutputs(_ptr(o.density), _ptr(o.total_energy), _ptr(o.hartree_potential),
+                                      _ptr(o.xc_potential), _ptr(o.xc_energy_density), _ptr(o.gap), _ptr(o.converged),
+                                      None, _ptr(o.stats))
+        self.free = list(range(S - 1, -1, -1))
+        self.busy = {}  # slot -> (tag, index of the step that starts it)
+        self.depth = int(depth)
+        self.ring = [(torch.empty(S, dtype=torch.int32).pin_memory(), torch.empty(S, dtype=torch.float64).pin_memory(),
+                      torch.empty(S, dtype=torch.uint8).pin_memory(), torch.cuda.Event()) for _ in range(self.depth)]
+        self.inflight = []  # ring indices, oldest first
+        self.next_ring = 0
+        self.steps = 0
+
+    def free_slots(self) -> int:
+        return len(self.free)
+
+    def busy_slots(self) -> int:
+        return len(self.busy)
+
+    def can_step(self) -> bool:
+        return len(self.inflight) < self.depth
+
+    def admit(self, requests):
+        """requests: list of (tag, locations, charges, num_electrons, initial_density); at most
+        ``free_slots()`` of them.  Enqueues the slots' inputs on the current stream."""
+        if not requests:
+            return []
+        if len(requests) > len(self.free):
+            raise ValueError("more requests than free slots")
+        a = len(requests)
+        loc = np.zeros((a, self.A))
+        z = np.zeros((a, self.A))
+        ne = np.zeros(a, dtype=np.int32)
+        n0 = np.empty((a, self.N))
+        slots = []
+        for i, (tag, l, c, e, d) in enumerate(requests):
+            l, c = np.asarray(l, dtype=np.float64).reshape(-1), np.asarray(c, dtype=np.float64).reshape(-1)
+            if len(l) > self.A or not 1 <= int(e) <= self.problem.max_electrons:
+                raise ValueError("request exceeds the runner's nuclei or electron limits")
+            loc[i, :len(l)], z[i, :len(c)], ne[i] = l, c, int(e)
+            n0[i] = np.asarray(d, dtype=np.float64).reshape(self.N)
+            s = self.free.pop()
+            self.busy[s] = (tag, self.steps)
+            slots.append(s)
+        dev = self.dev
+        idx = upload(np.asarray(slots, dtype=np.int64), np.int64, dev)
+        loc_d, z_d = upload(loc, np.float64, dev), upload(z, np.float64, dev)
+        tmp = torch.empty((a, self.N), dtype=torch.float64, device=dev)
+        p0, p1 = self.interaction_fn.params
+        _lib.check(_lib.load().qex1d_external_potential(a, self.N, _ptr(self.grids), self.A, _ptr(loc_d), _ptr(z_d),
+                                                        self.interaction_fn.kind, p0, p1, _ptr(tmp), _stream()),
+                   "qex1d_external_potential")
+        self.vext.index_copy_(0, idx, tmp)
+        self.n0.index_copy_(0, idx, upload(n0, np.float64, dev))
+        self.ne.index_copy_(0, idx, upload(ne, np.int32, dev))
+        self.admit_flags.index_fill_(0, idx, 1)
+        self._keep_admit = (idx, loc_d, z_d, tmp)
+        return slots
+
+    def step(self):
+        """One iteration for every busy slot, then the read-back of the slot states (non-blocking)."""
+        if not self.can_step():
+            raise RuntimeError("too many steps in flight: poll() first")
+        _lib.check(_lib.load().qex1d_slots_step(ctypes.byref(self.problem), ctypes.byref(self.ins),
+                                                ctypes.byref(self.outs), _ptr(self.kslot), _ptr(self.admit_flags),
+                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_slots_step")
+        self.admit_flags.zero_()
+        r = self.next_ring
+        self.next_ring = (r + 1) % self.depth
+        hk, he, hc, ev = self.ring[r]
+        hk.copy_(self.kslot, non_blocking=True)
+        he.copy_(self.out.total_energy[:, self.T - 1], non_blocking=True)
+        hc.copy_(self.out.converged[:, self.T - 1], non_blocking=True)
+        ev.record()
+        self.inflight.append((r, self.steps))
+        self.steps += 1
+
+    def poll(self, wait=False):
+        """[(tag, energy, converged)] of the requests completed by the steps that have finished."""
+        done = []
+        while self.inflight:
+            r, j = self.inflight[0]
+            hk, he, hc, ev = self.ring[r]
+            if wait:
+                ev.synchronize()
+            elif not ev.query():
+                break
+            self.inflight.pop(0)
+            # a request started at step a finishes at step a + T (T iterations, then its energy-only pass)
+            finished = [s for s, (_, a) in self.busy.items() if a + self.T == j]
+            ks = hk.numpy()
+            for s in finished:
+                if ks[s] != -2:
+                    raise _lib.QexError(f"slot {s}: expected a finished solve after step {j}, state {int(ks[s])}")
+                tag, _ = self.busy.pop(s)
+                done.append((tag, float(he[s]), bool(hc[s])))
+                self.free.append(s)
+        return done

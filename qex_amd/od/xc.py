@@ -23,6 +23,11 @@ import torch
 
 from .. import _lib
 
+def _nd(x) -> int:
+    """Array rank of a parameter leaf; -1 for containers (lists, tuples, dicts)."""
+    return -1 if isinstance(x, (list, tuple, dict)) else np.ndim(x)
+
+
 A_EXP = 1.071295
 KAPPA_EXP = 1.0 / 2.385345
 
@@ -100,27 +105,41 @@ class LocalMLPXC(XCFunctional):
 
     xc_kind = _lib.QEX1D_XC_LOCAL_MLP
 
-    def __init__(self, params=None, window=17, activation="silu", norm=2.0):
+    def __init__(self, params=None, window=17, activation="silu", norm=2.0, output="linear"):
         if window < 1 or window % 2 == 0:
             raise ValueError("window must be a positive odd number")
         if activation not in _lib.ACTIVATIONS:
             raise ValueError(f"unsupported activation {activation!r}")
+        if output not in _lib.OUTPUTS:
+            raise ValueError(f"unsupported output transform {output!r}")
         self.window, self.activation, self.norm = int(window), activation, float(norm)
+        self.output = output
         self.params = None if params is None else self._canon(params)
 
     @staticmethod
     def _canon(params):
+        """(W, b) of every Dense layer, in order, from flat [(W, b), ...] or nested stax parameters
+        (activation layers contribute ``()``; stax.serial nests a sub-network's list)."""
         layers = []
-        for item in params:
-            if isinstance(item, (tuple, list)) and len(item) == 2:
-                w, b = item
-                layers.append((np.asarray(w, dtype=np.float64), np.asarray(b, dtype=np.float64).reshape(-1)))
+
+        def walk(p):
+            if isinstance(p, dict):
+                for k in sorted(p):
+                    walk(p[k])
+            elif isinstance(p, (list, tuple)):
+                if len(p) == 2 and _nd(p[0]) == 2 and _nd(p[1]) in (1, 2):
+                    layers.append((np.asarray(p[0], dtype=np.float64), np.asarray(p[1], dtype=np.float64).reshape(-1)))
+                else:
+                    for q in p:
+                        walk(q)
+
+        walk(params)
         if not layers:
             raise ValueError("no Dense layers in params")
         return layers
 
     def with_params(self, params):
-        return LocalMLPXC(params, self.window, self.activation, self.norm)
+        return LocalMLPXC(params, self.window, self.activation, self.norm, self.output)
 
     def widths(self):
         ws = [self.params[0][0].shape[0]] + [w.shape[1] for w, _ in self.params]
@@ -140,6 +159,7 @@ class LocalMLPXC(XCFunctional):
             problem.mlp_widths[i] = int(w)
         problem.mlp_activation = _lib.ACTIVATIONS[self.activation]
         problem.mlp_norm = self.norm
+        problem.mlp_output = _lib.OUTPUTS[self.output]
 
     def packed(self):
         return np.concatenate([np.concatenate([w.reshape(-1), b.reshape(-1)]) for w, b in self.params])
@@ -161,7 +181,12 @@ class LocalMLPXC(XCFunctional):
             x = x @ torch.as_tensor(w, dtype=x.dtype, device=x.device) + torch.as_tensor(b, dtype=x.dtype, device=x.device)
             if li < len(self.params) - 1:
                 x = act(x)
-        return x[..., 0]
+        return _output(self.output, x[..., 0])
+
+
+def _output(kind, y):
+    """The network-output transform: identity, or -swish (jax_dft neural_xc.negativity_transform)."""
+    return -torch.nn.functional.silu(y) if kind == "negativity" else y
 
 
 class GlobalMLPXC(LocalMLPXC):
@@ -176,15 +201,18 @@ class GlobalMLPXC(LocalMLPXC):
     encoding = "global"
     xc_kind = _lib.QEX1D_XC_GLOBAL_MLP
 
-    def __init__(self, params=None, activation="tanh", norm=2.0):
+    def __init__(self, params=None, activation="tanh", norm=2.0, output="linear"):
         if activation not in _lib.ACTIVATIONS:
             raise ValueError(f"unsupported activation {activation!r}")
+        if output not in _lib.OUTPUTS:
+            raise ValueError(f"unsupported output transform {output!r}")
         self.window = None
         self.activation, self.norm = activation, float(norm)
+        self.output = output
         self.params = None if params is None else self._canon(params)
 
     def with_params(self, params):
-        return GlobalMLPXC(params, self.activation, self.norm)
+        return GlobalMLPXC(params, self.activation, self.norm, self.output)
 
     def configure(self, problem):
         if self.params is None:
@@ -202,6 +230,7 @@ class GlobalMLPXC(LocalMLPXC):
             problem.mlp_widths[i] = int(w)
         problem.mlp_activation = _lib.ACTIVATIONS[self.activation]
         problem.mlp_norm = self.norm
+        problem.mlp_output = _lib.OUTPUTS[self.output]
 
     def energy_density(self, n):
         x = n / self.norm
@@ -213,7 +242,7 @@ class GlobalMLPXC(LocalMLPXC):
             x = x @ torch.as_tensor(w, dtype=x.dtype, device=x.device) + torch.as_tensor(b, dtype=x.dtype, device=x.device)
             if li < len(self.params) - 1:
                 x = act(x)
-        return x[..., 0]
+        return _output(self.output, x[..., 0])
 
 
 def get_lda_xc_energy_density_fn():
@@ -226,11 +255,15 @@ def resolve(fn) -> XCFunctional:
     ``params=``) into the functional the kernels evaluate."""
     if isinstance(fn, XCFunctional):
         return fn
-    if isinstance(fn, functools.partial) and isinstance(fn.func, XCFunctional):
+    if isinstance(fn, functools.partial):
         params = fn.keywords.get("params") if fn.keywords else None
         if params is None and fn.args:
             params = fn.args[0]
-        return fn.func.with_params(params) if params is not None else fn.func
+        inner = fn.func
+        if isinstance(inner, XCFunctional):
+            return inner.with_params(params) if params is not None else inner
+        if isinstance(getattr(inner, "qex_functional", None), XCFunctional):
+            return inner.with_params(params) if params is not None else inner.qex_functional
     inner = getattr(fn, "qex_functional", None)
     if isinstance(inner, XCFunctional):
         return inner

@@ -5,9 +5,13 @@ functional and loop settings.  Each 3D request is a molecule (``qex_amd.td.gto.M
 RKS-LDA.
 
 The dispatcher runs in one host thread and never blocks on the device:
-  * 1D: every released request still waiting is packed into the next batched launch (up to
-    ``max_batch``), one ``qex1d_kohn_sham`` per batch, on stream A, while the previous batch may
-    still be running (at most ``depth_1d`` batches in flight);
+  * 1D, continuous batching (``runner_1d`` an ``engine.SlotRunner``): a table of slots on stream A,
+    each running its own SCF iteration (``qex1d_slots_step``); a waiting request is admitted into a
+    free slot at the next step, so requests join and leave the running batch between iterations,
+    and their work overlaps on the device with the solves already running;
+  * 1D, batch per launch (``runner_1d`` an ``engine.BatchRunner``): every released request still
+    waiting is packed into the next batched launch (up to ``max_batch``), one ``qex1d_kohn_sham``
+    per batch, on stream A (at most ``depth_1d`` batches in flight);
   * 3D: the 3D requests waiting (up to ``batch_3d``) are enqueued together on one of
     ``streams_3d`` streams (the least loaded): integrals, grid and AO values per molecule, then
     one batched SCF (``qex3d_scf_batch``: every cycle stage is one launch over the molecules, so the
@@ -68,6 +72,8 @@ class ServeStats:
     batches_1d: int = 0
     batch_sizes: list = field(default_factory=list)
     batch_sizes_3d: list = field(default_factory=list)
+    work_1d: list = field(default_factory=list)  # per completed 1D batch: (electron counts, iterations)
+    work_3d: list = field(default_factory=list)  # per completed 3D solve: (n_ao, grid points, cycles)
     host_s: dict = field(default_factory=lambda: {"dispatch_1d": 0.0, "dispatch_3d": 0.0, "complete_1d": 0.0,
                                                   "complete_3d": 0.0})
 
@@ -103,8 +109,10 @@ class Server:
     def __init__(self, runner_1d, solver_3d, max_batch=64, depth_1d=2, depth_3d=2, streams_3d=3,
                  memory_budget=64 << 30, grid_points_per_atom=5500, batch_3d=16, batch_async_3d=None):
         self.runner = runner_1d
+        self.slots = runner_1d if hasattr(runner_1d, "step") else None  # continuous batching
         self.solver_3d = solver_3d
-        self.max_batch = min(max_batch, runner_1d.max_batch) if runner_1d is not None else max_batch
+        self.max_batch = (runner_1d.S if self.slots is not None else min(max_batch, runner_1d.max_batch)) \
+            if runner_1d is not None else max_batch
         self.depth_1d, self.depth_3d = depth_1d, depth_3d
         self.batch_3d = max(1, int(batch_3d))
         if batch_async_3d is None:
@@ -170,14 +178,48 @@ class Server:
                         for k, r in enumerate(f.reqs):
                             stats.completions.append(Completion(r.rid, "1d", r.arrival, done, float(e[k]),
                                                                 bool(conv[k])))
+                        stats.work_1d.append(([int(r.payload[2]) for r in f.reqs], self.runner.T))
                     else:
                         for r, mf in zip(f.reqs, f.handle):
                             e = mf.collect()
                             stats.completions.append(Completion(r.rid, "3d", r.arrival, done, e, mf.converged))
+                            grids = getattr(mf, "grids", None)
+                            stats.work_3d.append((mf.mol.nao, grids.size if grids is not None else 0,
+                                                  getattr(mf, "cycles", 0)))
                     stats.host_s["complete_" + kind] += time.perf_counter() - th
                     progressed = True
+            if self.slots is not None:
+                # continuous batching: completions, admission into free slots, one step when busy
+                th = time.perf_counter()
+                fin = self.slots.poll()
+                if fin:
+                    done = time.perf_counter() - t0
+                    for r, e, conv in fin:
+                        stats.completions.append(Completion(r.rid, "1d", r.arrival, done, e, conv))
+                        stats.work_1d.append(([int(r.payload[2])], self.slots.T))
+                        used -= self.slots.T * self.slots.N * 8 * 6
+                    stats.host_s["complete_1d"] += time.perf_counter() - th
+                    progressed = True
+                th = time.perf_counter()
+                take = []
+                need = self.slots.T * self.slots.N * 8 * 6
+                while pend_1d and len(take) < self.slots.free_slots() and (
+                        used + need <= self.memory_budget or not (take or self.slots.busy_slots())):
+                    r = pend_1d.popleft()
+                    take.append((r, r.payload[0], r.payload[1], r.payload[2], r.payload[3]))
+                    used += need
+                with torch.cuda.stream(self.stream_1d):
+                    if take:
+                        self.slots.admit(take)
+                        progressed = True
+                    if self.slots.busy_slots() and self.slots.can_step():
+                        self.slots.step()
+                        stats.batches_1d += 1
+                        stats.batch_sizes.append(self.slots.busy_slots())
+                        progressed = True
+                stats.host_s["dispatch_1d"] += time.perf_counter() - th
             # dispatch
-            if pend_1d and len(inflight_1d) < self.depth_1d:
+            if self.slots is None and pend_1d and len(inflight_1d) < self.depth_1d:
                 batch = []
                 nb = 0
                 while pend_1d and len(batch) < self.max_batch:
@@ -196,7 +238,7 @@ class Server:
                 progressed = True
             k3 = min(range(len(inflight_3d)), key=lambda q: len(inflight_3d[q]))
             if pend_3d and len(inflight_3d[k3]) < self.depth_3d:
-                busy = inflight_1d or any(inflight_3d)
+                busy = inflight_1d or any(inflight_3d) or (self.slots is not None and self.slots.busy_slots() > 0)
                 batch, nb = [], 0
                 while pend_3d and len(batch) < self.batch_3d:
                     r = pend_3d[0]
@@ -213,8 +255,9 @@ class Server:
                     inflight_3d[k3].append(_Inflight(ev, batch, mfs, nb))
                     stats.batch_sizes_3d.append(len(batch))
                     progressed = True
+            slots_busy = self.slots is not None and self.slots.busy_slots() > 0
             if not progressed:
-                if i < total and not (pend_1d or pend_3d or inflight_1d or any(inflight_3d)):
+                if i < total and not (pend_1d or pend_3d or inflight_1d or any(inflight_3d) or slots_busy):
                     wait = reqs[i].arrival - (time.perf_counter() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 1e-3))

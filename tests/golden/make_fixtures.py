@@ -19,6 +19,11 @@ Outputs (all float64 unless noted):
   ``distances_x100`` (R,) int64.
 * ``od_h4_rows.npz``, ``od_h2_h2_rows.npz``: the same for the 4-nucleus sets.
 * ``initial_density_h2_128_384.npy``: (2, 513).
+* ``od_h2_train/``: the jax_dft dataset layout (one ``.npy`` per field, as ``data/od/h2/``) restricted to
+  the rows at distances x100 = 128 and 384 (the reference trainer's ``dataset1``,
+  ``qedft/config/train_config.yaml``): ``densities`` (2, 513), ``total_energies`` (2,),
+  ``external_potentials``, ``locations``, ``nuclear_charges`` (int32), ``distances``,
+  ``distances_x100`` (int64), ``num_electrons``, ``grids``.
 """
 
 import os
@@ -65,6 +70,16 @@ def main(ref):
             num_electrons=np.int64(ld("num_electrons.npy")),
         )
         np.savez(os.path.join(here, f"od_{mol}_rows.npz"), **out)
+    d = os.path.join(ref, "data", "od", "h2")
+    ld = lambda f: np.load(os.path.join(d, f), allow_pickle=False)
+    sel = np.isin(ld("distances_x100.npy"), [128, 384])
+    out_dir = os.path.join(here, "od_h2_train")
+    os.makedirs(out_dir, exist_ok=True)
+    for f in ("densities", "total_energies", "external_potentials", "locations", "nuclear_charges", "distances",
+              "distances_x100"):
+        np.save(os.path.join(out_dir, f + ".npy"), ld(f + ".npy")[sel])
+    np.save(os.path.join(out_dir, "num_electrons.npy"), ld("num_electrons.npy"))
+    np.save(os.path.join(out_dir, "grids.npy"), ld("grids.npy"))
     dens = _extract_float_buffer(os.path.join(ref, "qedft", "models", "initial_density.pkl"), 2 * 513 * 8)
     np.save(os.path.join(here, "initial_density_h2_128_384.npy"), dens)
     print("fixtures written to", here)

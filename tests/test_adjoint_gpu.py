@@ -120,3 +120,40 @@ def test_gradient_is_batch_additive_and_deterministic(mods):
     g_sum = grad_of([0]) + grad_of([1]) + grad_of([2])
     torch.testing.assert_close(g_all, g_sum, rtol=1e-11, atol=1e-11 * float(g_all.abs().max()))
     assert torch.equal(grad_of([0, 1, 2]), g_all)
+
+
+@pytest.mark.parametrize("output,window,act", [("negativity", 1, "tanh"), ("negativity", 17, "silu"),
+                                               ("linear", 5, "tanh")])
+def test_reflected_gradient_matches_oracle(mods, output, window, act):
+    """The trainer's setting (enforce_reflection_symmetry about the grid centre, od/train.py:110) with
+    the negativity output transform (wrappers.py:201-203): device reverse pass against the oracle's."""
+    engine, train, utils, xc = mods
+    grids = np.linspace(-12.8, 12.8, 257)
+    dx = float(grids[1] - grids[0])
+    systems = [(np.array([-0.7, 0.7]), np.array([1.0, 1.0]), 2), (np.array([-1.5, 1.5]), np.array([1.0, 1.0]), 2),
+               (np.array([-0.4, 0.4]), np.array([2.0, 2.0]), 4)]
+    params = _params(7, [window, 16, 16, 1])
+    T, alpha, decay = 6, 0.5, 0.9
+    rng = np.random.default_rng(1)
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in systems])
+    n0 = n0 * (1 + 0.05 * rng.standard_normal(n0.shape) ** 2)  # not symmetric: the flip matters
+    f = xc.LocalMLPXC(params, window=window, activation=act, norm=2.0, output=output)
+    prob = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [s[2] for s in systems],
+                            utils.exponential_coulomb, f, T, alpha=alpha, alpha_decay=decay, num_mixing_iterations=1,
+                            loop="scan", want_orbitals=True, enforce_reflection_symmetry=True)
+    tE = np.array([-1.2, -0.9, -4.0])
+    tn = 0.9 * n0
+    _, g, res = train.value_and_grad(prob, n0, train.trajectory_loss, tE, tn, 2, dx, start=2)
+    g = g.cpu().numpy()
+    oxc = O.LocalMLP([w for w, _ in params], [b for _, b in params], act, window, 2.0, output=output)
+    go = np.zeros_like(g)
+    B = len(systems)
+    for s, (loc, z, ne) in enumerate(systems):
+        fw = A.forward_trajectory(loc, z, ne, T, grids, oxc, O.exponential_coulomb, n0[s], alpha, decay, 1,
+                                  reflect=True)
+        np.testing.assert_allclose(res.total_energy[s].cpu().numpy(), fw["energies"], atol=1e-8)
+        np.testing.assert_allclose(res.density[s].cpu().numpy(), fw["densities"], atol=1e-8)
+        _, dE, dn = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], tE[s], tn[s], 2, dx, start=2)
+        go += A.backward(fw, dE / B, dn / B)
+    scale = np.abs(go).max()
+    np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * scale)

@@ -125,3 +125,117 @@ def test_forward_trajectory_matches_scan_oracle():
                          num_mixing_iterations=1)
     np.testing.assert_allclose(fw["energies"], st.total_energy, atol=1e-10)
     np.testing.assert_allclose(fw["densities"], st.density, atol=1e-10)
+
+
+def test_negativity_transform_is_minus_swish():
+    """-swish(y) = -y sigmoid(y): 0 at 0, maximum 0.2785 at y = -1.2785 (where sigmoid(y)(1 + y(1 -
+    sigmoid(y))) = 0), -> -y for large y; the derivatives are checked by finite differences."""
+    y = np.linspace(-8, 8, 161)
+    g, d1, d2 = A.output3("negativity", y)
+    np.testing.assert_allclose(g, -y / (1 + np.exp(-y)), rtol=1e-15)
+    assert g.max() < 0.2785 and g.max() > 0.27
+    h = 1e-6
+    np.testing.assert_allclose(d1, (A.output3("negativity", y + h)[0] - A.output3("negativity", y - h)[0]) / (2 * h),
+                               atol=1e-8)
+    np.testing.assert_allclose(d2, (A.output3("negativity", y + h)[1] - A.output3("negativity", y - h)[1]) / (2 * h),
+                               atol=1e-8)
+    e, de = O.output_transform("negativity", y)
+    np.testing.assert_array_equal(e, g)
+    np.testing.assert_array_equal(de, d1)
+
+
+def test_mlp_second_order_with_output_transform_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    params = _net(4, [5, 7, 1], "tanh")
+    W, b = [w for w, _ in params], [x for _, x in params]
+    X, Xd = rng.standard_normal((9, 5)), rng.standard_normal((9, 5))
+    fbar, fdbar = rng.standard_normal(9), rng.standard_normal(9)
+
+    def phi(X_, W_):
+        f, fd, _, _, _ = A.mlp_second_order(W_, b, "tanh", X_, Xd, np.zeros(9), np.zeros(9), "negativity")
+        return float(fbar @ f + fdbar @ fd)
+
+    _, _, Xbar, dW, db = A.mlp_second_order(W, b, "tanh", X, Xd, fbar, fdbar, "negativity")
+    h = 1e-6
+    for (p, q) in [(0, 1), (5, 3)]:
+        E = np.zeros_like(X)
+        E[p, q] = h
+        assert abs((phi(X + E, W) - phi(X - E, W)) / (2 * h) - Xbar[p, q]) < 1e-7
+    for li, (p, q) in [(0, (2, 3)), (1, (4, 0))]:
+        Wp, Wm = [w.copy() for w in W], [w.copy() for w in W]
+        Wp[li][p, q] += h
+        Wm[li][p, q] -= h
+        assert abs((phi(X, Wp) - phi(X, Wm)) / (2 * h) - dW[li][p, q]) < 1e-7
+
+
+@pytest.mark.parametrize("output,window", [("negativity", 1), ("negativity", 5), ("linear", 3)])
+def test_reflected_trajectory_gradient_matches_finite_differences(output, window):
+    """enforce_reflection_symmetry about the grid centre (what KSDFTTrainer always sets,
+    qedft/train/od/train.py:110) with the output transform of wrap_with_negative_transform."""
+    grids = np.linspace(-8, 8, 129)
+    dx = O.get_dx(grids)
+    params = _net(6, [window, 8, 8, 1], "tanh")
+    loc, z, ne = np.array([-0.75, 0.75]), np.array([1.0, 1.0]), 2
+    rng = np.random.default_rng(2)
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True) * (1 + 0.05 * rng.standard_normal(129) ** 2)
+
+    def run(flat):
+        p = A.unflatten(flat, params)
+        xc = O.LocalMLP([w for w, _ in p], [b for _, b in p], "tanh", window, 2.0, output=output)
+        fw = A.forward_trajectory(loc, z, ne, 6, grids, xc, O.exponential_coulomb, n0, 0.5, 0.9, 1, reflect=True)
+        L, e1, d1 = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], -1.4, 0.95 * n0, ne, dx,
+                                                     start=2)
+        return L, fw, e1, d1
+
+    th = A.flatten(params)
+    _, fw, dE, dn = run(th)
+    g = A.backward(fw, dE, dn)
+    h = 1e-5
+    for _ in range(2):
+        d = rng.standard_normal(th.size)
+        d /= np.linalg.norm(d)
+        fd = (run(th + h * d)[0] - run(th - h * d)[0]) / (2 * h)
+        assert abs(fd - g @ d) < 1e-6 * max(1.0, abs(fd))
+    # the reflected forward is jit_kohn_sham with enforce_reflection_symmetry (S e(S n) wrapper)
+    xc = O.LocalMLP([w for w, _ in params], [b for _, b in params], "tanh", window, 2.0, output=output)
+    st = O.jit_kohn_sham(loc, z, ne, 6, grids, xc, O.exponential_coulomb, n0, alpha=0.5, alpha_decay=0.9,
+                         enforce_reflection_symmetry=True, num_mixing_iterations=1)
+    np.testing.assert_allclose(fw["energies"], st.total_energy, atol=1e-10)
+    np.testing.assert_allclose(fw["densities"], st.density, atol=1e-10)
+
+
+def test_stop_gradient_step_is_the_gradient_of_the_restarted_loss():
+    """With states of iterations <= s held constant (jax.lax.stop_gradient in jit_scf), and one mixing
+    difference, the gradient is that of the loss of a run that restarts from the stored density of
+    iteration s (mixing factor alpha decay^(s+1)) and only sees iterations s+1..T-1; checked by central
+    differences of that restarted run."""
+    grids = np.linspace(-8, 8, 129)
+    dx = O.get_dx(grids)
+    params = _net(8, [5, 8, 1], "tanh")
+    loc, z, ne = np.array([-0.7, 0.8]), np.array([1.0, 1.0]), 2
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True)
+    T, s, alpha, decay = 7, 2, 0.5, 0.9
+    xc0 = O.LocalMLP([w for w, _ in params], [b for _, b in params], "tanh", 5, 2.0)
+    fw = A.forward_trajectory(loc, z, ne, T, grids, xc0, O.exponential_coulomb, n0, alpha, decay, 1)
+    _, dE, dn = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], -1.3, 0.9 * n0, ne, dx, start=0)
+    g = A.backward(fw, dE, dn, stop_gradient_step=s)
+    n_restart = fw["densities"][s]
+    tail_E, tail_n = dE[s + 1:].copy(), dn[s + 1:].copy()
+
+    def restarted(flat):
+        p = A.unflatten(flat, params)
+        xc = O.LocalMLP([w for w, _ in p], [b for _, b in p], "tanh", 5, 2.0)
+        f2 = A.forward_trajectory(loc, z, ne, T - s - 1, grids, xc, O.exponential_coulomb, n_restart,
+                                  alpha * decay ** (s + 1), decay, 1)
+        return float(tail_E @ f2["energies"] + np.sum(tail_n * f2["densities"]))
+
+    th = A.flatten(params)
+    rng = np.random.default_rng(4)
+    h = 1e-5
+    for _ in range(2):
+        d = rng.standard_normal(th.size)
+        d /= np.linalg.norm(d)
+        fd = (restarted(th + h * d) - restarted(th - h * d)) / (2 * h)
+        assert abs(fd - g @ d) < 1e-7 * max(1.0, abs(fd))
+    assert np.abs(A.backward(fw, dE, dn, stop_gradient_step=T - 1)).max() == 0.0
+    assert np.abs(g - A.backward(fw, dE, dn)).max() > 1e-6

@@ -131,3 +131,68 @@ def test_one_at_a_time_path(fake_cuda):
     reqs = _requests(20, 2, rate=1000.0)
     st = serve.run_one_at_a_time(reqs, _Runner(), _SCF)
     assert [c.rid for c in st.completions] == [r.rid for r in reqs]
+
+
+class _Slots:
+    """Host stand-in of engine.SlotRunner: a request admitted before step a finishes at step a + T."""
+
+    T, N = 3, 5
+
+    def __init__(self, S=4, depth=2):
+        self.S, self.depth = S, depth
+        self.free = list(range(S))
+        self.busy = {}
+        self.steps = 0
+        self.inflight = []
+        self.max_busy = 0
+        self.admitted_at = {}
+
+    def free_slots(self):
+        return len(self.free)
+
+    def busy_slots(self):
+        return len(self.busy)
+
+    def can_step(self):
+        return len(self.inflight) < self.depth
+
+    def admit(self, reqs):
+        assert len(reqs) <= len(self.free)
+        for r in reqs:
+            s = self.free.pop()
+            self.busy[s] = (r[0], self.steps)
+            self.admitted_at[r[0].rid] = self.steps
+
+    def step(self):
+        assert self.can_step()
+        self.inflight.append(self.steps)
+        self.steps += 1
+        self.max_busy = max(self.max_busy, len(self.busy))
+
+    def poll(self):
+        done = []
+        while self.inflight:
+            j = self.inflight.pop(0)
+            for s, (r, a) in list(self.busy.items()):
+                if a + self.T == j:
+                    del self.busy[s]
+                    self.free.append(s)
+                    done.append((r, float(np.sum(r.payload[1])), True))
+        return done
+
+
+def test_continuous_batching_admits_into_free_slots_between_steps(fake_cuda):
+    """With a slot table, a request that arrives while others run joins at the next step once a slot
+    is free (no waiting for a whole batch to drain); every request completes once with its result."""
+    slots = _Slots(S=4)
+    srv = serve.Server(slots, _SCF, batch_async_3d=_batch_async)
+    burst = [serve.Request("1d", 0.0, (np.zeros(1), np.full(1, float(i % 3 + 1)), 1, np.zeros(5)), rid=i)
+             for i in range(10)]
+    st = srv.run(burst)
+    assert sorted(c.rid for c in st.completions) == list(range(10))
+    for c in st.completions:
+        assert c.energy == float(burst[c.rid].payload[1][0])
+    assert slots.max_busy == 4
+    # requests 4.. are admitted as earlier ones finish, at different steps (not all at once)
+    starts = sorted(slots.admitted_at[i] for i in range(10))
+    assert starts[:4] == [0, 0, 0, 0] and len(set(starts[4:])) >= 2 and starts[4] <= slots.T + 1

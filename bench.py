@@ -109,6 +109,7 @@ def parse_args():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="launch one 64 MiB device-to-device copy first (reference for HBM counters)")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c5")
+    ap.add_argument("--slots", type=int, default=128, help="c5: 1D slots of the continuous-batching table")
     ap.add_argument("--rate", type=float, default=50000.0,
                     help="c5: Poisson arrival rate of the timed stream (req/s; above capacity by default)")
     ap.add_argument("--rates", type=str, default="50,200,500,1000,2000",
@@ -226,14 +227,19 @@ def run_c2(args):
     # FP64 work the eigen kernel executes: 14 flops per Sturm step (per round 128 shifts, each
     # counted over all N rows by two lanes from the two ends) and ~30 N per eigenpair for the
     # LDL^T factorisation and the two solves
-    eig_flops = (float(stats[0]) * 128 * N_GRID * 14 + float(stats[1]) * 30 * N_GRID) / eig_launches
-    # MLP: forward + input-VJP at the input density and a forward at the output density
-    mlp_flops = args.batch * N_GRID * 3 * 2 * (17 * 64 + 64 * 64 + 64) * 1.0
-    pot_flops = args.batch * 2 * 2 * N_GRID * N_GRID * 1.0
-    flops_per_launch = {"eigen": eig_flops, "mlp": mlp_flops, "potential": pot_flops}
+    eig_exec = (float(stats[0]) * 128 * N_GRID * 14 + float(stats[1]) * 30 * N_GRID) / eig_launches
+    # algorithmic work per launch (one SCF iteration of the batch), as written in DESIGN.md: the
+    # eigensolve counted as the dense symmetric solve (4/3) N^3 + 2 k N^2 per system that the
+    # reference's eigh performs; MLP and Hartree as their GEMM-shaped FLOPs
+    ne_list = [int(m[2]) for m in mols]
+    flops_per_launch = {k: _work_1d(k, ne_list) for k in ("eigen", "mlp", "potential")}
     fpl = flops_per_launch.get(dom, 0.0)
     achieved = fpl / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
-    peak = 78.6  # FP64 TFLOP/s (MI355X)
+    bound, peak, _ = ROOF.get(dom, ("mfma", 78.6, "TFLOP/s"))
+    # critical-path model of the eigen kernel: per eigenpair, rounds x N/2 dependent Sturm steps of
+    # ~90 cycles (measured: one wave per SIMD) plus two LDL^T solves of N steps, at 2.4 GHz
+    rounds = float(stats[0]) / max(1.0, float(stats[1]))
+    eig_model_us = (rounds * (N_GRID / 2) * 90 + 2 * N_GRID * 60) / 2.4e3
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
     # (profiles/c2_batched_1d_scf_kernels.md, counters scaled by a 64 MiB copy), when present
     traffic = None
@@ -268,8 +274,12 @@ def run_c2(args):
         "kernel_tflops": {k: round(flops_per_launch[k] / (avg_ms[k] * 1e-3) / 1e12, 3)
                           for k in flops_per_launch if avg_ms.get(k, 0) > 0},
         "eigen_rounds_per_pair": float(stats[0] / max(1, stats[1])),
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic},
+        "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "work": "dense-equivalent FP64 FLOP per launch (algorithmic, DESIGN.md)",
+                     "eigen_executed_tflops": round(eig_exec / (avg_ms.get("eigen", 0) * 1e-3) / 1e12, 3)
+                     if avg_ms.get("eigen") else None,
+                     "eigen_latency_model_us_per_pair": round(eig_model_us, 1)},
         "cpu_baseline": {"value": float(len(per_solve) / oracle_wall), "unit": "solves/s", "cores": procs,
                          "kind": "port",
                          "sample": f"FP64 NumPy/SciPy oracle, the same {args.batch} solves, {procs} processes x 1 "
@@ -706,8 +716,11 @@ def run_c5(args):
 
     lib = _lib.load()
     f = xc.LocalMLPXC(list(zip(W, b)), window=17, activation="silu", norm=2.0)
-    runner = engine.BatchRunner(grids, utils.exponential_coulomb, f, T_ITERS, max_batch=args.batch, alpha=ALPHA,
+    # plain path: one system per launch; served path: a table of slots (continuous batching)
+    runner = engine.BatchRunner(grids, utils.exponential_coulomb, f, T_ITERS, max_batch=1, alpha=ALPHA,
                                 alpha_decay=1.0, num_mixing_iterations=1, loop="scan")
+    slots = engine.SlotRunner(grids, utils.exponential_coulomb, f, T_ITERS, slots=args.slots, alpha=ALPHA,
+                              alpha_decay=1.0, num_mixing_iterations=1, loop="scan", depth=3)
     mols = [gto.synthetic_chain(n, 100 + n) for n in C4_CHAINS]
     for m in mols:
         m.device_basis()
@@ -734,7 +747,7 @@ def run_c5(args):
             return e_ref is not None and abs(c.energy - e_ref) <= 1e-6
         return c.converged and abs(c.energy - ref3[C4_CHAINS[pick_3d[c.rid]]]["e_tot"]) <= 1e-6
 
-    server = serve.Server(runner, solver_3d, max_batch=args.batch, batch_3d=args.batch_3d, streams_3d=args.streams_3d,
+    server = serve.Server(slots, solver_3d, batch_3d=args.batch_3d, streams_3d=args.streams_3d,
                           depth_3d=args.depth_3d)
     # ---- warm-up: the first W steps of the stream ----
     if nwarm:
@@ -816,7 +829,7 @@ def run_c5(args):
                                "513-pt grid, LocalMLP-17 [64,64] SiLU, 20 iterations) / 30% 3D RKS-LDA chains "
                                "H2/H4/H8/H16 (conv 1e-8, CDIIS 8, <= 30 cycles); Poisson arrivals at "
                                f"{args.rate:g} req/s (above the served capacity: the stream is backlogged)",
-                   "requests_timed": ntimed, "arrival_rate": args.rate, "max_batch_1d": args.batch,
+                   "requests_timed": ntimed, "arrival_rate": args.rate, "slots_1d": args.slots,
                    "max_batch_3d": args.batch_3d, "streams_3d": args.streams_3d, "depth_3d": args.depth_3d,
                    "path": "served: continuous batching, FP64"},
         "latency_ms": {"p50": summ["p50_ms"], "p99": summ["p99_ms"], "p50_1d": summ.get("p50_ms_1d"),

@@ -215,6 +215,8 @@ const char* qex1d_vjp_last_error(void);
 #define QEX3D_RKS_LDA 0 /* Slater exchange + VWN5 correlation */
 #define QEX3D_RHF 1
 #define QEX3D_RKS_NN_GLOBAL 2 /* amplitude-encoded network on the whole grid density */
+#define QEX3D_RKS_NN_LOCAL 4  /* per-point network ("NN" xctype): exc_g = net(rho_g), vrho_g = d exc_g / d rho_g */
+#define QEX3D_MAX_NN_LOCAL_HIDDEN 512
 #define QEX3D_RKS_LDA_X 3     /* Slater exchange alone: pyscf's xc code "lda" (mf.xc = "lda" in
                                  qedft/train/td/trainer_legacy_no_jit.py:635 and
                                  qedft/data_io/td/dataset_generation.py:392) */
@@ -287,6 +289,27 @@ size_t qex3d_nr_rks_nn_workspace_bytes(int64_t ngrid, int32_t nao, int32_t h1, i
 int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
                     const double* nn_params, int32_t h1, int32_t h2, double scale, double* nelec_excsum,
                     double* vmat, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Per-point ("NN", local) neural XC: exc_g = -scale swish(MLP(rho_g)) with MLP = Dense(h1) -> gelu ->
+ * Dense(h2) -> gelu -> Dense(1) on the scalar rho_g (gelu in its tanh form), vrho_g = d exc_g / d rho_g
+ * (the VJP with a unit cotangent of exc_and_vrho_local, qedft/train/td/trainer_legacy_no_jit.py:56-63,
+ * MLP :96-107).  Parameters packed as W1[1][h1], b1, W2[h1][h2], b2, W3[h2], b3 (flax kernels (in, out));
+ * h1, h2 <= QEX3D_MAX_NN_LOCAL_HIDDEN.  nr_rks "NN" branch (qedft/train/td/numint.py:179-204):
+ * nelec_excsum = {sum rho w, sum rho w exc}, vmat = V + V^T with V = ao^T diag(w vrho / 2) ao.
+ * Same workspace as qex3d_nr_rks. */
+int qex3d_nr_rks_nn_local(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
+                          const double* nn_params, int32_t h1, int32_t h2, double scale, double* nelec_excsum,
+                          double* vmat, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The network XC evaluated on a density, as the custom eval_xc installed by define_xc_ returns it
+ * (qedft/train/td/trainer_legacy_no_jit.py:76-93 with exc_and_vrho_global / exc_and_vrho_local):
+ * is_global != 0: exc[0] = the network's scalar on rho[ngrid] (packing of qex3d_nr_rks_nn),
+ * vrho[g] = d exc / d rho_g; is_global == 0: exc[g], vrho[g] per point (packing of
+ * qex3d_nr_rks_nn_local). */
+size_t qex3d_eval_xc_nn_workspace_bytes(int64_t ngrid, int32_t h1, int32_t h2, int32_t is_global);
+int qex3d_eval_xc_nn(int32_t is_global, int64_t ngrid, const double* rho, const double* nn_params, int32_t h1,
+                     int32_t h2, double scale, double* exc, double* vrho, void* workspace, size_t workspace_bytes,
+                     void* stream);
 
 /* Batched generalized symmetric eigenproblem F C = S C diag(e), S positive definite, n <= 96:
  * e ascending, C^T S C = 1 (columns are eigenvectors).  Cholesky S = L L^T, X = L^-T, then the

@@ -176,13 +176,15 @@ def occupations(num_electrons):
 
 
 def forward_trajectory(locations, charges, num_electrons, num_iterations, grids, xc, interaction_fn,
-                       initial_density, alpha, alpha_decay, num_mixing_iterations, reflect=False):
+                       initial_density, alpha, alpha_decay, num_mixing_iterations, reflect=False, flip=None):
     """Scan-semantics forward (jit_kohn_sham without convergence freezing) keeping what the reverse
     pass needs.  reflect: enforce_reflection_symmetry about the grid centre (od/jit_scf.py:76-77),
     S = flip-and-average: the XC functional is the wrapped e_S(n) = S e(S n), so F_S(n) = e_S(n).n =
     F(S n) and v_xc = S grad F(S n); the energy is that of the raw output density n_raw (with the XC
-    part F(S n_raw)), and the density passed on is n_out = S n_raw (jax_dft flips after the energy)."""
-    S = O.flip_and_average_on_center if reflect else (lambda a: a)
+    part F(S n_raw)), and the density passed on is n_out = S n_raw (jax_dft flips after the energy).
+    ``flip``: another flip-and-average map (the python loop's, about the centre of the nuclei,
+    od/scf.py:162-167); with a negative convergence tolerance that loop is otherwise this one."""
+    S = (flip or O.flip_and_average_on_center) if reflect else (lambda a: a)
     grids = np.asarray(grids, dtype=np.float64)
     dx = O.get_dx(grids)
     v_ext = O.get_atomic_chain_potential(grids, np.asarray(locations, dtype=np.float64), np.asarray(charges),

@@ -476,6 +476,38 @@ class GlobalMLP3D:
         return float(np.sum(out)), gh
 
 
+class LocalMLP3D(GlobalMLP3D):
+    """The same MLP applied to every grid point's density (``exc_and_vrho_local`` at
+    ``trainer_legacy_no_jit.py:56-63``: network.apply(params, rho[:, None]).squeeze(), then the VJP
+    with a unit cotangent): exc_g = net(rho_g), vrho_g = d exc_g / d rho_g (points are independent)."""
+
+    def exc_vrho(self, rho):
+        h = np.asarray(rho, dtype=np.float64)[:, None]
+        zs = []
+        for W, b in zip(self.W[:-1], self.b[:-1]):
+            z = h @ W + b
+            zs.append(z)
+            h = gelu_tanh(z)
+        z = (h @ self.W[-1] + self.b[-1])[:, 0]
+        s = _sigmoid(z)
+        exc = -self.scale * z * s
+        g = (-self.scale * (s + z * s * (1.0 - s)))[:, None] * self.W[-1][:, 0][None, :]
+        for W, zz in zip(reversed(self.W[:-1]), reversed(zs)):
+            g = (g * gelu_tanh_grad(zz)) @ W.T
+        return exc, g[:, 0]
+
+
+def nr_rks_nn_local(ao, weights, dm, net):
+    """(nelec, excsum, vmat) of nr_rks's "NN" branch (td/numint.py:179-204): excsum = sum rho w exc and
+    V = ao^T diag(w vrho / 2) ao with vrho = d exc / d rho (not d(rho exc) / d rho), then V + V^T."""
+    rho = eval_rho(ao, dm)
+    exc, vrho = net.exc_vrho(rho)
+    den = rho * weights
+    aow = ao * (0.5 * weights * vrho)[:, None]
+    vmat = ao.T @ aow
+    return float(den.sum()), float(den @ exc), vmat + vmat.T
+
+
 def init_global_mlp(seed, n_in, hidden=(512, 512)):
     """Seeded weights: kernels N(0, 1/fan_in) (the variance of flax's lecun_normal), zero biases."""
     rng = np.random.default_rng(seed)
@@ -565,7 +597,7 @@ def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8,
     g = ints[3] if ints is not None else eri(mol)
     H = T + V
     nocc = mol.nelectron // 2
-    if method in ("rks", "rks_nn"):
+    if method in ("rks", "rks_nn", "rks_nn_local"):
         pts, w = grid if grid is not None else molecular_grid(mol)
         ao = eval_ao(mol, pts)
 
@@ -576,6 +608,8 @@ def scf(mol, method="rks", grid=None, max_cycle=30, conv_tol=1e-8, diis_space=8,
             return J - 0.5 * K, 0.5 * np.sum(D * J) - 0.25 * np.sum(D * K), np.nan
         if method == "rks_nn":
             nel, exc, vxc = nr_rks_nn_global(ao, w, D, net)
+        elif method == "rks_nn_local":
+            nel, exc, vxc = nr_rks_nn_local(ao, w, D, net)
         else:
             nel, exc, vxc = nr_rks_lda(ao, w, D, functional)
         return J + vxc, 0.5 * np.sum(D * J) + exc, nel

@@ -95,6 +95,8 @@ QEX3D_RKS_LDA = 0
 QEX3D_RHF = 1
 QEX3D_RKS_NN_GLOBAL = 2
 QEX3D_RKS_LDA_X = 3
+QEX3D_RKS_NN_LOCAL = 4
+QEX3D_MAX_NN_LOCAL_HIDDEN = 512
 QEX3D_LDA_SVWN5 = 0
 QEX3D_LDA_SLATER = 1
 QEX3D_MAX_NN_HIDDEN = 1024
@@ -161,6 +163,10 @@ SIGNATURES = {
     "qex3d_nr_rks_lda": (_I, [_I, ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),
     "qex3d_nr_rks_nn_workspace_bytes": (_S, [ctypes.c_int64, _I, _I, _I]),
     "qex3d_nr_rks_nn": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S, _P]),
+    "qex3d_nr_rks_nn_local": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S,
+                                    _P]),
+    "qex3d_eval_xc_nn_workspace_bytes": (_S, [ctypes.c_int64, _I, _I, _I]),
+    "qex3d_eval_xc_nn": (_I, [_I, ctypes.c_int64, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S, _P]),
     "qex3d_gen_eigh_workspace_bytes": (_S, [_I, _I]),
     "qex3d_gen_eigh": (_I, [_I, _I, _P, _P, _P, _P, _P, _S, _P]),
     "qex3d_scf_workspace_bytes": (_S, [ctypes.POINTER(Qex3dScfProblem)]),

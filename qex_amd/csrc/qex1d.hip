@@ -429,7 +429,7 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k_launch) {
   const int frozen = a.st[4 * s + 0];
   const int pending = a.st[4 * s + 1];
   const bool doPot = k >= 0 && k < a.T && !frozen;
-  const bool doE = pending >= 0;
+  const bool doE = pending >= 0 && pending < a.T;
   if (!doPot && !doE) return;
   const int N = a.N, stride = a.wmax + 1;
   double* dens = smem;                      // N
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k_
       const int s = (int)(g / N);
       const int k = iter_of(a, s, k_launch);
       if (k >= 0 && k < a.T && !a.st[4 * s + 0]) sh_any[0] = 1;
-      if (a.st[4 * s + 1] >= 0) sh_any[1] = 1;
+      if (a.st[4 * s + 1] >= 0 && a.st[4 * s + 1] < a.T) sh_any[1] = 1;
     }
   }
   __syncthreads();
@@ -611,8 +611,9 @@ __global__ __launch_bounds__(NT, NH <= 2 ? 3 : 2) void k_mlp_mfma(Dev1 a, int k_
       const bool inb = g < total;
       const int s = inb ? (int)(g / N) : 0, i = inb ? (int)(g % N) : 0;
       const int k = iter_of(a, s, k_launch);
-      const bool act = inb && (pass == 0 ? (k >= 0 && k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0));
-      const double* src = pass == 0 ? input_density(a, s, k) : a.o_xc + (size_t)s * N;
+      const bool act =
+          inb && (pass == 0 ? (k >= 0 && k < a.T && !a.st[4 * s + 0]) : (a.st[4 * s + 1] >= 0 && a.st[4 * s + 1] < a.T));
+      const double* src = !act ? nullptr : pass == 0 ? input_density(a, s, k) : a.o_xc + (size_t)s * N;
       int lo = 0, hi = -1;
       if (pass == 0 && sym && act) {
         lo = a.win[2 * s];
@@ -770,7 +771,7 @@ __global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k_launch) {
   const int s = blockIdx.x, N = a.N, tid = threadIdx.x;
   const int k = iter_of(a, s, k_launch);
   const bool doPot = k >= 0 && k < a.T && a.st[4 * s + 0] == 0;
-  const bool doE = a.st[4 * s + 1] >= 0;
+  const bool doE = a.st[4 * s + 1] >= 0 && a.st[4 * s + 1] < a.T;
   double* x = smem;                 // N
   double* acts = x + N;             // 6 x 128
   double* dzs = acts + 6 * 128;     // 6 x 128
@@ -827,7 +828,7 @@ __global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k_launch) {
   const int frozen = a.st[4 * s + 0];
   const int pending = a.st[4 * s + 1];
   const bool doPot = k >= 0 && k < a.T && !frozen;
-  const bool doE = pending >= 0;
+  const bool doE = pending >= 0 && pending < a.T;
   if (!doPot && !doE) return;
   const int N = a.N;
   double* tab = smem;        // N
@@ -1343,12 +1344,12 @@ __global__ __launch_bounds__(NT) void k_density(Dev1 a, int k_launch, double alp
   const int s = blockIdx.x, tid = threadIdx.x, N = a.N, T = a.T;
   const int k = iter_of(a, s, k_launch);
   if (k < 0 || k >= T) return;  // idle slot, or a slot in its final energy-only pass
-  double alpha_k = alpha_launch;
+  double alpha_k = alpha_launch;  // (set below in slot mode)
   if (a.kslot) {  // the same product the batch loop forms on the host, so slot and batch results agree bitwise
     alpha_k = a.alpha0;
     for (int i = 0; i < k; ++i) alpha_k *= a.decay;
   }
-  const bool frozen = a.st[4 * s + 0] != 0;
+  const bool frozen = k > 0 && a.st[4 * s + 0] != 0;  // a frozen state repeats iteration k - 1
   const bool warm_before = a.st[4 * s + 2] != 0;
   const size_t cur = ((size_t)s * T + k) * N;
   if (frozen) {
@@ -1498,15 +1499,22 @@ __device__ void finalize_system(const Dev1& a, int s) {
 
 __global__ void k_finalize(Dev1 a) { finalize_system(a, blockIdx.x); }
 
-// slot mode: the slots a new request was placed in start at iteration 0 with a cold state
+// slot mode: the slots a new request was placed in start at iteration 0 with a cold state; idle
+// slots (never admitted, or finished) are held with no pending energy and no frozen flag, whatever
+// the workspace held before (the caller's workspace is not cleared)
 __global__ void k_slot_begin(Dev1 a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.B || !a.admit[s]) return;
-  a.st[4 * s + 0] = 0;
-  a.st[4 * s + 1] = -1;
-  a.st[4 * s + 2] = 0;
-  a.st[4 * s + 3] = 0;
-  a.kslot[s] = 0;
+  if (s >= a.B) return;
+  if (a.admit[s]) {
+    a.st[4 * s + 0] = 0;
+    a.st[4 * s + 1] = -1;
+    a.st[4 * s + 2] = 0;
+    a.st[4 * s + 3] = 0;
+    a.kslot[s] = 0;
+  } else if (a.kslot[s] < 0 || a.kslot[s] > a.T) {
+    a.st[4 * s + 0] = 0;
+    a.st[4 * s + 1] = -1;
+  }
 }
 
 // slot mode, end of a step: a slot that ran its energy-only pass (iteration T) is finalised and

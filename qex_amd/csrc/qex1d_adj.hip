@@ -649,7 +649,9 @@ int adj_check(const qex1d_problem* p) {
   if (p->batch < 1 || p->num_grids < 5 || p->num_grids > QEX1D_MAX_GRIDS || !(p->dx > 0.0) || p->num_iterations < 1 ||
       p->max_electrons < 1 || p->max_electrons > QEX1D_MAX_ELECTRONS || p->num_mixing_iterations < 1)
     return faila(QEX_E_BADARG, "bad problem sizes");
-  if (p->loop != QEX1D_LOOP_SCAN) return faila(QEX_E_UNSUPPORTED, "the reverse pass follows the scan loop");
+  // with density_mse_converge_tolerance < 0 (checked below) the two loop kinds differ only in the
+  // reflection window, which the reverse pass takes per system
+  if (p->loop != QEX1D_LOOP_SCAN && p->loop != QEX1D_LOOP_PYTHON) return faila(QEX_E_BADARG, "bad loop kind");
   if (!(p->density_mse_converge_tolerance < 0.0))
     return faila(QEX_E_UNSUPPORTED, "the reverse pass needs density_mse_converge_tolerance < 0 (no frozen states)");
   if (p->xc_kind != QEX1D_XC_LOCAL_MLP) return faila(QEX_E_UNSUPPORTED, "the reverse pass needs a LOCAL_MLP functional");

@@ -33,6 +33,8 @@ thread_local char g_err3[512] = "";
 
 // the SCF methods whose XC is a local density approximation evaluated in the density epilogue
 __host__ __device__ inline bool is_lda(int method) { return method == QEX3D_RKS_LDA || method == QEX3D_RKS_LDA_X; }
+// methods whose XC energy is a per-point sum, rho w e_xc, accumulated with the Vxc partials
+__host__ __device__ inline bool has_point_exc(int method) { return is_lda(method) || method == QEX3D_RKS_NN_LOCAL; }
 
 int fail3(int code, const char* fmt, ...) {
   va_list ap;
@@ -1131,7 +1133,7 @@ __device__ __forceinline__ void gnn_vjp_body(int bid, int nblk, long ng, int h1,
     double v = 0.0;
     for (int c = lane; c < h1; c += 64) v = fma(row[c], sdz[c], v);
     v = wsum(v);
-    if (lane == 0) wv[g] = 0.5 * w[g] * v;
+    if (lane == 0) wv[g] = w ? 0.5 * w[g] * v : v;  // without weights: vrho itself
   }
 }
 
@@ -1150,6 +1152,81 @@ void gnn_dispatch(long ng, int h1, int h2, double scale, const double* P, const 
                      excsum);
   hipLaunchKernelGGL(k_gnn_vjp, dim3((unsigned)gnn_vjp_blocks(ng)), dim3(NT), (size_t)h1 * 8, st, ng, h1, P + o.W1,
                      dz1, w, wv);
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-point ("NN", local) neural XC: exc_g = -scale swish(y), y = MLP(rho_g) (Dense(h1) gelu Dense(h2)
+// gelu Dense(1) on the scalar rho_g), vrho_g = d exc_g / d rho_g.  Workgroup: NNL_P points, 16 threads
+// per point.  LDS: the layer-1 activations of the tile, then (reusing them) W1_u gelu'(z1_u); the
+// layer-2 cotangents W3_j gelu'(z2_j).  Per point 2 h1 h2 FMAs forward and 2 h1 h2 backward.
+// ---------------------------------------------------------------------------------------------
+constexpr int NNL_P = 16;
+constexpr int NNL_NT = NNL_P * 16;
+__host__ __device__ inline size_t nn_local_lds(int h1, int h2) { return (size_t)NNL_P * ((h1 + 1) + (h2 + 1)) * 8; }
+__host__ __device__ inline long nn_local_blocks(long ng) { return (ng + NNL_P - 1) / NNL_P; }
+
+__device__ void nn_local_body(long bid, long ng, int h1, int h2, double scale, const double* __restrict__ P,
+                              const double* __restrict__ rho, const double* __restrict__ w, double* wv, double* ex) {
+  extern __shared__ double sm[];
+  const int S1 = h1 + 1, S2 = h2 + 1;
+  double* a1 = sm;                     // NNL_P x S1: gelu(z1), then W1 gelu'(z1)
+  double* g2 = a1 + (size_t)NNL_P * S1;  // NNL_P x S2: W3 gelu'(z2)
+  const double* W1 = P;
+  const double* b1 = W1 + h1;
+  const double* W2 = b1 + h1;
+  const double* b2 = W2 + (size_t)h1 * h2;
+  const double* W3 = b2 + h2;
+  const double b3 = W3[h2];
+  const int tid = threadIdx.x, p = tid >> 4, q = tid & 15;
+  const long g = bid * NNL_P + p;
+  const bool valid = g < ng;
+  const double x = valid ? rho[g] : 0.0;
+  for (int u = q; u < h1; u += 16) a1[p * S1 + u] = gelu_t(fma(x, W1[u], b1[u]));
+  __syncthreads();
+  double outp = 0.0;
+  for (int j = q; j < h2; j += 16) {
+    double z = b2[j];
+    for (int u = 0; u < h1; ++u) z = fma(a1[p * S1 + u], W2[(size_t)u * h2 + j], z);
+    outp = fma(gelu_t(z), W3[j], outp);
+    g2[p * S2 + j] = W3[j] * gelu_t_grad(z);
+  }
+  outp += __shfl_xor(outp, 1, 64);
+  outp += __shfl_xor(outp, 2, 64);
+  outp += __shfl_xor(outp, 4, 64);
+  outp += __shfl_xor(outp, 8, 64);
+  __syncthreads();  // every thread of the tile is done with a1
+  for (int u = q; u < h1; u += 16) a1[p * S1 + u] = W1[u] * gelu_t_grad(fma(x, W1[u], b1[u]));
+  __syncthreads();
+  // d y / d rho = sum_u W1_u gelu'(z1_u) sum_j W2[u][j] W3_j gelu'(z2_j); this thread sums its j's
+  double acc = 0.0;
+  for (int u = 0; u < h1; ++u) {
+    double t = 0.0;
+    for (int j = q; j < h2; j += 16) t = fma(W2[(size_t)u * h2 + j], g2[p * S2 + j], t);
+    acc = fma(t, a1[p * S1 + u], acc);
+  }
+  acc += __shfl_xor(acc, 1, 64);
+  acc += __shfl_xor(acc, 2, 64);
+  acc += __shfl_xor(acc, 4, 64);
+  acc += __shfl_xor(acc, 8, 64);
+  if (q == 0 && valid) {
+    const double y = outp + b3;
+    const double sg = 1.0 / (1.0 + exp(-y));
+    const double exc = -scale * y * sg;
+    const double dexc = -scale * sg * (1.0 + y * (1.0 - sg));
+    if (w) {  // the nr_rks inputs: w vrho / 2 and rho w exc
+      const double ww = w[g];
+      wv[g] = 0.5 * ww * (dexc * acc);
+      ex[g] = x * ww * exc;
+    } else {  // eval_xc: vrho and exc themselves
+      wv[g] = dexc * acc;
+      ex[g] = exc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NNL_NT) void k_nn_local(long ng, int h1, int h2, double scale, const double* P,
+                                                     const double* rho, const double* w, double* wv, double* ex) {
+  nn_local_body(blockIdx.x, ng, h1, h2, scale, P, rho, w, wv, ex);
 }
 
 __global__ void k_copy1(const double* src, double* dst) {
@@ -1529,7 +1606,7 @@ __global__ __launch_bounds__(NT_SCF) void k_scf_energy(const Scf3* __restrict__ 
   }
   e1 = bsum(e1, red);
   e2 = bsum(e2, red);
-  const double etot = e1 + e2 + (is_lda(s.method) ? s.scal[3] : s.method == QEX3D_RKS_NN_GLOBAL ? s.scal[5] : 0.0) +
+  const double etot = e1 + e2 + (has_point_exc(s.method) ? s.scal[3] : s.method == QEX3D_RKS_NN_GLOBAL ? s.scal[5] : 0.0) +
                       s.enuc;
   if (cycle < 0) {
     if (threadIdx.x == 0) {
@@ -1650,10 +1727,16 @@ __global__ __launch_bounds__(NT) void k_rho_b(const Scf3* __restrict__ ss) {
     rho_occ_body(blockIdx.x, s.ng, s.n, s.nocc, s.ao, s.C, s.rho, nullptr, nullptr, nullptr, QEX3D_LDA_SVWN5);
 }
 
+__global__ __launch_bounds__(NNL_NT) void k_nn_local_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.method != QEX3D_RKS_NN_LOCAL || s.ist[0] || (long)blockIdx.x >= nn_local_blocks(s.ng)) return;
+  nn_local_body(blockIdx.x, s.ng, s.h1, s.h2, s.nn_scale, s.nn, s.rho, s.w, s.wv, s.ex);
+}
+
 __global__ __launch_bounds__(NT_VXC) void k_vxc_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
-  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.wv, is_lda(s.method) ? s.ex : nullptr,
+  vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.wv, has_point_exc(s.method) ? s.ex : nullptr,
            s.vpart, s.spart);
 }
 
@@ -2085,6 +2168,84 @@ int qex3d_nr_rks_nn(int64_t ngrid, int32_t nao, const double* ao, const double* 
   return launch_status("qex3d_nr_rks_nn");
 }
 
+int qex3d_nr_rks_nn_local(int64_t ngrid, int32_t nao, const double* ao, const double* weights, const double* dm,
+                          const double* nn_params, int32_t h1, int32_t h2, double scale, double* nelec_excsum,
+                          double* vmat, void* workspace, size_t workspace_bytes, void* stream) {
+  if (ngrid < 1 || nao < 1 || nao > QEX3D_MAX_NAO || !ao || !weights || !dm || !nn_params || !nelec_excsum || !vmat)
+    return fail3(QEX_E_BADARG, "bad nr_rks_nn_local arguments");
+  if (h1 < 1 || h2 < 1 || h1 > QEX3D_MAX_NN_LOCAL_HIDDEN || h2 > QEX3D_MAX_NN_LOCAL_HIDDEN)
+    return fail3(QEX_E_BADARG, "hidden widths must be in [1, %d]", QEX3D_MAX_NN_LOCAL_HIDDEN);
+  const size_t need = qex3d_nr_rks_workspace_bytes(ngrid, nao);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = vxc_blocks(ngrid);
+  double* rho = (double*)workspace;
+  double* vpart = (double*)((char*)workspace + al((size_t)ngrid * 8));
+  double* spart = (double*)((char*)vpart + al((size_t)nb * nao * nao * 8));
+  double* wv = (double*)((char*)spart + al((size_t)nb * 16) + al((size_t)nao * nao * 8));
+  double* ex = (double*)((char*)wv + al((size_t)ngrid * 8));
+  {
+    qexprof::Scope ps(qexprof::K_RHO, st);
+    rho_dispatch(ngrid, nao, ao, dm, rho, st);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_NN, st);
+    const size_t lds = nn_local_lds(h1, h2);
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_nn_local, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_nn_local, dim3((unsigned)nn_local_blocks(ngrid)), dim3(NNL_NT), lds, st, (long)ngrid, h1, h2,
+                       scale, nn_params, rho, weights, wv, ex);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_VXC, st);
+    hipLaunchKernelGGL(k_vxc, dim3((unsigned)nb), dim3(NT_VXC), vxc_lds(nao), st, (long)ngrid, nao, ao, weights, rho,
+                       wv, ex, vpart, spart);
+  }
+  {
+    qexprof::Scope ps(qexprof::K_OTHER, st);
+    double* P = (double*)((char*)spart + al((size_t)nb * 16));
+    vxc_reduce(nao, nb, vpart, spart, P, vmat, nelec_excsum, st);
+  }
+  return launch_status("qex3d_nr_rks_nn_local");
+}
+
+size_t qex3d_eval_xc_nn_workspace_bytes(int64_t ngrid, int32_t h1, int32_t h2, int32_t is_global) {
+  if (ngrid < 1 || h1 < 1 || h2 < 1) return 0;
+  if (!is_global) return 256;
+  if (!nn_widths_ok(h1, h2)) return 0;
+  return al((size_t)gnn_blocks(ngrid) * h1 * 8) + 2 * al((size_t)h1 * 8);
+}
+
+int qex3d_eval_xc_nn(int32_t is_global, int64_t ngrid, const double* rho, const double* nn_params, int32_t h1,
+                     int32_t h2, double scale, double* exc, double* vrho, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  if (ngrid < 1 || !rho || !nn_params || !exc || !vrho) return fail3(QEX_E_BADARG, "bad eval_xc_nn arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (!is_global) {
+    if (h1 < 1 || h2 < 1 || h1 > QEX3D_MAX_NN_LOCAL_HIDDEN || h2 > QEX3D_MAX_NN_LOCAL_HIDDEN)
+      return fail3(QEX_E_BADARG, "hidden widths must be in [1, %d]", QEX3D_MAX_NN_LOCAL_HIDDEN);
+    qexprof::Scope ps(qexprof::K_NN, st);
+    const size_t lds = nn_local_lds(h1, h2);
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_nn_local, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_nn_local, dim3((unsigned)nn_local_blocks(ngrid)), dim3(NNL_NT), lds, st, (long)ngrid, h1, h2,
+                       scale, nn_params, rho, (const double*)nullptr, vrho, exc);
+    return launch_status("qex3d_eval_xc_nn");
+  }
+  if (!nn_widths_ok(h1, h2)) return fail3(QEX_E_BADARG, "hidden widths must be in [1, %d]", QEX3D_MAX_NN_HIDDEN);
+  const size_t need = qex3d_eval_xc_nn_workspace_bytes(ngrid, h1, h2, is_global);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  char* q = (char*)workspace;
+  double* zpart = (double*)q;
+  q += al((size_t)gnn_blocks(ngrid) * h1 * 8);
+  double* z1 = (double*)q;
+  q += al((size_t)h1 * 8);
+  double* dz1 = (double*)q;
+  qexprof::Scope ps(qexprof::K_NN, st);
+  gnn_dispatch(ngrid, h1, h2, scale, nn_params, rho, nullptr, zpart, z1, dz1, exc, vrho, st);
+  return launch_status("qex3d_eval_xc_nn");
+}
+
 size_t qex3d_gen_eigh_workspace_bytes(int32_t batch, int32_t n) {
   if (batch < 1 || n < 1 || n > QEX3D_MAX_NAO) return 0;
   return al((size_t)batch * 8 * n * n * 8);
@@ -2112,8 +2273,14 @@ static int scf_check(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, con
   if (n < 1 || n > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "%snao must be in [1, %d]", pre, QEX3D_MAX_NAO);
   if (p->nelectron < 2 || (p->nelectron & 1) || p->nelectron / 2 > n)
     return fail3(QEX_E_BADARG, "%sclosed-shell nelectron required (got %d)", pre, p->nelectron);
-  if (!is_lda(p->method) && p->method != QEX3D_RHF && p->method != QEX3D_RKS_NN_GLOBAL)
+  if (!is_lda(p->method) && p->method != QEX3D_RHF && p->method != QEX3D_RKS_NN_GLOBAL &&
+      p->method != QEX3D_RKS_NN_LOCAL)
     return fail3(QEX_E_BADARG, "%sbad method", pre);
+  if (p->method == QEX3D_RKS_NN_LOCAL &&
+      (!in->nn_params || p->nn_h1 < 1 || p->nn_h2 < 1 || p->nn_h1 > QEX3D_MAX_NN_LOCAL_HIDDEN ||
+       p->nn_h2 > QEX3D_MAX_NN_LOCAL_HIDDEN))
+    return fail3(QEX_E_BADARG, "%snetwork parameters and hidden widths in [1, %d] required", pre,
+                 QEX3D_MAX_NN_LOCAL_HIDDEN);
   if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->ao || !in->weights))
     return fail3(QEX_E_BADARG, "%sRKS needs a grid", pre);
   if (p->method == QEX3D_RKS_NN_GLOBAL && (!in->nn_params || !nn_widths_ok(p->nn_h1, p->nn_h2)))
@@ -2149,7 +2316,9 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   std::vector<Scf3> rec(B);
   int nmax = 1, max_cycle = 0, sync_every = 0, npair_max = 1, h1_max = 1, h2_max = 1;
   long rho_blk = 0, vxc_blk = 0, sum_blk = 0, sym_blk = 0, j_blk = 1, l1_blk = 0, red_blk = 0, vjp_blk = 0;
-  bool any_grid = false, any_nn = false;
+  bool any_grid = false, any_nn = false, any_nnl = false;
+  long nnl_blk = 0;
+  size_t lds_nnl = 0;
   for (int b = 0; b < B; ++b) {
     const qex3d_scf_problem* p = ps + b;
     const int n = p->nao;
@@ -2199,7 +2368,7 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     s.ist = (int*)(w + L.ist);
     s.e_hist = outs[b].e_tot;
     s.nn = ins[b].nn_params;
-    s.h1 = scf_h1(p);
+    s.h1 = p->method == QEX3D_RKS_NN_LOCAL ? p->nn_h1 : scf_h1(p);
     s.h2 = p->nn_h2;
     s.nn_scale = p->nn_scale;
     s.zpart = (double*)(w + L.zpart);
@@ -2220,6 +2389,11 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
       sum_blk = vxc_sum_blocks(n * n) > sum_blk ? vxc_sum_blocks(n * n) : sum_blk;
       sym_blk = (n * n + 255) / 256 > sym_blk ? (n * n + 255) / 256 : sym_blk;
     }
+    if (p->method == QEX3D_RKS_NN_LOCAL) {
+      any_nnl = true;
+      nnl_blk = nn_local_blocks(ng) > nnl_blk ? nn_local_blocks(ng) : nnl_blk;
+      lds_nnl = nn_local_lds(p->nn_h1, p->nn_h2) > lds_nnl ? nn_local_lds(p->nn_h1, p->nn_h2) : lds_nnl;
+    }
     if (p->method == QEX3D_RKS_NN_GLOBAL) {
       any_nn = true;
       l1_blk = gnn_blocks(ng) > l1_blk ? gnn_blocks(ng) : l1_blk;
@@ -2234,6 +2408,8 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   if (lds_eigh + sizeof(qexeig::Smem) > 160 * 1024) return fail3(QEX_E_UNSUPPORTED, "nao too large for the LDS eigensolver");
   if (lds_eigh > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_scf_eigh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_eigh);
+  if (lds_nnl > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_nn_local_b, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_nnl);
   for (int b0 = 0; b0 < B; b0 += PUT_N) {
     ScfPut pk{};
     const int cnt = B - b0 < PUT_N ? B - b0 : PUT_N;
@@ -2246,6 +2422,10 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
       {
         qexprof::Scope ps_(qexprof::K_RHO, st);
         hipLaunchKernelGGL(k_rho_b, dim3((unsigned)rho_blk, nb), dim3(NT), rho_lds(nmax), st, dss);
+      }
+      if (any_nnl) {
+        qexprof::Scope ps_(qexprof::K_NN, st);
+        hipLaunchKernelGGL(k_nn_local_b, dim3((unsigned)nnl_blk, nb), dim3(NNL_NT), lds_nnl, st, dss);
       }
       if (any_nn) {
         qexprof::Scope ps_(qexprof::K_NN, st);

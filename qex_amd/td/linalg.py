@@ -9,20 +9,9 @@ from .. import _lib
 from ..od import engine
 
 
-def generalized_eigh(A, B):
-    """A, B: [..., n, n] (B symmetric positive definite).  Returns (w [..., n] ascending,
-    v [..., n, n] with eigenvectors in the columns, v^T B v = 1).  A is symmetrised; the
-    reference's diagonal shift for a numerically singular B is not applied (B must be SPD)."""
-    dev = engine.device()
-    A = engine.as_f64(A, dev)
-    B = engine.as_f64(B, dev)
-    if A.shape != B.shape or A.dim() < 2 or A.shape[-1] != A.shape[-2]:
-        raise ValueError("A and B must be [..., n, n] of the same shape")
-    n = A.shape[-1]
-    lead = A.shape[:-2]
-    Af = A.reshape(-1, n, n).contiguous()
-    Bf = B.reshape(-1, n, n).contiguous()
-    batch = Af.shape[0]
+def _device_gen_eigh(Af, Bf):
+    batch, n = Af.shape[0], Af.shape[-1]
+    dev = Af.device
     lib = _lib.load()
     ws = engine.workspace(lib.qex3d_gen_eigh_workspace_bytes(batch, n), dev)
     w = torch.empty((batch, n), dtype=torch.float64, device=dev)
@@ -30,4 +19,40 @@ def generalized_eigh(A, B):
     p = engine._ptr
     _lib.check3(lib.qex3d_gen_eigh(batch, n, p(Af), p(Bf), p(w), p(v), p(ws), ws.numel(), engine._stream()),
                 "qex3d_gen_eigh")
+    return w, v
+
+
+def generalized_eigh(A, B, *, eps=1.0e-12, scale=False, dtype=torch.float64):
+    """A, B: [..., n, n] real symmetric, B (numerically) positive definite.  Returns (w [..., n]
+    ascending, v [..., n, n] with eigenvectors in the columns, v^T B v = 1), as the reference does:
+
+    * A and B are symmetrised;
+    * scale=True applies the Jacobi scaling s = sqrt(diag B) to both (the eigenvectors are then those
+      of the scaled problem, as in the reference);
+    * B is shifted by (eps - lambda_min(B)) I when its smallest eigenvalue is below eps (lambda_min
+      from the device solve of B v = lambda v);
+    * Cholesky B = L L^T, the symmetric eigenproblem of L^-1 A L^-T, and v = L^-T U, in one workgroup
+      per matrix on the device."""
+    if dtype != torch.float64:
+        raise ValueError("the device eigensolver works in float64")
+    dev = engine.device()
+    A = engine.as_f64(A, dev)
+    B = engine.as_f64(B, dev)
+    if A.shape != B.shape or A.dim() < 2 or A.shape[-1] != A.shape[-2]:
+        raise ValueError("A and B must be [..., n, n] of the same shape")
+    n = A.shape[-1]
+    lead = A.shape[:-2]
+    Af = A.reshape(-1, n, n)
+    Bf = B.reshape(-1, n, n)
+    Af = (0.5 * (Af + Af.transpose(1, 2))).contiguous()
+    Bf = (0.5 * (Bf + Bf.transpose(1, 2))).contiguous()
+    if scale:
+        si = 1.0 / torch.sqrt(torch.diagonal(Bf, dim1=1, dim2=2))
+        Af = (si[:, :, None] * Af * si[:, None, :]).contiguous()
+        Bf = (si[:, :, None] * Bf * si[:, None, :]).contiguous()
+    eye = torch.eye(n, dtype=torch.float64, device=dev).expand(Bf.shape[0], n, n).contiguous()
+    lam_min = _device_gen_eigh(Bf, eye)[0][:, 0]
+    shift = torch.where(lam_min < eps, eps - lam_min, torch.zeros_like(lam_min))
+    Bf = (Bf + shift[:, None, None] * eye).contiguous()
+    w, v = _device_gen_eigh(Af, Bf)
     return w.reshape(*lead, n), v.reshape(*lead, n, n)

@@ -63,10 +63,12 @@ class NumInt:
     def __init__(self):
         self.network = None
 
-    def define_xc_(self, network, xctype="NN-AmplitudeEncoding"):
-        if xctype != "NN-AmplitudeEncoding":
-            raise NotImplementedError("the device path implements the amplitude-encoded (global) network")
-        self.network = network
+    def define_xc_(self, description=None, xctype="NN-AmplitudeEncoding", network=None, **kwargs):
+        """``description``: a network (``GlobalNNXC`` / ``LocalNNXC``) or, as the reference passes it,
+        ``functools.partial(qex_amd.td.xc.eval_xc, network=..., is_global_xc=...)``."""
+        from .xc import network_of
+
+        self.network = network_of(description if description is not None else network, xctype)
         return self
 
 
@@ -76,9 +78,14 @@ def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000,
     Slater + VWN5 for "lda,vwn" and its aliases), or the network installed with
     ``ni.define_xc_`` (NN-AmplitudeEncoding branch, td/numint.py:206-231), whose parameters may be
     replaced by ``params``."""
+    from .xc import LocalNNXC
+
     net = getattr(ni, "network", None)
     if net is not None:
-        return _nr_rks_nn(net if params is None else net.with_params(params), mol, grids, dms, hermi, ao)
+        net = net if params is None else net.with_params(params)
+        if isinstance(net, LocalNNXC):
+            return _nr_rks_nn_local(net, mol, grids, dms, hermi, ao)
+        return _nr_rks_nn(net, mol, grids, dms, hermi, ao)
     functional = lda_functional(xc_code)
     dev = engine.device()
     if ao is None:
@@ -95,6 +102,26 @@ def nr_rks(ni, mol, grids, xc_code, dms, relativity=0, hermi=0, max_memory=2000,
                                      engine._ptr(engine.as_f64(grids.weights, dev)), engine._ptr(dm),
                                      engine._ptr(out2), engine._ptr(vmat), engine._ptr(ws), ws.numel(),
                                      engine._stream()), "qex3d_nr_rks_lda")
+    return out2[0], out2[1], vmat
+
+
+def _nr_rks_nn_local(net, mol, grids, dms, hermi, ao):
+    """The "NN" branch (td/numint.py:179-204): per-point exc and vrho = d exc / d rho on the device."""
+    dev = engine.device()
+    if ao is None:
+        ao = eval_ao(mol, grids.coords)
+    dm = engine.as_f64(dms, dev)
+    if hermi == 0:
+        dm = (0.5 * (dm + dm.T)).contiguous()
+    ng, nao = ao.shape
+    lib = _lib.load()
+    ws = engine.workspace(lib.qex3d_nr_rks_workspace_bytes(ng, nao), dev)
+    out2 = torch.empty(2, dtype=torch.float64, device=dev)
+    vmat = torch.empty((nao, nao), dtype=torch.float64, device=dev)
+    p = engine._ptr
+    _lib.check3(lib.qex3d_nr_rks_nn_local(ng, nao, p(ao), p(engine.as_f64(grids.weights, dev)), p(dm), p(net.packed()),
+                                          net.h1, net.h2, net.scale, p(out2), p(vmat), p(ws), ws.numel(),
+                                          engine._stream()), "qex3d_nr_rks_nn_local")
     return out2[0], out2[1], vmat
 
 

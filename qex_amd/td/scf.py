@@ -65,7 +65,9 @@ class _SCF:
         method = self.method
         net = getattr(self, "network", None)
         if method in (_lib.QEX3D_RKS_LDA, _lib.QEX3D_RKS_LDA_X) and net is not None:
-            method = _lib.QEX3D_RKS_NN_GLOBAL
+            from .xc import LocalNNXC
+
+            method = _lib.QEX3D_RKS_NN_LOCAL if isinstance(net, LocalNNXC) else _lib.QEX3D_RKS_NN_GLOBAL
             if kwargs.get("params") is not None:
                 net = net.with_params(kwargs["params"])
         if method != _lib.QEX3D_RHF:
@@ -81,13 +83,15 @@ class _SCF:
             if net.n_grid != ng:
                 raise ValueError(f"network input width {net.n_grid} != grid size {ng}")
             p.nn_h1, p.nn_h2, p.nn_scale = net.h1, net.h2, net.scale
+        if method == _lib.QEX3D_RKS_NN_LOCAL:
+            p.nn_h1, p.nn_h2, p.nn_scale = net.h1, net.h2, net.scale
         e_hist = torch.empty(self.max_cycle + 1, dtype=torch.float64, device=dev)
         mo = torch.empty(n, dtype=torch.float64, device=dev)
         C = torch.empty((n, n), dtype=torch.float64, device=dev)
         D = torch.empty((n, n), dtype=torch.float64, device=dev)
         status = torch.zeros(8, dtype=torch.int32, device=dev)
         ptr = engine._ptr
-        packed = net.packed() if method == _lib.QEX3D_RKS_NN_GLOBAL else None
+        packed = net.packed() if method in (_lib.QEX3D_RKS_NN_GLOBAL, _lib.QEX3D_RKS_NN_LOCAL) else None
         ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w), ptr(packed))
         outs = _lib.Qex3dScfOutputs(ptr(e_hist), ptr(mo), ptr(C), ptr(D), ptr(status))
         self._keep = packed
@@ -105,7 +109,9 @@ class _SCF:
         self._pending = None
         return self.e_tot
 
-    def make_rdm1(self, mo_coeff=None, mo_occ=None):
+    def make_rdm1(self, mo_coeff=None, mo_occ=None, **kwargs):
+        """D = C_occ diag(occ) C_occ^T (``td/hf.py:303-308``); without arguments the converged density
+        matrix.  Extra keywords (``a0_repr=True`` at ``trainer_legacy_no_jit.py:272``) are accepted."""
         if mo_coeff is None:
             return self.dm
         C = engine.as_f64(mo_coeff, engine.device())
@@ -137,12 +143,14 @@ class RKS(_SCF):
         self._xc = code
         self.method = _lib.QEX3D_RKS_LDA_X if fn == _lib.QEX3D_LDA_SLATER else _lib.QEX3D_RKS_LDA
 
-    def define_xc_(self, network, xctype="NN-AmplitudeEncoding"):
-        """Install a global network XC (``trainer_legacy_no_jit.py:256-261``); kernel(params=...)
-        then evaluates it with those parameters."""
-        if xctype != "NN-AmplitudeEncoding":
-            raise NotImplementedError("the device path implements the amplitude-encoded (global) network")
-        self.network = network
+    def define_xc_(self, description=None, xctype="NN-AmplitudeEncoding", network=None, **kwargs):
+        """Install a network XC (``trainer_legacy_no_jit.py:256-261``: ``mf.define_xc_(description=
+        eval_xc_func, xctype="NN-AmplitudeEncoding" | "NN")``); kernel(params=...) then evaluates it
+        with those parameters.  ``description`` is a network (``GlobalNNXC`` / ``LocalNNXC``) or
+        ``functools.partial(qex_amd.td.xc.eval_xc, network=..., is_global_xc=...)``."""
+        from .xc import network_of
+
+        self.network = network_of(description if description is not None else network, xctype)
         return self
 
 

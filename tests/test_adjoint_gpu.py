@@ -157,3 +157,59 @@ def test_reflected_gradient_matches_oracle(mods, output, window, act):
         go += A.backward(fw, dE / B, dn / B)
     scale = np.abs(go).max()
     np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * scale)
+
+
+def test_python_loop_off_centre_reflection_gradient_matches_oracle(mods):
+    """scf.kohn_sham's loop (od/scf.py:434-560; reflection about the centre of the nuclei) with a
+    negative convergence tolerance: the device reverse pass against the oracle's with that flip."""
+    engine, train, utils, xc = mods
+    grids = np.linspace(-12.8, 12.8, 257)
+    dx = float(grids[1] - grids[0])
+    systems = [(np.array([0.3, 1.7]), np.array([1.0, 1.0]), 2), (np.array([-2.2, -0.6]), np.array([1.0, 2.0]), 3)]
+    params = _params(11, [5, 12, 12, 1])
+    T, alpha, decay = 5, 0.5, 0.9
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in systems])
+    f = xc.LocalMLPXC(params, window=5, activation="tanh", norm=2.0, output="negativity")
+    prob = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [s[2] for s in systems],
+                            utils.exponential_coulomb, f, T, alpha=alpha, alpha_decay=decay, num_mixing_iterations=1,
+                            loop="python", want_orbitals=True, enforce_reflection_symmetry=True)
+    tE, tn = np.array([-1.1, -2.3]), 0.9 * n0
+    _, g, res = train.value_and_grad(prob, n0, train.trajectory_loss, tE, tn, 2, dx, start=1)
+    g = g.cpu().numpy()
+    oxc = O.LocalMLP([w for w, _ in params], [b for _, b in params], "tanh", 5, 2.0, output="negativity")
+    go = np.zeros_like(g)
+    for s, (loc, z, ne) in enumerate(systems):
+        flip = lambda a, loc=loc: O.flip_and_average(loc, grids, a)
+        fw = A.forward_trajectory(loc, z, ne, T, grids, oxc, O.exponential_coulomb, n0[s], alpha, decay, 1,
+                                  reflect=True, flip=flip)
+        np.testing.assert_allclose(res.total_energy[s].cpu().numpy(), fw["energies"], atol=1e-8)
+        _, dE, dn = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], tE[s], tn[s], 2, dx, start=1)
+        go += A.backward(fw, dE / 2, dn / 2)
+    np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * np.abs(go).max())
+
+
+def test_stop_gradient_step_matches_oracle(mods):
+    """stop_gradient_step (jit_scf.py:199-204 via train.py:113): the device reverse pass with the
+    first iterations held constant against the oracle's (itself pinned by differences of the
+    restarted run)."""
+    engine, train, utils, xc = mods
+    grids = np.linspace(-12.8, 12.8, 257)
+    dx = float(grids[1] - grids[0])
+    systems = _systems()
+    params = _params(13, [9, 12, 12, 1])
+    T, s_stop = 7, 3
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in systems])
+    f = xc.LocalMLPXC(params, window=9, activation="silu", norm=2.0)
+    prob = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [s[2] for s in systems],
+                            utils.exponential_coulomb, f, T, alpha=0.4, alpha_decay=0.9, num_mixing_iterations=1,
+                            loop="scan", want_orbitals=True, stop_gradient_step=s_stop)
+    tE, tn = np.array([-1.2, -2.5, -4.0]), 0.9 * n0
+    _, g, _ = train.value_and_grad(prob, n0, train.trajectory_loss, tE, tn, 3, dx, start=0)
+    oxc = O.LocalMLP([w for w, _ in params], [b for _, b in params], "silu", 9, 2.0)
+    go = 0
+    for s, (loc, z, ne) in enumerate(systems):
+        fw = A.forward_trajectory(loc, z, ne, T, grids, oxc, O.exponential_coulomb, n0[s], 0.4, 0.9, 1)
+        _, dE, dn = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], tE[s], tn[s], 3, dx, start=0)
+        go = go + A.backward(fw, dE / 3, dn / 3, stop_gradient_step=s_stop)
+    g = g.cpu().numpy()
+    np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * np.abs(go).max())

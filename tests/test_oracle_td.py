@@ -217,3 +217,31 @@ def test_nn_amplitude_vmat_is_density_matrix_derivative_with_unit_weights():
     rho = T.eval_rho(ao, D)
     _, vrho = net.exc_vrho(rho)
     np.testing.assert_allclose(vw, ao.T @ (ao * (w * vrho)[:, None]), atol=1e-12)
+
+
+def test_local_network_vrho_is_the_pointwise_derivative():
+    """exc_and_vrho_local: per point exc(rho) and d exc / d rho (central differences, each point
+    independent of the others), with the reference's nr_rks "NN" assembly."""
+    rng = np.random.default_rng(3)
+    W = [rng.standard_normal((1, 12)), rng.standard_normal((12, 7)) / 3, rng.standard_normal((7, 1)) / 2]
+    b = [rng.standard_normal(12) * 0.1, rng.standard_normal(7) * 0.1, rng.standard_normal(1) * 0.1]
+    net = T.LocalMLP3D(W, b, scale=0.3)
+    rho = rng.uniform(0, 2, 50)
+    exc, vrho = net.exc_vrho(rho)
+    h = 1e-6
+    np.testing.assert_allclose(vrho, (net.exc_vrho(rho + h)[0] - net.exc_vrho(rho - h)[0]) / (2 * h), atol=1e-8)
+    # a point's value does not depend on the other points
+    r2 = rho.copy()
+    r2[7] += 0.3
+    np.testing.assert_array_equal(net.exc_vrho(r2)[0][:7], exc[:7])
+    # one point of the batch equals the same network on that point alone
+    assert net.exc_vrho(rho[3:4])[0][0] == pytest.approx(exc[3], abs=1e-15)
+
+
+def test_local_network_scf_runs_and_uses_the_pointwise_derivative():
+    mol = T.synthetic_chain(2, 4)
+    pts, w = T.molecular_grid(mol, n_rad=20, n_theta=6, n_phi=7)
+    W, b = T.init_global_mlp(2, 1, hidden=(16, 8))
+    net = T.LocalMLP3D(W, b, scale=0.5)
+    res = T.scf(mol, "rks_nn_local", grid=(pts, w), net=net, conv_tol=1e-10, max_cycle=60)
+    assert res.converged and np.isfinite(res.e_tot)

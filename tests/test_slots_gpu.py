@@ -30,6 +30,8 @@ def test_slot_results_equal_solo_solves(mods, decay, slots):
     f = xc.LocalMLPXC(list(zip(W, b)), window=9, activation="silu", norm=2.0)
     run = engine.SlotRunner(grids, utils.exponential_coulomb, f, T, slots=slots, alpha=alpha, alpha_decay=decay,
                             num_mixing_iterations=1, loop="scan", depth=2)
+    # the workspace is not cleared by the library: idle slots must stay idle whatever it holds
+    run.ws.fill_(0x5A)
     # staggered admissions: 3 now, more after two steps, the rest as slots free up
     pending = list(range(len(mols)))
     results = {}

@@ -2,7 +2,8 @@
 
 Integrals, grid, AO values, density and LDA potential matrix are compared elementwise; SCF runs
 are compared by total energy (1e-8 Ha), density matrix and orbital energies.  H2/6-31G RHF
-energies are the values printed in the reference notebook qedft/data/td/hydrogen_rhf.ipynb.
+energies are the values printed in notebooks/03_notebook_td_dataset.ipynb:40 and
+notebooks/04_notebook_td_trainer.ipynb (cell 5).
 """
 
 import numpy as np
@@ -85,11 +86,13 @@ def test_eval_ao_rho_nr_rks(qtd, n_atoms):
     rho = qtd.numint.eval_rho(mol, ao, dm).cpu().numpy()
     orho = otd.eval_rho(oao, dm)
     np.testing.assert_allclose(rho, orho, rtol=1e-11, atol=1e-13 * np.abs(orho).max())
-    nel, exc, vmat = qtd.numint.nr_rks(qtd.numint.NumInt(), mol, grids, "lda", dm)
-    onel, oexc, ovmat = otd.nr_rks_lda(oao, w, dm)
-    assert abs(float(nel) - onel) < 1e-10 * max(1, abs(onel))
-    assert abs(float(exc) - oexc) < 1e-10 * max(1, abs(oexc))
-    np.testing.assert_allclose(vmat.cpu().numpy(), ovmat, rtol=0, atol=1e-10 * np.abs(ovmat).max())
+    # pyscf's "lda,vwn" (Slater + VWN5) and bare "lda" (Slater exchange alone)
+    for code, fn in (("lda,vwn", "svwn5"), ("lda", "slater")):
+        nel, exc, vmat = qtd.numint.nr_rks(qtd.numint.NumInt(), mol, grids, code, dm)
+        onel, oexc, ovmat = otd.nr_rks_lda(oao, w, dm, fn)
+        assert abs(float(nel) - onel) < 1e-10 * max(1, abs(onel))
+        assert abs(float(exc) - oexc) < 1e-10 * max(1, abs(oexc))
+        np.testing.assert_allclose(vmat.cpu().numpy(), ovmat, rtol=0, atol=1e-10 * np.abs(ovmat).max())
 
 
 def test_eval_rho_nonsymmetric_dm_is_symmetrised(qtd):
@@ -218,3 +221,100 @@ def test_rks_global_network_scf_matches_oracle(qtd):
     ref2 = otd.scf(omol, "rks_nn", grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
                    conv_tol=1e-10, max_cycle=50, net=otd.GlobalMLP3D(W2, b2, scale=0.05))
     assert abs(e2 - ref2.e_tot) < 1e-8
+
+
+def _local_nets(hidden, seed):
+    from qex_amd.td.xc import LocalNNXC
+    W, b = otd.init_global_mlp(seed, 1, hidden=hidden)
+    rng = np.random.default_rng(seed + 2)
+    b = [rng.standard_normal(x.shape) * 0.1 for x in b]
+    return LocalNNXC(W, b, scale=0.2), otd.LocalMLP3D(W, b, scale=0.2)
+
+
+@pytest.mark.parametrize("hidden", [(16, 8), (512, 512)])
+def test_nr_rks_local_network_matches_oracle(qtd, hidden):
+    """The "NN" branch (td/numint.py:179-204) with the per-point network on the device, installed the
+    way the reference's trainer does (define_xc_(description=partial(eval_xc, network=...),
+    xctype="NN"))."""
+    import functools
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    grids = qtd.dft.Grids(mol).build()
+    w = grids.weights.cpu().numpy()
+    ao = qtd.numint.eval_ao(mol, grids.coords)
+    net, onet = _local_nets(hidden, 5)
+    dm = _random_dm(mol.nao, 6)
+    desc = functools.partial(qtd.xc.eval_xc, network=net, is_global_xc=False)
+    ni = qtd.numint.NumInt().define_xc_(description=desc, xctype="NN")
+    nel, exc, vmat = qtd.numint.nr_rks(ni, mol, grids, "NN", dm)
+    onel, oexc, ovmat = otd.nr_rks_nn_local(ao.cpu().numpy(), w, dm, onet)
+    assert abs(float(nel) - onel) < 1e-10 * abs(onel)
+    assert abs(float(exc) - oexc) < 1e-11 * max(1.0, abs(oexc))
+    np.testing.assert_allclose(vmat.cpu().numpy(), ovmat, rtol=0, atol=1e-10 * np.abs(ovmat).max())
+    # eval_xc itself: per-point exc and vrho
+    rho = qtd.numint.eval_rho(mol, ao, dm)
+    e, (v, _, _, _), _, _ = qtd.xc.eval_xc("NN", rho, network=net, is_global_xc=False)
+    oe, ov = onet.exc_vrho(rho.cpu().numpy())
+    np.testing.assert_allclose(e.cpu().numpy(), oe, rtol=0, atol=1e-12 * np.abs(oe).max())
+    np.testing.assert_allclose(v.cpu().numpy(), ov, rtol=0, atol=1e-12 * np.abs(ov).max())
+
+
+def test_eval_xc_global_network_matches_oracle(qtd):
+    mol, _ = _mol_pair(qtd, "chain", 2)
+    grids = qtd.dft.Grids(mol).build()
+    net, onet = _nets(grids.size, (64, 32), 4)
+    rho = torch.rand(grids.size, dtype=torch.float64, device="cuda") * 0.3
+    e, (v, _, _, _), _, _ = qtd.xc.eval_xc("NN-AmplitudeEncoding", rho, network=net, is_global_xc=True)
+    oe, ov = onet.exc_vrho(rho.cpu().numpy())
+    assert abs(float(e) - oe) < 1e-11 * max(1.0, abs(oe))
+    np.testing.assert_allclose(v.cpu().numpy(), ov, rtol=0, atol=1e-12 * np.abs(ov).max())
+
+
+def test_rks_local_network_scf_matches_oracle(qtd):
+    import functools
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    mf = qtd.scf.RKS(mol)
+    mf.grids = qtd.dft.Grids(mol).build()
+    net, onet = _local_nets((32, 16), 7)
+    mf = mf.define_xc_(description=functools.partial(qtd.xc.eval_xc, network=net, is_global_xc=False), xctype="NN")
+    mf.conv_tol, mf.max_cycle = 1e-10, 60
+    e = mf.kernel()
+    S, H, g = mf.integrals()
+    oS, oT, oV = otd.one_electron(omol)
+    grid = (mf.grids.coords.cpu().numpy(), mf.grids.weights.cpu().numpy())
+    ref = otd.scf(omol, "rks_nn_local", grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
+                  conv_tol=1e-10, max_cycle=60, net=onet)
+    assert ref.converged and mf.converged
+    assert abs(e - ref.e_tot) < 1e-8
+    np.testing.assert_allclose(mf.make_rdm1(a0_repr=True).cpu().numpy(), ref.dm, atol=1e-6)
+    # kernel(params=...) replaces the parameters, as the trainer's batch_loss_fn does
+    W2, b2 = otd.init_global_mlp(8, 1, hidden=(32, 16))
+    e2 = mf.kernel(params=list(zip(W2, b2)))
+    ref2 = otd.scf(omol, "rks_nn_local", grid=grid, ints=(oS, oT, oV, _pairs_to_full(g.cpu().numpy(), mol.nao)),
+                   conv_tol=1e-10, max_cycle=60, net=otd.LocalMLP3D(W2, b2, scale=0.2))
+    assert abs(e2 - ref2.e_tot) < 1e-8
+
+
+def test_generalized_eigh_shifts_a_singular_metric(qtd):
+    """generalized_eigh (generalized_eigensolver.py:264-330): symmetrised inputs, B shifted by
+    eps - lambda_min(B) when lambda_min(B) < eps, optional Jacobi scaling; against scipy on the
+    same shifted problem."""
+    import scipy.linalg
+    rng = np.random.default_rng(5)
+    n = 12
+    X = rng.standard_normal((n, n - 2))
+    B = X @ X.T  # rank n - 2: two zero eigenvalues
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w, v = qtd.linalg.generalized_eigh(A, B, eps=1e-6)
+    Bs = B + (1e-6 - np.linalg.eigvalsh(B)[0]) * np.eye(n)
+    ref = scipy.linalg.eigh(A, Bs, eigvals_only=True)
+    np.testing.assert_allclose(w.cpu().numpy(), ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+    V = v.cpu().numpy()
+    np.testing.assert_allclose(V.T @ Bs @ V, np.eye(n), atol=1e-6)
+    # a well-conditioned B is not shifted; Jacobi scaling leaves the eigenvalues unchanged
+    S = X @ X.T + np.eye(n)
+    w1, _ = qtd.linalg.generalized_eigh(A, S)
+    w2, _ = qtd.linalg.generalized_eigh(A, S, scale=True)
+    ref = scipy.linalg.eigh(A, S, eigvals_only=True)
+    np.testing.assert_allclose(w1.cpu().numpy(), ref, atol=1e-10 * np.abs(ref).max())
+    np.testing.assert_allclose(w2.cpu().numpy(), ref, atol=1e-9 * np.abs(ref).max())

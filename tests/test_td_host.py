@@ -40,3 +40,18 @@ def test_ao_layout_order():
     mol = gto.synthetic_chain(2, 1)
     assert mol.nao == 10
     assert [l for _, l, _, _ in mol.shells] == [0, 0, 1, 0, 0, 1]
+
+
+def test_eval_xc_checks_the_network_kind_before_the_device():
+    """is_global_xc must name the network's kind (a per-point network is local, the amplitude-encoding
+    one global); a mismatch is refused on the host."""
+    from qex_amd.td import xc
+
+    W, b = xc.init_local_mlp(0, hidden=(4, 4))
+    local = xc.LocalNNXC(W, b)
+    Wg, bg = xc.init_global_mlp(0, 6, hidden=(4, 4))
+    glob = xc.GlobalNNXC(Wg, bg)
+    with pytest.raises(ValueError, match="is_global_xc"):
+        xc.eval_xc("NN", np.ones(6), network=local, is_global_xc=True)
+    with pytest.raises(ValueError, match="is_global_xc"):
+        xc.eval_xc("NN-AmplitudeEncoding", np.ones(6), network=glob, is_global_xc=False)

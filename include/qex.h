@@ -165,6 +165,20 @@ int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int3
                                double* wavefunctions, void* workspace, size_t workspace_bytes,
                                void* stream);
 
+/* Allocation-free variant of qex1d_solve_noninteracting (same arguments, no workspace).  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops that build initial densities per request).  The
+ * workspace lives in a fixed device array of QEX1D_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a problem whose qex1d_noninteracting_workspace_bytes() exceeds it is refused with
+ * QEX_E_WORKSPACE (for N = 513 and N_e <= 8 that is up to 800 systems per call).  Calls on one GPU
+ * share that array: calls that overlap in time on the same GPU (different streams or host threads)
+ * race and must be serialised by the caller. */
+#define QEX1D_NOALLOC_SCRATCH_BYTES (16u << 20)
+int qex1d_solve_noninteracting_noalloc(int32_t batch, int32_t num_grids, double dx, int32_t max_electrons,
+                                       const double* potential, const int32_t* num_electrons, double* density,
+                                       double* total_eigen_energies, double* gap, double* eigenvalues,
+                                       double* wavefunctions, void* stream);
+
 /* v_ext[b][i] = -sum_a Z[b][a] u(x_i - R[b][a]) on grids[N].
  * Replaces jax_dft utils.get_atomic_chain_potential (od/scf.py:490-495). */
 int qex1d_external_potential(int32_t batch, int32_t num_grids, const double* grids, int32_t num_nuclei,

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libqex.so")
 QEX_OK = 0
 QEX_E_BADARG = -1
 QEX_E_WORKSPACE = -2
+QEX1D_NOALLOC_SCRATCH_BYTES = 16 << 20
 QEX1D_MAX_GRIDS = 1024
 QEX1D_MAX_ELECTRONS = 62
 QEX1D_MAX_MLP_LAYERS = 6
@@ -142,6 +143,7 @@ SIGNATURES = {
                               ctypes.POINTER(Qex1dOutputs), _P, _P, _P, _S, _P]),
     "qex1d_noninteracting_workspace_bytes": (_S, [_I, _I, _I]),
     "qex1d_solve_noninteracting": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
+    "qex1d_solve_noninteracting_noalloc": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "qex1d_external_potential": (_I, [_I, _I, _P, _I, _P, _P, _I, _D, _D, _P, _P]),
     "qex1d_hartree_potential": (_I, [_I, _I, _D, _P, _P, _P, _P]),
     "qex_profile_enable": (_I, [_I, _I]),

@@ -16,6 +16,7 @@
 // k_finalize assembles the total energies after the loop.  Reductions run in a fixed order
 // (wave butterflies + ordered cross-wave sums), so results do not depend on the batch.
 #include <hip/hip_runtime.h>
+#include <mutex>
 
 #include <math.h>
 #include <stdarg.h>
@@ -2043,6 +2044,60 @@ int qex1d_solve_noninteracting(int32_t batch, int32_t num_grids, double dx, int3
                      e.Tb, e.Tc, potential, num_electrons, e.vec, density, total_eigen_energies, gap, eigenvalues,
                      wavefunctions);
   return check_launch("qex1d_solve_noninteracting");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free non-interacting solve: the same kernels as qex1d_solve_noninteracting, with the
+// workspace in a fixed device array of QEX1D_NOALLOC_SCRATCH_BYTES.  Each GPU holds its own copy;
+// its address is looked up once per device (hipGetSymbolAddress) and kept here.  Calls that overlap
+// on one GPU share the array and race; the kernels overwrite every scratch byte they read
+// (cold-start inverse iteration, no state carried between calls), so nothing is cleared per call.
+// ---------------------------------------------------------------------------------------------
+namespace {
+__device__ __align__(256) unsigned char scratch[QEX1D_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// device address of this GPU's scratch array (the host-side symbol is only a stand-in)
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(QEX_E_LAUNCH, "no current HIP device (or device index >= %d)", kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr)
+      return fail(QEX_E_LAUNCH, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(err));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return QEX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int qex1d_solve_noninteracting_noalloc(int32_t batch, int32_t num_grids, double dx, int32_t max_electrons,
+                                       const double* potential, const int32_t* num_electrons, double* density,
+                                       double* total_eigen_energies, double* gap, double* eigenvalues,
+                                       double* wavefunctions, void* stream) {
+  const size_t need = qex1d_noninteracting_workspace_bytes(batch, num_grids, max_electrons);
+  if (need == 0) return fail(QEX_E_BADARG, "bad sizes (batch %d, num_grids %d, max_electrons %d)", batch, num_grids,
+                             max_electrons);
+  if (need > (size_t)QEX1D_NOALLOC_SCRATCH_BYTES)
+    return fail(QEX_E_WORKSPACE,
+                "batch %d x %d grid points x %d orbitals needs %zu scratch bytes; the allocation-free entry point "
+                "holds %zu (use qex1d_solve_noninteracting with a workspace, or split the batch)",
+                batch, num_grids, max_electrons / 2 + 1, need, (size_t)QEX1D_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != QEX_OK) return rc;
+  return qex1d_solve_noninteracting(batch, num_grids, dx, max_electrons, potential, num_electrons, density,
+                                    total_eigen_energies, gap, eigenvalues, wavefunctions, ws, need, stream);
 }
 
 int qex1d_external_potential(int32_t batch, int32_t num_grids, const double* grids, int32_t num_nuclei,

@@ -110,6 +110,8 @@ def parse_args():
                     help="launch one 64 MiB device-to-device copy first (reference for HBM counters)")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c5")
     ap.add_argument("--slots", type=int, default=128, help="c5: 1D slots of the continuous-batching table")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="c5 served path: 'mixed' = 3D grid work on FP32 matrix cores (FP64 elsewhere), or FP64")
     ap.add_argument("--rate", type=float, default=50000.0,
                     help="c5: Poisson arrival rate of the timed stream (req/s; above capacity by default)")
     ap.add_argument("--rates", type=str, default="50,200,500,1000,2000",
@@ -647,10 +649,13 @@ def _work_1d(kind, b_ne):
     return 0.0
 
 
-def _work_3d(kind, nao, ngrid, cycles):
+def _work_3d(kind, nao, ngrid, cycles, nocc=0, fused=False):
     """Work of one molecule's launches of ``kind`` over its SCF (guess + cycles): FLOPs, or bytes for
-    the Coulomb kernel."""
+    the Coulomb kernel.  ``fused``: the FP32 grid kernel (rho from the occupied orbitals, 2 ng n nocc,
+    and the Vxc contraction, 2 ng n^2, in one pass)."""
     n, c = nao, cycles + 1
+    if kind == "vxc" and fused:
+        return c * (2.0 * ngrid * n * n + 2.0 * ngrid * n * nocc)
     if kind in ("rho", "vxc"):
         return c * 2.0 * ngrid * n * n
     if kind == "scf_eigh":
@@ -671,14 +676,17 @@ ROOF = {  # kind -> (bound, peak, unit)
 }
 
 
-def _stream_work(stats, kind):
+ROOF_FP32_GRID = {"vxc": ("mfma", 157.3, "TFLOP/s")}  # the fused FP32 grid kernel: dense FP32 MFMA peak
+
+
+def _stream_work(stats, kind, fused=False):
     tot = 0.0
     if kind in ("eigen", "mlp", "potential"):
         for b_ne, T in stats.work_1d:
             tot += T * _work_1d(kind, b_ne)
     else:
-        for nao, ng, cyc in stats.work_3d:
-            tot += _work_3d(kind, nao, ng, cyc)
+        for nao, ng, cyc, nocc in stats.work_3d:
+            tot += _work_3d(kind, nao, ng, cyc, nocc, fused)
     return tot
 
 
@@ -725,10 +733,14 @@ def run_c5(args):
     for m in mols:
         m.device_basis()
 
-    def solver_3d(m):
+    def solver_3d(m, precision="fp64"):
         mf = tscf.RKS(m)
         mf.conv_tol, mf.max_cycle, mf.diis_space, mf.sync_every = 1e-8, 30, 8, 0
+        mf.precision = precision
         return mf
+
+    mixed = args.precision == "mixed"
+    served_3d = (lambda m: solver_3d(m, "fp32_grid")) if mixed else solver_3d
 
     def requests(lo, hi, arrivals):
         out = []
@@ -747,7 +759,7 @@ def run_c5(args):
             return e_ref is not None and abs(c.energy - e_ref) <= 1e-6
         return c.converged and abs(c.energy - ref3[C4_CHAINS[pick_3d[c.rid]]]["e_tot"]) <= 1e-6
 
-    server = serve.Server(slots, solver_3d, batch_3d=args.batch_3d, streams_3d=args.streams_3d,
+    server = serve.Server(slots, served_3d, batch_3d=args.batch_3d, streams_3d=args.streams_3d,
                           depth_3d=args.depth_3d)
     # ---- warm-up: the first W steps of the stream ----
     if nwarm:
@@ -788,6 +800,9 @@ def run_c5(args):
     value = n_good / elapsed
     summ = st.summary()
     same = sum(1 for c in st.completions if c.rid in plain_e and c.energy == plain_e[c.rid])
+    # served (mixed) against the plain FP64 path on the requests both solved
+    dplain = {k: [abs(c.energy - plain_e[c.rid]) for c in st.completions if c.rid in plain_e and c.kind == k]
+              for k in ("1d", "3d")}
     dE1 = [abs(c.energy - ref1[int(pick_1d[c.rid])]) for c in st.completions
            if c.kind == "1d" and int(pick_1d[c.rid]) in ref1]
     dE3 = [abs(c.energy - ref3[C4_CHAINS[pick_3d[c.rid]]]["e_tot"]) for c in st.completions if c.kind == "3d"]
@@ -803,8 +818,9 @@ def run_c5(args):
                                 "p99_ms_3d": round(s2.get("p99_ms_3d", float("nan")), 2),
                                 "converged": sum(1 for c in sw.completions if good(c))}
     # ---- roofline of the dominant kernel: algorithmic work of the timed launches / their summed time ----
-    bound, peak, unit = ROOF[dom]
-    work = _stream_work(st, dom)
+    fused = mixed and dom in ROOF_FP32_GRID
+    bound, peak, unit = ROOF_FP32_GRID[dom] if fused else ROOF[dom]
+    work = _stream_work(st, dom, fused)
     scale = 1e9 if unit == "GB/s" else 1e12
     achieved = work / (dom_ms * 1e-3) / scale if dom_ms > 0 else None
     traffic = None
@@ -823,7 +839,7 @@ def run_c5(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded 1D systems and LocalMLP weights, seeded hydrogen chains, Poisson arrivals)",
         "config": {"workload": f"c5 served mixed stream: {REQ_PER_STEP} requests per step, 70% 1D (c2 distribution: "
                                "513-pt grid, LocalMLP-17 [64,64] SiLU, 20 iterations) / 30% 3D RKS-LDA chains "
@@ -831,14 +847,17 @@ def run_c5(args):
                                f"{args.rate:g} req/s (above the served capacity: the stream is backlogged)",
                    "requests_timed": ntimed, "arrival_rate": args.rate, "slots_1d": args.slots,
                    "max_batch_3d": args.batch_3d, "streams_3d": args.streams_3d, "depth_3d": args.depth_3d,
-                   "path": "served: continuous batching, FP64"},
+                   "path": ("served: continuous batching; 3D AO values stored in FP32, rho from them with FP64 sums, "
+                            "Vxc contraction on FP32 matrix cores with FP64 sums across 16-point units, everything "
+                            "else FP64; plain path FP64") if mixed else "served: continuous batching, FP64"},
         "latency_ms": {"p50": summ["p50_ms"], "p99": summ["p99_ms"], "p50_1d": summ.get("p50_ms_1d"),
                        "p99_1d": summ.get("p99_ms_1d"), "p50_3d": summ.get("p50_ms_3d"),
                        "p99_3d": summ.get("p99_ms_3d")},
         "parity": {"converged_within_1e-6": n_good, "of": len(st.completions),
                    "max_abs_dE_1d_vs_oracle": float(max(dE1)) if dE1 else None,
                    "max_abs_dE_3d_vs_golden": float(max(dE3)) if dE3 else None,
-                   "bitwise_equal_to_plain": f"{same} of {nplain}"},
+                   "bitwise_equal_to_plain": f"{same} of {nplain}",
+                   "max_abs_dE_vs_plain_fp64": {k: (float(max(v)) if v else None) for k, v in dplain.items()}},
         "plain_one_at_a_time": {"throughput_per_s": plain_rate, "requests": nplain, "wall_s": plain.wall},
         "served_over_plain": value / plain_rate,
         "arrival_sweep": sweep,

@@ -238,6 +238,12 @@ const char* qex1d_vjp_last_error(void);
 #define QEX3D_LDA_SVWN5 0  /* Slater exchange + VWN5 correlation (pyscf "lda,vwn") */
 #define QEX3D_LDA_SLATER 1 /* Slater exchange alone (pyscf "lda") */
 #define QEX3D_MAX_NN_HIDDEN 1024
+/* Precision of the SCF's grid work (qex3d_scf_problem.precision) */
+#define QEX3D_PRECISION_FP64 0     /* every stage in FP64 */
+#define QEX3D_PRECISION_FP32_GRID 1 /* AO values stored in FP32; rho from them on FP64 matrix cores (FP64
+                                      sums); the Vxc contraction on FP32 matrix cores with FP32 sums over
+                                      16-point units and FP64 sums across units; the XC function,
+                                      energies, Coulomb, Fock, DIIS and eigensolve in FP64 */
 
 /* Basis on the device.  Shells are listed in AO order (an s shell is 1 AO, a p shell 3: x, y, z);
  * coefficients include the primitive normalisation and a normalised contraction. */
@@ -274,6 +280,14 @@ int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const do
 
 /* ao[g][mu] on ngrid points.  Replaces pyscf numint.eval_ao (td/eval_gto.py:20-70). */
 int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, double* ao, void* stream);
+
+/* The same AO values rounded to FP32 (computed in FP64) in the layout of the FP32 grid path:
+ * ao32[QEX3D_AO32_ROWS(ngrid)][QEX3D_AO32_LD(nao)], columns past nao and rows past ngrid zero.
+ * Input of qex3d_scf / qex3d_scf_batch with precision QEX3D_PRECISION_FP32_GRID (the mixed-precision
+ * form of the eval_ao call above). */
+#define QEX3D_AO32_LD(nao) ((((nao) + 15) / 16) * 16)
+#define QEX3D_AO32_ROWS(ngrid) ((((ngrid) + 63) / 64) * 64)
+int qex3d_eval_ao_f32(const qex3d_basis* b, int64_t ngrid, const double* coords, float* ao32, void* stream);
 
 /* rho_g = sum_{mu nu} ao[g][mu] dm[mu][nu] ao[g][nu].  Replaces eval_rho
  * (qedft/train/td/numint_legacy.py:351-373). */
@@ -344,6 +358,7 @@ typedef struct qex3d_scf_problem {
                          stops enqueueing once converged */
   int32_t nn_h1, nn_h2; /* QEX3D_RKS_NN_GLOBAL: hidden widths */
   double nn_scale;
+  int32_t precision; /* QEX3D_PRECISION_FP64 (0) or QEX3D_PRECISION_FP32_GRID (RKS methods) */
 } qex3d_scf_problem;
 
 typedef struct qex3d_scf_inputs {
@@ -353,6 +368,7 @@ typedef struct qex3d_scf_inputs {
   const double* ao;      /* [ngrid][nao] (RKS) */
   const double* weights; /* [ngrid] (RKS) */
   const double* nn_params; /* packed network (QEX3D_RKS_NN_GLOBAL) */
+  const float* ao_f32;     /* QEX3D_PRECISION_FP32_GRID: qex3d_eval_ao_f32 layout (ao may then be NULL) */
 } qex3d_scf_inputs;
 
 typedef struct qex3d_scf_outputs {

@@ -101,6 +101,18 @@ QEX3D_MAX_NN_LOCAL_HIDDEN = 512
 QEX3D_LDA_SVWN5 = 0
 QEX3D_LDA_SLATER = 1
 QEX3D_MAX_NN_HIDDEN = 1024
+QEX3D_PRECISION_FP64 = 0
+QEX3D_PRECISION_FP32_GRID = 1
+
+
+def ao32_ld(nao):
+    """Row length of the FP32 AO layout (QEX3D_AO32_LD)."""
+    return (nao + 15) // 16 * 16
+
+
+def ao32_rows(ngrid):
+    """Row count of the FP32 AO layout (QEX3D_AO32_ROWS)."""
+    return (ngrid + 63) // 64 * 64
 QEX3D_MAX_NAO = 96
 QEX3D_MAX_NPRIM = 8
 QEX3D_MAX_SCF_BATCH = 1024
@@ -117,11 +129,11 @@ class Qex3dScfProblem(ctypes.Structure):
     _fields_ = [("nao", ctypes.c_int32), ("nelectron", ctypes.c_int32), ("method", ctypes.c_int32),
                 ("max_cycle", ctypes.c_int32), ("diis_space", ctypes.c_int32), ("ngrid", ctypes.c_int64),
                 ("conv_tol", ctypes.c_double), ("energy_nuc", ctypes.c_double), ("sync_every", ctypes.c_int32), ("nn_h1", ctypes.c_int32),
-                ("nn_h2", ctypes.c_int32), ("nn_scale", ctypes.c_double)]
+                ("nn_h2", ctypes.c_int32), ("nn_scale", ctypes.c_double), ("precision", ctypes.c_int32)]
 
 
 class Qex3dScfInputs(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("S", "H", "eri", "ao", "weights", "nn_params")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("S", "H", "eri", "ao", "weights", "nn_params", "ao_f32")]
 
 
 class Qex3dScfOutputs(ctypes.Structure):
@@ -159,6 +171,7 @@ SIGNATURES = {
     "qex3d_eri": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _S, _P]),
     "qex3d_grid": (_I, [ctypes.POINTER(Qex3dBasis), _I, _P, _P, _I, _P, _P, _P, _P, _P]),
     "qex3d_eval_ao": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
+    "qex3d_eval_ao_f32": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
     "qex3d_eval_rho": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P]),
     "qex3d_nr_rks_workspace_bytes": (_S, [ctypes.c_int64, _I]),
     "qex3d_nr_rks": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _P, _P, _S, _P]),

@@ -675,6 +675,38 @@ __global__ void k_eval_ao(Basis b, long ng, const double* coords, double* ao) {
   }
 }
 
+// The same values rounded to FP32 in the padded layout of the FP32 grid path: rows of ld floats
+// (ld = nao rounded up to 16), rows past ng and columns past nao zero.  Thread per (row, shell);
+// the last shell's thread of a row also clears the row's padding columns.
+__global__ void k_eval_ao32(Basis b, long ng, long rows, int ld, const double* coords, float* ao) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * b.nshell) return;
+  const long g = idx / b.nshell;
+  const int s = (int)(idx % b.nshell);
+  int mu0 = 0;
+  for (int t = 0; t < s; ++t) mu0 += b.sh_l[t] == 0 ? 1 : 3;
+  const int nc = b.sh_l[s] == 0 ? 1 : 3;
+  float* row = ao + g * ld;
+  if (s == b.nshell - 1)
+    for (int c = b.nao; c < ld; ++c) row[c] = 0.0f;
+  if (g >= ng) {
+    for (int c = 0; c < nc; ++c) row[mu0 + c] = 0.0f;
+    return;
+  }
+  const double* C = b.xyz + 3 * b.sh_atom[s];
+  const double dx = coords[3 * g] - C[0], dy = coords[3 * g + 1] - C[1], dz = coords[3 * g + 2] - C[2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double rad = 0.0;
+  for (int q = 0; q < b.sh_nprim[s]; ++q) rad += b.coefs[b.sh_off[s] + q] * exp(-b.exps[b.sh_off[s] + q] * r2);
+  if (nc == 1) {
+    row[mu0] = (float)rad;
+  } else {
+    row[mu0] = (float)(rad * dx);
+    row[mu0 + 1] = (float)(rad * dy);
+    row[mu0 + 2] = (float)(rad * dz);
+  }
+}
+
 // rho_g = phi_g^T D phi_g for a tile of 64 points: P = Phi_tile D on FP64 matrix cores
 // (v_mfma_f64_16x16x4f64; lane l supplies A[l%16][l/16], B[l/16][l%16], receives D[4v+l/16][l%16]),
 // then rho = rowsum(Phi_tile * P).  The AO tile is staged in LDS (row stride npad + 1).
@@ -986,6 +1018,19 @@ void rho_dispatch(long ng, int n, const double* ao, const double* dm, double* rh
 __host__ __device__ inline int vxc_blocks(long ng) {
   const long nchunk = (ng + CH - 1) / CH;
   return (int)(nchunk < VXC_BLOCKS ? nchunk : VXC_BLOCKS);
+}
+
+// FP32 grid path sizes: AO row length, upper 16x16 tiles of the n x n Vxc, workgroups per molecule
+// (4 waves, each taking 16-point units in turn; at most X32_BLOCKS so the partials stay small)
+constexpr int X32_BLOCKS = 128;
+__host__ __device__ inline int x32_ld(int n) { return (n + 15) & ~15; }
+__host__ __device__ inline int x32_tiles(int n) {
+  const int nb = x32_ld(n) / 16;
+  return nb * (nb + 1) / 2;
+}
+__host__ __device__ inline int x32_blocks(long ng) {
+  const long wg = (ng + 63) / 64;
+  return (int)(wg < X32_BLOCKS ? wg : X32_BLOCKS);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1407,6 +1452,8 @@ struct Scf3 {
   const double* eri;
   const double* ao;
   const double* w;
+  const float* ao32;  // QEX3D_PRECISION_FP32_GRID: AO values [rows][x32_ld(n)]
+  int prec, x32blk;   // grid precision; workgroups of the FP32 grid kernels for this molecule
   // state (workspace)
   double *F, *Fd, *D, *Dlast, *C, *Cp, *mo, *J, *K, *Vxc, *vpart, *spart, *rho, *tmp1, *tmp2;
   double *diisF, *diisE, *diisB;  // [space][n*n], [space][n*n], [(space+1)^2]
@@ -1719,7 +1766,7 @@ __global__ void k_put_scf(ScfPut pk, int count, Scf3* dst) {
 // the SCF's density on the grid from its occupied orbitals (D = 2 C_occ C_occ^T by construction)
 __global__ __launch_bounds__(NT) void k_rho_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
-  if (s.method == QEX3D_RHF || s.ist[0] || (long)blockIdx.x >= (s.ng + 63) / 64) return;
+  if (s.method == QEX3D_RHF || s.prec || s.ist[0] || (long)blockIdx.x >= (s.ng + 63) / 64) return;
   if (is_lda(s.method))
     rho_occ_body(blockIdx.x, s.ng, s.n, s.nocc, s.ao, s.C, s.rho, s.w, s.wv, s.ex,
                  s.method == QEX3D_RKS_LDA_X ? QEX3D_LDA_SLATER : QEX3D_LDA_SVWN5);
@@ -1735,7 +1782,7 @@ __global__ __launch_bounds__(NNL_NT) void k_nn_local_b(const Scf3* __restrict__ 
 
 __global__ __launch_bounds__(NT_VXC) void k_vxc_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
-  if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
+  if (s.method == QEX3D_RHF || s.prec || s.ist[0] || (int)blockIdx.x >= s.nchunk) return;
   vxc_body(blockIdx.x, s.nchunk, s.ng, s.n, s.ao, s.w, s.rho, s.wv, has_point_exc(s.method) ? s.ex : nullptr,
            s.vpart, s.spart);
 }
@@ -1800,7 +1847,8 @@ Lay3 layout3(int n, long ng, int space, int h1 = 0) {
   L.J = take(nn);
   L.K = take(nn);
   L.Vxc = take(nn);
-  L.vpart = take((size_t)nchunk * nn);
+  const size_t v64 = (size_t)nchunk * nn, v32 = (size_t)x32_blocks(ng) * x32_tiles(n) * 256 * 8;
+  L.vpart = take(v64 > v32 ? v64 : v32);
   L.spart = take((size_t)nchunk * 16);
   L.rho = take((size_t)ng * 8);
   L.tmp1 = take(nn);
@@ -1908,7 +1956,7 @@ __global__ __launch_bounds__(256) void k_vxc_sum(int nn, int nchunk, const doubl
 __global__ __launch_bounds__(256) void k_vxc_sum_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   const int nn = s.n * s.n, nb = vxc_sum_blocks(nn);
-  if (s.method == QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= nb) return;
+  if (s.method == QEX3D_RHF || s.prec || s.ist[0] || (int)blockIdx.x >= nb) return;
   vxc_sum_body(blockIdx.x, nb, nn, s.nchunk, s.vpart, s.spart, s.tmp2, s.scal + 2);
 }
 
@@ -1923,8 +1971,312 @@ __global__ void k_sym_add(int n, const double* P, double* vmat) {
 
 __global__ __launch_bounds__(256) void k_sym_add_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
-  if (s.method == QEX3D_RHF || s.ist[0]) return;
+  if (s.method == QEX3D_RHF || s.prec || s.ist[0]) return;
   sym_add_body(blockIdx.x * blockDim.x + threadIdx.x, s.n, s.tmp2, s.Vxc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP32 grid path (QEX3D_PRECISION_FP32_GRID).  The AO values are FP32 rows of x32_ld(n) floats
+// (rows padded to a multiple of 64).  Each wave of a workgroup works alone on 16-point units
+// (unit u = points 16u..16u+15; the units of wave w of workgroup b are b*4+w, then every
+// 4*x32blk-th after it):
+//   1. the unit's 16 x ld AO block goes HBM -> registers (issued one unit ahead) -> the wave's LDS
+//      rows (row stride ld + 4 floats);
+//   2. rho_p = 2 sum_o (phi_p . C_o)^2 on FP64 matrix cores (FP32 AO values, FP64 C_occ from LDS and
+//      FP64 sums), the K dimension permuted so that a lane reads 4 consecutive AO values of its point
+//      in one 16-byte LDS read;
+//   3. the LDA (or the given w v) at the 16 points in FP64;
+//   4. the unit's rank-16 update of the upper 16x16 tiles of V = phi^T diag(w v) phi on f32 MFMA.
+// A wave sums its units in FP32; the 4 waves are summed in FP64 in a fixed order into the
+// workgroup's partial, and k_x32_sum_b sums the partials in FP64 in a fixed order.
+// f32 MFMA lane layout: A[l % 16][l / 16], B[l / 16][l % 16], D[4 (l / 16) + v][l % 16].
+// ---------------------------------------------------------------------------------------------
+typedef float f4_t __attribute__((ext_vector_type(4)));
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;  // pointer into global memory
+__device__ __forceinline__ f4_t mfma32(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+constexpr int NT_X32 = 256;
+enum { X32_FUSED = 0, X32_RHO = 1, X32_V = 2 };  // LDA in one pass; rho only; V from the given w v
+// dynamic LDS: the 4 waves' 16-row AO blocks (row stride ld + 4 floats), then C_occ in FP64
+// (ld rows of 16 * ceil(nocc / 16) columns, zero padded)
+size_t x32_tile_bytes(int n) { return (size_t)64 * (x32_ld(n) + 4) * 4; }
+size_t x32_lds(int n, int nocc) { return x32_tile_bytes(n) + (size_t)x32_ld(n) * 16 * ((nocc + 15) / 16) * 8; }
+
+// Order of the wave's LDS writes and reads among its own lanes.  The LDS executes one wave's
+// instructions in issue order, so only the compiler must be kept from moving LDS accesses across
+// this point; no wait is needed (a workgroup-scope fence here would also wait for the next unit's
+// global loads in flight, serialising every unit on HBM latency).
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int MODE, int NB>
+__device__ void x32_body(const Scf3& s, int bid, int nblk) {
+  extern __shared__ double x32_sm[];
+  constexpr int LD = 16 * NB, S = LD + 4, T = NB * (NB + 1) / 2, NV = NB;
+  __shared__ double sred[2][NT_X32 / 64];
+  float* smf = reinterpret_cast<float*>(x32_sm);
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, q = l >> 4, i = l & 15;
+  float* tile = smf + wv * 16 * S;
+  const long ng = s.ng, nunit = (ng + 15) / 16;
+  const int fn = s.method == QEX3D_RKS_LDA_X ? QEX3D_LDA_SLATER : QEX3D_LDA_SVWN5;
+  const int nob = (s.nocc + 15) / 16;
+  const bool point_exc = has_point_exc(s.method);
+  f4_t acc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) acc[t] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
+  double nel = 0.0, exc = 0.0;
+  // C_occ (FP64) into LDS once: Cs[mu][o] for mu < LD, o < 16 nob, zero outside n x nocc
+  double* Cs = reinterpret_cast<double*>(smf + 64 * S);
+  const int CW = 16 * nob;
+  if (MODE != X32_V) {
+    for (int t = threadIdx.x; t < LD * CW; t += NT_X32) {
+      const int mu = t / CW, o = t % CW;
+      Cs[t] = (mu < s.n && o < s.nocc) ? s.C[mu * s.n + o] : 0.0;
+    }
+    __syncthreads();
+  }
+  // global (not generic) pointers: their loads count only against vmcnt, so waits for LDS results
+  // do not also wait for the next unit's loads in flight
+  const gptr<const f4_t> ao4 = (gptr<const f4_t>)s.ao32;
+  const gptr<const double> gw = (gptr<const double>)s.w;
+  const long ustride = (long)nblk * 4;
+  long u = (long)bid * 4 + wv;
+  f4_t pre[NV];
+  if (u < nunit) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pre[k] = ao4[u * 4 * LD + l + 64 * k];
+  }
+  for (; u < nunit; u += ustride) {
+    // 1. stage the unit (float4 e of the 16 x LD block: row e / (LD / 4), column 4 (e % (LD / 4)))
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int e = l + 64 * k;
+      *reinterpret_cast<f4_t*>(tile + (e / (LD / 4)) * S + 4 * (e % (LD / 4))) = pre[k];
+    }
+    const long g = u * 16 + i;  // this lane's point in steps 2-3 (lanes i = 0..15 of each q group alike)
+    // the unit's per-point inputs are loaded before the next unit's AO block, so waiting for them
+    // leaves that prefetch in flight
+    const bool pt = l < 16 && g < ng;
+    const double ww = pt ? gw[g] : 0.0;
+    double rin = 0.0, wvin = 0.0, exin = 0.0;
+    if (MODE == X32_V && pt) {
+      rin = ((gptr<const double>)s.rho)[g];
+      wvin = ((gptr<const double>)s.wv)[g];
+      if (point_exc) exin = ((gptr<const double>)s.ex)[g];
+    }
+    {
+      // unconditional (the last unit reloads itself), so the compiler's wait for the per-point loads
+      // above counts these as still in flight instead of waiting for everything
+      const long un = u + ustride < nunit ? u + ustride : u;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) pre[k] = ao4[un * 4 * LD + l + 64 * k];
+    }
+    wave_lds_sync();
+    double rp = 0.0;
+    if (MODE != X32_V) {
+      // 2. D[p][o] = sum_mu phi[p][mu] C[mu][o] on FP64 matrix cores (the FP32 AO values are exact in
+      //    FP64, so rho is a smooth function of C and the SCF energy carries no rounding noise from
+      //    it): lane (point i, K slot q) supplies phi[i][16 c + 4 q + j] and C[16 c + 4 q + j][16 ob + i]
+      double sq[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int ob = 0; ob < nob; ++ob) {
+        d4_t ra[2] = {(d4_t){0.0, 0.0, 0.0, 0.0}, (d4_t){0.0, 0.0, 0.0, 0.0}};
+#pragma unroll 1
+        for (int c = 0; c < NB; ++c) {
+          const f4_t x = *reinterpret_cast<const f4_t*>(tile + i * S + 16 * c + 4 * q);
+          const double* cb = Cs + (16 * c + 4 * q) * CW + 16 * ob + i;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) ra[j & 1] = mfma64((double)x[j], cb[j * CW], ra[j & 1]);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double d = ra[0][v] + ra[1][v];
+          sq[v] = fma(d, d, sq[v]);
+        }
+      }
+      // FP64 layout D[4 v + l / 16][l % 16]: sum over the 16 orbitals (lanes of the q group) gives
+      // rho / 2 at point 4 v + q
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = sq[v];
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        sq[v] = x;
+      }
+      // lane i takes point i = 4 v + q from lane 16 (i % 4), slot i / 4
+      const int src = 16 * (i & 3);
+      const double t0 = __shfl(sq[0], src, 64), t1 = __shfl(sq[1], src, 64), t2 = __shfl(sq[2], src, 64),
+                   t3 = __shfl(sq[3], src, 64);
+      const int sl = i >> 2;
+      rp = 2.0 * (sl == 0 ? t0 : sl == 1 ? t1 : sl == 2 ? t2 : t3);
+    }
+    // 3. the XC at the unit's points (lanes 0..15)
+    double wvp = 0.0;
+    if (pt) {
+      if (MODE == X32_RHO) {
+        s.rho[g] = rp;
+      } else if (MODE == X32_FUSED) {
+        double e, v;
+        lda_point(fn, rp, &e, &v);
+        wvp = ww * v;
+        nel = fma(rp, ww, nel);
+        exc = fma(rp * ww, e, exc);
+      } else {
+        wvp = 2.0 * wvin;
+        nel = fma(rin, ww, nel);
+        exc += exin;
+      }
+    }
+    if (MODE != X32_RHO) {
+      // 4. V[16 bi + r][16 bj + c] += sum_p (w v)_p phi[p][16 bi + r] phi[p][16 bj + c], bi <= bj,
+      //    4 points per MFMA (K slot q = point 4 ks + q)
+      const float wf = (float)wvp;
+#pragma unroll 1
+      for (int ks = 0; ks < 4; ++ks) {
+        const float wk = __shfl(wf, 4 * ks + q, 64);
+        const float* rowp = tile + (4 * ks + q) * S + i;
+        float a[NB], aw[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          a[c] = rowp[16 * c];
+          aw[c] = a[c] * wk;
+        }
+        int t = 0;
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+          for (int bj = bi; bj < NB; ++bj) {
+            acc[t] = mfma32(aw[bi], a[bj], acc[t]);
+            ++t;
+          }
+      }
+    }
+    wave_lds_sync();  // the next unit's staging overwrites these rows
+  }
+  if (MODE == X32_RHO) return;
+  // the 4 waves' FP32 tiles through the LDS (NB tiles per pass), summed per element in FP64 in wave
+  // order into the workgroup partial
+  __syncthreads();
+  float* red = smf;  // [4 waves][NB tiles][256]
+  double* vp = s.vpart + (size_t)bid * T * 256;
+  constexpr int NP = (T + NB - 1) / NB;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+      if (t / NB == p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[(wv * NB + t - p * NB) * 256 + v * 64 + l] = acc[t][v];
+    __syncthreads();
+    const int cnt = T - p * NB < NB ? T - p * NB : NB;
+    for (int e = threadIdx.x; e < cnt * 256; e += NT_X32) {
+      double x = 0.0;
+#pragma unroll
+      for (int w = 0; w < NT_X32 / 64; ++w) x += (double)red[w * NB * 256 + e];
+      vp[p * NB * 256 + e] = x;
+    }
+    __syncthreads();
+  }
+  const double a = wsum(nel), b = wsum(exc);
+  if (l == 0) {
+    sred[0][wv] = a;
+    sred[1][wv] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+    for (int w = 0; w < NT_X32 / 64; ++w) {
+      x += sred[0][w];
+      y += sred[1][w];
+    }
+    s.spart[2 * bid] = x;
+    s.spart[2 * bid + 1] = y;
+  }
+}
+
+// one kernel per AO block count NB (registers sized for it); molecules of other widths exit
+template <int MODE, int NB>
+__global__ __launch_bounds__(NT_X32) void k_x32_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.prec != QEX3D_PRECISION_FP32_GRID || s.method == QEX3D_RHF || s.ist[0]) return;
+  if ((MODE == X32_FUSED) != is_lda(s.method) || x32_ld(s.n) != 16 * NB) return;
+  if ((int)blockIdx.x >= s.x32blk) return;
+  x32_body<MODE, NB>(s, blockIdx.x, s.x32blk);
+}
+
+template <int MODE, int NB>
+void x32_launch1(const long* blk, const size_t* lds, unsigned nmol, hipStream_t st, const Scf3* dss) {
+  if (lds[NB] > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_x32_b<MODE, NB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds[NB]);
+  hipLaunchKernelGGL((k_x32_b<MODE, NB>), dim3((unsigned)blk[NB], nmol), dim3(NT_X32), lds[NB], st, dss);
+}
+template <int MODE>
+void x32_launch(int nbmask, const long* blk, const size_t* lds, unsigned nmol, hipStream_t st, const Scf3* dss) {
+  if (nbmask & 2) x32_launch1<MODE, 1>(blk, lds, nmol, st, dss);
+  if (nbmask & 4) x32_launch1<MODE, 2>(blk, lds, nmol, st, dss);
+  if (nbmask & 8) x32_launch1<MODE, 3>(blk, lds, nmol, st, dss);
+  if (nbmask & 16) x32_launch1<MODE, 4>(blk, lds, nmol, st, dss);
+  if (nbmask & 32) x32_launch1<MODE, 5>(blk, lds, nmol, st, dss);
+  if (nbmask & 64) x32_launch1<MODE, 6>(blk, lds, nmol, st, dss);
+}
+
+// Vxc = the fixed-order FP64 sum of the workgroup partials (packed upper tiles, element
+// t * 256 + v * 64 + l = tile t's D[4 (l / 16) + v][l % 16]), written to both triangles; the extra
+// last workgroup sums (nelec, exc) as in vxc_sum_body.
+__global__ __launch_bounds__(256) void k_x32_sum_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  if (s.prec != QEX3D_PRECISION_FP32_GRID || s.method == QEX3D_RHF || s.ist[0]) return;
+  const int n = s.n, nb = x32_ld(n) / 16, T = x32_tiles(n), ne = T * 256;
+  const int nblk = vxc_sum_blocks(ne);
+  const int bid = blockIdx.x;
+  if (bid >= nblk) return;
+  __shared__ double red[8][33];
+  __shared__ double s2[2][256];
+  const int t = threadIdx.x, nparts = s.x32blk;
+  if (bid == nblk - 1) {
+    s2[0][t] = t < nparts ? s.spart[2 * t] : 0.0;
+    s2[1][t] = t < nparts ? s.spart[2 * t + 1] : 0.0;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) {
+        s2[0][t] += s2[0][t + h];
+        s2[1][t] += s2[1][t + h];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      s.scal[2] = s2[0][0];
+      s.scal[3] = s2[1][0];
+    }
+    return;
+  }
+  const int e = bid * 32 + (t & 31), part = t >> 5;
+  double a = 0.0;
+  if (e < ne)
+    for (int c = part; c < nparts; c += 8) a += s.vpart[(size_t)c * ne + e];
+  red[part][t & 31] = a;
+  __syncthreads();
+  if (t < 32 && e < ne) {
+    double v = 0.0;
+    for (int k = 0; k < 8; ++k) v += red[k][t];
+    int tt = e >> 8, bi = 0;
+    while (tt >= nb - bi) {
+      tt -= nb - bi;
+      ++bi;
+    }
+    const int bj = bi + tt, ll = e & 63, vv = (e >> 6) & 3;
+    const int row = 16 * bi + 4 * (ll >> 4) + vv, col = 16 * bj + (ll & 15);
+    if (row < n && col < n) {
+      s.Vxc[row * n + col] = v;
+      if (bi != bj) s.Vxc[col * n + row] = v;
+    }
+  }
 }
 
 // Vxc from the k_vxc chunk partials (nchunk <= VXC_BLOCKS = 256); P is n^2 scratch
@@ -2027,6 +2379,19 @@ int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, dou
   hipLaunchKernelGGL(k_eval_ao, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      make_basis(b), (long)ngrid, coords, ao);
   return launch_status("qex3d_eval_ao");
+}
+
+int qex3d_eval_ao_f32(const qex3d_basis* b, int64_t ngrid, const double* coords, float* ao32, void* stream) {
+  int rc = check_basis(b);
+  if (rc) return rc;
+  if (ngrid < 1 || !coords || !ao32) return fail3(QEX_E_BADARG, "bad eval_ao_f32 arguments");
+  if (b->nao > QEX3D_MAX_NAO) return fail3(QEX_E_UNSUPPORTED, "nao must be <= %d", QEX3D_MAX_NAO);
+  const long rows = QEX3D_AO32_ROWS((long)ngrid);
+  const long total = rows * b->nshell;
+  qexprof::Scope ps(qexprof::K_AO, (hipStream_t)stream);
+  hipLaunchKernelGGL(k_eval_ao32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     make_basis(b), (long)ngrid, rows, (int)QEX3D_AO32_LD(b->nao), coords, ao32);
+  return launch_status("qex3d_eval_ao_f32");
 }
 
 int qex3d_eval_rho(int64_t ngrid, int32_t nao, const double* ao, const double* dm, double* rho, void* stream) {
@@ -2281,8 +2646,11 @@ static int scf_check(const qex3d_scf_problem* p, const qex3d_scf_inputs* in, con
        p->nn_h2 > QEX3D_MAX_NN_LOCAL_HIDDEN))
     return fail3(QEX_E_BADARG, "%snetwork parameters and hidden widths in [1, %d] required", pre,
                  QEX3D_MAX_NN_LOCAL_HIDDEN);
-  if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->ao || !in->weights))
-    return fail3(QEX_E_BADARG, "%sRKS needs a grid", pre);
+  if (p->precision != QEX3D_PRECISION_FP64 && p->precision != QEX3D_PRECISION_FP32_GRID)
+    return fail3(QEX_E_BADARG, "%sbad precision %d", pre, p->precision);
+  const bool f32 = p->method != QEX3D_RHF && p->precision == QEX3D_PRECISION_FP32_GRID;
+  if (p->method != QEX3D_RHF && (p->ngrid < 1 || !in->weights || (f32 ? !in->ao_f32 : !in->ao)))
+    return fail3(QEX_E_BADARG, "%sRKS needs a grid and its AO values (%s)", pre, f32 ? "ao_f32" : "ao");
   if (p->method == QEX3D_RKS_NN_GLOBAL && (!in->nn_params || !nn_widths_ok(p->nn_h1, p->nn_h2)))
     return fail3(QEX_E_BADARG, "%snetwork parameters and hidden widths in [1, %d] required", pre,
                  QEX3D_MAX_NN_HIDDEN);
@@ -2316,7 +2684,10 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   std::vector<Scf3> rec(B);
   int nmax = 1, max_cycle = 0, sync_every = 0, npair_max = 1, h1_max = 1, h2_max = 1;
   long rho_blk = 0, vxc_blk = 0, sum_blk = 0, sym_blk = 0, j_blk = 1, l1_blk = 0, red_blk = 0, vjp_blk = 0;
-  bool any_grid = false, any_nn = false, any_nnl = false;
+  bool any_grid = false, any_nn = false, any_nnl = false, any_x32 = false, any_x32_nn = false;
+  long x32_blk[8] = {0, 0, 0, 0, 0, 0, 0, 0}, x32_sum_blk = 0;
+  int x32_mask_lda = 0, x32_mask_nn = 0;  // bit NB: a molecule of x32_ld(n) = 16 NB
+  size_t x32_lds_nb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   long nnl_blk = 0;
   size_t lds_nnl = 0;
   for (int b = 0; b < B; ++b) {
@@ -2343,6 +2714,9 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     s.eri = ins[b].eri;
     s.ao = ins[b].ao;
     s.w = ins[b].weights;
+    s.ao32 = ins[b].ao_f32;
+    s.prec = p->method == QEX3D_RHF ? QEX3D_PRECISION_FP64 : p->precision;
+    s.x32blk = x32_blocks(ng);
     s.Lc = (double*)(w + L.total);
     s.X = (double*)(w + L.total + al((size_t)n * n * 8));
     s.status_out = outs[b].status;
@@ -2382,7 +2756,15 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     if (p->sync_every > 0 && (sync_every == 0 || p->sync_every < sync_every)) sync_every = p->sync_every;
     npair_max = s.npair > npair_max ? s.npair : npair_max;
     j_blk = coulomb_blocks(s.npair) > j_blk ? coulomb_blocks(s.npair) : j_blk;
-    if (p->method != QEX3D_RHF) {
+    if (s.prec == QEX3D_PRECISION_FP32_GRID) {
+      any_x32 = true;
+      if (!is_lda(p->method)) any_x32_nn = true;
+      const int nbk = x32_ld(n) / 16;
+      (is_lda(p->method) ? x32_mask_lda : x32_mask_nn) |= 1 << nbk;
+      x32_blk[nbk] = s.x32blk > x32_blk[nbk] ? s.x32blk : x32_blk[nbk];
+      x32_lds_nb[nbk] = x32_lds(n, s.nocc) > x32_lds_nb[nbk] ? x32_lds(n, s.nocc) : x32_lds_nb[nbk];
+      x32_sum_blk = vxc_sum_blocks(x32_tiles(n) * 256) > x32_sum_blk ? vxc_sum_blocks(x32_tiles(n) * 256) : x32_sum_blk;
+    } else if (p->method != QEX3D_RHF) {
       any_grid = true;
       rho_blk = (ng + 63) / 64 > rho_blk ? (ng + 63) / 64 : rho_blk;
       vxc_blk = s.nchunk > vxc_blk ? s.nchunk : vxc_blk;
@@ -2418,11 +2800,17 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   }
   const unsigned nb = (unsigned)B;
   auto veff = [&]() {
+    if (any_x32) {
+      // FP32 grid molecules: LDA in one pass, network methods' density first
+      qexprof::Scope ps_(any_x32_nn ? qexprof::K_RHO : qexprof::K_VXC, st);
+      x32_launch<X32_FUSED>(x32_mask_lda, x32_blk, x32_lds_nb, nb, st, dss);
+      x32_launch<X32_RHO>(x32_mask_nn, x32_blk, x32_lds_nb, nb, st, dss);
+    }
     if (any_grid) {
-      {
-        qexprof::Scope ps_(qexprof::K_RHO, st);
-        hipLaunchKernelGGL(k_rho_b, dim3((unsigned)rho_blk, nb), dim3(NT), rho_lds(nmax), st, dss);
-      }
+      qexprof::Scope ps_(qexprof::K_RHO, st);
+      hipLaunchKernelGGL(k_rho_b, dim3((unsigned)rho_blk, nb), dim3(NT), rho_lds(nmax), st, dss);
+    }
+    if (any_grid || any_x32) {
       if (any_nnl) {
         qexprof::Scope ps_(qexprof::K_NN, st);
         hipLaunchKernelGGL(k_nn_local_b, dim3((unsigned)nnl_blk, nb), dim3(NNL_NT), lds_nnl, st, dss);
@@ -2434,14 +2822,24 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
         hipLaunchKernelGGL(k_gnn_mid_b, dim3(1, nb), dim3(1024), (size_t)(h1_max + 2 * h2_max) * 8, st, dss);
         hipLaunchKernelGGL(k_gnn_vjp_b, dim3((unsigned)vjp_blk, nb), dim3(NT), (size_t)h1_max * 8, st, dss);
       }
-      {
-        qexprof::Scope ps_(qexprof::K_VXC, st);
-        hipLaunchKernelGGL(k_vxc_b, dim3((unsigned)vxc_blk, nb), dim3(NT_VXC), vxc_lds(nmax), st, dss);
+      if (any_grid) {
+        {
+          qexprof::Scope ps_(qexprof::K_VXC, st);
+          hipLaunchKernelGGL(k_vxc_b, dim3((unsigned)vxc_blk, nb), dim3(NT_VXC), vxc_lds(nmax), st, dss);
+        }
+        // Vxc = P + P^T with P the fixed-order sum of the chunk partials; scal[2..3] = nelec, exc
+        qexprof::Scope ps_(qexprof::K_OTHER, st);
+        hipLaunchKernelGGL(k_vxc_sum_b, dim3((unsigned)sum_blk, nb), dim3(256), 0, st, dss);
+        hipLaunchKernelGGL(k_sym_add_b, dim3((unsigned)sym_blk, nb), dim3(256), 0, st, dss);
       }
-      // Vxc = P + P^T with P the fixed-order sum of the chunk partials; scal[2..3] = nelec, exc
-      qexprof::Scope ps_(qexprof::K_OTHER, st);
-      hipLaunchKernelGGL(k_vxc_sum_b, dim3((unsigned)sum_blk, nb), dim3(256), 0, st, dss);
-      hipLaunchKernelGGL(k_sym_add_b, dim3((unsigned)sym_blk, nb), dim3(256), 0, st, dss);
+      if (any_x32) {
+        if (any_x32_nn) {
+          qexprof::Scope ps_(qexprof::K_VXC, st);
+          x32_launch<X32_V>(x32_mask_nn, x32_blk, x32_lds_nb, nb, st, dss);
+        }
+        qexprof::Scope ps_(qexprof::K_OTHER, st);
+        hipLaunchKernelGGL(k_x32_sum_b, dim3((unsigned)x32_sum_blk, nb), dim3(256), 0, st, dss);
+      }
     }
     qexprof::Scope ps_(qexprof::K_COULOMB, st);
     hipLaunchKernelGGL(k_coulomb_b, dim3((unsigned)j_blk, nb), dim3(NT), (size_t)npair_max * 8, st, dss);

@@ -73,7 +73,7 @@ class ServeStats:
     batch_sizes: list = field(default_factory=list)
     batch_sizes_3d: list = field(default_factory=list)
     work_1d: list = field(default_factory=list)  # per completed 1D batch: (electron counts, iterations)
-    work_3d: list = field(default_factory=list)  # per completed 3D solve: (n_ao, grid points, cycles)
+    work_3d: list = field(default_factory=list)  # per completed 3D solve: (n_ao, grid points, cycles, n_occ)
     host_s: dict = field(default_factory=lambda: {"dispatch_1d": 0.0, "dispatch_3d": 0.0, "complete_1d": 0.0,
                                                   "complete_3d": 0.0})
 
@@ -185,7 +185,7 @@ class Server:
                             stats.completions.append(Completion(r.rid, "3d", r.arrival, done, e, mf.converged))
                             grids = getattr(mf, "grids", None)
                             stats.work_3d.append((mf.mol.nao, grids.size if grids is not None else 0,
-                                                  getattr(mf, "cycles", 0)))
+                                                  getattr(mf, "cycles", 0), getattr(mf.mol, "nelectron", 0) // 2))
                     stats.host_s["complete_" + kind] += time.perf_counter() - th
                     progressed = True
             if self.slots is not None:

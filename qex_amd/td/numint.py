@@ -23,6 +23,18 @@ def eval_ao(mol, coords, deriv=0):
     return ao
 
 
+def eval_ao_f32(mol, coords):
+    """The AO values rounded to FP32 in the layout of the FP32 grid SCF path: [rows, ld] with ld = nao
+    rounded up to 16 and rows = ngrid rounded up to 64, padding zero."""
+    dev = engine.device()
+    coords = engine.as_f64(coords, dev)
+    ng = coords.shape[0]
+    ao = torch.empty((_lib.ao32_rows(ng), _lib.ao32_ld(mol.nao)), dtype=torch.float32, device=dev)
+    _lib.check3(_lib.load().qex3d_eval_ao_f32(ctypes.byref(mol.device_basis()), ng, engine._ptr(coords),
+                                              engine._ptr(ao), engine._stream()), "qex3d_eval_ao_f32")
+    return ao
+
+
 def eval_rho(mol, ao, dm, non0tab=None, xctype="LDA", hermi=0, verbose=None):
     """rho_g = sum_{mu nu} ao[g, mu] D[mu, nu] ao[g, nu]; D is symmetrised when hermi=0."""
     dev = engine.device()

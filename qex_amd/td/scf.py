@@ -12,7 +12,7 @@ import torch
 from .. import _lib
 from ..od import engine
 from .dft import Grids
-from .numint import eval_ao, lda_functional
+from .numint import eval_ao, eval_ao_f32, lda_functional
 
 
 class _SCF:
@@ -24,6 +24,10 @@ class _SCF:
         self.max_cycle = 50
         self.diis_space = 8
         self.sync_every = 4  # host checks the device's convergence flag every 4 cycles (0: never)
+        # grid work precision: "fp64", or "fp32_grid" (AO values stored in FP32, rho from them with FP64
+        # sums, the Vxc contraction on FP32 matrix cores with FP64 sums across 16-point units;
+        # everything else FP64)
+        self.precision = "fp64"
         self.grids = None
         self.e_tot = None
         self.converged = False
@@ -70,15 +74,26 @@ class _SCF:
             method = _lib.QEX3D_RKS_NN_LOCAL if isinstance(net, LocalNNXC) else _lib.QEX3D_RKS_NN_GLOBAL
             if kwargs.get("params") is not None:
                 net = net.with_params(kwargs["params"])
+        if self.precision not in ("fp64", "fp32_grid"):
+            raise ValueError(f"precision must be 'fp64' or 'fp32_grid', got {self.precision!r}")
+        f32 = self.precision == "fp32_grid" and method != _lib.QEX3D_RHF
+        ao32 = None
         if method != _lib.QEX3D_RHF:
             if self.grids is None:
                 self.grids = Grids(mol).build()
-            if getattr(self, "_ao", None) is None:
-                self._ao = eval_ao(mol, self.grids.coords)
-            ao, w, ng = self._ao, self.grids.weights, self.grids.size
+            if f32:
+                if getattr(self, "_ao32", None) is None:
+                    self._ao32 = eval_ao_f32(mol, self.grids.coords)
+                ao32 = self._ao32
+            else:
+                if getattr(self, "_ao", None) is None:
+                    self._ao = eval_ao(mol, self.grids.coords)
+                ao = self._ao
+            w, ng = self.grids.weights, self.grids.size
         n = mol.nao
         p = _lib.Qex3dScfProblem(n, mol.nelectron, method, self.max_cycle, self.diis_space, ng,
                                  float(self.conv_tol), float(mol.energy_nuc()), int(self.sync_every))
+        p.precision = _lib.QEX3D_PRECISION_FP32_GRID if f32 else _lib.QEX3D_PRECISION_FP64
         if method == _lib.QEX3D_RKS_NN_GLOBAL:
             if net.n_grid != ng:
                 raise ValueError(f"network input width {net.n_grid} != grid size {ng}")
@@ -92,7 +107,7 @@ class _SCF:
         status = torch.zeros(8, dtype=torch.int32, device=dev)
         ptr = engine._ptr
         packed = net.packed() if method in (_lib.QEX3D_RKS_NN_GLOBAL, _lib.QEX3D_RKS_NN_LOCAL) else None
-        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w), ptr(packed))
+        ins = _lib.Qex3dScfInputs(ptr(S), ptr(H), ptr(eri), ptr(ao), ptr(w), ptr(packed), ptr(ao32))
         outs = _lib.Qex3dScfOutputs(ptr(e_hist), ptr(mo), ptr(C), ptr(D), ptr(status))
         self._keep = packed
         self._status = status

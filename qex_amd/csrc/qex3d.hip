@@ -313,14 +313,18 @@ __global__ __launch_bounds__(256) void k_int1e(Basis b, double* S, double* Tm, d
 
 // ---------------------------------------------------------------------------------------------
 // Two-electron integrals (ij|kl) as the pair matrix eri[P][Q], P = pair(i >= j), by
-// McMurchie-Davidson over s/p shells.  One thread per (bra AO pair P, ket shell pair SQ): each
-// primitive quartet computes F_0..F_L once, the Hermite integrals R_tuv (t+u+v <= L) once, the
-// bra contraction W[L2] = sum_L1 E^ij_L1 R_{L1+L2} once, and then every AO component pair (kl) of
-// SQ with a dot over its ket Hermite coefficients.  The (bra, ket) angular classes (0: ss, 1: sp
-// or ps, 2: pp) are template parameters, so every Hermite index is a compile-time constant.
-// Bra pairs are sorted by class and ket shell pairs by class, a wave has one ket shell pair, and a
-// wave's bra pairs share one class (except at the class boundaries).  Symmetry: only Q <= P is
-// computed; eri[P][Q] and eri[Q][P] are both written.
+// McMurchie-Davidson over s/p shells.  A group of 8 lanes per (bra AO pair P, ket shell pair SQ),
+// each lane taking every 8th primitive quartet: a quartet computes F_0..F_L once, the Hermite
+// integrals R_tuv (t+u+v <= L) once, the bra contraction W[L2] = sum_L1 E^ij_L1 R_{L1+L2} once, and
+// then every AO component pair (kl) of SQ with a dot over its ket Hermite coefficients; the group
+// sums its lanes by a fixed xor tree.  The (bra, ket) angular classes (0: ss, 1: sp or ps, 2: pp)
+// are template parameters, so every Hermite index is a compile-time constant.  Passes: primitive
+// pair records; shell lists; (ij|ij) for every AO pair; the Schwarz factor of every shell pair;
+// bra pairs and ket shell pairs sorted by class; per ket shell pair the list of bra pairs with
+// P >= Q's first pair (symmetry: only Q <= P is computed, eri[P][Q] and eri[Q][P] both written)
+// whose Schwarz product reaches ERI_SCHWARZ_TOL; the output cleared; the main pass over the lists
+// (a wave: one ket shell pair, 8 consecutive listed bra pairs of one class except at the class
+// boundaries).
 // ---------------------------------------------------------------------------------------------
 constexpr int SREC = 16;  // doubles per (shell pair, primitive pair) record
 constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
@@ -398,8 +402,16 @@ struct EriArgs {
   const int* bra;       // [npair] AO pairs sorted by class
   const int* ket;       // [nsp] shell pairs sorted by class
   const int* cnt;       // [0..2] bra class sizes, [3..5] ket class sizes
+  const double* sq;     // [nsp] Schwarz factor max sqrt|(ij|ij)| of each shell pair (main pass)
+  double* diag;         // [npair] (ij|ij) (diagonal pass)
+  const int* live;      // [nsp][npair] per sorted ket: its unscreened bra pairs with P >= qmin, in bra order
+  const int* live_cnt;  // [nsp][3] their counts per bra class
   double* eri;
 };
+
+// (ij|kl) with sqrt|(ij|ij)| sqrt|(kl|kl)| (shell-pair maxima) below this are written as zero: every
+// skipped integral is below it in magnitude (pyscf screens its direct SCF at 1e-13)
+constexpr double ERI_SCHWARZ_TOL = 1e-15;
 
 __device__ __forceinline__ void lmn3(int packed, int* l) {
   l[0] = packed & 3;
@@ -407,8 +419,10 @@ __device__ __forceinline__ void lmn3(int packed, int* l) {
   l[2] = (packed >> 4) & 3;
 }
 
-template <int CB, int CK>
-__device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, int sd) {
+// GS lanes (a group) share one (P, SQ): member m takes primitive quartets m, m + GS, ... of the
+// pp x pp, and the group's partial sums are combined by a fixed xor tree.
+template <int CB, int CK, bool DIAG = false, int GS = 1>  // DIAG: only (P|P), into g.diag[P]
+__device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, int sd, int m = 0) {
   constexpr int NB = h_count(CB), NK = h_count(CK), NR = h_count(CB + CK);
   constexpr int NCC = CK == 0 ? 1 : 3;  // components: (lc, ld) = (0,0): 1x1, (1,0)/(0,1): 3x1 or 1x3, (1,1): 3x3
   constexpr int NCD = CK == 2 ? 3 : 1;
@@ -424,14 +438,15 @@ __device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, 
   double acc[NCC * NCD];
 #pragma unroll
   for (int q = 0; q < NCC * NCD; ++q) acc[q] = 0.0;
-  for (int a = 0; a < pp; ++a) {
+  for (int qi = m; qi < pp * pp; qi += GS) {
+    const int a = qi / pp, c = qi % pp;
     const double* x1 = r1 + a * SREC;
     const double c1 = x1[4];
     if (c1 == 0.0) continue;
     const double p = x1[0];
     double E1[NB];
     e_dense<CB>(la, lb, x1 + 5, x1 + 8, x1[11], E1);
-    for (int c = 0; c < pp; ++c) {
+    {
       const double* x2 = r2 + c * SREC;
       const double c2 = x2[4];
       if (c2 == 0.0) continue;
@@ -493,13 +508,22 @@ __device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, 
       if (k < l) continue;  // same p shell: only k >= l
       const int Q = k * (k + 1) / 2 + l;
       if (Q > P) continue;
-      const double val = acc[cc * NCD + cd];
-      g.eri[(size_t)P * g.npair + Q] = val;
-      g.eri[(size_t)Q * g.npair + P] = val;
+      double val = acc[cc * NCD + cd];
+#pragma unroll
+      for (int o = 1; o < GS; o <<= 1) val += __shfl_xor(val, o, 64);
+      if (m != 0) continue;
+      if (DIAG) {
+        if (Q == P) g.diag[P] = val;
+      } else {
+        g.eri[(size_t)P * g.npair + Q] = val;
+        g.eri[(size_t)Q * g.npair + P] = val;
+      }
     }
 }
 
-// one wave per workgroup: blockIdx.y = ket shell pair (sorted), lanes = consecutive sorted bra pairs
+// one wave per workgroup: blockIdx.y = ket shell pair (sorted), groups of ERI_GS lanes = consecutive
+// sorted bra pairs (64 / ERI_GS per wave)
+constexpr int ERI_GS = 8;
 __global__ __launch_bounds__(64) void k_eri(EriArgs g) {
   const int ky = blockIdx.y;
   const int kc0 = g.cnt[3], kc1 = g.cnt[4];
@@ -509,28 +533,113 @@ __global__ __launch_bounds__(64) void k_eri(EriArgs g) {
   while (sc * (sc + 1) / 2 > SQ) --sc;
   while ((sc + 1) * (sc + 2) / 2 <= SQ) ++sc;
   const int sd = SQ - sc * (sc + 1) / 2;
-  const int qmin = g.ao_first[sc] * (g.ao_first[sc] + 1) / 2 + g.ao_first[sd];
-  const int bx = blockIdx.x * 64 + threadIdx.x;
-  if (bx >= g.npair) return;
-  const int P = g.bra[bx];
-  if (P < qmin) return;
-  const int bc0 = g.cnt[0], bc1 = g.cnt[1];
-  const int cb = bx < bc0 ? 0 : bx < bc0 + bc1 ? 1 : 2;
+  // this ket's live bra pairs (screened entries stay as the zero the output was filled with); waves
+  // past the live count leave at once
+  const int* lc3 = g.live_cnt + 3 * ky;
+  const int bc0 = lc3[0], bc1 = lc3[1], nlive = bc0 + bc1 + lc3[2];
+  const int bx = blockIdx.x * (64 / ERI_GS) + threadIdx.x / ERI_GS, m = threadIdx.x % ERI_GS;
+  if (bx >= nlive) return;
+  const int P = g.live[(size_t)ky * g.npair + bx];
   int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
   while (i * (i + 1) / 2 > P) --i;
   while ((i + 1) * (i + 2) / 2 <= P) ++i;
   const int j = P - i * (i + 1) / 2;
+  const int cb = bx < bc0 ? 0 : bx < bc0 + bc1 ? 1 : 2;
   switch (cb * 3 + ck) {
-    case 0: eri_body<0, 0>(g, P, i, j, SQ, sc, sd); break;
-    case 1: eri_body<0, 1>(g, P, i, j, SQ, sc, sd); break;
-    case 2: eri_body<0, 2>(g, P, i, j, SQ, sc, sd); break;
-    case 3: eri_body<1, 0>(g, P, i, j, SQ, sc, sd); break;
-    case 4: eri_body<1, 1>(g, P, i, j, SQ, sc, sd); break;
-    case 5: eri_body<1, 2>(g, P, i, j, SQ, sc, sd); break;
-    case 6: eri_body<2, 0>(g, P, i, j, SQ, sc, sd); break;
-    case 7: eri_body<2, 1>(g, P, i, j, SQ, sc, sd); break;
-    default: eri_body<2, 2>(g, P, i, j, SQ, sc, sd); break;
+    case 0: eri_body<0, 0, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 1: eri_body<0, 1, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 2: eri_body<0, 2, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 3: eri_body<1, 0, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 4: eri_body<1, 1, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 5: eri_body<1, 2, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 6: eri_body<2, 0, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    case 7: eri_body<2, 1, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+    default: eri_body<2, 2, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
   }
+}
+
+// Work lists of the main pass: one workgroup per sorted ket shell pair ky compacts, class segment
+// by class segment of the sorted bra list, the bra pairs P >= qmin whose Schwarz product passes.
+__global__ __launch_bounds__(256) void k_eri_live(EriArgs g, int* live, int* live_cnt) {
+  __shared__ int wtot[4];
+  __shared__ int base;
+  const int ky = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int SQ = g.ket[ky];
+  int sc = (int)((sqrt(8.0 * SQ + 1.0) - 1.0) * 0.5);
+  while (sc * (sc + 1) / 2 > SQ) --sc;
+  while ((sc + 1) * (sc + 2) / 2 <= SQ) ++sc;
+  const int sd = SQ - sc * (sc + 1) / 2;
+  const int qmin = g.ao_first[sc] * (g.ao_first[sc] + 1) / 2 + g.ao_first[sd];
+  const double sk = g.sq[SQ];
+  int* out = live + (size_t)ky * g.npair;
+  int seg0 = 0, off = 0;
+  for (int c = 0; c < 3; ++c) {
+    const int seg1 = seg0 + g.cnt[c];
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int b0 = seg0; b0 < seg1; b0 += 256) {
+      const int bx = b0 + tid;
+      bool flag = false;
+      if (bx < seg1) {
+        const int P = g.bra[bx];
+        int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > P) --i;
+        while ((i + 1) * (i + 2) / 2 <= P) ++i;
+        const int j = P - i * (i + 1) / 2;
+        const int sa = g.ao_shell[i], sb = g.ao_shell[j];
+        flag = P >= qmin && g.sq[sa * (sa + 1) / 2 + sb] * sk >= ERI_SCHWARZ_TOL;
+      }
+      const unsigned long long msk = __ballot(flag);
+      if (lane == 0) wtot[wv] = __popcll(msk);
+      __syncthreads();
+      int wb = 0;
+      for (int w = 0; w < wv; ++w) wb += wtot[w];
+      if (flag) out[off + base + wb + __popcll(msk & ((1ull << lane) - 1))] = g.bra[bx];
+      __syncthreads();
+      if (tid == 0) base += wtot[0] + wtot[1] + wtot[2] + wtot[3];
+      __syncthreads();
+    }
+    if (tid == 0) live_cnt[3 * ky + c] = base;
+    off += base;
+    seg0 = seg1;
+    __syncthreads();
+  }
+}
+
+// Diagonal pass: (ij|ij) for every AO pair P = (ij), into diag[P].
+__global__ __launch_bounds__(64) void k_eri_diag(EriArgs g) {
+  const int P = blockIdx.x * (64 / ERI_GS) + threadIdx.x / ERI_GS, m = threadIdx.x % ERI_GS;
+  if (P >= g.npair) return;
+  int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > P) --i;
+  while ((i + 1) * (i + 2) / 2 <= P) ++i;
+  const int j = P - i * (i + 1) / 2;
+  const int sa = g.ao_shell[i], sb = g.ao_shell[j];  // sa >= sb: shells are in AO order
+  const int SQ = sa * (sa + 1) / 2 + sb;
+  const int cb = (g.ao_lmn[i] != 0) + (g.ao_lmn[j] != 0);
+  if (cb == 0) eri_body<0, 0, true, ERI_GS>(g, P, i, j, SQ, sa, sb, m);
+  else if (cb == 1) eri_body<1, 1, true, ERI_GS>(g, P, i, j, SQ, sa, sb, m);
+  else eri_body<2, 2, true, ERI_GS>(g, P, i, j, SQ, sa, sb, m);
+}
+
+// sq[SP] = max over the AO pairs (ij) of shell pair SP of sqrt|(ij|ij)|
+__global__ void k_eri_schwarz(int nsp, const int* sh_l, const int* ao_first, const double* diag, double* sq) {
+  const int SP = blockIdx.x * blockDim.x + threadIdx.x;
+  if (SP >= nsp) return;
+  int sa = (int)((sqrt(8.0 * SP + 1.0) - 1.0) * 0.5);
+  while (sa * (sa + 1) / 2 > SP) --sa;
+  while ((sa + 1) * (sa + 2) / 2 <= SP) ++sa;
+  const int sb = SP - sa * (sa + 1) / 2;
+  const int na = sh_l[sa] == 0 ? 1 : 3, nb = sh_l[sb] == 0 ? 1 : 3;
+  double m = 0.0;
+  for (int x = 0; x < na; ++x)
+    for (int y = 0; y < nb; ++y) {
+      const int i = ao_first[sa] + x, j = ao_first[sb] + y;
+      if (j > i) continue;
+      const int P = i * (i + 1) / 2 + j;
+      m = fmax(m, sqrt(fabs(diag[P])));
+    }
+  sq[SP] = m;
 }
 
 // Shell-pair primitive records: p, P (3), c_a c_b exp(-mu |AB|^2), P - A (3), P - B (3), 1/(2p).
@@ -569,17 +678,21 @@ __global__ void k_eri_records(Basis b, int nsp, int maxp, double* rec) {
   o[4] = fabs(cab) * K * (M_PI / p) * sqrt(M_PI / p) < 1e-20 ? 0.0 : cab * K;
 }
 
-// Sorting by class (one workgroup of 1024): bra AO pairs by l(i) + l(j), ket shell pairs by
-// l(a) + l(b); stable (ascending index within a class); ao_first per shell.
+// One workgroup of 1024.  phase 0: ao_first (first AO of each shell).  phase 1: the work lists,
+// each a stable partition by class: bra AO pairs by l(i) + l(j), ket shell pairs by l(a) + l(b);
+// cnt[0..2] bra and [3..5] ket class sizes.
 __global__ __launch_bounds__(1024) void k_eri_setup(Basis b, int npair, int nsp, int* ao_first, int* bra, int* ket,
-                                                    int* cnt) {
+                                                    int* cnt, int phase) {
   __shared__ int wtot[16];
   __shared__ int base;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int s = tid; s < b.nshell; s += 1024) {
-    int f = 0;
-    for (int t = 0; t < s; ++t) f += b.sh_l[t] == 0 ? 1 : 3;
-    ao_first[s] = f;
+  if (phase == 0) {
+    for (int s = tid; s < b.nshell; s += 1024) {
+      int f = 0;
+      for (int t = 0; t < s; ++t) f += b.sh_l[t] == 0 ? 1 : 3;
+      ao_first[s] = f;
+    }
+    return;
   }
   for (int list = 0; list < 2; ++list) {
     const int n = list == 0 ? npair : nsp;
@@ -596,8 +709,7 @@ __global__ __launch_bounds__(1024) void k_eri_setup(Basis b, int npair, int nsp,
           while (x * (x + 1) / 2 > idx) --x;
           while ((x + 1) * (x + 2) / 2 <= idx) ++x;
           const int y = idx - x * (x + 1) / 2;
-          const int cls = list == 0 ? (b.ao_lmn[x] != 0) + (b.ao_lmn[y] != 0) : b.sh_l[x] + b.sh_l[y];
-          flag = cls == c;
+          flag = (list == 0 ? (b.ao_lmn[x] != 0) + (b.ao_lmn[y] != 0) : b.sh_l[x] + b.sh_l[y]) == c;
         }
         const unsigned long long m = __ballot(flag);
         if (lane == 0) wtot[wv] = __popcll(m);
@@ -615,6 +727,11 @@ __global__ __launch_bounds__(1024) void k_eri_setup(Basis b, int npair, int nsp,
       __syncthreads();
     }
   }
+}
+
+// the ERI pair matrix is cleared before the main pass (screened entries are not written)
+__global__ void k_fill0(double* x, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = 0.0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2310,7 +2427,8 @@ size_t qex3d_eri_workspace_bytes(const qex3d_basis* b) {
   const long np = (long)b->nao * (b->nao + 1) / 2;
   const long nsp = (long)b->nshell * (b->nshell + 1) / 2;
   return al((size_t)nsp * b->max_nprim * b->max_nprim * SREC * 8) + al((size_t)b->nshell * 4) + al((size_t)np * 4) +
-         al((size_t)nsp * 4) + al(64);
+         al((size_t)nsp * 4) + al(64) + al((size_t)nsp * 8) + al((size_t)np * 8) + al((size_t)nsp * np * 4) +
+         al((size_t)nsp * 3 * 4);
 }
 
 int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream) {
@@ -2334,11 +2452,19 @@ int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspa
   int* ket = (int*)w;
   w += al((size_t)nsp * 4);
   int* cnt = (int*)w;
+  w += al(64);
+  double* sq = (double*)w;
+  w += al((size_t)nsp * 8);
+  double* diag = (double*)w;
+  w += al((size_t)np * 8);
+  int* live = (int*)w;
+  w += al((size_t)nsp * np * 4);
+  int* live_cnt = (int*)w;
   const Basis d = make_basis(b);
   qexprof::Scope ps(qexprof::K_ERI, st);
   const long nrec = (long)nsp * maxp * maxp;
   hipLaunchKernelGGL(k_eri_records, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, st, d, nsp, maxp, rec);
-  hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt);
+  hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt, 0);
   EriArgs g{};
   g.nao = b->nao;
   g.npair = np;
@@ -2352,8 +2478,18 @@ int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspa
   g.bra = bra;
   g.ket = ket;
   g.cnt = cnt;
+  g.sq = sq;
+  g.diag = diag;
+  g.live = live;
+  g.live_cnt = live_cnt;
   g.eri = eri;
-  hipLaunchKernelGGL(k_eri, dim3((unsigned)((np + 63) / 64), (unsigned)nsp), dim3(64), 0, st, g);
+  hipLaunchKernelGGL(k_eri_diag, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS))), dim3(64), 0, st, g);
+  hipLaunchKernelGGL(k_eri_schwarz, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, nsp, b->shell_l, ao_first,
+                     diag, sq);
+  hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt, 1);
+  hipLaunchKernelGGL(k_eri_live, dim3((unsigned)nsp), dim3(256), 0, st, g, live, live_cnt);
+  hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, eri, (size_t)np * np);
+  hipLaunchKernelGGL(k_eri, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS)), (unsigned)nsp), dim3(64), 0, st, g);
   return launch_status("qex3d_eri");
 }
 

@@ -130,6 +130,41 @@ def test_eri_chain8_against_oracle(qex):
     np.testing.assert_allclose(got, fx["value"], rtol=0, atol=1e-12)
 
 
+def test_eri_schwarz_screening_on_the_16_atom_chain(qex):
+    """H16 (seed 116): the entries whose shell-pair Schwarz bound max sqrt|(ab|ab)| x max sqrt|(cd|cd)|
+    is below 1e-15 are written as zero, a sample of them evaluated by the oracle is below 1e-15 in
+    magnitude, and a sample of the kept entries matches the oracle to 1e-12."""
+    _, td = qex
+    mol = td.gto.synthetic_chain(16, 116)
+    gp = mol.intor_eri_pairs().cpu().numpy()
+    n = mol.nao
+    ao_shell = np.concatenate([[s] * (1 if l == 0 else 3) for s, (_, l, _, _) in enumerate(mol.shells)])
+    pairs = [(i, j) for i in range(n) for j in range(i + 1)]
+    omol = otd.synthetic_chain(16, 116)
+    info = [otd._prims(omol, ao) for ao in omol.ao_list()]
+    pdat = [otd._eri_pair_data(info, *p) for p in pairs]
+    # the bound from the oracle's own diagonal (the device zeroes diagonal entries it screens too)
+    d = np.sqrt(np.abs([otd._eri_quartet(x, x) for x in pdat]))
+    sp = np.array([ao_shell[i] * (ao_shell[i] + 1) // 2 + ao_shell[j] for i, j in pairs])
+    smax = np.zeros(sp.max() + 1)
+    np.maximum.at(smax, sp, d)
+    bound = smax[sp][:, None] * smax[sp][None, :]
+    screened = bound < 1e-15
+    clear = np.abs(bound / 1e-15 - 1.0) > 1e-6  # away from the threshold (rounding of the bound)
+    print(f"H16: {screened.mean():.3f} of the pair matrix screened", flush=True)
+    assert 0.2 < screened.mean() < 0.8
+    assert np.all(gp[screened & clear] == 0.0)
+    rng = np.random.default_rng(5)
+    zi = np.argwhere(screened)
+    ki = np.argwhere(~screened)
+    for P, Q in zi[rng.choice(len(zi), 12, replace=False)]:
+        v = otd._eri_quartet(pdat[P], pdat[Q])
+        assert abs(v) < 1e-15, (P, Q, v)
+    for P, Q in ki[rng.choice(len(ki), 12, replace=False)]:
+        v = otd._eri_quartet(pdat[P], pdat[Q])
+        assert abs(gp[P, Q] - v) < 1e-12, (P, Q, gp[P, Q], v)
+
+
 def test_slater_exchange_rks_matches_oracle_and_notebook(qex):
     """RKS with pyscf's xc "lda" (Slater exchange alone): device SCF against the oracle on the same
     grid, and within the notebook's grid error of its printed -1.03718794786902 Ha."""

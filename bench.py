@@ -111,7 +111,8 @@ def parse_args():
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c5")
     ap.add_argument("--slots", type=int, default=128, help="c5: 1D slots of the continuous-batching table")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
-                    help="c5 served path: 'mixed' = 3D grid work on FP32 matrix cores (FP64 elsewhere), or FP64")
+                    help="c4 and the c5 served path: 'mixed' = 3D AO values in FP32 and the Vxc contraction on FP32 "
+                         "matrix cores (FP64 elsewhere), or FP64 throughout")
     ap.add_argument("--rate", type=float, default=50000.0,
                     help="c5: Poisson arrival rate of the timed stream (req/s; above capacity by default)")
     ap.add_argument("--rates", type=str, default="50,200,500,1000,2000",
@@ -341,6 +342,7 @@ def run_c4(args):
     def make(m):
         mf = tscf.RKS(m)
         mf.conv_tol, mf.max_cycle, mf.diis_space = 1e-8, 30, 8
+        mf.precision = "fp32_grid" if args.precision == "mixed" else "fp64"
         return mf
 
     def solve(m):
@@ -349,6 +351,7 @@ def run_c4(args):
         return mf
 
     batched = args.batch_3d > 1
+    mixed = args.precision == "mixed"
 
     def step():
         """One step: the four chains from their basis; returns {chain: solved object}, latencies."""
@@ -398,7 +401,8 @@ def run_c4(args):
         nao, ng = m.nao, mf.grids.size
         cyc = mf.cycles + 1  # guess + cycles
         work["rho"] += cyc * 2.0 * ng * nao * nao
-        work["vxc"] += cyc * 2.0 * ng * nao * nao
+        # FP32 grid: one fused kernel (rho from the occupied orbitals and the Vxc contraction)
+        work["vxc"] += cyc * (2.0 * ng * nao * nao + (2.0 * ng * nao * (m.nelectron // 2) if mixed else 0.0))
         npair = nao * (nao + 1) // 2
         work["coulomb"] += cyc * 8.0 * npair * npair  # bytes: one pass over the ERI pair matrix
         work["scf_eigh"] += cyc * 9.0 * nao ** 3
@@ -408,7 +412,9 @@ def run_c4(args):
     kern = {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.KERNEL_KINDS) if cnt[i]}
     dom = max(kern, key=lambda k: kern[k][0])
     if dom in ("rho", "vxc", "scf_eigh", "scf_fock"):
-        bound, unit, peak = "mfma", "TFLOP/s", 78.6
+        bound, unit, peak = ("mfma" if dom in ("rho", "vxc") else "fp64-latency"), "TFLOP/s", 78.6
+        if mixed and dom == "vxc":
+            peak = 157.3  # the fused FP32 grid kernel: dense FP32 MFMA peak
         achieved = work[dom] / (kern[dom][0] * 1e-3) / 1e12
     elif dom == "coulomb":
         bound, unit, peak = "hbm", "GB/s", 8000.0
@@ -432,14 +438,15 @@ def run_c4(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded hydrogen chains and basis exponents)",
         "config": {"workload": "c4 3D RKS-LDA: H2/H4/H8/H16 chains, 5 AOs/atom, 50x110 grid/atom, CDIIS(8), "
                                "<=30 cycles, conv 1e-8; one step = the four chains solved from their basis",
                    "chains": list(C4_CHAINS), "nao": [m.nao for m in mols],
                    "ngrid": [last[n].grids.size for n in C4_CHAINS],
-                   "path": "FP64, the four chains as one batched SCF (qex3d_scf_batch)" if batched else
-                           "plain FP64, one solve at a time"},
+                   "path": ("FP32 grid work (AO values, Vxc contraction on FP32 matrix cores), FP64 elsewhere; "
+                            if mixed else "FP64; ") + ("the four chains as one batched SCF (qex3d_scf_batch)" if batched
+                                                      else "one solve at a time")},
         "latency_ms": {str(n): {"p50": float(np.percentile(lat[n], 50)), "p99": float(np.percentile(lat[n], 99))}
                        for n in C4_CHAINS} | {"all_p50": float(np.percentile(all_lat, 50)),
                                                "all_p99": float(np.percentile(all_lat, 99))},
@@ -723,6 +730,11 @@ def run_c5(args):
     from qex_amd.td import scf as tscf
 
     lib = _lib.load()
+    if args.calibrate_copy:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     f = xc.LocalMLPXC(list(zip(W, b)), window=17, activation="silu", norm=2.0)
     # plain path: one system per launch; served path: a table of slots (continuous batching)
     runner = engine.BatchRunner(grids, utils.exponential_coulomb, f, T_ITERS, max_batch=1, alpha=ALPHA,

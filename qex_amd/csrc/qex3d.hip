@@ -412,6 +412,8 @@ struct EriArgs {
 // (ij|kl) with sqrt|(ij|ij)| sqrt|(kl|kl)| (shell-pair maxima) below this are written as zero: every
 // skipped integral is below it in magnitude (pyscf screens its direct SCF at 1e-13)
 constexpr double ERI_SCHWARZ_TOL = 1e-15;
+// bases with fewer AO pairs are not screened: their diagonal pass would cost more than it saves
+constexpr int ERI_SCREEN_MIN_PAIRS = 2048;
 
 __device__ __forceinline__ void lmn3(int packed, int* l) {
   l[0] = packed & 3;
@@ -729,9 +731,10 @@ __global__ __launch_bounds__(1024) void k_eri_setup(Basis b, int npair, int nsp,
   }
 }
 
-// the ERI pair matrix is cleared before the main pass (screened entries are not written)
-__global__ void k_fill0(double* x, size_t n) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = 0.0;
+// x[0..n) = v (the ERI pair matrix is cleared before the main pass: screened entries are not
+// written; unscreened bases get Schwarz factors of 1)
+__global__ void k_fill(double* x, size_t n, double v) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = v;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2483,12 +2486,16 @@ int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspa
   g.live = live;
   g.live_cnt = live_cnt;
   g.eri = eri;
-  hipLaunchKernelGGL(k_eri_diag, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS))), dim3(64), 0, st, g);
-  hipLaunchKernelGGL(k_eri_schwarz, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, nsp, b->shell_l, ao_first,
-                     diag, sq);
+  if (np >= ERI_SCREEN_MIN_PAIRS) {
+    hipLaunchKernelGGL(k_eri_diag, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS))), dim3(64), 0, st, g);
+    hipLaunchKernelGGL(k_eri_schwarz, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, nsp, b->shell_l, ao_first,
+                       diag, sq);
+  } else {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, sq, (size_t)nsp, 1.0);
+  }
   hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt, 1);
   hipLaunchKernelGGL(k_eri_live, dim3((unsigned)nsp), dim3(256), 0, st, g, live, live_cnt);
-  hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, eri, (size_t)np * np);
+  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, eri, (size_t)np * np, 0.0);
   hipLaunchKernelGGL(k_eri, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS)), (unsigned)nsp), dim3(64), 0, st, g);
   return launch_status("qex3d_eri");
 }

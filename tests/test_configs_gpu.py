@@ -251,3 +251,40 @@ def test_c5_served_stream_matches_solo_and_references(qex):
             n = chains[int(pick3[c.rid])]
             assert c.converged
             assert abs(c.energy - golden[n]["e_tot"]) < 1e-6
+
+
+def test_c5_served_mixed_path_with_slots(qex):
+    """The bench's served path: 1D requests through a continuous-batching slot table
+    (SlotRunner), 3D requests batched with the FP32 grid work.  1D results bitwise equal to the
+    plain one-at-a-time FP64 path, 3D within 1e-7 Ha of it and 1e-6 Ha of the golden energies."""
+    od, td = qex
+    from qex_amd import serve
+    grids, pool, pool_n0, W, b, reqs, pick1, pick3, chains = _c5_requests(od, td, 96, 2000.0, 79)
+    f = od.xc.LocalMLPXC(list(zip(W, b)), window=17, activation="silu", norm=2.0)
+    runner = od.engine.BatchRunner(grids, od.utils.exponential_coulomb, f, 20, max_batch=1, alpha=0.3,
+                                   alpha_decay=1.0, num_mixing_iterations=1, loop="scan")
+    slots = od.engine.SlotRunner(grids, od.utils.exponential_coulomb, f, 20, slots=32, alpha=0.3, alpha_decay=1.0,
+                                 num_mixing_iterations=1, loop="scan", depth=3)
+
+    def solver(m, precision="fp64"):
+        mf = td.scf.RKS(m)
+        mf.conv_tol, mf.max_cycle, mf.diis_space, mf.sync_every = 1e-8, 30, 8, 0
+        mf.precision = precision
+        return mf
+
+    print("c5 mixed: plain pass", flush=True)
+    plain = {c.rid: c.energy for c in serve.run_one_at_a_time(reqs, runner, solver).completions}
+    print("c5 mixed: served pass", flush=True)
+    st = serve.Server(slots, lambda m: solver(m, "fp32_grid"), batch_3d=32, streams_3d=2, depth_3d=1).run(reqs)
+    assert sorted(c.rid for c in st.completions) == list(range(len(reqs)))
+    golden = _c4_golden()
+    worst = 0.0
+    for c in st.completions:
+        if c.kind == "1d":
+            assert c.energy == plain[c.rid], c.rid
+        else:
+            assert c.converged
+            worst = max(worst, abs(c.energy - plain[c.rid]))
+            assert abs(c.energy - golden[chains[int(pick3[c.rid])]]["e_tot"]) < 1e-6
+    print(f"c5 mixed: 3D max |dE| vs plain FP64 {worst:.2e}", flush=True)
+    assert worst < 1e-7

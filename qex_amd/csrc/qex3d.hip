@@ -1710,9 +1710,12 @@ __global__ __launch_bounds__(NT_SCF) void k_scf_fock(const Scf3* __restrict__ ss
 constexpr int NT_EIG = 1024;
 size_t eigh_lds(int n) { return (size_t)2 * n * (n + 1) * 8; }
 
-__global__ __launch_bounds__(NT_EIG) void k_scf_eigh(const Scf3* __restrict__ ss) {
+// full = 0 (the SCF cycles): only the orbitals the cycle uses (the n_occ + 2 lowest; C's other
+// columns zero, their energies unrefined); full = 1 (after the last cycle, every molecule, frozen or
+// not): the whole solve, whose first n_occ + 2 columns equal the cycles' bit for bit.
+__global__ __launch_bounds__(NT_EIG) void k_scf_eigh(const Scf3* __restrict__ ss, int full) {
   const Scf3 s = ss[blockIdx.x];
-  if (s.ist[0]) return;
+  if (s.ist[0] && !full) return;
   extern __shared__ double sm[];
   __shared__ qexeig::Smem sh;
   const int n = s.n, nn = n * n, S = n + 1;
@@ -1729,7 +1732,7 @@ __global__ __launch_bounds__(NT_EIG) void k_scf_eigh(const Scf3* __restrict__ ss
     }
   }
   __syncthreads();
-  qexeig::symeig(n, A, Z, S, s.mo, s.eigf, sh);
+  qexeig::symeig(n, A, Z, S, s.mo, s.eigf, sh, full ? n : s.nocc + 2);
   mfma_gemm(n, n, n, s.X, n, false, Z, S, false, s.C, n, 1.0);  // C = X Z
   mfma_gemm(n, n, s.nocc, s.C, n, false, s.C, n, true, s.D, n, 2.0);  // D = 2 C_occ C_occ^T
 }
@@ -2995,9 +2998,9 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     qexprof::Scope ps_(kind, st);
     hipLaunchKernelGGL(kernel, dim3(nb), dim3(NT_SCF), 0, st, dss, args...);
   };
-  auto eigh = [&]() {
+  auto eigh = [&](int full) {
     qexprof::Scope ps_(qexprof::K_EIGH, st);
-    hipLaunchKernelGGL(k_scf_eigh, dim3(nb), dim3(NT_EIG), lds_eigh, st, dss);
+    hipLaunchKernelGGL(k_scf_eigh, dim3(nb), dim3(NT_EIG), lds_eigh, st, dss, full);
   };
   // core-Hamiltonian guess: F = H (k_scf_init zeroes J, K, Vxc and D with a kernel, not a memset)
   per_mol(qexprof::K_OTHER, k_scf_init);
@@ -3010,12 +3013,12 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     hipLaunchKernelGGL(k_scf_orthogonaliser, dim3(nb), dim3(NT_SCF), lds_o, st, dss);
   }
   per_mol_wide(qexprof::K_FOCK, k_scf_fock, -1);
-  eigh();
+  eigh(0);
   veff();
   per_mol_wide(qexprof::K_ENERGY, k_scf_energy, -1);
   for (int c = 0; c < max_cycle; ++c) {
     per_mol_wide(qexprof::K_FOCK, k_scf_fock, c);
-    eigh();
+    eigh(0);
     veff();
     per_mol_wide(qexprof::K_ENERGY, k_scf_energy, c);
     if (sync_every > 0 && (c + 1) % sync_every == 0 && c + 1 < max_cycle) {
@@ -3028,6 +3031,7 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
       if (left == 0) break;
     }
   }
+  eigh(1);  // all orbitals and orbital energies of the final Fock matrix
   per_mol(qexprof::K_OTHER, k_scf_finish);
   return launch_status(name);
 }

@@ -50,6 +50,7 @@ struct Smem {
   double sg, alpha, iv, v0;  // reflector scalars of the current Householder step
   int bs[MAXN], bt[MAXN], loc[MAXN], perm[MAXN];
   unsigned char cnt[2][1024];
+  unsigned char need[MAXN];  // slot's vector wanted (nvec < n: its eigenvalue ranks below nvec)
   double norm;
 };
 
@@ -161,7 +162,12 @@ __device__ __forceinline__ double col_norm2(const double* Y, int S, int s, int t
 // of the j-th smallest eigenvalue; w: ascending eigenvalues (any memory).  f: global scratch of
 // 2 n^2 doubles (3 n^2 are reserved by the callers).  Must be called by every thread of the workgroup (blockDim.x a multiple of 64,
 // blockDim.x >= n).
-__device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, double* w, double* f, Smem& sh) {
+// nvec < n: only the eigenvectors of the nvec smallest eigenvalues (ranked by their multisection
+// values, whole clusters) are computed and back-transformed, into columns 0..nvec-1 of Z, exactly as
+// the full solve computes them; the other eigenvalues are the multisection values (not refined).
+__device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, double* w, double* f, Smem& sh,
+                                       int nvec = 1 << 30) {
+  if (nvec > n) nvec = n;
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   if (n == 1) {
     if (tid == 0) {
@@ -315,6 +321,14 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
     if (act && g == 0) sh.lam[slot] = 0.5 * (lo + hi);
   }
   __syncthreads();
+  // wanted vectors: every slot when nvec = n; else the slots whose multisection value ranks below
+  // nvec (ties by slot)
+  for (int i = tid; i < n; i += nt) {
+    int r = 0;
+    if (nvec < n)
+      for (int j = 0; j < n; ++j) r += (sh.lam[j] < sh.lam[i]) || (sh.lam[j] == sh.lam[i] && j < i);
+    sh.need[i] = r < nvec;
+  }
   // ---- 4. eigenvectors of T (column `slot` of Z holds the block's rows) ----
   for (int idx = tid; idx < n * n; idx += nt) Z[(idx / n) * S + idx % n] = 0.0;
   __syncthreads();
@@ -324,9 +338,15 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
     if (slot < n) {
       const int s = sh.bs[slot], t = sh.bt[slot];
       const bool leader = slot == s || sh.lam[slot] - sh.lam[slot - 1] >= ctol;
+      int last = slot;
+      bool want = false;
       if (leader) {
-        int last = slot;
         while (last + 1 < t && sh.lam[last + 1] - sh.lam[last] < ctol) ++last;
+        for (int a = slot; a <= last; ++a) want = want || sh.need[a];
+        if (!want)
+          for (int a = slot; a <= last; ++a) sh.rq[a] = sh.lam[a];  // not refined: no vector
+      }
+      if (leader && want) {
         for (int a = slot; a <= last; ++a) {
           if (t - s == 1) {
             Z[s * S + a] = 1.0;
@@ -382,7 +402,7 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
     const int m = n - k - 1;
     const double* vk = A + (size_t)(k + 1) * S + k;  // v_i = vk[i * S]
     // sh.lo[c] = v^T Z[k+1:, c], one thread per column
-    if (grp < n) {
+    if (grp < nvec) {
       double acc = 0.0;
       for (int i = sub; i < m; i += 8) acc = fma(vk[i * S], Z[(size_t)(k + 1 + i) * S + grp], acc);
       acc = group8_sum(acc);
@@ -390,12 +410,12 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
     }
     __syncthreads();
     const int c0 = lane, c1 = lane + 64;
-    const double s0 = c0 < n ? 2.0 * sh.lo[c0] : 0.0, s1 = c1 < n ? 2.0 * sh.lo[c1] : 0.0;
+    const double s0 = c0 < nvec ? 2.0 * sh.lo[c0] : 0.0, s1 = c1 < nvec ? 2.0 * sh.lo[c1] : 0.0;
     for (int i = wv; i < m; i += nw) {
       const double vi = vk[i * S];
       double* row = Z + (size_t)(k + 1 + i) * S;
-      if (c0 < n) row[c0] -= vi * s0;
-      if (c1 < n) row[c1] -= vi * s1;
+      if (c0 < nvec) row[c0] -= vi * s0;
+      if (c1 < nvec) row[c1] -= vi * s1;
     }
     __syncthreads();
   }

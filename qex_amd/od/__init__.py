@@ -1,2 +1,2 @@
 """1D Kohn-Sham engine (jax_dft-style loop) on the device."""
-from . import engine, eval, jit_scf, scf, utils, xc  # noqa: F401
+from . import dataset, engine, eval, jit_scf, scf, train, utils, xc  # noqa: F401

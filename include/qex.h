@@ -270,7 +270,8 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
  * i(i+1)/2 + j).  The workspace holds primitive-pair records (npair * max_nprim^2 * 128 bytes).
  * Schwarz screening (bases of >= 2048 AO pairs): an entry whose bound max sqrt|(ab|ab)| over its two
  * shell pairs multiplies to below 1e-15 is written as 0 (so every skipped integral is below 1e-15 in
- * magnitude).
+ * magnitude); primitive quartets whose s-type magnitude bound c_ab c_cd 2 pi^(5/2) / (p q sqrt(p + q)) is
+ * below 1e-17 (1e-20 with a p function) are skipped.
  * Replaces pyscf int2e (libcint) used by get_jk, qedft/train/td/hf_legacy.py:452-471. */
 size_t qex3d_eri_workspace_bytes(const qex3d_basis* b);
 int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream);

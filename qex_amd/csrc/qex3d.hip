@@ -412,6 +412,9 @@ struct EriArgs {
 // (ij|kl) with sqrt|(ij|ij)| sqrt|(kl|kl)| (shell-pair maxima) below this are written as zero: every
 // skipped integral is below it in magnitude (pyscf screens its direct SCF at 1e-13)
 constexpr double ERI_SCHWARZ_TOL = 1e-15;
+// primitive quartets whose c_ab c_cd 2 pi^(5/2) / (p q sqrt(p + q)) is below this (1e-20 when a p
+// function is involved) are skipped
+constexpr double ERI_PRIM_TOL = 1e-17;
 // bases with fewer AO pairs are not screened: their diagonal pass would cost more than it saves
 constexpr int ERI_SCREEN_MIN_PAIRS = 2048;
 
@@ -453,6 +456,10 @@ __device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, 
       const double c2 = x2[4];
       if (c2 == 0.0) continue;
       const double q = x2[0];
+      // the quartet's (ss|ss)-type magnitude bound; p functions add Hermite factors (distances,
+      // 1/2p), so their quartets get a threshold three orders lower
+      const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
+      if (fabs(pref) < (CB + CK == 0 ? ERI_PRIM_TOL : 1e-3 * ERI_PRIM_TOL)) continue;
       const double alpha = p * q / (p + q);
       const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
       double F[CB + CK + 1];
@@ -469,7 +476,6 @@ __device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, 
           w = fma(E1[k1], R[h_index(HT[k1] + HT[k2], HU[k1] + HU[k2], HV[k1] + HV[k2])], w);
         W[k2] = ((HT[k2] + HU[k2] + HV[k2]) & 1) ? -w : w;
       }
-      const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
 #pragma unroll
       for (int cc = 0; cc < NCC; ++cc)
 #pragma unroll

@@ -119,7 +119,7 @@ def parse_args():
                     help="c5: arrival rates of the latency sweep (req/s)")
     ap.add_argument("--streams-3d", type=int, default=2, help="c5: concurrent 3D streams")
     ap.add_argument("--depth-3d", type=int, default=1, help="c5: 3D batches in flight per stream")
-    ap.add_argument("--batch-3d", type=int, default=96,
+    ap.add_argument("--batch-3d", type=int, default=256,
                     help="c4: 1 solves the chains one at a time, >1 solves them as one batched SCF; "
                          "c5: most 3D requests per batched dispatch")
     return ap.parse_args()

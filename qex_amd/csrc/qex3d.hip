@@ -1404,6 +1404,12 @@ __global__ void k_copy1(const double* src, double* dst) {
   if (threadIdx.x == 0) *dst = *src;
 }
 
+constexpr int JT = 64;  // Coulomb tile edge (pair-matrix rows and columns)
+__host__ __device__ inline int jtiles(int npair) {
+  const int nb = (npair + JT - 1) / JT;
+  return nb * (nb + 1) / 2;
+}
+
 // J_ij = sum_kl (ij|kl) D_kl over the pair matrix (row P = pair(i,j)); K (optional) by gathers.
 __host__ __device__ inline int coulomb_blocks(int npair) {
   const int b = (npair + 3) / 4;  // four rows per workgroup pass
@@ -1591,7 +1597,8 @@ struct Scf3 {
   int h1, h2;
   double nn_scale;
   double *zpart, *z1, *dz1, *wv, *ex;
-  double* eigf;  // eigensolver scratch (3 n^2)
+  double* eigf;   // eigensolver scratch (3 n^2)
+  double* jpart;  // Coulomb tile partials (RKS): jtiles(npair) x 2 x JT
 };
 
 // Fock + CDIIS (one workgroup of NT_SCF threads).  cycle < 0: plain F = H + Veff (no DIIS).
@@ -1942,16 +1949,75 @@ __global__ __launch_bounds__(NT) void k_gnn_vjp_b(const Scf3* __restrict__ ss) {
   gnn_vjp_body(blockIdx.x, nb, s.ng, s.h1, s.nn + nn_offsets(s.ng, s.h1, s.h2).W1, s.dz1, s.w, s.wv);
 }
 
+// Hartree-Fock (J and K): the full pair matrix, row by row
 __global__ __launch_bounds__(NT) void k_coulomb_b(const Scf3* __restrict__ ss) {
   const Scf3 s = ss[blockIdx.y];
   const int nb = coulomb_blocks(s.npair);
-  if (s.ist[0] || (int)blockIdx.x >= nb) return;
-  coulomb_body(blockIdx.x, nb, s.n, s.npair, s.eri, s.D, s.J, s.method == QEX3D_RHF ? s.K : nullptr);
+  if (s.method != QEX3D_RHF || s.ist[0] || (int)blockIdx.x >= nb) return;
+  coulomb_body(blockIdx.x, nb, s.n, s.npair, s.eri, s.D, s.J, s.K);
+}
+
+// Kohn-Sham (J only): the lower block triangle of the symmetric pair matrix, half its bytes.
+// Tile t = (pb, qb), qb <= pb, covers rows P in block pb and columns Q in block qb (JT each): its
+// row sums sum_Q (P|Q) D'_Q and, off the diagonal, its column sums sum_P (P|Q) D'_P, which stand
+// for the upper triangle's (Q|P).  k_coulomb_jsum_b adds a row's partials in a fixed order.
+__device__ __forceinline__ double dprime(int q, int n, int npair, const double* dm) {
+  if (q >= npair) return 0.0;
+  int k = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+  while (k * (k + 1) / 2 > q) --k;
+  while ((k + 1) * (k + 2) / 2 <= q) ++k;
+  const int l = q - k * (k + 1) / 2;
+  return k == l ? dm[k * n + l] : dm[k * n + l] + dm[l * n + k];
+}
+__global__ __launch_bounds__(256) void k_coulomb_tiles_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  const int npair = s.npair, t = blockIdx.x;
+  if (s.method == QEX3D_RHF || s.ist[0] || t >= jtiles(npair)) return;
+  int pb = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (pb * (pb + 1) / 2 > t) --pb;
+  while ((pb + 1) * (pb + 2) / 2 <= t) ++pb;
+  const int qb = t - pb * (pb + 1) / 2;
+  __shared__ double dq[JT], dp[JT], colp[4][JT];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid < JT) dq[tid] = dprime(qb * JT + tid, s.n, npair, s.D);
+  else if (tid < 2 * JT) dp[tid - JT] = dprime(pb * JT + tid - JT, s.n, npair, s.D);
+  __syncthreads();
+  const int Q = qb * JT + lane;
+  const double dQ = dq[lane];
+  double* rowp = s.jpart + (size_t)t * 2 * JT;
+  double cs = 0.0;
+  for (int r = wv; r < JT; r += 4) {
+    const int P = pb * JT + r;
+    const double v = (P < npair && Q < npair) ? s.eri[(size_t)P * npair + Q] : 0.0;
+    const double rs = wsum(v * dQ);
+    if (lane == 0) rowp[r] = rs;
+    cs = fma(v, dp[r], cs);
+  }
+  if (pb != qb) {
+    colp[wv][lane] = cs;
+    __syncthreads();
+    if (tid < JT) rowp[JT + tid] = ((colp[0][tid] + colp[1][tid]) + colp[2][tid]) + colp[3][tid];
+  }
+}
+__global__ __launch_bounds__(256) void k_coulomb_jsum_b(const Scf3* __restrict__ ss) {
+  const Scf3 s = ss[blockIdx.y];
+  const int npair = s.npair, P = blockIdx.x * 256 + threadIdx.x;
+  if (s.method == QEX3D_RHF || s.ist[0] || P >= npair) return;
+  const int nbk = (npair + JT - 1) / JT, pb = P / JT, r = P % JT;
+  double sum = 0.0;
+  for (int qb = 0; qb <= pb; ++qb) sum += s.jpart[(size_t)(pb * (pb + 1) / 2 + qb) * 2 * JT + r];
+  for (int p2 = pb + 1; p2 < nbk; ++p2) sum += s.jpart[(size_t)(p2 * (p2 + 1) / 2 + pb) * 2 * JT + JT + r];
+  int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > P) --i;
+  while ((i + 1) * (i + 2) / 2 <= P) ++i;
+  const int j = P - i * (i + 1) / 2;
+  s.J[i * s.n + j] = sum;
+  s.J[j * s.n + i] = sum;
 }
 
 struct Lay3 {
   size_t F, Fd, D, Dlast, C, Cp, mo, J, K, Vxc, vpart, spart, rho, tmp1, tmp2, diisF, diisE, diisB, scal, ist;
-  size_t zpart, z1, dz1, wv, ex, eigf, total;
+  size_t zpart, z1, dz1, wv, ex, eigf, jpart, total;
 };
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1993,6 +2059,7 @@ Lay3 layout3(int n, long ng, int space, int h1 = 0) {
   L.wv = take((size_t)ng * 8);
   L.ex = take((size_t)ng * 8);
   L.eigf = take((size_t)3 * n * n * 8);
+  L.jpart = take((size_t)jtiles(n * (n + 1) / 2) * 2 * JT * 8);
   L.total = off;
   return L;
 }
@@ -2836,7 +2903,7 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
   std::vector<Scf3> rec(B);
   int nmax = 1, max_cycle = 0, sync_every = 0, npair_max = 1, h1_max = 1, h2_max = 1;
   long rho_blk = 0, vxc_blk = 0, sum_blk = 0, sym_blk = 0, j_blk = 1, l1_blk = 0, red_blk = 0, vjp_blk = 0;
-  bool any_grid = false, any_nn = false, any_nnl = false, any_x32 = false, any_x32_nn = false;
+  bool any_grid = false, any_nn = false, any_nnl = false, any_x32 = false, any_x32_nn = false, any_rhf = false;
   long x32_blk[8] = {0, 0, 0, 0, 0, 0, 0, 0}, x32_sum_blk = 0;
   int x32_mask_lda = 0, x32_mask_nn = 0;  // bit NB: a molecule of x32_ld(n) = 16 NB
   size_t x32_lds_nb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2903,11 +2970,13 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
     s.wv = (double*)(w + L.wv);
     s.ex = (double*)(w + L.ex);
     s.eigf = (double*)(w + L.eigf);
+    s.jpart = (double*)(w + L.jpart);
     nmax = n > nmax ? n : nmax;
     max_cycle = p->max_cycle > max_cycle ? p->max_cycle : max_cycle;
     if (p->sync_every > 0 && (sync_every == 0 || p->sync_every < sync_every)) sync_every = p->sync_every;
     npair_max = s.npair > npair_max ? s.npair : npair_max;
     j_blk = coulomb_blocks(s.npair) > j_blk ? coulomb_blocks(s.npair) : j_blk;
+    if (p->method == QEX3D_RHF) any_rhf = true;
     if (s.prec == QEX3D_PRECISION_FP32_GRID) {
       any_x32 = true;
       if (!is_lda(p->method)) any_x32_nn = true;
@@ -2994,7 +3063,11 @@ static int scf_run(int B, const qex3d_scf_problem* ps, const qex3d_scf_inputs* i
       }
     }
     qexprof::Scope ps_(qexprof::K_COULOMB, st);
-    hipLaunchKernelGGL(k_coulomb_b, dim3((unsigned)j_blk, nb), dim3(NT), (size_t)npair_max * 8, st, dss);
+    if (any_rhf) hipLaunchKernelGGL(k_coulomb_b, dim3((unsigned)j_blk, nb), dim3(NT), (size_t)npair_max * 8, st, dss);
+    if (any_grid || any_x32) {
+      hipLaunchKernelGGL(k_coulomb_tiles_b, dim3((unsigned)jtiles(npair_max), nb), dim3(256), 0, st, dss);
+      hipLaunchKernelGGL(k_coulomb_jsum_b, dim3((unsigned)((npair_max + 255) / 256), nb), dim3(256), 0, st, dss);
+    }
   };
   auto per_mol = [&](int kind, auto kernel, auto... args) {
     qexprof::Scope ps_(kind, st);

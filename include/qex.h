@@ -267,7 +267,8 @@ typedef struct qex3d_basis {
 int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* stream);
 
 /* ERIs (ij|kl) as the [npair][npair] matrix over i>=j, k>=l pairs (npair = nao(nao+1)/2, pair index
- * i(i+1)/2 + j).  The workspace holds primitive-pair records (npair * max_nprim^2 * 128 bytes).
+ * i(i+1)/2 + j), computed per shell quartet (McMurchie-Davidson).  The workspace holds primitive-pair
+ * records (nshell(nshell+1)/2 * max_nprim^2 * 128 bytes) and the work lists.
  * Schwarz screening (bases of >= 2048 AO pairs): an entry whose bound max sqrt|(ab|ab)| over its two
  * shell pairs multiplies to below 1e-15 is written as 0 (so every skipped integral is below 1e-15 in
  * magnitude); primitive quartets whose s-type magnitude bound c_ab c_cd 2 pi^(5/2) / (p q sqrt(p + q)) is
@@ -275,6 +276,16 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
  * Replaces pyscf int2e (libcint) used by get_jk, qedft/train/td/hf_legacy.py:452-471. */
 size_t qex3d_eri_workspace_bytes(const qex3d_basis* b);
 int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The ERIs of B molecules in one launch sequence (eri: B device pointers, each [npair_b][npair_b]):
+ * every result equals qex3d_eri on that molecule alone, bit for bit; the molecules' work overlaps
+ * instead of paying each launch's latency in turn.  Supplies the per-molecule eri
+ * inputs of the padded batch (qedft/train/td/scf_functions_masked.py:917-967, eri argument at :920),
+ * which the reference builds molecule by molecule with pyscf int2e; no batched counterpart there. */
+#define QEX3D_MAX_ERI_BATCH 1024
+size_t qex3d_eri_batch_workspace_bytes(int32_t B, const qex3d_basis* bases);
+int qex3d_eri_batch(int32_t B, const qex3d_basis* bases, double* const* eri, void* workspace, size_t workspace_bytes,
+                    void* stream);
 
 /* Atom-centred grid: natm * nrad * nang points from radial (r, w r^2) and angular (unit vectors,
  * weights summing to 4 pi) base tables, Becke partition.  Replaces pyscf dft.Grids.build

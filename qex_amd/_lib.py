@@ -116,6 +116,7 @@ def ao32_rows(ngrid):
 QEX3D_MAX_NAO = 96
 QEX3D_MAX_NPRIM = 8
 QEX3D_MAX_SCF_BATCH = 1024
+QEX3D_MAX_ERI_BATCH = 1024
 
 
 class Qex3dBasis(ctypes.Structure):
@@ -169,6 +170,8 @@ SIGNATURES = {
     "qex3d_int1e": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _P, _P]),
     "qex3d_eri_workspace_bytes": (_S, [ctypes.POINTER(Qex3dBasis)]),
     "qex3d_eri": (_I, [ctypes.POINTER(Qex3dBasis), _P, _P, _S, _P]),
+    "qex3d_eri_batch_workspace_bytes": (_S, [_I, ctypes.POINTER(Qex3dBasis)]),
+    "qex3d_eri_batch": (_I, [_I, ctypes.POINTER(Qex3dBasis), _P, _P, _S, _P]),
     "qex3d_grid": (_I, [ctypes.POINTER(Qex3dBasis), _I, _P, _P, _I, _P, _P, _P, _P, _P]),
     "qex3d_eval_ao": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
     "qex3d_eval_ao_f32": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),

@@ -14,6 +14,7 @@
 // The SCF loop is enqueued without host synchronisation; a device flag freezes the state once
 // |dE| < tol and |g| / sqrt(n_vir n_occ) < sqrt(tol) (pyscf's criterion).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <vector>
 
 #include <math.h>
@@ -313,18 +314,19 @@ __global__ __launch_bounds__(256) void k_int1e(Basis b, double* S, double* Tm, d
 
 // ---------------------------------------------------------------------------------------------
 // Two-electron integrals (ij|kl) as the pair matrix eri[P][Q], P = pair(i >= j), by
-// McMurchie-Davidson over s/p shells.  A group of 8 lanes per (bra AO pair P, ket shell pair SQ),
-// each lane taking every 8th primitive quartet: a quartet computes F_0..F_L once, the Hermite
-// integrals R_tuv (t+u+v <= L) once, the bra contraction W[L2] = sum_L1 E^ij_L1 R_{L1+L2} once, and
-// then every AO component pair (kl) of SQ with a dot over its ket Hermite coefficients; the group
-// sums its lanes by a fixed xor tree.  The (bra, ket) angular classes (0: ss, 1: sp or ps, 2: pp)
-// are template parameters, so every Hermite index is a compile-time constant.  Passes: primitive
-// pair records; shell lists; (ij|ij) for every AO pair; the Schwarz factor of every shell pair;
-// bra pairs and ket shell pairs sorted by class; per ket shell pair the list of bra pairs with
-// P >= Q's first pair (symmetry: only Q <= P is computed, eri[P][Q] and eri[Q][P] both written)
-// whose Schwarz product reaches ERI_SCHWARZ_TOL; the output cleared; the main pass over the lists
-// (a wave: one ket shell pair, 8 consecutive listed bra pairs of one class except at the class
-// boundaries).
+// McMurchie-Davidson over s/p shells.  The main pass works on shell quartets: a group of 8 lanes
+// per (bra shell pair, ket shell pair), each lane taking every 8th primitive quartet.  A quartet
+// computes F_0..F_L and the Hermite integrals R_tuv (t+u+v <= L) once; then, for every AO component
+// pair (ij) of the bra shell pair, the bra contraction W = sum_L1 E^ij_L1 R_{L1+L2} over the
+// nonzero Hermite coefficients of (ij) only, and for every component pair (kl) of the ket a dot
+// with the ket's nonzero coefficients; the group sums its lanes by a fixed xor tree.  The (bra,
+// ket) angular classes (0: ss, 1: sp or ps, 2: pp) and the component indices are compile-time
+// constants.  Passes: primitive pair records; shell-pair lists; (ij|ij) for every AO pair; the
+// Schwarz factor of every shell pair; shell pairs sorted by class (bra pairs further by their
+// primitive-pair count); per ket shell pair the list of bra shell pairs holding a pair P >= Q's
+// first pair (symmetry: only Q <= P is written, eri[P][Q] and eri[Q][P] both) whose Schwarz
+// product reaches ERI_SCHWARZ_TOL; the output cleared; the main pass, one launch per bra class (pp
+// bras in two: ket classes ss/sp, and pp|pp in three passes over the ket's first component).
 // ---------------------------------------------------------------------------------------------
 constexpr int SREC = 16;  // doubles per (shell pair, primitive pair) record
 constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
@@ -334,6 +336,27 @@ __host__ __device__ constexpr int h_count(int L) { return (L + 1) * (L + 2) * (L
 __host__ __device__ constexpr int h_index(int t, int u, int v) {
   return (t + u + v) * (t + u + v + 1) * (t + u + v + 2) / 6 + (u + v) * (u + v + 1) / 2 + v;
 }
+// (t, u, v) of Hermite index k as constant expressions
+__host__ __device__ constexpr int h_n(int k) {
+  int N = 0;
+  while (h_count(N) <= k) ++N;
+  return N;
+}
+__host__ __device__ constexpr int h_rem(int k) { return k - h_count(h_n(k) - 1); }
+__host__ __device__ constexpr int h_s(int r) {
+  int s = 0;
+  while ((s + 1) * (s + 2) / 2 <= r) ++s;
+  return s;
+}
+__host__ __device__ constexpr int h_tv(int k) { return h_rem(k) - h_s(h_rem(k)) * (h_s(h_rem(k)) + 1) / 2; }
+__host__ __device__ constexpr int h_tu(int k) { return h_s(h_rem(k)) - h_tv(k); }
+__host__ __device__ constexpr int h_tt(int k) { return h_n(k) - h_s(h_rem(k)); }
+constexpr bool h_roundtrip() {
+  for (int k = 0; k < 35; ++k)
+    if (h_index(h_tt(k), h_tu(k), h_tv(k)) != k) return false;
+  return true;
+}
+static_assert(h_roundtrip(), "Hermite index decomposition");
 __device__ constexpr int HT[35] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0,
                                    4, 3, 3, 2, 2, 2, 1, 1, 1, 1, 0, 0, 0, 0, 0};
 __device__ constexpr int HU[35] = {0, 0, 1, 0, 0, 1, 0, 2, 1, 0, 0, 1, 0, 2, 1, 0, 3, 2, 1, 0,
@@ -393,21 +416,24 @@ __device__ __forceinline__ void e_dense(const int* la, const int* lb, const doub
 }
 
 struct EriArgs {
+  Basis bs;
   int nao, npair, nsp, maxp;
-  const int* sh_l;
-  const int* ao_shell;
-  const int* ao_lmn;
-  const int* ao_first;  // [nshell] first AO of each shell
-  const double* rec;    // [nsp][maxp^2][SREC]
-  const int* bra;       // [npair] AO pairs sorted by class
-  const int* ket;       // [nsp] shell pairs sorted by class
-  const int* cnt;       // [0..2] bra class sizes, [3..5] ket class sizes
-  const double* sq;     // [nsp] Schwarz factor max sqrt|(ij|ij)| of each shell pair (main pass)
-  double* diag;         // [npair] (ij|ij) (diagonal pass)
-  const int* live;      // [nsp][npair] per sorted ket: its unscreened bra pairs with P >= qmin, in bra order
-  const int* live_cnt;  // [nsp][3] their counts per bra class
+  int screen;        // Schwarz screening (bases of >= ERI_SCREEN_MIN_PAIRS AO pairs)
+  int k_off, p_off;  // first ket shell pair / screened AO pair of this molecule in the batch's grids
+  int m_off;         // first ket shell pair of this molecule in its main-pass launch
+  long rec_off;      // first primitive-pair record of this molecule in the batch's record grid
+  int* ao_first;     // [nshell] first AO of each shell
+  double* rec;       // [nsp][maxp^2][SREC], the shell pair's na * nb primitive pairs first
+  int* bra;          // [nsp] shell pairs sorted by class, then by primitive-pair count
+  int* ket;          // [nsp] shell pairs sorted by class
+  int* cnt;          // [0..2] bra class sizes, [3..5] ket class sizes
+  double* sq;        // [nsp] Schwarz factor max sqrt|(ij|ij)| of each shell pair
+  double* diag;      // [npair] (ij|ij) (diagonal pass)
+  int* live;         // [nsp][nsp] per sorted ket: its unscreened bra shell pairs holding a P >= qmin, in bra order
+  int* live_cnt;     // [nsp][3] their counts per bra class
   double* eri;
 };
+static_assert(sizeof(EriArgs) % 8 == 0, "EriArgs is copied in 8-byte words");
 
 // (ij|kl) with sqrt|(ij|ij)| sqrt|(kl|kl)| (shell-pair maxima) below this are written as zero: every
 // skipped integral is below it in magnitude (pyscf screens its direct SCF at 1e-13)
@@ -418,168 +444,261 @@ constexpr double ERI_PRIM_TOL = 1e-17;
 // bases with fewer AO pairs are not screened: their diagonal pass would cost more than it saves
 constexpr int ERI_SCREEN_MIN_PAIRS = 2048;
 
+// x >= y with idx = x (x + 1) / 2 + y
+__device__ __forceinline__ void tri_split(int idx, int* x, int* y) {
+  int r = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while (r * (r + 1) / 2 > idx) --r;
+  while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+  *x = r;
+  *y = idx - r * (r + 1) / 2;
+}
+
+// lanes per AO pair in the diagonal pass
+constexpr int ERI_GS = 8;
+
 __device__ __forceinline__ void lmn3(int packed, int* l) {
   l[0] = packed & 3;
   l[1] = (packed >> 2) & 3;
   l[2] = (packed >> 4) & 3;
 }
 
-// GS lanes (a group) share one (P, SQ): member m takes primitive quartets m, m + GS, ... of the
-// pp x pp, and the group's partial sums are combined by a fixed xor tree.
-template <int CB, int CK, bool DIAG = false, int GS = 1>  // DIAG: only (P|P), into g.diag[P]
-__device__ void eri_body(const EriArgs& g, int P, int i, int j, int SQ, int sc, int sd, int m = 0) {
-  constexpr int NB = h_count(CB), NK = h_count(CK), NR = h_count(CB + CK);
-  constexpr int NCC = CK == 0 ? 1 : 3;  // components: (lc, ld) = (0,0): 1x1, (1,0)/(0,1): 3x1 or 1x3, (1,1): 3x3
-  constexpr int NCD = CK == 2 ? 3 : 1;
+// (P|P) for the diagonal pass: GS lanes (a group) share one bra AO pair P = (ij); member m takes
+// primitive quartets m, m + GS, ... of the shell pair with itself and the group's partial sums are
+// combined by a fixed xor tree.  CB: 0 (ss), 1 (sp), 2 (pp) pair.
+template <int CB, int GS>
+__device__ void eri_diag_body(const EriArgs& g, int P, int i, int j, int m) {
+  constexpr int NB = h_count(CB), NR = h_count(2 * CB);
   int la[3], lb[3];
-  lmn3(g.ao_lmn[i], la);
-  lmn3(g.ao_lmn[j], lb);
-  const int sa = g.ao_shell[i], sb = g.ao_shell[j];
+  lmn3(g.bs.ao_lmn[i], la);
+  lmn3(g.bs.ao_lmn[j], lb);
+  const int sa = g.bs.ao_shell[i], sb = g.bs.ao_shell[j];  // sa >= sb: shells are in AO order
   const int pp = g.maxp * g.maxp;
   const double* r1 = g.rec + (size_t)(sa * (sa + 1) / 2 + sb) * pp * SREC;
-  const double* r2 = g.rec + (size_t)SQ * pp * SREC;
-  const int lc = g.sh_l[sc], ld = g.sh_l[sd];
-  const int c0 = g.ao_first[sc], d0 = g.ao_first[sd];
-  double acc[NCC * NCD];
-#pragma unroll
-  for (int q = 0; q < NCC * NCD; ++q) acc[q] = 0.0;
-  for (int qi = m; qi < pp * pp; qi += GS) {
-    const int a = qi / pp, c = qi % pp;
+  const int npa = g.bs.sh_nprim[sa] * g.bs.sh_nprim[sb], nq = npa * npa;
+  double acc = 0.0;
+  for (int qi = m; qi < nq; qi += GS) {
+    const int a = qi / npa, c = qi - a * npa;
     const double* x1 = r1 + a * SREC;
-    const double c1 = x1[4];
-    if (c1 == 0.0) continue;
-    const double p = x1[0];
-    double E1[NB];
+    const double* x2 = r1 + c * SREC;
+    const double c1 = x1[4], c2 = x2[4];
+    if (c1 == 0.0 || c2 == 0.0) continue;
+    const double p = x1[0], q = x2[0];
+    const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
+    if (fabs(pref) < (CB == 0 ? ERI_PRIM_TOL : 1e-3 * ERI_PRIM_TOL)) continue;
+    double E1[NB], E2[NB];
     e_dense<CB>(la, lb, x1 + 5, x1 + 8, x1[11], E1);
-    {
-      const double* x2 = r2 + c * SREC;
-      const double c2 = x2[4];
-      if (c2 == 0.0) continue;
-      const double q = x2[0];
-      // the quartet's (ss|ss)-type magnitude bound; p functions add Hermite factors (distances,
-      // 1/2p), so their quartets get a threshold three orders lower
-      const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
-      if (fabs(pref) < (CB + CK == 0 ? ERI_PRIM_TOL : 1e-3 * ERI_PRIM_TOL)) continue;
-      const double alpha = p * q / (p + q);
-      const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
-      double F[CB + CK + 1];
-      boys_upto<CB + CK>(alpha * (X * X + Y * Y + Z * Z), F);
-      double R[NR];
-      rtable<CB + CK>(alpha, X, Y, Z, F, R);
-      // W[L2] = sum_L1 E1[L1] R[L1 + L2], with the ket sign (-1)^|L2|
-      double W[NK];
+    e_dense<CB>(la, lb, x2 + 5, x2 + 8, x2[11], E2);
+    const double alpha = p * q / (p + q);
+    const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
+    double F[2 * CB + 1];
+    boys_upto<2 * CB>(alpha * (X * X + Y * Y + Z * Z), F);
+    double R[NR];
+    rtable<2 * CB>(alpha, X, Y, Z, F, R);
+    double v = 0.0;
 #pragma unroll
-      for (int k2 = 0; k2 < NK; ++k2) {
-        double w = 0.0;
+    for (int k2 = 0; k2 < NB; ++k2) {
+      double w = 0.0;
 #pragma unroll
-        for (int k1 = 0; k1 < NB; ++k1)
-          w = fma(E1[k1], R[h_index(HT[k1] + HT[k2], HU[k1] + HU[k2], HV[k1] + HV[k2])], w);
-        W[k2] = ((HT[k2] + HU[k2] + HV[k2]) & 1) ? -w : w;
-      }
-#pragma unroll
-      for (int cc = 0; cc < NCC; ++cc)
-#pragma unroll
-        for (int cd = 0; cd < NCD; ++cd) {
-          // component directions: a p function on the ket side points along its component index
-          int lk[3] = {0, 0, 0}, lm[3] = {0, 0, 0};
-          if (CK == 2) {
-            lk[cc] = 1;
-            lm[cd] = 1;
-          } else if (CK == 1) {
-            if (lc == 1) lk[cc] = 1;
-            else lm[cc] = 1;
-          }
-          double E2[NK];
-          e_dense<CK>(lk, lm, x2 + 5, x2 + 8, x2[11], E2);
-          double v = 0.0;
-#pragma unroll
-          for (int k2 = 0; k2 < NK; ++k2) v = fma(E2[k2], W[k2], v);
-          acc[cc * NCD + cd] = fma(pref, v, acc[cc * NCD + cd]);
-        }
+      for (int k1 = 0; k1 < NB; ++k1)
+        w = fma(E1[k1], R[h_index(HT[k1] + HT[k2], HU[k1] + HU[k2], HV[k1] + HV[k2])], w);
+      v = fma(E2[k2], ((HT[k2] + HU[k2] + HV[k2]) & 1) ? -w : w, v);
     }
+    acc = fma(pref, v, acc);
   }
 #pragma unroll
-  for (int cc = 0; cc < NCC; ++cc)
+  for (int o = 1; o < GS; o <<= 1) acc += __shfl_xor(acc, o, 64);
+  if (m == 0) g.diag[P] = acc;
+}
+
+// compile-time loop: f(std::integral_constant<int, 0>), ..., f(std::integral_constant<int, N - 1>)
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn&& f) {
+  if constexpr (N > 0) {
+    static_for<N - 1>(f);
+    f(std::integral_constant<int, N - 1>{});
+  }
+}
+
+// Nonzero Hermite coefficients of one AO component pair of a shell pair (K = 1; the records fold
+// the Gaussian product factor into c).  Component BC of class C: C = 0 no p function; C = 1 one p
+// function along BC, on the first shell when `first` is set (coefficients PA_d or PB_d, 1/2p);
+// C = 2 the first shell's p along BC / 3 and the second's along BC % 3 (along one direction: the
+// three coefficients of E^{11}; else the product of two E^{10}/E^{01} pairs).
+// hsum<C, BC, K2>(...) = sum over those coefficients e_tuv of e_tuv * R[(t, u, v) + k2].
+template <int C, int BC, int K2>
+__device__ __forceinline__ double hsum(bool first, const double* PA, const double* PB, double i2p, const double* R) {
+  constexpr int T2 = h_tt(K2), U2 = h_tu(K2), V2 = h_tv(K2);
+  if constexpr (C == 0) {
+    return R[K2];
+  } else if constexpr (C == 1) {
+    constexpr int D = BC;
+    constexpr int up = h_index(T2 + (D == 0), U2 + (D == 1), V2 + (D == 2));
+    const double e0 = first ? PA[D] : PB[D];
+    return fma(e0, R[K2], i2p * R[up]);
+  } else {
+    constexpr int DX = BC / 3, DY = BC % 3;
+    if constexpr (DX == DY) {
+      constexpr int u1 = h_index(T2 + (DX == 0), U2 + (DX == 1), V2 + (DX == 2));
+      constexpr int u2 = h_index(T2 + 2 * (DX == 0), U2 + 2 * (DX == 1), V2 + 2 * (DX == 2));
+      const double e0 = fma(PA[DX], PB[DX], i2p), e1 = (PA[DX] + PB[DX]) * i2p, e2 = i2p * i2p;
+      return fma(e0, R[K2], fma(e1, R[u1], e2 * R[u2]));
+    } else {
+      constexpr int ux = h_index(T2 + (DX == 0), U2 + (DX == 1), V2 + (DX == 2));
+      constexpr int uy = h_index(T2 + (DY == 0), U2 + (DY == 1), V2 + (DY == 2));
+      constexpr int uxy = h_index(T2 + (DX == 0) + (DY == 0), U2 + (DX == 1) + (DY == 1), V2 + (DX == 2) + (DY == 2));
+      const double a0 = PA[DX], b0 = PB[DY];
+      return fma(a0 * b0, R[K2], fma(i2p * b0, R[ux], fma(a0 * i2p, R[uy], i2p * i2p * R[uxy])));
+    }
+  }
+}
+
+// The shell quartet (SP = (sa, sb) | SQ = (sc, sd)) of classes (CB, CK) for one group of ERI_GS
+// lanes; KCC >= 0 (pp|pp): only the ket components whose first index is KCC.
+template <int CB, int CK, int GS, int KCC = -1>
+__device__ void eri_shell_body(const EriArgs& g, int SP, int sa, int sb, int SQ, int sc, int sd, int m) {
+  constexpr int L = CB + CK, NR = h_count(L), NK = h_count(CK);
+  constexpr int NBC = CB == 0 ? 1 : CB == 1 ? 3 : 9;
+  constexpr int NKC = CK == 0 ? 1 : CK == 1 ? 3 : (KCC >= 0 ? 3 : 9);
+  constexpr int KC0 = (CK == 2 && KCC >= 0) ? 3 * KCC : 0;
+  const bool pa = g.bs.sh_l[sa] == 1, pc = g.bs.sh_l[sc] == 1;  // class 1: the p function is on the first shell
+  const int pp = g.maxp * g.maxp;
+  const double* r1 = g.rec + (size_t)SP * pp * SREC;
+  const double* r2 = g.rec + (size_t)SQ * pp * SREC;
+  const int npc = g.bs.sh_nprim[sc] * g.bs.sh_nprim[sd], nq = g.bs.sh_nprim[sa] * g.bs.sh_nprim[sb] * npc;
+  double acc[NBC][NKC];
 #pragma unroll
-    for (int cd = 0; cd < NCD; ++cd) {
-      int k, l;
-      if (CK == 2) {
-        k = c0 + cc;
-        l = d0 + cd;
-      } else if (CK == 1) {
-        k = lc == 1 ? c0 + cc : c0;
-        l = lc == 1 ? d0 : d0 + cc;
-      } else {
-        k = c0;
-        l = d0;
-      }
-      if (k < l) continue;  // same p shell: only k >= l
-      const int Q = k * (k + 1) / 2 + l;
-      if (Q > P) continue;
-      double val = acc[cc * NCD + cd];
+  for (int x = 0; x < NBC; ++x)
+#pragma unroll
+    for (int y = 0; y < NKC; ++y) acc[x][y] = 0.0;
+  for (int qi = m; qi < nq; qi += GS) {
+    const int a = qi / npc, c = qi - a * npc;
+    const double* x1 = r1 + a * SREC;
+    const double* x2 = r2 + c * SREC;
+    const double c1 = x1[4], c2 = x2[4];
+    if (c1 == 0.0 || c2 == 0.0) continue;
+    const double p = x1[0], q = x2[0];
+    // the quartet's (ss|ss)-type magnitude bound; p functions add Hermite factors (distances,
+    // 1/2p), so their quartets get a threshold three orders lower
+    const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
+    if (fabs(pref) < (L == 0 ? ERI_PRIM_TOL : 1e-3 * ERI_PRIM_TOL)) continue;
+    const double alpha = p * q / (p + q);
+    const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
+    double F[L + 1];
+    boys_upto<L>(alpha * (X * X + Y * Y + Z * Z), F);
+    double R[NR];
+    rtable<L>(alpha, X, Y, Z, F, R);
+    static_for<NBC>([&](auto bc) {
+      constexpr int BC = decltype(bc)::value;
+      // W[k2] = (-1)^|k2| sum_L1 E^ij_L1 R_{L1 + k2}
+      double W[NK];
+      static_for<NK>([&](auto k2) {
+        constexpr int K2 = decltype(k2)::value;
+        const double w = hsum<CB, BC, K2>(pa, x1 + 5, x1 + 8, x1[11], R);
+        W[K2] = ((h_tt(K2) + h_tu(K2) + h_tv(K2)) & 1) ? -w : w;
+      });
+      static_for<NKC>([&](auto kc) {
+        constexpr int KI = decltype(kc)::value;
+        acc[BC][KI] = fma(pref, hsum<CK, KC0 + KI, 0>(pc, x2 + 5, x2 + 8, x2[11], W), acc[BC][KI]);
+      });
+    });
+  }
+  const int a0 = g.ao_first[sa], b0 = g.ao_first[sb], c0 = g.ao_first[sc], d0 = g.ao_first[sd];
+#pragma unroll
+  for (int x = 0; x < NBC; ++x)
+#pragma unroll
+    for (int y = 0; y < NKC; ++y) {
+      double val = acc[x][y];
 #pragma unroll
       for (int o = 1; o < GS; o <<= 1) val += __shfl_xor(val, o, 64);
-      if (m != 0) continue;
-      if (DIAG) {
-        if (Q == P) g.diag[P] = val;
-      } else {
-        g.eri[(size_t)P * g.npair + Q] = val;
-        g.eri[(size_t)Q * g.npair + P] = val;
-      }
+      const int i = CB == 0 ? a0 : CB == 1 ? (pa ? a0 + x : a0) : a0 + x / 3;
+      const int j = CB == 0 ? b0 : CB == 1 ? (pa ? b0 : b0 + x) : b0 + x % 3;
+      const int kc = KC0 + y;
+      const int k = CK == 0 ? c0 : CK == 1 ? (pc ? c0 + kc : c0) : c0 + kc / 3;
+      const int l = CK == 0 ? d0 : CK == 1 ? (pc ? d0 : d0 + kc) : d0 + kc % 3;
+      if (m != 0 || i < j || k < l) continue;  // same p shell: only i >= j, k >= l
+      const int P = i * (i + 1) / 2 + j, Q = k * (k + 1) / 2 + l;
+      if (Q > P) continue;
+      g.eri[(size_t)P * g.npair + Q] = val;
+      g.eri[(size_t)Q * g.npair + P] = val;
     }
 }
 
-// one wave per workgroup: blockIdx.y = ket shell pair (sorted), groups of ERI_GS lanes = consecutive
-// sorted bra pairs (64 / ERI_GS per wave)
-constexpr int ERI_GS = 8;
-__global__ __launch_bounds__(64) void k_eri(EriArgs g) {
-  const int ky = blockIdx.y;
+// the molecule b of a batch with off(b) <= idx < off(b + 1) (off non-decreasing from off(0) = 0)
+template <class Off>
+__device__ __forceinline__ int eri_find(int B, long idx, Off off) {
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off(mid) <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Main pass for bra class CB: blockIdx.y = sorted ket shell pair over the batch's molecules, whose
+// class must be in KMASK; the gridDim.x waves of a ket take chunks of 64 / GS consecutive live bra
+// shell pairs of class CB in turn, GS lanes per shell quartet (screened entries stay as the zero the
+// output was cleared to).
+template <int CB, int KMASK, int GS>
+__global__ __launch_bounds__(64) void k_eri_cls(const EriArgs* __restrict__ gs, int B) {
+  const int b = eri_find(B, blockIdx.y, [&](int x) { return (long)gs[x].m_off; });
+  const EriArgs& g = gs[b];
+  const int ky = blockIdx.y - g.m_off;
   const int kc0 = g.cnt[3], kc1 = g.cnt[4];
   const int ck = ky < kc0 ? 0 : ky < kc0 + kc1 ? 1 : 2;
+  if (!((KMASK >> ck) & 1)) return;
   const int SQ = g.ket[ky];
-  int sc = (int)((sqrt(8.0 * SQ + 1.0) - 1.0) * 0.5);
-  while (sc * (sc + 1) / 2 > SQ) --sc;
-  while ((sc + 1) * (sc + 2) / 2 <= SQ) ++sc;
-  const int sd = SQ - sc * (sc + 1) / 2;
-  // this ket's live bra pairs (screened entries stay as the zero the output was filled with); waves
-  // past the live count leave at once
+  int sc, sd;
+  tri_split(SQ, &sc, &sd);
   const int* lc3 = g.live_cnt + 3 * ky;
-  const int bc0 = lc3[0], bc1 = lc3[1], nlive = bc0 + bc1 + lc3[2];
-  const int bx = blockIdx.x * (64 / ERI_GS) + threadIdx.x / ERI_GS, m = threadIdx.x % ERI_GS;
-  if (bx >= nlive) return;
-  const int P = g.live[(size_t)ky * g.npair + bx];
-  int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
-  while (i * (i + 1) / 2 > P) --i;
-  while ((i + 1) * (i + 2) / 2 <= P) ++i;
-  const int j = P - i * (i + 1) / 2;
-  const int cb = bx < bc0 ? 0 : bx < bc0 + bc1 ? 1 : 2;
-  switch (cb * 3 + ck) {
-    case 0: eri_body<0, 0, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 1: eri_body<0, 1, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 2: eri_body<0, 2, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 3: eri_body<1, 0, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 4: eri_body<1, 1, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 5: eri_body<1, 2, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 6: eri_body<2, 0, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    case 7: eri_body<2, 1, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
-    default: eri_body<2, 2, false, ERI_GS>(g, P, i, j, SQ, sc, sd, m); break;
+  const int nlive = lc3[CB];
+  const int* list = g.live + (size_t)ky * g.nsp + (CB == 0 ? 0 : CB == 1 ? lc3[0] : lc3[0] + lc3[1]);
+  const int m = threadIdx.x % GS;
+  for (int b0 = blockIdx.x * (64 / GS); b0 < nlive; b0 += gridDim.x * (64 / GS)) {
+    const int bx = b0 + threadIdx.x / GS;
+    if (bx >= nlive) break;
+    const int SP = list[bx];
+    int sa, sb;
+    tri_split(SP, &sa, &sb);
+    if (ck == 0) {
+      eri_shell_body<CB, 0, GS>(g, SP, sa, sb, SQ, sc, sd, m);
+    } else if (ck == 1) {
+      eri_shell_body<CB, 1, GS>(g, SP, sa, sb, SQ, sc, sd, m);
+    } else if constexpr (CB == 2) {
+      eri_shell_body<2, 2, GS, 0>(g, SP, sa, sb, SQ, sc, sd, m);
+      eri_shell_body<2, 2, GS, 1>(g, SP, sa, sb, SQ, sc, sd, m);
+      eri_shell_body<2, 2, GS, 2>(g, SP, sa, sb, SQ, sc, sd, m);
+    } else {
+      eri_shell_body<CB, 2, GS>(g, SP, sa, sb, SQ, sc, sd, m);
+    }
   }
+}
+
+// The main pass with GS lanes per shell quartet and xw waves per ket shell pair over nket kets.
+template <int GS>
+void eri_main(const EriArgs* gs, int B, int nket, int xw, hipStream_t st) {
+  const dim3 eg((unsigned)xw, (unsigned)nket);
+  hipLaunchKernelGGL((k_eri_cls<2, 4, GS>), eg, dim3(64), 0, st, gs, B);
+  hipLaunchKernelGGL((k_eri_cls<2, 3, GS>), eg, dim3(64), 0, st, gs, B);
+  hipLaunchKernelGGL((k_eri_cls<1, 7, GS>), eg, dim3(64), 0, st, gs, B);
+  hipLaunchKernelGGL((k_eri_cls<0, 7, GS>), eg, dim3(64), 0, st, gs, B);
 }
 
 // Work lists of the main pass: one workgroup per sorted ket shell pair ky compacts, class segment
-// by class segment of the sorted bra list, the bra pairs P >= qmin whose Schwarz product passes.
-__global__ __launch_bounds__(256) void k_eri_live(EriArgs g, int* live, int* live_cnt) {
+// by class segment of the sorted bra list, the bra shell pairs holding a pair P >= qmin whose
+// Schwarz product passes.
+__global__ __launch_bounds__(256) void k_eri_live(const EriArgs* __restrict__ gs, int B) {
   __shared__ int wtot[4];
   __shared__ int base;
-  const int ky = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int bm = eri_find(B, blockIdx.x, [&](int x) { return (long)gs[x].k_off; });
+  const EriArgs& g = gs[bm];
+  const int ky = blockIdx.x - g.k_off, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int SQ = g.ket[ky];
-  int sc = (int)((sqrt(8.0 * SQ + 1.0) - 1.0) * 0.5);
-  while (sc * (sc + 1) / 2 > SQ) --sc;
-  while ((sc + 1) * (sc + 2) / 2 <= SQ) ++sc;
-  const int sd = SQ - sc * (sc + 1) / 2;
+  int sc, sd;
+  tri_split(SQ, &sc, &sd);
   const int qmin = g.ao_first[sc] * (g.ao_first[sc] + 1) / 2 + g.ao_first[sd];
   const double sk = g.sq[SQ];
-  int* out = live + (size_t)ky * g.npair;
+  int* out = g.live + (size_t)ky * g.nsp;
   int seg0 = 0, off = 0;
   for (int c = 0; c < 3; ++c) {
     const int seg1 = seg0 + g.cnt[c];
@@ -589,13 +708,12 @@ __global__ __launch_bounds__(256) void k_eri_live(EriArgs g, int* live, int* liv
       const int bx = b0 + tid;
       bool flag = false;
       if (bx < seg1) {
-        const int P = g.bra[bx];
-        int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
-        while (i * (i + 1) / 2 > P) --i;
-        while ((i + 1) * (i + 2) / 2 <= P) ++i;
-        const int j = P - i * (i + 1) / 2;
-        const int sa = g.ao_shell[i], sb = g.ao_shell[j];
-        flag = P >= qmin && g.sq[sa * (sa + 1) / 2 + sb] * sk >= ERI_SCHWARZ_TOL;
+        const int SP = g.bra[bx];
+        int sa, sb;
+        tri_split(SP, &sa, &sb);
+        const int i = g.ao_first[sa] + (g.bs.sh_l[sa] == 0 ? 0 : 2);
+        const int j = g.ao_first[sb] + (g.bs.sh_l[sb] == 0 ? 0 : 2);
+        flag = i * (i + 1) / 2 + j >= qmin && g.sq[SP] * sk >= ERI_SCHWARZ_TOL;
       }
       const unsigned long long msk = __ballot(flag);
       if (lane == 0) wtot[wv] = __popcll(msk);
@@ -607,140 +725,174 @@ __global__ __launch_bounds__(256) void k_eri_live(EriArgs g, int* live, int* liv
       if (tid == 0) base += wtot[0] + wtot[1] + wtot[2] + wtot[3];
       __syncthreads();
     }
-    if (tid == 0) live_cnt[3 * ky + c] = base;
+    if (tid == 0) g.live_cnt[3 * ky + c] = base;
     off += base;
     seg0 = seg1;
     __syncthreads();
   }
 }
 
-// Diagonal pass: (ij|ij) for every AO pair P = (ij), into diag[P].
-__global__ __launch_bounds__(64) void k_eri_diag(EriArgs g) {
-  const int P = blockIdx.x * (64 / ERI_GS) + threadIdx.x / ERI_GS, m = threadIdx.x % ERI_GS;
-  if (P >= g.npair) return;
-  int i = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
-  while (i * (i + 1) / 2 > P) --i;
-  while ((i + 1) * (i + 2) / 2 <= P) ++i;
-  const int j = P - i * (i + 1) / 2;
-  const int sa = g.ao_shell[i], sb = g.ao_shell[j];  // sa >= sb: shells are in AO order
-  const int SQ = sa * (sa + 1) / 2 + sb;
-  const int cb = (g.ao_lmn[i] != 0) + (g.ao_lmn[j] != 0);
-  if (cb == 0) eri_body<0, 0, true, ERI_GS>(g, P, i, j, SQ, sa, sb, m);
-  else if (cb == 1) eri_body<1, 1, true, ERI_GS>(g, P, i, j, SQ, sa, sb, m);
-  else eri_body<2, 2, true, ERI_GS>(g, P, i, j, SQ, sa, sb, m);
+// Diagonal pass over the screened molecules' AO pairs: (ij|ij) for every AO pair P = (ij), into
+// diag[P].
+__global__ __launch_bounds__(64) void k_eri_diag(const EriArgs* __restrict__ gs, int B, int total) {
+  const int t = blockIdx.x * (64 / ERI_GS) + threadIdx.x / ERI_GS, m = threadIdx.x % ERI_GS;
+  if (t >= total) return;
+  const EriArgs& g = gs[eri_find(B, t, [&](int x) { return (long)gs[x].p_off; })];
+  const int P = t - g.p_off;
+  int i, j;
+  tri_split(P, &i, &j);
+  const int cb = (g.bs.ao_lmn[i] != 0) + (g.bs.ao_lmn[j] != 0);
+  if (cb == 0) eri_diag_body<0, ERI_GS>(g, P, i, j, m);
+  else if (cb == 1) eri_diag_body<1, ERI_GS>(g, P, i, j, m);
+  else eri_diag_body<2, ERI_GS>(g, P, i, j, m);
 }
 
-// sq[SP] = max over the AO pairs (ij) of shell pair SP of sqrt|(ij|ij)|
-__global__ void k_eri_schwarz(int nsp, const int* sh_l, const int* ao_first, const double* diag, double* sq) {
-  const int SP = blockIdx.x * blockDim.x + threadIdx.x;
-  if (SP >= nsp) return;
-  int sa = (int)((sqrt(8.0 * SP + 1.0) - 1.0) * 0.5);
-  while (sa * (sa + 1) / 2 > SP) --sa;
-  while ((sa + 1) * (sa + 2) / 2 <= SP) ++sa;
-  const int sb = SP - sa * (sa + 1) / 2;
-  const int na = sh_l[sa] == 0 ? 1 : 3, nb = sh_l[sb] == 0 ? 1 : 3;
+// sq[SP] = max over the AO pairs (ij) of shell pair SP of sqrt|(ij|ij)| (1 for unscreened molecules)
+__global__ void k_eri_schwarz(const EriArgs* __restrict__ gs, int B, int total) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const EriArgs& g = gs[eri_find(B, t, [&](int x) { return (long)gs[x].k_off; })];
+  const int SP = t - g.k_off;
+  if (!g.screen) {
+    g.sq[SP] = 1.0;
+    return;
+  }
+  int sa, sb;
+  tri_split(SP, &sa, &sb);
+  const int na = g.bs.sh_l[sa] == 0 ? 1 : 3, nb = g.bs.sh_l[sb] == 0 ? 1 : 3;
   double m = 0.0;
   for (int x = 0; x < na; ++x)
     for (int y = 0; y < nb; ++y) {
-      const int i = ao_first[sa] + x, j = ao_first[sb] + y;
+      const int i = g.ao_first[sa] + x, j = g.ao_first[sb] + y;
       if (j > i) continue;
       const int P = i * (i + 1) / 2 + j;
-      m = fmax(m, sqrt(fabs(diag[P])));
+      m = fmax(m, sqrt(fabs(g.diag[P])));
     }
-  sq[SP] = m;
+  g.sq[SP] = m;
 }
 
-// Shell-pair primitive records: p, P (3), c_a c_b exp(-mu |AB|^2), P - A (3), P - B (3), 1/(2p).
+// Shell-pair primitive records: p, P (3), c_a c_b exp(-mu |AB|^2), P - A (3), P - B (3), 1/(2p);
+// record r = pa * nb + pb of the shell pair's maxp^2 slots for r < na * nb, the rest unused (c = 0).
 // Primitive pairs whose Gaussian product prefactor is below 1e-20 get c = 0 (they contribute
 // < 1e-18 to any integral) and are skipped.
-__global__ void k_eri_records(Basis b, int nsp, int maxp, double* rec) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int pp = maxp * maxp;
-  if (idx >= (long)nsp * pp) return;
+__global__ void k_eri_records(const EriArgs* __restrict__ gs, int B, long total) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const EriArgs& g = gs[eri_find(B, t, [&](int x) { return gs[x].rec_off; })];
+  const Basis& b = g.bs;
+  const long idx = t - g.rec_off;
+  const int pp = g.maxp * g.maxp;
   const int SP = (int)(idx / pp), r = (int)(idx % pp);
-  int sa = (int)((sqrt(8.0 * SP + 1.0) - 1.0) * 0.5);
-  while (sa * (sa + 1) / 2 > SP) --sa;
-  while ((sa + 1) * (sa + 2) / 2 <= SP) ++sa;
-  const int sb = SP - sa * (sa + 1) / 2;
-  double* o = rec + idx * SREC;
-  const int pa = r / maxp, pb = r % maxp;
-  if (pa >= b.sh_nprim[sa] || pb >= b.sh_nprim[sb]) {
+  int sa, sb;
+  tri_split(SP, &sa, &sb);
+  double* o = g.rec + idx * SREC;
+  const int nb = b.sh_nprim[sb];
+  if (r >= b.sh_nprim[sa] * nb) {
     o[4] = 0.0;
     return;
   }
+  const int pa = r / nb, pb = r % nb;
   const double* A = b.xyz + 3 * b.sh_atom[sa];
-  const double* B = b.xyz + 3 * b.sh_atom[sb];
+  const double* Bc = b.xyz + 3 * b.sh_atom[sb];
   const double ea = b.exps[b.sh_off[sa] + pa], eb = b.exps[b.sh_off[sb] + pb];
   const double p = ea + eb, mu = ea * eb / p;
   const double cab = b.coefs[b.sh_off[sa] + pa] * b.coefs[b.sh_off[sb] + pb];
-  const double r2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+  const double r2 = (A[0] - Bc[0]) * (A[0] - Bc[0]) + (A[1] - Bc[1]) * (A[1] - Bc[1]) + (A[2] - Bc[2]) * (A[2] - Bc[2]);
   const double K = exp(-mu * r2);
   o[0] = p;
   for (int d = 0; d < 3; ++d) {
-    const double Pd = (ea * A[d] + eb * B[d]) / p;
+    const double Pd = (ea * A[d] + eb * Bc[d]) / p;
     o[1 + d] = Pd;
     o[5 + d] = Pd - A[d];
-    o[8 + d] = Pd - B[d];
+    o[8 + d] = Pd - Bc[d];
   }
   o[11] = 0.5 / p;
   o[4] = fabs(cab) * K * (M_PI / p) * sqrt(M_PI / p) < 1e-20 ? 0.0 : cab * K;
 }
 
-// One workgroup of 1024.  phase 0: ao_first (first AO of each shell).  phase 1: the work lists,
-// each a stable partition by class: bra AO pairs by l(i) + l(j), ket shell pairs by l(a) + l(b);
-// cnt[0..2] bra and [3..5] ket class sizes.
-__global__ __launch_bounds__(1024) void k_eri_setup(Basis b, int npair, int nsp, int* ao_first, int* bra, int* ket,
-                                                    int* cnt, int phase) {
+// One workgroup of 1024 per molecule: ao_first (first AO of each shell), then the work lists of
+// shell pairs, each a stable partition by class l(a) + l(b), the bra list within a class by
+// primitive-pair count, largest first; cnt[0..2] bra and [3..5] ket class sizes.
+__global__ __launch_bounds__(1024) void k_eri_setup(const EriArgs* __restrict__ gs) {
   __shared__ int wtot[16];
   __shared__ int base;
+  __shared__ unsigned long long keys;
+  const EriArgs& g = gs[blockIdx.x];
+  const Basis& b = g.bs;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (phase == 0) {
-    for (int s = tid; s < b.nshell; s += 1024) {
-      int f = 0;
-      for (int t = 0; t < s; ++t) f += b.sh_l[t] == 0 ? 1 : 3;
-      ao_first[s] = f;
-    }
-    return;
+  for (int s = tid; s < b.nshell; s += 1024) {
+    int f = 0;
+    for (int t = 0; t < s; ++t) f += b.sh_l[t] == 0 ? 1 : 3;
+    g.ao_first[s] = f;
   }
+  const int n = g.nsp;
   for (int list = 0; list < 2; ++list) {
-    const int n = list == 0 ? npair : nsp;
-    int* out = list == 0 ? bra : ket;
+    int* out = list == 0 ? g.bra : g.ket;
     int off = 0;
     for (int c = 0; c < 3; ++c) {
-      if (tid == 0) base = 0;
-      __syncthreads();
-      for (int c0 = 0; c0 < n; c0 += 1024) {
-        const int idx = c0 + tid;
-        bool flag = false;
-        if (idx < n) {
-          int x = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-          while (x * (x + 1) / 2 > idx) --x;
-          while ((x + 1) * (x + 2) / 2 <= idx) ++x;
-          const int y = idx - x * (x + 1) / 2;
-          flag = (list == 0 ? (b.ao_lmn[x] != 0) + (b.ao_lmn[y] != 0) : b.sh_l[x] + b.sh_l[y]) == c;
-        }
-        const unsigned long long m = __ballot(flag);
-        if (lane == 0) wtot[wv] = __popcll(m);
-        __syncthreads();
-        int wb = 0;
-        for (int w = 0; w < wv; ++w) wb += wtot[w];
-        if (flag) out[off + base + wb + __popcll(m & ((1ull << lane) - 1))] = idx;
-        __syncthreads();
-        if (tid == 0)
-          for (int w = 0; w < 16; ++w) base += wtot[w];
-        __syncthreads();
+      // the bra pairs of a class are further ordered by their primitive-pair count (largest first), so
+      // the lanes of a wave loop over the same number of primitive quartets
+      if (tid == 0) {
+        base = 0;
+        keys = 0ull;
       }
-      if (tid == 0) cnt[list * 3 + c] = base;
+      __syncthreads();
+      for (int idx = tid; list == 0 && idx < n; idx += 1024) {
+        int x, y;
+        tri_split(idx, &x, &y);
+        if (b.sh_l[x] + b.sh_l[y] == c) atomicOr(&keys, 1ull << (b.sh_nprim[x] * b.sh_nprim[y] - 1));
+      }
+      __syncthreads();
+      const unsigned long long present = list == 0 ? keys : 1ull;
+      for (int v = 63; v >= 0; --v) {
+        if (!((present >> v) & 1ull)) continue;
+        for (int c0 = 0; c0 < n; c0 += 1024) {
+          const int idx = c0 + tid;
+          bool flag = false;
+          if (idx < n) {
+            int x, y;
+            tri_split(idx, &x, &y);
+            flag = b.sh_l[x] + b.sh_l[y] == c && (list == 1 || b.sh_nprim[x] * b.sh_nprim[y] == v + 1);
+          }
+          const unsigned long long m = __ballot(flag);
+          if (lane == 0) wtot[wv] = __popcll(m);
+          __syncthreads();
+          int wb = 0;
+          for (int w = 0; w < wv; ++w) wb += wtot[w];
+          if (flag) out[off + base + wb + __popcll(m & ((1ull << lane) - 1))] = idx;
+          __syncthreads();
+          if (tid == 0)
+            for (int w = 0; w < 16; ++w) base += wtot[w];
+          __syncthreads();
+        }
+      }
+      if (tid == 0) g.cnt[list * 3 + c] = base;
       off += base;
       __syncthreads();
     }
   }
 }
 
-// x[0..n) = v (the ERI pair matrix is cleared before the main pass: screened entries are not
-// written; unscreened bases get Schwarz factors of 1)
-__global__ void k_fill(double* x, size_t n, double v) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = v;
+// The pair matrices cleared before the main pass (screened entries are not written); blockIdx.y =
+// molecule.
+__global__ void k_eri_clear(const EriArgs* __restrict__ gs) {
+  const EriArgs& g = gs[blockIdx.y];
+  const size_t n = (size_t)g.npair * g.npair;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    g.eri[t] = 0.0;
+}
+
+// EriArgs records into device memory from kernel arguments (no host copy to wait for)
+constexpr int ERI_PUT_N = 12;
+struct EriPut {
+  EriArgs r[ERI_PUT_N];
+};
+__global__ void k_put_eri(EriPut pk, int count, EriArgs* dst) {
+  constexpr int W = sizeof(EriArgs) / 8;
+  for (int t = threadIdx.x; t < count * W; t += blockDim.x) {
+    const int i = t / W, w = t % W;
+    reinterpret_cast<unsigned long long*>(dst + i)[w] = reinterpret_cast<const unsigned long long*>(&pk.r[i])[w];
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2500,80 +2652,129 @@ int qex3d_int1e(const qex3d_basis* b, double* S, double* T, double* V, void* str
   return launch_status("qex3d_int1e");
 }
 
-// workspace: shell-pair records, then ao_first [nshell], bra [npair], ket [nsp], class sizes [8]
-size_t qex3d_eri_workspace_bytes(const qex3d_basis* b) {
-  if (check_basis(b) != QEX_OK) return 0;
+// Per molecule: shell-pair records, ao_first [nshell], bra [nsp], ket [nsp], class sizes [8],
+// Schwarz factors [nsp], (ij|ij) [npair], live lists [nsp][nsp] and their counts [nsp][3]; a batch
+// adds its EriArgs records in front.
+static size_t eri_mol_bytes(const qex3d_basis* b) {
   const long np = (long)b->nao * (b->nao + 1) / 2;
   const long nsp = (long)b->nshell * (b->nshell + 1) / 2;
-  return al((size_t)nsp * b->max_nprim * b->max_nprim * SREC * 8) + al((size_t)b->nshell * 4) + al((size_t)np * 4) +
-         al((size_t)nsp * 4) + al(64) + al((size_t)nsp * 8) + al((size_t)np * 8) + al((size_t)nsp * np * 4) +
+  return al((size_t)nsp * b->max_nprim * b->max_nprim * SREC * 8) + al((size_t)b->nshell * 4) + al((size_t)nsp * 4) +
+         al((size_t)nsp * 4) + al(64) + al((size_t)nsp * 8) + al((size_t)np * 8) + al((size_t)nsp * nsp * 4) +
          al((size_t)nsp * 3 * 4);
 }
 
-int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream) {
-  int rc = check_basis(b);
-  if (rc) return rc;
-  if (!eri) return fail3(QEX_E_BADARG, "null output");
-  const size_t need = qex3d_eri_workspace_bytes(b);
+size_t qex3d_eri_batch_workspace_bytes(int32_t B, const qex3d_basis* bases) {
+  if (B < 1 || B > QEX3D_MAX_ERI_BATCH || !bases) return 0;
+  size_t s = al((size_t)B * sizeof(EriArgs));
+  for (int i = 0; i < B; ++i) {
+    if (check_basis(bases + i) != QEX_OK) return 0;
+    s += eri_mol_bytes(bases + i);
+  }
+  return s;
+}
+
+size_t qex3d_eri_workspace_bytes(const qex3d_basis* b) { return qex3d_eri_batch_workspace_bytes(1, b); }
+
+int qex3d_eri_batch(int32_t B, const qex3d_basis* bases, double* const* eri, void* workspace, size_t workspace_bytes,
+                    void* stream) {
+  if (B < 1 || B > QEX3D_MAX_ERI_BATCH) return fail3(QEX_E_BADARG, "batch size %d outside 1..%d", B, QEX3D_MAX_ERI_BATCH);
+  if (!bases || !eri) return fail3(QEX_E_BADARG, "null bases or outputs");
+  for (int i = 0; i < B; ++i) {
+    const int rc = check_basis(bases + i);
+    if (rc) return rc;
+    if (!eri[i]) return fail3(QEX_E_BADARG, "null output of molecule %d", i);
+  }
+  const size_t need = qex3d_eri_batch_workspace_bytes(B, bases);
   if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
-  const int np = b->nao * (b->nao + 1) / 2;
-  const int nsp = b->nshell * (b->nshell + 1) / 2;
-  const int maxp = b->max_nprim;
   hipStream_t st = (hipStream_t)stream;
   ensure_boys(st);
   char* w = (char*)workspace;
-  double* rec = (double*)w;
-  w += al((size_t)nsp * maxp * maxp * SREC * 8);
-  int* ao_first = (int*)w;
-  w += al((size_t)b->nshell * 4);
-  int* bra = (int*)w;
-  w += al((size_t)np * 4);
-  int* ket = (int*)w;
-  w += al((size_t)nsp * 4);
-  int* cnt = (int*)w;
-  w += al(64);
-  double* sq = (double*)w;
-  w += al((size_t)nsp * 8);
-  double* diag = (double*)w;
-  w += al((size_t)np * 8);
-  int* live = (int*)w;
-  w += al((size_t)nsp * np * 4);
-  int* live_cnt = (int*)w;
-  const Basis d = make_basis(b);
-  qexprof::Scope ps(qexprof::K_ERI, st);
-  const long nrec = (long)nsp * maxp * maxp;
-  hipLaunchKernelGGL(k_eri_records, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, st, d, nsp, maxp, rec);
-  hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt, 0);
-  EriArgs g{};
-  g.nao = b->nao;
-  g.npair = np;
-  g.nsp = nsp;
-  g.maxp = maxp;
-  g.sh_l = b->shell_l;
-  g.ao_shell = b->ao_shell;
-  g.ao_lmn = b->ao_lmn;
-  g.ao_first = ao_first;
-  g.rec = rec;
-  g.bra = bra;
-  g.ket = ket;
-  g.cnt = cnt;
-  g.sq = sq;
-  g.diag = diag;
-  g.live = live;
-  g.live_cnt = live_cnt;
-  g.eri = eri;
-  if (np >= ERI_SCREEN_MIN_PAIRS) {
-    hipLaunchKernelGGL(k_eri_diag, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS))), dim3(64), 0, st, g);
-    hipLaunchKernelGGL(k_eri_schwarz, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, nsp, b->shell_l, ao_first,
-                       diag, sq);
-  } else {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, sq, (size_t)nsp, 1.0);
+  EriArgs* dgs = (EriArgs*)w;
+  w += al((size_t)B * sizeof(EriArgs));
+  // lanes per shell quartet, per molecule (so a molecule's result does not depend on its batch): a
+  // molecule has about nsp^2 / 2 shell quartets, each up to max_nprim^4 primitive quartets; small
+  // ones split a quartet over a whole wave (latency), larger ones over 16 lanes (throughput).  The
+  // records are ordered wave-split molecules first, so each main-pass launch covers a contiguous run.
+  auto wide = [&](int i) {
+    const double nsp = 0.5 * bases[i].nshell * (bases[i].nshell + 1);
+    return 0.5 * nsp * nsp < 16384.0;
+  };
+  std::vector<int> order;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < B; ++i)
+      if (wide(i) == (pass == 0)) order.push_back(i);
+  std::vector<EriArgs> gs((size_t)B);
+  long nrec = 0;
+  int nket = 0, ndiag = 0, npmax = 0, nwide = 0, kwide = 0;
+  for (int o = 0; o < B; ++o) {
+    const int i = order[o];
+    const qex3d_basis* b = bases + i;
+    EriArgs& g = gs[o];
+    g = EriArgs{};
+    g.bs = make_basis(b);
+    g.nao = b->nao;
+    g.npair = b->nao * (b->nao + 1) / 2;
+    g.nsp = b->nshell * (b->nshell + 1) / 2;
+    g.maxp = b->max_nprim;
+    g.screen = g.npair >= ERI_SCREEN_MIN_PAIRS;
+    g.k_off = nket;
+    g.p_off = ndiag;
+    g.rec_off = nrec;
+    if (wide(i)) {
+      g.m_off = nket;
+      ++nwide;
+      kwide += g.nsp;
+    } else {
+      g.m_off = nket - kwide;
+    }
+    nrec += (long)g.nsp * g.maxp * g.maxp;
+    nket += g.nsp;
+    if (g.screen) ndiag += g.npair;
+    npmax = g.npair > npmax ? g.npair : npmax;
+    g.rec = (double*)w;
+    w += al((size_t)g.nsp * g.maxp * g.maxp * SREC * 8);
+    g.ao_first = (int*)w;
+    w += al((size_t)b->nshell * 4);
+    g.bra = (int*)w;
+    w += al((size_t)g.nsp * 4);
+    g.ket = (int*)w;
+    w += al((size_t)g.nsp * 4);
+    g.cnt = (int*)w;
+    w += al(64);
+    g.sq = (double*)w;
+    w += al((size_t)g.nsp * 8);
+    g.diag = (double*)w;
+    w += al((size_t)g.npair * 8);
+    g.live = (int*)w;
+    w += al((size_t)g.nsp * g.nsp * 4);
+    g.live_cnt = (int*)w;
+    w += al((size_t)g.nsp * 3 * 4);
+    g.eri = eri[i];
   }
-  hipLaunchKernelGGL(k_eri_setup, dim3(1), dim3(1024), 0, st, d, np, nsp, ao_first, bra, ket, cnt, 1);
-  hipLaunchKernelGGL(k_eri_live, dim3((unsigned)nsp), dim3(256), 0, st, g, live, live_cnt);
-  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, eri, (size_t)np * np, 0.0);
-  hipLaunchKernelGGL(k_eri, dim3((unsigned)((np + 64 / ERI_GS - 1) / (64 / ERI_GS)), (unsigned)nsp), dim3(64), 0, st, g);
-  return launch_status("qex3d_eri");
+  qexprof::Scope ps(qexprof::K_ERI, st);
+  for (int b0 = 0; b0 < B; b0 += ERI_PUT_N) {
+    EriPut pk{};
+    const int cnt = B - b0 < ERI_PUT_N ? B - b0 : ERI_PUT_N;
+    for (int i = 0; i < cnt; ++i) pk.r[i] = gs[b0 + i];
+    hipLaunchKernelGGL(k_put_eri, dim3(1), dim3(256), 0, st, pk, cnt, dgs + b0);
+  }
+  hipLaunchKernelGGL(k_eri_records, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, st, dgs, B, nrec);
+  hipLaunchKernelGGL(k_eri_setup, dim3((unsigned)B), dim3(1024), 0, st, dgs);
+  if (ndiag)
+    hipLaunchKernelGGL(k_eri_diag, dim3((unsigned)((ndiag + 64 / ERI_GS - 1) / (64 / ERI_GS))), dim3(64), 0, st, dgs, B,
+                       ndiag);
+  hipLaunchKernelGGL(k_eri_schwarz, dim3((unsigned)((nket + 255) / 256)), dim3(256), 0, st, dgs, B, nket);
+  hipLaunchKernelGGL(k_eri_live, dim3((unsigned)nket), dim3(256), 0, st, dgs, B);
+  const unsigned clear_x = (unsigned)((size_t)npmax * npmax / 256 / 8 + 1 < 1024 ? (size_t)npmax * npmax / 256 / 8 + 1
+                                                                                : 1024);
+  hipLaunchKernelGGL(k_eri_clear, dim3(clear_x, (unsigned)B), dim3(256), 0, st, dgs);
+  if (nwide) eri_main<64>(dgs, nwide, kwide, 64, st);
+  if (nwide < B) eri_main<16>(dgs + nwide, B - nwide, nket - kwide, 16, st);
+  return launch_status("qex3d_eri_batch");
+}
+
+int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspace_bytes, void* stream) {
+  return qex3d_eri_batch(1, b, &eri, workspace, workspace_bytes, stream);
 }
 
 int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const double* rad_w, int32_t nang,

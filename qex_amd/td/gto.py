@@ -129,6 +129,26 @@ class Mole:
         return g
 
 
+def intor_eri_pairs_batch(mols):
+    """The ERI pair matrices of several molecules in one launch sequence (``qex3d_eri_batch``), each
+    equal to its ``mol.intor_eri_pairs()`` bit for bit."""
+    mols = list(mols)
+    if not mols:
+        return []
+    dev = engine.device()
+    lib = _lib.load()
+    B = len(mols)
+    bases = (_lib.Qex3dBasis * B)(*[m.device_basis() for m in mols])
+    outs = [torch.empty((m.nao * (m.nao + 1) // 2,) * 2, dtype=torch.float64, device=dev) for m in mols]
+    ptrs = (ctypes.c_void_p * B)(*[o.data_ptr() for o in outs])
+    nbytes = lib.qex3d_eri_batch_workspace_bytes(B, bases)
+    if nbytes == 0:
+        raise ValueError("invalid basis in the ERI batch")
+    ws = engine.workspace(nbytes, dev)
+    _lib.check3(lib.qex3d_eri_batch(B, bases, ptrs, engine._ptr(ws), ws.numel(), engine._stream()), "qex3d_eri_batch")
+    return outs
+
+
 def synthetic_chain(n_atoms, seed):
     """Hydrogen chain along z with bonds U[0.5, 1.5] Angstrom and, per atom, 2 contracted s shells and
     1 p shell of 3 primitives (exponents log-uniform in [0.1, 20], random positive coefficients),

@@ -11,6 +11,7 @@ import torch
 
 from .. import _lib
 from ..od import engine
+from . import gto
 from .dft import Grids
 from .numint import eval_ao, eval_ao_f32, lda_functional
 
@@ -179,6 +180,11 @@ def kernel_batch_async(mfs, **kwargs):
         return mfs
     if len(mfs) > _lib.QEX3D_MAX_SCF_BATCH:
         raise ValueError(f"at most {_lib.QEX3D_MAX_SCF_BATCH} molecules per batch")
+    # the two-electron integrals of every molecule without them, in one launch sequence
+    todo = [mf for mf in mfs if mf._ints is None and hasattr(mf.mol, "device_basis")]
+    for mf, eri in zip(todo, gto.intor_eri_pairs_batch([mf.mol for mf in todo])):
+        S, T, V = mf.mol.intor_1e()
+        mf._ints = (S, (T + V).contiguous(), eri)
     preps = [mf._prepare(**kwargs) for mf in mfs]
     B = len(mfs)
     P = (_lib.Qex3dScfProblem * B)(*[x[0] for x in preps])

@@ -96,3 +96,32 @@ def test_batch_of_copies_and_a_second_smaller_batch(qtd):
     qtd.scf.kernel_batch(mfs[:2])
     for i, mf in enumerate(mfs[:2]):
         _assert_same(_snapshot(mf), r, f"second batch copy {i}")
+
+
+def test_eri_batch_equals_solo_bitwise(qtd):
+    """qex3d_eri_batch over molecules of every size class (wave-split small ones, 16-lane large ones,
+    Schwarz-screened H16), repeated and in mixed order: each pair matrix equals the molecule's own
+    qex3d_eri bit for bit; a second, smaller batch reusing the objects gives the same again."""
+    from qex_amd.td.gto import Mole
+    mols = [qtd.gto.synthetic_chain(16, 116), Mole([("H", (0, 0, 0)), ("H", (0, 0, 0.74))], basis="6-31g"),
+            qtd.gto.synthetic_chain(8, 108), qtd.gto.synthetic_chain(4, 104), qtd.gto.synthetic_chain(2, 102),
+            qtd.gto.synthetic_chain(16, 116), qtd.gto.synthetic_chain(6, 11)]
+    solo = [m.intor_eri_pairs() for m in mols]
+    for batch in (mols, mols[2:5]):
+        got = qtd.gto.intor_eri_pairs_batch(batch)
+        for m, g in zip(batch, got):
+            ref = solo[mols.index(m)]
+            assert g.shape == ref.shape
+            assert torch.equal(g, ref), f"nao {m.nao}: max diff {(g - ref).abs().max().item():.3e}"
+
+
+def test_eri_batch_argument_checks(qtd):
+    import ctypes
+    from qex_amd import _lib
+    lib = _lib.load()
+    assert lib.qex3d_eri_batch_workspace_bytes(0, None) == 0
+    m = qtd.gto.synthetic_chain(2, 1)
+    bases = (_lib.Qex3dBasis * 1)(m.device_basis())
+    assert lib.qex3d_eri_batch(0, bases, None, None, 0, None) == _lib.QEX_E_BADARG
+    ptrs = (ctypes.c_void_p * 1)(None)
+    assert lib.qex3d_eri_batch(1, bases, ptrs, None, 0, None) == _lib.QEX_E_BADARG

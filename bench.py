@@ -26,7 +26,11 @@ c2 LocalMLP-17 network, 15 iterations; loss mean (E - E*)^2 + 1e-2 mean dx sum (
 targets from the device LDA solve; gradients checked by central differences (h = 1e-4) along 10
 unit directions from the seed.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch 64] [--seed 1234] [--workload c5|c2|c3|c4]
+Workload eri (--workload eri): the two-electron integrals of the served stream's 3D share, one
+step = the ERI pair matrices of 76 chains (19 each of H2/H4/H8/H16, seeds 102..116) in one batched
+call (qex3d_eri_batch), each checked bit for bit against its solo call.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch 64] [--seed 1234] [--workload c5|c2|c3|c4|eri]
 """
 
 from __future__ import annotations
@@ -108,7 +112,7 @@ def parse_args():
                     help="no oracle / CPU baseline (for profiler runs, which must not start child processes)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="launch one 64 MiB device-to-device copy first (reference for HBM counters)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c5")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "eri"], default="c5")
     ap.add_argument("--slots", type=int, default=128, help="c5: 1D slots of the continuous-batching table")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="c4 and the c5 served path: 'mixed' = 3D AO values in FP32 and the Vxc contraction on FP32 "
@@ -136,6 +140,8 @@ def main():
         run_c4(args)
     elif args.workload == "c2":
         run_c2(args)
+    elif args.workload == "eri":
+        run_eri(args)
     else:
         run_c5(args)
 
@@ -464,6 +470,118 @@ def run_c4(args):
     print(json.dumps(result))
 
 
+ERI_FLOPS = os.path.join(ROOT, "profiles", "eri_fp64_ops.json")
+
+
+def _eri_fp64_ops():
+    """Executed FP64 operations of one qex3d_eri call per chain (by n_ao), from the committed rocprofv3
+    counter run (SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64 x 64 lanes, FMA counted twice); None if absent."""
+    if not os.path.exists(ERI_FLOPS):
+        return None
+    return {int(k): float(v) for k, v in json.load(open(ERI_FLOPS))["fp64_ops_per_call"].items()}
+
+
+def _eri_hbm_bytes():
+    """HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by a 64 MiB copy) of one qex3d_eri call per chain
+    (by n_ao), from the committed counter runs; None if absent."""
+    if not os.path.exists(ERI_FLOPS):
+        return None
+    d = json.load(open(ERI_FLOPS)).get("hbm_bytes_per_call")
+    return {int(k): float(v) for k, v in d.items()} if d else None
+
+
+def run_eri(args):
+    import torch
+
+    from oracle import td as otd
+    from qex_amd import _lib
+    from qex_amd.td import gto
+
+    lib = _lib.load()
+    if args.calibrate_copy:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
+    mols = [gto.synthetic_chain(n, 100 + n) for n in C4_CHAINS]
+    # one solo call per chain first, in chain order (the profile's per-molecule groups), kept as the
+    # bitwise reference of the batched results
+    solo = [m.intor_eri_pairs() for m in mols]
+    batch = mols * 19
+    for _ in range(max(args.warmup, 1)):
+        outs = gto.intor_eri_pairs_batch(batch)
+    torch.cuda.synchronize()
+    nk = len(_lib.KERNEL_KINDS)
+    ke = _lib.KERNEL_KINDS.index("eri")
+    _lib.check(lib.qex_profile_kinds(1 << ke), "qex_profile_kinds")
+    _lib.check(lib.qex_profile_enable(1, 4 * args.steps + 8), "qex_profile_enable")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs = gto.intor_eri_pairs_batch(batch)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int32 * nk)()
+    _lib.check(lib.qex_profile_read(ms, cnt, nk), "qex_profile_read")
+    lib.qex_profile_enable(0, 1)
+    lib.qex_profile_kinds(0xFFFFFFFF)
+    value = len(batch) * args.steps / elapsed
+    same = sum(bool(torch.equal(o, solo[i % len(mols)])) for i, o in enumerate(outs))
+    # parity of the small chains against the oracle (pair matrix of its full (ij|kl) array)
+    dmax = {}
+    for m, n in zip(mols[:2] if not args.skip_oracle else [], C4_CHAINS[:2]):
+        full = otd.eri(otd.synthetic_chain(n, 100 + n))
+        iu = [(i, j) for i in range(m.nao) for j in range(i + 1)]
+        idx = np.array(iu)
+        ref = full[idx[:, 0], idx[:, 1]][:, idx[:, 0], idx[:, 1]]
+        dmax[f"H{n}"] = float(np.abs(solo[C4_CHAINS.index(n)].cpu().numpy() - ref).max())
+    # bounded CPU sample: the oracle's ERIs of H2 and H4, one thread, ~15 s
+    from threadpoolctl import threadpool_limits
+    done, t = 0, time.perf_counter()
+    with threadpool_limits(1):
+        while not args.skip_oracle and time.perf_counter() - t < 12.0:
+            for n in C4_CHAINS[:2]:
+                otd.eri(otd.synthetic_chain(n, 100 + n))
+                done += 1
+    cpu_rate = done / (time.perf_counter() - t) if done else None
+    ops = _eri_fp64_ops()
+    hbm = _eri_hbm_bytes()
+    avg_ms = ms[ke] / cnt[ke] if cnt[ke] else None
+    achieved = None
+    if ops and avg_ms:
+        achieved = sum(ops[m.nao] for m in batch) / (avg_ms * 1e-3) / 1e12
+    result = {
+        "metric": "ERI pair matrices/s (served 3D chain mix)",
+        "value": value,
+        "unit": "molecules/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(elapsed / args.steps * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded hydrogen chains and basis exponents)",
+        "config": {"workload": "ERIs of 76 chains per step (19 each of H2/H4/H8/H16: n_ao 10/20/40/80, 3 primitives "
+                               "per shell) in one qex3d_eri_batch call",
+                   "nao": [m.nao for m in mols]},
+        "parity": {"bitwise_equal_to_solo": f"{same} of {len(outs)}", "max_abs_vs_oracle": dmax},
+        "roofline": {"bound": "fp64-valu", "kernel": "eri", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": (achieved / 78.6) if achieved is not None else None,
+                     "traffic": sum(hbm[m.nao] for m in batch) if hbm else None,
+                     "work": "executed FP64 operations per call from rocprofv3 SQ FP64 VALU counters (profiles/"
+                             "eri_fp64_ops.json), summed over the batch's molecules",
+                     "launches": int(cnt[ke]), "avg_launch_ms": avg_ms},
+        "cpu_baseline": None if cpu_rate is None else {
+            "value": cpu_rate, "unit": "molecules/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy oracle ERIs (McMurchie-Davidson, 8-fold symmetry) of H2 and H4 alternately for "
+                      f"~12 s, one thread: {done} molecules (H8/H16 left out: ~100 s / ~30 min each); {cpu_model()}"},
+    }
+    print(json.dumps(result))
+
+
 def _c3_oracle_sample(grids, systems, W, b, n0, targets_E, targets_n, budget_s=10.0):
     """Bounded CPU sample: the oracle's forward and reverse pass on the first systems, one thread."""
     from threadpoolctl import threadpool_limits
@@ -684,10 +802,15 @@ ROOF = {  # kind -> (bound, peak, unit)
 
 
 ROOF_FP32_GRID = {"vxc": ("mfma", 157.3, "TFLOP/s")}  # the fused FP32 grid kernel: dense FP32 MFMA peak
+if os.path.exists(ERI_FLOPS):  # the ERI: executed FP64 operations per molecule from the committed counter run
+    ROOF["eri"] = ("fp64-valu", 78.6, "TFLOP/s")
 
 
 def _stream_work(stats, kind, fused=False):
     tot = 0.0
+    if kind == "eri":  # one ERI computation per served 3D request
+        ops = _eri_fp64_ops()
+        return sum(ops[nao] for nao, _, _, _ in stats.work_3d)
     if kind in ("eigen", "mlp", "potential"):
         for b_ne, T in stats.work_1d:
             tot += T * _work_1d(kind, b_ne)
@@ -837,7 +960,11 @@ def run_c5(args):
     achieved = work / (dom_ms * 1e-3) / scale if dom_ms > 0 else None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "c5_traffic.json")
-    if os.path.exists(tpath):
+    if dom == "eri":  # bytes of one ERI computation per served 3D request, per timed call
+        hbm = _eri_hbm_bytes()
+        if hbm and dom_n:
+            traffic = sum(hbm[nao] for nao, _, _, _ in st.work_3d) / dom_n
+    elif os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch", {}).get(dom)
     brk_all = {k: {"ms_total": round(v[0], 3), "launches": int(v[1])} for k, v in brk.items()}
     result = {
@@ -878,7 +1005,9 @@ def run_c5(args):
         "kernels_breakdown_one_step": brk_all,
         "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": (achieved / peak) if achieved is not None else None, "traffic": traffic,
-                     "launches": int(dom_n), "avg_launch_ms": (dom_ms / dom_n) if dom_n else None},
+                     "launches": int(dom_n), "avg_launch_ms": (dom_ms / dom_n) if dom_n else None,
+                     "work": ("executed FP64 operations per molecule from rocprofv3 SQ FP64 VALU counters "
+                              "(profiles/eri_fp64_ops.json)") if dom == "eri" else "algorithmic"},
         "cpu_baseline": None if cpu is None else {
             "value": cpu[0], "unit": "solves/s", "cores": procs, "kind": "port",
             "sample": f"FP64 NumPy/SciPy oracle draining the first 64 requests of the timed stream "

@@ -373,22 +373,23 @@ __device__ __forceinline__ void boys_upto(double T, double* F) {
     const double dT = T - BOYS_H * i;
     const double* r = g_boys_tab + i * BOYS_NM + L;
     double f = r[6];
-    f = fma(f, -dT / 6.0, r[5]);
-    f = fma(f, -dT / 5.0, r[4]);
-    f = fma(f, -dT / 4.0, r[3]);
-    f = fma(f, -dT / 3.0, r[2]);
-    f = fma(f, -dT / 2.0, r[1]);
+    f = fma(f, dT * (-1.0 / 6.0), r[5]);
+    f = fma(f, dT * (-1.0 / 5.0), r[4]);
+    f = fma(f, dT * (-1.0 / 4.0), r[3]);
+    f = fma(f, dT * (-1.0 / 3.0), r[2]);
+    f = fma(f, dT * -0.5, r[1]);
     f = fma(f, -dT, r[0]);
     F[L] = f;
     if (L > 0) {
-      const double et = exp(-T);
+      const double et = exp(-T), t2 = 2.0 * T;
 #pragma unroll
-      for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + et) / (2 * m + 1);
+      for (int m = L - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], et) * (1.0 / (2 * m + 1));
     }
   } else {
-    F[0] = 0.5 * sqrt(M_PI / T);
+    const double rt = rsqrt(T);
+    F[0] = 0.88622692545275801 * rt;  // sqrt(pi) / 2 / sqrt(T)
     if (L > 0) {
-      const double et = exp(-T), i2t = 0.5 / T;
+      const double et = exp(-T), i2t = 0.5 * rt * rt;
 #pragma unroll
       for (int m = 1; m <= L; ++m) F[m] = ((2 * m - 1) * F[m - 1] - et) * i2t;
     }
@@ -483,12 +484,13 @@ __device__ void eri_diag_body(const EriArgs& g, int P, int i, int j, int m) {
     const double c1 = x1[4], c2 = x2[4];
     if (c1 == 0.0 || c2 == 0.0) continue;
     const double p = x1[0], q = x2[0];
-    const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
+    const double rs = rsqrt(p + q);
+    const double pref = x1[12] * x2[12] * (TWO_PI_2_5 * rs);
     if (fabs(pref) < (CB == 0 ? ERI_PRIM_TOL : 1e-3 * ERI_PRIM_TOL)) continue;
     double E1[NB], E2[NB];
     e_dense<CB>(la, lb, x1 + 5, x1 + 8, x1[11], E1);
     e_dense<CB>(la, lb, x2 + 5, x2 + 8, x2[11], E2);
-    const double alpha = p * q / (p + q);
+    const double alpha = p * q * (rs * rs);
     const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
     double F[2 * CB + 1];
     boys_upto<2 * CB>(alpha * (X * X + Y * Y + Z * Z), F);
@@ -579,9 +581,10 @@ __device__ void eri_shell_body(const EriArgs& g, int SP, int sa, int sb, int SQ,
     const double p = x1[0], q = x2[0];
     // the quartet's (ss|ss)-type magnitude bound; p functions add Hermite factors (distances,
     // 1/2p), so their quartets get a threshold three orders lower
-    const double pref = c1 * c2 * TWO_PI_2_5 / (p * q * sqrt(p + q));
+    const double rs = rsqrt(p + q);
+    const double pref = x1[12] * x2[12] * (TWO_PI_2_5 * rs);
     if (fabs(pref) < (L == 0 ? ERI_PRIM_TOL : 1e-3 * ERI_PRIM_TOL)) continue;
-    const double alpha = p * q / (p + q);
+    const double alpha = p * q * (rs * rs);
     const double X = x1[1] - x2[1], Y = x1[2] - x2[2], Z = x1[3] - x2[3];
     double F[L + 1];
     boys_upto<L>(alpha * (X * X + Y * Y + Z * Z), F);
@@ -771,7 +774,7 @@ __global__ void k_eri_schwarz(const EriArgs* __restrict__ gs, int B, int total) 
   g.sq[SP] = m;
 }
 
-// Shell-pair primitive records: p, P (3), c_a c_b exp(-mu |AB|^2), P - A (3), P - B (3), 1/(2p);
+// Shell-pair primitive records: p, P (3), c = c_a c_b exp(-mu |AB|^2), P - A (3), P - B (3), 1/(2p), c/p;
 // record r = pa * nb + pb of the shell pair's maxp^2 slots for r < na * nb, the rest unused (c = 0).
 // Primitive pairs whose Gaussian product prefactor is below 1e-20 get c = 0 (they contribute
 // < 1e-18 to any integral) and are skipped.
@@ -808,6 +811,7 @@ __global__ void k_eri_records(const EriArgs* __restrict__ gs, int B, long total)
   }
   o[11] = 0.5 / p;
   o[4] = fabs(cab) * K * (M_PI / p) * sqrt(M_PI / p) < 1e-20 ? 0.0 : cab * K;
+  o[12] = o[4] / p;
 }
 
 // One workgroup of 1024 per molecule: ao_first (first AO of each shell), then the work lists of

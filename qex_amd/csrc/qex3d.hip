@@ -329,6 +329,7 @@ __global__ __launch_bounds__(256) void k_int1e(Basis b, double* S, double* Tm, d
 // bras in two: ket classes ss/sp, and pp|pp in three passes over the ket's first component).
 // ---------------------------------------------------------------------------------------------
 constexpr int SREC = 16;  // doubles per (shell pair, primitive pair) record
+constexpr int EREC = 13;  // the leading doubles of a ket record, staged in LDS by the main pass
 constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
 
 // Hermite index k <-> (t, u, v), ordered by N = t+u+v, then by u+v, then by v
@@ -554,10 +555,11 @@ __device__ __forceinline__ double hsum(bool first, const double* PA, const doubl
   }
 }
 
-// The shell quartet (SP = (sa, sb) | SQ = (sc, sd)) of classes (CB, CK) for one group of ERI_GS
-// lanes; KCC >= 0 (pp|pp): only the ket components whose first index is KCC.
+// The shell quartet (SP = (sa, sb) | (sc, sd)) of classes (CB, CK) for one group of GS lanes, member
+// m; r2: the ket shell pair's primitive-pair records (EREC doubles each); KCC >= 0 (pp|pp): only the
+// ket components whose first index is KCC.
 template <int CB, int CK, int GS, int KCC = -1>
-__device__ void eri_shell_body(const EriArgs& g, int SP, int sa, int sb, int SQ, int sc, int sd, int m) {
+__device__ void eri_shell_body(const EriArgs& g, int SP, int sa, int sb, const double* r2, int sc, int sd, int m) {
   constexpr int L = CB + CK, NR = h_count(L), NK = h_count(CK);
   constexpr int NBC = CB == 0 ? 1 : CB == 1 ? 3 : 9;
   constexpr int NKC = CK == 0 ? 1 : CK == 1 ? 3 : (KCC >= 0 ? 3 : 9);
@@ -565,7 +567,6 @@ __device__ void eri_shell_body(const EriArgs& g, int SP, int sa, int sb, int SQ,
   const bool pa = g.bs.sh_l[sa] == 1, pc = g.bs.sh_l[sc] == 1;  // class 1: the p function is on the first shell
   const int pp = g.maxp * g.maxp;
   const double* r1 = g.rec + (size_t)SP * pp * SREC;
-  const double* r2 = g.rec + (size_t)SQ * pp * SREC;
   const int npc = g.bs.sh_nprim[sc] * g.bs.sh_nprim[sd], nq = g.bs.sh_nprim[sa] * g.bs.sh_nprim[sb] * npc;
   double acc[NBC][NKC];
 #pragma unroll
@@ -575,7 +576,7 @@ __device__ void eri_shell_body(const EriArgs& g, int SP, int sa, int sb, int SQ,
   for (int qi = m; qi < nq; qi += GS) {
     const int a = qi / npc, c = qi - a * npc;
     const double* x1 = r1 + a * SREC;
-    const double* x2 = r2 + c * SREC;
+    const double* x2 = r2 + c * EREC;
     const double c1 = x1[4], c2 = x2[4];
     if (c1 == 0.0 || c2 == 0.0) continue;
     const double p = x1[0], q = x2[0];
@@ -657,6 +658,16 @@ __global__ __launch_bounds__(64) void k_eri_cls(const EriArgs* __restrict__ gs, 
   const int nlive = lc3[CB];
   const int* list = g.live + (size_t)ky * g.nsp + (CB == 0 ? 0 : CB == 1 ? lc3[0] : lc3[0] + lc3[1]);
   const int m = threadIdx.x % GS;
+  if (nlive == 0) return;
+  // the ket shell pair's primitive-pair records, shared by the whole wave (LDS instead of a global
+  // load per primitive quartet)
+  __shared__ double kr[QEX3D_MAX_NPRIM * QEX3D_MAX_NPRIM * EREC];
+  {
+    const int npc = g.bs.sh_nprim[sc] * g.bs.sh_nprim[sd];
+    const double* src = g.rec + (size_t)SQ * g.maxp * g.maxp * SREC;
+    for (int t = threadIdx.x; t < npc * EREC; t += 64) kr[t] = src[(t / EREC) * SREC + t % EREC];
+    __syncthreads();
+  }
   for (int b0 = blockIdx.x * (64 / GS); b0 < nlive; b0 += gridDim.x * (64 / GS)) {
     const int bx = b0 + threadIdx.x / GS;
     if (bx >= nlive) break;
@@ -664,15 +675,15 @@ __global__ __launch_bounds__(64) void k_eri_cls(const EriArgs* __restrict__ gs, 
     int sa, sb;
     tri_split(SP, &sa, &sb);
     if (ck == 0) {
-      eri_shell_body<CB, 0, GS>(g, SP, sa, sb, SQ, sc, sd, m);
+      eri_shell_body<CB, 0, GS>(g, SP, sa, sb, kr, sc, sd, m);
     } else if (ck == 1) {
-      eri_shell_body<CB, 1, GS>(g, SP, sa, sb, SQ, sc, sd, m);
+      eri_shell_body<CB, 1, GS>(g, SP, sa, sb, kr, sc, sd, m);
     } else if constexpr (CB == 2) {
-      eri_shell_body<2, 2, GS, 0>(g, SP, sa, sb, SQ, sc, sd, m);
-      eri_shell_body<2, 2, GS, 1>(g, SP, sa, sb, SQ, sc, sd, m);
-      eri_shell_body<2, 2, GS, 2>(g, SP, sa, sb, SQ, sc, sd, m);
+      eri_shell_body<2, 2, GS, 0>(g, SP, sa, sb, kr, sc, sd, m);
+      eri_shell_body<2, 2, GS, 1>(g, SP, sa, sb, kr, sc, sd, m);
+      eri_shell_body<2, 2, GS, 2>(g, SP, sa, sb, kr, sc, sd, m);
     } else {
-      eri_shell_body<CB, 2, GS>(g, SP, sa, sb, SQ, sc, sd, m);
+      eri_shell_body<CB, 2, GS>(g, SP, sa, sb, kr, sc, sd, m);
     }
   }
 }

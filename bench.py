@@ -582,6 +582,17 @@ def run_eri(args):
     print(json.dumps(result))
 
 
+# c3 kinds: the MLP adjoint and the eigen adjoint run on the FP64 vector units (no matrix cores); the
+# forward MLP on FP64 matrix cores; the forward eigensolver is latency-bound
+C3_BOUND = {"adj_mlp": "fp64-valu", "adj_eig": "fp64-latency", "mlp": "mfma", "eigen": "fp64-latency"}
+
+
+def _c3_traffic():
+    """HBM bytes per launch by kind from the committed c3 counter runs (profiles/c3_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "c3_traffic.json")
+    return json.load(open(path)).get("bytes_per_launch", {}) if os.path.exists(path) else {}
+
+
 def _c3_oracle_sample(grids, systems, W, b, n0, targets_E, targets_n, budget_s=10.0):
     """Bounded CPU sample: the oracle's forward and reverse pass on the first systems, one thread."""
     from threadpoolctl import threadpool_limits
@@ -709,8 +720,9 @@ def run_c3(args):
         "kernels_ms_total": {k_: round(v[0], 3) for k_, v in kern.items()},
         "kernels_launches": {k_: int(v[1]) for k_, v in kern.items()},
         "kernels_ms_avg": {k_: round(v_, 4) for k_, v_ in avg.items()},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": (achieved / 78.6) if achieved else None, "traffic": None},
+        "roofline": {"bound": C3_BOUND.get(dom, "fp64-valu"), "kernel": dom, "achieved": achieved, "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": (achieved / 78.6) if achieved else None,
+                     "traffic": _c3_traffic().get(dom)},
         "cpu_baseline": {"value": cpu_rate, "unit": "systems/s", "cores": 1, "kind": "port",
                          "sample": f"FP64 NumPy oracle forward + hand-written reverse pass, 1 thread, first {cpu_done} "
                                    f"systems (~10 s); {cpu_model()}"},

@@ -14,7 +14,8 @@
 //   Hartree + XC     nbar += dx U vbar + H_F vbar, thetabar += d(vbar . grad F)/dtheta, F = sum e(n) n
 // The MLP terms come from reverse mode over the network's forward and forward-tangent passes, one
 // wavefront per point with one lane per hidden unit; each wavefront accumulates its parameter
-// partials in registers and adds them to its own slot, and the slots are summed in a fixed order.
+// partials in registers, the waves of a workgroup combine them in LDS in a fixed order and add the
+// sum to the workgroup's own slot, and the slots are summed in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -28,7 +29,8 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int ADJ_SLOTS = 1024;  // wavefronts of the MLP adjoint launch = parameter-partial slots
+constexpr int ADJ_SLOTS = 1024;  // wavefronts of the MLP adjoint launch
+constexpr int ADJ_WG = ADJ_SLOTS / (NT / 64);  // its workgroups = parameter-partial slots
 constexpr int HMAX = 64;        // hidden width (one lane per unit)
 constexpr int WMAXW = QEX1D_MAX_MLP_WINDOW;
 thread_local char g_erra[512] = "";
@@ -147,7 +149,7 @@ struct Adj {
   double* xbar;  // [B][N][W]
   double* fv;    // [B][N]
   double* fdv;   // [B][N]
-  double* thp;   // [ADJ_SLOTS][P]
+  double* thp;   // [ADJ_WG][P]
 };
 
 __device__ __forceinline__ const double* input_density(const Adj& a, int s, int k) {
@@ -396,22 +398,34 @@ __global__ __launch_bounds__(NT, 1) void k_adj_mlp(Adj a, const double* xin, con
     }
     __builtin_amdgcn_wave_barrier();
   }
-  // add this wavefront's partials to its slot (slots are owned: no races)
-  double* g = a.thp + (size_t)slot * a.P;
-  if (lane < H1) {
+  // the workgroup's waves add their partials into LDS one after another (the weights are no longer
+  // needed, and their area holds the P parameters), then the sum goes to the workgroup's slot (slots
+  // are owned: no races; a fixed order)
+  __syncthreads();
+  double* r = sm;
+  for (int t = threadIdx.x; t < a.P; t += NT) r[t] = 0.0;
+  __syncthreads();
+  for (int w = 0; w < NT / 64; ++w) {
+    if (wv == w) {
+      if (lane < H1) {
 #pragma unroll
-    for (int o = 0; o < WMAXW; ++o)
-      if (o < W) g[a.w1 + o * H1 + lane] += gW1[o];
-    g[a.b1 + lane] += gb1;
-  }
-  if (NL == 2 && lane < H2) {
+        for (int o = 0; o < WMAXW; ++o)
+          if (o < W) r[a.w1 + o * H1 + lane] += gW1[o];
+        r[a.b1 + lane] += gb1;
+      }
+      if (NL == 2 && lane < H2) {
 #pragma unroll
-    for (int v = 0; v < HMAX; ++v)
-      if (v < H1) g[a.w2 + v * H2 + lane] += gW2[v];
-    g[a.b2 + lane] += gb2;
+        for (int v = 0; v < HMAX; ++v)
+          if (v < H1) r[a.w2 + v * H2 + lane] += gW2[v];
+        r[a.b2 + lane] += gb2;
+      }
+      if (lane < HL) r[a.w3 + lane] += gW3;
+      if (lane == 0) r[a.b3] += gb3;
+    }
+    __syncthreads();
   }
-  if (lane < HL) g[a.w3 + lane] += gW3;
-  if (lane == 0) g[a.b3] += gb3;
+  double* g = a.thp + (size_t)blockIdx.x * a.P;
+  for (int t = threadIdx.x; t < a.P; t += NT) g[t] += r[t];
 }
 
 // dst[s][i] += sa[s] * (A1[s][i] + F[s][i]) + A2[s][i] + window^T(xbar)_i / norm (A1, A2 optional)
@@ -628,7 +642,7 @@ __global__ __launch_bounds__(256) void k_adj_reduce(int P, const double* thp, do
   const int t = threadIdx.x, q = blockIdx.x * 32 + (t & 31), part = t >> 5;
   double g = 0.0;
   if (q < P)
-    for (int sl = part; sl < ADJ_SLOTS; sl += 8) g += thp[(size_t)sl * P + q];
+    for (int sl = part; sl < ADJ_WG; sl += 8) g += thp[(size_t)sl * P + q];
   red[part][t & 31] = g;
   __syncthreads();
   if (t < 32 && q < P) {
@@ -695,7 +709,7 @@ LayA layout_adj(const qex1d_problem* p) {
   L.xbar = take(B * N * W * 8);
   L.fv = take(B * N * 8);
   L.fdv = take(B * N * 8);
-  L.thp = take((size_t)ADJ_SLOTS * param_count(p) * 8);
+  L.thp = take((size_t)ADJ_WG * param_count(p) * 8);
   L.cs = take(B * 8);
   L.ones = take(B * 8);
   L.total = off;
@@ -802,9 +816,9 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
   auto mlp = [&](const double* xin, const double* tin, const double* F1, const double* sf, const double* F2) {
     qexprof::Scope ps(qexprof::K_ADJ_MLP, st);
     if (a.NL == 2)
-      hipLaunchKernelGGL(k_adj_mlp<2>, dim3(ADJ_SLOTS / (NT / 64)), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
+      hipLaunchKernelGGL(k_adj_mlp<2>, dim3(ADJ_WG), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
     else
-      hipLaunchKernelGGL(k_adj_mlp<1>, dim3(ADJ_SLOTS / (NT / 64)), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
+      hipLaunchKernelGGL(k_adj_mlp<1>, dim3(ADJ_WG), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
   };
   const unsigned gBN = (unsigned)((BN + NT - 1) / NT);
   // alpha_k as the forward computes it
@@ -818,7 +832,7 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
   {
     hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.nbar, BN);
     hipLaunchKernelGGL(k_zero_a, dim3(256), dim3(NT), 0, st, a.dbar, BN * a.T);
-    hipLaunchKernelGGL(k_zero_a, dim3(1024), dim3(NT), 0, st, a.thp, (long)ADJ_SLOTS * a.P);
+    hipLaunchKernelGGL(k_zero_a, dim3(1024), dim3(NT), 0, st, a.thp, (long)ADJ_WG * a.P);
     // with one mixing difference nothing reaches the stopped iterations; with more,
     // the differences of those iterations still enter later mixes and are reversed
     const int k_end = a.M == 1 ? (a.sgs >= 0 ? a.sgs + 1 : 0) : 0;

@@ -32,7 +32,7 @@ extern "C" {
 #define QEX1D_MAX_GRIDS 1024
 #define QEX1D_MAX_ELECTRONS 62
 #define QEX1D_MAX_MLP_LAYERS 6
-#define QEX1D_MAX_MLP_WIDTH 128
+#define QEX1D_MAX_MLP_WIDTH 1024 /* hidden widths; also bounded by LDS (the problem is refused if it does not fit) */
 #define QEX1D_MAX_MLP_WINDOW 33
 
 /* SCF loop semantics.

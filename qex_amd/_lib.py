@@ -19,6 +19,7 @@ QEX1D_NOALLOC_SCRATCH_BYTES = 16 << 20
 QEX1D_MAX_GRIDS = 1024
 QEX1D_MAX_ELECTRONS = 62
 QEX1D_MAX_MLP_LAYERS = 6
+QEX1D_MAX_MLP_WIDTH = 1024
 
 QEX1D_LOOP_PYTHON = 0
 QEX1D_LOOP_SCAN = 1

@@ -56,7 +56,6 @@ using qexband::ldlt_side_backward;
 constexpr int NT = 256;     // threads per workgroup (4 wave64)
 constexpr int NWAVE = NT / 64;
 constexpr int TILE = 64;    // output points per k_potential workgroup
-constexpr int MLP_P = 32;   // points per k_mlp workgroup (8 lanes per point)
 constexpr double EPS = 2.220446049250313e-16;
 
 thread_local char g_err[512] = "";
@@ -219,7 +218,8 @@ struct Dev1 {
   int widths[QEX1D_MAX_MLP_LAYERS + 1];
   int woff[QEX1D_MAX_MLP_LAYERS];  // offset of W_l in the packed parameter array
   int boff[QEX1D_MAX_MLP_LAYERS];  // offset of b_l
-  int wmax;                        // widest hidden layer
+  int wmax;                        // widest layer input (window or hidden)
+  int hmax;                        // widest hidden layer
   double dx, tol, xc_c, xc_p, xc_eps, norm;
   double Td, Tb, Tc;  // kinetic stencil: diagonal, first and second off-diagonal
   // inputs
@@ -330,19 +330,22 @@ __global__ void k_init(Dev1 a) {
 }
 
 // --------------------------------------------------------------------------------------------
-// k_mlp: windowed per-point MLP.  Workgroup = MLP_P points x 8 lanes; lane q of a point owns
-// units q, q+8, ... of every layer.  Pass 1 (input density, unless frozen): e and the
-// input-VJP G[i][o] = m_i de_i/dx_i[o].  Pass 2 (pending output density): e only.
+// k_mlp: windowed per-point MLP.  Workgroup = NT / LPP points x LPP lanes (LPP = 8, or 64 for
+// networks too wide for 32 points' activations in LDS); lane q of a point owns units q, q+LPP, ...
+// of every layer.  Pass 1 (input density, unless frozen): e and the input-VJP
+// G[i][o] = m_i de_i/dx_i[o].  Pass 2 (pending output density): e only.
 // --------------------------------------------------------------------------------------------
+template <int LPP>
 __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_lds, int lo, int hi, int i, int q,
                                int pl, double* actA, double* actB, double* dact, bool backward,
                                double* e_out, double* G_out) {
   // dens_lds holds the (already reflected when needed) density on the whole grid
+  constexpr int P = NT / LPP;
   const int N = a.N, W = a.W, h = W / 2, stride = a.wmax + 1;
   const bool valid = i < N;
   double* in = actA + pl * stride;
   // window input / norm
-  for (int o = q; o < W; o += 8) {
+  for (int o = q; o < W; o += LPP) {
     const int j = i + o - h;
     in[o] = (valid && j >= 0 && j < N) ? dens_lds[j] / a.norm : 0.0;
   }
@@ -356,13 +359,13 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
     const double* Wl = a.mlp + a.woff[l];
     const double* bl = a.mlp + a.boff[l];
     const double* x = cur + pl * stride;
-    for (int u = q; u < fo; u += 8) {
+    for (int u = q; u < fo; u += LPP) {
       double z = bl[u];
       for (int kk = 0; kk < fi; ++kk) z = fma(x[kk], Wl[kk * fo + u], z);
       double dz;
       const double y = act_fwd(a.act, z, &dz);
       nxt[pl * stride + u] = y;
-      if (backward) dact[((size_t)l * MLP_P + pl) * stride + u] = dz;
+      if (backward) dact[((size_t)l * P + pl) * stride + u] = dz;
     }
     __syncthreads();
     double* t = cur;
@@ -373,11 +376,10 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
   const int fi = a.widths[L - 1];
   const double* WL = a.mlp + a.woff[L - 1];
   double part = 0.0;
-  for (int kk = q; kk < fi; kk += 8) part = fma(cur[pl * stride + kk], WL[kk], part);
-  // 8-lane reduction (lanes of a point are contiguous: pl*8 + q)
-  part += __shfl_xor(part, 1, 64);
-  part += __shfl_xor(part, 2, 64);
-  part += __shfl_xor(part, 4, 64);
+  for (int kk = q; kk < fi; kk += LPP) part = fma(cur[pl * stride + kk], WL[kk], part);
+  // LPP-lane reduction (lanes of a point are contiguous: pl*LPP + q)
+#pragma unroll
+  for (int o = 1; o < LPP; o <<= 1) part += __shfl_xor(part, o, 64);
   double de;
   const double e = out_transform(a.mlpout, part + a.mlp[a.boff[L - 1]], &de);
   if (q == 0 && valid) *e_out = e;
@@ -389,8 +391,8 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
   const double m = valid ? dens_lds[i] * de : 0.0;
   // gradient w.r.t. last hidden activations
   double* g = nxt;  // free buffer
-  for (int kk = q; kk < fi; kk += 8)
-    g[pl * stride + kk] = m * WL[kk] * dact[((size_t)(L - 2) * MLP_P + pl) * stride + kk];
+  for (int kk = q; kk < fi; kk += LPP)
+    g[pl * stride + kk] = m * WL[kk] * dact[((size_t)(L - 2) * P + pl) * stride + kk];
   __syncthreads();
   double* gcur = g;
   double* gnxt = cur;
@@ -399,10 +401,10 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
     const int fi2 = a.widths[l], fo2 = a.widths[l + 1];
     const double* Wl = a.mlp + a.woff[l];
     const double* gz = gcur + pl * stride;
-    for (int kk = q; kk < fi2; kk += 8) {
+    for (int kk = q; kk < fi2; kk += LPP) {
       double acc = 0.0;
       for (int u = 0; u < fo2; ++u) acc = fma(gz[u], Wl[kk * fo2 + u], acc);
-      gnxt[pl * stride + kk] = acc * dact[((size_t)(l - 1) * MLP_P + pl) * stride + kk];
+      gnxt[pl * stride + kk] = acc * dact[((size_t)(l - 1) * P + pl) * stride + kk];
     }
     __syncthreads();
     double* t = gcur;
@@ -414,7 +416,7 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
     const int fo1 = a.widths[1];
     const double* W0 = a.mlp + a.woff[0];
     const double* gz = gcur + pl * stride;
-    for (int o = q; o < W; o += 8) {
+    for (int o = q; o < W; o += LPP) {
       double acc = 0.0;
       for (int u = 0; u < fo1; ++u) acc = fma(gz[u], W0[o * fo1 + u], acc);
       if (valid) G_out[o] = acc;
@@ -423,7 +425,9 @@ __device__ void mlp_point_pass(const Dev1& a, const double* __restrict__ dens_ld
   __syncthreads();
 }
 
+template <int LPP>
 __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k_launch) {
+  constexpr int P = NT / LPP;
   extern __shared__ double smem[];
   const int s = blockIdx.y;
   const int k = iter_of(a, s, k_launch);
@@ -434,11 +438,11 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k_launch) {
   if (!doPot && !doE) return;
   const int N = a.N, stride = a.wmax + 1;
   double* dens = smem;                      // N
-  double* actA = dens + N;                  // MLP_P * stride
-  double* actB = actA + MLP_P * stride;     // MLP_P * stride
-  double* dact = actB + MLP_P * stride;     // (L-1) * MLP_P * stride
-  const int pl = threadIdx.x >> 3, q = threadIdx.x & 7;
-  const int i = blockIdx.x * MLP_P + pl;
+  double* actA = dens + N;              // P * stride
+  double* actB = actA + P * stride;     // P * stride
+  double* dact = actB + P * stride;     // (L-1) * P * stride
+  const int pl = threadIdx.x / LPP, q = threadIdx.x % LPP;
+  const int i = blockIdx.x * P + pl;
   int lo = 0, hi = N - 1;
   const bool sym = a.reflect && xc_is_local(a.xc);
   if (a.reflect) {
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k_launch) {
     }
     __syncthreads();
     double eo;
-    mlp_point_pass(a, dens, lo, hi, i, q, pl, actA, actB, dact, true, &eo,
+    mlp_point_pass<LPP>(a, dens, lo, hi, i, q, pl, actA, actB, dact, true, &eo,
                    a.mlp_G + ((size_t)s * N + (i < N ? i : 0)) * a.W);
     if (q == 0 && i < N) a.mlp_e[(size_t)s * N + i] = eo;
     __syncthreads();
@@ -464,7 +468,7 @@ __global__ __launch_bounds__(NT) void k_mlp(Dev1 a, int k_launch) {
     for (int t = threadIdx.x; t < N; t += NT) dens[t] = ox[t];
     __syncthreads();
     double eo;
-    mlp_point_pass(a, dens, lo, hi, i, q, pl, actA, actB, dact, false, &eo, nullptr);
+    mlp_point_pass<LPP>(a, dens, lo, hi, i, q, pl, actA, actB, dact, false, &eo, nullptr);
     if (q == 0 && i < N) a.mlp_eout[(size_t)s * N + i] = eo;
   }
 }
@@ -726,41 +730,53 @@ __device__ double mlp_v_unsym(const Dev1& a, int s, int i) {
 
 // Global MLP on the whole density, one workgroup per system: f(n_in) with the input gradient
 // (potential dx df/dn into mlp_e) and f at the pending output density; scalars in
-// mlp_eout[2 s] (f(n_in)) and mlp_eout[2 s + 1] (f(n_out)).
+// mlp_eout[2 s] (f(n_in)) and mlp_eout[2 s + 1] (f(n_out)).  Layer activations and derivatives
+// in LDS, hmax doubles per layer (HB).
 __device__ double gmlp_forward(const Dev1& a, const double* x, double* acts, double* dzs, double* part, double* red) {
-  const int N = a.N, tid = threadIdx.x;
+  const int N = a.N, tid = threadIdx.x, HB = a.hmax;
   const int H1 = a.widths[1];
-  // layer 1: z[u] = b[u] + sum_i x_i W[i][u], split over NT / H1 interleaved row groups
-  const int G = NT / H1, u = tid % H1, g = tid / H1;
-  double acc = 0.0;
-  if (g < G)
-    for (int i = g; i < N; i += G) acc = fma(x[i], a.mlp[a.woff[0] + (size_t)i * H1 + u], acc);
-  part[tid] = acc;
-  __syncthreads();
-  if (tid < H1) {
-    double z = a.mlp[a.boff[0] + tid];
-    for (int q = 0; q < G; ++q) z += part[q * H1 + tid];
-    double dz;
-    acts[tid] = act_fwd(a.act, z, &dz);
-    dzs[tid] = dz;
+  if (H1 <= NT) {
+    // layer 1: z[u] = b[u] + sum_i x_i W[i][u], split over NT / H1 interleaved row groups
+    const int G = NT / H1, u = tid % H1, g = tid / H1;
+    double acc = 0.0;
+    if (g < G)
+      for (int i = g; i < N; i += G) acc = fma(x[i], a.mlp[a.woff[0] + (size_t)i * H1 + u], acc);
+    part[tid] = acc;
+    __syncthreads();
+    if (tid < H1) {
+      double z = a.mlp[a.boff[0] + tid];
+      for (int q = 0; q < G; ++q) z += part[q * H1 + tid];
+      double dz;
+      acts[tid] = act_fwd(a.act, z, &dz);
+      dzs[tid] = dz;
+    }
+  } else {
+    // wide first layer: every thread takes units tid, tid + NT, ... over the whole input
+    for (int u = tid; u < H1; u += NT) {
+      double z = a.mlp[a.boff[0] + u];
+      for (int i = 0; i < N; ++i) z = fma(x[i], a.mlp[a.woff[0] + (size_t)i * H1 + u], z);
+      double dz;
+      acts[u] = act_fwd(a.act, z, &dz);
+      dzs[u] = dz;
+    }
   }
   __syncthreads();
-  // hidden layers 2 .. L-1 (activations of layer l at acts + (l-1) * 128)
+  // hidden layers 2 .. L-1 (activations of layer l at acts + (l-1) * HB)
   for (int l = 1; l < a.nlay - 1; ++l) {
     const int fi = a.widths[l], fo = a.widths[l + 1];
-    const double* prev = acts + (l - 1) * 128;
-    if (tid < fo) {
-      double z = a.mlp[a.boff[l] + tid];
-      for (int v = 0; v < fi; ++v) z = fma(prev[v], a.mlp[a.woff[l] + v * fo + tid], z);
+    const double* prev = acts + (l - 1) * HB;
+    for (int u = tid; u < fo; u += NT) {
+      double z = a.mlp[a.boff[l] + u];
+      for (int v = 0; v < fi; ++v) z = fma(prev[v], a.mlp[a.woff[l] + v * fo + u], z);
       double dz;
-      acts[l * 128 + tid] = act_fwd(a.act, z, &dz);
-      dzs[l * 128 + tid] = dz;
+      acts[l * HB + u] = act_fwd(a.act, z, &dz);
+      dzs[l * HB + u] = dz;
     }
     __syncthreads();
   }
   const int L = a.nlay - 1, fl = a.widths[L];
   double o = 0.0;
-  for (int v = tid; v < fl; v += NT) o = fma(acts[(L - 1) * 128 + v], a.mlp[a.woff[L] + v], o);
+  for (int v = tid; v < fl; v += NT) o = fma(acts[(L - 1) * HB + v], a.mlp[a.woff[L] + v], o);
   double r[1] = {o};
   block_sum<1>(r, red);
   return r[0] + a.mlp[a.boff[L]];
@@ -769,15 +785,15 @@ __device__ double gmlp_forward(const Dev1& a, const double* x, double* acts, dou
 __global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k_launch) {
   extern __shared__ double smem[];
   __shared__ double red[NWAVE * 8];
-  const int s = blockIdx.x, N = a.N, tid = threadIdx.x;
+  const int s = blockIdx.x, N = a.N, tid = threadIdx.x, HB = a.hmax;
   const int k = iter_of(a, s, k_launch);
   const bool doPot = k >= 0 && k < a.T && a.st[4 * s + 0] == 0;
   const bool doE = a.st[4 * s + 1] >= 0 && a.st[4 * s + 1] < a.T;
-  double* x = smem;                 // N
-  double* acts = x + N;             // 6 x 128
-  double* dzs = acts + 6 * 128;     // 6 x 128
-  double* part = dzs + 6 * 128;     // NT
-  double* dl = part + NT;           // 2 x 128 backward buffers
+  double* x = smem;                       // N
+  double* acts = x + N;                   // (nlay - 1) x HB
+  double* dzs = acts + (a.nlay - 1) * HB;  // (nlay - 1) x HB
+  double* part = dzs + (a.nlay - 1) * HB;  // NT
+  double* dl = part + NT;                 // 2 x HB backward buffers
   const double in = 1.0 / a.norm;
   if (doPot) {
     const double* n = input_density(a, s, k);
@@ -788,15 +804,15 @@ __global__ __launch_bounds__(NT) void k_gmlp(Dev1 a, int k_launch) {
     // backward: d f / d acts of the last hidden layer = df W_L, then through the hidden layers
     const int L = a.nlay - 1;
     double* cur = dl;
-    double* nxt = dl + 128;
-    for (int v = tid; v < a.widths[L]; v += NT) cur[v] = df * a.mlp[a.woff[L] + v] * dzs[(L - 1) * 128 + v];
+    double* nxt = dl + HB;
+    for (int v = tid; v < a.widths[L]; v += NT) cur[v] = df * a.mlp[a.woff[L] + v] * dzs[(L - 1) * HB + v];
     __syncthreads();
     for (int l = L - 1; l >= 1; --l) {
       const int fi = a.widths[l], fo = a.widths[l + 1];
-      if (tid < fi) {
+      for (int v = tid; v < fi; v += NT) {
         double g = 0.0;
-        for (int u = 0; u < fo; ++u) g = fma(a.mlp[a.woff[l] + tid * fo + u], cur[u], g);
-        nxt[tid] = g * dzs[(l - 1) * 128 + tid];
+        for (int u = 0; u < fo; ++u) g = fma(a.mlp[a.woff[l] + v * fo + u], cur[u], g);
+        nxt[v] = g * dzs[(l - 1) * HB + v];
       }
       __syncthreads();
       double* t2 = cur;
@@ -1678,8 +1694,8 @@ int validate1d(const qex1d_problem* p) {
     if (p->mlp_widths[0] != p->num_grids) return fail(QEX_E_BADARG, "GLOBAL_MLP input width must be num_grids");
     if (p->mlp_widths[p->mlp_layers] != 1) return fail(QEX_E_UNSUPPORTED, "MLP output width must be 1");
     for (int l = 1; l < p->mlp_layers; ++l)
-      if (p->mlp_widths[l] < 1 || p->mlp_widths[l] > 128)
-        return fail(QEX_E_UNSUPPORTED, "GLOBAL_MLP hidden widths must be in [1, 128]");
+      if (p->mlp_widths[l] < 1 || p->mlp_widths[l] > QEX1D_MAX_MLP_WIDTH)
+        return fail(QEX_E_UNSUPPORTED, "GLOBAL_MLP hidden widths must be in [1, %d]", QEX1D_MAX_MLP_WIDTH);
     if (p->mlp_activation < 0 || p->mlp_activation > QEX_ACT_GELU_TANH) return fail(QEX_E_BADARG, "bad activation");
     if (!(p->mlp_norm != 0.0)) return fail(QEX_E_BADARG, "mlp_norm must be nonzero");
   }
@@ -1698,9 +1714,13 @@ int validate1d(const qex1d_problem* p) {
   return QEX_OK;
 }
 
-size_t mlp_lds_bytes(const Dev1& a) {
-  const size_t stride = a.wmax + 1;
-  return ((size_t)a.N + 2 * MLP_P * stride + (size_t)(a.nlay - 1) * MLP_P * stride) * 8;
+size_t mlp_lds_bytes(const Dev1& a, int lpp) {
+  const size_t stride = a.wmax + 1, P = NT / lpp;
+  return ((size_t)a.N + 2 * P * stride + (size_t)(a.nlay - 1) * P * stride) * 8;
+}
+constexpr size_t LDS_MAX = 160 * 1024;
+size_t gmlp_lds_bytes(const Dev1& a) {
+  return ((size_t)a.N + 2 * (size_t)(a.nlay - 1) * a.hmax + NT + 2 * (size_t)a.hmax) * 8;
 }
 
 size_t eigen_lds_bytes(int N) { return (size_t)(6 * N + 16) * 8; }
@@ -1789,7 +1809,8 @@ namespace {
 struct Plan1d {
   Dev1 a;
   EigArgs e;
-  size_t lds_pot, lds_eig, lds_mlp;
+  size_t lds_pot, lds_eig, lds_mlp, lds_gmlp;
+  int mlp_lpp;  // lanes per point of the generic local-MLP kernel
   bool mlp_mfma;
   dim3 gmlp_mfma, gpot, geig, gmlp;
 };
@@ -1844,6 +1865,8 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
       off += a.widths[l + 1];
       if (l + 1 < a.nlay) a.wmax = a.wmax > a.widths[l + 1] ? a.wmax : a.widths[l + 1];
     }
+    a.hmax = 1;
+    for (int l = 1; l < a.nlay; ++l) a.hmax = a.hmax > a.widths[l] ? a.hmax : a.widths[l];
   }
   a.table = in->interaction_table;
   a.vext = in->external_potential;
@@ -1901,15 +1924,20 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
   bool mlp_mfma = a.xc == QEX1D_XC_LOCAL_MLP && a.nlay - 1 <= MLP_LH && a.W <= MLP_S - 1;
   for (int l = 1; l < a.nlay && mlp_mfma; ++l) mlp_mfma = a.widths[l] <= MLP_HMAX;
   P->mlp_mfma = mlp_mfma;
-  P->lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(MP * MLP_S + MP) * 8 : mlp_lds_bytes(a);
-  if (P->lds_mlp > 160 * 1024) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", P->lds_mlp);
+  // the generic path: 8 lanes per point (32 points per workgroup) when their activations fit in
+  // LDS, else 64 lanes per point (4 points)
+  P->mlp_lpp = mlp_lds_bytes(a, 8) <= LDS_MAX ? 8 : 64;
+  P->lds_mlp = a.xc != QEX1D_XC_LOCAL_MLP ? 0 : mlp_mfma ? (size_t)(MP * MLP_S + MP) * 8 : mlp_lds_bytes(a, P->mlp_lpp);
+  if (P->lds_mlp > LDS_MAX) return fail(QEX_E_UNSUPPORTED, "MLP too wide for LDS (%zu bytes)", P->lds_mlp);
+  P->lds_gmlp = a.xc != QEX1D_XC_GLOBAL_MLP ? 0 : gmlp_lds_bytes(a);
+  if (P->lds_gmlp > LDS_MAX) return fail(QEX_E_UNSUPPORTED, "global MLP too wide for LDS (%zu bytes)", P->lds_gmlp);
   P->gmlp_mfma = dim3((unsigned)(((long)a.B * a.N + MP - 1) / MP));
   P->gpot = dim3(a.nTiles, a.B);
   P->geig = dim3(a.K * a.B);
-  P->gmlp = dim3((a.N + MLP_P - 1) / MLP_P, a.B);
+  P->gmlp = dim3((a.N + NT / P->mlp_lpp - 1) / (NT / P->mlp_lpp), a.B);
   if (P->lds_eig > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_eig);
-  const void* mlp_fn = (const void*)k_mlp;
+  const void* mlp_fn = P->mlp_lpp == 8 ? (const void*)k_mlp<8> : (const void*)k_mlp<64>;
   if (mlp_mfma) {
     const void* fns[4] = {(const void*)k_mlp_mfma<1>, (const void*)k_mlp_mfma<2>, (const void*)k_mlp_mfma<3>,
                           (const void*)k_mlp_mfma<4>};
@@ -1917,6 +1945,8 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
   }
   if (P->lds_mlp > 64 * 1024)
     (void)hipFuncSetAttribute(mlp_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_mlp);
+  if (P->lds_gmlp > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_gmlp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_gmlp);
   return QEX_OK;
 }
 
@@ -1933,12 +1963,15 @@ void launch_iteration(const Plan1d& P, int k, double alpha_k, bool energy_only, 
         case 3: hipLaunchKernelGGL(k_mlp_mfma<3>, P.gmlp_mfma, dim3(NT), P.lds_mlp, stream, a, k); break;
         default: hipLaunchKernelGGL(k_mlp_mfma<4>, P.gmlp_mfma, dim3(NT), P.lds_mlp, stream, a, k); break;
       }
-    } else
-      hipLaunchKernelGGL(k_mlp, P.gmlp, dim3(NT), P.lds_mlp, stream, a, k);
+    } else if (P.mlp_lpp == 8) {
+      hipLaunchKernelGGL(k_mlp<8>, P.gmlp, dim3(NT), P.lds_mlp, stream, a, k);
+    } else {
+      hipLaunchKernelGGL(k_mlp<64>, P.gmlp, dim3(NT), P.lds_mlp, stream, a, k);
+    }
   }
   if (a.xc == QEX1D_XC_GLOBAL_MLP) {
     Scope ps(qexprof::K_MLP, stream);
-    hipLaunchKernelGGL(k_gmlp, dim3(a.B), dim3(NT), (size_t)(a.N + 12 * 128 + NT + 256) * 8, stream, a, k);
+    hipLaunchKernelGGL(k_gmlp, dim3(a.B), dim3(NT), P.lds_gmlp, stream, a, k);
   }
   {
     Scope ps(qexprof::K_POTENTIAL, stream);

@@ -195,7 +195,7 @@ class GlobalMLPXC(LocalMLPXC):
     v_xc = dx df/dn (od/scf.py:32-76).
 
     ``params``: list of (W, b) per Dense layer, W of shape (fan_in, fan_out); the first fan-in is
-    the number of grid points, hidden widths at most 128, output width 1.
+    the number of grid points, hidden widths at most ``QEX1D_MAX_MLP_WIDTH`` (1024), output width 1.
     """
 
     encoding = "global"
@@ -222,8 +222,8 @@ class GlobalMLPXC(LocalMLPXC):
             raise ValueError(f"first layer fan-in {ws[0]} != number of grid points {problem.num_grids}")
         if len(self.params) > _lib.QEX1D_MAX_MLP_LAYERS:
             raise ValueError("too many layers")
-        if ws[-1] != 1 or max(ws[1:-1], default=1) > 128:
-            raise ValueError("GlobalMLPXC needs output width 1 and hidden widths <= 128")
+        if ws[-1] != 1 or max(ws[1:-1], default=1) > _lib.QEX1D_MAX_MLP_WIDTH:
+            raise ValueError(f"GlobalMLPXC needs output width 1 and hidden widths <= {_lib.QEX1D_MAX_MLP_WIDTH}")
         problem.xc_kind = self.xc_kind
         problem.mlp_layers = len(self.params)
         for i, w in enumerate(ws):

@@ -96,7 +96,9 @@ def test_global_mlp_validation():
     p.mlp_widths[0] = p.num_grids - 1  # input width must be the grid size
     assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
     p.mlp_widths[0] = p.num_grids
-    p.mlp_widths[1] = 129  # hidden widths at most 128
+    p.mlp_widths[1] = 513  # the reference notebook's global network width
+    assert lib.qex1d_workspace_bytes(ctypes.byref(p)) > 0
+    p.mlp_widths[1] = _lib.QEX1D_MAX_MLP_WIDTH + 1  # hidden widths at most QEX1D_MAX_MLP_WIDTH
     assert lib.qex1d_workspace_bytes(ctypes.byref(p)) == 0
     assert "hidden" in lib.qex_last_error().decode()
 

@@ -332,3 +332,42 @@ def test_eval_kohn_sham_lda_python_loop():
                       enforce_reflection_symmetry=True)
     st = qeval.kohn_sham(None, loc, np.array([1, 1]), 2, 8, grids, None, use_amplitude_encoding=False, use_lda=True)
     _compare(st, ref, O.get_dx(grids))
+
+
+def test_notebook_width_global_mlp_513():
+    """The reference notebook's global network (notebooks/models/02b_notebook_od_mlp.ipynb:645:
+    build_global_mlp(n_neurons=513, n_layers=2, activation="tanh")) on the 513-point grid: hidden
+    layers wider than a workgroup, amplitude-encoded loop, three iterations against the oracle."""
+    from qex_amd.od import scf, utils, xc
+
+    grids = np.linspace(-25.6, 25.6, 513)
+    loc, Z = np.array([-0.7, 0.7]), np.array([1, 1])
+    n0 = 2 * O.gaussian(grids, 0.0, 1.0)
+    W, b = _global_mlp([513, 513, 513, 1], 31, scale=0.5)
+    ref = O.kohn_sham(loc, Z, 2, 3, grids, O.GlobalMLP(W, b, "tanh", 2.0), O.exponential_coulomb, n0)
+    f = xc.GlobalMLPXC(list(zip(W, b)), activation="tanh", norm=2.0)
+    st = scf.kohn_sham_amplitude_encoded(loc, Z, 2, 3, grids, f, utils.exponential_coulomb, initial_density=n0)
+    _compare(st, ref, O.get_dx(grids))
+    np.testing.assert_allclose(st.numpy().xc_energy_density, ref.xc_energy_density, rtol=1e-10, atol=1e-10)
+
+
+def test_notebook_width_local_mlp_513():
+    """The reference notebook's local network (02b_notebook_od_mlp.ipynb:370: build_local_mlp(
+    n_neurons=num_grids, n_layers=2, activation="tanh"), one density value per point) with 513-wide
+    hidden layers: the generic kernel with 64 lanes per point (32 points' activations do not fit in
+    LDS), scan loop, four iterations against the oracle."""
+    from qex_amd.od import engine, utils, xc
+
+    grids = np.linspace(-25.6, 25.6, 513)
+    W, b = _global_mlp([1, 513, 513, 1], 37, scale=0.5)
+    f = xc.LocalMLPXC(list(zip(W, b)), window=1, activation="tanh", norm=2.0)
+    loc, Z = np.array([-0.7, 0.7]), np.array([1.0, 1.0])
+    prob = engine.Problem1D(grids, [loc], [Z], [2], utils.exponential_coulomb, f, 4, loop="scan")
+    n0 = engine.solve_noninteracting(prob.vext, prob.num_electrons, prob.grids).density
+    res = prob.run(n0)
+    ref = O.jit_kohn_sham(loc, Z, 2, 4, grids, O.LocalMLP(W, b, "tanh", 1, 2.0), O.exponential_coulomb,
+                          n0[0].cpu().numpy())
+    for t in range(4):
+        assert abs(float(res.total_energy[0, t]) - ref.total_energy[t]) < E_TOL, t
+        assert _l2(res.density[0, t].cpu().numpy(), ref.density[t], O.get_dx(grids)) < L2_TOL, t
+        np.testing.assert_allclose(_np(res.xc_potential[0, t]), ref.xc_potential[t], rtol=0, atol=1e-9)

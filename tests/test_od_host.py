@@ -32,10 +32,13 @@ def test_global_mlp_configure_checks_shapes():
     p.num_grids = 40
     with pytest.raises(ValueError, match="grid points"):
         xc.GlobalMLPXC(list(zip(W, b))).configure(p)
-    W2, b2 = O.glorot_normal_mlp(3, [41, 200, 1])
+    W2, b2 = O.glorot_normal_mlp(3, [41, 513, 1])  # the reference notebook's width is accepted
     p.num_grids = 41
+    xc.GlobalMLPXC(list(zip(W2, b2))).configure(p)
+    assert list(p.mlp_widths[:3]) == [41, 513, 1]
+    W3, b3 = O.glorot_normal_mlp(3, [41, _lib.QEX1D_MAX_MLP_WIDTH + 1, 1])
     with pytest.raises(ValueError, match="hidden"):
-        xc.GlobalMLPXC(list(zip(W2, b2))).configure(p)
+        xc.GlobalMLPXC(list(zip(W3, b3))).configure(p)
 
 
 def test_global_mlp_is_amplitude_encoded():

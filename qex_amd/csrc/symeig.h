@@ -98,33 +98,76 @@ __device__ __forceinline__ double start_entry(int slot, int i) {
 }
 
 // Inverse pivots 1/D_i of the LDL^T factorisation of T_block - lam (|D| >= tiny), stored in column
-// `col` of Dm (row stride S).
+// `col` of Dm (row stride S).  d and e^2 are read 4 rows per LDS wait (one thread runs the whole
+// recurrence, so only the arithmetic stays on its dependency chain; 8 rows spill registers).
 __device__ __forceinline__ void ldl_pivots(const double* d, const double* e2, int s, int t, double lam, double tiny, double* Dm,
                            int S, int col) {
   double q = d[s] - lam;
   if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
   double qi = 1.0 / q;
   Dm[s * S + col] = qi;
-  for (int i = s + 1; i < t; ++i) {
-    q = (d[i] - lam) - e2[i - 1] * qi;
+  auto step = [&](int i, double di, double ei) {
+    q = (di - lam) - ei * qi;
     if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
     qi = 1.0 / q;
     Dm[i * S + col] = qi;
+  };
+  int i = s + 1;
+  for (; i + 4 <= t; i += 4) {
+    double dv[4], ev[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      dv[u] = d[i + u];
+      ev[u] = e2[i + u - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) step(i + u, dv[u], ev[u]);
   }
+  for (; i < t; ++i) step(i, d[i], e2[i - 1]);
 }
 
 // (T_block - lam) x = b in place in column `col` of Y, with the inverse pivots from ldl_pivots:
-// forward y_i = b_i - e_i-1 (1/D_i-1) y_i-1, then x_i = (y_i - e_i x_i+1) / D_i.
+// forward y_i = b_i - e_i-1 (1/D_i-1) y_i-1, then x_i = (y_i - e_i x_i+1) / D_i.  The operands of
+// 4 rows are loaded per LDS wait; the chain carries one FMA (and one multiply) per row.
 __device__ __forceinline__ void ldl_solve(const double* e, int s, int t, const double* Dm, double* Y, int S, int col) {
   double prev = Y[s * S + col];
-  for (int i = s + 1; i < t; ++i) {
+  int i = s + 1;
+  for (; i + 4 <= t; i += 4) {
+    double yv[4], cv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      yv[u] = Y[(i + u) * S + col];
+      cv[u] = e[i + u - 1] * Dm[(i + u - 1) * S + col];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      prev = yv[u] - cv[u] * prev;
+      Y[(i + u) * S + col] = prev;
+    }
+  }
+  for (; i < t; ++i) {
     const double y = Y[i * S + col] - e[i - 1] * Dm[(i - 1) * S + col] * prev;
     Y[i * S + col] = y;
     prev = y;
   }
   double next = Y[(t - 1) * S + col] * Dm[(t - 1) * S + col];
   Y[(t - 1) * S + col] = next;
-  for (int i = t - 2; i >= s; --i) {
+  i = t - 2;
+  for (; i - 3 >= s; i -= 4) {
+    double yv[4], ev[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      yv[u] = Y[(i - u) * S + col];
+      ev[u] = e[i - u];
+      dv[u] = Dm[(i - u) * S + col];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      next = (yv[u] - ev[u] * next) * dv[u];
+      Y[(i - u) * S + col] = next;
+    }
+  }
+  for (; i >= s; --i) {
     next = (Y[i * S + col] - e[i] * next) * Dm[i * S + col];
     Y[i * S + col] = next;
   }
@@ -366,8 +409,8 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
             }
           }
           const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
-          double rq = 0.0;
           for (int i = s; i < t; ++i) Z[i * S + a] *= nr;
+          double rq = 0.0;
           for (int i = s; i < t; ++i) {
             double ty = sh.d[i] * Z[i * S + a];
             if (i > s) ty = fma(sh.e[i - 1], Z[(i - 1) * S + a], ty);

@@ -363,6 +363,17 @@ size_t qex3d_gen_eigh_workspace_bytes(int32_t batch, int32_t n);
 int qex3d_gen_eigh(int32_t batch, int32_t n, const double* F, const double* S, double* mo_energy, double* mo_coeff,
                    void* workspace, size_t workspace_bytes, void* stream);
 
+/* Reverse pass of the batched symmetric eigendecomposition A = V diag(w) V^T (w [batch][n] ascending,
+ * v [batch][n][n] row-major with eigenvectors in the columns) for cotangents gw [batch][n] and gv
+ * [batch][n][n] (either may be NULL): gA = sym(V (F o (U gv) + diag(gw)) U), F_ij = 1/(w_j - w_i)
+ * and 0 where |w_j - w_i| < eps^0.6 (degenerate pairs, whose numerator is taken as zero).
+ * exact = 0 reproduces qedft/train/td/generalized_eigensolver.py:158-219 (degen_eigh_bwd) as written:
+ * its eivect = jnp.transpose(eivec, axes=(-2, -1)) is the identity on a 2-D array, so U = V;
+ * exact = 1 uses U = V^T, the derivative of the decomposition. */
+size_t qex3d_eigh_vjp_workspace_bytes(int32_t batch, int32_t n);
+int qex3d_eigh_vjp(int32_t batch, int32_t n, const double* w, const double* v, const double* gw, const double* gv,
+                   int32_t exact, double* gA, void* workspace, size_t workspace_bytes, void* stream);
+
 typedef struct qex3d_scf_problem {
   int32_t nao, nelectron, method, max_cycle, diis_space;
   int64_t ngrid;

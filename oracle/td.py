@@ -544,6 +544,26 @@ def gen_eigh(F, S):
     return scipy.linalg.eigh(F, S)
 
 
+def degen_eigh_bwd(eival, eivec, g_eival=None, g_eivec=None, exact=False):
+    """Reverse pass of A = V diag(w) V^T with degenerate pairs masked
+    (qedft/train/td/generalized_eigensolver.py:158-219): M = F o (U gV) + diag(gw) with
+    F_ij = 1/(w_j - w_i), zero where |w_j - w_i| < eps^0.6, result sym(V M U).  The reference's code
+    takes U = jnp.transpose(eivec, axes=(-2, -1)), the identity on a 2-D array, so U = V (exact=False
+    reproduces it); exact=True takes U = V^T, the derivative of the decomposition (pinned by finite
+    differences in tests/test_oracle_td.py)."""
+    w = np.asarray(eival, dtype=np.float64)
+    V = np.asarray(eivec, dtype=np.float64)
+    U = V.T if exact else V
+    res = np.zeros_like(V)
+    if g_eivec is not None:
+        F = w[None, :] - w[:, None]
+        F = np.where(np.abs(F) < np.finfo(np.float64).eps ** 0.6, np.inf, F)
+        res = V @ ((1.0 / F) * (U @ np.asarray(g_eivec, dtype=np.float64))) @ U
+    if g_eival is not None:
+        res = res + V @ (np.asarray(g_eival, dtype=np.float64)[:, None] * U)
+    return 0.5 * (res + res.T)
+
+
 def make_rdm1(C, nocc):
     """D = 2 C_occ C_occ^T (hf.py:303-308, aufbau occupation of the N_e/2 lowest)."""
     Co = C[:, :nocc]

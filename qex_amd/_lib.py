@@ -188,6 +188,8 @@ SIGNATURES = {
     "qex3d_eval_xc_nn": (_I, [_I, ctypes.c_int64, _P, _P, _I, _I, ctypes.c_double, _P, _P, _P, _S, _P]),
     "qex3d_gen_eigh_workspace_bytes": (_S, [_I, _I]),
     "qex3d_gen_eigh": (_I, [_I, _I, _P, _P, _P, _P, _P, _S, _P]),
+    "qex3d_eigh_vjp_workspace_bytes": (_S, [_I, _I]),
+    "qex3d_eigh_vjp": (_I, [_I, _I, _P, _P, _P, _P, _I, _P, _P, _S, _P]),
     "qex3d_scf_workspace_bytes": (_S, [ctypes.POINTER(Qex3dScfProblem)]),
     "qex3d_scf": (_I, [ctypes.POINTER(Qex3dScfProblem), ctypes.POINTER(Qex3dScfInputs),
                        ctypes.POINTER(Qex3dScfOutputs), _P, _S, _P]),

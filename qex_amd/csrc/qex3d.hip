@@ -1941,6 +1941,43 @@ __global__ __launch_bounds__(NT_EIG) void k_gen_eigh(int n, const double* F, con
   (void)Zg;
 }
 
+// Reverse pass of the symmetric eigendecomposition A = V diag(w) V^T with degenerate pairs masked
+// (qedft/train/td/generalized_eigensolver.py:158-219, degen_eigh_bwd): with U = V^T (exact) or U = V
+// (the reference's code: its transpose with axes (-2, -1) leaves a 2-D array unchanged),
+//   M = F o (U gV) + diag(gw),  F_ij = 1 / (w_j - w_i), 0 where |w_j - w_i| < eps^0.6,
+//   gA = sym(V M U);
+// either cotangent may be absent.  One workgroup per matrix; products on FP64 matrix cores.
+__global__ __launch_bounds__(NT_EIG) void k_eigh_vjp(int n, const double* w, const double* v, const double* gw,
+                                                     const double* gv, double* gA, double* ws, int exact) {
+  const size_t nn = (size_t)n * n, b = blockIdx.x;
+  const double* W = w + b * n;
+  const double* V = v + b * nn;
+  double* M = ws + b * 2 * nn;
+  double* T = M + nn;
+  double* G = gA + b * nn;
+  const bool ut = exact != 0;  // U = V^T: op(U) reads V transposed
+  if (gv) mfma_gemm(n, n, n, V, n, ut, gv + b * nn, n, false, M, n, 1.0);  // U gV
+  const double thr = 4.05381695970951e-10;  // float64 eps^0.6, eps = 2^-52
+  for (size_t t = threadIdx.x; t < nn; t += blockDim.x) {
+    const int i = (int)(t / n), j = (int)(t % n);
+    const double f = W[j] - W[i];
+    double m = gv && fabs(f) >= thr ? M[t] / f : 0.0;
+    if (i == j && gw) m += gw[b * n + i];
+    M[t] = m;
+  }
+  __syncthreads();
+  mfma_gemm(n, n, n, M, n, false, V, n, ut, T, n, 1.0);  // M U
+  mfma_gemm(n, n, n, V, n, false, T, n, false, G, n, 1.0);  // V M U
+  for (size_t t = threadIdx.x; t < nn; t += blockDim.x) {
+    const int i = (int)(t / n), j = (int)(t % n);
+    if (i < j) {
+      const double s = 0.5 * (G[i * n + j] + G[j * n + i]);
+      G[i * n + j] = s;
+      G[j * n + i] = s;
+    }
+  }
+}
+
 // Energies and convergence (one workgroup of NT_SCF threads).
 __global__ __launch_bounds__(NT_SCF) void k_scf_energy(const Scf3* __restrict__ ss, int cycle) {
   const Scf3 s = ss[blockIdx.x];
@@ -3064,6 +3101,22 @@ int qex3d_gen_eigh(int32_t batch, int32_t n, const double* F, const double* S, d
   hipLaunchKernelGGL(k_gen_eigh, dim3(batch), dim3(NT_EIG), lds, (hipStream_t)stream, n, F, S, mo_energy, mo_coeff,
                      (double*)workspace);
   return launch_status("qex3d_gen_eigh");
+}
+
+size_t qex3d_eigh_vjp_workspace_bytes(int32_t batch, int32_t n) {
+  if (batch < 1 || n < 1 || n > QEX3D_MAX_NAO) return 0;
+  return al((size_t)batch * 2 * n * n * 8);
+}
+
+int qex3d_eigh_vjp(int32_t batch, int32_t n, const double* w, const double* v, const double* gw, const double* gv,
+                   int32_t exact, double* gA, void* workspace, size_t workspace_bytes, void* stream) {
+  if (batch < 1 || n < 1 || n > QEX3D_MAX_NAO || !w || !v || !gA)
+    return fail3(QEX_E_BADARG, "bad eigh_vjp arguments (n must be in [1, %d])", QEX3D_MAX_NAO);
+  const size_t need = qex3d_eigh_vjp_workspace_bytes(batch, n);
+  if (!workspace || workspace_bytes < need) return fail3(QEX_E_WORKSPACE, "workspace too small: need %zu", need);
+  hipLaunchKernelGGL(k_eigh_vjp, dim3(batch), dim3(NT_EIG), 0, (hipStream_t)stream, n, w, v, gw, gv, gA,
+                     (double*)workspace, exact);
+  return launch_status("qex3d_eigh_vjp");
 }
 
 // argument checks of one molecule; `pre` prefixes the message (the molecule's index in a batch)

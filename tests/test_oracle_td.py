@@ -245,3 +245,60 @@ def test_local_network_scf_runs_and_uses_the_pointwise_derivative():
     net = T.LocalMLP3D(W, b, scale=0.5)
     res = T.scf(mol, "rks_nn_local", grid=(pts, w), net=net, conv_tol=1e-10, max_cycle=60)
     assert res.converged and np.isfinite(res.e_tot)
+
+
+def _sym(rng, n):
+    a = rng.standard_normal((n, n))
+    return 0.5 * (a + a.T)
+
+
+def test_degen_eigh_bwd_exact_matches_finite_differences():
+    """generalized_eigensolver.py:158-219 in its exact form (U = V^T) is the derivative of the
+    decomposition: <gA, E> equals central differences of L = c.w + <G, V> along symmetric E
+    (eigenvector signs aligned with the unperturbed ones)."""
+    rng = np.random.default_rng(3)
+    n = 6
+    A = _sym(rng, n)
+    c, G = rng.standard_normal(n), rng.standard_normal((n, n))
+    w0, V0 = np.linalg.eigh(A)
+
+    def loss(M):
+        w, V = np.linalg.eigh(M)
+        V = V * np.sign(np.sum(V * V0, axis=0))[None, :]
+        return c @ w + np.sum(G * V)
+
+    gA = T.degen_eigh_bwd(w0, V0, c, G, exact=True)
+    for k in range(4):
+        E = _sym(rng, n)
+        h = 1e-6
+        fd = (loss(A + h * E) - loss(A - h * E)) / (2 * h)
+        assert abs(np.sum(gA * E) - fd) < 1e-7 * max(1.0, abs(fd)), k
+
+
+def test_degen_eigh_bwd_reference_form_equals_exact_for_symmetric_eigenvectors():
+    """The reference's code uses U = V (its transpose with axes (-2, -1) is the identity on a 2-D
+    array); that agrees with the exact form exactly when V = V^T (a reflector) and not otherwise."""
+    rng = np.random.default_rng(4)
+    n = 7
+    u = rng.standard_normal(n)
+    H = np.eye(n) - 2.0 * np.outer(u, u) / (u @ u)  # symmetric and orthogonal
+    w = np.sort(rng.standard_normal(n))
+    gw, gV = rng.standard_normal(n), rng.standard_normal((n, n))
+    np.testing.assert_allclose(T.degen_eigh_bwd(w, H, gw, gV), T.degen_eigh_bwd(w, H, gw, gV, exact=True),
+                               rtol=1e-12, atol=1e-12)
+    w1, V1 = np.linalg.eigh(_sym(rng, n))
+    assert not np.allclose(T.degen_eigh_bwd(w1, V1, gw, gV), T.degen_eigh_bwd(w1, V1, gw, gV, exact=True))
+
+
+def test_degen_eigh_bwd_masks_degenerate_pairs():
+    """A pair closer than eps^0.6 contributes nothing through its eigenvector coupling: the result stays
+    finite, and an eigenvector cotangent inside the degenerate pair's span (antisymmetric in the pair)
+    gives zero from that pair."""
+    rng = np.random.default_rng(5)
+    n = 5
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    w = np.array([-1.0, 0.5, 0.5 + 1e-11, 2.0, 3.0])
+    gV = np.zeros((n, n))
+    gV[:, 1], gV[:, 2] = Q[:, 2], -Q[:, 1]  # U gV is nonzero only on the (1, 2) pair
+    g = T.degen_eigh_bwd(w, Q, None, gV, exact=True)
+    assert np.all(np.isfinite(g)) and np.abs(g).max() < 1e-12

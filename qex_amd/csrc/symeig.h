@@ -195,10 +195,38 @@ __device__ __forceinline__ double dot8(const double* a, int sa, const double* b,
   return ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
 }
 
+// sum_i a[i * S] b[i * S] over rows s..t-1 of two columns: 4 rows per LDS wait, 4 partial sums
+// combined in a fixed order
+__device__ __forceinline__ double col_dot4(const double* a, const double* b, int S, int s, int t) {
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+  int i = s;
+  for (; i + 4 <= t; i += 4) {
+    const double a0 = a[i * S], a1 = a[(i + 1) * S], a2 = a[(i + 2) * S], a3 = a[(i + 3) * S];
+    const double b0 = b[i * S], b1 = b[(i + 1) * S], b2 = b[(i + 2) * S], b3 = b[(i + 3) * S];
+    c0 = fma(a0, b0, c0);
+    c1 = fma(a1, b1, c1);
+    c2 = fma(a2, b2, c2);
+    c3 = fma(a3, b3, c3);
+  }
+  for (; i < t; ++i) c0 = fma(a[i * S], b[i * S], c0);
+  return (c0 + c1) + (c2 + c3);
+}
+
 __device__ __forceinline__ double col_norm2(const double* Y, int S, int s, int t, int col) {
-  double r = 0.0;
-  for (int i = s; i < t; ++i) r = fma(Y[i * S + col], Y[i * S + col], r);
-  return r;
+  return col_dot4(Y + col, Y + col, S, s, t);
+}
+
+// x[i * S] *= f over rows s..t-1, 4 rows per LDS wait
+__device__ __forceinline__ void col_scale4(double* x, int S, int s, int t, double f) {
+  int i = s;
+  for (; i + 4 <= t; i += 4) {
+    const double x0 = x[i * S], x1 = x[(i + 1) * S], x2 = x[(i + 2) * S], x3 = x[(i + 3) * S];
+    x[i * S] = x0 * f;
+    x[(i + 1) * S] = x1 * f;
+    x[(i + 2) * S] = x2 * f;
+    x[(i + 3) * S] = x3 * f;
+  }
+  for (; i < t; ++i) x[i * S] *= f;
 }
 
 // A: n x n symmetric in LDS (row stride S, destroyed); Z: n x n LDS output, column j = eigenvector
@@ -399,25 +427,34 @@ __device__ __forceinline__ void symeig(int n, double* A, double* Z, int S, doubl
           ldl_pivots(sh.d, sh.e2, s, t, sh.lam[a], tiny, A, S, a);
           for (int i = s; i < t; ++i) Z[i * S + a] = start_entry(a, i);
           for (int it = 0; it < 3; ++it) {
-            const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
-            for (int i = s; i < t; ++i) Z[i * S + a] *= nr;
+            col_scale4(Z + a, S, s, t, 1.0 / sqrt(col_norm2(Z, S, s, t, a)));
             ldl_solve(sh.e, s, t, A, Z, S, a);
             for (int b = slot; b < a; ++b) {
-              double dp = 0.0;
-              for (int i = s; i < t; ++i) dp = fma(Z[i * S + b], Z[i * S + a], dp);
+              const double dp = col_dot4(Z + b, Z + a, S, s, t);
               for (int i = s; i < t; ++i) Z[i * S + a] = fma(-dp, Z[i * S + b], Z[i * S + a]);
             }
           }
-          const double nr = 1.0 / sqrt(col_norm2(Z, S, s, t, a));
-          for (int i = s; i < t; ++i) Z[i * S + a] *= nr;
-          double rq = 0.0;
-          for (int i = s; i < t; ++i) {
-            double ty = sh.d[i] * Z[i * S + a];
-            if (i > s) ty = fma(sh.e[i - 1], Z[(i - 1) * S + a], ty);
-            if (i + 1 < t) ty = fma(sh.e[i], Z[(i + 1) * S + a], ty);
-            rq = fma(Z[i * S + a], ty, rq);
+          col_scale4(Z + a, S, s, t, 1.0 / sqrt(col_norm2(Z, S, s, t, a)));
+          // Rayleigh quotient z^T T z = sum_i d_i z_i^2 + 2 sum_i e_i z_i z_i+1 (the eigenvalue to
+          // ~eps |T|), 4 partial sums
+          double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+          {
+            const double* z = Z + a;
+            int i = s;
+            for (; i + 4 <= t; i += 4) {
+              const double z0 = z[i * S], z1 = z[(i + 1) * S], z2 = z[(i + 2) * S], z3 = z[(i + 3) * S];
+              const double z4 = i + 4 < t ? z[(i + 4) * S] : 0.0;
+              q0 = fma(z0, fma(sh.d[i], z0, 2.0 * sh.e[i] * z1), q0);
+              q1 = fma(z1, fma(sh.d[i + 1], z1, 2.0 * sh.e[i + 1] * z2), q1);
+              q2 = fma(z2, fma(sh.d[i + 2], z2, 2.0 * sh.e[i + 2] * z3), q2);
+              q3 = fma(z3, fma(sh.d[i + 3], z3, i + 4 < t ? 2.0 * sh.e[i + 3] * z4 : 0.0), q3);
+            }
+            for (; i < t; ++i) {
+              const double zi = z[i * S];
+              q0 = fma(zi, fma(sh.d[i], zi, i + 1 < t ? 2.0 * sh.e[i] * z[(i + 1) * S] : 0.0), q0);
+            }
           }
-          sh.rq[a] = rq;  // Rayleigh quotient: the eigenvalue to ~eps |T|
+          sh.rq[a] = (q0 + q1) + (q2 + q3);
         }
       }
     }

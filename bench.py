@@ -123,6 +123,8 @@ def parse_args():
                     help="c5: arrival rates of the latency sweep (req/s)")
     ap.add_argument("--streams-3d", type=int, default=2, help="c5: concurrent 3D streams")
     ap.add_argument("--depth-3d", type=int, default=1, help="c5: 3D batches in flight per stream")
+    ap.add_argument("--iters-per-step-1d", type=int, default=1,
+                    help="c5: slot iterations per 1D step (one read-back of the slot states per step)")
     ap.add_argument("--batch-3d", type=int, default=256,
                     help="c4: 1 solves the chains one at a time, >1 solves them as one batched SCF; "
                          "c5: most 3D requests per batched dispatch")
@@ -877,7 +879,8 @@ def run_c5(args):
     runner = engine.BatchRunner(grids, utils.exponential_coulomb, f, T_ITERS, max_batch=1, alpha=ALPHA,
                                 alpha_decay=1.0, num_mixing_iterations=1, loop="scan")
     slots = engine.SlotRunner(grids, utils.exponential_coulomb, f, T_ITERS, slots=args.slots, alpha=ALPHA,
-                              alpha_decay=1.0, num_mixing_iterations=1, loop="scan", depth=3)
+                              alpha_decay=1.0, num_mixing_iterations=1, loop="scan", depth=3,
+                              iterations_per_step=args.iters_per_step_1d)
     mols = [gto.synthetic_chain(n, 100 + n) for n in C4_CHAINS]
     for m in mols:
         m.device_basis()
@@ -999,6 +1002,7 @@ def run_c5(args):
                                "H2/H4/H8/H16 (conv 1e-8, CDIIS 8, <= 30 cycles); Poisson arrivals at "
                                f"{args.rate:g} req/s (above the served capacity: the stream is backlogged)",
                    "requests_timed": ntimed, "arrival_rate": args.rate, "slots_1d": args.slots,
+                   "iterations_per_step_1d": args.iters_per_step_1d,
                    "max_batch_3d": args.batch_3d, "streams_3d": args.streams_3d, "depth_3d": args.depth_3d,
                    "path": ("served: continuous batching; 3D AO values stored in FP32, rho from them with FP64 sums, "
                             "Vxc contraction on FP32 matrix cores with FP64 sums across 16-point units, everything "

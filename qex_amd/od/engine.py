@@ -377,15 +377,16 @@ class SlotRunner:
     soon as a slot frees up instead of waiting for the next batched launch.
 
     ``admit(requests)`` places up to ``free_slots()`` requests (locations, charges, N_e, initial
-    density) into free slots; ``step()`` enqueues one iteration for every busy slot on the current
-    stream, and a device-to-host copy of the slot states and final energies into pinned memory
-    with an event; ``poll()`` returns the requests whose solve completed in the steps whose events
-    have fired, as (tag, total energy, converged).  A request's result equals its solo
+    density) into free slots; ``step()`` enqueues ``iterations_per_step`` iterations for every busy
+    slot on the current stream (a slot whose solve ends inside the group idles for the rest of it),
+    then a device-to-host copy of the slot states and final energies into pinned memory with an
+    event; ``poll()`` returns the requests whose solve completed in the steps whose events have
+    fired, as (tag, total energy, converged).  A request's result equals its solo
     ``qex1d_kohn_sham`` result bit for bit.  Nothing here synchronises the host with the device."""
 
     def __init__(self, grids, interaction_fn, xc, num_iterations, slots=64, max_electrons=8, alpha=0.5,
                  alpha_decay=0.9, num_mixing_iterations=2, density_mse_converge_tolerance=-1.0, loop="scan",
-                 depth=3, max_nuclei=4):
+                 depth=3, max_nuclei=4, iterations_per_step=1):
         dev = device()
         check_uniform(grids)
         self.dev = dev
@@ -435,7 +436,9 @@ class SlotRunner:
                                       _ptr(o.xc_potential), _ptr(o.xc_energy_density), _ptr(o.gap), _ptr(o.converged),
                                       None, _ptr(o.stats))
         self.free = list(range(S - 1, -1, -1))
-        self.busy = {}  # slot -> (tag, index of the step that starts it)
+        self.busy = {}  # slot -> (tag, index of the slot iteration call that starts it)
+        self.iters_per_step = max(1, int(iterations_per_step))
+        self.calls = 0  # slot iteration calls enqueued so far
         self.depth = int(depth)
         self.ring = [(torch.empty(S, dtype=torch.int32).pin_memory(), torch.empty(S, dtype=torch.float64).pin_memory(),
                       torch.empty(S, dtype=torch.uint8).pin_memory(), torch.cuda.Event()) for _ in range(self.depth)]
@@ -472,7 +475,7 @@ class SlotRunner:
             loc[i, :len(l)], z[i, :len(c)], ne[i] = l, c, int(e)
             n0[i] = np.asarray(d, dtype=np.float64).reshape(self.N)
             s = self.free.pop()
-            self.busy[s] = (tag, self.steps)
+            self.busy[s] = (tag, self.calls)
             slots.append(s)
         dev = self.dev
         idx = upload(np.asarray(slots, dtype=np.int64), np.int64, dev)
@@ -490,13 +493,18 @@ class SlotRunner:
         return slots
 
     def step(self):
-        """One iteration for every busy slot, then the read-back of the slot states (non-blocking)."""
+        """``iterations_per_step`` iterations for every busy slot, then the read-back of the slot states
+        (non-blocking)."""
         if not self.can_step():
             raise RuntimeError("too many steps in flight: poll() first")
-        _lib.check(_lib.load().qex1d_slots_step(ctypes.byref(self.problem), ctypes.byref(self.ins),
-                                                ctypes.byref(self.outs), _ptr(self.kslot), _ptr(self.admit_flags),
-                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_slots_step")
-        self.admit_flags.zero_()
+        lib = _lib.load()
+        for _ in range(self.iters_per_step):
+            _lib.check(lib.qex1d_slots_step(ctypes.byref(self.problem), ctypes.byref(self.ins), ctypes.byref(self.outs),
+                                            _ptr(self.kslot), _ptr(self.admit_flags), _ptr(self.ws), self.ws.numel(),
+                                            _stream()), "qex1d_slots_step")
+            if _ == 0:
+                self.admit_flags.zero_()  # admissions take effect in the step's first iteration
+            self.calls += 1
         r = self.next_ring
         self.next_ring = (r + 1) % self.depth
         hk, he, hc, ev = self.ring[r]
@@ -504,7 +512,7 @@ class SlotRunner:
         he.copy_(self.out.total_energy[:, self.T - 1], non_blocking=True)
         hc.copy_(self.out.converged[:, self.T - 1], non_blocking=True)
         ev.record()
-        self.inflight.append((r, self.steps))
+        self.inflight.append((r, self.calls))
         self.steps += 1
 
     def poll(self, wait=False):
@@ -518,8 +526,9 @@ class SlotRunner:
             elif not ev.query():
                 break
             self.inflight.pop(0)
-            # a request started at step a finishes at step a + T (T iterations, then its energy-only pass)
-            finished = [s for s, (_, a) in self.busy.items() if a + self.T == j]
+            # a request first processed by call a finishes with call a + T (T iterations, then its
+            # energy-only pass); the polled step ended with call j - 1
+            finished = [s for s, (_, a) in self.busy.items() if a + self.T < j]
             ks = hk.numpy()
             for s in finished:
                 if ks[s] != -2:

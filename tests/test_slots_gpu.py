@@ -19,8 +19,8 @@ def mods():
     return engine, utils, xc
 
 
-@pytest.mark.parametrize("decay,slots", [(1.0, 8), (0.9, 5)])
-def test_slot_results_equal_solo_solves(mods, decay, slots):
+@pytest.mark.parametrize("decay,slots,per_step", [(1.0, 8, 1), (0.9, 5, 1), (1.0, 6, 3)])
+def test_slot_results_equal_solo_solves(mods, decay, slots, per_step):
     engine, utils, xc = mods
     grids = np.linspace(-12.8, 12.8, 257)
     T, alpha = 6, 0.3
@@ -29,7 +29,7 @@ def test_slot_results_equal_solo_solves(mods, decay, slots):
     n0 = [O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in mols]
     f = xc.LocalMLPXC(list(zip(W, b)), window=9, activation="silu", norm=2.0)
     run = engine.SlotRunner(grids, utils.exponential_coulomb, f, T, slots=slots, alpha=alpha, alpha_decay=decay,
-                            num_mixing_iterations=1, loop="scan", depth=2)
+                            num_mixing_iterations=1, loop="scan", depth=2, iterations_per_step=per_step)
     # the workspace is not cleared by the library: idle slots must stay idle whatever it holds
     run.ws.fill_(0x5A)
     # staggered admissions: 3 now, more after two steps, the rest as slots free up

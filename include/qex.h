@@ -189,6 +189,24 @@ int qex1d_external_potential(int32_t batch, int32_t num_grids, const double* gri
 int qex1d_hartree_potential(int32_t batch, int32_t num_grids, double dx, const double* table,
                             const double* density, double* out, void* stream);
 
+/* Workspace bytes for qex1d_xc_evaluate (0 on bad arguments). */
+size_t qex1d_xc_workspace_bytes(const qex1d_problem* p);
+
+/* The XC functional p describes (xc_kind, xc_c/xc_p/xc_eps, network widths, activation, norm and
+ * output transform; batch, num_grids, dx) on p->batch given densities [B][N], with the network
+ * kernels the SCF runs; the SCF-only fields of p are not read, and no reflection is applied.
+ * Outputs (each may be NULL):
+ *  - energy_density: [B][N] for LOCAL kinds (e_xc per electron at each point), [B] for GLOBAL
+ *    kinds (POWER_GLOBAL: sum_i f(n_i); GLOBAL_MLP: the network's f(n));
+ *  - xc_energy [B]: LOCAL kinds dx sum_i e_i n_i, GLOBAL kinds dx times the value above;
+ *  - xc_potential [B][N]: LOCAL kinds (dE/dn_i)/dx = e_i + sum_j n_j de_j/dn_i, GLOBAL kinds dE/dn_i.
+ * Replaces qedft od/scf.py:32-76 get_xc_energy_amplitude_encoded / get_xc_potential_amplitude_encoded
+ * (GLOBAL kinds) and jax_dft scf.get_xc_energy / get_xc_potential (LOCAL kinds), and the
+ * functional's own evaluation xc_energy_density_fn(density). */
+int qex1d_xc_evaluate(const qex1d_problem* p, const double* xc_params, const double* density,
+                      double* energy_density, double* xc_energy, double* xc_potential, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
 /* Opt-in kernel timing for measurement.  While enabled, qex1d_kohn_sham records a hipEvent pair
  * on its stream around each kernel launch (at most max_launches).  qex_profile_read waits for
  * the recorded events and returns, per kernel kind (0 MLP, 1 potential, 2 eigen, 3 density,

@@ -160,6 +160,8 @@ SIGNATURES = {
     "qex1d_solve_noninteracting_noalloc": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "qex1d_external_potential": (_I, [_I, _I, _P, _I, _P, _P, _I, _D, _D, _P, _P]),
     "qex1d_hartree_potential": (_I, [_I, _I, _D, _P, _P, _P, _P]),
+    "qex1d_xc_workspace_bytes": (_S, [ctypes.POINTER(Qex1dProblem)]),
+    "qex1d_xc_evaluate": (_I, [ctypes.POINTER(Qex1dProblem), _P, _P, _P, _P, _P, _P, _S, _P]),
     "qex_profile_enable": (_I, [_I, _I]),
     "qex_profile_read": (_I, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), _I]),
     "qex_profile_kinds": (_I, [ctypes.c_uint32]),

@@ -1627,6 +1627,43 @@ __global__ __launch_bounds__(NT) void k_hartree(int N, double dx, const double* 
   out[(size_t)s * N + i] = dx * acc;
 }
 
+// k_xc_eval: the XC functional on given densities (a.n0), one workgroup per system, after the
+// network kernel has run on them as iteration 0 of a one-iteration batch (its outputs in mlp_e,
+// mlp_G, mlp_eout).  LOCAL kinds: e_i and v_i = e_i + sum_j n_j de_j/dn_i per point (the
+// derivative of sum_j e_j n_j), E = dx sum_i e_i n_i.  GLOBAL kinds: v_i = dE/dn_i, the value
+// (sum_i f(n_i), or the network's f(n)) and E = dx * value.  eps, vxc, exc may each be null.
+__global__ __launch_bounds__(NT) void k_xc_eval(Dev1 a, double* eps, double* vxc, double* exc) {
+  __shared__ double red[NWAVE];
+  const int s = blockIdx.x, N = a.N;
+  const double* n = a.n0 + (size_t)s * N;
+  const bool local = xc_is_local(a.xc);
+  double acc[1] = {0.0};
+  for (int i = threadIdx.x; i < N; i += NT) {
+    double e = 0.0, v = 0.0;
+    if (a.xc == QEX1D_XC_GLOBAL_MLP) {
+      v = a.mlp_e[(size_t)s * N + i];
+    } else if (a.xc == QEX1D_XC_LOCAL_MLP) {
+      e = a.mlp_e[(size_t)s * N + i];
+      v = mlp_v_unsym(a, s, i);
+    } else {
+      xc_point(a, n[i], &e, &v);
+    }
+    if (local) {
+      if (eps) eps[(size_t)s * N + i] = e;
+      acc[0] = fma(e, n[i], acc[0]);
+    } else {
+      acc[0] += e;
+    }
+    if (vxc) vxc[(size_t)s * N + i] = v;
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) {
+    const double val = a.xc == QEX1D_XC_GLOBAL_MLP ? a.mlp_eout[2 * s] : acc[0];
+    if (!local && eps) eps[s] = val;
+    if (exc) exc[s] = a.dx * val;
+  }
+}
+
 // --------------------------------------------------------------------------------------------
 // host-side layout
 // --------------------------------------------------------------------------------------------
@@ -1815,6 +1852,9 @@ struct Plan1d {
   dim3 gmlp_mfma, gpot, geig, gmlp;
 };
 
+void fill_network(const qex1d_problem* p, Dev1& a);
+int plan_network(Plan1d* P);
+
 int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, void* workspace,
            size_t workspace_bytes, Plan1d* P) {
   int rc = validate1d(p);
@@ -1849,25 +1889,7 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
   a.xc_p = p->xc_p;
   a.xc_eps = p->xc_eps;
   kinetic(p->dx, &a.Td, &a.Tb, &a.Tc);
-  if (a.xc == QEX1D_XC_LOCAL_MLP || a.xc == QEX1D_XC_GLOBAL_MLP) {
-    a.nlay = p->mlp_layers;
-    a.act = p->mlp_activation;
-    a.norm = p->mlp_norm;
-    a.mlpout = p->mlp_output;
-    a.W = p->mlp_widths[0];
-    int off = 0;
-    a.wmax = a.W;
-    for (int l = 0; l <= a.nlay; ++l) a.widths[l] = p->mlp_widths[l];
-    for (int l = 0; l < a.nlay; ++l) {
-      a.woff[l] = off;
-      off += a.widths[l] * a.widths[l + 1];
-      a.boff[l] = off;
-      off += a.widths[l + 1];
-      if (l + 1 < a.nlay) a.wmax = a.wmax > a.widths[l + 1] ? a.wmax : a.widths[l + 1];
-    }
-    a.hmax = 1;
-    for (int l = 1; l < a.nlay; ++l) a.hmax = a.hmax > a.widths[l] ? a.hmax : a.widths[l];
-  }
+  fill_network(p, a);
   a.table = in->interaction_table;
   a.vext = in->external_potential;
   a.n0 = in->initial_density;
@@ -1921,6 +1943,37 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
 
   P->lds_pot = (size_t)(3 * a.N + 2 * NT + 3 * NWAVE) * 8;
   P->lds_eig = eigen_lds_bytes(a.N);
+  P->geig = dim3(a.K * a.B);
+  if (P->lds_eig > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_eig);
+  return plan_network(P);
+}
+
+// The network's fields of the device view (widths, packed offsets, widest layers), from p.
+void fill_network(const qex1d_problem* p, Dev1& a) {
+  if (a.xc != QEX1D_XC_LOCAL_MLP && a.xc != QEX1D_XC_GLOBAL_MLP) return;
+  a.nlay = p->mlp_layers;
+  a.act = p->mlp_activation;
+  a.norm = p->mlp_norm;
+  a.mlpout = p->mlp_output;
+  a.W = p->mlp_widths[0];
+  int off = 0;
+  a.wmax = a.W;
+  for (int l = 0; l <= a.nlay; ++l) a.widths[l] = p->mlp_widths[l];
+  for (int l = 0; l < a.nlay; ++l) {
+    a.woff[l] = off;
+    off += a.widths[l] * a.widths[l + 1];
+    a.boff[l] = off;
+    off += a.widths[l + 1];
+    if (l + 1 < a.nlay) a.wmax = a.wmax > a.widths[l + 1] ? a.wmax : a.widths[l + 1];
+  }
+  a.hmax = 1;
+  for (int l = 1; l < a.nlay; ++l) a.hmax = a.hmax > a.widths[l] ? a.hmax : a.widths[l];
+}
+
+// Launch shapes and LDS of the network kernels and of the potential kernel's grid.
+int plan_network(Plan1d* P) {
+  const Dev1& a = P->a;
   bool mlp_mfma = a.xc == QEX1D_XC_LOCAL_MLP && a.nlay - 1 <= MLP_LH && a.W <= MLP_S - 1;
   for (int l = 1; l < a.nlay && mlp_mfma; ++l) mlp_mfma = a.widths[l] <= MLP_HMAX;
   P->mlp_mfma = mlp_mfma;
@@ -1933,10 +1986,7 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
   if (P->lds_gmlp > LDS_MAX) return fail(QEX_E_UNSUPPORTED, "global MLP too wide for LDS (%zu bytes)", P->lds_gmlp);
   P->gmlp_mfma = dim3((unsigned)(((long)a.B * a.N + MP - 1) / MP));
   P->gpot = dim3(a.nTiles, a.B);
-  P->geig = dim3(a.K * a.B);
   P->gmlp = dim3((a.N + NT / P->mlp_lpp - 1) / (NT / P->mlp_lpp), a.B);
-  if (P->lds_eig > 64 * 1024)
-    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_eig);
   const void* mlp_fn = P->mlp_lpp == 8 ? (const void*)k_mlp<8> : (const void*)k_mlp<64>;
   if (mlp_mfma) {
     const void* fns[4] = {(const void*)k_mlp_mfma<1>, (const void*)k_mlp_mfma<2>, (const void*)k_mlp_mfma<3>,
@@ -1950,9 +2000,8 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
   return QEX_OK;
 }
 
-// One SCF iteration's launches (the XC network, potentials and deferred energies, then unless
-// energy_only the eigensolve and density/mixing); in slot mode every slot runs its own iteration.
-void launch_iteration(const Plan1d& P, int k, double alpha_k, bool energy_only, hipStream_t stream) {
+// The XC network's launch of iteration k (none for the closed-form functionals).
+void launch_network(const Plan1d& P, int k, hipStream_t stream) {
   const Dev1& a = P.a;
   if (a.xc == QEX1D_XC_LOCAL_MLP) {
     Scope ps(qexprof::K_MLP, stream);
@@ -1973,6 +2022,13 @@ void launch_iteration(const Plan1d& P, int k, double alpha_k, bool energy_only, 
     Scope ps(qexprof::K_MLP, stream);
     hipLaunchKernelGGL(k_gmlp, dim3(a.B), dim3(NT), P.lds_gmlp, stream, a, k);
   }
+}
+
+// One SCF iteration's launches (the XC network, potentials and deferred energies, then unless
+// energy_only the eigensolve and density/mixing); in slot mode every slot runs its own iteration.
+void launch_iteration(const Plan1d& P, int k, double alpha_k, bool energy_only, hipStream_t stream) {
+  const Dev1& a = P.a;
+  launch_network(P, k, stream);
   {
     Scope ps(qexprof::K_POTENTIAL, stream);
     hipLaunchKernelGGL(k_potential, P.gpot, dim3(NT), P.lds_pot, stream, a, k);
@@ -2153,6 +2209,99 @@ int qex1d_hartree_potential(int32_t batch, int32_t num_grids, double dx, const d
   hipLaunchKernelGGL(k_hartree, dim3((num_grids + NT - 1) / NT, batch), dim3(NT), (size_t)2 * num_grids * 8,
                      (hipStream_t)stream_, num_grids, dx, table, density, out);
   return check_launch("qex1d_hartree_potential");
+}
+
+}  // extern "C"
+
+namespace {
+// Workspace of qex1d_xc_evaluate: the per-system status words the network kernels read and the
+// network outputs (per-point values, input gradients, scalars).
+struct XcLayout {
+  size_t st, mlp_e, mlp_G, mlp_eout, total;
+};
+
+// p with the SCF-only fields set to accepted values: the evaluation reads the functional and sizes.
+qex1d_problem xc_problem(const qex1d_problem* p) {
+  qex1d_problem q = *p;
+  q.num_iterations = 1;
+  q.max_electrons = 2;
+  q.num_mixing_iterations = 1;
+  q.loop = QEX1D_LOOP_PYTHON;
+  q.reflect = 0;
+  return q;
+}
+
+XcLayout layout_xc(const qex1d_problem* p) {
+  const size_t B = p->batch, N = p->num_grids;
+  const bool mlp = p->xc_kind == QEX1D_XC_LOCAL_MLP || p->xc_kind == QEX1D_XC_GLOBAL_MLP;
+  const size_t W = p->xc_kind == QEX1D_XC_LOCAL_MLP ? p->mlp_widths[0] : 0;
+  XcLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  L.st = take(B * 4 * 4);
+  L.mlp_e = take(mlp ? B * N * 8 : 0);
+  L.mlp_G = take(mlp ? B * N * W * 8 : 0);
+  L.mlp_eout = take(mlp ? B * (N > 2 ? N : 2) * 8 : 0);
+  L.total = off;
+  return L;
+}
+}  // namespace
+
+extern "C" {
+
+size_t qex1d_xc_workspace_bytes(const qex1d_problem* p) {
+  if (!p) return 0;
+  const qex1d_problem q = xc_problem(p);
+  if (validate1d(&q) != QEX_OK) return 0;
+  return layout_xc(&q).total;
+}
+
+int qex1d_xc_evaluate(const qex1d_problem* p, const double* xc_params, const double* density, double* energy_density,
+                      double* xc_energy, double* xc_potential, void* workspace, size_t workspace_bytes,
+                      void* stream_) {
+  if (!p) return fail(QEX_E_BADARG, "null problem");
+  const qex1d_problem q = xc_problem(p);
+  int rc = validate1d(&q);
+  if (rc) return rc;
+  if (!density) return fail(QEX_E_BADARG, "null density");
+  const bool mlp = q.xc_kind == QEX1D_XC_LOCAL_MLP || q.xc_kind == QEX1D_XC_GLOBAL_MLP;
+  if (mlp && !xc_params) return fail(QEX_E_BADARG, "MLP functionals need xc_params");
+  const XcLayout L = layout_xc(&q);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(QEX_E_WORKSPACE, "workspace too small: need %zu bytes, got %zu", L.total, workspace_bytes);
+  char* ws = (char*)workspace;
+  Plan1d P{};
+  Dev1& a = P.a;
+  a.B = q.batch;
+  a.N = q.num_grids;
+  a.T = 1;
+  a.K = 2;
+  a.M = 1;
+  a.nTiles = (a.N + TILE - 1) / TILE;
+  a.xc = q.xc_kind;
+  a.dx = q.dx;
+  a.xc_c = q.xc_c;
+  a.xc_p = q.xc_p;
+  a.xc_eps = q.xc_eps;
+  fill_network(&q, a);
+  a.n0 = density;
+  a.mlp = xc_params;
+  a.st = (int*)(ws + L.st);
+  a.mlp_e = (double*)(ws + L.mlp_e);
+  a.mlp_G = (double*)(ws + L.mlp_G);
+  a.mlp_eout = (double*)(ws + L.mlp_eout);
+  rc = plan_network(&P);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  // status words: not frozen, no deferred energy, so the network kernels evaluate iteration 0 on n0
+  hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
+  launch_network(P, 0, stream);
+  hipLaunchKernelGGL(k_xc_eval, dim3(a.B), dim3(NT), 0, stream, a, energy_density, xc_potential, xc_energy);
+  return check_launch("qex1d_xc_evaluate");
 }
 
 }  // extern "C"

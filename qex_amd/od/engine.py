@@ -102,6 +102,41 @@ def hartree_potential(density, grids, interaction_fn) -> torch.Tensor:
     return out[0] if single else out
 
 
+def xc_evaluate(xc, density, dx: float):
+    """The functional ``xc`` on densities [N] or [B, N] with grid spacing ``dx`` through
+    ``qex1d_xc_evaluate``: returns (energy density, E_xc, v_xc).  Energy density: [.., N] for local
+    functionals, [..] for global (amplitude-encoded) ones; E_xc [..]; v_xc [.., N] (local:
+    (dE/dn)/dx, global: dE/dn)."""
+    dev = device()
+    f = xcmod.resolve(xc)
+    n = as_f64(density, dev)
+    single = n.dim() == 1
+    if single:
+        n = n[None]
+    if n.dim() != 2:
+        raise ValueError("density must be [N] or [B, N]")
+    B, N = n.shape
+    if N > _lib.QEX1D_MAX_GRIDS:
+        raise ValueError(f"num_grids {N} exceeds {_lib.QEX1D_MAX_GRIDS}")
+    p = _lib.Qex1dProblem()
+    p.batch, p.num_grids, p.dx = B, N, float(dx)
+    f.configure(p)
+    params = f.device_params(dev)
+    lib = _lib.load()
+    nbytes = lib.qex1d_xc_workspace_bytes(ctypes.byref(p))
+    if nbytes == 0:
+        _lib.check(-1, "qex1d_xc_workspace_bytes")
+    ws = workspace(nbytes, dev)
+    eps = torch.empty((B, N) if f.encoding == "local" else (B,), dtype=torch.float64, device=dev)
+    exc = torch.empty(B, dtype=torch.float64, device=dev)
+    vxc = torch.empty((B, N), dtype=torch.float64, device=dev)
+    _lib.check(lib.qex1d_xc_evaluate(ctypes.byref(p), _ptr(params), _ptr(n), _ptr(eps), _ptr(exc), _ptr(vxc),
+                                     _ptr(ws), ws.numel(), _stream()), "qex1d_xc_evaluate")
+    if single:
+        return eps[0], exc[0], vxc[0]
+    return eps, exc, vxc
+
+
 @dataclass
 class NonInteracting:
     density: torch.Tensor

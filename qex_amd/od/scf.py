@@ -7,7 +7,8 @@ Mirrors ``jax_dft.scf`` as the reference calls it (``od/eval.py:67-89``) and
 * ``kohn_sham_amplitude_encoded`` -- ``od/scf.py:434-560`` (global XC, E = dx sum f(n));
 * ``solve_noninteracting_system``, ``get_initial_density``, ``get_hartree_potential``,
   ``get_hartree_energy``, ``get_external_potential_energy``, ``get_xc_energy``,
-  ``get_xc_energy_amplitude_encoded``, ``KohnShamState``.
+  ``get_xc_potential``, ``get_xc_energy_amplitude_encoded``, ``get_xc_potential_amplitude_encoded``,
+  ``KohnShamState``.
 
 Semantics of the loop (``od/scf.py:517-560``): convergence is tested on the unmixed
 (reflected) output density of the same iteration; the stored density is the mixed one and
@@ -168,15 +169,60 @@ def get_external_potential_energy(external_potential, density, grids):
     return (engine.as_f64(external_potential, dev) * engine.as_f64(density, dev)).sum(-1) * get_dx(grids)
 
 
+def _xc(density, xc_energy_density_fn, grids, encoding):
+    """(E_xc, v_xc) of a qex functional through ``qex1d_xc_evaluate``, None for other callables.
+    A functional of the other encoding is summed as the reference's formula reads its values
+    (global: dx sum_i e_i; local: dx f sum_i n_i), without a potential (v_xc None)."""
+    try:
+        f = xcmod.resolve(xc_energy_density_fn)
+    except TypeError:
+        return None
+    dx = get_dx(grids)
+    if f.encoding != encoding and encoding == "global" and isinstance(f, xcmod.PowerLaw):
+        f = _global_functional(f)
+    if f.encoding == encoding:
+        _, exc, vxc = engine.xc_evaluate(f, density, dx)
+        return exc, vxc
+    eps, _, _ = engine.xc_evaluate(f, density, dx)
+    if encoding == "global":
+        return eps.sum(-1) * dx, None
+    n = engine.as_f64(density, engine.device())
+    return eps * n.sum(-1) * dx, None
+
+
 def get_xc_energy(density, xc_energy_density_fn, grids):
+    """dx * sum(e(n) n) (jax_dft ``scf.get_xc_energy``), on the device for qex functionals; any
+    other callable is called on the device density."""
+    r = _xc(density, xc_energy_density_fn, grids, "local")
+    if r is not None:
+        return r[0]
     n = engine.as_f64(density, engine.device())
     return (xc_energy_density_fn(n) * n).sum(-1) * get_dx(grids)
 
 
+def get_xc_potential(density, xc_energy_density_fn, grids):
+    """d E_xc / dn / dx (jax_dft ``scf.get_xc_potential``) for a qex functional, on the device."""
+    r = _xc(density, xc_energy_density_fn, grids, "local")
+    if r is None or r[1] is None:
+        raise TypeError("get_xc_potential needs a local qex_amd functional")
+    return r[1]
+
+
 def get_xc_energy_amplitude_encoded(density, xc_energy_density_fn, grids):
-    """dx * sum(f(n)) (``od/scf.py:32-50``)."""
+    """dx * sum(f(n)) (``od/scf.py:32-50``), on the device for qex functionals."""
+    r = _xc(density, xc_energy_density_fn, grids, "global")
+    if r is not None:
+        return r[0]
     n = engine.as_f64(density, engine.device())
     return torch.sum(xc_energy_density_fn(n)) * get_dx(grids)
+
+
+def get_xc_potential_amplitude_encoded(density, xc_energy_density_fn, grids):
+    """d/dn [dx * sum(f(n))] (``od/scf.py:53-76``) for a qex functional, on the device."""
+    r = _xc(density, xc_energy_density_fn, grids, "global")
+    if r is None or r[1] is None:
+        raise TypeError("get_xc_potential_amplitude_encoded needs a global qex_amd functional or PowerLaw")
+    return r[1]
 
 
 def get_total_eigen_energies(num_electrons, eigen_energies):

@@ -46,12 +46,18 @@ class XCFunctional:
         return self
 
     def energy_density(self, density: torch.Tensor) -> torch.Tensor:
+        """The functional written in torch: its definition, for host-side checks of the
+        parameters and wrappers.  Calling the functional evaluates it on the device instead."""
         return torch.zeros_like(density)
 
     def __call__(self, density, params=None):
+        """xc_energy_density_fn(density[, params]) on the device (``qex1d_xc_evaluate``, the
+        network kernels of the SCF): per point for local functionals, one value per system for
+        global ones; density [N] or [B, N]."""
+        from . import engine
+
         fn = self.with_params(params) if params is not None else self
-        t = density if isinstance(density, torch.Tensor) else torch.as_tensor(np.asarray(density, dtype=np.float64))
-        return fn.energy_density(t.to(torch.float64))
+        return engine.xc_evaluate(fn, density, 1.0)[0]
 
 
 class NoXC(XCFunctional):

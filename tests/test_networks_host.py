@@ -69,8 +69,11 @@ def test_build_xc_functional_wrappers_and_device_description():
     pairs = [q for q in params[0] if q != ()]
     ref = O.LocalMLP([w for w, _ in pairs], [b for _, b in pairs], "tanh", 1, 2.0,
                      output="negativity").energy_density(n)
-    np.testing.assert_allclose(fn(n, params).numpy(), ref, atol=1e-14)
-    assert float(fn(n, params).max()) <= 0.2785
+    # the torch definition of the wrapped functional (calling fn evaluates on the device, which
+    # tests/test_xc_gpu.py covers)
+    got = fn.with_params(params).energy_density(torch.as_tensor(n)).numpy()
+    np.testing.assert_allclose(got, ref, atol=1e-14)
+    assert float(got.max()) <= 0.2785
     # global network types are amplitude encoded (wrappers.py:180-193)
     gnet = N.GlobalMLP({"n_neurons": 8, "n_layers": 1, "wrap_self_interaction": False}).build_network(grids)
     ginit, gfn = N.build_xc_functional(gnet, grids, {"network_type": "mlp_ksr"})

@@ -53,6 +53,14 @@ extern "C" {
 #define QEX1D_XC_NONE 4         /* no XC (v_xc = 0) */
 #define QEX1D_XC_GLOBAL_MLP 5   /* MLP on the whole density (global, amplitude encoded):
                                    classical_models.py:177-224 build_global_mlp */
+/* Any other functional (an opaque xc_energy_density_fn), evaluated by the caller between
+ * iterations (qex1d_kohn_sham_step): xc_params is then an exchange buffer [3][B][N] the caller fills
+ * before each step: [0] the energy density at the input density of the iteration (GLOBAL: the
+ * scalar f at [0][b][0]), [1] the potential there (LOCAL: d(sum_j e_j n_j)/dn_i; GLOBAL:
+ * dE/dn_i), [2] the energy density at the pending output density (GLOBAL: scalar at [2][b][0]).
+ * The densities to evaluate at come from qex1d_external_xc_densities. */
+#define QEX1D_XC_EXTERNAL_LOCAL 6
+#define QEX1D_XC_EXTERNAL_GLOBAL 7
 
 /* Transform applied to the network output of LOCAL_MLP / GLOBAL_MLP (the jax_dft neural_xc layer
  * that qedft/models/wrappers.py:201-203 appends with wrap_with_negative_transform). */
@@ -102,7 +110,8 @@ typedef struct qex1d_inputs {
   const double* initial_density;    /* [B][N] */
   const int32_t* num_electrons;     /* [B]   1 <= N_e <= max_electrons */
   const int32_t* reflect_window;    /* [B][2] inclusive index window (lo, hi); used if reflect */
-  const double* xc_params;          /* LOCAL_MLP: W1[in][out], b1, W2, b2, ... packed, float64 */
+  const double* xc_params;          /* LOCAL_MLP: W1[in][out], b1, W2, b2, ... packed, float64;
+                                       EXTERNAL kinds: the exchange buffer [3][B][N] */
 } qex1d_inputs;
 
 typedef struct qex1d_outputs {
@@ -149,6 +158,25 @@ int qex1d_kohn_sham(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_
  * in the reference beyond that. */
 int qex1d_slots_step(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, int32_t* slot_iteration,
                      const int32_t* admit, void* workspace, size_t workspace_bytes, void* stream);
+
+/* qex1d_kohn_sham one pass at a time, for functionals the caller evaluates between passes (the
+ * EXTERNAL xc kinds; any kind is accepted).  Call with iteration = 0, 1, ..., T in order on one
+ * stream: pass k < T runs SCF iteration k, pass T only evaluates the last energies and finalises the
+ * outputs.  Pass 0 resets the state kept in the workspace.  The T + 1 passes give the results of
+ * qex1d_kohn_sham, bit for bit, when the functional's values are the same.  Replaces the python
+ * loop of jax_dft scf.kohn_sham (and qedft od/scf.py:517-560) with an opaque
+ * xc_energy_density_fn. */
+int qex1d_kohn_sham_step(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out,
+                         int32_t iteration, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The densities an EXTERNAL functional is evaluated at before pass `iteration` of
+ * qex1d_kohn_sham_step, [B][N] each: density_in, the input density of that iteration (flipped and
+ * averaged about the system's reflection window for LOCAL kinds when p->reflect), and
+ * density_out, the output density whose energy the pass completes (that of iteration - 1; a copy
+ * of density_in at iteration 0, where no energy is pending). */
+int qex1d_external_xc_densities(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out,
+                                int32_t iteration, void* workspace, size_t workspace_bytes, double* density_in,
+                                double* density_out, void* stream);
 
 /* Workspace bytes for qex1d_solve_noninteracting. */
 size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, int32_t max_electrons);

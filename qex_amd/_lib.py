@@ -30,6 +30,8 @@ QEX1D_XC_POWER_GLOBAL = 2
 QEX1D_XC_LOCAL_MLP = 3
 QEX1D_XC_NONE = 4
 QEX1D_XC_GLOBAL_MLP = 5
+QEX1D_XC_EXTERNAL_LOCAL = 6
+QEX1D_XC_EXTERNAL_GLOBAL = 7
 
 QEX1D_MLP_OUTPUT_LINEAR = 0
 QEX1D_MLP_OUTPUT_NEGATIVITY = 1
@@ -155,6 +157,10 @@ SIGNATURES = {
                              ctypes.POINTER(Qex1dOutputs), _P, _S, _P]),
     "qex1d_slots_step": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
                               ctypes.POINTER(Qex1dOutputs), _P, _P, _P, _S, _P]),
+    "qex1d_kohn_sham_step": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
+                                  ctypes.POINTER(Qex1dOutputs), _I, _P, _S, _P]),
+    "qex1d_external_xc_densities": (_I, [ctypes.POINTER(Qex1dProblem), ctypes.POINTER(Qex1dInputs),
+                                         ctypes.POINTER(Qex1dOutputs), _I, _P, _S, _P, _P, _P]),
     "qex1d_noninteracting_workspace_bytes": (_S, [_I, _I, _I]),
     "qex1d_solve_noninteracting": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _S, _P]),
     "qex1d_solve_noninteracting_noalloc": (_I, [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P]),

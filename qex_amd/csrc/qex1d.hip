@@ -268,7 +268,12 @@ struct Dev1 {
 // iteration of system s in this launch: the launch's, or the slot's own in slot mode
 __device__ __forceinline__ int iter_of(const Dev1& a, int s, int k_launch) { return a.kslot ? a.kslot[s] : k_launch; }
 
-__device__ __forceinline__ bool xc_is_local(int xc) { return xc != QEX1D_XC_POWER_GLOBAL && xc != QEX1D_XC_GLOBAL_MLP; }
+__device__ __host__ __forceinline__ bool xc_is_local(int xc) {
+  return xc != QEX1D_XC_POWER_GLOBAL && xc != QEX1D_XC_GLOBAL_MLP && xc != QEX1D_XC_EXTERNAL_GLOBAL;
+}
+__device__ __host__ __forceinline__ bool xc_is_external(int xc) {
+  return xc == QEX1D_XC_EXTERNAL_LOCAL || xc == QEX1D_XC_EXTERNAL_GLOBAL;
+}
 
 // transform of the network output (QEX1D_MLP_OUTPUT_*) and its derivative
 __device__ __forceinline__ double out_transform(int kind, double y, double* d) {
@@ -893,6 +898,25 @@ __global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k_launch) {
         // potential dx df/dn from k_gmlp; the scalar f(n_in) is carried by point 0
         v = a.mlp_e[(size_t)s * N + i];
         e = i == 0 ? a.mlp_eout[2 * s] : 0.0;
+      } else if (a.xc == QEX1D_XC_EXTERNAL_GLOBAL) {
+        // the caller's exchange buffer: [0] f(n_in) at point 0, [1] dE/dn
+        const size_t BN = (size_t)a.B * N;
+        v = a.mlp[BN + (size_t)s * N + i];
+        e = i == 0 ? a.mlp[(size_t)s * N] : 0.0;
+      } else if (a.xc == QEX1D_XC_EXTERNAL_LOCAL) {
+        // [0] e at the (flipped and averaged) input density, [1] d(sum_j e_j n_j)/dn_i there;
+        // symmetrised like the network's values
+        const size_t BN = (size_t)a.B * N;
+        const int im = sym ? mirror(i, lo, hi) : i;
+        const double* ex = a.mlp + (size_t)s * N;
+        const double* vx = a.mlp + BN + (size_t)s * N;
+        if (im != i) {
+          v = 0.5 * (vx[i] + vx[im]);
+          e = 0.5 * (ex[i] + ex[im]);
+        } else {
+          v = vx[i];
+          e = ex[i];
+        }
       } else if (a.xc == QEX1D_XC_LOCAL_MLP) {
         const int im = sym ? mirror(i, lo, hi) : i;
         const double vu = mlp_v_unsym(a, s, i);
@@ -922,6 +946,10 @@ __global__ __launch_bounds__(NT) void k_potential(Dev1 a, int k_launch) {
       eH = 0.5 * a.dx * a.dx * oraw[i] * KE;
       if (a.xc == QEX1D_XC_GLOBAL_MLP) {
         eX = i == 0 ? a.dx * a.mlp_eout[2 * s + 1] : 0.0;
+      } else if (a.xc == QEX1D_XC_EXTERNAL_GLOBAL) {
+        eX = i == 0 ? a.dx * a.mlp[2 * (size_t)a.B * N + (size_t)s * N] : 0.0;
+      } else if (a.xc == QEX1D_XC_EXTERNAL_LOCAL) {
+        eX = a.dx * a.mlp[2 * (size_t)a.B * N + (size_t)s * N + i] * ox;
       } else if (a.xc == QEX1D_XC_LOCAL_MLP) {
         eX = a.dx * a.mlp_eout[(size_t)s * N + i] * ox;
       } else if (a.xc != QEX1D_XC_NONE) {
@@ -1627,6 +1655,26 @@ __global__ __launch_bounds__(NT) void k_hartree(int N, double dx, const double* 
   out[(size_t)s * N + i] = dx * acc;
 }
 
+// k_xc_gather: the densities a caller-evaluated (EXTERNAL) functional needs before pass k: the
+// iteration's input density (flipped and averaged for LOCAL kinds under reflection, as the
+// network kernels read it) and the pending output density o_xc (the input density at k = 0).
+__global__ __launch_bounds__(NT) void k_xc_gather(Dev1 a, int k, double* din, double* dout) {
+  const int s = blockIdx.x, N = a.N;
+  const double* nin = input_density(a, s, k);
+  const bool sym = a.reflect && xc_is_local(a.xc);
+  int lo = 0, hi = N - 1;
+  if (sym) {
+    lo = a.win[2 * s];
+    hi = a.win[2 * s + 1];
+  }
+  for (int t = threadIdx.x; t < N; t += NT) {
+    double x = nin[t];
+    if (sym) x = 0.5 * (x + nin[mirror(t, lo, hi)]);
+    din[(size_t)s * N + t] = x;
+    dout[(size_t)s * N + t] = k == 0 ? x : a.o_xc[(size_t)s * N + t];
+  }
+}
+
 // k_xc_eval: the XC functional on given densities (a.n0), one workgroup per system, after the
 // network kernel has run on them as iteration 0 of a one-iteration batch (its outputs in mlp_e,
 // mlp_G, mlp_eout).  LOCAL kinds: e_i and v_i = e_i + sum_j n_j de_j/dn_i per point (the
@@ -1721,7 +1769,7 @@ int validate1d(const qex1d_problem* p) {
   if (p->max_electrons / 2 + 1 > p->num_grids) return fail(QEX_E_BADARG, "too many electrons for the grid");
   if (p->num_mixing_iterations < 1) return fail(QEX_E_BADARG, "num_mixing_iterations must be >= 1");
   if (p->loop != QEX1D_LOOP_PYTHON && p->loop != QEX1D_LOOP_SCAN) return fail(QEX_E_BADARG, "bad loop kind");
-  if (p->xc_kind < 0 || p->xc_kind > QEX1D_XC_GLOBAL_MLP) return fail(QEX_E_BADARG, "bad xc_kind");
+  if (p->xc_kind < 0 || p->xc_kind > QEX1D_XC_EXTERNAL_GLOBAL) return fail(QEX_E_BADARG, "bad xc_kind");
   if ((p->xc_kind == QEX1D_XC_GLOBAL_MLP || p->xc_kind == QEX1D_XC_LOCAL_MLP) &&
       p->mlp_output != QEX1D_MLP_OUTPUT_LINEAR && p->mlp_output != QEX1D_MLP_OUTPUT_NEGATIVITY)
     return fail(QEX_E_BADARG, "bad mlp_output");
@@ -1865,6 +1913,8 @@ int plan1d(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* 
   if (p->reflect && !in->reflect_window) return fail(QEX_E_BADARG, "reflect needs reflect_window");
   if ((p->xc_kind == QEX1D_XC_LOCAL_MLP || p->xc_kind == QEX1D_XC_GLOBAL_MLP) && !in->xc_params)
     return fail(QEX_E_BADARG, "MLP functionals need xc_params");
+  if (xc_is_external(p->xc_kind) && !in->xc_params)
+    return fail(QEX_E_BADARG, "EXTERNAL functionals need the exchange buffer in xc_params");
   if (!out->density || !out->total_energy || !out->hartree_potential || !out->xc_potential ||
       !out->xc_energy_density || !out->gap || !out->converged)
     return fail(QEX_E_BADARG, "missing output array");
@@ -2090,6 +2140,42 @@ int qex1d_slots_step(const qex1d_problem* p, const qex1d_inputs* in, const qex1d
   return check_launch("qex1d_slots_step");
 }
 
+int qex1d_kohn_sham_step(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out, int32_t iteration,
+                         void* workspace, size_t workspace_bytes, void* stream_) {
+  Plan1d P;
+  const int rc = plan1d(p, in, out, workspace, workspace_bytes, &P);
+  if (rc) return rc;
+  const Dev1& a = P.a;
+  if (iteration < 0 || iteration > a.T) return fail(QEX_E_BADARG, "iteration must be in [0, %d]", a.T);
+  hipStream_t stream = (hipStream_t)stream_;
+  if (iteration == 0) {
+    hipLaunchKernelGGL(k_init, dim3((a.B + 255) / 256), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_pairs, dim3(1), dim3(PAIR_NT), 0, stream, a.B, a.K, a.ne, P.e.pairs, nullptr, a.T);
+  }
+  double alpha_k = p->alpha;  // the product qex1d_kohn_sham's loop forms
+  for (int i = 0; i < iteration; ++i) alpha_k *= p->alpha_decay;
+  launch_iteration(P, iteration, alpha_k, iteration == a.T, stream);
+  if (iteration == a.T) {
+    Scope ps(qexprof::K_FINALIZE, stream);
+    hipLaunchKernelGGL(k_finalize, dim3(a.B), dim3(64), 0, stream, a);
+  }
+  return check_launch("qex1d_kohn_sham_step");
+}
+
+int qex1d_external_xc_densities(const qex1d_problem* p, const qex1d_inputs* in, const qex1d_outputs* out,
+                                int32_t iteration, void* workspace, size_t workspace_bytes, double* density_in,
+                                double* density_out, void* stream_) {
+  Plan1d P;
+  const int rc = plan1d(p, in, out, workspace, workspace_bytes, &P);
+  if (rc) return rc;
+  const Dev1& a = P.a;
+  if (iteration < 0 || iteration > a.T) return fail(QEX_E_BADARG, "iteration must be in [0, %d]", a.T);
+  if (!density_in || !density_out) return fail(QEX_E_BADARG, "null density_in or density_out");
+  hipLaunchKernelGGL(k_xc_gather, dim3(a.B), dim3(NT), 0, (hipStream_t)stream_, a, (int)iteration, density_in,
+                     density_out);
+  return check_launch("qex1d_external_xc_densities");
+}
+
 size_t qex1d_noninteracting_workspace_bytes(int32_t batch, int32_t num_grids, int32_t max_electrons) {
   if (batch < 1 || num_grids < 5 || num_grids > QEX1D_MAX_GRIDS || max_electrons < 1 ||
       max_electrons > QEX1D_MAX_ELECTRONS)
@@ -2268,6 +2354,7 @@ int qex1d_xc_evaluate(const qex1d_problem* p, const double* xc_params, const dou
   int rc = validate1d(&q);
   if (rc) return rc;
   if (!density) return fail(QEX_E_BADARG, "null density");
+  if (xc_is_external(q.xc_kind)) return fail(QEX_E_UNSUPPORTED, "EXTERNAL functionals are evaluated by the caller");
   const bool mlp = q.xc_kind == QEX1D_XC_LOCAL_MLP || q.xc_kind == QEX1D_XC_GLOBAL_MLP;
   if (mlp && !xc_params) return fail(QEX_E_BADARG, "MLP functionals need xc_params");
   const XcLayout L = layout_xc(&q);

@@ -285,15 +285,53 @@ class Problem1D:
             raise ValueError(f"initial_density must have shape {(self.B, self.N)}, got {tuple(n0.shape)}")
         n0 = n0.contiguous()
         out = out or self.allocate_outputs()
+        external = xcmod.is_external(self.xc)
+        exch = torch.zeros((3, self.B, self.N), dtype=torch.float64, device=self.dev) if external else None
+        ins = _lib.Qex1dInputs(_ptr(self.table), _ptr(self.vext), _ptr(n0), _ptr(self.ne_t), _ptr(self.win),
+                               _ptr(exch if external else self.xc_params))
+        outs = _lib.Qex1dOutputs(_ptr(out.density), _ptr(out.total_energy), _ptr(out.hartree_potential),
+                                 _ptr(out.xc_potential), _ptr(out.xc_energy_density), _ptr(out.gap),
+                                 _ptr(out.converged), _ptr(out.eigenvalues), _ptr(out.stats), _ptr(out.orbitals))
+        self._keep = (n0, ins, outs, exch)  # keep alive until the stream has consumed them
+        self._last_n0 = n0
+        lib = _lib.load()
+        if not external:
+            _lib.check(lib.qex1d_kohn_sham(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs),
+                                           _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham")
+            return out
+        # an opaque functional: one device pass per iteration, the callable evaluated between passes
+        # on the densities the device hands over (qex1d_external_xc_densities)
+        d_in = torch.empty((self.B, self.N), dtype=torch.float64, device=self.dev)
+        d_out = torch.empty_like(d_in)
+        dx = float(self.problem.dx)
+        for k in range(self.T + 1):
+            _lib.check(lib.qex1d_external_xc_densities(ctypes.byref(self.problem), ctypes.byref(ins),
+                                                       ctypes.byref(outs), k, _ptr(self.ws), self.ws.numel(),
+                                                       _ptr(d_in), _ptr(d_out), _stream()),
+                       "qex1d_external_xc_densities")
+            exch.copy_(self.xc.exchange(d_in, d_out, dx))
+            _lib.check(lib.qex1d_kohn_sham_step(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs), k,
+                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham_step")
+        return out
+
+    def run_stepwise(self, initial_density, out: BatchResult | None = None) -> BatchResult:
+        """``run`` as T + 1 calls of ``qex1d_kohn_sham_step`` (built-in functionals): the same
+        results as the one-call loop, bit for bit."""
+        n0 = as_f64(initial_density, self.dev)
+        if n0.dim() == 1:
+            n0 = n0[None]
+        n0 = n0.contiguous()
+        out = out or self.allocate_outputs()
         ins = _lib.Qex1dInputs(_ptr(self.table), _ptr(self.vext), _ptr(n0), _ptr(self.ne_t), _ptr(self.win),
                                _ptr(self.xc_params))
         outs = _lib.Qex1dOutputs(_ptr(out.density), _ptr(out.total_energy), _ptr(out.hartree_potential),
                                  _ptr(out.xc_potential), _ptr(out.xc_energy_density), _ptr(out.gap),
                                  _ptr(out.converged), _ptr(out.eigenvalues), _ptr(out.stats), _ptr(out.orbitals))
-        self._keep = (n0, ins, outs)  # keep alive until the stream has consumed them
-        self._last_n0 = n0
-        _lib.check(_lib.load().qex1d_kohn_sham(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs),
-                                               _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham")
+        self._keep = (n0, ins, outs)
+        lib = _lib.load()
+        for k in range(self.T + 1):
+            _lib.check(lib.qex1d_kohn_sham_step(ctypes.byref(self.problem), ctypes.byref(ins), ctypes.byref(outs), k,
+                                                _ptr(self.ws), self.ws.numel(), _stream()), "qex1d_kohn_sham_step")
         return out
 
 

@@ -49,7 +49,10 @@ class KohnShamState(NamedTuple):
 
 
 def _global_functional(fn):
-    f = xcmod.resolve(fn)
+    try:
+        f = xcmod.resolve(fn)
+    except TypeError:  # an opaque callable: evaluated between device passes
+        return xcmod.ExternalXC(fn, "global")
     if f.encoding == "global":
         return f
     if isinstance(f, xcmod.PowerLaw):
@@ -58,7 +61,10 @@ def _global_functional(fn):
 
 
 def _local_functional(fn):
-    f = xcmod.resolve(fn)
+    try:
+        f = xcmod.resolve(fn)
+    except TypeError:  # an opaque callable: evaluated between device passes
+        return xcmod.ExternalXC(fn, "local")
     if f.encoding != "local":
         raise ValueError("kohn_sham needs a local xc_energy_density_fn; use kohn_sham_amplitude_encoded")
     return f

@@ -3,7 +3,10 @@
 Each functional is a callable ``fn(density, params=None)`` (the ``xc_energy_density_fn``
 convention of ``qedft/models/wrappers.py:85-113`` and ``jax_dft.xc``) and also describes
 itself to the kernels (``xc_kind`` and device parameters).  Calling it evaluates the energy
-density with torch, for inspection; inside the SCF loop only the HIP kernels run.
+density on the device (``qex1d_xc_evaluate``); ``energy_density`` is the same functional written
+in torch, for host-side checks.  Inside the SCF loop only the HIP kernels run.  Any other
+callable runs through ``ExternalXC``: the SCF then goes one device pass at a time and evaluates
+the callable (torch, with autograd for the potential) between passes.
 
 * ``LDAExp``: jax_dft ``xc.get_lda_xc_energy_density_fn()`` for the exponential
   interaction (``od/eval.py:78-81``).
@@ -249,6 +252,61 @@ class GlobalMLPXC(LocalMLPXC):
             if li < len(self.params) - 1:
                 x = act(x)
         return _output(self.output, x[..., 0])
+
+
+class ExternalXC(XCFunctional):
+    """An opaque ``xc_energy_density_fn(density)`` written in torch: the SCF runs one pass at a time
+    (``qex1d_kohn_sham_step``) and evaluates it between passes on the densities the device hands
+    over, with torch autograd for the potential.  Local: fn(n) is the energy density per point;
+    global (amplitude encoded): the values of fn(n) are summed to f (``od/scf.py:32-76``)."""
+
+    def __init__(self, fn, encoding="local"):
+        if encoding not in ("local", "global"):
+            raise ValueError("encoding must be 'local' or 'global'")
+        self.fn, self.encoding = fn, encoding
+        self.xc_kind = _lib.QEX1D_XC_EXTERNAL_LOCAL if encoding == "local" else _lib.QEX1D_XC_EXTERNAL_GLOBAL
+
+    def energy_density(self, n):
+        return self.fn(n)
+
+    def __call__(self, density, params=None):
+        if params is not None:
+            return self.fn(density, params=params)
+        return self.fn(density)
+
+    def _values(self, n):
+        """Per system: e [N] (local) or the summed scalar f (global)."""
+        out = [torch.as_tensor(self.fn(n[b]), dtype=torch.float64, device=n.device) for b in range(n.shape[0])]
+        if self.encoding == "local":
+            return torch.stack([o.reshape(n.shape[1]) for o in out])
+        return torch.stack([o.sum() for o in out])
+
+    def exchange(self, n_in, n_out, dx):
+        """The exchange buffer [3, B, N] of the EXTERNAL kinds: values at n_in, the potential at
+        n_in (local: d(sum_j e_j n_j)/dn_i; global: d(dx f)/dn_i), values at n_out."""
+        B, N = n_in.shape
+        buf = torch.zeros((3, B, N), dtype=torch.float64, device=n_in.device)
+        x = n_in.detach().clone().requires_grad_(True)
+        with torch.enable_grad():
+            e = self._values(x)
+            total = (e * x).sum() if self.encoding == "local" else dx * e.sum()
+            if not total.requires_grad:
+                raise TypeError("xc_energy_density_fn must be a differentiable torch function of the density")
+            (v,) = torch.autograd.grad(total, x)
+        with torch.no_grad():
+            eo = self._values(n_out.detach())
+        if self.encoding == "local":
+            buf[0] = e.detach()
+            buf[2] = eo
+        else:
+            buf[0, :, 0] = e.detach()
+            buf[2, :, 0] = eo
+        buf[1] = v
+        return buf
+
+
+def is_external(f) -> bool:
+    return isinstance(f, ExternalXC)
 
 
 def get_lda_xc_energy_density_fn():

@@ -252,8 +252,9 @@ int32_t qex_abi_version(void);
 
 /* Reverse pass of qex1d_kohn_sham for the scan loop with a LOCAL_MLP functional (1 or 2 hidden
  * layers, widths <= 64, either output transform), with or without reflection symmetry (the flip
- * inside reflect_window), density_mse_converge_tolerance < 0 and the problem's stop_gradient_iterations:
- * given the forward's
+ * inside reflect_window), or a GLOBAL_MLP functional (the same widths; the amplitude-encoded loop
+ * of qedft od/jit_scf.py:15-78, 247-330), density_mse_converge_tolerance < 0 and the
+ * problem's stop_gradient_iterations: given the forward's
  * inputs, its outputs density / hartree_potential / xc_potential / orbitals, and cotangents of the
  * stored energies d_total_energy [B][T] and densities d_density [B][T][N] (either may be NULL), writes
  * d_xc_params [P] = the gradient with respect to the packed MLP parameters, summed over the batch.

@@ -1,6 +1,7 @@
 """Reverse-mode gradient of a loss on the 1D scan-semantics Kohn-Sham trajectory with respect to the
-parameters of a windowed LocalMLP XC functional (test infrastructure only: the FP64 reference that
-the device adjoint is checked against).
+parameters of a windowed LocalMLP XC functional, or of a GlobalMLP one (amplitude encoded: F = f,
+v_xc = dx grad f; under reflection only the density passed on is flipped, not the functional's
+inputs) (test infrastructure only: the FP64 reference that the device adjoint is checked against).
 
 What the reference differentiates (``qedft/train/od/train.py:58-126, 250-323``): ``jax.grad`` of a
 loss built from the states of ``jit_scf.kohn_sham`` (``od/jit_scf.py:143-244``), unrolled over the
@@ -143,6 +144,10 @@ class LocalMLPAdjoint:
                                               np.zeros(len(n)), getattr(self.xc, "output", "linear"))
         return float(f @ n), f + self._scatter(Xbar), list(zip(dW, db))
 
+    def potential(self, n):
+        """v_xc = grad_n F."""
+        return self.energy_and_grads(n)[1]
+
     def vxc_vjp(self, n, vbar):
         """(nbar, thetabar) of vbar . v_xc(n): H_F vbar and d(vbar . grad F)/dtheta."""
         X = self.xc._windows(n)
@@ -150,6 +155,35 @@ class LocalMLPAdjoint:
         f, fd, Xbar, dW, db = mlp_second_order(self.xc.weights, self.xc.biases, self.xc.activation, X, Xd, vbar, n,
                                                getattr(self.xc, "output", "linear"))
         return fd + self._scatter(Xbar), list(zip(dW, db))
+
+
+class GlobalMLPAdjoint:
+    """The same pieces for ``od.GlobalMLP`` (amplitude encoded, ``od/scf.py:32-76``): E_xc = dx f(n),
+    v_xc = dx grad_n f, so F = f in the energy and the potential carries the factor dx."""
+
+    def __init__(self, xc, dx):
+        self.xc, self.dx = xc, dx
+
+    def _x(self, n):
+        return (np.asarray(n, dtype=np.float64) / self.xc.norm)[None, :]
+
+    def energy_and_grads(self, n):
+        """f, grad_n f, df/dtheta."""
+        X = self._x(n)
+        f, _, Xbar, dW, db = mlp_second_order(self.xc.weights, self.xc.biases, self.xc.activation, X,
+                                              np.zeros_like(X), np.ones(1), np.zeros(1), self.xc.output)
+        return float(f[0]), Xbar[0] / self.xc.norm, list(zip(dW, db))
+
+    def potential(self, n):
+        return self.dx * self.energy_and_grads(n)[1]
+
+    def vxc_vjp(self, n, vbar):
+        """(nbar, thetabar) of vbar . v_xc(n) = dx grad f . vbar: dx H_f vbar and its theta-gradient."""
+        X = self._x(n)
+        Xd = self._x(vbar)
+        _, _, Xbar, dW, db = mlp_second_order(self.xc.weights, self.xc.biases, self.xc.activation, X, Xd,
+                                              np.zeros(1), np.full(1, self.dx), self.xc.output)
+        return Xbar[0] / self.xc.norm, list(zip(dW, db))
 
 
 def flatten(params):
@@ -191,7 +225,11 @@ def forward_trajectory(locations, charges, num_electrons, num_iterations, grids,
                                          interaction_fn)
     Wc = O.connection_weights(num_iterations, num_mixing_iterations)
     T = O.get_kinetic_matrix(grids)
-    adj = LocalMLPAdjoint(xc)
+    glob = isinstance(xc, O.GlobalMLP)
+    adj = GlobalMLPAdjoint(xc, dx) if glob else LocalMLPAdjoint(xc)
+    # the XC functional sees flipped densities only when local (od/jit_scf.py:15-78 applies no flip to
+    # the amplitude-encoded functional: its potential is at n, its energy at the raw output)
+    Sx = (lambda a: a) if glob else S
     K = (num_electrons + 1) // 2
     occ = occupations(num_electrons)
     n = np.asarray(initial_density, dtype=np.float64).copy()
@@ -200,14 +238,13 @@ def forward_trajectory(locations, charges, num_electrons, num_iterations, grids,
     a = alpha
     for k in range(num_iterations):
         vh = O.get_hartree_potential(n, grids, interaction_fn)
-        _, vxc, _ = adj.energy_and_grads(S(n))
-        vxc = S(vxc)
+        vxc = Sx(adj.potential(Sx(n)))
         vks = v_ext + vh + vxc
         eps, U = np.linalg.eigh(T + np.diag(vks))
         psi = U[:, :K].T  # (K, N), unit 2-norm
         n_raw = (occ[:, None] * psi ** 2).sum(0) / dx
         n_out = S(n_raw)
-        Fo, vxc_out, _ = adj.energy_and_grads(n_out)
+        Fo, vxc_out, _ = adj.energy_and_grads(Sx(n_raw))
         E = (float(occ @ eps[:K]) - dx * vks @ n_raw + O.get_hartree_energy(n_raw, grids, interaction_fn)
              + dx * Fo + dx * v_ext @ n_raw)
         diffs[k] = n_out - n
@@ -215,7 +252,7 @@ def forward_trajectory(locations, charges, num_electrons, num_iterations, grids,
         rec.append(dict(n=n, vks=vks, eps=eps, U=U, n_raw=n_raw, n_out=n_out, E=E, alpha=a, mixed=mixed))
         n = mixed
         a *= alpha_decay
-    return dict(rec=rec, v_ext=v_ext, Wc=Wc, dx=dx, occ=occ, K=K, adj=adj, grids=grids, S=S,
+    return dict(rec=rec, v_ext=v_ext, Wc=Wc, dx=dx, occ=occ, K=K, adj=adj, grids=grids, S=S, Sx=Sx,
                 interaction_fn=interaction_fn,
                 energies=np.array([r["E"] for r in rec]), densities=np.stack([r["mixed"] for r in rec]))
 
@@ -245,7 +282,7 @@ def backward(fw, dE, dn, stop_gradient_step=-1):
     stop_gradient_step: the states of iterations k <= it are constants (jax.lax.stop_gradient on the
     scan state in jax_dft jit_scf.kohn_sham): neither the loss's cotangents on them nor those flowing
     back from later iterations reach them; the differences they contribute to later mixes still do."""
-    rec, dx, adj, S = fw["rec"], fw["dx"], fw["adj"], fw["S"]
+    rec, dx, adj, S, Sx = fw["rec"], fw["dx"], fw["adj"], fw["S"], fw["Sx"]
     grids, ifn, v_ext, Wc = fw["grids"], fw["interaction_fn"], fw["v_ext"], fw["Wc"]
     Tn = len(rec)
     N = len(grids)
@@ -261,14 +298,14 @@ def backward(fw, dE, dn, stop_gradient_step=-1):
         nraw_bar = S(dbar[k])  # cotangent of n_raw (n_out = S n_raw)
         nbar -= dbar[k]
         if dE[k] != 0.0 and k > stop_gradient_step:
-            _, vxc_out, dth = adj.energy_and_grads(r["n_out"])
+            _, vxc_out, dth = adj.energy_and_grads(Sx(r["n_raw"]))
             vh_out = O.get_hartree_potential(r["n_raw"], grids, ifn)
-            nraw_bar += dE[k] * dx * (-r["vks"] + vh_out + S(vxc_out) + v_ext)
+            nraw_bar += dE[k] * dx * (-r["vks"] + vh_out + Sx(vxc_out) + v_ext)
             theta += dE[k] * dx * flatten(dth)
         vbar = eig_adjoint(r["eps"], r["U"], fw["occ"], nraw_bar, dx)
         nbar += O.get_hartree_potential(vbar, grids, ifn)
-        nb2, dth2 = adj.vxc_vjp(S(r["n"]), S(vbar))
-        nbar += S(nb2)
+        nb2, dth2 = adj.vxc_vjp(Sx(r["n"]), Sx(vbar))
+        nbar += Sx(nb2)
         theta += flatten(dth2)
         nbar_next = nbar
     return theta

@@ -16,6 +16,10 @@
 // wavefront per point with one lane per hidden unit; each wavefront accumulates its parameter
 // partials in registers, the waves of a workgroup combine them in LDS in a fixed order and add the
 // sum to the workgroup's own slot, and the slots are summed in a fixed order.
+// A GLOBAL_MLP functional (amplitude encoded: E_xc = dx f(n), v_xc = dx grad f) takes the same
+// steps with F = f in the energy and F = dx f in the potential, through k_adj_gmlp (one workgroup
+// per system); under reflection its functional sees the unflipped densities (the forward flips only
+// the density passed on), so only the mixing cotangents are flipped.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -428,6 +432,142 @@ __global__ __launch_bounds__(NT, 1) void k_adj_mlp(Adj a, const double* xin, con
   for (int t = threadIdx.x; t < a.P; t += NT) g[t] += r[t];
 }
 
+// Global-network second-order adjoint (GLOBAL_MLP: f(n) = g(MLP(n / norm)), one scalar per
+// system).  Per system s: x = xin[s] / norm, tangent xt = tin[s] / norm (zero without tin),
+// Phi = sf[s] f + fd fdot (sf null: 0; fdot = grad f . tin).  dst[s][i] += dPhi/dn_i (+ A2[s][i]);
+// the parameter gradient of Phi is added to slot blockIdx.x.  A workgroup takes systems blockIdx.x,
+// blockIdx.x + gridDim.x, ... in order, so each slot is written by one workgroup in a fixed order.
+// Hidden widths <= 64: unit u of a layer is lane u of a wave; the input layer's sums over the N
+// grid points are split over the 4 waves and combined in a fixed order.
+template <int NL>
+__global__ __launch_bounds__(NT) void k_adj_gmlp(Adj a, const double* xin, const double* tin, const double* sf,
+                                                 double fd, const double* A2, double* dst) {
+  extern __shared__ double sm[];
+  constexpr int NW = NT / 64;
+  const int N = a.N, H1 = a.H1, H2 = a.H2, HL = NL == 2 ? H2 : H1;
+  const int t = threadIdx.x, u = t & 63, q = t >> 6;
+  double* x = sm;                 // N
+  double* xd = x + N;             // N
+  double* part = xd + N;          // 2 x NW x 64
+  double* h1 = part + 2 * NW * 64;  // activations, tangents, act', act'' and the two adjoints, 6 x 64
+  double* h2 = h1 + 6 * 64;         // the same for layer 2
+  double* sc = h2 + 6 * 64;         // scalars: fbar', fdbar'
+  const double* W1 = a.mlp + a.w1;
+  const double* W2 = a.mlp + a.w2;
+  const double* W3 = a.mlp + a.w3;
+  const double in = 1.0 / a.norm;
+  double* g = a.thp + (size_t)blockIdx.x * a.P;
+  for (int s = blockIdx.x; s < a.B; s += gridDim.x) {
+    for (int i = t; i < N; i += NT) {
+      x[i] = xin[(size_t)s * N + i] * in;
+      xd[i] = tin ? tin[(size_t)s * N + i] * in : 0.0;
+    }
+    __syncthreads();
+    // layer 1: z = b1 + W1^T x, zd = W1^T xd, the grid points split over the waves
+    {
+      double acc = 0.0, accd = 0.0;
+      if (u < H1)
+        for (int i = q; i < N; i += NW) {
+          const double w = W1[(size_t)i * H1 + u];
+          acc = fma(x[i], w, acc);
+          accd = fma(xd[i], w, accd);
+        }
+      part[q * 64 + u] = acc;
+      part[(NW + q) * 64 + u] = accd;
+    }
+    __syncthreads();
+    if (t < H1) {
+      double z = a.mlp[a.b1 + t], zd = 0.0;
+      for (int w = 0; w < NW; ++w) {
+        z += part[w * 64 + t];
+        zd += part[(NW + w) * 64 + t];
+      }
+      double av, d1, d2;
+      act3(a.act, z, &av, &d1, &d2);
+      h1[t] = av;
+      h1[64 + t] = d1 * zd;
+      h1[128 + t] = d1;
+      h1[192 + t] = d2;
+      h1[256 + t] = zd;
+    }
+    __syncthreads();
+    if (NL == 2 && t < H2) {
+      double z = a.mlp[a.b2 + t], zd = 0.0;
+      for (int v = 0; v < H1; ++v) {
+        const double w = W2[v * H2 + t];
+        z = fma(h1[v], w, z);
+        zd = fma(h1[64 + v], w, zd);
+      }
+      double av, d1, d2;
+      act3(a.act, z, &av, &d1, &d2);
+      h2[t] = av;
+      h2[64 + t] = d1 * zd;
+      h2[128 + t] = d1;
+      h2[192 + t] = d2;
+      h2[256 + t] = zd;
+    }
+    __syncthreads();
+    const double* hl = NL == 2 ? h2 : h1;
+    if (t == 0) {
+      double y = a.mlp[a.b3], yd = 0.0;
+      for (int v = 0; v < HL; ++v) {
+        y = fma(hl[v], W3[v], y);
+        yd = fma(hl[64 + v], W3[v], yd);
+      }
+      double g1, g2;
+      (void)out3(a.mlpout, y, &g1, &g2);
+      const double fb = sf ? sf[s] : 0.0;
+      sc[0] = fb * g1 + fd * g2 * yd;  // cotangent of y
+      sc[1] = fd * g1;                 // cotangent of yd
+      g[a.b3] += sc[0];
+    }
+    __syncthreads();
+    const double yb = sc[0], ydb = sc[1];
+    // last hidden layer: its adjoints (zb, zdb) from the output layer
+    double* hL = NL == 2 ? h2 : h1;
+    if (t < HL) {
+      const double w = W3[t];
+      g[a.w3 + t] += hL[t] * yb + hL[64 + t] * ydb;
+      const double hb = yb * w, hdb = ydb * w;
+      hL[320 + t] = hb * hL[128 + t] + hdb * hL[192 + t] * hL[256 + t];  // zb
+      hL[256 + t] = hdb * hL[128 + t];                                 // zdb (zd no longer needed)
+    }
+    __syncthreads();
+    if (NL == 2) {
+      // W2 (H1 x H2): lane u = row, waves over the columns
+      if (u < H1)
+        for (int v = q; v < H2; v += NW)
+          g[a.w2 + u * H2 + v] += h1[u] * h2[320 + v] + h1[64 + u] * h2[256 + v];
+      if (t < H2) g[a.b2 + t] += h2[320 + t];
+      __syncthreads();
+      if (t < H1) {
+        double hb = 0.0, hdb = 0.0;
+        for (int v = 0; v < H2; ++v) {
+          const double w = W2[t * H2 + v];
+          hb = fma(w, h2[320 + v], hb);
+          hdb = fma(w, h2[256 + v], hdb);
+        }
+        const double zb = hb * h1[128 + t] + hdb * h1[192 + t] * h1[256 + t];
+        h1[256 + t] = hdb * h1[128 + t];
+        h1[320 + t] = zb;
+      }
+      __syncthreads();
+    }
+    // layer 1: W1 (N x H1), b1, and the input cotangent
+    if (u < H1)
+      for (int i = q; i < N; i += NW) g[a.w1 + (size_t)i * H1 + u] += x[i] * h1[320 + u] + xd[i] * h1[256 + u];
+    if (t < H1) g[a.b1 + t] += h1[320 + t];
+    for (int i = t; i < N; i += NT) {
+      double xb = 0.0;
+      for (int v = 0; v < H1; ++v) xb = fma(W1[(size_t)i * H1 + v], h1[320 + v], xb);
+      double r = xb * in;
+      if (A2) r += A2[(size_t)s * N + i];
+      dst[(size_t)s * N + i] += r;
+    }
+    __syncthreads();
+  }
+}
+
 // dst[s][i] += sa[s] * (A1[s][i] + F[s][i]) + A2[s][i] + window^T(xbar)_i / norm (A1, A2 optional)
 __global__ void k_adj_gather(Adj a, double* dst, const double* A1, const double* A2, const double* F,
                              const double* sa) {
@@ -668,11 +808,16 @@ int adj_check(const qex1d_problem* p) {
   if (p->loop != QEX1D_LOOP_SCAN && p->loop != QEX1D_LOOP_PYTHON) return faila(QEX_E_BADARG, "bad loop kind");
   if (!(p->density_mse_converge_tolerance < 0.0))
     return faila(QEX_E_UNSUPPORTED, "the reverse pass needs density_mse_converge_tolerance < 0 (no frozen states)");
-  if (p->xc_kind != QEX1D_XC_LOCAL_MLP) return faila(QEX_E_UNSUPPORTED, "the reverse pass needs a LOCAL_MLP functional");
+  const bool glob = p->xc_kind == QEX1D_XC_GLOBAL_MLP;
+  if (p->xc_kind != QEX1D_XC_LOCAL_MLP && !glob)
+    return faila(QEX_E_UNSUPPORTED, "the reverse pass needs a LOCAL_MLP or GLOBAL_MLP functional");
   if (p->mlp_layers != 2 && p->mlp_layers != 3)
     return faila(QEX_E_UNSUPPORTED, "the reverse pass supports 1 or 2 hidden layers");
-  if (p->mlp_widths[0] < 1 || p->mlp_widths[0] > WMAXW || p->mlp_widths[0] % 2 == 0)
+  if (glob) {
+    if (p->mlp_widths[0] != p->num_grids) return faila(QEX_E_BADARG, "GLOBAL_MLP input width must be num_grids");
+  } else if (p->mlp_widths[0] < 1 || p->mlp_widths[0] > WMAXW || p->mlp_widths[0] % 2 == 0) {
     return faila(QEX_E_UNSUPPORTED, "window must be odd and <= %d", WMAXW);
+  }
   for (int l = 1; l < p->mlp_layers; ++l)
     if (p->mlp_widths[l] < 1 || p->mlp_widths[l] > HMAX)
       return faila(QEX_E_UNSUPPORTED, "hidden widths must be in [1, %d] for the reverse pass", HMAX);
@@ -694,7 +839,8 @@ int param_count(const qex1d_problem* p) {
 LayA layout_adj(const qex1d_problem* p) {
   LayA L{};
   size_t off = 0;
-  const size_t B = p->batch, N = p->num_grids, T = p->num_iterations, W = p->mlp_widths[0];
+  const size_t B = p->batch, N = p->num_grids, T = p->num_iterations;
+  const size_t W = p->xc_kind == QEX1D_XC_GLOBAL_MLP ? 0 : p->mlp_widths[0];  // window cotangents (local only)
   auto take = [&](size_t b) {
     const size_t o = off;
     off = al(off + b);
@@ -806,10 +952,13 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
   const size_t mlp_lds = (2 * (size_t)a.W * a.H1 + a.H1 + (a.NL == 2 ? 2 * (size_t)a.H1 * a.H2 + a.H2 : 0) + HL +
                           (size_t)(NT / 64) * 6 * 64) * 8;
   const size_t eig_lds = (size_t)(4 + 5) * a.N * 8;
-  if (mlp_lds > 150 * 1024 || eig_lds > 150 * 1024) return faila(QEX_E_UNSUPPORTED, "problem too large for LDS");
+  const bool glob = p->xc_kind == QEX1D_XC_GLOBAL_MLP;
+  const size_t gmlp_lds = (2 * (size_t)a.N + 2 * (NT / 64) * 64 + 12 * 64 + 8) * 8;
+  if ((!glob && mlp_lds > 150 * 1024) || eig_lds > 150 * 1024)
+    return faila(QEX_E_UNSUPPORTED, "problem too large for LDS");
   if (eig_lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)k_adj_eig, hipFuncAttributeMaxDynamicSharedMemorySize, (int)eig_lds);
-  if (mlp_lds > 64 * 1024) {
+  if (!glob && mlp_lds > 64 * 1024) {
     (void)hipFuncSetAttribute((const void*)k_adj_mlp<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_lds);
     (void)hipFuncSetAttribute((const void*)k_adj_mlp<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_lds);
   }
@@ -819,6 +968,15 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
       hipLaunchKernelGGL(k_adj_mlp<2>, dim3(ADJ_WG), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
     else
       hipLaunchKernelGGL(k_adj_mlp<1>, dim3(ADJ_WG), dim3(NT), mlp_lds, st, a, xin, tin, F1, sf, F2);
+  };
+  // the global network: one workgroup per system (at most ADJ_WG, each then taking several in turn)
+  const unsigned gG = (unsigned)(a.B < ADJ_WG ? a.B : ADJ_WG);
+  auto gmlp = [&](const double* xin, const double* tin, const double* sf, double fd, const double* A2, double* dst) {
+    qexprof::Scope ps(qexprof::K_ADJ_MLP, st);
+    if (a.NL == 2)
+      hipLaunchKernelGGL(k_adj_gmlp<2>, dim3(gG), dim3(NT), gmlp_lds, st, a, xin, tin, sf, fd, A2, dst);
+    else
+      hipLaunchKernelGGL(k_adj_gmlp<1>, dim3(gG), dim3(NT), gmlp_lds, st, a, xin, tin, sf, fd, A2, dst);
   };
   const unsigned gBN = (unsigned)((BN + NT - 1) / NT);
   // alpha_k as the forward computes it
@@ -845,7 +1003,10 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
       hipLaunchKernelGGL(k_toeplitz, dim3((a.N + NT - 1) / NT, a.B), dim3(NT), (size_t)2 * a.N * 8, st, a.N, a.dx,
                          a.table, a.nout, a.tmp);
       hipLaunchKernelGGL(k_adj_energy, dim3(gBN), dim3(NT), 0, st, a, k, cs);
-      if (a.reflect) {
+      if (glob) {
+        // dx f(n_raw): nob += c grad f(n_raw), thetabar += c df/dtheta (no flip of the XC input)
+        gmlp(a.nout, nullptr, cs, 0.0, nullptr, a.nob);
+      } else if (a.reflect) {
         // dx F(S n_raw): nob += c S grad F(S n_raw) (vbar and tmp are free until the eigen adjoint)
         hipLaunchKernelGGL(k_sym_copy, dim3(gBN), dim3(NT), 0, st, a, a.vbar, a.nout);
         mlp(a.vbar, nullptr, a.vbar, cs, nullptr);
@@ -866,7 +1027,10 @@ int qex1d_kohn_sham_vjp(const qex1d_problem* p, const qex1d_inputs* in, const qe
                          a.table, a.vbar, a.tmp);
       // the input densities n_k of all systems, gathered contiguously into nout (free again)
       hipLaunchKernelGGL(k_adj_gather_input, dim3(gBN), dim3(NT), 0, st, a, k, a.nout);
-      if (a.reflect) {
+      if (glob) {
+        // v_xc = dx grad f: nbar += dx U vbar + dx H_f vbar, thetabar += dx d(vbar . grad f)/dtheta
+        gmlp(a.nout, a.vbar, nullptr, a.dx, a.tmp, a.nbar);
+      } else if (a.reflect) {
         // v_xc(n) = S grad F(S n): the second-order adjoint at (S n, S vbar), its n-cotangent mapped by S
         hipLaunchKernelGGL(k_sym_inplace, dim3(gBN), dim3(NT), 0, st, a, a.nout);
         hipLaunchKernelGGL(k_sym_inplace, dim3(gBN), dim3(NT), 0, st, a, a.vbar);

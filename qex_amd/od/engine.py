@@ -417,7 +417,7 @@ class BatchRunner:
 
 def kohn_sham_vjp(problem: Problem1D, result: BatchResult, d_total_energy=None, d_density=None) -> torch.Tensor:
     """Gradient of sum(d_total_energy * total_energy) + sum(d_density * density) with respect to the
-    packed parameters of the problem's LocalMLP functional (summed over the batch), by the device
+    packed parameters of the problem's LocalMLP or GlobalMLP functional (summed over the batch), by the device
     reverse pass (``qex1d_kohn_sham_vjp``).  ``result`` must come from ``problem.run`` with
     ``want_orbitals=True`` on the same initial densities."""
     if result.orbitals is None:

@@ -91,8 +91,6 @@ class KSDFTTrainer:
     def value_and_grad_fn(self, dataset, train_set, initial_density, grids, xc_fn, spec):
         """flat parameters -> (loss, gradient), forward and reverse pass on the device."""
         cfg = self.config
-        if cfg.get("use_amplitude_encoding", False):
-            raise NotImplementedError("the device reverse pass differentiates local functionals only")
         if cfg.get("density_mse_converge_tolerance", -1.0) >= 0:
             raise NotImplementedError("the device reverse pass needs density_mse_converge_tolerance < 0")
         T = int(cfg["num_iterations"])

@@ -213,3 +213,73 @@ def test_stop_gradient_step_matches_oracle(mods):
         go = go + A.backward(fw, dE / 3, dn / 3, stop_gradient_step=s_stop)
     g = g.cpu().numpy()
     np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * np.abs(go).max())
+
+
+@pytest.mark.parametrize("act,sizes,m,out", [("tanh", [257, 16, 8, 1], 1, "linear"),
+                                             ("silu", [257, 24, 1], 2, "negativity")])
+def test_global_mlp_device_gradient_matches_oracle(mods, act, sizes, m, out):
+    """GLOBAL_MLP (amplitude encoded) reverse pass: the device gradient against the oracle's
+    reverse pass (GlobalMLPAdjoint, pinned by finite differences in tests/test_oracle_adjoint.py)."""
+    engine, train, utils, xc = mods
+    grids = np.linspace(-12.8, 12.8, 257)
+    dx = float(grids[1] - grids[0])
+    systems = _systems()
+    params = _params(6, sizes)
+    T, alpha, decay = 6, 0.3, 0.9
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in systems])
+    f = xc.GlobalMLPXC([(w, b) for w, b in params], activation=act, norm=2.0, output=out)
+    prob = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [s[2] for s in systems],
+                            utils.exponential_coulomb, f, T, alpha=alpha, alpha_decay=decay,
+                            num_mixing_iterations=m, loop="scan", want_orbitals=True)
+    tE = np.array([-1.2, -2.5, -4.0])
+    tn = 0.9 * n0
+    _, g, res = train.value_and_grad(prob, n0, train.c3_loss, tE, tn, dx)
+    g = g.cpu().numpy()
+    oxc = O.GlobalMLP([w for w, _ in params], [b for _, b in params], act, 2.0, output=out)
+    go = np.zeros_like(g)
+    B = len(systems)
+    for s, (loc, z, ne) in enumerate(systems):
+        fw = A.forward_trajectory(loc, z, ne, T, grids, oxc, O.exponential_coulomb, n0[s], alpha, decay, m)
+        np.testing.assert_allclose(res.total_energy[s].cpu().numpy(), fw["energies"], atol=1e-8)
+        dE = np.zeros(T)
+        dn = np.zeros((T, len(grids)))
+        dE[-1] = 2 * (fw["energies"][-1] - tE[s]) / B
+        dn[-1] = 2 * 1e-2 * dx * (fw["densities"][-1] - tn[s]) / B
+        go += A.backward(fw, dE, dn)
+    scale = np.abs(go).max()
+    np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * scale)
+
+
+def test_global_mlp_reflected_device_gradient_matches_oracle(mods):
+    """GLOBAL_MLP with reflection about the grid centre (the trainer's setting): only the density
+    passed on is flipped, the functional sees unflipped densities (od/jit_scf.py:15-78)."""
+    engine, train, utils, xc = mods
+    grids = np.linspace(-12.8, 12.8, 257)
+    dx = float(grids[1] - grids[0])
+    systems = _systems()
+    params = _params(7, [257, 16, 16, 1])
+    T, alpha, decay, m = 6, 0.3, 0.9, 2
+    n0 = np.stack([O.gaussian_initial_density(grids, l, z, ne, weighted=True) for l, z, ne in systems])
+    f = xc.GlobalMLPXC([(w, b) for w, b in params], activation="tanh", norm=2.0)
+    prob = engine.Problem1D(grids, [s[0] for s in systems], [s[1] for s in systems], [s[2] for s in systems],
+                            utils.exponential_coulomb, f, T, alpha=alpha, alpha_decay=decay,
+                            num_mixing_iterations=m, loop="scan", want_orbitals=True,
+                            enforce_reflection_symmetry=True)
+    tE = np.array([-1.2, -2.5, -4.0])
+    tn = 0.9 * n0
+    _, g, res = train.value_and_grad(prob, n0, train.c3_loss, tE, tn, dx)
+    g = g.cpu().numpy()
+    oxc = O.GlobalMLP([w for w, _ in params], [b for _, b in params], "tanh", 2.0)
+    go = np.zeros_like(g)
+    B = len(systems)
+    for s, (loc, z, ne) in enumerate(systems):
+        fw = A.forward_trajectory(loc, z, ne, T, grids, oxc, O.exponential_coulomb, n0[s], alpha, decay, m,
+                                  reflect=True)
+        np.testing.assert_allclose(res.total_energy[s].cpu().numpy(), fw["energies"], atol=1e-8)
+        dE = np.zeros(T)
+        dn = np.zeros((T, len(grids)))
+        dE[-1] = 2 * (fw["energies"][-1] - tE[s]) / B
+        dn[-1] = 2 * 1e-2 * dx * (fw["densities"][-1] - tn[s]) / B
+        go += A.backward(fw, dE, dn)
+    scale = np.abs(go).max()
+    np.testing.assert_allclose(g, go, rtol=0, atol=1e-6 * scale)

@@ -127,9 +127,6 @@ def test_dataset_loader_reads_the_jax_dft_layout():
 
 def test_trainer_rejects_what_the_device_reverse_pass_lacks():
     from qex_amd.od.trainer import KSDFTTrainer
-    tr = KSDFTTrainer({"use_amplitude_encoding": True, "num_iterations": 15}, N.LocalMLP(), GOLDEN)
-    with pytest.raises(NotImplementedError):
-        tr.value_and_grad_fn(None, None, None, None, None, None)
     tr = KSDFTTrainer({"density_mse_converge_tolerance": 1e-6, "num_iterations": 15}, N.LocalMLP(), GOLDEN)
     with pytest.raises(NotImplementedError):
         tr.value_and_grad_fn(None, None, None, None, None, None)

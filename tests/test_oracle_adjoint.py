@@ -239,3 +239,107 @@ def test_stop_gradient_step_is_the_gradient_of_the_restarted_loss():
         assert abs(fd - g @ d) < 1e-7 * max(1.0, abs(fd))
     assert np.abs(A.backward(fw, dE, dn, stop_gradient_step=T - 1)).max() == 0.0
     assert np.abs(g - A.backward(fw, dE, dn)).max() > 1e-6
+
+
+def test_global_vxc_vjp_is_hessian_vector_product():
+    """GlobalMLPAdjoint: nbar of vbar . v_xc is the derivative of v_xc = dx grad f along vbar, and the
+    energy gradient is the forward grad f."""
+    grids = np.linspace(-5, 5, 41)
+    dx = O.get_dx(grids)
+    p = _net(3, [41, 9, 7, 1], "tanh")
+    xc = O.GlobalMLP([w for w, _ in p], [b for _, b in p], "tanh", 2.0, output="negativity")
+    adj = A.GlobalMLPAdjoint(xc, dx)
+    rng = np.random.default_rng(4)
+    n = np.abs(rng.standard_normal(41)) * 0.3
+    vbar = rng.standard_normal(41)
+    nbar, dth = adj.vxc_vjp(n, vbar)
+    h = 1e-6
+    fd = (adj.potential(n + h * vbar) - adj.potential(n - h * vbar)) / (2 * h)
+    np.testing.assert_allclose(nbar, fd, atol=1e-8)
+    f, g, _ = adj.energy_and_grads(n)
+    assert f == pytest.approx(xc.value(n), abs=1e-14)
+    np.testing.assert_allclose(g, xc.grad(n), atol=1e-13)
+    # theta-gradient of vbar . v_xc against differences in one weight of each layer
+    W, b = [w for w, _ in p], [x for _, x in p]
+    for li, (i, j) in [(0, (5, 2)), (1, (3, 4)), (2, (6, 0))]:
+        Wp = [w.copy() for w in W]
+        Wm = [w.copy() for w in W]
+        Wp[li][i, j] += h
+        Wm[li][i, j] -= h
+        vp = A.GlobalMLPAdjoint(O.GlobalMLP(Wp, b, "tanh", 2.0, output="negativity"), dx).potential(n)
+        vm = A.GlobalMLPAdjoint(O.GlobalMLP(Wm, b, "tanh", 2.0, output="negativity"), dx).potential(n)
+        assert abs((vp - vm) @ vbar / (2 * h) - dth[li][0][i, j]) < 1e-7
+
+
+@pytest.mark.parametrize("act,ne,m", [("tanh", 2, 1), ("silu", 3, 2)])
+def test_global_trajectory_gradient_matches_finite_differences(act, ne, m):
+    grids = np.linspace(-8, 8, 65)
+    dx = O.get_dx(grids)
+    params = _net(6, [65, 8, 6, 1], act)
+    loc = np.array([-0.7, 0.8])
+    z = np.array([1.0, float(ne - 1)])
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True)
+    rng = np.random.default_rng(12)
+
+    def run(flat):
+        p = A.unflatten(flat, params)
+        xc = O.GlobalMLP([w for w, _ in p], [b for _, b in p], act, 2.0)
+        fw = A.forward_trajectory(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, 0.3, 0.9, m)
+        L, e, d = A.c3_loss_and_cotangents(fw["energies"], fw["densities"], -1.5, 0.95 * n0, dx)
+        return L, fw, e, d
+
+    th = A.flatten(params)
+    _, fw, dE, dn = run(th)
+    g = A.backward(fw, dE, dn)
+    h = 1e-5
+    for _ in range(2):
+        d = rng.standard_normal(th.size)
+        d /= np.linalg.norm(d)
+        fd = (run(th + h * d)[0] - run(th - h * d)[0]) / (2 * h)
+        assert abs(fd - g @ d) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_global_forward_trajectory_matches_scan_oracle():
+    grids = np.linspace(-8, 8, 65)
+    params = _net(5, [65, 8, 8, 1], "tanh")
+    xc = O.GlobalMLP([w for w, _ in params], [b for _, b in params], "tanh", 2.0)
+    loc, z, ne = np.array([-0.7, 0.7]), np.array([1.0, 1.0]), 2
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True)
+    fw = A.forward_trajectory(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, 0.3, 1.0, 1)
+    st = O.jit_kohn_sham(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, alpha=0.3, alpha_decay=1.0,
+                         num_mixing_iterations=1)
+    np.testing.assert_allclose(fw["energies"], st.total_energy, atol=1e-10)
+    np.testing.assert_allclose(fw["densities"], st.density, atol=1e-10)
+
+
+def test_global_reflected_trajectory_matches_scan_oracle_and_differences():
+    """Global functional under reflection (od/jit_scf.py:15-78): only the density passed on is
+    flipped.  The trajectory equals the scan oracle's, and its gradient central differences."""
+    grids = np.linspace(-8, 8, 65)
+    dx = O.get_dx(grids)
+    params = _net(8, [65, 8, 1], "tanh")
+    loc, z, ne = np.array([-0.9, 0.6]), np.array([1.0, 1.0]), 2
+    n0 = O.gaussian_initial_density(grids, loc, z, ne, weighted=True)
+
+    def run(flat):
+        p = A.unflatten(flat, params)
+        xc = O.GlobalMLP([w for w, _ in p], [b for _, b in p], "tanh", 2.0)
+        fw = A.forward_trajectory(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, 0.3, 0.9, 2, reflect=True)
+        L, e, d = A.c3_loss_and_cotangents(fw["energies"], fw["densities"], -1.5, 0.95 * n0, dx)
+        return L, fw, e, d
+
+    th = A.flatten(params)
+    _, fw, dE, dn = run(th)
+    xc = O.GlobalMLP([w for w, _ in params], [b for _, b in params], "tanh", 2.0)
+    st = O.jit_kohn_sham(loc, z, ne, 5, grids, xc, O.exponential_coulomb, n0, alpha=0.3, alpha_decay=0.9,
+                         num_mixing_iterations=2, enforce_reflection_symmetry=True)
+    np.testing.assert_allclose(fw["energies"], st.total_energy, atol=1e-10)
+    np.testing.assert_allclose(fw["densities"], st.density, atol=1e-10)
+    g = A.backward(fw, dE, dn)
+    rng = np.random.default_rng(3)
+    h = 1e-5
+    for _ in range(2):
+        d = rng.standard_normal(th.size)
+        d /= np.linalg.norm(d)
+        fd = (run(th + h * d)[0] - run(th - h * d)[0]) / (2 * h)
+        assert abs(fd - g @ d) < 1e-6 * max(1.0, abs(fd))

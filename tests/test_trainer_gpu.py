@@ -102,3 +102,38 @@ def test_trainer_evaluate_runs_the_python_loop(mods, tmp_path):
     assert np.all(np.isfinite(st.total_energy))
     lda = tr.evaluate(ck, [128], use_lda=True)
     assert lda.total_energy.shape == (1, 12)
+
+
+def test_trainer_amplitude_encoding_loss_and_gradient_match_oracle(mods):
+    """use_amplitude_encoding: a GlobalMLP network (no self-interaction layer) trained through the
+    amplitude-encoded scan loop (od/train.py:117-121 picks jit_scf_od.kohn_sham_amplitude_encoded);
+    the device loss and gradient against the oracle's reverse pass."""
+    import functools
+
+    from qex_amd.od import xc as xcmod
+
+    networks, trainer = mods
+    cfg = _config(network_type="mlp_ksr", use_amplitude_encoding=True, wrap_self_interaction=False)
+    tr = trainer.KSDFTTrainer(cfg, networks.GlobalMLP(cfg), GOLDEN, seed=3, log=lambda *a: None)
+    dataset, train_set, n0, grids = tr.prepare_dataset()
+    init_fn, xc_fn = tr._functional(grids)
+    params = init_fn(4)
+    spec, flat = networks.flatten(params)
+    vg = tr.value_and_grad_fn(dataset, train_set, n0, grids, xc_fn, spec)
+    loss, grad = vg(flat)
+    f = xcmod.resolve(functools.partial(xc_fn, params=params))
+    assert f.encoding == "global" and f.output == "negativity"
+    oxc = O.GlobalMLP([w for w, _ in f.params], [b for _, b in f.params], "tanh", 2.0, output="negativity")
+    dx = float(grids[1] - grids[0])
+    B = len(train_set.total_energy)
+    lo, go = 0.0, np.zeros_like(grad)
+    for s in range(B):
+        on0 = O.solve_noninteracting_system(train_set.external_potential[s], 2, grids)[0]
+        fw = A.forward_trajectory(train_set.locations[s], train_set.nuclear_charges[s], 2, 15, grids, oxc,
+                                  O.exponential_coulomb, on0, 0.5, 0.9, 1, reflect=True)
+        L, dE, dn = A.trajectory_loss_and_cotangents(fw["energies"], fw["densities"], train_set.total_energy[s],
+                                                     train_set.density[s], 2, dx, discount=0.9, start=10)
+        lo += L / B
+        go += A.backward(fw, dE, dn) / B
+    assert abs(loss - lo) < 1e-9 * max(1.0, abs(lo))
+    np.testing.assert_allclose(grad, go, rtol=0, atol=1e-6 * np.abs(go).max())

@@ -65,14 +65,15 @@ def workload(batch, seed):
     return grids, mols, W, b, n0
 
 
-def _oracle_one(args):
-    os.environ["OMP_NUM_THREADS"] = "1"
+def _oracle_one(args, threads=1):
+    if threads == 1:
+        os.environ["OMP_NUM_THREADS"] = "1"
     from threadpoolctl import threadpool_limits
 
     from oracle import od as O
 
     grids, (loc, z, ne), W, b, n0 = args
-    with threadpool_limits(1):
+    with threadpool_limits(threads):
         t = time.perf_counter()
         st = O.jit_kohn_sham(loc, z, ne, T_ITERS, grids, O.LocalMLP(W, b, "silu", 17, 2.0), O.exponential_coulomb,
                              n0, alpha=ALPHA, alpha_decay=1.0, num_mixing_iterations=1)
@@ -469,7 +470,8 @@ def run_c4(args):
                      "frac": (achieved / peak) if achieved is not None else None, "traffic": None},
         "cpu_baseline": {"value": cpu_rate, "unit": "solves/s", "cores": 1, "kind": "port",
                          "sample": f"FP64 NumPy/SciPy oracle, 1 thread, H2 and H4 chains end to end (integrals, "
-                                   f"grid, SCF) repeated for ~10 s: {cpu_done} solves; {cpu_model()}"},
+                                   f"grid, SCF) repeated for ~10 s: {cpu_done} solves; H8 and H16 left out (each "
+                                   f"takes over 2 minutes on one core, beyond a bounded sample); {cpu_model()}"},
     }
     print(json.dumps(result))
 
@@ -745,13 +747,14 @@ def _blas_vendor():
         return "unknown"
 
 
-def _cpu_3d_one(n_atoms):
-    os.environ["OMP_NUM_THREADS"] = "1"
+def _cpu_3d_one(n_atoms, threads=1):
+    if threads == 1:
+        os.environ["OMP_NUM_THREADS"] = "1"
     from threadpoolctl import threadpool_limits
 
     from oracle import td
 
-    with threadpool_limits(1):
+    with threadpool_limits(threads):
         t = time.perf_counter()
         mol = td.synthetic_chain(n_atoms, 100 + n_atoms)
         S, T, V = td.one_electron(mol)
@@ -761,20 +764,40 @@ def _cpu_3d_one(n_atoms):
         return n_atoms, r.e_tot, time.perf_counter() - t
 
 
-def _c5_cpu_sample(grids, pool, W, b, pool_n0, kinds, pick_1d, pick_3d, procs, count=64):
-    """The oracle draining the first ``count`` requests of the timed stream with ``procs`` processes x 1
-    BLAS thread, except the H16 requests (386 s per solve on one core), which are left out of the sample."""
+def _c5_cpu_sample(grids, pool, W, b, pool_n0, kinds, pick_1d, pick_3d, procs, count=64, repeats=3):
+    """Throughput mode: the oracle draining the first ``count`` requests of the timed stream with
+    ``procs`` processes x 1 BLAS thread, ``repeats`` times (the first pass of the pool is not timed; the
+    median rate is reported), except the H16 requests (386 s per solve on one core), which are left
+    out of the sample."""
     jobs1 = [(grids, pool[pick_1d[i]], W, b, pool_n0[pick_1d[i]]) for i in range(count) if kinds[i] == "1d"]
     jobs3 = [C4_CHAINS[pick_3d[i]] for i in range(count) if kinds[i] == "3d" and C4_CHAINS[pick_3d[i]] != 16]
     skipped = sum(1 for i in range(count) if kinds[i] == "3d" and C4_CHAINS[pick_3d[i]] == 16)
     ctx = mp.get_context("spawn")
-    t = time.perf_counter()
+    walls = []
     with ctx.Pool(procs) as pool_:
-        r3 = pool_.map_async(_cpu_3d_one, jobs3, chunksize=1)
-        r1 = pool_.map(_oracle_one, jobs1, chunksize=1)
-        r3 = r3.get()
-    wall = time.perf_counter() - t
-    return (len(jobs1) + len(jobs3)) / wall, len(jobs1), len(jobs3), skipped, wall
+        pool_.map(_oracle_one, jobs1[:procs], chunksize=1)  # warm-up: imports in every worker
+        for _ in range(repeats):
+            t = time.perf_counter()
+            r3 = pool_.map_async(_cpu_3d_one, jobs3, chunksize=1)
+            pool_.map(_oracle_one, jobs1, chunksize=1)
+            r3.get()
+            walls.append(time.perf_counter() - t)
+    wall = float(np.median(walls))
+    return (len(jobs1) + len(jobs3)) / wall, len(jobs1), len(jobs3), skipped, wall, walls
+
+
+def _c5_cpu_latency(grids, pool, W, b, pool_n0, kinds, pick_1d, pick_3d, threads, count=8):
+    """Latency mode: one process with BLAS threads = all cores solving the first ``count`` requests
+    (H16 left out) one after another, after one untimed 1D solve; per-solve seconds."""
+    i1 = next(i for i in range(len(kinds)) if kinds[i] == "1d")
+    _oracle_one((grids, pool[pick_1d[i1]], W, b, pool_n0[pick_1d[i1]]), threads)
+    per = {"1d": [], "3d": []}
+    for i in range(count):
+        if kinds[i] == "1d":
+            per["1d"].append(_oracle_one((grids, pool[pick_1d[i]], W, b, pool_n0[pick_1d[i]]), threads)[2])
+        elif C4_CHAINS[pick_3d[i]] != 16:
+            per["3d"].append(_cpu_3d_one(C4_CHAINS[pick_3d[i]], threads)[2])
+    return per
 
 
 # algorithmic work per unit, by kernel kind (what roofline.achieved counts; written out in DESIGN.md)
@@ -854,12 +877,14 @@ def run_c5(args):
     procs = args.procs or max(1, min(16, ncores))
     # ---- FP64 oracle (host cores, before the GPU is touched): every 1D system of the stream ----
     if args.skip_oracle:
-        ref1, cpu = {}, None
+        ref1, cpu, cpu_lat = {}, None, None
     else:
         used = sorted(set(int(i) for i in pick_1d[kinds == "1d"]))
         out, _ = oracle_batch(grids, [pool[i] for i in used], W, b, np.stack([pool_n0[i] for i in used]), procs)
         ref1 = {i: o[0][-1] for i, o in zip(used, out)}
         cpu = _c5_cpu_sample(grids, pool, W, b, pool_n0, kinds[nwarm:], pick_1d[nwarm:], pick_3d[nwarm:], procs)
+        cpu_lat = _c5_cpu_latency(grids, pool, W, b, pool_n0, kinds[nwarm:], pick_1d[nwarm:], pick_3d[nwarm:],
+                                  procs)
 
     import torch
 
@@ -1030,7 +1055,14 @@ def run_c5(args):
             "value": cpu[0], "unit": "solves/s", "cores": procs, "kind": "port",
             "sample": f"FP64 NumPy/SciPy oracle draining the first 64 requests of the timed stream "
                       f"({cpu[1]} 1D, {cpu[2]} 3D H2/H4/H8; {cpu[3]} H16 left out at 386 s each on one core), "
-                      f"{procs} processes x 1 thread, {cpu[4]:.1f} s; BLAS {_blas_vendor()}; {cpu_model()}"},
+                      f"{procs} processes x 1 thread, median of 3 repeats "
+                      f"({', '.join(f'{w:.1f}' for w in cpu[5])} s); BLAS {_blas_vendor()}; {cpu_model()}; "
+                      f"os.cpu_count() {os.cpu_count()}, affinity {ncores}",
+            "latency": {"mode": f"one process, BLAS threads = {procs}, the first 8 requests one at a time "
+                                f"(H16 left out)",
+                        **{f"p{q}_s_{k}": (float(np.percentile(v, q)) if v else None)
+                           for k, v in cpu_lat.items() for q in (50, 99)},
+                        "solves": {k: len(v) for k, v in cpu_lat.items()}}},
     }
     print(json.dumps(result))
 

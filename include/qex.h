@@ -340,6 +340,18 @@ int qex3d_eri_batch(int32_t B, const qex3d_basis* bases, double* const* eri, voi
 int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const double* rad_w, int32_t nang,
                const double* ang_xyz, const double* ang_w, double* coords, double* weights, void* stream);
 
+/* Fuzzy-cell partition of the atom-centred grid: cell functions P_A = prod_B (1 - g(mu_AB)) / 2 with
+ * Becke's g (three iterations of 1.5 mu - 0.5 mu^3) or Stratmann-Scuseria-Frisch's
+ * (CPL 257, 213 (1996), eq. 14; pyscf gen_grid.stratmann, which the reference selects with
+ * grids.becke_scheme at trainer_legacy_no_jit.py:250).  No atomic-size adjustment (homonuclear
+ * chains need none). */
+#define QEX3D_PARTITION_BECKE 0
+#define QEX3D_PARTITION_STRATMANN 1
+/* qex3d_grid with the partition scheme chosen. */
+int qex3d_grid_partition(const qex3d_basis* b, int32_t scheme, int32_t nrad, const double* rad_r, const double* rad_w,
+                         int32_t nang, const double* ang_xyz, const double* ang_w, double* coords, double* weights,
+                         void* stream);
+
 /* ao[g][mu] on ngrid points.  Replaces pyscf numint.eval_ao (td/eval_gto.py:20-70). */
 int qex3d_eval_ao(const qex3d_basis* b, int64_t ngrid, const double* coords, double* ao, void* stream);
 

@@ -332,8 +332,19 @@ def angular_product(n_theta=10, n_phi=11):
     return dirs, w
 
 
-def becke_weights(coords_pts, atom_coords, owner):
-    """Becke fuzzy-cell weight of each point's owning atom (3 iterations of p(mu) = 1.5 mu - 0.5 mu^3)."""
+def stratmann_step(mu, a=0.64):
+    """Stratmann-Scuseria-Frisch cell step (CPL 257, 213 (1996), eq. 14; pyscf gen_grid.stratmann,
+    selected at td/trainer_legacy_no_jit.py:250): (35 z - 35 z^3 + 21 z^5 - 5 z^7) / 16 with z = mu / a,
+    and sign(mu) for |mu| >= a."""
+    mu = np.asarray(mu, dtype=np.float64)
+    z = mu / a
+    g = (35 * z - 35 * z ** 3 + 21 * z ** 5 - 5 * z ** 7) / 16
+    return np.where(mu <= -a, -1.0, np.where(mu >= a, 1.0, g))
+
+
+def becke_weights(coords_pts, atom_coords, owner, scheme="becke"):
+    """Fuzzy-cell weight of each point's owning atom: Becke (3 iterations of p(mu) = 1.5 mu - 0.5 mu^3)
+    or Stratmann (``stratmann_step``)."""
     A = len(atom_coords)
     d = np.linalg.norm(coords_pts[:, None, :] - atom_coords[None, :, :], axis=-1)  # (G, A)
     P = np.ones((len(coords_pts), A))
@@ -343,13 +354,16 @@ def becke_weights(coords_pts, atom_coords, owner):
                 continue
             rab = np.linalg.norm(atom_coords[a] - atom_coords[b])
             mu = (d[:, a] - d[:, b]) / rab
-            for _ in range(3):
-                mu = 1.5 * mu - 0.5 * mu ** 3
+            if scheme == "stratmann":
+                mu = stratmann_step(mu)
+            else:
+                for _ in range(3):
+                    mu = 1.5 * mu - 0.5 * mu ** 3
             P[:, a] *= 0.5 * (1 - mu)
     return P[np.arange(len(coords_pts)), owner] / np.sum(P, axis=1)
 
 
-def molecular_grid(mol, n_rad=50, n_theta=10, n_phi=11):
+def molecular_grid(mol, n_rad=50, n_theta=10, n_phi=11, scheme="becke"):
     r, wr = radial_treutler(n_rad)
     dirs, wa = angular_product(n_theta, n_phi)
     pts, wts, own = [], [], []
@@ -361,7 +375,7 @@ def molecular_grid(mol, n_rad=50, n_theta=10, n_phi=11):
     pts = np.concatenate(pts)
     wts = np.concatenate(wts)
     own = np.concatenate(own)
-    return pts, wts * becke_weights(pts, mol.coords, own)
+    return pts, wts * becke_weights(pts, mol.coords, own, scheme)
 
 
 def eval_ao(mol, pts):

@@ -117,6 +117,8 @@ def ao32_rows(ngrid):
     """Row count of the FP32 AO layout (QEX3D_AO32_ROWS)."""
     return (ngrid + 63) // 64 * 64
 QEX3D_MAX_NAO = 96
+QEX3D_PARTITION_BECKE = 0
+QEX3D_PARTITION_STRATMANN = 1
 QEX3D_MAX_NPRIM = 8
 QEX3D_MAX_SCF_BATCH = 1024
 QEX3D_MAX_ERI_BATCH = 1024
@@ -182,6 +184,7 @@ SIGNATURES = {
     "qex3d_eri_batch_workspace_bytes": (_S, [_I, ctypes.POINTER(Qex3dBasis)]),
     "qex3d_eri_batch": (_I, [_I, ctypes.POINTER(Qex3dBasis), _P, _P, _S, _P]),
     "qex3d_grid": (_I, [ctypes.POINTER(Qex3dBasis), _I, _P, _P, _I, _P, _P, _P, _P, _P]),
+    "qex3d_grid_partition": (_I, [ctypes.POINTER(Qex3dBasis), _I, _I, _P, _P, _I, _P, _P, _P, _P, _P]),
     "qex3d_eval_ao": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
     "qex3d_eval_ao_f32": (_I, [ctypes.POINTER(Qex3dBasis), ctypes.c_int64, _P, _P, _P]),
     "qex3d_eval_rho": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P]),

@@ -913,7 +913,22 @@ __global__ void k_put_eri(EriPut pk, int count, EriArgs* dst) {
 // ---------------------------------------------------------------------------------------------
 // grid and AO values
 // ---------------------------------------------------------------------------------------------
-__global__ void k_grid(Basis b, int nrad, const double* rr, const double* rw, int nang, const double* ax,
+// cell-function step of the fuzzy partition: Becke's three iterations of p(mu) = 1.5 mu - 0.5 mu^3,
+// or Stratmann-Scuseria-Frisch (CPL 257, 213 (1996), eq. 14): the odd polynomial
+// (35 z - 35 z^3 + 21 z^5 - 5 z^7) / 16, z = mu / 0.64, and sign(mu) for |mu| >= 0.64
+__device__ __forceinline__ double cell_step(int scheme, double mu) {
+  if (scheme == QEX3D_PARTITION_STRATMANN) {
+    constexpr double A = 0.64;
+    if (mu <= -A) return -1.0;
+    if (mu >= A) return 1.0;
+    const double z = mu / A, z2 = z * z;
+    return (1.0 / 16.0) * (z * (35.0 + z2 * (-35.0 + z2 * (21.0 - 5.0 * z2))));
+  }
+  for (int it = 0; it < 3; ++it) mu = 1.5 * mu - 0.5 * mu * mu * mu;
+  return mu;
+}
+
+__global__ void k_grid(Basis b, int scheme, int nrad, const double* rr, const double* rw, int nang, const double* ax,
                        const double* aw, double* coords, double* weights) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long per = (long)nrad * nang;
@@ -923,7 +938,7 @@ __global__ void k_grid(Basis b, int nrad, const double* rr, const double* rw, in
   const double* C = b.xyz + 3 * a;
   double x[3];
   for (int d = 0; d < 3; ++d) x[d] = C[d] + rr[ir] * ax[3 * ia + d];
-  // Becke cell functions with 3 iterations of p(mu) = 1.5 mu - 0.5 mu^3
+  // fuzzy cell functions P_A = prod_B (1 - step(mu_AB)) / 2
   double Pown = 0.0, Psum = 0.0;
   for (int A = 0; A < b.natm; ++A) {
     const double* RA = b.xyz + 3 * A;
@@ -934,9 +949,7 @@ __global__ void k_grid(Basis b, int nrad, const double* rr, const double* rw, in
       const double* RB = b.xyz + 3 * Bq;
       const double dB = sqrt((x[0] - RB[0]) * (x[0] - RB[0]) + (x[1] - RB[1]) * (x[1] - RB[1]) + (x[2] - RB[2]) * (x[2] - RB[2]));
       const double rab = sqrt((RA[0] - RB[0]) * (RA[0] - RB[0]) + (RA[1] - RB[1]) * (RA[1] - RB[1]) + (RA[2] - RB[2]) * (RA[2] - RB[2]));
-      double mu = (dA - dB) / rab;
-      for (int it = 0; it < 3; ++it) mu = 1.5 * mu - 0.5 * mu * mu * mu;
-      PA *= 0.5 * (1.0 - mu);
+      PA *= 0.5 * (1.0 - cell_step(scheme, (dA - dB) / rab));
     }
     Psum += PA;
     if (A == a) Pown = PA;
@@ -2831,14 +2844,23 @@ int qex3d_eri(const qex3d_basis* b, double* eri, void* workspace, size_t workspa
 
 int qex3d_grid(const qex3d_basis* b, int32_t nrad, const double* rad_r, const double* rad_w, int32_t nang,
                const double* ang_xyz, const double* ang_w, double* coords, double* weights, void* stream) {
+  return qex3d_grid_partition(b, QEX3D_PARTITION_BECKE, nrad, rad_r, rad_w, nang, ang_xyz, ang_w, coords, weights,
+                              stream);
+}
+
+int qex3d_grid_partition(const qex3d_basis* b, int32_t scheme, int32_t nrad, const double* rad_r, const double* rad_w,
+                         int32_t nang, const double* ang_xyz, const double* ang_w, double* coords, double* weights,
+                         void* stream) {
   int rc = check_basis(b);
   if (rc) return rc;
+  if (scheme != QEX3D_PARTITION_BECKE && scheme != QEX3D_PARTITION_STRATMANN)
+    return fail3(QEX_E_BADARG, "bad partition scheme %d", scheme);
   if (nrad < 1 || nang < 1 || !rad_r || !rad_w || !ang_xyz || !ang_w || !coords || !weights)
     return fail3(QEX_E_BADARG, "bad grid arguments");
   const long ng = (long)b->natm * nrad * nang;
   qexprof::Scope ps(qexprof::K_GRID, (hipStream_t)stream);
-  hipLaunchKernelGGL(k_grid, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, (hipStream_t)stream, make_basis(b), nrad,
-                     rad_r, rad_w, nang, ang_xyz, ang_w, coords, weights);
+  hipLaunchKernelGGL(k_grid, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, (hipStream_t)stream, make_basis(b),
+                     (int)scheme, nrad, rad_r, rad_w, nang, ang_xyz, ang_w, coords, weights);
   return launch_status("qex3d_grid");
 }
 

@@ -1,7 +1,7 @@
 """Molecular integration grid (``pyscf.dft.Grids`` as built at
 ``qedft/train/td/trainer_legacy_no_jit.py:248-251``): Treutler-Ahlrichs M4 radial points on a
-Chebyshev (2nd kind) base, a Gauss-Legendre(cos theta) x uniform-phi angular product, and Becke
-partitioning on the device (``qex3d_grid``)."""
+Chebyshev (2nd kind) base, a Gauss-Legendre(cos theta) x uniform-phi angular product, and Becke or
+Stratmann partitioning on the device (``qex3d_grid_partition``)."""
 
 from __future__ import annotations
 
@@ -50,10 +50,31 @@ def _device_tables(n_rad, n_theta, n_phi, dev):
     return _TABLES[key]
 
 
+def original_becke(g):
+    """Becke's cell step, three iterations of 1.5 mu - 0.5 mu^3 (pyscf gen_grid.original_becke); as
+    ``Grids.becke_scheme`` it selects that partition on the device."""
+    for _ in range(3):
+        g = 1.5 * g - 0.5 * g ** 3
+    return g
+
+
+def stratmann(g):
+    """Stratmann-Scuseria-Frisch cell step (pyscf gen_grid.stratmann, chosen by the reference at
+    td/trainer_legacy_no_jit.py:250); as ``Grids.becke_scheme`` it selects that partition on the device."""
+    a = 0.64
+    z = np.asarray(g, dtype=np.float64) / a
+    out = (35 * z - 35 * z ** 3 + 21 * z ** 5 - 5 * z ** 7) / 16
+    return np.where(z <= -1, -1.0, np.where(z >= 1, 1.0, out))
+
+
 class Grids:
+    """Atom-centred grid built on the device (``qex3d_grid_partition``); ``becke_scheme`` is
+    ``original_becke`` (default) or ``stratmann``, as pyscf's attribute of that name."""
+
     def __init__(self, mol, n_rad=50, n_theta=10, n_phi=11):
         self.mol = mol
         self.n_rad, self.n_theta, self.n_phi = n_rad, n_theta, n_phi
+        self.becke_scheme = original_becke
         self.coords = None
         self.weights = None
 
@@ -64,9 +85,13 @@ class Grids:
         self.coords = torch.empty((ng, 3), dtype=torch.float64, device=dev)
         self.weights = torch.empty(ng, dtype=torch.float64, device=dev)
         p = lambda t: ctypes.c_void_p(t.data_ptr())
-        _lib.check3(_lib.load().qex3d_grid(ctypes.byref(self.mol.device_basis()), nr, p(tabs[0]), p(tabs[1]),
-                                           na, p(tabs[2]), p(tabs[3]), p(self.coords), p(self.weights),
-                                           engine._stream()), "qex3d_grid")
+        schemes = {original_becke: _lib.QEX3D_PARTITION_BECKE, stratmann: _lib.QEX3D_PARTITION_STRATMANN}
+        if self.becke_scheme not in schemes:
+            raise ValueError("becke_scheme must be qex_amd.td.dft.original_becke or qex_amd.td.dft.stratmann")
+        _lib.check3(_lib.load().qex3d_grid_partition(ctypes.byref(self.mol.device_basis()), schemes[self.becke_scheme],
+                                                     nr, p(tabs[0]), p(tabs[1]), na, p(tabs[2]), p(tabs[3]),
+                                                     p(self.coords), p(self.weights), engine._stream()),
+                    "qex3d_grid_partition")
         self._keep = tabs
         return self
 

@@ -302,3 +302,32 @@ def test_degen_eigh_bwd_masks_degenerate_pairs():
     gV[:, 1], gV[:, 2] = Q[:, 2], -Q[:, 1]  # U gV is nonzero only on the (1, 2) pair
     g = T.degen_eigh_bwd(w, Q, None, gV, exact=True)
     assert np.all(np.isfinite(g)) and np.abs(g).max() < 1e-12
+
+
+def test_stratmann_step_closed_form_properties():
+    """The Stratmann cell step (CPL 257, 213 (1996), eq. 14): odd, zero at 0, +-1 with zero slope at
+    mu = +-0.64 (C1 join), monotone in between; the Becke step is its three-fold iterated cubic."""
+    mu = np.linspace(-1, 1, 2001)
+    g = T.stratmann_step(mu)
+    np.testing.assert_allclose(g, -T.stratmann_step(-mu), atol=0)
+    assert T.stratmann_step(0.0) == 0.0
+    assert T.stratmann_step(0.64) == 1.0 and T.stratmann_step(-0.64) == -1.0
+    h = 1e-6
+    assert abs((T.stratmann_step(0.64 - h) - T.stratmann_step(0.64 - 2 * h)) / h) < 1e-4
+    assert np.all(np.diff(g) >= 0)
+    # polynomial value at z = 1/2 worked by hand: (35/2 - 35/8 + 21/32 - 5/128) / 16
+    assert T.stratmann_step(0.32) == pytest.approx((35 / 2 - 35 / 8 + 21 / 32 - 5 / 128) / 16, abs=1e-15)
+
+
+@pytest.mark.parametrize("scheme", ["becke", "stratmann"])
+def test_partition_of_unity_and_gaussian_integral(scheme):
+    """Cell weights of every atom sum to one at any point; a normalised Gaussian on each atom of a
+    chain integrates to the atom count on the partitioned grid."""
+    mol = T.synthetic_chain(4, 104)
+    pts = np.random.default_rng(0).normal(size=(200, 3)) * 2.0
+    natm = len(mol.coords)
+    tot = sum(T.becke_weights(pts, mol.coords, np.full(len(pts), a), scheme) for a in range(natm))
+    np.testing.assert_allclose(tot, 1.0, atol=1e-13)
+    p, w = T.molecular_grid(mol, scheme=scheme)
+    dens = sum((2 / np.pi) ** 1.5 * np.exp(-2 * np.sum((p - c) ** 2, axis=1)) for c in mol.coords)
+    assert abs(w @ dens - natm) < 5e-4  # quadrature error of the 50 x 110-point atomic grids

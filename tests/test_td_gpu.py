@@ -62,6 +62,23 @@ def test_grid_matches_oracle(qtd):
     np.testing.assert_allclose(grids.weights.cpu().numpy(), w, rtol=1e-11, atol=1e-13 * w.max())
 
 
+def test_stratmann_grid_matches_oracle(qtd):
+    """grids.becke_scheme = stratmann (trainer_legacy_no_jit.py:250) on the device against the
+    oracle's Stratmann partition (a different partition from Becke's); other schemes are refused."""
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    grids = qtd.dft.Grids(mol)
+    grids.becke_scheme = qtd.dft.stratmann
+    grids.build()
+    pts, w = otd.molecular_grid(omol, scheme="stratmann")
+    np.testing.assert_allclose(grids.coords.cpu().numpy(), pts, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(grids.weights.cpu().numpy(), w, rtol=1e-11, atol=1e-13 * w.max())
+    wb = qtd.dft.Grids(mol).build().weights.cpu().numpy()
+    assert np.max(np.abs(wb - w)) > 1e-6  # a different partition
+    grids.becke_scheme = print
+    with pytest.raises(ValueError, match="becke_scheme"):
+        grids.build()
+
+
 def _random_dm(n, seed):
     rng = np.random.default_rng(seed)
     c = rng.standard_normal((n, n)) * 0.3

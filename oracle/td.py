@@ -332,6 +332,61 @@ def angular_product(n_theta=10, n_phi=11):
     return dirs, w
 
 
+# Lebedev-Laikov octahedral rules: orbit weights (summing to 1 over the points) and the orbits'
+# parameters; a1: (1,0,0) x6, a2: (0,1,1)/sqrt2 x12, a3: (1,1,1)/sqrt3 x8, b(l): (l,l,m) x24,
+# c(p): (p,q,0) x24.  n points -> (degree, [(orbit, parameter, weight)]).
+LEBEDEV = {
+    6: (3, [("a1", None, 1 / 6)]),
+    14: (5, [("a1", None, 1 / 15), ("a3", None, 3 / 40)]),
+    26: (7, [("a1", None, 1 / 21), ("a2", None, 4 / 105), ("a3", None, 9 / 280)]),
+    38: (9, [("a1", None, 1 / 105), ("a3", None, 9 / 280), ("c", 0.4597008433809831, 1 / 35)]),
+    50: (11, [("a1", None, 0.012698412698412698), ("a2", None, 0.022574955908289243),
+              ("a3", None, 0.02109375), ("b", 0.30151134457776363, 0.020173335537918871)]),
+    110: (17, [("a1", None, 0.3828270494937162e-2), ("a3", None, 0.9793737512487512e-2),
+               ("b", 0.1851156353447362, 0.8211737283191111e-2), ("b", 0.6904210483822922, 0.9942814891178103e-2),
+               ("b", 0.3956894730559419, 0.9595471336070963e-2), ("c", 0.4783690288121502, 0.9694996361663028e-2)]),
+}
+
+
+def _orbit(kind, par):
+    import itertools
+
+    pts = set()
+    if kind == "a1":
+        base = [(1.0, 0.0, 0.0)]
+    elif kind == "a2":
+        h = 1 / np.sqrt(2)
+        base = [(0.0, h, h)]
+    elif kind == "a3":
+        t = 1 / np.sqrt(3)
+        base = [(t, t, t)]
+    elif kind == "b":
+        base = [(par, par, np.sqrt(1 - 2 * par * par))]
+    else:  # c
+        base = [(par, np.sqrt(1 - par * par), 0.0)]
+    for v in base:
+        for perm in itertools.permutations(v):
+            for sg in itertools.product((1, -1), repeat=3):
+                pts.add(tuple(round(s_ * c, 15) + 0.0 for s_, c in zip(sg, perm)))
+    return np.array(sorted(pts))
+
+
+def angular_lebedev(n):
+    """Lebedev-Laikov rule with n points: unit vectors (n, 3) and weights summing to 4 pi (the angular
+    grid pyscf's dft.Grids uses; level 0 takes 50 points for hydrogen)."""
+    if n not in LEBEDEV:
+        raise ValueError(f"Lebedev rules available: {sorted(LEBEDEV)}")
+    dirs, w = [], []
+    for kind, par, wt in LEBEDEV[n][1]:
+        o = _orbit(kind, par)
+        dirs.append(o)
+        w.append(np.full(len(o), 4 * np.pi * wt))
+    dirs = np.concatenate(dirs)
+    if len(dirs) != n:
+        raise AssertionError("orbit sizes do not add up")
+    return dirs, np.concatenate(w)
+
+
 def stratmann_step(mu, a=0.64):
     """Stratmann-Scuseria-Frisch cell step (CPL 257, 213 (1996), eq. 14; pyscf gen_grid.stratmann,
     selected at td/trainer_legacy_no_jit.py:250): (35 z - 35 z^3 + 21 z^5 - 5 z^7) / 16 with z = mu / a,
@@ -363,9 +418,11 @@ def becke_weights(coords_pts, atom_coords, owner, scheme="becke"):
     return P[np.arange(len(coords_pts)), owner] / np.sum(P, axis=1)
 
 
-def molecular_grid(mol, n_rad=50, n_theta=10, n_phi=11, scheme="becke"):
+def molecular_grid(mol, n_rad=50, n_theta=10, n_phi=11, scheme="becke", n_ang=None):
+    """Atom-centred grid: Treutler radial x (product, or Lebedev with n_ang points) angular, fuzzy
+    cells by ``scheme``."""
     r, wr = radial_treutler(n_rad)
-    dirs, wa = angular_product(n_theta, n_phi)
+    dirs, wa = angular_product(n_theta, n_phi) if n_ang is None else angular_lebedev(n_ang)
     pts, wts, own = [], [], []
     for a, C in enumerate(mol.coords):
         p = C[None, None, :] + r[:, None, None] * dirs[None, :, :]

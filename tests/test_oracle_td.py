@@ -331,3 +331,42 @@ def test_partition_of_unity_and_gaussian_integral(scheme):
     p, w = T.molecular_grid(mol, scheme=scheme)
     dens = sum((2 / np.pi) ** 1.5 * np.exp(-2 * np.sum((p - c) ** 2, axis=1)) for c in mol.coords)
     assert abs(w @ dens - natm) < 5e-4  # quadrature error of the 50 x 110-point atomic grids
+
+
+def _sphere_monomial(a, b, c):
+    """Exact integral of x^a y^b z^c over the unit sphere (Gamma-function closed form)."""
+    import math
+
+    if a % 2 or b % 2 or c % 2:
+        return 0.0
+    g = math.gamma
+    return 2 * g((a + 1) / 2) * g((b + 1) / 2) * g((c + 1) / 2) / g((a + b + c + 3) / 2)
+
+
+@pytest.mark.parametrize("n", sorted(T.LEBEDEV))
+def test_lebedev_rules_are_exact_to_their_degree(n):
+    """Each Lebedev rule integrates every monomial up to its degree exactly (a wrong weight or orbit
+    parameter breaks this) and misses at the next even degree, so the degree is its own."""
+    deg = T.LEBEDEV[n][0]
+    d, w = T.angular_lebedev(n)
+    assert d.shape == (n, 3)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-15)
+    for a in range(deg + 1):
+        for b in range(deg + 1 - a):
+            for c in range(deg + 1 - a - b):
+                v = w @ (d[:, 0] ** a * d[:, 1] ** b * d[:, 2] ** c)
+                assert abs(v - _sphere_monomial(a, b, c)) < 1e-13, (a, b, c)
+    D = deg + 1
+    miss = max(abs(w @ (d[:, 0] ** a * d[:, 1] ** b * d[:, 2] ** (D - a - b)) - _sphere_monomial(a, b, D - a - b))
+               for a in range(0, D + 1, 2) for b in range(0, D + 1 - a, 2))
+    assert miss > 1e-8
+
+
+def test_device_lebedev_tables_equal_the_oracle():
+    from qex_amd.td import dft
+
+    for n in sorted(T.LEBEDEV):
+        d, w = dft.angular_lebedev(n)
+        do, wo = T.angular_lebedev(n)
+        np.testing.assert_allclose(d, do, rtol=0, atol=2e-15)
+        np.testing.assert_allclose(w, wo, rtol=1e-15, atol=0)

@@ -79,6 +79,24 @@ def test_stratmann_grid_matches_oracle(qtd):
         grids.build()
 
 
+def test_lebedev_grid_and_scf_energy_match_oracle(qtd):
+    """A pyscf-level-0-like grid (10 Treutler radial points x 50-point Lebedev, Stratmann cells)
+    built on the device against the oracle's, and the RKS energy of the H4 chain on it."""
+    mol, omol = _mol_pair(qtd, "chain", 4)
+    grids = qtd.dft.Grids(mol, n_rad=10, n_ang=50)
+    grids.becke_scheme = qtd.dft.stratmann
+    grids.build()
+    pts, w = otd.molecular_grid(omol, n_rad=10, n_ang=50, scheme="stratmann")
+    assert grids.size == 4 * 10 * 50
+    np.testing.assert_allclose(grids.coords.cpu().numpy(), pts, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(grids.weights.cpu().numpy(), w, rtol=1e-11, atol=1e-13 * w.max())
+    mf = qtd.scf.RKS(mol)
+    mf.grids = grids
+    e = mf.kernel()
+    ref = otd.scf(omol, "rks", grid=(pts, w), conv_tol=1e-10, max_cycle=60)
+    assert abs(e - ref.e_tot) < 1e-8, (e, ref.e_tot)
+
+
 def _random_dm(n, seed):
     rng = np.random.default_rng(seed)
     c = rng.standard_normal((n, n)) * 0.3

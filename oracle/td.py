@@ -37,6 +37,27 @@ BOHR = 0.52917721092  # Angstrom per bohr (pyscf param.BOHR)
 # basis sets (public data)
 # ---------------------------------------------------------------------------
 STO3G_H = [(0, [3.42525091, 0.62391373, 0.16885540], [0.15432897, 0.53532814, 0.44463454])]
+
+
+def sto3g_shells(zeta_1s, zeta_2sp=None):
+    """STO-3G shells of a first-row atom (Hehre, Stewart, Pople 1969): the exponent-1 fits of the 1s
+    and 2sp Slater functions, exponents scaled by zeta^2 (2s and 2p share the 2sp exponents)."""
+    e1 = np.array([2.227660584, 0.4057711562, 0.1098175104])
+    shells = [(0, zeta_1s ** 2 * e1, [0.1543289673, 0.5353281423, 0.4446345422])]
+    if zeta_2sp is not None:
+        e2 = zeta_2sp ** 2 * np.array([0.9942027, 0.2310313, 0.0751386])
+        shells += [(0, e2, [-0.09996722919, 0.3995128261, 0.7001154689]),
+                   (1, e2, [0.1559162750, 0.6076837186, 0.3919573931])]
+    return shells
+
+
+def water_molecule(shells_o, shells_h, r_oh=0.9572, angle=104.52):
+    """H2O at the experimental geometry (Angstrom, degrees), O at the origin."""
+    a = math.radians(angle)
+    r = r_oh / BOHR
+    coords = np.array([[0, 0, 0], [r * math.sin(a / 2), 0, r * math.cos(a / 2)],
+                       [-r * math.sin(a / 2), 0, r * math.cos(a / 2)]])
+    return build_molecule(coords, [8, 1, 1], [shells_o, shells_h, shells_h])
 B631G_H = [(0, [18.7311370, 2.8253937, 0.6401217], [0.03349460, 0.23472695, 0.81375733]),
            (0, [0.1612778], [1.0])]
 

@@ -19,12 +19,36 @@ from ..od import engine
 
 BOHR = 0.52917721092
 
+ELEMENTS = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10}
+
+
+def _sto3g():
+    """STO-3G (Hehre, Stewart, Pople, J. Chem. Phys. 51, 2657 (1969)): least-squares 3-Gaussian fits
+    of 1s and 2sp Slater functions with exponent 1, scaled by the standard molecular zeta^2 of each
+    element (H 1.24; He 1.69; Li-Ne 1s 2.69 ... 9.64, 2sp 0.80 ... 2.88).  Gives the tabulated
+    exponents, e.g. H 3.42525091, 0.62391373, 0.16885540; O 130.70932, 23.808866, 6.4436083 and
+    5.0331512, 1.1695960, 0.3803892."""
+    e1 = np.array([2.227660584, 0.4057711562, 0.1098175104])
+    c1 = [0.1543289673, 0.5353281423, 0.4446345422]
+    e2 = np.array([0.9942027, 0.2310313, 0.0751386])
+    c2s = [-0.09996722919, 0.3995128261, 0.7001154689]
+    c2p = [0.1559162750, 0.6076837186, 0.3919573931]
+    zeta = {"He": (1.69, None), "Li": (2.69, 0.80), "Be": (3.68, 1.15), "B": (4.68, 1.50), "C": (5.67, 1.72),
+            "N": (6.67, 1.95), "O": (7.66, 2.25), "F": (8.65, 2.55), "Ne": (9.64, 2.88)}
+    lib = {"H": [(0, [3.42525091, 0.62391373, 0.16885540], [0.15432897, 0.53532814, 0.44463454])]}
+    for el, (z1, z2) in zeta.items():
+        shells = [(0, list(z1 * z1 * e1), c1)]
+        if z2 is not None:
+            shells += [(0, list(z2 * z2 * e2), c2s), (1, list(z2 * z2 * e2), c2p)]
+        lib[el] = shells
+    return lib
+
+
 BASIS_LIBRARY = {
-    "sto-3g": {"H": [(0, [3.42525091, 0.62391373, 0.16885540], [0.15432897, 0.53532814, 0.44463454])]},
+    "sto-3g": _sto3g(),
     "6-31g": {"H": [(0, [18.7311370, 2.8253937, 0.6401217], [0.03349460, 0.23472695, 0.81375733]),
                     (0, [0.1612778], [1.0])]},
 }
-ELEMENTS = {"H": 1, "He": 2, "Li": 3, "Be": 4}
 
 
 def prim_norm(alpha, l):

@@ -370,3 +370,29 @@ def test_device_lebedev_tables_equal_the_oracle():
         do, wo = T.angular_lebedev(n)
         np.testing.assert_allclose(d, do, rtol=0, atol=2e-15)
         np.testing.assert_allclose(w, wo, rtol=1e-15, atol=0)
+
+
+def test_sto3g_first_row_data_and_water_energy():
+    """The zeta-scaled STO-3G fits reproduce the tabulated exponents (C, O), and the RHF energy of
+    water at the experimental geometry (0.9572 A, 104.52 deg) is the standard -74.962928 Ha."""
+    o = T.sto3g_shells(7.66, 2.25)
+    np.testing.assert_allclose(o[0][1], [130.70932, 23.808861, 6.4436083], rtol=1e-6)
+    np.testing.assert_allclose(o[1][1], [5.0331513, 1.1695961, 0.3803890], rtol=1e-6)
+    c = T.sto3g_shells(5.67, 1.72)
+    np.testing.assert_allclose(c[0][1], [71.616837, 13.045096, 3.5305122], rtol=1e-6)
+    np.testing.assert_allclose(c[1][1], [2.9412494, 0.6834831, 0.2222899], rtol=1e-6)
+    r = T.scf(T.water_molecule(o, T.STO3G_H), method="rhf", conv_tol=1e-12, max_cycle=100)
+    assert r.e_tot == pytest.approx(-74.962928, abs=2e-6)
+
+
+def test_device_basis_library_equals_the_oracle_sto3g():
+    from qex_amd.td import gto
+
+    lib = gto.BASIS_LIBRARY["sto-3g"]
+    for el, (z1, z2) in {"O": (7.66, 2.25), "C": (5.67, 1.72), "He": (1.69, None), "Li": (2.69, 0.80)}.items():
+        ref = T.sto3g_shells(z1, z2)
+        assert len(lib[el]) == len(ref)
+        for (l, e, c), (lo, eo, co) in zip(lib[el], ref):
+            assert l == lo
+            np.testing.assert_allclose(e, eo, rtol=1e-15)
+            np.testing.assert_allclose(c, co, rtol=0)

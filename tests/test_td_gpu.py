@@ -97,6 +97,26 @@ def test_lebedev_grid_and_scf_energy_match_oracle(qtd):
     assert abs(e - ref.e_tot) < 1e-8, (e, ref.e_tot)
 
 
+def test_water_sto3g_rhf_and_rks_match_oracle(qtd):
+    """A first-row molecule: H2O/STO-3G (O with s and p shells, Z = 8) at the experimental geometry.
+    RHF on the device against the oracle and the standard -74.962928 Ha; RKS-LDA on the device
+    grid against the oracle's SCF on the same grid."""
+    import math
+
+    a, r = math.radians(104.52), 0.9572
+    h1 = (r * math.sin(a / 2), 0.0, r * math.cos(a / 2))
+    h2 = (-r * math.sin(a / 2), 0.0, r * math.cos(a / 2))
+    mol = qtd.gto.Mole([("O", (0.0, 0.0, 0.0)), ("H", h1), ("H", h2)], basis="sto-3g")
+    omol = otd.water_molecule(otd.sto3g_shells(7.66, 2.25), otd.STO3G_H)
+    assert mol.nao == 7
+    e_rhf = qtd.scf.RHF(mol).kernel()
+    ref = otd.scf(omol, method="rhf", conv_tol=1e-12, max_cycle=100)
+    assert abs(e_rhf - ref.e_tot) < 1e-8 and abs(e_rhf - -74.962928) < 2e-6, (e_rhf, ref.e_tot)
+    e_rks = qtd.scf.RKS(mol).kernel()
+    ref = otd.scf(omol, "rks", grid=otd.molecular_grid(omol), conv_tol=1e-10, max_cycle=60)
+    assert abs(e_rks - ref.e_tot) < 1e-8, (e_rks, ref.e_tot)
+
+
 def _random_dm(n, seed):
     rng = np.random.default_rng(seed)
     c = rng.standard_normal((n, n)) * 0.3

@@ -240,7 +240,7 @@ def run_c2(args):
     # counted over all N rows by two lanes from the two ends) and ~30 N per eigenpair for the
     # LDL^T factorisation and the two solves
     eig_exec = (float(stats[0]) * 128 * N_GRID * 14 + float(stats[1]) * 30 * N_GRID) / eig_launches
-    # algorithmic work per launch (one SCF iteration of the batch), as written in DESIGN.md: the
+    # algorithmic work per launch (one SCF iteration of the batch): the
     # eigensolve counted as the dense symmetric solve (4/3) N^3 + 2 k N^2 per system that the
     # reference's eigh performs; MLP and Hartree as their GEMM-shaped FLOPs
     ne_list = [int(m[2]) for m in mols]
@@ -290,7 +290,7 @@ def run_c2(args):
         "eigen_rounds_per_pair": float(stats[0] / max(1, stats[1])),
         "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "work": "dense-equivalent FP64 FLOP per launch (algorithmic, DESIGN.md)",
+                     "work": "dense-equivalent FP64 FLOP per launch (algorithmic: (4/3) N^3 + 2 k N^2 per system)",
                      "eigen_executed_tflops": round(eig_exec / (avg_ms.get("eigen", 0) * 1e-3) / 1e12, 3)
                      if avg_ms.get("eigen") else None,
                      "eigen_latency_model_us_per_pair": round(eig_model_us, 1)},
@@ -800,7 +800,7 @@ def _c5_cpu_latency(grids, pool, W, b, pool_n0, kinds, pick_1d, pick_3d, threads
     return per
 
 
-# algorithmic work per unit, by kernel kind (what roofline.achieved counts; written out in DESIGN.md)
+# algorithmic work per unit, by kernel kind (what roofline.achieved counts)
 def _work_1d(kind, b_ne):
     """FLOPs of one 1D launch of ``kind`` over systems with electron counts ``b_ne`` (N = 513)."""
     N = N_GRID
